@@ -1,0 +1,60 @@
+"""Build libettk_align.so (hipcc, gfx950) in-tree, and the oracle/checker libraries under oracle/.
+
+Usage: python -m ettk_amd.build [--no-oracle]
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+LIB = PKG / "libettk_align.so"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 in any kernel
+HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+SOURCES = ["fft.hip", "ccc.hip", "so3.hip", "host.cpp"]
+
+
+def build_lib(verbose: bool = False) -> Path:
+    objs = []
+    for src in SOURCES:
+        obj = CSRC / ("_" + src + ".o")
+        lang = ["-x", "hip"] if src.endswith(".cpp") else []
+        cmd = [HIPCC, *HIP_FLAGS, *lang, "-c", str(CSRC / src), "-o", str(obj)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"hipcc failed on {src}")
+        if verbose:
+            sys.stderr.write(r.stderr)
+        objs.append(str(obj))
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs, "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("link of libettk_align.so failed")
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+def build_oracle() -> None:
+    """The FP64 CPU oracle (test infrastructure) and, where the reference tree exists, the soft-1.0 checker."""
+    subprocess.run(["make", "-C", str(ROOT / "oracle"), "-s", "liboracle.so"], check=True)
+    if Path("/root/reference/external/soft-1.0").is_dir():
+        subprocess.run(["make", "-C", str(ROOT / "oracle"), "-s", "ref"], check=True)
+
+
+if __name__ == "__main__":
+    build_lib(verbose="-v" in sys.argv)
+    if "--no-oracle" not in sys.argv:
+        build_oracle()
+    print(LIB)

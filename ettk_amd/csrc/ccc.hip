@@ -1,0 +1,465 @@
+// ccc.hip -- kernels of the wedge-constrained FFT cross-correlation (gfx950).
+//
+// Device side of nbfFourierImageMetric::executeFourierNewHalf (inc/em/nbfFourierImageMetric.h:1078-1323)
+// for a batch of (particle, transform) evaluations:
+//   reslice  -> nbfWedgedSubImage3D::getImage (inc/em/nbfWedgedSubImage3D.h:603-828), VTK tricubic
+//   spec     -> nbfFourierFilter::executeHalf (nbfFourierFilter.h:587-606) with the smoothed analytic wedge
+//               of nbfWedge3D::getImageHalf (nbfWedge3D.h:313-360) generated on the fly, and the Hermitian
+//               weighted sums of normalizeFourierHalf (nbfCorrelationImageMetric.h:498-530)
+//   combine  -> the quadratic expression of FIM.h:1202-1203
+//   dftA/BC  -> the zero-padded (2N)^3 c2r of FIM.h:1209-1214 evaluated only on the search window
+//               (FIM.h:1222-1247) as three separable partial DFTs, followed by the spherical mask and the
+//               first-maximum search of locateMaxima (nbfCorrelationImageMetric.h:304-323)
+// Per-evaluation sums are written as per-block partials and reduced in a fixed order (deterministic).
+#include "kernels.hpp"
+
+#include <cfloat>
+
+namespace ettk {
+
+static constexpr int kThreads = 256;
+
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+// block sum (result valid in thread 0)
+__device__ __forceinline__ float block_sum(float v, float* sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    float r = 0.f;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x + 63) / 64; i++) r += sh[i];
+    __syncthreads();
+    return r;
+}
+
+// ------------------------------------------------------------------ reslice
+__device__ __forceinline__ void keys_w(float f, float* w) {  // vtkTricubicInterpWeights
+    float fm1 = f - 1.f, fd2 = f * 0.5f, ft3 = f * 3.f;
+    w[0] = -fd2 * fm1 * fm1;
+    w[1] = ((ft3 - 2.f) * fd2 - 1.f) * fm1;
+    w[2] = -((ft3 - 4.f) * f - 1.f) * fd2;
+    w[3] = f * fd2 * fm1;
+}
+
+__global__ void __launch_bounds__(kThreads) k_reslice(const float* __restrict__ parts, const EvalGeom* __restrict__ geom,
+                                                       int N, float* __restrict__ out, float* __restrict__ partials) {
+    __shared__ float sh[8];
+    const int e = blockIdx.y;
+    const EvalGeom g = geom[e];
+    const float* vol = parts + (size_t)g.particle * N * N * N;
+    const long n3 = (long)N * N * N;
+    const long chunk = (n3 + gridDim.x - 1) / gridDim.x;
+    const long o0 = blockIdx.x * chunk, o1 = min(n3, o0 + chunk);
+    float s = 0.f, c = 0.f;
+    const float lo = -0.5f, hi = (float)N - 0.5f;
+    for (long o = o0 + threadIdx.x; o < o1; o += blockDim.x) {
+        int i = (int)(o / ((long)N * N)), j = (int)((o / N) % N), k = (int)(o % N);
+        float px = g.m[0] * i + g.m[3] * j + g.m[6] * k + g.m[9];
+        float py = g.m[1] * i + g.m[4] * j + g.m[7] * k + g.m[10];
+        float pz = g.m[2] * i + g.m[5] * j + g.m[8] * k + g.m[11];
+        float v = -FLT_MAX;
+        if (px >= lo && px <= hi && py >= lo && py <= hi && pz >= lo && pz <= hi) {
+            float xx = px + 7.62939453125e-06f, yy = py + 7.62939453125e-06f, zz = pz + 7.62939453125e-06f;
+            int ix = (int)floorf(xx), iy = (int)floorf(yy), iz = (int)floorf(zz);
+            float wx[4], wy[4], wz[4];
+            keys_w(xx - ix, wx);
+            keys_w(yy - iy, wy);
+            keys_w(zz - iz, wz);
+            int cx[4], cy[4], cz[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                cx[q] = min(max(ix - 1 + q, 0), N - 1);
+                cy[q] = min(max(iy - 1 + q, 0), N - 1);
+                cz[q] = min(max(iz - 1 + q, 0), N - 1);
+            }
+            float acc = 0.f;
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                float ay = 0.f;
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const float* row = vol + ((size_t)cz[a] * N + cy[b]) * N;
+                    ay += wy[b] * (wx[0] * row[cx[0]] + wx[1] * row[cx[1]] + wx[2] * row[cx[2]] + wx[3] * row[cx[3]]);
+                }
+                acc += wz[a] * ay;
+            }
+            v = acc;
+            s += v;
+            c += 1.f;
+        }
+        out[(size_t)e * n3 + o] = v;
+    }
+    float ts = block_sum(s, sh);
+    float tc = block_sum(c, sh);
+    if (threadIdx.x == 0) {
+        partials[((size_t)e * gridDim.x + blockIdx.x) * 2 + 0] = ts;
+        partials[((size_t)e * gridDim.x + blockIdx.x) * 2 + 1] = tc;
+    }
+}
+
+__global__ void k_reslice_mean(const float* __restrict__ partials, int nb, int n_eval, float* __restrict__ stats) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_eval) return;
+    double s = 0, c = 0;
+    for (int b = 0; b < nb; b++) {
+        s += partials[((size_t)e * nb + b) * 2];
+        c += partials[((size_t)e * nb + b) * 2 + 1];
+    }
+    stats[e] = (float)(s / (double)(float)c);  // meanInside is a float (WSub.h:794-797)
+}
+
+void launch_reslice(const float* parts, const EvalGeom* geom, int n_eval, int N, float* out, float* partials,
+                    int nb, hipStream_t s) {
+    dim3 grid(nb, n_eval);
+    k_reslice<<<grid, kThreads, 0, s>>>(parts, geom, N, out, partials);
+    ETTK_HIP(hipGetLastError());
+}
+void launch_reslice_mean(const float* partials, int nb, int n_eval, float* stats, hipStream_t s) {
+    k_reslice_mean<<<(n_eval + 127) / 128, 128, 0, s>>>(partials, nb, n_eval, stats);
+    ETTK_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ smoothed analytic wedge
+// binary wedge (v1.d)(v2.d) <= 0 on the half grid, d = (i - N/2, j - N/2, k - (H-1)), smoothed with the
+// separable 3-tap Gaussian (sigma 2, radius 1) clipped and renormalised at the borders of the half grid
+__device__ __forceinline__ float wedge_bin(const float* v1, const float* v2, int i, int j, int k, int N, int H) {
+    float di = (float)(i - N / 2), dj = (float)(j - N / 2), dk = (float)(k - (H - 1));
+    float a = v1[0] * di + v1[1] * dj + v1[2] * dk;
+    float b = v2[0] * di + v2[1] * dj + v2[2] * dk;
+    return (a * b <= 0.f) ? 1.f : 0.f;
+}
+__device__ __forceinline__ void tap3(int idx, int n, float* w) {
+    const float g = 0.88249690258459540286f;  // exp(-1/8)
+    float l = idx > 0 ? g : 0.f, r = idx < n - 1 ? g : 0.f;
+    float inv = 1.f / (l + 1.f + r);
+    w[0] = l * inv;
+    w[1] = inv;
+    w[2] = r * inv;
+}
+__device__ float wedge_smooth(const float* v1, const float* v2, int i, int j, int k, int N, int H) {
+    float wi[3], wj[3], wk[3];
+    tap3(i, N, wi);
+    tap3(j, N, wj);
+    tap3(k, H, wk);
+    float acc = 0.f;
+#pragma unroll
+    for (int a = -1; a <= 1; a++) {
+        if (wi[a + 1] == 0.f) continue;
+        float sj = 0.f;
+#pragma unroll
+        for (int b = -1; b <= 1; b++) {
+            if (wj[b + 1] == 0.f) continue;
+            float sk = 0.f;
+#pragma unroll
+            for (int c = -1; c <= 1; c++) {
+                if (wk[c + 1] == 0.f) continue;
+                sk += wk[c + 1] * wedge_bin(v1, v2, i + a, j + b, k + c, N, H);
+            }
+            sj += wj[b + 1] * sk;
+        }
+        acc += wi[a + 1] * sj;
+    }
+    return acc;
+}
+
+__global__ void k_wedge_ref(WedgeRef w, int N, float* __restrict__ W) {
+    const int H = N / 2 + 1;
+    long nh = (long)N * N * H;
+    for (long h = blockIdx.x * (long)blockDim.x + threadIdx.x; h < nh; h += (long)gridDim.x * blockDim.x) {
+        int i = (int)(h / ((long)N * H)), j = (int)((h / H) % N), k = (int)(h % H);
+        W[h] = w.full ? 1.f : wedge_smooth(w.v1, w.v2, i, j, k, N, H);
+    }
+}
+void launch_wedge_ref(const WedgeRef& w, int N, float* W, hipStream_t s) {
+    k_wedge_ref<<<512, kThreads, 0, s>>>(w, N, W);
+    ETTK_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ spectral filter
+__global__ void __launch_bounds__(kThreads) k_spec_filter(const float2* __restrict__ F, int f_stride, float2* __restrict__ X,
+                                                           const float* __restrict__ filter, const float* __restrict__ W1,
+                                                           const EvalGeom* __restrict__ geom, int N, int mcf, int apply,
+                                                           float* __restrict__ partials, float* __restrict__ wf_partials) {
+    __shared__ float sh[8];
+    const int e = blockIdx.y;
+    const int H = N / 2 + 1;
+    const long nh = (long)N * N * H;
+    const EvalGeom& g = geom[e];
+    const long chunk = (nh + gridDim.x - 1) / gridDim.x;
+    const long h0 = blockIdx.x * chunk, h1 = min(nh, h0 + chunk);
+    const long dc = ((long)(N / 2) * N + N / 2) * H + (H - 1);
+    float s = 0.f, swf = 0.f;
+    for (long h = h0 + threadIdx.x; h < h1; h += blockDim.x) {
+        int i = (int)(h / ((long)N * H)), j = (int)((h / H) % N), k = (int)(h % H);
+        float2 z = F[(size_t)e * f_stride + h];
+        if (mcf) {
+            float m2 = z.x * z.x + z.y * z.y;
+            float C = sqrtf(sqrtf(m2));
+            if (C > 0.f) z = make_float2(z.x / C, z.y / C);
+        }
+        float w = W1 ? W1[h] : 1.f;
+        if (!g.full) w *= wedge_smooth(g.v1, g.v2, i, j, k, N, H);
+        float fw = filter[h] * w;
+        if (apply) z = cscale(z, fw);
+        if (h == dc) z = make_float2(0.f, 0.f);
+        X[(size_t)e * nh + h] = z;
+        s += (k < H - 1 ? 2.f : 1.f) * (z.x * z.x + z.y * z.y);
+        swf += fw;
+    }
+    float ts = block_sum(s, sh);
+    float tw = block_sum(swf, sh);
+    if (threadIdx.x == 0) {
+        partials[(size_t)e * gridDim.x + blockIdx.x] = ts;
+        if (wf_partials) wf_partials[(size_t)e * gridDim.x + blockIdx.x] = tw;
+    }
+}
+
+void launch_spec_filter(const float2* F, int f_batch_stride, float2* X, const float* filter, const float* W1,
+                        const EvalGeom* geom, int n_eval, int N, int mcf, int apply_filter, float* partials,
+                        float* wf_partials, int nb, hipStream_t s) {
+    dim3 grid(nb, n_eval);
+    k_spec_filter<<<grid, kThreads, 0, s>>>(F, f_batch_stride, X, filter, W1, geom, N, mcf, apply_filter, partials,
+                                            wf_partials);
+    ETTK_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ reference-side real-space mask
+__global__ void __launch_bounds__(kThreads) k_mask_ref(float* __restrict__ R, const float* __restrict__ m1, int N,
+                                                        float* __restrict__ partials) {
+    __shared__ float sh[8];
+    const int e = blockIdx.y;
+    const long n3 = (long)N * N * N;
+    const long chunk = (n3 + gridDim.x - 1) / gridDim.x;
+    const long o0 = blockIdx.x * chunk, o1 = min(n3, o0 + chunk);
+    float s = 0.f;
+    for (long o = o0 + threadIdx.x; o < o1; o += blockDim.x) {
+        float r = R[(size_t)e * n3 + o], m = m1[o];
+        s += r * r * m;
+        R[(size_t)e * n3 + o] = r * m;
+    }
+    float t = block_sum(s, sh);
+    if (threadIdx.x == 0) partials[(size_t)e * gridDim.x + blockIdx.x] = t;
+}
+void launch_mask_ref(float* R, const float* m1, int n_eval, int N, float* partials, int nb, hipStream_t s) {
+    dim3 grid(nb, n_eval);
+    k_mask_ref<<<grid, kThreads, 0, s>>>(R, m1, N, partials);
+    ETTK_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ per-evaluation scalars
+__global__ void k_ccc_scalars(const float* p_n1, const float* p_n2, const float* p_k1, const float* p_wf, int nb,
+                              int nbr, int n_eval, int N, float* scalars) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_eval) return;
+    double s1 = 0, s2 = 0, k1 = 0, wf = 0;
+    for (int b = 0; b < nb; b++) {
+        s1 += p_n1[(size_t)e * nb + b];
+        s2 += p_n2[(size_t)e * nb + b];
+        wf += p_wf[(size_t)e * nb + b];
+    }
+    for (int b = 0; b < nbr; b++) k1 += p_k1[(size_t)e * nbr + b];
+    double n3 = (double)N * N * N;
+    float n1 = (float)sqrt(s1 / n3), n2 = (float)sqrt(s2 / n3);
+    scalars[e * 4 + 0] = n1;
+    scalars[e * 4 + 1] = n2;
+    scalars[e * 4 + 2] = (float)(k1 / ((double)n1 * (double)n1));  // k1_m of the normalised reference
+    scalars[e * 4 + 3] = (float)wf;
+}
+void launch_ccc_scalars(const float* p_n1, const float* p_n2, const float* p_k1, const float* p_wf, int nb,
+                        int nbr, int n_eval, int N, float* scalars, hipStream_t s) {
+    k_ccc_scalars<<<(n_eval + 127) / 128, 128, 0, s>>>(p_n1, p_n2, p_k1, p_wf, nb, nbr, n_eval, N, scalars);
+    ETTK_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ combine + first partial DFT (along i)
+__global__ void __launch_bounds__(kThreads) k_combine_dftA(const float2* __restrict__ F1m, const float2* __restrict__ F2,
+                                                            const float2* __restrict__ Q, const float2* __restrict__ Mfft,
+                                                            const float* __restrict__ scalars,
+                                                            const float2* __restrict__ tabA, CccSizes z,
+                                                            float2* __restrict__ G1) {
+    extern __shared__ float2 lds[];
+    const int N = z.N, H = z.H, E = z.E;
+    float2* C = lds;             // [i][k]
+    float2* A = lds + N * H;     // [x][i]
+    const int j = blockIdx.x, e = blockIdx.y;
+    const long nh = (long)N * N * H;
+    const float n1 = scalars[e * 4 + 0], n2 = scalars[e * 4 + 1];
+    const float n3 = (float)N * N * N;
+    const float a = 2.f * n3 / (n1 * n2);
+    const float b = 1.f / (n2 * n2);
+    for (int t = threadIdx.x; t < N * H; t += blockDim.x) {
+        int i = t / H, k = t % H;
+        long h = ((long)i * N + j) * H + k;
+        float2 f1 = F1m[(size_t)e * nh + h], f2 = F2[(size_t)e * nh + h], q = Q[(size_t)e * nh + h], m = Mfft[h];
+        float2 c = csub(cscale(cmulc(f1, f2), a), cscale(cmulc(m, q), b));
+        if ((i + j + k) & 1) c = make_float2(-c.x, -c.y);
+        C[t] = c;
+    }
+    for (int t = threadIdx.x; t < E * N; t += blockDim.x) A[t] = tabA[t];
+    __syncthreads();
+    for (int t = threadIdx.x; t < E * H; t += blockDim.x) {
+        int x = t / H, k = t % H;
+        float2 acc = make_float2(0.f, 0.f);
+        for (int i = 0; i < N; i++) acc = cadd(acc, cmul(A[x * N + i], C[i * H + k]));
+        G1[(((size_t)e * E + x) * N + j) * H + k] = acc;
+    }
+}
+void launch_combine_dftA(const float2* F1m, const float2* F2, const float2* Q, const float2* Mfft,
+                         const float* scalars, const float2* tabA, const CccSizes& z, int n_eval, float2* G1,
+                         hipStream_t s) {
+    dim3 grid(z.N, n_eval);
+    size_t shm = (size_t)(z.N * z.H + z.E * z.N) * sizeof(float2);
+    k_combine_dftA<<<grid, kThreads, shm, s>>>(F1m, F2, Q, Mfft, scalars, tabA, z, G1);
+    ETTK_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ second and third partial DFTs + argmax
+__global__ void __launch_bounds__(kThreads) k_dftBC_argmax(const float2* __restrict__ G1, const float2* __restrict__ tabA,
+                                                            const float2* __restrict__ tabK, CccSizes z,
+                                                            float* __restrict__ best_val, int* __restrict__ best_idx) {
+    extern __shared__ float2 lds[];
+    const int N = z.N, H = z.H, E = z.E;
+    float2* g1 = lds;                 // [j][k]
+    float2* A = g1 + N * H;           // [y][j]
+    float2* g2 = A + E * N;           // [y][k]
+    float2* K = g2 + E * H;           // [zz][k]
+    __shared__ float sv[kThreads];
+    __shared__ int si[kThreads];
+    const int x = blockIdx.x, e = blockIdx.y;
+    const float2* src = G1 + ((size_t)e * E + x) * N * H;
+    for (int t = threadIdx.x; t < N * H; t += blockDim.x) g1[t] = src[t];
+    for (int t = threadIdx.x; t < E * N; t += blockDim.x) A[t] = tabA[t];
+    for (int t = threadIdx.x; t < E * H; t += blockDim.x) K[t] = tabK[t];
+    __syncthreads();
+    for (int t = threadIdx.x; t < E * H; t += blockDim.x) {
+        int y = t / H, k = t % H;
+        float2 acc = make_float2(0.f, 0.f);
+        for (int j = 0; j < N; j++) acc = cadd(acc, cmul(A[y * N + j], g1[j * H + k]));
+        g2[t] = acc;
+    }
+    __syncthreads();
+    const float scale = 8.f / ((float)(2 * N) * (float)(2 * N) * (float)(2 * N));
+    float bv = -FLT_MAX;
+    int bi = 0x7fffffff;
+    const int dx = z.ub2 - x;
+    for (int t = threadIdx.x; t < E * E; t += blockDim.x) {
+        int y = t / E, zz = t % E;
+        int dy = z.ub2 - y, dz = z.ub2 - zz;
+        float rad = (float)(dx * dx + dy * dy + dz * dz);
+        if (!(rad < z.restr2)) continue;
+        float acc = 0.f;
+        for (int k = 0; k < H; k++) {
+            float2 p = g2[y * H + k], q = K[zz * H + k];
+            acc += p.x * q.x - p.y * q.y;
+        }
+        acc *= scale;
+        if (acc > bv || (acc == bv && t < bi)) { bv = acc; bi = t; }
+    }
+    sv[threadIdx.x] = bv;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            float v2 = sv[threadIdx.x + st];
+            int i2 = si[threadIdx.x + st];
+            if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 < si[threadIdx.x])) {
+                sv[threadIdx.x] = v2;
+                si[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        best_val[(size_t)e * E + x] = sv[0];
+        best_idx[(size_t)e * E + x] = si[0];
+    }
+}
+void launch_dftBC_argmax(const float2* G1, const float2* tabA, const float2* tabK, const CccSizes& z, int n_eval,
+                         float* best_val, int* best_idx, hipStream_t s) {
+    dim3 grid(z.E, n_eval);
+    size_t shm = (size_t)(z.N * z.H + z.E * z.N + 2 * z.E * z.H) * sizeof(float2);
+    k_dftBC_argmax<<<grid, kThreads, shm, s>>>(G1, tabA, tabK, z, best_val, best_idx);
+    ETTK_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ final reduction per evaluation
+__global__ void k_peak_final(const float* __restrict__ best_val, const int* __restrict__ best_idx,
+                             const float* __restrict__ scalars, float sum_filter, int overlap, CccSizes z, int n_eval,
+                             float* __restrict__ out) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_eval) return;
+    const int E = z.E;
+    float bv = -FLT_MAX;
+    int bx = -1, byz = 0;
+    for (int x = 0; x < E; x++) {
+        float v = best_val[(size_t)e * E + x];
+        int id = best_idx[(size_t)e * E + x];
+        if (id == 0x7fffffff) continue;
+        if (bx < 0 || v > bv) { bv = v; bx = x; byz = id; }
+    }
+    float px = 0.f, py = 0.f, pz = 0.f;
+    double maxT = -(double)FLT_MAX;
+    if (bx >= 0) {
+        px = (float)bx;
+        py = (float)(byz / E);
+        pz = (float)(byz % E);
+        maxT = bv;
+    }
+    const float k1m = scalars[e * 4 + 2], wf = scalars[e * 4 + 3];
+    double peak = fmax(0.0, (double)k1m - maxT);
+    if (overlap) peak = peak / (double)wf * (double)sum_filter;
+    float cp = (float)peak;
+    double big = (double)(2 * z.N) * (2 * z.N) * (2 * z.N);
+    cp = (float)((double)cp / big / big);
+    float c = (E - 1) / 2.0f;
+    out[e * 4 + 0] = cp;
+    out[e * 4 + 1] = (px - c) / 2.0f;
+    out[e * 4 + 2] = (py - c) / 2.0f;
+    out[e * 4 + 3] = (pz - c) / 2.0f;
+}
+void launch_peak_final(const float* best_val, const int* best_idx, const float* scalars, float sum_filter,
+                       int overlap, const CccSizes& z, int n_eval, float* out, hipStream_t s) {
+    k_peak_final<<<(n_eval + 127) / 128, 128, 0, s>>>(best_val, best_idx, scalars, sum_filter, overlap, z, n_eval, out);
+    ETTK_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ helpers used by the plan set-up
+__global__ void k_xfast_to_internal(const float* __restrict__ in, float* __restrict__ out, int N) {
+    long n3 = (long)N * N * N;
+    for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < n3; o += (long)gridDim.x * blockDim.x) {
+        int i = (int)(o / ((long)N * N)), j = (int)((o / N) % N), k = (int)(o % N);
+        out[o] = in[((size_t)k * N + j) * N + i];
+    }
+}
+void launch_xfast_to_internal(const float* in, float* out, int N, hipStream_t s) {
+    k_xfast_to_internal<<<1024, kThreads, 0, s>>>(in, out, N);
+    ETTK_HIP(hipGetLastError());
+}
+__global__ void k_internal_to_xfast(const float* __restrict__ in, float* __restrict__ out, int N) {
+    long n3 = (long)N * N * N;
+    for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < n3; o += (long)gridDim.x * blockDim.x) {
+        int x = (int)(o % N), y = (int)((o / N) % N), zz = (int)(o / ((long)N * N));
+        out[o] = in[((size_t)x * N + y) * N + zz];
+    }
+}
+void launch_internal_to_xfast(const float* in, float* out, int N, hipStream_t s) {
+    k_internal_to_xfast<<<1024, kThreads, 0, s>>>(in, out, N);
+    ETTK_HIP(hipGetLastError());
+}
+__global__ void k_mcf_inplace(float2* __restrict__ F, long n) {
+    for (long h = blockIdx.x * (long)blockDim.x + threadIdx.x; h < n; h += (long)gridDim.x * blockDim.x) {
+        float2 z = F[h];
+        float C = sqrtf(sqrtf(z.x * z.x + z.y * z.y));
+        if (C > 0.f) F[h] = make_float2(z.x / C, z.y / C);
+    }
+}
+void launch_mcf_inplace(float2* F, long n, hipStream_t s) {
+    k_mcf_inplace<<<1024, kThreads, 0, s>>>(F, n);
+    ETTK_HIP(hipGetLastError());
+}
+
+}  // namespace ettk

@@ -1,0 +1,99 @@
+// common.hpp -- shared host/device definitions of libettk_align (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/ettk_align.h"
+
+namespace ettk {
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string& msg);
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define ETTK_HIP(call)                                                                                  \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            throw ::ettk::Error{ETTK_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)};      \
+    } while (0)
+#define ETTK_REQUIRE(cond, code, msg)                                                                   \
+    do {                                                                                                \
+        if (!(cond)) throw ::ettk::Error{(code), (msg)};                                                \
+    } while (0)
+
+// ------------------------------------------------------------------ complex helpers (float2)
+__host__ __device__ inline float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__host__ __device__ inline float2 cmulc(float2 a, float2 b) { return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); }  // a*conj(b)
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// ------------------------------------------------------------------ device buffers
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void ensure(size_t count) {
+        if (count <= n) return;
+        release();
+        ETTK_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        ensure(count);
+        ETTK_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+// ------------------------------------------------------------------ FFT plans (twiddle tables)
+struct FftTables {
+    // twiddle[L] = exp(-2 pi i t / L), t in [0, L), for every supported length, packed
+    DevBuf<float2> tw;
+    int offset[9] = {};  // offsets per supported length index
+};
+int fft_len_index(int L);  // -1 when unsupported
+void fft_tables_init(FftTables& T, hipStream_t s);
+
+// batched 3-D transforms on [b][i][j][k] volumes (k fastest), FFTW conventions:
+//   r2c: real n^3 -> half n x n x (n/2+1) complex, forward (e^{-i})
+//   c2r: half -> real, backward (e^{+i}), unnormalised; transforms along i and j first, then the
+//        half-complex transform along k (real part of the k=0 and k=n/2 terms)
+// The r2c load applies an element-wise prologue selected by `pro`.
+enum Prologue : int {
+    PRO_NONE = 0,        // x
+    PRO_SQUARE_SHIFT,    // x*x*(-1)^(i+j+k)
+    PRO_MASK_SHIFT,      // x*aux0[i,j,k]*(-1)^(i+j+k)
+    PRO_RESLICED,        // valid(x) ? (x - mean_b) * aux0 * (-1)^(i+j+k) : 0   (mean from stats[b])
+    PRO_SHIFT,           // x*(-1)^(i+j+k)
+};
+struct R2CArgs {
+    const float* in;      // [b][n][n][n]
+    float2* out;          // [b][n][n][n/2+1]
+    int n, batch;
+    int pro;
+    const float* aux0;    // mask (n^3) for PRO_MASK_SHIFT / PRO_RESLICED
+    const float* stats;   // per batch (mean) for PRO_RESLICED
+};
+void fft3_r2c(const FftTables& T, const R2CArgs& a, float2* scratch, hipStream_t s);
+// c2r: in is preserved when `in_scratch` != in (the i/j passes run out of place into in_scratch)
+void fft3_c2r(const FftTables& T, const float2* in, float2* in_scratch, float* out, int n, int batch, hipStream_t s);
+
+}  // namespace ettk

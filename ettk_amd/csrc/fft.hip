@@ -1,0 +1,342 @@
+// fft.hip -- batched 3-D FFTs for gfx950: LDS-resident Stockham pencils (radix 4/3/2), one pass per axis.
+//
+// Replaces the FFTW double-precision plans of nbfFourierFilter::initializeFFTWhalf
+// (inc/em/nbfFourierFilter.h:270-358): fftw_plan_dft_r2c_3d / fftw_plan_dft_c2r_3d on [i][j][k] arrays
+// with the half spectrum along k.  Each workgroup stages P pencils of length L in LDS, runs the log_r(L)
+// Stockham stages there and writes them back, so every pass reads and writes each element of HBM once.
+//   r2c: pass k (pairs of real rows packed as one complex row, split after the transform), pass j, pass i
+//   c2r: pass i, pass j (complex, backward), then pass k (pairs of half rows -> two real rows)
+#include "common.hpp"
+
+#include <cmath>
+
+namespace ettk {
+
+static const int kLens[9] = {16, 32, 48, 64, 96, 128, 192, 256, 0};
+
+int fft_len_index(int L) {
+    for (int i = 0; kLens[i]; i++)
+        if (kLens[i] == L) return i;
+    return -1;
+}
+
+void fft_tables_init(FftTables& T, hipStream_t s) {
+    std::vector<float2> h;
+    for (int i = 0; kLens[i]; i++) {
+        T.offset[i] = (int)h.size();
+        int L = kLens[i];
+        for (int t = 0; t < L; t++) {
+            double a = -2.0 * M_PI * (double)t / (double)L;
+            h.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+        }
+    }
+    T.tw.upload(h.data(), h.size(), s);
+    ETTK_HIP(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------------ compile-time radix plans
+template <int L> struct Radices;
+template <> struct Radices<16>  { static constexpr int n = 2; static constexpr int r[4] = {4, 4, 1, 1}; };
+template <> struct Radices<32>  { static constexpr int n = 3; static constexpr int r[4] = {4, 4, 2, 1}; };
+template <> struct Radices<48>  { static constexpr int n = 3; static constexpr int r[4] = {4, 4, 3, 1}; };
+template <> struct Radices<64>  { static constexpr int n = 3; static constexpr int r[4] = {4, 4, 4, 1}; };
+template <> struct Radices<96>  { static constexpr int n = 4; static constexpr int r[4] = {4, 4, 3, 2}; };
+template <> struct Radices<128> { static constexpr int n = 4; static constexpr int r[4] = {4, 4, 4, 2}; };
+template <> struct Radices<192> { static constexpr int n = 4; static constexpr int r[4] = {4, 4, 4, 3}; };
+template <> struct Radices<256> { static constexpr int n = 4; static constexpr int r[4] = {4, 4, 4, 4}; };
+
+template <int L> struct Shape {
+    static constexpr int TPP = L / 4;                       // threads per pencil
+    static constexpr int P = (1024 / L) > 0 ? 1024 / L : 1; // pencils per workgroup
+    static constexpr int THREADS = P * TPP;
+    static constexpr int LD = L + 1;                        // padded LDS row
+};
+
+__device__ __forceinline__ float2 mul_i(float2 z, bool inv) {  // forward: -i z, inverse: +i z
+    return inv ? make_float2(-z.y, z.x) : make_float2(z.y, -z.x);
+}
+
+template <int R>
+__device__ __forceinline__ void butterfly(float2* x, bool inv) {
+    if constexpr (R == 2) {
+        float2 a = x[0], b = x[1];
+        x[0] = cadd(a, b);
+        x[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        const float s = inv ? 0.86602540378443864676f : -0.86602540378443864676f;
+        float2 t1 = cadd(x[1], x[2]);
+        float2 t2 = make_float2(x[0].x - 0.5f * t1.x, x[0].y - 0.5f * t1.y);
+        float2 d = csub(x[1], x[2]);
+        float2 id = make_float2(-s * d.y, s * d.x);  // i*s*d
+        x[0] = cadd(x[0], t1);
+        x[1] = cadd(t2, id);
+        x[2] = csub(t2, id);
+    } else if constexpr (R == 4) {
+        float2 a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
+        float2 c = cadd(x[1], x[3]), d = csub(x[1], x[3]);
+        float2 md = mul_i(d, inv);  // forward: -i d
+        x[0] = cadd(a, c);
+        x[2] = csub(a, c);
+        x[1] = cadd(b, md);
+        x[3] = csub(b, md);
+    }
+}
+
+template <int L, int R>
+__device__ __forceinline__ void stockham_stage(float2* pen, int lane, int Ns, const float2* tw, bool inv) {
+    constexpr int NB = L / R;
+    constexpr int TPP = Shape<L>::TPP;
+    constexpr int MAXB = (NB + TPP - 1) / TPP;
+    float2 v[MAXB][R];
+    int base[MAXB];
+    const int step = L / (Ns * R);
+#pragma unroll
+    for (int c = 0; c < MAXB; c++) {
+        int bf = lane + c * TPP;
+        base[c] = -1;
+        if (bf < NB) {
+            int k = bf % Ns;
+#pragma unroll
+            for (int m = 0; m < R; m++) {
+                float2 x = pen[bf + m * NB];
+                if (m > 0) {
+                    float2 w = tw[m * k * step];
+                    if (inv) w.y = -w.y;
+                    x = cmul(x, w);
+                }
+                v[c][m] = x;
+            }
+            butterfly<R>(v[c], inv);
+            base[c] = (bf / Ns) * Ns * R + k;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < MAXB; c++)
+        if (base[c] >= 0)
+#pragma unroll
+            for (int m = 0; m < R; m++) pen[base[c] + m * Ns] = v[c][m];
+    __syncthreads();
+}
+
+template <int L, int S>
+__device__ __forceinline__ void run_stage(float2* pen, int lane, int& Ns, const float2* tw, bool inv) {
+    using RD = Radices<L>;
+    if constexpr (S < RD::n) {
+        constexpr int R = RD::r[S];
+        stockham_stage<L, R>(pen, lane, Ns, tw, inv);
+        Ns *= R;
+    }
+}
+
+// in-LDS transform of one pencil (every thread of the block must call it: it synchronises)
+template <int L>
+__device__ __forceinline__ void lds_fft(float2* pen, int lane, const float2* tw, bool inv) {
+    int Ns = 1;
+    run_stage<L, 0>(pen, lane, Ns, tw, inv);
+    run_stage<L, 1>(pen, lane, Ns, tw, inv);
+    run_stage<L, 2>(pen, lane, Ns, tw, inv);
+    run_stage<L, 3>(pen, lane, Ns, tw, inv);
+}
+
+// ------------------------------------------------------------------ r2c, pass along k (real rows)
+template <int L>
+__device__ __forceinline__ float r2c_load(const R2CArgs& a, long row, int k, float x) {
+    if (a.pro == PRO_NONE) return x;
+    long ij = row % ((long)L * L);
+    int i = (int)(ij / L), j = (int)(ij % L);
+    float sg = ((i + j + k) & 1) ? -1.0f : 1.0f;
+    switch (a.pro) {
+        case PRO_SQUARE_SHIFT: return x * x * sg;
+        case PRO_SHIFT: return x * sg;
+        case PRO_MASK_SHIFT: return x * a.aux0[ij * L + k] * sg;
+        case PRO_RESLICED: {
+            if (x == -3.402823466e+38f) return 0.0f;  // outside the particle box: -FLT_MAX background
+            long b = row / ((long)L * L);
+            return (x - a.stats[b]) * a.aux0[ij * L + k] * sg;
+        }
+    }
+    return x;
+}
+
+template <int L>
+__global__ void __launch_bounds__(Shape<L>::THREADS) k_r2c_rows(R2CArgs a, const float2* __restrict__ twg) {
+    using S = Shape<L>;
+    constexpr int H = L / 2 + 1;
+    __shared__ float2 lds[S::P * S::LD];
+    __shared__ float2 tw[L];
+    const long npairs = (long)a.batch * L * L / 2;
+    const long p0 = (long)blockIdx.x * S::P;
+    for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
+    // load P pairs of rows (2L contiguous floats each) and pack row 2q + i*row 2q+1
+    for (int t = threadIdx.x; t < S::P * 2 * L; t += blockDim.x) {
+        int q = t / (2 * L), r = t % (2 * L);
+        long pair = p0 + q;
+        float x = 0.0f;
+        if (pair < npairs) {
+            long row = pair * 2 + r / L;
+            x = r2c_load<L>(a, row, r % L, a.in[row * L + (r % L)]);
+        }
+        float* f = reinterpret_cast<float*>(&lds[q * S::LD + (r % L)]);
+        f[r / L] = x;
+    }
+    __syncthreads();
+    const int q = threadIdx.x / S::TPP, lane = threadIdx.x % S::TPP;
+    lds_fft<L>(&lds[q * S::LD], lane, tw, false);
+    // split Z = A + iB into the half spectra of the two rows
+    for (int t = threadIdx.x; t < S::P * 2 * H; t += blockDim.x) {
+        int qq = t / (2 * H), r = t % (2 * H);
+        long pair = p0 + qq;
+        if (pair >= npairs) continue;
+        int which = r / H, k = r % H;
+        float2 z = lds[qq * S::LD + k], zc = cconj(lds[qq * S::LD + ((L - k) % L)]);
+        float2 o = which == 0 ? cscale(cadd(z, zc), 0.5f) : make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
+        a.out[(pair * 2 + which) * H + k] = o;
+    }
+}
+
+// ------------------------------------------------------------------ complex pass along a strided axis
+struct StridedArgs {
+    const float2* in;
+    float2* out;
+    long npencils;
+    long inner;         // pencils with consecutive addresses
+    long outer_stride;  // address step between groups of `inner` pencils
+    long es;            // element stride along the transformed axis
+    int inv;
+};
+
+template <int L>
+__global__ void __launch_bounds__(Shape<L>::THREADS) k_fft_strided(StridedArgs a, const float2* __restrict__ twg) {
+    using S = Shape<L>;
+    __shared__ float2 lds[S::P * S::LD];
+    __shared__ float2 tw[L];
+    const long q0 = (long)blockIdx.x * S::P;
+    for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
+    for (int t = threadIdx.x; t < S::P * L; t += blockDim.x) {
+        int q = t % S::P, l = t / S::P;
+        long pen = q0 + q;
+        float2 x = make_float2(0.f, 0.f);
+        if (pen < a.npencils) {
+            long base = (pen / a.inner) * a.outer_stride + (pen % a.inner);
+            x = a.in[base + l * a.es];
+        }
+        lds[q * S::LD + l] = x;
+    }
+    __syncthreads();
+    const int q = threadIdx.x / S::TPP, lane = threadIdx.x % S::TPP;
+    lds_fft<L>(&lds[q * S::LD], lane, tw, a.inv != 0);
+    for (int t = threadIdx.x; t < S::P * L; t += blockDim.x) {
+        int qq = t % S::P, l = t / S::P;
+        long pen = q0 + qq;
+        if (pen >= a.npencils) continue;
+        long base = (pen / a.inner) * a.outer_stride + (pen % a.inner);
+        a.out[base + l * a.es] = lds[qq * S::LD + l];
+    }
+}
+
+// ------------------------------------------------------------------ c2r, pass along k (half rows)
+template <int L>
+__global__ void __launch_bounds__(Shape<L>::THREADS) k_c2r_rows(const float2* __restrict__ in, float* __restrict__ out,
+                                                                  long npairs, const float2* __restrict__ twg) {
+    using S = Shape<L>;
+    constexpr int H = L / 2 + 1;
+    __shared__ float2 lds[S::P * S::LD];
+    __shared__ float2 tw[L];
+    const long p0 = (long)blockIdx.x * S::P;
+    for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
+    // Z(k) = A(k) + i B(k) over the full length, with A(L-k) = conj(A(k)) and real A(0), A(L/2)
+    for (int t = threadIdx.x; t < S::P * L; t += blockDim.x) {
+        int q = t / L, k = t % L;
+        long pair = p0 + q;
+        float2 z = make_float2(0.f, 0.f);
+        if (pair < npairs) {
+            const float2* ra = in + (pair * 2) * H;
+            const float2* rb = ra + H;
+            float2 A, Bv;
+            if (k <= L / 2) {
+                A = ra[k];
+                Bv = rb[k];
+                if (k == 0 || k == L / 2) { A.y = 0.f; Bv.y = 0.f; }
+            } else {
+                A = cconj(ra[L - k]);
+                Bv = cconj(rb[L - k]);
+            }
+            z = make_float2(A.x - Bv.y, A.y + Bv.x);
+        }
+        lds[q * S::LD + k] = z;
+    }
+    __syncthreads();
+    const int q = threadIdx.x / S::TPP, lane = threadIdx.x % S::TPP;
+    lds_fft<L>(&lds[q * S::LD], lane, tw, true);
+    for (int t = threadIdx.x; t < S::P * 2 * L; t += blockDim.x) {
+        int qq = t / (2 * L), r = t % (2 * L);
+        long pair = p0 + qq;
+        if (pair >= npairs) continue;
+        float2 z = lds[qq * S::LD + (r % L)];
+        out[pair * 2 * L + r] = (r < L) ? z.x : z.y;
+    }
+}
+
+// ------------------------------------------------------------------ host launchers
+template <int L>
+static void launch_r2c(const FftTables& T, const R2CArgs& a, float2* scratch, hipStream_t s) {
+    using S = Shape<L>;
+    constexpr int H = L / 2 + 1;
+    const float2* tw = T.tw.p + T.offset[fft_len_index(L)];
+    long npairs = (long)a.batch * L * L / 2;
+    // pass k: real rows -> a.out (half rows)
+    k_r2c_rows<L><<<(unsigned)((npairs + S::P - 1) / S::P), S::THREADS, 0, s>>>(a, tw);
+    // pass j: pencils over (b, i, k), element stride H
+    StridedArgs sj{a.out, a.out, (long)a.batch * L * H, (long)H, (long)L * H, (long)H, 0};
+    k_fft_strided<L><<<(unsigned)((sj.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(sj, tw);
+    // pass i: pencils over (b, j, k), element stride L*H
+    StridedArgs si{a.out, a.out, (long)a.batch * L * H, (long)L * H, (long)L * L * H, (long)L * H, 0};
+    k_fft_strided<L><<<(unsigned)((si.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(si, tw);
+    (void)scratch;
+}
+
+template <int L>
+static void launch_c2r(const FftTables& T, const float2* in, float2* tmp, float* out, int batch, hipStream_t s) {
+    using S = Shape<L>;
+    constexpr int H = L / 2 + 1;
+    const float2* tw = T.tw.p + T.offset[fft_len_index(L)];
+    StridedArgs si{in, tmp, (long)batch * L * H, (long)L * H, (long)L * L * H, (long)L * H, 1};
+    k_fft_strided<L><<<(unsigned)((si.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(si, tw);
+    StridedArgs sj{tmp, tmp, (long)batch * L * H, (long)H, (long)L * H, (long)H, 1};
+    k_fft_strided<L><<<(unsigned)((sj.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(sj, tw);
+    long npairs = (long)batch * L * L / 2;
+    k_c2r_rows<L><<<(unsigned)((npairs + S::P - 1) / S::P), S::THREADS, 0, s>>>(tmp, out, npairs, tw);
+}
+
+void fft3_r2c(const FftTables& T, const R2CArgs& a, float2* scratch, hipStream_t s) {
+    switch (a.n) {
+        case 16: launch_r2c<16>(T, a, scratch, s); break;
+        case 32: launch_r2c<32>(T, a, scratch, s); break;
+        case 48: launch_r2c<48>(T, a, scratch, s); break;
+        case 64: launch_r2c<64>(T, a, scratch, s); break;
+        case 96: launch_r2c<96>(T, a, scratch, s); break;
+        case 128: launch_r2c<128>(T, a, scratch, s); break;
+        case 192: launch_r2c<192>(T, a, scratch, s); break;
+        case 256: launch_r2c<256>(T, a, scratch, s); break;
+        default: throw Error{ETTK_ERR_UNSUPPORTED, "fft3_r2c: unsupported size " + std::to_string(a.n)};
+    }
+    ETTK_HIP(hipGetLastError());
+}
+
+void fft3_c2r(const FftTables& T, const float2* in, float2* tmp, float* out, int n, int batch, hipStream_t s) {
+    switch (n) {
+        case 16: launch_c2r<16>(T, in, tmp, out, batch, s); break;
+        case 32: launch_c2r<32>(T, in, tmp, out, batch, s); break;
+        case 48: launch_c2r<48>(T, in, tmp, out, batch, s); break;
+        case 64: launch_c2r<64>(T, in, tmp, out, batch, s); break;
+        case 96: launch_c2r<96>(T, in, tmp, out, batch, s); break;
+        case 128: launch_c2r<128>(T, in, tmp, out, batch, s); break;
+        case 192: launch_c2r<192>(T, in, tmp, out, batch, s); break;
+        case 256: launch_c2r<256>(T, in, tmp, out, batch, s); break;
+        default: throw Error{ETTK_ERR_UNSUPPORTED, "fft3_c2r: unsupported size " + std::to_string(n)};
+    }
+    ETTK_HIP(hipGetLastError());
+}
+
+}  // namespace ettk

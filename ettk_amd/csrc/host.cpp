@@ -1,0 +1,485 @@
+// host.cpp -- plan set-up, the constrained-CCC batch driver and the C ABI entry points of libettk_align.
+#include <cfloat>
+#include <cmath>
+#include <mutex>
+
+#include "plan.hpp"
+
+namespace ettk {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+// ============================================================== FP64 transforms (vtkTransform semantics)
+Mat4 operator*(const Mat4& A, const Mat4& B) {
+    Mat4 C;
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++)
+            C(r, c) = A(r, 0) * B(0, c) + A(r, 1) * B(1, c) + A(r, 2) * B(2, c) + A(r, 3) * B(3, c);
+    return C;
+}
+Mat4 mat_rot(double angle, double x, double y, double z) {
+    Mat4 M = Mat4::ident();
+    if (angle == 0.0 || (x == 0.0 && y == 0.0 && z == 0.0)) return M;
+    angle = angle * M_PI / 180.0;
+    double w = std::cos(0.5 * angle), f = std::sin(0.5 * angle) / std::sqrt(x * x + y * y + z * z);
+    x *= f; y *= f; z *= f;
+    double ww = w * w, wx = w * x, wy = w * y, wz = w * z;
+    double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+    double s = ww - xx - yy - zz;
+    M(0, 0) = xx * 2 + s;    M(1, 0) = (xy + wz) * 2; M(2, 0) = (xz - wy) * 2;
+    M(0, 1) = (xy - wz) * 2; M(1, 1) = yy * 2 + s;    M(2, 1) = (yz + wx) * 2;
+    M(0, 2) = (xz + wy) * 2; M(1, 2) = (yz - wx) * 2; M(2, 2) = zz * 2 + s;
+    return M;
+}
+Mat4 mat_translate(double x, double y, double z) {
+    Mat4 M = Mat4::ident();
+    M(0, 3) = x; M(1, 3) = y; M(2, 3) = z;
+    return M;
+}
+Mat4 mat_scale(double x, double y, double z) {
+    Mat4 M = Mat4::ident();
+    M(0, 0) = x; M(1, 1) = y; M(2, 2) = z;
+    return M;
+}
+double mat_det(const Mat4& A) {
+    double d = 0;
+    for (int c = 0; c < 4; c++) {
+        double s[3][3];
+        for (int r = 1; r < 4; r++)
+            for (int k = 0, cc = 0; k < 4; k++)
+                if (k != c) s[r - 1][cc++] = A(r, k);
+        double m = s[0][0] * (s[1][1] * s[2][2] - s[1][2] * s[2][1]) - s[0][1] * (s[1][0] * s[2][2] - s[1][2] * s[2][0]) +
+                   s[0][2] * (s[1][0] * s[2][1] - s[1][1] * s[2][0]);
+        d += ((c & 1) ? -1.0 : 1.0) * A(0, c) * m;
+    }
+    return d;
+}
+Mat4 mat_inverse(const Mat4& A) {
+    double a[4][8];
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) {
+            a[r][c] = A(r, c);
+            a[r][c + 4] = (r == c) ? 1.0 : 0.0;
+        }
+    for (int c = 0; c < 4; c++) {
+        int piv = c;
+        for (int r = c + 1; r < 4; r++)
+            if (std::fabs(a[r][c]) > std::fabs(a[piv][c])) piv = r;
+        for (int k = 0; k < 8 && piv != c; k++) std::swap(a[c][k], a[piv][k]);
+        double d = a[c][c];
+        for (int k = 0; k < 8; k++) a[c][k] /= d;
+        for (int r = 0; r < 4; r++)
+            if (r != c) {
+                double f = a[r][c];
+                for (int k = 0; k < 8; k++) a[r][k] -= f * a[c][k];
+            }
+    }
+    Mat4 I;
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) I(r, c) = a[r][c + 4];
+    return I;
+}
+Mat4 strip_translation(const Mat4& t) { return mat_translate(-t(0, 3), -t(1, 3), -t(2, 3)) * t; }
+Mat4 zyz(double a, double b, double g) { return mat_rot(-g, 0, 0, 1) * mat_rot(-b, 0, 1, 0) * mat_rot(-a, 0, 0, 1); }
+bool transform_ok(const Mat4& t, double tr, double rot) {
+    if (std::fabs(t(0, 3)) > tr || std::fabs(t(1, 3)) > tr || std::fabs(t(2, 3)) > tr) return false;
+    double px = t(0, 2), py = t(1, 2), pz = t(2, 2), n = std::sqrt(px * px + py * py + pz * pz);
+    double c = n != 0.0 ? pz / n : 0.0;
+    double ang = c >= 1.0 ? 0.0 : std::fabs(std::acos(c) * 180.0 / M_PI);
+    return ang <= rot;
+}
+Mat4 euler_of(const ettk_particle& d) {
+    return mat_rot(-d.normal[2], 0, 0, 1) * mat_rot(-d.normal[0], 1, 0, 0) * mat_rot(-d.normal[1], 0, 0, 1);
+}
+
+// ============================================================== constant arrays of a plan
+// separable vtkImageGaussianSmooth on an A(i,j,k) float array: axes 2, 1, 0, kernel clipped and renormalised
+static void smooth3(std::vector<float>& A, int n0, int n1, int n2, double sd, double rf) {
+    int dims[3] = {n0, n1, n2};
+    long st[3] = {(long)n1 * n2, n2, 1};
+    int rad = (int)(sd * rf);
+    std::vector<float> o(A.size());
+    for (int ax = 2; ax >= 0; ax--) {
+        int n = dims[ax];
+        for (long b = 0; b < (long)A.size(); b++) {
+            int idx = (int)((b / st[ax]) % n);
+            int lo = std::max(-rad, -idx), hi = std::min(rad, n - 1 - idx);
+            double ks = 0, s = 0;
+            for (int x = lo; x <= hi; x++) ks += std::exp(-(double)(x * x) / (sd * sd * 2.0));
+            for (int x = lo; x <= hi; x++) s += std::exp(-(double)(x * x) / (sd * sd * 2.0)) / ks * A[b + x * st[ax]];
+            o[b] = (float)s;
+        }
+        A.swap(o);
+    }
+}
+
+static void wedge_normals(float lw, float uw, const Mat4& t, WedgeRef& w) {
+    double l = lw < -90 ? -90.0 : lw, u = uw > 90 ? 90.0 : uw;
+    w.full = (u - l >= 180) ? 1 : 0;
+    double n1[3] = {std::cos(-(90.0 - u) * M_PI / 180.0), 0.0, -std::sin(-(90.0 - u) * M_PI / 180.0)};
+    double n2[3] = {std::cos(-(90.0 - l) * M_PI / 180.0), 0.0, -std::sin(-(90.0 - l) * M_PI / 180.0)};
+    Mat4 R = strip_translation(t);
+    for (int r = 0; r < 3; r++) {
+        w.v1[r] = (float)(R(r, 0) * n1[0] + R(r, 1) * n1[1] + R(r, 2) * n1[2]);
+        w.v2[r] = (float)(R(r, 0) * n2[0] + R(r, 1) * n2[1] + R(r, 2) * n2[2]);
+    }
+}
+
+static void plan_constants(ettk_plan* P) {
+    const ettk_params& p = P->p;
+    const int N = P->N, H = P->H;
+    const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * H;
+    hipStream_t s = P->ctx->stream;
+    // spherical raised-cosine image mask (nbfImageFilter::buildMask, IFilt.h:384-472)
+    std::vector<float> m1(n3, 1.0f);
+    if (p.mask_radius != 0.0f) {
+        float c = (float)(N / 2.0);
+        float r2 = std::min((float)(N - 1), (float)(p.mask_radius + p.mask_sigma / 2.0));
+        float r1 = std::max(0.0f, (float)(p.mask_radius - p.mask_sigma / 2.0));
+        float kappa = (float)(M_PI / (r2 - r1));
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++)
+                for (int k = 0; k < N; k++) {
+                    float di = i - c, dj = j - c, dk = k - c;
+                    float r = std::sqrt(di * di + dj * dj + dk * dk);
+                    float v = r <= r1 ? 1.0f : (r >= r2 ? 0.0f : (float)((1.0 + std::cos(kappa * (r - r1))) / 2.0));
+                    m1[((size_t)i * N + j) * N + k] = v;
+                }
+    }
+    // square FFT window softened by a Gaussian of sigma 2.5 (FIM.h:642-666)
+    std::vector<float> mf(n3, 0.0f);
+    int sq = p.square_mask;
+    if (sq > 0 && sq < std::floor(N / 2.0)) {
+        for (int i = sq; i <= N - 1 - sq; i++)
+            for (int j = sq; j <= N - 1 - sq; j++)
+                for (int k = sq; k <= N - 1 - sq; k++) mf[((size_t)i * N + j) * N + k] = 1.0f;
+    } else {
+        std::fill(mf.begin(), mf.end(), 1.0f);
+    }
+    smooth3(mf, N, N, N, 2.5, 2.5 * 0.75);
+    // band-pass x B-factor filter on the half spectrum, DC = 0 (FFilt.h:492-547, 738-797)
+    std::vector<float> f(nh, 1.0f);
+    f[((size_t)(N / 2) * N + N / 2) * H + (H - 1)] = 0.0f;
+    bool bp = (p.bp_lo > 0.0f) || (p.bp_hi < 1.0f);
+    int c = N / 2;
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++)
+            for (int k = 0; k < H; k++) {
+                double ti = (1.0 * i - c) / c, tj = (1.0 * j - c) / c, tk = (1.0 * k - c) / c;
+                float rad = (float)std::sqrt(ti * ti + tj * tj + tk * tk);
+                size_t o = ((size_t)i * N + j) * H + k;
+                if (bp) {
+                    double lo = p.bp_lo, hi = p.bp_hi, lv = p.bp_lo_decay, hv = p.bp_hi_decay;
+                    float b = 1.0f;
+                    if (rad < lo) b = (float)std::exp(-(lo - rad) * (lo - rad) / 2.0 / lv / lv);
+                    if (rad > hi) b = (float)std::exp(-(rad - hi) * (rad - hi) / 2.0 / hv / hv);
+                    f[o] *= b;
+                }
+                if (p.bfactor != 0.0f) f[o] *= (float)std::exp(-(double)p.bfactor * (double)rad * rad);
+            }
+    P->dummy_filter = !(bp || p.bfactor != 0.0f);
+    P->sum_filter = 0;
+    for (float v : f) P->sum_filter += v;
+    P->h_m1 = m1;
+    P->m1.upload(m1.data(), n3, s);
+    P->maskfft.upload(mf.data(), n3, s);
+    P->filter.upload(f.data(), nh, s);
+    // M = r2c(m1 * shift)
+    P->Mfft.ensure(nh);
+    R2CArgs a{P->m1.p, P->Mfft.p, N, 1, PRO_SHIFT, nullptr, nullptr};
+    fft3_r2c(P->ctx->fft, a, nullptr, s);
+    // search window of the upsampled correlation and its partial-DFT tables (FIM.h:1222-1247)
+    double center = N;  // (2N)/2
+    double restr = center;
+    if (P->trans_restrict < (double)FLT_MAX) restr = std::max(P->trans_restrict * 2.0, 1.0);
+    int I0 = (int)std::max(0.0, center - std::floor(restr));
+    int I1 = (int)std::min(center + std::floor(restr), (double)(2 * N - 1));
+    int E = I1 - I0 + 1;
+    P->z.N = N;
+    P->z.H = H;
+    P->z.E = E;
+    P->z.I0 = I0;
+    P->z.ub2 = (E - 1) / 2;
+    P->z.restr2 = (float)restr * (float)restr;
+    size_t lds_a = (size_t)(N * H + E * N) * sizeof(float2);
+    size_t lds_b = (size_t)(N * H + E * N + 2 * E * H) * sizeof(float2) + 2048;
+    ETTK_REQUIRE(lds_a <= 160 * 1024 && lds_b <= 160 * 1024, ETTK_ERR_UNSUPPORTED,
+                 "translation search window too large for the LDS-resident upsampled DFT (reduce trans_restrict)");
+    std::vector<float2> tA((size_t)E * N), tK((size_t)E * H);
+    for (int x = 0; x < E; x++) {
+        int u = I0 + x - N;
+        for (int i = 0; i < N; i++) {
+            int fr = i - N / 2;
+            double ang = M_PI * (double)fr * (double)u / (double)N;
+            double sg = (fr & 1) ? -1.0 : 1.0;
+            tA[(size_t)x * N + i] = make_float2((float)(sg * std::cos(ang)), (float)(sg * std::sin(ang)));
+        }
+        for (int k = 0; k < H; k++) {
+            int fr = k - N / 2;
+            double ang = M_PI * (double)fr * (double)u / (double)N;
+            double sg = (fr & 1) ? -1.0 : 1.0;
+            double w = (k == H - 1) ? 1.0 : 2.0;
+            tK[(size_t)x * H + k] = make_float2((float)(w * sg * std::cos(ang)), (float)(w * sg * std::sin(ang)));
+        }
+    }
+    P->tabA.upload(tA.data(), tA.size(), s);
+    P->tabK.upload(tK.data(), tK.size(), s);
+    ETTK_HIP(hipStreamSynchronize(s));
+}
+
+void ensure_ccc_workspace(ettk_plan* P, int cap) {
+    if (cap <= P->cap) return;
+    const int N = P->N, H = P->H;
+    const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * H;
+    P->nb_real = (int)((n3 + 1023) / 1024);
+    P->nb_spec = (int)((nh + 1023) / 1024);
+    P->R1.ensure(n3 * cap);
+    P->R2.ensure(n3 * cap);
+    P->S1.ensure(nh * cap);
+    P->S2.ensure(nh * cap);
+    P->S3.ensure(nh * cap);
+    P->G1.ensure((size_t)P->z.E * N * H * cap);
+    P->p_res.ensure((size_t)2 * P->nb_real * cap);
+    P->p_n1.ensure((size_t)P->nb_spec * cap);
+    P->p_n2.ensure((size_t)P->nb_spec * cap);
+    P->p_wf.ensure((size_t)P->nb_spec * cap);
+    P->p_k1.ensure((size_t)P->nb_real * cap);
+    P->stats.ensure(cap);
+    P->scalars.ensure((size_t)4 * cap);
+    P->bval.ensure((size_t)P->z.E * cap);
+    P->bidx.ensure((size_t)P->z.E * cap);
+    P->out4.ensure((size_t)4 * cap);
+    P->geom.ensure(cap);
+    P->h_out4.resize((size_t)4 * cap);
+    P->cap = cap;
+}
+
+// ============================================================== constrained CCC batch
+static EvalGeom make_geom(const ettk_plan* P, const ettk_particle& d, const Mat4& t, int particle) {
+    EvalGeom g{};
+    const int N = P->N;
+    Mat4 fin = euler_of(d) * Mat4::from(d.matrix) * t;
+    if (mat_det(fin) == 0.0) fin = Mat4::ident();
+    const double half = N / 2.0, pos = N / 2.0;
+    // input index = fin3 * (q - half - cut z) + fin_t + pos
+    double w0[3] = {-half, -half, -half - d.cut_offset};
+    for (int r = 0; r < 3; r++) {
+        g.m[0 + r] = (float)fin(r, 0);
+        g.m[3 + r] = (float)fin(r, 1);
+        g.m[6 + r] = (float)fin(r, 2);
+        g.m[9 + r] = (float)(fin(r, 0) * w0[0] + fin(r, 1) * w0[1] + fin(r, 2) * w0[2] + fin(r, 3) + pos);
+    }
+    WedgeRef w{};
+    wedge_normals(d.lwedge, d.uwedge, mat_inverse(fin), w);
+    for (int r = 0; r < 3; r++) {
+        g.v1[r] = w.v1[r];
+        g.v2[r] = w.v2[r];
+    }
+    g.full = w.full || !P->p.missing_wedge;
+    g.particle = particle;
+    return g;
+}
+
+void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx, Mat4* t,
+               float* dist) {
+    ETTK_REQUIRE(P->have_ref, ETTK_ERR_STATE, "reference not set");
+    const int N = P->N, H = P->H;
+    const long nh = (long)N * N * H;
+    hipStream_t s = P->ctx->stream;
+    const int CH = 128;
+    ensure_ccc_workspace(P, std::min(n, CH));
+    std::vector<EvalGeom> g;
+    for (int e0 = 0; e0 < n; e0 += P->cap) {
+        int E = std::min(P->cap, n - e0);
+        g.resize(E);
+        for (int e = 0; e < E; e++) g[e] = make_geom(P, descs[pidx[e0 + e]], t[e0 + e], pidx[e0 + e]);
+        ETTK_HIP(hipMemcpyAsync(P->geom.p, g.data(), E * sizeof(EvalGeom), hipMemcpyHostToDevice, s));
+        ETTK_HIP(hipEventRecord(P->ev0, s));
+        const float* W1 = P->p.missing_wedge ? P->W1.p : nullptr;
+        int apply = P->dummy_filter ? 0 : 1;
+        // reference side: F1 = mcf(F1raw) * filter * W1 W2 -> r1 = c2r -> k1_m, r1 m1 -> F1m
+        launch_spec_filter(P->F1mcf.p, 0, P->S1.p, P->filter.p, W1, P->geom.p, E, N, 0, apply, P->p_n1.p, P->p_wf.p,
+                           P->nb_spec, s);
+        fft3_c2r(P->ctx->fft, P->S1.p, P->S1.p, P->R1.p, N, E, s);
+        launch_mask_ref(P->R1.p, P->m1.p, E, N, P->p_k1.p, P->nb_real, s);
+        R2CArgs a1{P->R1.p, P->S1.p, N, E, PRO_NONE, nullptr, nullptr};
+        fft3_r2c(P->ctx->fft, a1, nullptr, s);
+        // particle side
+        launch_reslice(d_parts, P->geom.p, E, N, P->R2.p, P->p_res.p, P->nb_real, s);
+        launch_reslice_mean(P->p_res.p, P->nb_real, E, P->stats.p, s);
+        R2CArgs a2{P->R2.p, P->S2.p, N, E, PRO_RESLICED, P->maskfft.p, P->stats.p};
+        fft3_r2c(P->ctx->fft, a2, nullptr, s);
+        launch_spec_filter(P->S2.p, (int)nh, P->S2.p, P->filter.p, W1, P->geom.p, E, N, P->p.mcf, apply, P->p_n2.p,
+                           nullptr, P->nb_spec, s);
+        fft3_c2r(P->ctx->fft, P->S2.p, P->S3.p, P->R2.p, N, E, s);
+        R2CArgs a3{P->R2.p, P->S3.p, N, E, PRO_SQUARE_SHIFT, nullptr, nullptr};
+        fft3_r2c(P->ctx->fft, a3, nullptr, s);
+        // combine, upsampled correlation on the search window, peak
+        launch_ccc_scalars(P->p_n1.p, P->p_n2.p, P->p_k1.p, P->p_wf.p, P->nb_spec, P->nb_real, E, N, P->scalars.p, s);
+        launch_combine_dftA(P->S1.p, P->S2.p, P->S3.p, P->Mfft.p, P->scalars.p, P->tabA.p, P->z, E, P->G1.p, s);
+        launch_dftBC_argmax(P->G1.p, P->tabA.p, P->tabK.p, P->z, E, P->bval.p, P->bidx.p, s);
+        launch_peak_final(P->bval.p, P->bidx.p, P->scalars.p, (float)P->sum_filter,
+                          (P->p.overlap_norm && P->p.missing_wedge) ? 1 : 0, P->z, E, P->out4.p, s);
+        ETTK_HIP(hipEventRecord(P->ev1, s));
+        ETTK_HIP(hipMemcpyAsync(P->h_out4.data(), P->out4.p, (size_t)4 * E * sizeof(float), hipMemcpyDeviceToHost, s));
+        ETTK_HIP(hipStreamSynchronize(s));
+        float ms = 0;
+        ETTK_HIP(hipEventElapsedTime(&ms, P->ev0, P->ev1));
+        P->timing.ccc_ms += ms;
+        P->timing.n_ccc += E;
+        for (int e = 0; e < E; e++) {
+            dist[e0 + e] = P->h_out4[e * 4];
+            double sx = P->h_out4[e * 4 + 1], sy = P->h_out4[e * 4 + 2], sz = P->h_out4[e * 4 + 3];
+            t[e0 + e] = t[e0 + e] * mat_translate(-sx, -sy, -sz);
+        }
+    }
+}
+
+}  // namespace ettk
+
+// ============================================================== C ABI
+using namespace ettk;
+
+#define ETTK_API_BEGIN try {
+#define ETTK_API_END                                     \
+    }                                                    \
+    catch (const ettk::Error& e) {                       \
+        set_error(e.msg);                                \
+        return e.code;                                   \
+    }                                                    \
+    catch (const std::exception& e) {                    \
+        set_error(e.what());                             \
+        return ETTK_ERR_ALLOC;                           \
+    }                                                    \
+    set_error("");                                       \
+    return ETTK_OK;
+
+extern "C" {
+
+const char* ettk_last_error(void) { return g_err.c_str(); }
+
+int ettk_ctx_create(int device, ettk_ctx** out) {
+    ETTK_API_BEGIN
+    ETTK_REQUIRE(out != nullptr, ETTK_ERR_ARG, "ettk_ctx_create: null output");
+    int n = 0;
+    ETTK_HIP(hipGetDeviceCount(&n));
+    ETTK_REQUIRE(device >= 0 && device < n, ETTK_ERR_ARG, "ettk_ctx_create: no such HIP device");
+    ETTK_HIP(hipSetDevice(device));
+    auto* c = new ettk_ctx();
+    c->device = device;
+    ETTK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    fft_tables_init(c->fft, c->stream);
+    *out = c;
+    ETTK_API_END
+}
+
+int ettk_ctx_destroy(ettk_ctx* ctx) {
+    ETTK_API_BEGIN
+    if (ctx) {
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->fft.tw.release();
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    }
+    ETTK_API_END
+}
+
+int ettk_plan_create(ettk_ctx* ctx, const ettk_params* params, ettk_plan** out) {
+    ETTK_API_BEGIN
+    ETTK_REQUIRE(ctx && params && out, ETTK_ERR_ARG, "ettk_plan_create: null argument");
+    const ettk_params& p = *params;
+    ETTK_REQUIRE(p.N >= 16 && p.N % 2 == 0 && fft_len_index(p.N) >= 0 && fft_len_index(2 * p.N) >= 0, ETTK_ERR_UNSUPPORTED,
+                 "ettk_plan_create: box size must be one of 16, 32, 48, 64, 96, 128");
+    ETTK_REQUIRE(p.B >= 4 && fft_len_index(2 * p.B) >= 0, ETTK_ERR_UNSUPPORTED,
+                 "ettk_plan_create: bandwidth must be one of 8, 16, 24, 32, 48, 64, 96, 128");
+    ETTK_REQUIRE(p.n_candidates >= 0 && p.n_candidates <= ETTK_MAX_CANDIDATES, ETTK_ERR_ARG,
+                 "ettk_plan_create: n_candidates out of range");
+    ETTK_HIP(hipSetDevice(ctx->device));
+    auto* P = new ettk_plan();
+    try {
+        P->ctx = ctx;
+        P->p = p;
+        P->N = p.N;
+        P->H = p.N / 2 + 1;
+        P->trans_restrict = p.trans_restrict < 0 ? (double)FLT_MAX : (double)p.trans_restrict;
+        P->rot_restrict = std::max((double)p.rot_restrict, 1.0);
+        ETTK_HIP(hipEventCreate(&P->ev0));
+        ETTK_HIP(hipEventCreate(&P->ev1));
+        plan_constants(P);
+    } catch (...) {
+        delete P;
+        throw;
+    }
+    *out = P;
+    ETTK_API_END
+}
+
+int ettk_plan_destroy(ettk_plan* P) {
+    ETTK_API_BEGIN
+    if (P) {
+        (void)hipStreamSynchronize(P->ctx->stream);
+        so3_state_destroy(P);
+        if (P->ev0) (void)hipEventDestroy(P->ev0);
+        if (P->ev1) (void)hipEventDestroy(P->ev1);
+        delete P;
+    }
+    ETTK_API_END
+}
+
+int ettk_reference_set(ettk_plan* P, const float* d_ref, const ettk_particle* ref_desc) {
+    ETTK_API_BEGIN
+    ETTK_REQUIRE(P && d_ref && ref_desc, ETTK_ERR_ARG, "ettk_reference_set: null argument");
+    ETTK_HIP(hipSetDevice(P->ctx->device));
+    const int N = P->N, H = P->H;
+    const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * H;
+    hipStream_t s = P->ctx->stream;
+    P->refd = *ref_desc;
+    P->ref.ensure(n3);
+    launch_xfast_to_internal(d_ref, P->ref.p, N, s);
+    // W1: wedge of the reference at its own alignment (FIM.h:1101-1105)
+    P->W1.ensure(nh);
+    WedgeRef w{};
+    Mat4 fin = euler_of(*ref_desc) * Mat4::from(ref_desc->matrix);
+    wedge_normals(ref_desc->lwedge, ref_desc->uwedge, mat_inverse(fin), w);
+    launch_wedge_ref(w, N, P->W1.p, s);
+    // F1raw = r2c(I1 * maskfft * shift), MCF applied once (executeHalf order: MCF, then filter x wedge)
+    P->F1mcf.ensure(nh);
+    R2CArgs a{P->ref.p, P->F1mcf.p, N, 1, PRO_MASK_SHIFT, P->maskfft.p, nullptr};
+    fft3_r2c(P->ctx->fft, a, nullptr, s);
+    if (P->p.mcf) launch_mcf_inplace(P->F1mcf.p, (long)nh, s);
+    ETTK_HIP(hipStreamSynchronize(s));
+    P->have_ref = true;
+    so3_reference_set(P);
+    ETTK_API_END
+}
+
+int ettk_ccc_eval_batch(ettk_plan* P, const float* d_particles, int n_particles, const ettk_particle* descs,
+                        int n_evals, const int* particle_index, const double* transforms, float* out_distance,
+                        double* out_transforms) {
+    ETTK_API_BEGIN
+    ETTK_REQUIRE(P && d_particles && descs && particle_index && transforms && out_distance && out_transforms,
+                 ETTK_ERR_ARG, "ettk_ccc_eval_batch: null argument");
+    ETTK_REQUIRE(n_evals >= 0 && n_particles > 0, ETTK_ERR_ARG, "ettk_ccc_eval_batch: bad sizes");
+    for (int e = 0; e < n_evals; e++)
+        ETTK_REQUIRE(particle_index[e] >= 0 && particle_index[e] < n_particles, ETTK_ERR_ARG,
+                     "ettk_ccc_eval_batch: particle index out of range");
+    ETTK_HIP(hipSetDevice(P->ctx->device));
+    std::vector<Mat4> t(n_evals);
+    for (int e = 0; e < n_evals; e++) t[e] = Mat4::from(transforms + (size_t)e * 16);
+    P->timing = Timing{};
+    ccc_batch(P, d_particles, descs, n_evals, particle_index, t.data(), out_distance);
+    for (int e = 0; e < n_evals; e++) std::memcpy(out_transforms + (size_t)e * 16, t[e].a, sizeof(t[e].a));
+    ETTK_API_END
+}
+
+int ettk_last_timing(ettk_plan* P, double* ccc_ms, long long* n_ccc, double* total_ms) {
+    ETTK_API_BEGIN
+    ETTK_REQUIRE(P, ETTK_ERR_ARG, "ettk_last_timing: null plan");
+    if (ccc_ms) *ccc_ms = P->timing.ccc_ms;
+    if (n_ccc) *n_ccc = P->timing.n_ccc;
+    if (total_ms) *total_ms = P->timing.total_ms;
+    ETTK_API_END
+}
+
+}  // extern "C"

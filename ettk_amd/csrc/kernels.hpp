@@ -1,0 +1,60 @@
+// kernels.hpp -- launch wrappers of the alignment kernels (device code in *.hip).
+#pragma once
+#include "common.hpp"
+
+namespace ettk {
+
+// per-evaluation geometry computed on the host in FP64
+struct EvalGeom {
+    float m[12];      // reslice map: input index = m[0..2]*i + m[3..5]*j + m[6..8]*k + m[9..11]  (rows x,y,z)
+    float v1[3];      // rotated wedge plane normals of the particle (nbfWedge3D::getImageHalf)
+    float v2[3];
+    int full;         // particle wedge spans >= 180 degrees (W2 == 1)
+    int particle;     // index of the particle volume
+};
+
+// smoothed analytic wedge parameters of the reference
+struct WedgeRef {
+    float v1[3], v2[3];
+    int full;
+};
+
+struct CccSizes {
+    int N, H, E;          // box, half length, crop edge of the upsampled search window
+    int I0;               // first crop index on the (2N) grid
+    float restr2;         // restriction^2 for the spherical search mask
+    int ub2;              // (E-1)/2
+};
+
+// reslice (tricubic Keys a=-1/2, clamp border, -FLT_MAX outside): out [e][i][j][k]
+void launch_reslice(const float* parts, const EvalGeom* geom, int n_eval, int N, float* out, float* partials,
+                    int nb, hipStream_t s);
+// mean of the valid voxels per evaluation, from the reslice partials (nb per eval): stats[e] = mean
+void launch_reslice_mean(const float* partials, int nb, int n_eval, float* stats, hipStream_t s);
+// spectral filter of one side: X = (mcf ? F/|F|^.5 : F) * filter * W1 * W2(e); DC = 0; writes Hermitian
+// weighted |X|^2 partial sums (and sum(W*filter) when `wf_partials`)
+void launch_spec_filter(const float2* F, int f_batch_stride, float2* X, const float* filter, const float* W1,
+                        const EvalGeom* geom, int n_eval, int N, int mcf, int apply_filter, float* partials,
+                        float* wf_partials, int nb, hipStream_t s);
+// r1 *= m1, partial sums of r1^2 m1
+void launch_mask_ref(float* R, const float* m1, int n_eval, int N, float* partials, int nb, hipStream_t s);
+// per-evaluation scalars: norm1, norm2, k1_m, sum(W filter)
+void launch_ccc_scalars(const float* p_n1, const float* p_n2, const float* p_k1, const float* p_wf, int nb_spec,
+                        int nb_real, int n_eval, int N, float* scalars, hipStream_t s);
+// C = (-M conj(Q)/n2^2 + 2 F1m/n1 conj(F2/n2 N^3)) (-1)^(i+j+k), then G1[e][x][j][k] = sum_i A[x][i] C(i,j,k)
+void launch_combine_dftA(const float2* F1m, const float2* F2, const float2* Q, const float2* Mfft,
+                         const float* scalars, const float2* tabA, const CccSizes& z, int n_eval, float2* G1,
+                         hipStream_t s);
+// G2 = sum_j A[y][j] G1, T = sum_k Re(G2 K[z][k]) on the crop, spherical mask, first argmax per (e, x)
+void launch_dftBC_argmax(const float2* G1, const float2* tabA, const float2* tabK, const CccSizes& z, int n_eval,
+                         float* best_val, int* best_idx, hipStream_t s);
+void launch_wedge_ref(const WedgeRef& w, int N, float* W, hipStream_t s);
+void launch_xfast_to_internal(const float* in, float* out, int N, hipStream_t s);
+void launch_internal_to_xfast(const float* in, float* out, int N, hipStream_t s);
+void launch_mcf_inplace(float2* F, long n, hipStream_t s);
+// final per evaluation: first maximum over x, distance and shift
+void launch_peak_final(const float* best_val, const int* best_idx, const float* scalars, float sum_filter,
+                       int overlap, const CccSizes& z, int n_eval, float* out /* [e][4] = dist, sx, sy, sz */,
+                       hipStream_t s);
+
+}  // namespace ettk

@@ -1,0 +1,91 @@
+// plan.hpp -- host-side state of libettk_align: device context, per-size plan, FP64 transform helpers.
+#pragma once
+#include <array>
+#include <cmath>
+#include <cstring>
+
+#include "kernels.hpp"
+
+namespace ettk {
+
+// ------------------------------------------------------------------ 4x4 (vtkTransform semantics, FP64)
+struct Mat4 {
+    double a[16];
+    double& operator()(int r, int c) { return a[r * 4 + c]; }
+    double operator()(int r, int c) const { return a[r * 4 + c]; }
+    static Mat4 ident() {
+        Mat4 M;
+        for (int i = 0; i < 16; i++) M.a[i] = (i % 5 == 0) ? 1.0 : 0.0;
+        return M;
+    }
+    static Mat4 from(const double* p) {
+        Mat4 M;
+        std::memcpy(M.a, p, sizeof(M.a));
+        return M;
+    }
+};
+Mat4 operator*(const Mat4& A, const Mat4& B);
+Mat4 mat_rot(double deg, double x, double y, double z);  // vtkTransformConcatenation::Rotate (quaternion)
+Mat4 mat_translate(double x, double y, double z);
+Mat4 mat_scale(double x, double y, double z);
+Mat4 mat_inverse(const Mat4& A);
+double mat_det(const Mat4& A);
+Mat4 strip_translation(const Mat4& t);
+Mat4 zyz(double a, double b, double g);  // RotateZ(-g) RotateY(-b) RotateZ(-a), pre-multiplied
+bool transform_ok(const Mat4& t, double trans_restrict, double rot_restrict);  // nbfImageMetric::isTransformValid
+Mat4 euler_of(const ettk_particle& d);   // RotateZ(-n2) RotateX(-n0) RotateZ(-n1)
+
+// ------------------------------------------------------------------ state
+struct Timing {
+    double ccc_ms = 0, total_ms = 0;
+    long long n_ccc = 0;
+};
+
+}  // namespace ettk
+
+struct ettk_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    ettk::FftTables fft;
+};
+
+struct ettk_plan {
+    ettk_ctx* ctx = nullptr;
+    ettk_params p{};
+    int N = 0, H = 0;
+    bool dummy_filter = true;
+    double sum_filter = 0;
+    double trans_restrict = 0, rot_restrict = 180;
+    ettk::CccSizes z{};
+    // constant device data
+    ettk::DevBuf<float> maskfft, m1, filter, W1;
+    ettk::DevBuf<float2> Mfft, F1mcf, tabA, tabK;
+    std::vector<float> h_m1;
+    // reference
+    bool have_ref = false;
+    ettk_particle refd{};
+    ettk::DevBuf<float> ref;
+    // CCC workspace
+    int cap = 0;
+    int nb_real = 0, nb_spec = 0;
+    ettk::DevBuf<float> R1, R2, p_res, p_n1, p_n2, p_k1, p_wf, stats, scalars, bval, out4;
+    ettk::DevBuf<int> bidx;
+    ettk::DevBuf<float2> S1, S2, S3, G1;
+    ettk::DevBuf<ettk::EvalGeom> geom;
+    std::vector<float> h_out4;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ettk::Timing timing;
+    // S^2 / SO(3) state (so3.hip)
+    struct So3State* so3 = nullptr;
+    // averaging accumulators
+    ettk::DevBuf<float> avg_re, avg_im, avg_acc;
+};
+
+namespace ettk {
+// constrained CCC for a batch of evaluations: on return t[e] <- t[e] * Translate(-shift), dist[e]
+void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
+               Mat4* t, float* dist);
+void ensure_ccc_workspace(ettk_plan* P, int cap);
+void so3_state_destroy(ettk_plan* P);
+void so3_reference_set(ettk_plan* P);
+}  // namespace ettk

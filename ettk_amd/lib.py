@@ -1,0 +1,164 @@
+"""ctypes binding of libettk_align.so (the C ABI of include/ettk_align.h).
+
+Device buffers are torch tensors on the HIP device; the library receives their data pointers.  Importing
+this module never falls back to another implementation: if the shared library is missing, loading fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from .abi import Params, Particle, Result, particle_array
+
+LIB_PATH = Path(__file__).resolve().parent / "libettk_align.so"
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m ettk_amd.build`")
+        lib = C.CDLL(str(LIB_PATH))
+        vp, ip, dp, fp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_float)
+        sig = {
+            "ettk_last_error": ([], C.c_char_p),
+            "ettk_ctx_create": ([C.c_int, C.POINTER(vp)], C.c_int),
+            "ettk_ctx_destroy": ([vp], C.c_int),
+            "ettk_plan_create": ([vp, C.POINTER(Params), C.POINTER(vp)], C.c_int),
+            "ettk_plan_destroy": ([vp], C.c_int),
+            "ettk_reference_set": ([vp, vp, C.POINTER(Particle)], C.c_int),
+            "ettk_ccc_eval_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, ip, dp, fp, dp], C.c_int),
+            "ettk_project_batch": ([vp, vp, C.c_int, C.POINTER(Particle), fp], C.c_int),
+            "ettk_so3_correlate_batch": ([vp, C.c_int, dp, dp, dp], C.c_int),
+            "ettk_align_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, C.POINTER(Result)], C.c_int),
+            "ettk_ccc_grad_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, ip, dp, dp, dp, dp], C.c_int),
+            "ettk_avg_reset": ([vp], C.c_int),
+            "ettk_avg_accumulate": ([vp, vp, C.c_int, C.POINTER(Particle), fp, dp], C.c_int),
+            "ettk_avg_finalize": ([vp, vp], C.c_int),
+            "ettk_last_timing": ([vp, dp, C.POINTER(C.c_longlong), dp], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = lib
+    return _lib
+
+
+class EttkError(RuntimeError):
+    pass
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise EttkError(f"ettk error {rc}: {load().ettk_last_error().decode()}")
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        self.h = C.c_void_p()
+        check(self.lib.ettk_ctx_create(device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            check(self.lib.ettk_ctx_destroy(self.h))
+            self.h = C.c_void_p()
+
+
+class Plan:
+    """One box size / bandwidth / parameter set (ettk_plan)."""
+
+    def __init__(self, ctx: Context, params: Params):
+        self.lib = ctx.lib
+        self.ctx = ctx
+        self.params = params
+        self.h = C.c_void_p()
+        check(self.lib.ettk_plan_create(ctx.h, C.byref(params), C.byref(self.h)))
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            check(self.lib.ettk_plan_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def set_reference(self, d_ref, desc: Particle | None = None):
+        desc = desc or Particle()
+        self._keep = [d_ref]
+        check(self.lib.ettk_reference_set(self.h, C.c_void_p(d_ref.data_ptr()), C.byref(desc)))
+
+    def ccc(self, d_parts, descs, pidx, transforms):
+        n = len(pidx)
+        arr = particle_array(descs)
+        pidx = np.ascontiguousarray(pidx, dtype=np.int32)
+        t = np.ascontiguousarray(np.asarray(transforms, dtype=np.float64).reshape(n, 16))
+        dist = np.zeros(n, np.float32)
+        tout = np.zeros((n, 16), np.float64)
+        check(self.lib.ettk_ccc_eval_batch(self.h, C.c_void_p(d_parts.data_ptr()), d_parts.shape[0], arr, n,
+                                           _ip(pidx), _dp(t), _fp(dist), _dp(tout)))
+        return dist, tout.reshape(n, 4, 4)
+
+    def project(self, d_parts, descs):
+        n = d_parts.shape[0]
+        B = self.params.B
+        out = np.zeros((n, 2 * B, 2 * B), np.float32)
+        check(self.lib.ettk_project_batch(self.h, C.c_void_p(d_parts.data_ptr()), n, particle_array(descs), _fp(out)))
+        return out
+
+    def so3_correlate(self, sig: np.ndarray, pat: np.ndarray) -> np.ndarray:
+        B = self.params.B
+        sig = np.ascontiguousarray(sig, dtype=np.float64).reshape(-1, 2 * B, 2 * B)
+        pat = np.ascontiguousarray(pat, dtype=np.float64).reshape(-1, 2 * B, 2 * B)
+        n = sig.shape[0]
+        out = np.zeros((n, 2 * B, 2 * B, 2 * B), np.float64)
+        check(self.lib.ettk_so3_correlate_batch(self.h, n, _dp(sig), _dp(pat), _dp(out)))
+        return out
+
+    def align(self, d_parts, descs, mode: int = 0):
+        n = d_parts.shape[0]
+        res = (Result * n)()
+        check(self.lib.ettk_align_batch(self.h, C.c_void_p(d_parts.data_ptr()), n, particle_array(descs), mode, res))
+        return list(res)
+
+    def ccc_grad(self, d_parts, descs, pidx, seeds, params6):
+        n = len(pidx)
+        pidx = np.ascontiguousarray(pidx, dtype=np.int32)
+        seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.float64).reshape(n, 16))
+        x = np.ascontiguousarray(np.asarray(params6, dtype=np.float64).reshape(n, 6))
+        val = np.zeros(n, np.float64)
+        grad = np.zeros((n, 6), np.float64)
+        check(self.lib.ettk_ccc_grad_batch(self.h, C.c_void_p(d_parts.data_ptr()), d_parts.shape[0],
+                                           particle_array(descs), n, _ip(pidx), _dp(seeds), _dp(x), _dp(val),
+                                           _dp(grad)))
+        return val, grad
+
+    def average(self, d_parts, descs, weights, matrices, d_out):
+        n = d_parts.shape[0]
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        m = np.ascontiguousarray(np.asarray(matrices, dtype=np.float64).reshape(n, 16))
+        check(self.lib.ettk_avg_reset(self.h))
+        check(self.lib.ettk_avg_accumulate(self.h, C.c_void_p(d_parts.data_ptr()), n, particle_array(descs), _fp(w),
+                                           _dp(m)))
+        check(self.lib.ettk_avg_finalize(self.h, C.c_void_p(d_out.data_ptr())))
+
+    def last_timing(self):
+        ms = C.c_double()
+        n = C.c_longlong()
+        tot = C.c_double()
+        check(self.lib.ettk_last_timing(self.h, C.byref(ms), C.byref(n), C.byref(tot)))
+        return ms.value, n.value, tot.value

@@ -1,0 +1,159 @@
+/*
+ * ettk_align.h -- C ABI of the MI355X sub-volume alignment library (libettk_align.so).
+ *
+ * The library replaces the hot path that ETTK runs behind nbfCorrelationImageMetric<float,3>
+ * (inc/em/nbfCorrelationImageMetric.h): the per-pair metric nbfProjectionRotationMetric3D::execute
+ * (inc/em/nbfProjectionRotationMetric3D.h:416-1052), the constrained CCC
+ * nbfFourierImageMetric::executeFourierNewHalf (inc/em/nbfFourierImageMetric.h:1078-1323) and the
+ * amplitude-weighted Fourier average nbfWedgedAverageImage3D::getImageMPI
+ * (inc/em/nbfWedgedAverageImage3D.h:261-356).  The reference has no FFI of its own; these entry points
+ * are what a C/C++ caller (or the ctypes binding shown in INTEGRATION.md) binds in place of the templated
+ * classes.  Every function returns 0 (ETTK_OK) or a negative ETTK_ERR_* code; the text of the last error
+ * of the calling thread is returned by ettk_last_error().  No exceptions cross this boundary.
+ *
+ * Conventions shared by every entry point
+ *   - Volumes are float32 N^3 arrays in VTK/MRC order, x fastest: v[(z*N + y)*N + x], the order in which
+ *     nbfMrcReader hands data to the reference (inc/io/nbfVTKInterface.cpp:77,134-135).
+ *   - Arguments named d_* are device (HBM) pointers on the plan's GPU; all other pointers are host memory.
+ *   - 4x4 transforms are row-major doubles, exactly vtkMatrix4x4::DeepCopy(double[16], ...) order.
+ *   - Angles are degrees, as in the reference.
+ *   - One plan owns one HIP stream; calls on one plan are serialised by the caller (the reference metric
+ *     is not re-entrant either: inc/em/nbfWedgedSubImage3D.h:172-201).
+ */
+#ifndef ETTK_ALIGN_H
+#define ETTK_ALIGN_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ETTK_OK                0
+#define ETTK_ERR_ARG          -1
+#define ETTK_ERR_HIP          -2
+#define ETTK_ERR_ALLOC        -3
+#define ETTK_ERR_STATE        -4
+#define ETTK_ERR_UNSUPPORTED  -5
+
+#define ETTK_MAX_CANDIDATES    8
+
+/* Alignment parameters.  Each field names the reference setter it mirrors. */
+typedef struct ettk_params {
+    int   N;               /* cubic box size, even (nbfFourierFilter::initializeFFTWhalf asserts even)   */
+    int   B;               /* S^2 bandwidth; nbfProjectionRotationMetric3D::B (public float member)     */
+    float bp_lo;           /* nbfFourierFilter::bandPassOn(lowCut, highCut, lowVariance, highVariance)    */
+    float bp_hi;           /*   active only when lowCut > 0 or highCut < 1 (FFilt.h:700-712)              */
+    float bp_lo_decay;
+    float bp_hi_decay;
+    float bfactor;         /* nbfFourierFilter::bfactorOn; 0 = off                                        */
+    float mask_radius;     /* nbfImageFilter::setMaskSize(x,x,x,sigma); 0 = mask off (mask == 1)          */
+    float mask_sigma;      /*   raised-cosine width of the spherical mask (IFilt.h:426-446)               */
+    int   square_mask;     /* nbfImageFilter::squareMaskSize (edge of the FFT window, FIM.h:642-664)      */
+    int   mcf;             /* setToUseMutualCorrelation: F <- F/|F|^(1/2)                                 */
+    int   overlap_norm;    /* setToComputeOverlapNormalizedDistances                                      */
+    int   missing_wedge;   /* setMissingWedgeCompensation                                                 */
+    int   n_peaks;         /* setNumberOfCandidatePeaksToSearch (Kp, default 5)                           */
+    int   n_candidates;    /* setNumberOfCandidates (Kc, default 1), <= ETTK_MAX_CANDIDATES              */
+    float rot_restrict;    /* setRotationSearchRestriction, degrees (clamped to >= 1 as in the reference) */
+    float trans_restrict;  /* setTranslationSearchRestriction, voxels; < 0 means unrestricted             */
+    int   refine;          /* refinementOn()/Off(): Powell refinement of the Kc best candidates          */
+} ettk_params;
+
+/* One particle (nbfWedgedSubImage3D held as a fixed image, WSub.h:124-148) or the reference. */
+typedef struct ettk_particle {
+    float  lwedge;         /* nbfWedge3D::set(lwedge, uwedge): missing-wedge limits about Y, degrees     */
+    float  uwedge;         /*   uwedge - lwedge >= 180 means no missing wedge                             */
+    float  normal[3];      /* Euler "normal" angles of the sub-volume (WSub.h:741-749); 0 for fixed images */
+    float  cut_offset;     /* nbfWedgedSubImage3D::setCutOffset                                            */
+    double matrix[16];     /* current alignment (nbfWedgedSubImage3D::setTransform), row-major            */
+} ettk_particle;
+
+/* Result of one pairwise alignment (reference x particle), the row of D in getDistances
+   (CorrMetric.h:2509-2530): distance, then the 4x4 of each refined candidate. */
+typedef struct ettk_result {
+    float  distance;                                 /* correlationPeak of the best candidate          */
+    double matrix[16];                               /* best transform (metric->getTransform())        */
+    int    n_cand;                                   /* candidates refined (<= Kc)                     */
+    float  cand_distance[ETTK_MAX_CANDIDATES];       /* candidateCorrelationPeaks                      */
+    double cand_matrix[ETTK_MAX_CANDIDATES][16];     /* candidateTransforms                            */
+    float  so3_angles[ETTK_MAX_CANDIDATES][3];       /* (alpha, beta, gamma) of each refined candidate */
+    int    n_ccc;                                    /* constrained-CCC evaluations spent on the pair  */
+    int    use_window;                               /* bit0: reference window, bit1: particle window  */
+} ettk_result;
+
+typedef struct ettk_ctx  ettk_ctx;
+typedef struct ettk_plan ettk_plan;
+
+/* Text of the last error raised on the calling thread ("" when none). */
+const char* ettk_last_error(void);
+
+/* Device context: selects the HIP device, creates the stream. */
+int ettk_ctx_create(int device, ettk_ctx** out);
+int ettk_ctx_destroy(ettk_ctx* ctx);
+
+/* Plan for one box size and bandwidth: masks, filters, FFT twiddles, Legendre and Wigner-d tables.
+   Replaces the lazily built state of nbfFourierImageMetric::reinitializeHalf (FIM.h:634-694),
+   nbfFourierFilter::updateFilterHalf (FFilt.h:492-547) and
+   nbfProjectionRotationMetric3D::initializeHarmonics (ProjRot.h:1055-1134). */
+int ettk_plan_create(ettk_ctx* ctx, const ettk_params* params, ettk_plan** out);
+int ettk_plan_destroy(ettk_plan* plan);
+
+/* Reference volume (input1 of the metric): setInput1 (nbfImageMetric.h:184-189).  The volume is used as
+   is (an average is already RMS-normalised, WAvg.h:241-243); its wedge is analytic from desc->l/uwedge.
+   Caches the reference side of the CCC (F1m, k1_m: FIM.h:1123-1153) and its S^2 projection Pp1
+   (ProjRot.h:445-453). */
+int ettk_reference_set(ettk_plan* plan, const float* d_ref, const ettk_particle* ref_desc);
+
+/* Constrained CCC at given transforms: nbfFourierImageMetric::executeFourierNewHalf (FIM.h:1078-1323)
+   for n_evals (particle, transform) pairs.  transforms[e*16] is the candidate transform; on return
+   out_transforms[e*16] holds transform * Translate(-shift) (FIM.h:1312-1322) and out_distance[e] the
+   correlationPeak.  particle_index[e] selects the particle in d_particles (n_particles volumes). */
+int ettk_ccc_eval_batch(ettk_plan* plan, const float* d_particles, int n_particles,
+                        const ettk_particle* descs, int n_evals, const int* particle_index,
+                        const double* transforms, float* out_distance, double* out_transforms);
+
+/* S^2 projection of |FFT| (nbfProjectionRotationMetric3D::cartesianToSpherical, ProjRot.h:265-413) of each
+   particle at its current alignment: out_pp is n x (2B x 2B) float, row-major [polar][azimuth]. */
+int ettk_project_batch(ettk_plan* plan, const float* d_particles, int n, const ettk_particle* descs,
+                       float* out_pp);
+
+/* SO(3) correlation of S^2 functions on the Driscoll-Healy grid (ProjRot.h:1137-1295, soft-1.0
+   FST_semi_memo + so3CombineCoef + Inverse_SO3_Naive): for each of n pairs, signal sig[i] and pattern
+   pat[i] (2B x 2B doubles, host) give out_c[i] ((2B)^3 doubles, layout [alpha][beta][gamma]). */
+int ettk_so3_correlate_batch(ettk_plan* plan, int n, const double* sig, const double* pat, double* out_c);
+
+/* Full pairwise alignment of n particles against the reference: nbfProjectionRotationMetric3D::execute
+   (ProjRot.h:416-1052) as called from getDistances type 0/3 (CorrMetric.h:1874-2008).
+   mode 0 = plain: particle by particle, Powell refinement driven from the host (the reference method);
+   mode 1 = batched: all particles' stages in shared launches, Powell refinement in lock-step batches;
+   mode 2 = batched + adjoint: as 1, refinement by projected gradient ascent on the continuous CCC using
+            the adjoint kernel (no reference counterpart; see ettk_ccc_grad_batch). */
+int ettk_align_batch(ettk_plan* plan, const float* d_particles, int n, const ettk_particle* descs,
+                     int mode, ettk_result* out);
+
+/* Continuous constrained CCC and its gradient with respect to (rotX, rotY, rotZ, tx, ty, tz) around the
+   seed transforms (degrees, voxels), through a trilinear rotation of the particle.  No reference
+   counterpart: it replaces the Powell line searches of nbfFourierImageMetric::refine (FIM.h:1702-1809).
+   params6[e*6] = (rx, ry, rz, tx, ty, tz); seeds[e*16]; out_value[e] = masked CCC score (higher is better);
+   out_grad[e*6]. */
+int ettk_ccc_grad_batch(ettk_plan* plan, const float* d_particles, int n_particles,
+                        const ettk_particle* descs, int n_evals, const int* particle_index,
+                        const double* seeds, const double* params6, double* out_value, double* out_grad);
+
+/* Amplitude-weighted Fourier average (WAvg.h:261-356 and CorrMetric.h:927-980).
+   ettk_avg_reset zeroes the (re, im, acc) half-spectrum accumulators; ettk_avg_accumulate adds n particles
+   resliced with matrices[i*16] and weighted by weights[i]; ettk_avg_finalize writes the RMS-normalised
+   average (N^3 float, x fastest) to d_out. */
+int ettk_avg_reset(ettk_plan* plan);
+int ettk_avg_accumulate(ettk_plan* plan, const float* d_particles, int n, const ettk_particle* descs,
+                        const float* weights, const double* matrices);
+int ettk_avg_finalize(ettk_plan* plan, float* d_out);
+
+/* Per-kernel timing of the last ettk_align_batch / ettk_ccc_eval_batch call, measured with HIP events on
+   the plan's stream: total milliseconds spent in the constrained-CCC pipeline and number of CCC
+   evaluations it ran. */
+int ettk_last_timing(ettk_plan* plan, double* ccc_ms, long long* n_ccc, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ETTK_ALIGN_H */

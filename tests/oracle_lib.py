@@ -1,0 +1,132 @@
+"""ctypes loader of the FP64 CPU oracle (oracle/liboracle.so) and of the soft-1.0 checker built from the
+reference sources (oracle/_ref/libsoftref.so).  Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from ettk_amd.abi import Params, Particle, Result
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE = ROOT / "oracle" / "liboracle.so"
+SOFTREF = ROOT / "oracle" / "_ref" / "libsoftref.so"
+
+_o = None
+_r = None
+
+
+def oracle() -> C.CDLL:
+    global _o
+    if _o is None:
+        if not ORACLE.exists():
+            import subprocess
+            subprocess.run(["make", "-C", str(ROOT / "oracle"), "-s", "liboracle.so"], check=True)
+        o = C.CDLL(str(ORACLE))
+        o.orc_plan_create.restype = C.c_void_p
+        o.orc_plan_create.argtypes = [C.POINTER(Params)]
+        o.orc_plan_destroy.argtypes = [C.c_void_p]
+        o.orc_set_reference.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Particle)]
+        o.orc_ccc.restype = C.c_float
+        o.orc_ccc.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Particle), C.c_void_p]
+        o.orc_project.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Particle), C.c_int, C.c_void_p]
+        o.orc_get_pp1.argtypes = [C.c_void_p, C.c_void_p]
+        o.orc_align.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Particle), C.POINTER(Result)]
+        o.orc_align_many.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(Particle), C.POINTER(Result), C.c_int]
+        o.orc_s2_rotate.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double]
+        o.orc_wigner_d.argtypes = [C.c_int, C.c_double, C.c_void_p]
+        o.orc_reslice.argtypes = [C.c_void_p, C.c_int, C.POINTER(Particle), C.c_void_p, C.c_void_p]
+        o.orc_wedge_half.argtypes = [C.c_int, C.POINTER(Particle), C.c_void_p, C.c_void_p]
+        o.orc_spherical_wedge.argtypes = [C.c_int, C.POINTER(Particle), C.c_int, C.c_void_p]
+        o.orc_masks.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        o.orc_ccc_count.restype = C.c_int
+        o.orc_ccc_count.argtypes = [C.c_void_p]
+        o.orc_average.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(Particle), C.c_void_p, C.c_void_p,
+                                  C.c_void_p]
+        _o = o
+    return _o
+
+
+def softref():
+    """The soft-1.0 checker, or None where the reference tree was not available to build it."""
+    global _r
+    if _r is None and SOFTREF.exists():
+        r = C.CDLL(str(SOFTREF))
+        r.ref_seanindex.restype = C.c_int
+        r.ref_so3_coef_loc.restype = C.c_int
+        r.ref_total_coeffs_so3.restype = C.c_int
+        for f in ("ref_rotate",):
+            getattr(r, f).argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double]
+        _r = r
+    return _r
+
+
+def vp(a: np.ndarray) -> C.c_void_p:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class OraclePlan:
+    def __init__(self, params: Params):
+        self.o = oracle()
+        self.params = params
+        self.h = C.c_void_p(self.o.orc_plan_create(C.byref(params)))
+
+    def __del__(self):
+        try:
+            self.o.orc_plan_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_reference(self, ref: np.ndarray, desc: Particle | None = None):
+        self._ref = np.ascontiguousarray(ref, dtype=np.float32)
+        self.o.orc_set_reference(self.h, vp(self._ref), C.byref(desc or Particle()))
+
+    def ccc(self, vol: np.ndarray, desc: Particle, t: np.ndarray):
+        vol = np.ascontiguousarray(vol, dtype=np.float32)
+        tt = np.ascontiguousarray(np.asarray(t, dtype=np.float64).reshape(16)).copy()
+        d = self.o.orc_ccc(self.h, vp(vol), C.byref(desc), vp(tt))
+        return float(d), tt.reshape(4, 4)
+
+    def project(self, vol: np.ndarray, desc: Particle, is_reference: bool = False) -> np.ndarray:
+        B = self.params.B
+        out = np.zeros((2 * B, 2 * B), np.float32)
+        vol = np.ascontiguousarray(vol, dtype=np.float32)
+        self.o.orc_project(self.h, vp(vol), C.byref(desc), int(is_reference), vp(out))
+        return out
+
+    def align(self, vol: np.ndarray, desc: Particle) -> Result:
+        r = Result()
+        vol = np.ascontiguousarray(vol, dtype=np.float32)
+        self.o.orc_align(self.h, vp(vol), C.byref(desc), C.byref(r))
+        return r
+
+    def align_many(self, vols: np.ndarray, descs, nthreads: int = 0):
+        from ettk_amd.abi import particle_array
+        n = vols.shape[0]
+        res = (Result * n)()
+        vols = np.ascontiguousarray(vols, dtype=np.float32)
+        self.o.orc_align_many(self.h, n, vp(vols), particle_array(descs), res, nthreads)
+        return list(res)
+
+    def ccc_count(self) -> int:
+        return self.o.orc_ccc_count(self.h)
+
+
+def so3_correlate(B: int, sig: np.ndarray, pat: np.ndarray, scaled: bool = False) -> np.ndarray:
+    o = oracle()
+    n = 2 * B
+    C_ = np.zeros((n, n, n))
+    s = np.ascontiguousarray(sig, dtype=np.float64)
+    p = np.ascontiguousarray(pat, dtype=np.float64)
+    (o.orc_so3_correlate_scaled if scaled else o.orc_so3_correlate)(B, vp(s), vp(p), vp(C_))
+    return C_
+
+
+def s2_forward(B: int, f: np.ndarray):
+    o = oracle()
+    re = np.zeros(B * B)
+    im = np.zeros(B * B)
+    o.orc_s2_forward(B, vp(np.ascontiguousarray(f, dtype=np.float64)), vp(re), vp(im))
+    return re + 1j * im
