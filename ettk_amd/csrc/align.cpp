@@ -1,0 +1,648 @@
+// align.cpp -- host driver of the pairwise alignment (nbfProjectionRotationMetric3D::execute) on top of the
+// GPU stages: S^2 projections and SO(3) correlations in batches, candidate extraction, the direct/mirror
+// CCC test of the candidate peaks and the Powell refinement of the best candidates.
+//
+// Modes: 0 = plain (particle after particle, one CCC evaluation per launch, the reference's control flow);
+//        1 = batched (every stage over all particles; the per-particle control flow runs in one host thread per
+//            particle and every CCC request of all threads is served by one batched launch);
+//        2 = batched + adjoint refinement (refine.cpp).
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <queue>
+#include <thread>
+
+#include "so3.hpp"
+
+namespace ettk {
+
+// ------------------------------------------------------------------ spherical wedge (host, FP64)
+// nbfWedgedSubImage3D::getSphericalWedgeImage -> nbfWedge3D::getSphericalImage (Wedge.h:364-429) on the
+// grid theta_i = i pi/(2B-1), phi_j = j 2pi/(2B-1) (Pixel arithmetic), then conditionSphericalWindow
+// (ProjRot.h:1322-1345: Gaussian sigma 2, radius factor 1)
+std::vector<float> spherical_wedge_host(int B, const ettk_particle& d, bool wedge_on) {
+    const int n = 2 * B;
+    std::vector<float> W((size_t)n * n, 1.0f);
+    double l = d.lwedge < -90 ? -90.0 : d.lwedge, u = d.uwedge > 90 ? 90.0 : d.uwedge;
+    if (wedge_on && u - l < 180) {
+        Mat4 R = strip_translation(mat_inverse(euler_of(d) * Mat4::from(d.matrix)));
+        double n1[3] = {std::cos(-(90.0 - u) * M_PI / 180.0), 0.0, -std::sin(-(90.0 - u) * M_PI / 180.0)};
+        double n2[3] = {std::cos(-(90.0 - l) * M_PI / 180.0), 0.0, -std::sin(-(90.0 - l) * M_PI / 180.0)};
+        double v1[3], v2[3];
+        for (int r = 0; r < 3; r++) {
+            v1[r] = R(r, 0) * n1[0] + R(r, 1) * n1[1] + R(r, 2) * n1[2];
+            v2[r] = R(r, 0) * n2[0] + R(r, 1) * n2[1] + R(r, 2) * n2[2];
+        }
+        float phiK = (float)(M_PI / (n - 1)), thetaK = (float)(2.0 * M_PI / (n - 1));
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                float si = std::sin((float)(i * phiK)), ci = std::cos((float)(i * phiK));
+                float sj = std::sin((float)(j * thetaK)), cj = std::cos((float)(j * thetaK));
+                double a = v1[0] * (double)(cj * si) + v1[1] * (double)(sj * si) + v1[2] * (double)ci;
+                double b = v2[0] * (double)(cj * si) + v2[1] * (double)(sj * si) + v2[2] * (double)ci;
+                W[(size_t)i * n + j] = (a * b <= 0.0) ? 1.0f : 0.0f;
+            }
+    }
+    if (wedge_on) smooth3(W, n, n, 1, 2.0, 1.0);
+    return W;
+}
+
+// ------------------------------------------------------------------ candidates (ProjRot.h:1593-1702)
+struct Cand {
+    float a, b, g, v;
+};
+struct CandCmp {
+    bool operator()(const Cand& x, const Cand& y) const { return x.v > y.v; }  // greaterPD: min-heap on [3]
+};
+using CandQueue = std::priority_queue<Cand, std::vector<Cand>, CandCmp>;
+
+static void eliminate_equivalent(std::vector<Cand>& pos, CandQueue& pq, float tol) {
+    const float NEG = -FLT_MAX;
+    std::vector<Mat4> mats(pos.size());
+    for (size_t i = 0; i < pos.size(); i++) mats[i] = zyz(pos[i].a, pos[i].b, pos[i].g);
+    for (size_t i = 0; i < pos.size(); i++) {
+        for (size_t j = 0; j < pos.size(); j++) {
+            if (pos[i].v == NEG) break;
+            if (i != j && pos[j].v > NEG) {
+                double err = 0;
+                for (int m = 0; m < 16; m++) err += std::fabs(mats[i].a[m] - mats[j].a[m]);
+                if (err < tol) {
+                    if (pos[i].v > pos[j].v) pos[i].v = NEG;
+                    else pos[j].v = NEG;
+                }
+            }
+        }
+    }
+    for (auto& p : pos)
+        if (p.v > NEG) pq.push(p);
+}
+
+static void candidates_from_maxima(const ettk_plan* P, std::vector<std::pair<int, double>>& lm, CandQueue& pq) {
+    const int B = P->p.B, n = 2 * B;
+    std::sort(lm.begin(), lm.end());  // blitz iteration order (alpha slowest)
+    std::vector<Cand> pos;
+    for (auto& e : lm) {
+        int t = e.first;
+        int a = t / (n * n), b = (t / n) % n, g = t % n;
+        float alpha = (float)(a / (double)B * 180.0);
+        float beta = (float)((2 * b + 1.0) / (4. * B) * 180.0);
+        if ((float)B <= 32) beta = (float)((2 * b + 0.0) / (4. * B) * 180.0);
+        float gamma = (float)(g / (double)B * 180.0);
+        if (transform_ok(zyz(alpha, beta, gamma), P->trans_restrict, P->rot_restrict))
+            pos.push_back(Cand{alpha, beta, gamma, (float)(-e.second)});
+    }
+    eliminate_equivalent(pos, pq, 0.25f);
+}
+
+struct SearchOut {
+    CandQueue pq1, pq2;
+    bool two = false;
+    int use_window = 0;
+};
+
+// projections + SO(3) correlations + local maxima for particles [p0, p0+cn)
+static void search_chunk(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int p0, int cn,
+                         std::vector<SearchOut>& out) {
+    So3State& S = so3_state(P);
+    const int B = P->p.B, n = 2 * B, nn = n * n;
+    const long n3 = (long)n * n * n;
+    hipStream_t s = P->ctx->stream;
+    S.Pp2.ensure((size_t)cn * nn);
+    S.W2.ensure((size_t)cn * nn);
+    S.Mx.ensure(cn);
+    std::vector<int> pidx(cn);
+    for (int q = 0; q < cn; q++) pidx[q] = p0 + q;
+    project_images(P, d_parts, descs, pidx.data(), cn, false, S.Pp2.p);
+    launch_ppmax(S.Pp1.p, S.Pp2.p, cn, B, S.Mx.p, s);
+    std::vector<float> W2all((size_t)cn * nn);
+    std::vector<int> uw2(cn);
+    for (int q = 0; q < cn; q++) {
+        std::vector<float> w = spherical_wedge_host(B, descs[p0 + q], P->p.missing_wedge != 0);
+        double sw = 0;
+        for (float v : w) sw += v;
+        uw2[q] = P->p.missing_wedge && (sw <= 0.95f * (float)nn);
+        std::copy(w.begin(), w.end(), W2all.begin() + (size_t)q * nn);
+    }
+    S.W2.upload(W2all.data(), W2all.size(), s);
+    const bool uw1 = S.use_window1;
+    std::vector<SoftInst> inst;
+    std::vector<ComboOp> ops(2 * cn, ComboOp{CO_COPY, {0, 0, 0, 0}});
+    for (int q = 0; q < cn; q++) {
+        int base = (int)inst.size();
+        if (uw1 && uw2[q]) {
+            inst.push_back({q, RC_P1SQ_W1, RC_W2, 1});
+            inst.push_back({q, RC_W1, RC_P2SQ_W2, 1});
+            inst.push_back({q, RC_P1_W1, RC_P2_W2, 1});
+            inst.push_back({q, RC_W1, RC_W2, 1});
+            inst.push_back({q, RC_P1, RC_P2, 0});
+            ops[2 * q] = ComboOp{CO_BOTH, {base, base + 1, base + 2, base + 3}};
+            ops[2 * q + 1] = ComboOp{CO_COPY, {base + 4, 0, 0, 0}};
+        } else if (uw1) {
+            inst.push_back({q, RC_W1, RC_P2SQ, 1});
+            inst.push_back({q, RC_P1_W1, RC_P2, 1});
+            inst.push_back({q, RC_P1, RC_P2, 0});
+            ops[2 * q] = ComboOp{CO_DIV_SQRT, {base + 1, base, 0, 0}};
+            ops[2 * q + 1] = ComboOp{CO_COPY, {base + 2, 0, 0, 0}};
+        } else if (uw2[q]) {
+            inst.push_back({q, RC_P1SQ, RC_W2, 1});
+            inst.push_back({q, RC_P1, RC_P2_W2, 1});
+            inst.push_back({q, RC_P1, RC_P2, 0});
+            ops[2 * q] = ComboOp{CO_DIV_SQRT, {base + 1, base, 0, 0}};
+            ops[2 * q + 1] = ComboOp{CO_COPY, {base + 2, 0, 0, 0}};
+        } else {
+            inst.push_back({q, RC_P1, RC_P2, 0});
+            ops[2 * q] = ComboOp{CO_COPY, {base, 0, 0, 0}};
+            ops[2 * q + 1] = ComboOp{CO_COPY, {base, 0, 0, 0}};
+        }
+    }
+    const int nq = (int)inst.size();
+    S.C.ensure((size_t)nq * n3);
+    S.Cc.ensure((size_t)2 * cn * n3);
+    run_soft(P, inst, S.Pp2.p, S.W2.p, S.Mx.p, S.C.p);
+    S.ops.upload(ops.data(), ops.size(), s);
+    double f = std::floor(P->rot_restrict * B / 90.0 - 1.0);
+    int b0 = (int)std::max(1.0, std::min(f, (double)(n - 1)));
+    launch_combine(S.C.p, S.ops.p, 2 * cn, B, b0, S.Cc.p, s);
+    const int cap = 16384;
+    S.lm_count.ensure(2 * cn);
+    S.lm_idx.ensure((size_t)2 * cn * cap);
+    S.lm_val.ensure((size_t)2 * cn * cap);
+    ETTK_HIP(hipMemsetAsync(S.lm_count.p, 0, 2 * cn * sizeof(int), s));
+    launch_localmax(S.Cc.p, 2 * cn, B, cap, S.lm_count.p, S.lm_idx.p, S.lm_val.p, s);
+    std::vector<int> cnt(2 * cn);
+    ETTK_HIP(hipMemcpyAsync(cnt.data(), S.lm_count.p, 2 * cn * sizeof(int), hipMemcpyDeviceToHost, s));
+    ETTK_HIP(hipStreamSynchronize(s));
+    std::vector<int> idx;
+    std::vector<double> val;
+    for (int v = 0; v < 2 * cn; v++) {
+        int q = v / 2;
+        bool two = uw1 || uw2[q];
+        if (v % 2 == 1 && !two) continue;
+        ETTK_REQUIRE(cnt[v] <= cap, ETTK_ERR_UNSUPPORTED, "more SO(3) local maxima than the candidate list holds");
+        idx.resize(cnt[v]);
+        val.resize(cnt[v]);
+        if (cnt[v] > 0) {
+            ETTK_HIP(hipMemcpyAsync(idx.data(), S.lm_idx.p + (size_t)v * cap, cnt[v] * sizeof(int), hipMemcpyDeviceToHost, s));
+            ETTK_HIP(hipMemcpyAsync(val.data(), S.lm_val.p + (size_t)v * cap, cnt[v] * sizeof(double), hipMemcpyDeviceToHost, s));
+            ETTK_HIP(hipStreamSynchronize(s));
+        }
+        std::vector<std::pair<int, double>> lm(cnt[v]);
+        for (int k = 0; k < cnt[v]; k++) lm[k] = {idx[k], val[k]};
+        SearchOut& o = out[p0 + q];
+        o.two = two;
+        o.use_window = (uw1 ? 1 : 0) | (uw2[q] ? 2 : 0);
+        candidates_from_maxima(P, lm, v % 2 == 0 ? o.pq1 : o.pq2);
+    }
+}
+
+// ------------------------------------------------------------------ Powell (vnl_powell, maxfev = 1)
+// Restatement of vnl_powell::minimize (ITK 4.2.1 vnl/algo/vnl_powell.cxx:60-175) with
+// vnl_bracket_minimum and vnl_brent_minimizer::minimize_given_bounds, as nbfFourierImageMetric::refine
+// configures it (FIM.h:1742-1753): initial step 5, line-search x tolerance 1e-2, f tolerance 0.1.
+namespace {
+struct PowellState {
+    std::function<double(const double*)> f;
+    double end_error = 0;
+    long nevals = 0;
+    void report(double e) {
+        if (nevals == 0) end_error = e;
+        if (e < end_error) end_error = e;
+        nevals++;
+    }
+};
+const double GOLD = 1.618033988749894848, CGOLD = 0.381966011250105152;
+struct Line {
+    PowellState* S;
+    double x0[3], dx[3];
+    double operator()(double lam) {
+        double x[3] = {x0[0] + lam * dx[0], x0[1] + lam * dx[1], x0[2] + lam * dx[2]};
+        double e = S->f(x);
+        S->report(e);
+        return e;
+    }
+};
+void bracket(Line& f, double& a, double& b, double& c) {
+    const double EPS = 1e-7, EPSqr = 1e-14;
+    if (b == a) b = a + 1.0;
+    double fa = f(a), fb = f(b);
+    if (fb > fa) { std::swap(a, b); std::swap(fa, fb); }
+    c = b + GOLD * (b - a);
+    double fc = f(c);
+    while (fc < fb) {
+        double x1 = a - b, f1 = fa - fb, x2 = c - b, f2 = fc - fb;
+        double p = x2 * x2 * f1 - x1 * x1 * f2, q = 2 * (x2 * f1 - x1 * f2);
+        if (q >= 0 && q < EPSqr) q = EPSqr;
+        else if (q < 0 && q + EPSqr > 0) q = -1.0 * EPSqr;
+        double du = p / q, tol = EPS * (1.0 + std::max(std::fabs(b), std::fabs(c)));
+        if (du >= 0 && du < tol) du = tol;
+        else if (du < 0 && du + tol > 0) du = -1.0 * tol;
+        double u = b + du;
+        if ((u - c) < tol && (u - c) >= 0) u += tol;
+        else if ((c - u) < tol && (c - u) >= 0) u -= tol;
+        double ulim = b + 100 * (c - b), fu = 0.0;
+        if ((u - b) * (c - u) > 0.0) {
+            fu = f(u);
+            if (fu < fc) {
+                a = b; fa = fb; b = u; fb = fu;
+                if (a > c) { std::swap(a, c); std::swap(fa, fc); }
+                return;
+            } else if (fu > fb) {
+                c = u; fc = fu;
+                if (a > c) { std::swap(a, c); std::swap(fa, fc); }
+                return;
+            }
+            u = c + GOLD * (c - b);
+            fu = f(u);
+        } else if ((u - c) * (ulim - u) > 0.0) {
+            fu = f(u);
+            if (fu > fc) {
+                a = b; fa = fb; b = c; fb = fc; c = u; fc = fu;
+                if (a > c) { std::swap(a, c); std::swap(fa, fc); }
+                return;
+            }
+        } else if ((ulim - c) * (u - ulim) >= 0) {
+            u = ulim;
+            fu = f(u);
+        } else {
+            u = c + GOLD * (c - b);
+            fu = f(u);
+        }
+        a = b; b = c; c = u;
+        fa = fb; fb = fc; fc = fu;
+    }
+    if (a > c) { std::swap(a, c); std::swap(fa, fc); }
+}
+double brent(Line& f, double ax, double bx, double cx, double xtol, double& fmin) {
+    const double EPS = 1e-8;
+    double fb = f(bx);
+    double x = bx, w = x, v = w, u, fx = fb, fw = fx, fv = fw, fu;
+    double m = 0.5 * (ax + cx), tol = EPS * std::fabs(x) + xtol, tol2 = 2 * tol, d = 0.0, e = 0.0;
+    while (std::fabs(x - m) > (tol2 - 0.5 * (cx - ax))) {
+        double p = 0.0, q = 0.0, r = 0.0;
+        if (std::fabs(e) > tol) {
+            r = (x - w) * (fx - fv);
+            q = (x - v) * (fx - fw);
+            p = (x - v) * q - (x - w) * r;
+            q = 2 * (q - r);
+            if (q > 0) p *= -1.0; else q *= -1.0;
+            r = e; e = d;
+        }
+        if (std::fabs(p) < std::fabs(0.5 * q * r) && p > (q * (ax - x)) && p < (q * (cx - x))) {
+            d = p / q;
+            u = x + d;
+            if (u - ax < tol2 || cx - u < tol2) d = (x < m ? tol : -tol);
+        } else {
+            e = (x < m ? cx : ax) - x;
+            d = CGOLD * e;
+        }
+        if (std::fabs(d) >= tol) u = x + d;
+        else u = (d > 0) ? x + tol : x - tol;
+        fu = f(u);
+        if (fu <= fx) {
+            if (u < x) cx = x; else ax = x;
+            v = w; fv = fw; w = x; fw = fx; x = u; fx = fu;
+        } else {
+            if (u < x) ax = u; else cx = u;
+            if (fu <= fw || w == x) { v = w; fv = fw; w = u; fw = fu; }
+            else if (fu <= fv || v == x || v == w) { v = u; fv = fu; }
+        }
+        m = 0.5 * (ax + cx);
+        tol = EPS * std::fabs(x) + xtol;
+        tol2 = 2 * tol;
+    }
+    fmin = fx;
+    return x;
+}
+void powell(PowellState& S, double p[3], double step, double lxtol, double ftol) {
+    const int n = 3;
+    double xi[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, ptt[3], xit[3], pt[3];
+    double fret = S.f(p);
+    S.report(fret);
+    for (int j = 0; j < n; j++) pt[j] = p[j];
+    for (int iter = 0; iter < 1; iter++) {
+        double fp = fret, del = 0.0;
+        int ibig = 0;
+        for (int i = 0; i < n; i++) {
+            for (int j = 0; j < n; j++) xit[j] = xi[j][i];
+            double fptt = fret;
+            Line L{&S, {p[0], p[1], p[2]}, {xit[0], xit[1], xit[2]}};
+            double ax = 0.0, xx = step, bx, fm;
+            bracket(L, ax, xx, bx);
+            xx = brent(L, ax, xx, bx, lxtol, fm);
+            fret = fm;
+            for (int j = 0; j < n; j++) p[j] = L.x0[j] + xx * L.dx[j];
+            if (std::fabs(fptt - fret) > del) { del = std::fabs(fptt - fret); ibig = i; }
+        }
+        if (2.0 * std::fabs(fp - fret) <= ftol * (std::fabs(fp) + std::fabs(fret))) return;
+        for (int j = 0; j < n; j++) {
+            ptt[j] = 2.0 * p[j] - pt[j];
+            xit[j] = p[j] - pt[j];
+            pt[j] = p[j];
+        }
+        double fptt = S.f(ptt);
+        S.report(fret);
+        if (fptt < fp) {
+            double t = 2.0 * (fp - 2.0 * fret + fptt) * (fp - fret - del) * (fp - fret - del) - del * (fp - fptt) * (fp - fptt);
+            if (t < 0.0) {
+                Line L{&S, {p[0], p[1], p[2]}, {xit[0], xit[1], xit[2]}};
+                double ax = 0.0, xx = 1.0, bx, fm;
+                bracket(L, ax, xx, bx);
+                xx = brent(L, ax, xx, bx, lxtol, fm);
+                fret = fm;
+                for (int j = 0; j < n; j++) p[j] = L.x0[j] + xx * L.dx[j];
+                for (int j = 0; j < n; j++) {
+                    xi[j][ibig] = xi[j][n - 1];
+                    xi[j][n - 1] = xit[j];
+                }
+            }
+        }
+    }
+}
+}  // namespace
+
+// ------------------------------------------------------------------ per-particle control flow
+using EvalFn = std::function<float(int particle, Mat4& t)>;
+
+// nbfFourierImageMetric::refine (FIM.h:1702-1809) with f (:129-190)
+static float refine_powell(const ettk_plan* P, int pi, Mat4& t, bool run, const EvalFn& eval) {
+    Mat4 seed = strip_translation(t);
+    float last = 0.0f;
+    auto trial = [&](const double* x) {
+        return seed * mat_rot(x[0], 1, 0, 0) * mat_rot(x[1], 0, 1, 0) * mat_rot(x[2], 0, 0, 1);
+    };
+    PowellState S;
+    S.f = [&](const double* x) -> double {
+        Mat4 tt = trial(x);
+        if (!transform_ok(tt, P->trans_restrict, P->rot_restrict)) return (double)FLT_MAX;
+        float peak = eval(pi, tt);
+        last = peak;
+        if (!transform_ok(tt, P->trans_restrict, P->rot_restrict)) return (double)FLT_MAX;
+        return (double)peak;
+    };
+    double x[3] = {0, 0, 0};
+    if (run) powell(S, x, 5.0, 1e-2, 1e-1);
+    if (S.end_error != (double)FLT_MAX) {
+        Mat4 vt = trial(x);
+        last = eval(pi, vt);
+        t = vt;
+    } else {
+        last = eval(pi, t);
+    }
+    return last;
+}
+
+// candidate loop of nbfProjectionRotationMetric3D::execute (ProjRot.h:678-901)
+static void candidate_stage(const ettk_plan* P, int pi, SearchOut& so, const EvalFn& eval, ettk_result& out) {
+    std::vector<Cand> combined;
+    int nq = so.two ? 2 : 1;
+    int nccc = 0;
+    auto counted = [&](int p, Mat4& t) {
+        nccc++;
+        return eval(p, t);
+    };
+    for (int q = 0; q < nq; q++) {
+        CandQueue& pq = q == 0 ? so.pq1 : so.pq2;
+        int peaks = P->p.n_peaks;
+        if (q == 1) peaks = (int)std::max(1.0, std::floor(P->p.n_peaks / 3.0));
+        int index = 0;
+        while (index < peaks && !pq.empty()) {
+            Cand p = pq.top();
+            if (p.v == FLT_MAX) break;
+            Mat4 t1 = zyz(p.a, p.b, p.g), td = t1;
+            float D = counted(pi, td);
+            if (transform_ok(td, P->trans_restrict, P->rot_restrict)) {
+                Mat4 mir = mat_scale(-1, -1, -1) * t1;
+                float Dm = counted(pi, mir);
+                if (Dm >= D) index++;
+                combined.push_back(Cand{p.a, p.b, p.g, D});
+            }
+            pq.pop();
+        }
+    }
+    if (combined.empty()) combined.push_back(Cand{0, 0, 0, 1.0f});
+    CandQueue pqc;
+    eliminate_equivalent(combined, pqc, 2.5f);
+    float best = FLT_MAX;
+    Mat4 bestT = Mat4::ident();
+    int index = 0;
+    while (!pqc.empty() && index < P->p.n_candidates) {
+        Cand p = pqc.top();
+        Mat4 t = zyz(p.a, p.b, p.g);
+        float dist = refine_powell(P, pi, t, P->p.refine != 0, counted);
+        if (dist < best) {
+            best = dist;
+            bestT = t;
+        }
+        out.cand_distance[index] = dist;
+        std::memcpy(out.cand_matrix[index], t.a, sizeof(t.a));
+        out.so3_angles[index][0] = p.a;
+        out.so3_angles[index][1] = p.b;
+        out.so3_angles[index][2] = p.g;
+        pqc.pop();
+        index++;
+    }
+    out.n_cand = index;
+    out.distance = best;
+    std::memcpy(out.matrix, bestT.a, sizeof(bestT.a));
+    out.n_ccc = nccc;
+    out.use_window = so.use_window;
+}
+
+// ------------------------------------------------------------------ batched evaluation service
+// Every particle's control flow runs in its own host thread; eval() parks the request, and the last thread
+// to arrive runs one ccc_batch over all parked requests.  Each evaluation is independent of the others in
+// the batch, so every particle follows exactly the trajectory of the plain mode.
+class BatchEval {
+   public:
+    BatchEval(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n_threads)
+        : P_(P), d_parts_(d_parts), descs_(descs), active_(n_threads) {}
+    float eval(int pi, Mat4& t) {
+        std::unique_lock<std::mutex> lk(mu_);
+        if (failed_) throw err_;
+        float result = 0.f;
+        reqs_.push_back(Req{pi, &t, &result});
+        long my = gen_;
+        if ((int)reqs_.size() == active_) run_locked();
+        else cv_.wait(lk, [&] { return gen_ != my; });
+        if (failed_) throw err_;
+        return result;
+    }
+    void done() {
+        std::unique_lock<std::mutex> lk(mu_);
+        active_--;
+        if (active_ > 0 && (int)reqs_.size() == active_) run_locked();
+    }
+    void fail(const Error& e) {
+        std::unique_lock<std::mutex> lk(mu_);
+        if (!failed_) { failed_ = true; err_ = e; }
+        active_--;
+        gen_++;
+        cv_.notify_all();
+    }
+    long batches = 0;
+
+   private:
+    struct Req {
+        int pi;
+        Mat4* t;
+        float* out;
+    };
+    void run_locked() {
+        std::vector<int> pidx(reqs_.size());
+        std::vector<Mat4> t(reqs_.size());
+        std::vector<float> d(reqs_.size());
+        for (size_t i = 0; i < reqs_.size(); i++) {
+            pidx[i] = reqs_[i].pi;
+            t[i] = *reqs_[i].t;
+        }
+        try {
+            ccc_batch(P_, d_parts_, descs_, (int)reqs_.size(), pidx.data(), t.data(), d.data());
+            for (size_t i = 0; i < reqs_.size(); i++) {
+                *reqs_[i].t = t[i];
+                *reqs_[i].out = d[i];
+            }
+        } catch (const Error& e) {
+            failed_ = true;
+            err_ = e;
+        }
+        batches++;
+        reqs_.clear();
+        gen_++;
+        cv_.notify_all();
+    }
+    ettk_plan* P_;
+    const float* d_parts_;
+    const ettk_particle* descs_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    int active_;
+    long gen_ = 0;
+    std::vector<Req> reqs_;
+    bool failed_ = false;
+    Error err_{0, ""};
+};
+
+void refine_adjoint_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n,
+                          std::vector<SearchOut*>& so, ettk_result* out);
+
+void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, int mode, ettk_result* out) {
+    ETTK_REQUIRE(P->have_ref, ETTK_ERR_STATE, "reference not set");
+    ETTK_REQUIRE(mode >= 0 && mode <= 2, ETTK_ERR_ARG, "ettk_align_batch: mode must be 0, 1 or 2");
+    std::memset(out, 0, sizeof(ettk_result) * n);
+    P->timing = Timing{};
+    hipEvent_t a, b;
+    ETTK_HIP(hipEventCreate(&a));
+    ETTK_HIP(hipEventCreate(&b));
+    ETTK_HIP(hipEventRecord(a, P->ctx->stream));
+    std::vector<SearchOut> so(n);
+    if (mode == 0) {
+        EvalFn ev = [&](int pi, Mat4& t) {
+            float d;
+            ccc_batch(P, d_parts, descs, 1, &pi, &t, &d);
+            return d;
+        };
+        for (int i = 0; i < n; i++) {
+            search_chunk(P, d_parts, descs, i, 1, so);
+            candidate_stage(P, i, so[i], ev, out[i]);
+        }
+    } else {
+        const int CH = 32;
+        for (int p0 = 0; p0 < n; p0 += CH) search_chunk(P, d_parts, descs, p0, std::min(CH, n - p0), so);
+        if (mode == 1) {
+            int T = std::min(n, 1024);
+            BatchEval be(P, d_parts, descs, T);
+            EvalFn ev = [&](int pi, Mat4& t) { return be.eval(pi, t); };
+            std::vector<std::thread> th;
+            for (int w = 0; w < T; w++)
+                th.emplace_back([&, w] {
+                    try {
+                        for (int i = w; i < n; i += T) candidate_stage(P, i, so[i], ev, out[i]);
+                        be.done();
+                    } catch (const Error& e) {
+                        be.fail(e);
+                    }
+                });
+            for (auto& t : th) t.join();
+        } else {
+            std::vector<SearchOut*> sp(n);
+            for (int i = 0; i < n; i++) sp[i] = &so[i];
+            refine_adjoint_batch(P, d_parts, descs, n, sp, out);
+        }
+    }
+    ETTK_HIP(hipEventRecord(b, P->ctx->stream));
+    ETTK_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    ETTK_HIP(hipEventElapsedTime(&ms, a, b));
+    P->timing.total_ms = ms;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+}
+
+void refine_adjoint_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n,
+                          std::vector<SearchOut*>& so, ettk_result* out) {
+    (void)P; (void)d_parts; (void)descs; (void)n; (void)so; (void)out;
+    throw Error{ETTK_ERR_UNSUPPORTED, "mode 2 (adjoint refinement) is not available in this build"};
+}
+
+}  // namespace ettk
+
+// ============================================================== C ABI
+using namespace ettk;
+
+extern "C" {
+
+int ettk_project_batch(ettk_plan* P, const float* d_particles, int n, const ettk_particle* descs, float* out_pp) {
+    try {
+        ETTK_REQUIRE(P && d_particles && descs && out_pp && n > 0, ETTK_ERR_ARG, "ettk_project_batch: bad argument");
+        ETTK_HIP(hipSetDevice(P->ctx->device));
+        So3State& S = so3_state(P);
+        const int B = P->p.B;
+        S.Pp2.ensure((size_t)n * 4 * B * B);
+        project_images(P, d_particles, descs, nullptr, n, false, S.Pp2.p);
+        ETTK_HIP(hipMemcpyAsync(out_pp, S.Pp2.p, (size_t)n * 4 * B * B * sizeof(float), hipMemcpyDeviceToHost, P->ctx->stream));
+        ETTK_HIP(hipStreamSynchronize(P->ctx->stream));
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    }
+    set_error("");
+    return ETTK_OK;
+}
+
+int ettk_so3_correlate_batch(ettk_plan* P, int n, const double* sig, const double* pat, double* out_c) {
+    try {
+        ETTK_REQUIRE(P && sig && pat && out_c && n > 0, ETTK_ERR_ARG, "ettk_so3_correlate_batch: bad argument");
+        ETTK_HIP(hipSetDevice(P->ctx->device));
+        So3State& S = so3_state(P);
+        const int B = P->p.B;
+        const size_t n3 = (size_t)8 * B * B * B;
+        S.C.ensure(n3 * n);
+        soft_given(P, n, sig, pat, S.C.p);
+        ETTK_HIP(hipMemcpyAsync(out_c, S.C.p, n3 * n * sizeof(double), hipMemcpyDeviceToHost, P->ctx->stream));
+        ETTK_HIP(hipStreamSynchronize(P->ctx->stream));
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    }
+    set_error("");
+    return ETTK_OK;
+}
+
+int ettk_align_batch(ettk_plan* P, const float* d_particles, int n, const ettk_particle* descs, int mode,
+                     ettk_result* out) {
+    try {
+        ETTK_REQUIRE(P && d_particles && descs && out && n > 0, ETTK_ERR_ARG, "ettk_align_batch: bad argument");
+        ETTK_HIP(hipSetDevice(P->ctx->device));
+        align_run(P, d_particles, descs, n, mode, out);
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    }
+    set_error("");
+    return ETTK_OK;
+}
+
+}  // extern "C"

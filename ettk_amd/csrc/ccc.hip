@@ -313,6 +313,11 @@ void launch_combine_dftA(const float2* F1m, const float2* F2, const float2* Q, c
                          hipStream_t s) {
     dim3 grid(z.N, n_eval);
     size_t shm = (size_t)(z.N * z.H + z.E * z.N) * sizeof(float2);
+    static size_t set_a = 0;
+    if (shm > set_a) {
+        ETTK_HIP(hipFuncSetAttribute((const void*)k_combine_dftA, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        set_a = shm;
+    }
     k_combine_dftA<<<grid, kThreads, shm, s>>>(F1m, F2, Q, Mfft, scalars, tabA, z, G1);
     ETTK_HIP(hipGetLastError());
 }
@@ -382,6 +387,11 @@ void launch_dftBC_argmax(const float2* G1, const float2* tabA, const float2* tab
                          float* best_val, int* best_idx, hipStream_t s) {
     dim3 grid(z.E, n_eval);
     size_t shm = (size_t)(z.N * z.H + z.E * z.N + 2 * z.E * z.H) * sizeof(float2);
+    static size_t set_b = 0;
+    if (shm > set_b) {
+        ETTK_HIP(hipFuncSetAttribute((const void*)k_dftBC_argmax, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        set_b = shm;
+    }
     k_dftBC_argmax<<<grid, kThreads, shm, s>>>(G1, tabA, tabK, z, best_val, best_idx);
     ETTK_HIP(hipGetLastError());
 }

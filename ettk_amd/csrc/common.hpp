@@ -83,14 +83,16 @@ enum Prologue : int {
     PRO_MASK_SHIFT,      // x*aux0[i,j,k]*(-1)^(i+j+k)
     PRO_RESLICED,        // valid(x) ? (x - mean_b) * aux0 * (-1)^(i+j+k) : 0   (mean from stats[b])
     PRO_SHIFT,           // x*(-1)^(i+j+k)
+    PRO_PAD_MASKED,      // zero-padded ... constant-padded masked small volume (S^2 projection), see fft.hip
 };
 struct R2CArgs {
-    const float* in;      // [b][n][n][n]
+    const float* in;      // [b][n][n][n]  (PRO_PAD_MASKED: [b][ns][ns][ns])
     float2* out;          // [b][n][n][n/2+1]
     int n, batch;
     int pro;
-    const float* aux0;    // mask (n^3) for PRO_MASK_SHIFT / PRO_RESLICED
-    const float* stats;   // per batch (mean) for PRO_RESLICED
+    const float* aux0;    // mask (n^3) for PRO_MASK_SHIFT / PRO_RESLICED; (ns^3) for PRO_PAD_MASKED
+    const float* stats;   // per batch: mean (PRO_RESLICED); {valid mean, image mean, pad constant, mask on} (PAD)
+    int ns = 0;           // small box size for PRO_PAD_MASKED
 };
 void fft3_r2c(const FftTables& T, const R2CArgs& a, float2* scratch, hipStream_t s);
 // c2r: in is preserved when `in_scratch` != in (the i/j passes run out of place into in_scratch)

@@ -141,11 +141,27 @@ __device__ __forceinline__ void lds_fft(float2* pen, int lane, const float2* tw,
 
 // ------------------------------------------------------------------ r2c, pass along k (real rows)
 template <int L>
-__device__ __forceinline__ float r2c_load(const R2CArgs& a, long row, int k, float x) {
-    if (a.pro == PRO_NONE) return x;
+__device__ __forceinline__ float r2c_load(const R2CArgs& a, long row, int k) {
     long ij = row % ((long)L * L);
     int i = (int)(ij / L), j = (int)(ij % L);
     float sg = ((i + j + k) & 1) ? -1.0f : 1.0f;
+    if (a.pro == PRO_PAD_MASKED) {
+        // cartesianToSpherical (ProjRot.h:265-318): the masked image (image - mean) * mask + mean sits in the
+        // corner [0, ns)^3 of the (2 ns)^3 box, the rest holds the value of voxel (0,0,0); times (-1)^(i+j+k)
+        const long b = row / ((long)L * L);
+        const int ns = a.ns;
+        const float* st = a.stats + b * 4;
+        if (i < ns && j < ns && k < ns) {
+            long o = ((long)i * ns + j) * ns + k;
+            float x = a.in[b * (long)ns * ns * ns + o];
+            float img = (x == -3.402823466e+38f) ? 0.0f : x - st[0];
+            float v = st[3] != 0.f ? (img - st[1]) * a.aux0[o] + st[1] : img;
+            return v * sg;
+        }
+        return st[2] * sg;
+    }
+    float x = a.in[row * L + k];
+    if (a.pro == PRO_NONE) return x;
     switch (a.pro) {
         case PRO_SQUARE_SHIFT: return x * x * sg;
         case PRO_SHIFT: return x * sg;
@@ -175,7 +191,7 @@ __global__ void __launch_bounds__(Shape<L>::THREADS) k_r2c_rows(R2CArgs a, const
         float x = 0.0f;
         if (pair < npairs) {
             long row = pair * 2 + r / L;
-            x = r2c_load<L>(a, row, r % L, a.in[row * L + (r % L)]);
+            x = r2c_load<L>(a, row, r % L);
         }
         float* f = reinterpret_cast<float*>(&lds[q * S::LD + (r % L)]);
         f[r / L] = x;

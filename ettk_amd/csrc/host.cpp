@@ -95,7 +95,7 @@ Mat4 euler_of(const ettk_particle& d) {
 
 // ============================================================== constant arrays of a plan
 // separable vtkImageGaussianSmooth on an A(i,j,k) float array: axes 2, 1, 0, kernel clipped and renormalised
-static void smooth3(std::vector<float>& A, int n0, int n1, int n2, double sd, double rf) {
+void smooth3(std::vector<float>& A, int n0, int n1, int n2, double sd, double rf) {
     int dims[3] = {n0, n1, n2};
     long st[3] = {(long)n1 * n2, n2, 1};
     int rad = (int)(sd * rf);
@@ -256,7 +256,7 @@ void ensure_ccc_workspace(ettk_plan* P, int cap) {
 }
 
 // ============================================================== constrained CCC batch
-static EvalGeom make_geom(const ettk_plan* P, const ettk_particle& d, const Mat4& t, int particle) {
+EvalGeom make_geom_public(const ettk_plan* P, const ettk_particle& d, const Mat4& t, int particle) {
     EvalGeom g{};
     const int N = P->N;
     Mat4 fin = euler_of(d) * Mat4::from(d.matrix) * t;
@@ -293,7 +293,7 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
     for (int e0 = 0; e0 < n; e0 += P->cap) {
         int E = std::min(P->cap, n - e0);
         g.resize(E);
-        for (int e = 0; e < E; e++) g[e] = make_geom(P, descs[pidx[e0 + e]], t[e0 + e], pidx[e0 + e]);
+        for (int e = 0; e < E; e++) g[e] = make_geom_public(P, descs[pidx[e0 + e]], t[e0 + e], pidx[e0 + e]);
         ETTK_HIP(hipMemcpyAsync(P->geom.p, g.data(), E * sizeof(EvalGeom), hipMemcpyHostToDevice, s));
         ETTK_HIP(hipEventRecord(P->ev0, s));
         const float* W1 = P->p.missing_wedge ? P->W1.p : nullptr;

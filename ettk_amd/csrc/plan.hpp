@@ -34,8 +34,11 @@ Mat4 strip_translation(const Mat4& t);
 Mat4 zyz(double a, double b, double g);  // RotateZ(-g) RotateY(-b) RotateZ(-a), pre-multiplied
 bool transform_ok(const Mat4& t, double trans_restrict, double rot_restrict);  // nbfImageMetric::isTransformValid
 Mat4 euler_of(const ettk_particle& d);   // RotateZ(-n2) RotateX(-n0) RotateZ(-n1)
+// separable vtkImageGaussianSmooth on an A(i,j,k) float array (axes 2, 1, 0; clipped, renormalised kernel)
+void smooth3(std::vector<float>& A, int n0, int n1, int n2, double sd, double rf);
 
 // ------------------------------------------------------------------ state
+struct So3State;
 struct Timing {
     double ccc_ms = 0, total_ms = 0;
     long long n_ccc = 0;
@@ -76,7 +79,7 @@ struct ettk_plan {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ettk::Timing timing;
     // S^2 / SO(3) state (so3.hip)
-    struct So3State* so3 = nullptr;
+    ettk::So3State* so3 = nullptr;
     // averaging accumulators
     ettk::DevBuf<float> avg_re, avg_im, avg_acc;
 };

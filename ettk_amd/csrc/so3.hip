@@ -1,31 +1,769 @@
-// so3.hip -- S^2 projection and SO(3) rotational search (filled in below the CCC path).
-#include "plan.hpp"
+// so3.hip -- S^2 projection of |FFT| and the SO(3) rotational search (gfx950).
+//
+// Device side of nbfProjectionRotationMetric3D (inc/em/nbfProjectionRotationMetric3D.h):
+//   cartesianToSpherical (:265-413)  -> reslice, mask statistics, constant-padded (2N)^3 r2c, |F| (or
+//                                       |F|^(1/2) with MCF), tricubic wrap-around sampling on the
+//                                       Driscoll-Healy grid, radial sum (FP32, like the float Pp of the reference)
+//   soft / softScaled (:1137-1319)   -> S^2 forward transforms, coefficient combination and inverse SO(3)
+//                                       synthesis (FP64, as REAL in the reference), extrema, rotateFct of the
+//                                       pattern (Wigner-D rotation of its coefficients) and the affine rescale
+//   execute (:526-619)               -> branch combination, rotation-range restriction, 3x3x3 local maxima
+// The S^2/SO(3) mathematics is re-derived from the published method (Driscoll-Healy quadrature, Kostelec &
+// Rockmore 2008): direct quadrature with FP64 Legendre tables and FP64 Wigner-d three-term recurrences.
+#include <cfloat>
+#include <cmath>
+
+#include "so3.hpp"
 
 namespace ettk {
-struct So3State {};
-}  // namespace ettk
 
-namespace ettk {
+static constexpr int kT = 256;
+
+// ============================================================== tables (host, FP64)
+static double lfact(int n) { return std::lgamma((double)n + 1.0); }
+
+static void legendre_host(int B, double theta, std::vector<double>& P) {
+    P.assign((size_t)B * B, 0.0);
+    double x = std::cos(theta), s = std::sin(theta);
+    double pmm = std::sqrt(1.0 / (4.0 * M_PI));
+    for (int m = 0; m < B; m++) {
+        if (m > 0) pmm *= -std::sqrt((2.0 * m + 1) / (2.0 * m)) * s;
+        P[(size_t)m * B + m] = pmm;
+        if (m + 1 < B) P[(size_t)(m + 1) * B + m] = x * std::sqrt(2.0 * m + 3) * pmm;
+        for (int l = m + 2; l < B; l++) {
+            double a = std::sqrt((4.0 * l * l - 1) / ((double)l * l - (double)m * m));
+            double b = std::sqrt(((double)(l - 1) * (l - 1) - (double)m * m) / (4.0 * (l - 1) * (l - 1) - 1));
+            P[(size_t)l * B + m] = a * (x * P[(size_t)(l - 1) * B + m] - b * P[(size_t)(l - 2) * B + m]);
+        }
+    }
+}
+
+__host__ __device__ static inline double wig_seed(int m1, int m2, double beta) {
+    int j = abs(m1) > abs(m2) ? abs(m1) : abs(m2);
+    double c = cos(beta / 2), s = sin(beta / 2);
+    int m, sgn = 1;
+    bool top;  // d^j_{j,m} (top) or d^j_{-j,m}
+    if (m1 == j) { top = true; m = m2; }
+    else if (m1 == -j) { top = false; m = m2; }
+    else if (m2 == j) { top = true; m = m1; sgn = ((m1 - j) & 1) ? -1 : 1; }
+    else { top = false; m = m1; sgn = ((m1 + j) & 1) ? -1 : 1; }
+    double lg = 0.5 * (lgamma(2.0 * j + 1.0) - lgamma((double)(j + m) + 1.0) - lgamma((double)(j - m) + 1.0));
+    double v = top ? exp(lg) * pow(c, (double)(j + m)) * pow(-s, (double)(j - m))
+                   : exp(lg) * pow(c, (double)(j - m)) * pow(s, (double)(j + m));
+    return sgn * v;
+}
+// one step of the three-term recurrence in l: d^{l+1} from d^l, d^{l-1}
+__host__ __device__ static inline double wig_next(int l, int m1, int m2, double cb, double dcur, double dprev) {
+    if (l == 0) return cb;
+    double a = sqrt(((double)(l + 1) * (l + 1) - (double)m1 * m1) * ((double)(l + 1) * (l + 1) - (double)m2 * m2));
+    double bt = sqrt(((double)l * l - (double)m1 * m1) * ((double)l * l - (double)m2 * m2));
+    return ((2.0 * l + 1) * ((double)l * (l + 1) * cb - (double)m1 * m2) * dcur - (l + 1.0) * bt * dprev) / (l * a);
+}
+
+void so3_tables(ettk_plan* P) {
+    So3State& S = *P->so3;
+    const int B = S.B, n = 2 * B, W = 2 * B - 1;
+    hipStream_t s = P->ctx->stream;
+    // Driscoll-Healy weights
+    std::vector<double> w(n);
+    for (int j = 0; j < n; j++) {
+        double th = M_PI * (2 * j + 1) / (4.0 * B), acc = 0;
+        for (int k = 0; k < B; k++) acc += std::sin((2 * k + 1) * th) / (2 * k + 1);
+        w[j] = (2.0 / B) * std::sin(th) * acc;
+    }
+    // Legendre tables [m][l][j]
+    std::vector<double> LF((size_t)B * B * n), LI((size_t)B * B * n), Pl;
+    for (int j = 0; j < n; j++) {
+        legendre_host(B, M_PI * (2 * j + 1) / (4.0 * B), Pl);
+        for (int m = 0; m < B; m++)
+            for (int l = m; l < B; l++) {
+                LI[((size_t)m * B + l) * n + j] = Pl[(size_t)l * B + m];
+                LF[((size_t)m * B + l) * n + j] = Pl[(size_t)l * B + m] * w[j] * 2.0 * M_PI / n;
+            }
+    }
+    S.legF.upload(LF.data(), LF.size(), s);
+    S.legI.upload(LI.data(), LI.size(), s);
+    // Wigner-d table [beta][pair][l - l0] with sqrt((2l+1)/(8 pi^2))
+    std::vector<int> off((size_t)W * W);
+    int tot = 0;
+    for (int m1 = -(B - 1); m1 <= B - 1; m1++)
+        for (int m2 = -(B - 1); m2 <= B - 1; m2++) {
+            off[(size_t)(m1 + B - 1) * W + (m2 + B - 1)] = tot;
+            tot += B - std::max(std::abs(m1), std::abs(m2));
+        }
+    S.d_slice = tot;
+    std::vector<double> D((size_t)tot * n);
+    for (int kb = 0; kb < n; kb++) {
+        double beta = M_PI * (2 * kb + 1) / (4.0 * B), cb = std::cos(beta);
+        for (int m1 = -(B - 1); m1 <= B - 1; m1++)
+            for (int m2 = -(B - 1); m2 <= B - 1; m2++) {
+                int l0 = std::max(std::abs(m1), std::abs(m2));
+                double dprev = 0, dcur = wig_seed(m1, m2, beta);
+                size_t o = (size_t)kb * tot + off[(size_t)(m1 + B - 1) * W + (m2 + B - 1)];
+                for (int l = l0; l < B; l++) {
+                    D[o + (l - l0)] = std::sqrt((2.0 * l + 1) / (8.0 * M_PI * M_PI)) * dcur;
+                    double dn = wig_next(l, m1, m2, cb, dcur, dprev);
+                    dprev = dcur;
+                    dcur = dn;
+                }
+            }
+    }
+    S.dtab.upload(D.data(), D.size(), s);
+    S.doff.upload(off.data(), off.size(), s);
+    std::vector<double2> tw(n);
+    for (int t = 0; t < n; t++) tw[t] = make_double2(std::cos(2 * M_PI * t / n), -std::sin(2 * M_PI * t / n));
+    S.tw.upload(tw.data(), tw.size(), s);
+    ETTK_HIP(hipStreamSynchronize(s));
+}
+static void so3_kernel_attributes(int B);
+
+// ============================================================== projection kernels
+// statistics of the masked image: st[0] = valid mean (from reslice), st[1] = mean of the image, st[2] = pad
+// constant (value of voxel (0,0,0) after masking), st[3] = 1 when the image mask is on
+__global__ void k_proj_stats(const float* __restrict__ R, const float* __restrict__ res_partials, int nb_res,
+                             const float* __restrict__ mask, int N, int mask_on, int is_ref, float* __restrict__ st) {
+    __shared__ double sh[kT];
+    const int b = blockIdx.x;
+    const long n3 = (long)N * N * N;
+    const float* r = R + (size_t)b * n3;
+    float mv = 0.f;
+    if (!is_ref) {
+        double s = 0, c = 0;
+        for (int q = 0; q < nb_res; q++) {
+            s += res_partials[((size_t)b * nb_res + q) * 2];
+            c += res_partials[((size_t)b * nb_res + q) * 2 + 1];
+        }
+        mv = (float)(s / (double)(float)c);
+    }
+    double acc = 0;
+    for (long o = threadIdx.x; o < n3; o += blockDim.x) {
+        float x = r[o];
+        acc += (x == -FLT_MAX) ? 0.0 : (double)(x - mv);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st2 = blockDim.x / 2; st2 > 0; st2 >>= 1) {
+        if (threadIdx.x < st2) sh[threadIdx.x] += sh[threadIdx.x + st2];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double mean = sh[0] / (double)n3;
+        float x0 = r[0];
+        double img0 = (x0 == -FLT_MAX) ? 0.0 : (double)(x0 - mv);
+        double c = mask_on ? (img0 - mean) * mask[0] + mean : img0;
+        st[b * 4 + 0] = mv;
+        st[b * 4 + 1] = (float)mean;
+        st[b * 4 + 2] = (float)c;
+        st[b * 4 + 3] = mask_on ? 1.f : 0.f;
+    }
+}
+
+// |F| (|F|^(1/2) with MCF) of the (2N)^3 half spectrum
+__global__ void k_magnitude(const float2* __restrict__ F, float* __restrict__ mag, long n, int mcf) {
+    for (long h = blockIdx.x * (long)blockDim.x + threadIdx.x; h < n; h += (long)gridDim.x * blockDim.x) {
+        float2 z = F[h];
+        float m2 = z.x * z.x + z.y * z.y;
+        mag[h] = mcf ? sqrtf(sqrtf(m2)) : sqrtf(m2);
+    }
+}
+
+// full-size magnitude volume as ProjRot.h:327-338 leaves it: A(i,j,k) for k <= M/2 is |F| except the i = 0
+// plane (0); for k > M/2: i = 0 -> 0, j = 0 -> the pad constant (the loop never overwrites those voxels of
+// the padded image), otherwise the mirrored value |F|(M-i, M-j, M-k)
+__device__ __forceinline__ float mag_at(const float* __restrict__ mag, int M, int i, int j, int k, float padc) {
+    const int Hm = M / 2 + 1;
+    if (i == 0) return 0.f;
+    if (k <= M / 2) return mag[((size_t)i * M + j) * Hm + k];
+    if (j == 0) return padc;
+    return mag[((size_t)(M - i) * M + (M - j)) * Hm + (M - k)];
+}
+
+__global__ void __launch_bounds__(kT) k_sph_sample(const float* __restrict__ mag, const float* __restrict__ st, int M,
+                                                    int B, int bandL, int bandU, float* __restrict__ Pp) {
+    const int b = blockIdx.y;
+    const int n = 2 * B;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * n) return;
+    const int jp = t / n, kt = t % n;
+    const float* mg = mag + (size_t)b * M * M * (M / 2 + 1);
+    const float padc = st[b * 4 + 2];
+    const float Bf = (float)B;
+    const double phi = M_PI / 4.0 / Bf + jp * (2.0 * M_PI / 4.0 / Bf);
+    const double th = kt * (M_PI / Bf);
+    const double sp = sin(phi), cp = cos(phi), ct = cos(th), stt = sin(th);
+    const double origin = -(M - 1) / 2.0 - 0.5;
+    double sum = 0;
+    for (int r = bandL; r <= bandU; r++) {
+        double p[3] = {r * sp * ct - origin, r * sp * stt - origin, r * cp - origin};
+        int idx[3];
+        double w[3][4];
+        for (int a = 0; a < 3; a++) {
+            double xx = p[a] + 7.62939453125e-06;
+            idx[a] = (int)floor(xx);
+            double f = xx - idx[a];
+            double fm1 = f - 1, fd2 = f * 0.5, ft3 = f * 3;
+            w[a][0] = -fd2 * fm1 * fm1;
+            w[a][1] = ((ft3 - 2) * fd2 - 1) * fm1;
+            w[a][2] = -((ft3 - 4) * f - 1) * fd2;
+            w[a][3] = f * fd2 * fm1;
+        }
+        double v = 0;
+        for (int cz = 0; cz < 4; cz++) {
+            int kz = ((idx[2] - 1 + cz) % M + M) % M;
+            double vy = 0;
+            for (int cy = 0; cy < 4; cy++) {
+                int jy = ((idx[1] - 1 + cy) % M + M) % M;
+                double vx = 0;
+                for (int cx = 0; cx < 4; cx++) {
+                    int ix = ((idx[0] - 1 + cx) % M + M) % M;
+                    vx += w[0][cx] * (double)mag_at(mg, M, ix, jy, kz, padc);
+                }
+                vy += w[1][cy] * vx;
+            }
+            v += w[2][cz] * vy;
+        }
+        sum += v;
+    }
+    Pp[(size_t)b * n * n + t] = (float)sum;
+}
+
+// ============================================================== soft instances: assemble S^2 samples
+// recipe codes (the F1/F2 arrays of ProjRot.h:526-584, evaluated in float like the Pixel arrays there)
+__device__ __forceinline__ float recipe_val(int code, float p1, float p2, float w1, float w2) {
+    switch (code) {
+        case RC_P1SQ_W1: return p1 * p1 * w1;
+        case RC_W1: return w1;
+        case RC_P1_W1: return p1 * w1;
+        case RC_P2SQ_W2: return p2 * p2 * w2;
+        case RC_W2: return w2;
+        case RC_P2_W2: return w2 * p2;
+        case RC_P1: return p1;
+        case RC_P2: return p2;
+        case RC_P1SQ: return p1 * p1;
+        case RC_P2SQ: return p2 * p2;
+    }
+    return 0.f;
+}
+
+__global__ void k_assemble(const SoftInst* __restrict__ inst, const float* __restrict__ Pp1, const float* __restrict__ Pp2,
+                           const float* __restrict__ W1, const float* __restrict__ W2, const float* __restrict__ Mx,
+                           int B, double* __restrict__ sig, double* __restrict__ pat) {
+    const int q = blockIdx.y;
+    const int n2 = 4 * B * B;
+    const SoftInst I = inst[q];
+    const float M = Mx[I.particle];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += gridDim.x * blockDim.x) {
+        float p1 = Pp1[t] / M, p2 = Pp2[(size_t)I.particle * n2 + t] / M;
+        float w1 = W1[t], w2 = W2[(size_t)I.particle * n2 + t];
+        sig[(size_t)q * n2 + t] = (double)recipe_val(I.sig, p1, p2, w1, w2);
+        pat[(size_t)q * n2 + t] = (double)recipe_val(I.pat, p1, p2, w1, w2);
+    }
+}
+
+// M = max(max Pp1, max Pp2) per particle (ProjRot.h:467)
+__global__ void k_ppmax(const float* __restrict__ Pp1, const float* __restrict__ Pp2, int n2, float* __restrict__ Mx) {
+    __shared__ float sh[kT];
+    const int b = blockIdx.x;
+    float m = -FLT_MAX;
+    for (int t = threadIdx.x; t < n2; t += blockDim.x) m = fmaxf(m, fmaxf(Pp1[t], Pp2[(size_t)b * n2 + t]));
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) Mx[b] = sh[0];
+}
+
+// ============================================================== S^2 transforms (FP64)
+// ring DFT: fm[q][j][m] = sum_k f[q][j][k] e^{-i m phi_k}, m = 0..2B-1 (negative orders at 2B+m)
+__global__ void k_ring_dft(const double* __restrict__ f, const double2* __restrict__ tw, int B, double2* __restrict__ fm) {
+    const int n = 2 * B, q = blockIdx.y;
+    __shared__ double2 stw[256];
+    for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = tw[t];
+    __syncthreads();
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n * n; t += gridDim.x * blockDim.x) {
+        int j = t / n, m = t % n;
+        const double* row = f + ((size_t)q * n + j) * n;
+        double re = 0, im = 0;
+        for (int k = 0; k < n; k++) {
+            double2 w = stw[(m * k) % n];
+            re += row[k] * w.x;
+            im += row[k] * w.y;
+        }
+        fm[((size_t)q * n + j) * n + m] = make_double2(re, im);
+    }
+}
+// Legendre quadrature: c[q][l^2+l+m] = sum_j LF[|m|][l][j] fm[q][j][m] ((-1)^m for m < 0)
+__global__ void k_legendre_fwd(const double2* __restrict__ fm, const double* __restrict__ LF, int B, double2* __restrict__ c) {
+    const int n = 2 * B, q = blockIdx.y;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < B * B; t += gridDim.x * blockDim.x) {
+        int l = (int)sqrt((double)t);
+        while (l * l > t) l--;
+        while ((l + 1) * (l + 1) <= t) l++;
+        int m = t - l * l - l;
+        int am = m < 0 ? -m : m;
+        const double* tab = LF + ((size_t)am * B + l) * n;
+        const double2* src = fm + (size_t)q * n * n;
+        int col = m < 0 ? n + m : m;
+        double re = 0, im = 0;
+        for (int j = 0; j < n; j++) {
+            double2 z = src[(size_t)j * n + col];
+            re += tab[j] * z.x;
+            im += tab[j] * z.y;
+        }
+        double sg = (m < 0 && (am & 1)) ? -1.0 : 1.0;
+        c[(size_t)q * B * B + t] = make_double2(sg * re, sg * im);
+    }
+}
+// inverse: g[q][j][m] = sum_l c(l,m) Pbar_l^|m|(theta_j) ((-1)^m for m < 0)
+__global__ void k_legendre_inv(const double2* __restrict__ c, const double* __restrict__ LI, int B, double2* __restrict__ g) {
+    const int n = 2 * B, q = blockIdx.y;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n * n; t += gridDim.x * blockDim.x) {
+        int j = t / n, col = t % n;
+        int m = col < B ? col : col - n;  // col == B holds order -B: zero
+        double re = 0, im = 0;
+        if (col != B) {
+            int am = m < 0 ? -m : m;
+            double sg = (m < 0 && (am & 1)) ? -1.0 : 1.0;
+            for (int l = am; l < B; l++) {
+                double pv = LI[((size_t)am * B + l) * n + j] * sg;
+                double2 z = c[(size_t)q * B * B + l * l + l + m];
+                re += pv * z.x;
+                im += pv * z.y;
+            }
+        }
+        g[((size_t)q * n + j) * n + col] = make_double2(re, im);
+    }
+}
+// f[q][j][k] = Re sum_m g[q][j][m] e^{+i m phi_k}
+__global__ void k_ring_idft(const double2* __restrict__ g, const double2* __restrict__ tw, int B, double* __restrict__ f) {
+    const int n = 2 * B, q = blockIdx.y;
+    __shared__ double2 stw[256];
+    for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = tw[t];
+    __syncthreads();
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n * n; t += gridDim.x * blockDim.x) {
+        int j = t / n, k = t % n;
+        const double2* row = g + ((size_t)q * n + j) * n;
+        double re = 0;
+        for (int m = 0; m < n; m++) {
+            double2 w = stw[(m * k) % n];  // e^{-i}, conjugated below
+            re += row[m].x * w.x + row[m].y * w.y;
+        }
+        f[((size_t)q * n + j) * n + k] = re;
+    }
+}
+
+// ============================================================== SO(3) synthesis
+// T1[q][beta][m1][m2] = sum_l sig(l,-m1) (-1)^(m1+m2) conj(pat(l,-m2)) sqrt((2l+1)/8pi^2) d^l_{m1m2}(beta)
+__global__ void __launch_bounds__(kT) k_so3_synth(const double2* __restrict__ cs, const double2* __restrict__ cp,
+                                                   const double* __restrict__ dtab, const int* __restrict__ doff,
+                                                   int d_slice, int B, double2* __restrict__ T1) {
+    const int kb = blockIdx.x, q = blockIdx.y;
+    const int n = 2 * B, W = 2 * B - 1;
+    const double2* S = cs + (size_t)q * B * B;
+    const double2* Pt = cp + (size_t)q * B * B;
+    const double* D = dtab + (size_t)kb * d_slice;
+    double2* out = T1 + ((size_t)q * n + kb) * n * n;
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+        int c1 = t / n, c2 = t % n;
+        int m1 = c1 < B ? c1 : c1 - n, m2 = c2 < B ? c2 : c2 - n;
+        double re = 0, im = 0;
+        if (c1 != B && c2 != B) {
+            int am1 = m1 < 0 ? -m1 : m1, am2 = m2 < 0 ? -m2 : m2;
+            int l0 = am1 > am2 ? am1 : am2;
+            const double* d = D + doff[(m1 + B - 1) * W + (m2 + B - 1)];
+            for (int l = l0; l < B; l++) {
+                double2 a = S[l * l + l - m1], b = Pt[l * l + l - m2];
+                // a * conj(b)
+                double pr = a.x * b.x + a.y * b.y, pi = a.y * b.x - a.x * b.y;
+                double dv = d[l - l0];
+                re += pr * dv;
+                im += pi * dv;
+            }
+            if ((m1 + m2) & 1) { re = -re; im = -im; }
+        }
+        out[t] = make_double2(re, im);
+    }
+}
+// T2[q][beta][alpha][m2] = sum_m1 T1[q][beta][m1][m2] e^{-i m1 alpha}
+__global__ void k_dft_m1(const double2* __restrict__ T1, const double2* __restrict__ tw, int B, double2* __restrict__ T2) {
+    const int n = 2 * B, kb = blockIdx.x, q = blockIdx.y;
+    __shared__ double2 stw[256];
+    for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = tw[t];
+    __syncthreads();
+    const double2* in = T1 + ((size_t)q * n + kb) * n * n;
+    double2* out = T2 + ((size_t)q * n + kb) * n * n;
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+        int a = t / n, m2 = t % n;
+        double re = 0, im = 0;
+        for (int m1 = 0; m1 < n; m1++) {
+            double2 z = in[(size_t)m1 * n + m2], w = stw[(m1 * a) % n];
+            re += z.x * w.x - z.y * w.y;
+            im += z.x * w.y + z.y * w.x;
+        }
+        out[t] = make_double2(re, im);
+    }
+}
+// C[q][alpha][beta][gamma] = Re sum_m2 T2[q][beta][alpha][m2] e^{-i m2 gamma}
+__global__ void k_dft_m2(const double2* __restrict__ T2, const double2* __restrict__ tw, int B, double* __restrict__ C) {
+    const int n = 2 * B, kb = blockIdx.x, q = blockIdx.y;
+    __shared__ double2 stw[256];
+    for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = tw[t];
+    __syncthreads();
+    const double2* in = T2 + ((size_t)q * n + kb) * n * n;
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+        int a = t / n, g = t % n;
+        double re = 0;
+        for (int m2 = 0; m2 < n; m2++) {
+            double2 z = in[(size_t)a * n + m2], w = stw[(m2 * g) % n];
+            re += z.x * w.x - z.y * w.y;
+        }
+        C[(((size_t)q * n + a) * n + kb) * n + g] = re;
+    }
+}
+
+// ============================================================== softScaled pieces
+// first maximum and first minimum in row-major order (ProjRot.h:1562-1576)
+__global__ void k_extrema(const double* __restrict__ C, long n3, double* __restrict__ vals, int* __restrict__ pos) {
+    __shared__ double smax[kT], smin[kT];
+    __shared__ int imax[kT], imin[kT];
+    const int q = blockIdx.x;
+    const double* c = C + (size_t)q * n3;
+    double bx = -DBL_MAX, bn = DBL_MAX;
+    int ix = 0x7fffffff, in = 0x7fffffff;
+    for (long o = threadIdx.x; o < n3; o += blockDim.x) {
+        double v = c[o];
+        if (v > bx) { bx = v; ix = (int)o; }
+        if (v < bn) { bn = v; in = (int)o; }
+    }
+    smax[threadIdx.x] = bx; imax[threadIdx.x] = ix;
+    smin[threadIdx.x] = bn; imin[threadIdx.x] = in;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            int o = threadIdx.x + s;
+            if (smax[o] > smax[threadIdx.x] || (smax[o] == smax[threadIdx.x] && imax[o] < imax[threadIdx.x])) {
+                smax[threadIdx.x] = smax[o]; imax[threadIdx.x] = imax[o];
+            }
+            if (smin[o] < smin[threadIdx.x] || (smin[o] == smin[threadIdx.x] && imin[o] < imin[threadIdx.x])) {
+                smin[threadIdx.x] = smin[o]; imin[threadIdx.x] = imin[o];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        vals[q * 2 + 0] = smin[0];
+        vals[q * 2 + 1] = smax[0];
+        pos[q * 2 + 0] = imin[0];
+        pos[q * 2 + 1] = imax[0];
+    }
+}
+
+// rotation of the pattern's coefficients by the Euler angles of the extrema (rotateFct, rotate_so3.c:695-751):
+// r(l,m) = sum_m' e^{-i m alpha} d^l_{m m'}(beta) e^{-i m' gamma} c(l,m').  Block (q, which): which = 0 min, 1 max.
+__global__ void k_rotate_coef(const double2* __restrict__ c, const int* __restrict__ pos, int B, double2* __restrict__ r) {
+    extern __shared__ double2 acc[];  // [l][m + B - 1]
+    const int q = blockIdx.x, which = blockIdx.y;
+    const int n = 2 * B, W = 2 * B - 1;
+    const int p = pos[q * 2 + which];
+    const int a = p / (n * n), b = (p / n) % n, g = p % n;
+    // angles are Pixel (float) in the reference
+    const double alpha = (double)(float)(a / (double)B * M_PI);
+    const double beta = (double)(float)((2 * b + 1.0) / (4. * B) * M_PI);
+    const double gamma = (double)(float)(g / (double)B * M_PI);
+    const double2* cc = c + (size_t)q * B * B;
+    for (int t = threadIdx.x; t < B * W; t += blockDim.x) acc[t] = make_double2(0, 0);
+    __syncthreads();
+    const double cb = cos(beta);
+    for (int m = -(B - 1) + (int)threadIdx.x; m <= B - 1; m += blockDim.x) {
+        for (int mp = -(B - 1); mp <= B - 1; mp++) {
+            int l0 = max(abs(m), abs(mp));
+            double dprev = 0, dcur = wig_seed(m, mp, beta);
+            double ph = -(m * alpha + mp * gamma);
+            double cr = cos(ph), ci = sin(ph);
+            for (int l = l0; l < B; l++) {
+                double2 z = cc[l * l + l + mp];
+                double Dr = cr * dcur, Di = ci * dcur;
+                double2 o = acc[l * W + (m + B - 1)];
+                o.x += Dr * z.x - Di * z.y;
+                o.y += Dr * z.y + Di * z.x;
+                acc[l * W + (m + B - 1)] = o;
+                double dn = wig_next(l, m, mp, cb, dcur, dprev);
+                dprev = dcur;
+                dcur = dn;
+            }
+        }
+    }
+    __syncthreads();
+    double2* out = r + ((size_t)q * 2 + which) * B * B;
+    for (int t = threadIdx.x; t < B * B; t += blockDim.x) {
+        int l = (int)sqrt((double)t);
+        while (l * l > t) l--;
+        while ((l + 1) * (l + 1) <= t) l++;
+        int m = t - l * l - l;
+        out[t] = acc[l * W + (m + B - 1)];
+    }
+}
+
+// wMin / wMax = sum F1 * R * sin(theta_i) and the affine rescale of C (ProjRot.h:1303-1318)
+__global__ void k_wsum(const double* __restrict__ F1, const double* __restrict__ Rr, int B, double* __restrict__ wv) {
+    __shared__ double sh[kT];
+    const int q = blockIdx.x, which = blockIdx.y;
+    const int n = 2 * B;
+    const double* f = F1 + (size_t)q * n * n;
+    const double* rr = Rr + ((size_t)q * 2 + which) * n * n;
+    double acc = 0;
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+        int i = t / n;
+        acc += f[t] * rr[t] * sin(M_PI * (2 * i + 1) / 4 / (double)(float)B);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) wv[q * 2 + which] = sh[0];
+}
+__global__ void k_rescale(double* __restrict__ C, long n3, const double* __restrict__ vals, const double* __restrict__ wv,
+                          const SoftInst* __restrict__ inst) {
+    const int q = blockIdx.y;
+    if (!inst[q].scaled) return;
+    const double mn = vals[q * 2], mx = vals[q * 2 + 1], wmin = wv[q * 2], wmax = wv[q * 2 + 1];
+    double* c = C + (size_t)q * n3;
+    for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < n3; o += (long)gridDim.x * blockDim.x)
+        c[o] = (c[o] - mn) / (mx - mn) * (wmax - wmin) + wmin;
+}
+
+// ============================================================== branch combination, restriction, local maxima
+// out[p] = op(C[i0], C[i1], ...) per branch (ProjRot.h:526-584); restriction of beta rows (:602)
+__global__ void k_combine(const double* __restrict__ C, const ComboOp* __restrict__ ops, long n3, int n, int b0,
+                          double* __restrict__ out) {
+    const int p = blockIdx.y;
+    const ComboOp op = ops[p];
+    double* o = out + (size_t)p * n3;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n3; t += (long)gridDim.x * blockDim.x) {
+        int b = (int)((t / n) % n);
+        double v;
+        const double* c0 = C + (size_t)op.i[0] * n3;
+        switch (op.kind) {
+            case CO_COPY: v = c0[t]; break;
+            case CO_DIV_SQRT: v = c0[t] / sqrt(C[(size_t)op.i[1] * n3 + t]); break;
+            case CO_BOTH: {
+                double a = sqrt(c0[t]) * sqrt(C[(size_t)op.i[1] * n3 + t]);
+                a = 2.0 * (1.0 - C[(size_t)op.i[2] * n3 + t] / a);
+                v = -a / C[(size_t)op.i[3] * n3 + t];
+                break;
+            }
+            default: v = 0;
+        }
+        if (b >= b0) v = -(double)FLT_MAX;
+        o[t] = v;
+    }
+}
+
+// 26-neighbour >= test with a -FLT_MAX border (localMaxima3D3x3 + BordStrategyConst, ProjRot.h:67-79, 605-610);
+// nonzero results are appended to a per-volume list (order restored on the host by sorting the index)
+__global__ void k_localmax(const double* __restrict__ C, int n, int cap, int* __restrict__ count,
+                           int* __restrict__ idx, double* __restrict__ val) {
+    const int p = blockIdx.y;
+    const long n3 = (long)n * n * n;
+    const double* c = C + (size_t)p * n3;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n3; t += (long)gridDim.x * blockDim.x) {
+        int a = (int)(t / ((long)n * n)), b = (int)((t / n) % n), g = (int)(t % n);
+        double v = c[t];
+        bool ok = true;
+        for (int da = -1; da <= 1 && ok; da++)
+            for (int db = -1; db <= 1 && ok; db++)
+                for (int dg = -1; dg <= 1 && ok; dg++) {
+                    if (!da && !db && !dg) continue;
+                    int aa = a + da, bb = b + db, gg = g + dg;
+                    double nv = (aa < 0 || bb < 0 || gg < 0 || aa >= n || bb >= n || gg >= n)
+                                    ? -(double)FLT_MAX
+                                    : c[((size_t)aa * n + bb) * n + gg];
+                    if (!(v >= nv)) ok = false;
+                }
+        if (ok && v != 0.0) {
+            int slot = atomicAdd(&count[p], 1);
+            if (slot < cap) {
+                idx[(size_t)p * cap + slot] = (int)t;
+                val[(size_t)p * cap + slot] = v;
+            }
+        }
+    }
+}
+
+// ============================================================== host drivers
 void so3_state_destroy(ettk_plan* P) {
     delete P->so3;
     P->so3 = nullptr;
 }
-void so3_reference_set(ettk_plan* P) { (void)P; }
+
+So3State& so3_state(ettk_plan* P);
+static So3State& so3(ettk_plan* P) { return so3_state(P); }
+So3State& so3_state(ettk_plan* P) {
+    if (!P->so3) {
+        P->so3 = new So3State();
+        P->so3->B = P->p.B;
+        so3_tables(P);
+        so3_kernel_attributes(P->p.B);
+    }
+    return *P->so3;
+}
+
+static int band_lo(const ettk_plan* P) {
+    const ettk_params& p = P->p;
+    bool bp = (p.bp_lo > 0.0f) || (p.bp_hi < 1.0f);
+    float lo = bp ? p.bp_lo : 0.0f;
+    return std::max(4, (int)std::ceil(lo * (2 * P->N - 1)));
+}
+static int band_hi(const ettk_plan* P) {
+    const ettk_params& p = P->p;
+    bool bp = (p.bp_lo > 0.0f) || (p.bp_hi < 1.0f);
+    float hi = bp ? p.bp_hi : 1.0f;
+    return (int)std::ceil(std::min(0.5f, hi) * (2 * P->N - 1));
+}
+
+// projections of `n` images: is_ref -> R already holds the internal image (no reslice)
+void project_images(ettk_plan* P, const float* d_parts, const ettk_particle* descs, const int* pidx, int n, bool is_ref,
+                    float* d_Pp) {
+    So3State& S = so3(P);
+    const int N = P->N, M = 2 * N, B = S.B, Hm = N + 1;
+    const size_t n3 = (size_t)N * N * N, nbig = (size_t)M * M * Hm;
+    hipStream_t s = P->ctx->stream;
+    const int CH = 32;
+    S.projR.ensure(n3 * CH);
+    S.projF.ensure(nbig * CH);
+    S.projMag.ensure(nbig * CH);
+    S.projStats.ensure(4 * CH);
+    int nb_res = (int)((n3 + 1023) / 1024);
+    S.projPartials.ensure((size_t)2 * nb_res * CH);
+    S.projGeom.ensure(CH);
+    std::vector<EvalGeom> g(CH);
+    for (int c0 = 0; c0 < n; c0 += CH) {
+        int cn = std::min(CH, n - c0);
+        if (is_ref) {
+            ETTK_HIP(hipMemcpyAsync(S.projR.p, P->ref.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+        } else {
+            for (int q = 0; q < cn; q++) {
+                int pi = pidx ? pidx[c0 + q] : c0 + q;
+                g[q] = make_geom_public(P, descs[pi], Mat4::ident(), pi);
+            }
+            ETTK_HIP(hipMemcpyAsync(S.projGeom.p, g.data(), cn * sizeof(EvalGeom), hipMemcpyHostToDevice, s));
+            launch_reslice(d_parts, S.projGeom.p, cn, N, S.projR.p, S.projPartials.p, nb_res, s);
+        }
+        k_proj_stats<<<cn, kT, 0, s>>>(S.projR.p, S.projPartials.p, nb_res, P->m1.p, N, P->p.mask_radius != 0.0f ? 1 : 0,
+                                       is_ref ? 1 : 0, S.projStats.p);
+        ETTK_HIP(hipGetLastError());
+        R2CArgs a{S.projR.p, S.projF.p, M, cn, PRO_PAD_MASKED, P->m1.p, S.projStats.p, N};
+        fft3_r2c(P->ctx->fft, a, nullptr, s);
+        k_magnitude<<<2048, kT, 0, s>>>(S.projF.p, S.projMag.p, (long)nbig * cn, P->p.mcf);
+        ETTK_HIP(hipGetLastError());
+        dim3 grid((4 * B * B + kT - 1) / kT, cn);
+        k_sph_sample<<<grid, kT, 0, s>>>(S.projMag.p, S.projStats.p, M, B, band_lo(P), band_hi(P),
+                                         d_Pp + (size_t)c0 * 4 * B * B);
+        ETTK_HIP(hipGetLastError());
+    }
+}
+
+void so3_reference_set(ettk_plan* P) {
+    So3State& S = so3(P);
+    const int B = S.B;
+    S.Pp1.ensure((size_t)4 * B * B);
+    project_images(P, nullptr, nullptr, nullptr, 1, true, S.Pp1.p);
+    // spherical wedge of the reference (host, FP64), smoothed
+    std::vector<float> W1 = spherical_wedge_host(B, P->refd, P->p.missing_wedge != 0);
+    S.W1.upload(W1.data(), W1.size(), P->ctx->stream);
+    double sw = 0;
+    for (float v : W1) sw += v;
+    S.use_window1 = P->p.missing_wedge && (sw <= 0.95f * (float)W1.size());
+    ETTK_HIP(hipStreamSynchronize(P->ctx->stream));
+}
+
+// soft / softScaled for a list of instances whose samples are already in S.sig / S.pat; C is [q][a][b][g]
+void soft_core(ettk_plan* P, const std::vector<SoftInst>& inst, double* d_C) {
+    So3State& S = so3(P);
+    const int B = S.B, n = 2 * B, nq = (int)inst.size();
+    const long n3 = (long)n * n * n;
+    hipStream_t s = P->ctx->stream;
+    S.fm.ensure((size_t)nq * n * n);
+    S.cs.ensure((size_t)nq * B * B);
+    S.cp.ensure((size_t)nq * B * B);
+    S.T1.ensure((size_t)nq * n3);
+    S.T2.ensure((size_t)nq * n3);
+    S.vals.ensure((size_t)2 * nq);
+    S.pos.ensure((size_t)2 * nq);
+    S.rot.ensure((size_t)2 * nq * B * B);
+    S.g.ensure((size_t)2 * nq * n * n);
+    S.Rr.ensure((size_t)2 * nq * n * n);
+    S.wv.ensure((size_t)2 * nq);
+    k_ring_dft<<<dim3(16, nq), kT, 0, s>>>(S.sig.p, S.tw.p, B, S.fm.p);
+    k_legendre_fwd<<<dim3(8, nq), kT, 0, s>>>(S.fm.p, S.legF.p, B, S.cs.p);
+    k_ring_dft<<<dim3(16, nq), kT, 0, s>>>(S.pat.p, S.tw.p, B, S.fm.p);
+    k_legendre_fwd<<<dim3(8, nq), kT, 0, s>>>(S.fm.p, S.legF.p, B, S.cp.p);
+    k_so3_synth<<<dim3(n, nq), kT, 0, s>>>(S.cs.p, S.cp.p, S.dtab.p, S.doff.p, S.d_slice, B, S.T1.p);
+    k_dft_m1<<<dim3(n, nq), kT, 0, s>>>(S.T1.p, S.tw.p, B, S.T2.p);
+    k_dft_m2<<<dim3(n, nq), kT, 0, s>>>(S.T2.p, S.tw.p, B, d_C);
+    ETTK_HIP(hipGetLastError());
+    bool any_scaled = false;
+    for (auto& q : inst) any_scaled |= q.scaled != 0;
+    if (any_scaled) {
+        k_extrema<<<nq, kT, 0, s>>>(d_C, n3, S.vals.p, S.pos.p);
+        size_t shm = (size_t)B * (2 * B - 1) * sizeof(double2);
+        k_rotate_coef<<<dim3(nq, 2), kT, shm, s>>>(S.cp.p, S.pos.p, B, S.rot.p);
+        k_legendre_inv<<<dim3(16, 2 * nq), kT, 0, s>>>(S.rot.p, S.legI.p, B, S.g.p);
+        k_ring_idft<<<dim3(16, 2 * nq), kT, 0, s>>>(S.g.p, S.tw.p, B, S.Rr.p);
+        k_wsum<<<dim3(nq, 2), kT, 0, s>>>(S.sig.p, S.Rr.p, B, S.wv.p);
+        k_rescale<<<dim3(256, nq), kT, 0, s>>>(d_C, n3, S.vals.p, S.wv.p, S.inst.p);
+        ETTK_HIP(hipGetLastError());
+    }
+}
+
+void run_soft(ettk_plan* P, const std::vector<SoftInst>& inst, const float* d_Pp2, const float* d_W2, const float* d_Mx,
+              double* d_C) {
+    So3State& S = so3(P);
+    const int B = S.B, n = 2 * B, nq = (int)inst.size();
+    hipStream_t s = P->ctx->stream;
+    S.inst.ensure(nq);
+    ETTK_HIP(hipMemcpyAsync(S.inst.p, inst.data(), nq * sizeof(SoftInst), hipMemcpyHostToDevice, s));
+    S.sig.ensure((size_t)nq * n * n);
+    S.pat.ensure((size_t)nq * n * n);
+    k_assemble<<<dim3(16, nq), kT, 0, s>>>(S.inst.p, S.Pp1.p, d_Pp2, S.W1.p, d_W2, d_Mx, B, S.sig.p, S.pat.p);
+    ETTK_HIP(hipGetLastError());
+    soft_core(P, inst, d_C);
+}
+
+void soft_given(ettk_plan* P, int nq, const double* h_sig, const double* h_pat, double* d_C) {
+    So3State& S = so3(P);
+    const int B = S.B, n = 2 * B;
+    hipStream_t s = P->ctx->stream;
+    std::vector<SoftInst> inst(nq, SoftInst{0, 0, 0, 0});
+    S.inst.ensure(nq);
+    ETTK_HIP(hipMemcpyAsync(S.inst.p, inst.data(), nq * sizeof(SoftInst), hipMemcpyHostToDevice, s));
+    S.sig.upload(h_sig, (size_t)nq * n * n, s);
+    S.pat.upload(h_pat, (size_t)nq * n * n, s);
+    soft_core(P, inst, d_C);
+}
+
+static void so3_kernel_attributes(int B) {
+    int shm = B * (2 * B - 1) * (int)sizeof(double2);
+    ETTK_HIP(hipFuncSetAttribute((const void*)k_rotate_coef, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+}
+
+void launch_combine(const double* C, const ComboOp* d_ops, int np, int B, int b0, double* out, hipStream_t s) {
+    const int n = 2 * B;
+    k_combine<<<dim3(256, np), kT, 0, s>>>(C, d_ops, (long)n * n * n, n, b0, out);
+    ETTK_HIP(hipGetLastError());
+}
+void launch_localmax(const double* C, int np, int B, int cap, int* count, int* idx, double* val, hipStream_t s) {
+    const int n = 2 * B;
+    k_localmax<<<dim3(256, np), kT, 0, s>>>(C, n, cap, count, idx, val);
+    ETTK_HIP(hipGetLastError());
+}
+void launch_ppmax(const float* Pp1, const float* Pp2, int np, int B, float* Mx, hipStream_t s) {
+    k_ppmax<<<np, kT, 0, s>>>(Pp1, Pp2, 4 * B * B, Mx);
+    ETTK_HIP(hipGetLastError());
+}
+
 }  // namespace ettk
-
-using namespace ettk;
-
-extern "C" {
-int ettk_project_batch(ettk_plan*, const float*, int, const ettk_particle*, float*) { return ETTK_ERR_UNSUPPORTED; }
-int ettk_so3_correlate_batch(ettk_plan*, int, const double*, const double*, double*) { return ETTK_ERR_UNSUPPORTED; }
-int ettk_align_batch(ettk_plan*, const float*, int, const ettk_particle*, int, ettk_result*) { return ETTK_ERR_UNSUPPORTED; }
-int ettk_ccc_grad_batch(ettk_plan*, const float*, int, const ettk_particle*, int, const int*, const double*,
-                        const double*, double*, double*) {
-    return ETTK_ERR_UNSUPPORTED;
-}
-int ettk_avg_reset(ettk_plan*) { return ETTK_ERR_UNSUPPORTED; }
-int ettk_avg_accumulate(ettk_plan*, const float*, int, const ettk_particle*, const float*, const double*) {
-    return ETTK_ERR_UNSUPPORTED;
-}
-int ettk_avg_finalize(ettk_plan*, float*) { return ETTK_ERR_UNSUPPORTED; }
-}

@@ -1,0 +1,61 @@
+// so3.hpp -- S^2 projection / SO(3) search state and launch wrappers.
+#pragma once
+#include "plan.hpp"
+
+namespace ettk {
+
+// sample recipes of the soft() calls (ProjRot.h:526-584)
+enum Recipe : int {
+    RC_P1SQ_W1 = 0, RC_W1, RC_P1_W1, RC_P2SQ_W2, RC_W2, RC_P2_W2, RC_P1, RC_P2, RC_P1SQ, RC_P2SQ
+};
+struct SoftInst {
+    int particle;  // slot of the particle inside the current chunk
+    int sig, pat;  // recipes of signal (F1) and pattern (F2)
+    int scaled;    // softScaled (1) or soft (0)
+};
+enum ComboKind : int { CO_COPY = 0, CO_DIV_SQRT, CO_BOTH };
+struct ComboOp {
+    int kind;
+    int i[4];      // instance indices
+};
+
+struct So3State {
+    int B = 0;
+    int d_slice = 0;
+    bool use_window1 = false;
+    DevBuf<double> legF, legI, dtab;
+    DevBuf<int> doff;
+    DevBuf<double2> tw;
+    // reference
+    DevBuf<float> Pp1, W1;
+    // projection workspace
+    DevBuf<float> projR, projMag, projStats, projPartials;
+    DevBuf<float2> projF;
+    DevBuf<EvalGeom> projGeom;
+    // soft workspace
+    DevBuf<SoftInst> inst;
+    DevBuf<double> sig, pat, vals, Rr, wv;
+    DevBuf<double2> fm, cs, cp, T1, T2, rot, g;
+    DevBuf<int> pos;
+    // search workspace (align driver)
+    DevBuf<float> Pp2, W2, Mx;
+    DevBuf<double> C, Cc;
+    DevBuf<ComboOp> ops;
+    DevBuf<int> lm_count, lm_idx;
+    DevBuf<double> lm_val;
+};
+
+EvalGeom make_geom_public(const ettk_plan* P, const ettk_particle& d, const Mat4& t, int particle);
+// smoothed analytic spherical wedge (nbfWedge3D::getSphericalImage + conditionSphericalWindow), host FP64
+std::vector<float> spherical_wedge_host(int B, const ettk_particle& d, bool wedge_on);
+void project_images(ettk_plan* P, const float* d_parts, const ettk_particle* descs, const int* pidx, int n, bool is_ref,
+                    float* d_Pp);
+void run_soft(ettk_plan* P, const std::vector<SoftInst>& inst, const float* d_Pp2, const float* d_W2, const float* d_Mx,
+              double* d_C);
+void soft_given(ettk_plan* P, int nq, const double* h_sig, const double* h_pat, double* d_C);
+So3State& so3_state(ettk_plan* P);
+void launch_combine(const double* C, const ComboOp* d_ops, int np, int B, int b0, double* out, hipStream_t s);
+void launch_localmax(const double* C, int np, int B, int cap, int* count, int* idx, double* val, hipStream_t s);
+void launch_ppmax(const float* Pp1, const float* Pp2, int np, int B, float* Mx, hipStream_t s);
+
+}  // namespace ettk
