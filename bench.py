@@ -1,0 +1,191 @@
+"""Benchmark of the sub-volume alignment hot path on one MI355X.
+
+Workload (the plain GPU scale-up configuration): 1024 particles of 64^3 fp32 from a 40-blob reference,
+per-particle missing wedges +-U[45,70] degrees about Y, SNR U[0.05,0.2], SH bandwidth B=32, aligned to the
+reference (S^2 projection, SO(3) search, direct/mirror constrained CCC at Kp=5 peaks, Kc=1 refined candidate).
+Inputs are generated on the GPU and resident in HBM before timing starts.
+
+A step is one alignment pass of the headline form over the whole batch.  Forms measured:
+  plain    - the reference's control flow particle after particle (one CCC per launch), on a subset
+  batched  - every stage over all particles; per-particle control flow served by batched launches
+The JSON line reports the batched form's particles/s as `value`, the plain form and the ratio beside it,
+a roofline record for the dominant kernel (HIP events inside the library, on its stream) and the FP64 CPU
+oracle timed on this host as a reported baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "aligned particles/s, 64^3 B=32: batched+adjoint vs plain GPU at ≤0.5° error"
+HBM_PEAK_GBS = 8000.0
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1024, help="particles per step")
+    ap.add_argument("--plain-n", type=int, default=16, help="particles timed in the plain form")
+    ap.add_argument("--cpu-n", type=int, default=8, help="particles of the CPU oracle sample (0 = skip)")
+    ap.add_argument("--seed", type=int, default=2024)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only (DESIGN.md)"}))
+        return 2
+
+    import torch
+
+    from ettk_amd.abi import default_params
+    from ettk_amd.lib import Context, Plan
+    from ettk_amd.synth import blob_reference, make_particles_device, rotation_error_deg
+
+    N, B = 64, 32
+    p = default_params(N, B)
+    ref = blob_reference(N, seed=args.seed)
+    t0 = time.time()
+    d_parts, descs, truth = make_particles_device(ref, args.n, seed=args.seed + 1, half_wedge=(45.0, 70.0),
+                                                  snr=(0.05, 0.2))
+    gen_s = time.time() - t0
+    ctx = Context(0)
+    plan = Plan(ctx, p)
+    d_ref = torch.from_numpy(ref).cuda()
+    torch.cuda.synchronize()
+    plan.set_reference(d_ref)
+
+    # ---------------------------------------------------------------- plain form on a subset
+    sub = d_parts[: args.plain_n].contiguous()
+    torch.cuda.synchronize()
+    plan.align(sub, descs[: args.plain_n], mode=0)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res_plain = plan.align(sub, descs[: args.plain_n], mode=0)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    plain_pps = args.plain_n / plain_s
+
+    # ---------------------------------------------------------------- batched form (Powell, the reference method)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res_b = plan.align(d_parts, descs, mode=1)
+    torch.cuda.synchronize()
+    batched_pps = args.n / (time.perf_counter() - t0)
+
+    # ---------------------------------------------------------------- batched + adjoint form (headline)
+    for _ in range(args.warmup):
+        plan.align(d_parts, descs, mode=2)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    step_ms = []
+    res = None
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = plan.align(d_parts, descs, mode=2)
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - t0) * 1e3)
+    kernels = {}
+    for name in ("k_fft_strided<64>", "k_r2c_rows<64>", "k_c2r_rows<64>", "k_reslice", "k_spec_filter", "k_mask_ref",
+                 "k_combine_dftA", "k_dftBC_argmax", "k_fft_strided<128>", "k_r2c_rows<128>",
+                 "k_reslice_ref", "k_refine_dot", "k_refine_adj", "k_refine_moments"):
+        ms, nl, by = ctx.profile_query(name)
+        if nl:
+            kernels[name] = {"ms": ms, "launches": nl, "bytes": by}
+    host = {}
+    for name in ("search_gpu_wall", "search_host", "ccc_batch_wall", "lm_count"):
+        ms, nl, _ = ctx.profile_query(name)
+        host[name] = {"total": round(ms, 2), "count": nl}
+    ctx.profile(False)
+    ms_per_step = float(np.mean(step_ms))
+    value = args.n / (ms_per_step / 1e3)
+
+    # consistency of the two forms on the shared subset, and accuracy against the ground truth
+    same = sum(1 for a, b in zip(res_plain, res_b[: args.plain_n])
+               if a.n_ccc == b.n_ccc and np.array_equal(a.transform(), b.transform()))
+    rot_err = np.array([rotation_error_deg(r.transform(), truth[i]) for i, r in enumerate(res)])
+    rot_err_b = np.array([rotation_error_deg(r.transform(), truth[i]) for i, r in enumerate(res_b)])
+    sh_err = np.array([np.abs(r.transform()[:3, 3] - truth[i][:3, 3]).max() for i, r in enumerate(res)])
+    sh_err_b = np.array([np.abs(r.transform()[:3, 3] - truth[i][:3, 3]).max() for i, r in enumerate(res_b)])
+    n_ccc = np.array([r.n_ccc for r in res])
+    n_ccc_b = np.array([r.n_ccc for r in res_b])
+
+    # ---------------------------------------------------------------- roofline of the dominant kernel
+    dom = max(kernels.items(), key=lambda kv: kv[1]["ms"])
+    avg_ms = dom[1]["ms"] / dom[1]["launches"]
+    bytes_per_launch = dom[1]["bytes"] / dom[1]["launches"]
+    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": bytes_per_launch}
+
+    # ---------------------------------------------------------------- CPU baseline: FP64 oracle on host cores
+    cpu = None
+    if args.cpu_n > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import OraclePlan
+
+        cores = os.cpu_count() or 1
+        threads = max(1, min(args.cpu_n, 16, cores))
+        vols = d_parts[: args.cpu_n].cpu().numpy()
+        orc = OraclePlan(p)
+        orc.set_reference(ref)
+        t0 = time.perf_counter()
+        orc.align_many(vols, descs[: args.cpu_n], nthreads=threads)
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": round(args.cpu_n / cpu_s, 4), "unit": "particles/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_n} particles of this workload, FP64 oracle, one particle per OpenMP thread, "
+                         f"{cpu_s:.1f} s wall"}
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (40-blob reference, rotated/shifted/wedged/noisy particles generated on the GPU)",
+        "config": {"workload": "plain GPU scale-up: 1024 particles 64^3, B=32, wedges +-U[45,70], SNR U[0.05,0.2]",
+                   "particles": args.n, "box": N, "bandwidth": B, "form": "batched+adjoint",
+                   "Kp": p.n_peaks, "Kc": p.n_candidates, "trans_restrict": p.trans_restrict},
+        "forms": {"plain": {"particles_per_s": round(plain_pps, 3), "particles": args.plain_n},
+                  "batched": {"particles_per_s": round(batched_pps, 3), "particles": args.n,
+                              "ccc_per_particle": round(float(n_ccc_b.mean()), 2),
+                              "rot_err_deg_median": round(float(np.median(rot_err_b)), 3),
+                              "shift_err_vox_median": round(float(np.median(sh_err_b)), 3)},
+                  "batched+adjoint": {"particles_per_s": round(value, 3), "particles": args.n,
+                                      "ccc_per_particle": round(float(n_ccc.mean()), 2),
+                                      "refine_iterations": 20,
+                                      "rot_err_deg_median": round(float(np.median(rot_err)), 3),
+                                      "shift_err_vox_median": round(float(np.median(sh_err)), 3),
+                                      "frac_rot_err_below_5deg": round(float(np.mean(rot_err < 5.0)), 3)},
+                  "batched+adjoint_over_plain": round(value / plain_pps, 2),
+                  "batched_over_plain": round(batched_pps / plain_pps, 2),
+                  "plain_batched_identical": f"{same}/{args.plain_n}"},
+        "kernels": kernels,
+        "host_stages": host,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "gen_s": round(gen_s, 1),
+    }
+    print(json.dumps(line))
+    plan.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -8,9 +8,11 @@
 //        2 = batched + adjoint refinement (refine.cpp).
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <queue>
 #include <thread>
@@ -59,20 +61,58 @@ struct CandCmp {
 };
 using CandQueue = std::priority_queue<Cand, std::vector<Cand>, CandCmp>;
 
+// The reference compares every pair (i, j) in index order and drops the worse of two rotations whose 4x4
+// matrices are within `tol` in L1 norm.  A pair within tol differs by less than tol in every matrix element, so
+// its third columns fall in the same or adjacent cells of a grid of pitch tol: only those pairs are visited,
+// in the same (i, then increasing j) order, which leaves every decision of the exhaustive loop unchanged.
 static void eliminate_equivalent(std::vector<Cand>& pos, CandQueue& pq, float tol) {
     const float NEG = -FLT_MAX;
-    std::vector<Mat4> mats(pos.size());
-    for (size_t i = 0; i < pos.size(); i++) mats[i] = zyz(pos[i].a, pos[i].b, pos[i].g);
-    for (size_t i = 0; i < pos.size(); i++) {
-        for (size_t j = 0; j < pos.size(); j++) {
-            if (pos[i].v == NEG) break;
-            if (i != j && pos[j].v > NEG) {
-                double err = 0;
-                for (int m = 0; m < 16; m++) err += std::fabs(mats[i].a[m] - mats[j].a[m]);
-                if (err < tol) {
-                    if (pos[i].v > pos[j].v) pos[i].v = NEG;
-                    else pos[j].v = NEG;
-                }
+    const size_t n = pos.size();
+    std::vector<Mat4> mats(n);
+    for (size_t i = 0; i < n; i++) mats[i] = zyz(pos[i].a, pos[i].b, pos[i].g);
+    auto close = [&](size_t i, size_t j) {
+        double err = 0;
+        for (int m = 0; m < 16; m++) err += std::fabs(mats[i].a[m] - mats[j].a[m]);
+        return err < tol;
+    };
+    auto visit = [&](size_t i, size_t j) {
+        if (i != j && pos[j].v > NEG && close(i, j)) {
+            if (pos[i].v > pos[j].v) pos[i].v = NEG;
+            else pos[j].v = NEG;
+        }
+    };
+    if (n < 64) {
+        for (size_t i = 0; i < n; i++)
+            for (size_t j = 0; j < n; j++) {
+                if (pos[i].v == NEG) break;
+                visit(i, j);
+            }
+    } else {
+        // matrix elements of a rotation lie in [-1, 1]: a dense grid of side G covers every cell (+1 margin)
+        const int G = (int)std::ceil(2.0 / tol) + 3;
+        auto cell = [&](size_t i, int c) {
+            long v = (long)std::floor(mats[i].a[c] / (double)tol) + G / 2;
+            return (int)std::min<long>(std::max<long>(v, 1), G - 2);
+        };
+        std::vector<int> head((size_t)G * G * G, -1), next(n, -1), key(n);
+        for (size_t i = n; i-- > 0;) {  // each cell's chain in increasing index order
+            int k = (cell(i, 2) * G + cell(i, 6)) * G + cell(i, 10);
+            key[i] = k;
+            next[i] = head[k];
+            head[k] = (int)i;
+        }
+        std::vector<size_t> js;
+        for (size_t i = 0; i < n; i++) {
+            if (pos[i].v == NEG) continue;
+            js.clear();
+            for (int a = -1; a <= 1; a++)
+                for (int b = -1; b <= 1; b++)
+                    for (int c = -1; c <= 1; c++)
+                        for (int j = head[key[i] + (a * G + b) * G + c]; j >= 0; j = next[j]) js.push_back((size_t)j);
+            std::sort(js.begin(), js.end());
+            for (size_t j : js) {
+                if (pos[i].v == NEG) break;
+                visit(i, j);
             }
         }
     }
@@ -110,6 +150,8 @@ static void search_chunk(ettk_plan* P, const float* d_parts, const ettk_particle
     const int B = P->p.B, n = 2 * B, nn = n * n;
     const long n3 = (long)n * n * n;
     hipStream_t s = P->ctx->stream;
+    KernelProf* kp = P->ctx->fft.prof;
+    auto tg0 = std::chrono::steady_clock::now();
     S.Pp2.ensure((size_t)cn * nn);
     S.W2.ensure((size_t)cn * nn);
     S.Mx.ensure(cn);
@@ -175,27 +217,50 @@ static void search_chunk(ettk_plan* P, const float* d_parts, const ettk_particle
     std::vector<int> cnt(2 * cn);
     ETTK_HIP(hipMemcpyAsync(cnt.data(), S.lm_count.p, 2 * cn * sizeof(int), hipMemcpyDeviceToHost, s));
     ETTK_HIP(hipStreamSynchronize(s));
-    std::vector<int> idx;
-    std::vector<double> val;
+    auto th0 = std::chrono::steady_clock::now();
+    if (kp) kp->add_host("search_gpu_wall", std::chrono::duration<double, std::milli>(th0 - tg0).count());
+    std::vector<std::vector<int>> idx(2 * cn);
+    std::vector<std::vector<double>> val(2 * cn);
     for (int v = 0; v < 2 * cn; v++) {
         int q = v / 2;
         bool two = uw1 || uw2[q];
         if (v % 2 == 1 && !two) continue;
         ETTK_REQUIRE(cnt[v] <= cap, ETTK_ERR_UNSUPPORTED, "more SO(3) local maxima than the candidate list holds");
-        idx.resize(cnt[v]);
-        val.resize(cnt[v]);
+        idx[v].resize(cnt[v]);
+        val[v].resize(cnt[v]);
         if (cnt[v] > 0) {
-            ETTK_HIP(hipMemcpyAsync(idx.data(), S.lm_idx.p + (size_t)v * cap, cnt[v] * sizeof(int), hipMemcpyDeviceToHost, s));
-            ETTK_HIP(hipMemcpyAsync(val.data(), S.lm_val.p + (size_t)v * cap, cnt[v] * sizeof(double), hipMemcpyDeviceToHost, s));
-            ETTK_HIP(hipStreamSynchronize(s));
+            ETTK_HIP(hipMemcpyAsync(idx[v].data(), S.lm_idx.p + (size_t)v * cap, cnt[v] * sizeof(int), hipMemcpyDeviceToHost, s));
+            ETTK_HIP(hipMemcpyAsync(val[v].data(), S.lm_val.p + (size_t)v * cap, cnt[v] * sizeof(double), hipMemcpyDeviceToHost, s));
         }
-        std::vector<std::pair<int, double>> lm(cnt[v]);
-        for (int k = 0; k < cnt[v]; k++) lm[k] = {idx[k], val[k]};
-        SearchOut& o = out[p0 + q];
-        o.two = two;
-        o.use_window = (uw1 ? 1 : 0) | (uw2[q] ? 2 : 0);
-        candidates_from_maxima(P, lm, v % 2 == 0 ? o.pq1 : o.pq2);
+        if (kp && kp->on) kp->add_host("lm_count", (double)cnt[v]);
     }
+    ETTK_HIP(hipStreamSynchronize(s));
+    for (int q = 0; q < cn; q++) {
+        SearchOut& o = out[p0 + q];
+        o.two = uw1 || uw2[q];
+        o.use_window = (uw1 ? 1 : 0) | (uw2[q] ? 2 : 0);
+    }
+    // candidate lists of the volumes are independent: extract them on several host threads
+    auto work = [&](int v) {
+        int q = v / 2;
+        if (v % 2 == 1 && !out[p0 + q].two) return;
+        std::vector<std::pair<int, double>> lm(idx[v].size());
+        for (size_t k = 0; k < lm.size(); k++) lm[k] = {idx[v][k], val[v][k]};
+        SearchOut& o = out[p0 + q];
+        candidates_from_maxima(P, lm, v % 2 == 0 ? o.pq1 : o.pq2);
+    };
+    int nthr = std::max(1, std::min<int>(2 * cn, (int)std::thread::hardware_concurrency()));
+    if (nthr <= 1) {
+        for (int v = 0; v < 2 * cn; v++) work(v);
+    } else {
+        std::vector<std::thread> th;
+        for (int w = 0; w < nthr; w++)
+            th.emplace_back([&, w] {
+                for (int v = w; v < 2 * cn; v += nthr) work(v);
+            });
+        for (auto& t : th) t.join();
+    }
+    if (kp) kp->add_host("search_host", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count());
 }
 
 // ------------------------------------------------------------------ Powell (vnl_powell, maxfev = 1)
@@ -394,8 +459,11 @@ static float refine_powell(const ettk_plan* P, int pi, Mat4& t, bool run, const 
     return last;
 }
 
-// candidate loop of nbfProjectionRotationMetric3D::execute (ProjRot.h:678-901)
-static void candidate_stage(const ettk_plan* P, int pi, SearchOut& so, const EvalFn& eval, ettk_result& out) {
+// candidate loop of nbfProjectionRotationMetric3D::execute (ProjRot.h:678-901).  With `seeds` set (batched
+// + adjoint mode) each of the Kc best candidates gets the refine(t, 0) evaluation only (one CCC at the candidate
+// rotation, FIM.h:1758-1766) and its transform is handed back as a seed of the continuous refinement.
+static void candidate_stage(const ettk_plan* P, int pi, SearchOut& so, const EvalFn& eval, ettk_result& out,
+                            std::vector<Mat4>* seeds = nullptr) {
     std::vector<Cand> combined;
     int nq = so.two ? 2 : 1;
     int nccc = 0;
@@ -431,7 +499,8 @@ static void candidate_stage(const ettk_plan* P, int pi, SearchOut& so, const Eva
     while (!pqc.empty() && index < P->p.n_candidates) {
         Cand p = pqc.top();
         Mat4 t = zyz(p.a, p.b, p.g);
-        float dist = refine_powell(P, pi, t, P->p.refine != 0, counted);
+        float dist = refine_powell(P, pi, t, seeds == nullptr && P->p.refine != 0, counted);
+        if (seeds) seeds->push_back(t);
         if (dist < best) {
             best = dist;
             bestT = t;
@@ -581,10 +650,58 @@ void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
     (void)hipEventDestroy(b);
 }
 
+void refine_with_seeds(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
+                       const Mat4* seeds, int n_iter, Mat4* out_pose, double* out_score);
+
+// batched + adjoint mode: candidate stage with batched CCCs, then lock-step continuous refinement of every
+// seed, then one constrained CCC per refined pose for the reported distance (the continuous pose is kept)
 void refine_adjoint_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n,
                           std::vector<SearchOut*>& so, ettk_result* out) {
-    (void)P; (void)d_parts; (void)descs; (void)n; (void)so; (void)out;
-    throw Error{ETTK_ERR_UNSUPPORTED, "mode 2 (adjoint refinement) is not available in this build"};
+    std::vector<std::vector<Mat4>> seeds(n);
+    {
+        int T = std::min(n, 1024);
+        BatchEval be(P, d_parts, descs, T);
+        EvalFn ev = [&](int pi, Mat4& t) { return be.eval(pi, t); };
+        std::vector<std::thread> th;
+        for (int w = 0; w < T; w++)
+            th.emplace_back([&, w] {
+                try {
+                    for (int i = w; i < n; i += T) candidate_stage(P, i, *so[i], ev, out[i], &seeds[i]);
+                    be.done();
+                } catch (const Error& e) {
+                    be.fail(e);
+                }
+            });
+        for (auto& t : th) t.join();
+    }
+    std::vector<int> pidx, cand;
+    std::vector<Mat4> sd;
+    for (int i = 0; i < n; i++)
+        for (size_t c = 0; c < seeds[i].size(); c++) {
+            pidx.push_back(i);
+            cand.push_back((int)c);
+            sd.push_back(seeds[i][c]);
+        }
+    const int nj = (int)sd.size();
+    if (nj == 0) return;
+    std::vector<Mat4> pose(nj), fin(nj);
+    std::vector<double> score(nj);
+    refine_with_seeds(P, d_parts, descs, nj, pidx.data(), sd.data(), 20, pose.data(), score.data());
+    std::vector<float> dist(nj);
+    fin = pose;
+    ccc_batch(P, d_parts, descs, nj, pidx.data(), fin.data(), dist.data());
+    std::vector<float> best(n, FLT_MAX);
+    for (int j = 0; j < nj; j++) {
+        int i = pidx[j], c = cand[j];
+        out[i].cand_distance[c] = dist[j];
+        std::memcpy(out[i].cand_matrix[c], pose[j].a, sizeof(pose[j].a));
+        out[i].n_ccc++;
+        if (dist[j] < best[i]) {
+            best[i] = dist[j];
+            out[i].distance = dist[j];
+            std::memcpy(out[i].matrix, pose[j].a, sizeof(pose[j].a));
+        }
+    }
 }
 
 }  // namespace ettk

@@ -45,25 +45,55 @@ __device__ __forceinline__ void keys_w(float f, float* w) {  // vtkTricubicInter
     w[3] = f * fd2 * fm1;
 }
 
+// One workgroup reslices an 8x8x8 tile of the output.  The input voxels its 64-tap stencils can touch lie in
+// the clamped bounding box of the tile's transformed corners (at most 17 voxels per axis for a rigid map);
+// that box is staged in LDS once and every tap is read from LDS.  A non-rigid map whose box does not fit falls
+// back to reading the taps from global memory.
+static constexpr int kTile = 8, kBox = 18;
 __global__ void __launch_bounds__(kThreads) k_reslice(const float* __restrict__ parts, const EvalGeom* __restrict__ geom,
                                                        int N, float* __restrict__ out, float* __restrict__ partials) {
     __shared__ float sh[8];
+    __shared__ float box[kBox * kBox * kBox];
     const int e = blockIdx.y;
     const EvalGeom g = geom[e];
     const float* vol = parts + (size_t)g.particle * N * N * N;
     const long n3 = (long)N * N * N;
-    const long chunk = (n3 + gridDim.x - 1) / gridDim.x;
-    const long o0 = blockIdx.x * chunk, o1 = min(n3, o0 + chunk);
+    const int nt = N / kTile;
+    const int ti = blockIdx.x / (nt * nt), tj = (blockIdx.x / nt) % nt, tk = blockIdx.x % nt;
+    const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * kTile;
+    const float tol = 7.62939453125e-06f;
+    // bounding box of the stencils in input index space (per axis: x, y, z)
+    int lo[3], dim[3];
+    bool fits = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float c0 = g.m[a] * i0 + g.m[3 + a] * j0 + g.m[6 + a] * k0 + g.m[9 + a];
+        float ext = (kTile - 1) * (fabsf(g.m[a]) + fabsf(g.m[3 + a]) + fabsf(g.m[6 + a]));
+        float mn = c0 + (kTile - 1) * (fminf(g.m[a], 0.f) + fminf(g.m[3 + a], 0.f) + fminf(g.m[6 + a], 0.f));
+        int l = min(max((int)floorf(mn + tol) - 1, 0), N - 1);
+        int h = min(max((int)floorf(mn + ext + tol) + 2, 0), N - 1);
+        lo[a] = l;
+        dim[a] = h - l + 1;
+        if (dim[a] > kBox) fits = false;
+    }
+    if (fits) {
+        const int nbox = dim[0] * dim[1] * dim[2];
+        for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
+            int x = t % dim[0], y = (t / dim[0]) % dim[1], z = t / (dim[0] * dim[1]);
+            box[t] = vol[((size_t)(lo[2] + z) * N + (lo[1] + y)) * N + (lo[0] + x)];
+        }
+    }
+    __syncthreads();
     float s = 0.f, c = 0.f;
-    const float lo = -0.5f, hi = (float)N - 0.5f;
-    for (long o = o0 + threadIdx.x; o < o1; o += blockDim.x) {
-        int i = (int)(o / ((long)N * N)), j = (int)((o / N) % N), k = (int)(o % N);
+    const float lob = -0.5f, hib = (float)N - 0.5f;
+    for (int t = threadIdx.x; t < kTile * kTile * kTile; t += blockDim.x) {
+        const int i = i0 + t / (kTile * kTile), j = j0 + (t / kTile) % kTile, k = k0 + t % kTile;
         float px = g.m[0] * i + g.m[3] * j + g.m[6] * k + g.m[9];
         float py = g.m[1] * i + g.m[4] * j + g.m[7] * k + g.m[10];
         float pz = g.m[2] * i + g.m[5] * j + g.m[8] * k + g.m[11];
         float v = -FLT_MAX;
-        if (px >= lo && px <= hi && py >= lo && py <= hi && pz >= lo && pz <= hi) {
-            float xx = px + 7.62939453125e-06f, yy = py + 7.62939453125e-06f, zz = pz + 7.62939453125e-06f;
+        if (px >= lob && px <= hib && py >= lob && py <= hib && pz >= lob && pz <= hib) {
+            float xx = px + tol, yy = py + tol, zz = pz + tol;
             int ix = (int)floorf(xx), iy = (int)floorf(yy), iz = (int)floorf(zz);
             float wx[4], wy[4], wz[4];
             keys_w(xx - ix, wx);
@@ -77,21 +107,40 @@ __global__ void __launch_bounds__(kThreads) k_reslice(const float* __restrict__ 
                 cz[q] = min(max(iz - 1 + q, 0), N - 1);
             }
             float acc = 0.f;
+            if (fits) {
 #pragma unroll
-            for (int a = 0; a < 4; a++) {
-                float ay = 0.f;
-#pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    const float* row = vol + ((size_t)cz[a] * N + cy[b]) * N;
-                    ay += wy[b] * (wx[0] * row[cx[0]] + wx[1] * row[cx[1]] + wx[2] * row[cx[2]] + wx[3] * row[cx[3]]);
+                for (int q = 0; q < 4; q++) {
+                    cx[q] -= lo[0];
+                    cy[q] -= lo[1];
+                    cz[q] -= lo[2];
                 }
-                acc += wz[a] * ay;
+#pragma unroll
+                for (int a = 0; a < 4; a++) {
+                    float ay = 0.f;
+#pragma unroll
+                    for (int b = 0; b < 4; b++) {
+                        const float* row = box + (cz[a] * dim[1] + cy[b]) * dim[0];
+                        ay += wy[b] * (wx[0] * row[cx[0]] + wx[1] * row[cx[1]] + wx[2] * row[cx[2]] + wx[3] * row[cx[3]]);
+                    }
+                    acc += wz[a] * ay;
+                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < 4; a++) {
+                    float ay = 0.f;
+#pragma unroll
+                    for (int b = 0; b < 4; b++) {
+                        const float* row = vol + ((size_t)cz[a] * N + cy[b]) * N;
+                        ay += wy[b] * (wx[0] * row[cx[0]] + wx[1] * row[cx[1]] + wx[2] * row[cx[2]] + wx[3] * row[cx[3]]);
+                    }
+                    acc += wz[a] * ay;
+                }
             }
             v = acc;
             s += v;
             c += 1.f;
         }
-        out[(size_t)e * n3 + o] = v;
+        out[(size_t)e * n3 + ((size_t)i * N + j) * N + k] = v;
     }
     float ts = block_sum(s, sh);
     float tc = block_sum(c, sh);
@@ -112,8 +161,11 @@ __global__ void k_reslice_mean(const float* __restrict__ partials, int nb, int n
     stats[e] = (float)(s / (double)(float)c);  // meanInside is a float (WSub.h:794-797)
 }
 
+int reslice_blocks(int N) { return (N / kTile) * (N / kTile) * (N / kTile); }
+
 void launch_reslice(const float* parts, const EvalGeom* geom, int n_eval, int N, float* out, float* partials,
                     int nb, hipStream_t s) {
+    ETTK_REQUIRE(N % kTile == 0 && nb == reslice_blocks(N), ETTK_ERR_ARG, "launch_reslice: block count mismatch");
     dim3 grid(nb, n_eval);
     k_reslice<<<grid, kThreads, 0, s>>>(parts, geom, N, out, partials);
     ETTK_HIP(hipGetLastError());

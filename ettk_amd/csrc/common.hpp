@@ -63,8 +63,84 @@ struct DevBuf {
     }
 };
 
+// ------------------------------------------------------------------ per-kernel HIP-event timing
+// When enabled, every instrumented launch is bracketed by two events on its stream; totals are read back by
+// ettk_profile_query.  Algorithmic bytes per launch are accumulated beside the times.
+struct KernelProf {
+    bool on = false;
+    struct Rec {
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+        double bytes = 0;
+        double ms = 0;
+        long long launches = 0;
+    };
+    std::vector<std::pair<std::string, Rec>> recs;
+    std::vector<hipEvent_t> pool;
+    Rec& rec(const char* name) {
+        for (auto& r : recs)
+            if (r.first == name) return r.second;
+        recs.emplace_back(name, Rec{});
+        return recs.back().second;
+    }
+    hipEvent_t take() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        ETTK_HIP(hipEventCreate(&e));
+        return e;
+    }
+    // fold recorded event pairs into totals (call after the stream has been synchronised)
+    void collect() {
+        for (auto& r : recs) {
+            for (auto& p : r.second.ev) {
+                float ms = 0;
+                ETTK_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+                r.second.ms += ms;
+                pool.push_back(p.first);
+                pool.push_back(p.second);
+            }
+            r.second.ev.clear();
+        }
+    }
+    void reset() {
+        collect();
+        recs.clear();
+    }
+    void add_host(const char* name, double ms) {
+        if (!on) return;
+        Rec& r = rec(name);
+        r.ms += ms;
+        r.launches++;
+    }
+};
+struct ProfScope {
+    KernelProf* P;
+    KernelProf::Rec* r = nullptr;
+    hipEvent_t b = nullptr;
+    hipStream_t s;
+    ProfScope(KernelProf* prof, const char* name, double bytes, hipStream_t st) : P(prof), s(st) {
+        if (P && P->on) {
+            r = &P->rec(name);
+            r->bytes += bytes;
+            r->launches++;
+            b = P->take();
+            ETTK_HIP(hipEventRecord(b, s));
+        }
+    }
+    ~ProfScope() {
+        if (r) {
+            hipEvent_t e = P->take();
+            if (hipEventRecord(e, s) == hipSuccess) r->ev.emplace_back(b, e);
+        }
+    }
+};
+
 // ------------------------------------------------------------------ FFT plans (twiddle tables)
 struct FftTables {
+    KernelProf* prof = nullptr;
     // twiddle[L] = exp(-2 pi i t / L), t in [0, L), for every supported length, packed
     DevBuf<float2> tw;
     int offset[9] = {};  // offsets per supported length index

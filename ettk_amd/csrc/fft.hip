@@ -169,7 +169,7 @@ __device__ __forceinline__ float r2c_load(const R2CArgs& a, long row, int k) {
         case PRO_RESLICED: {
             if (x == -3.402823466e+38f) return 0.0f;  // outside the particle box: -FLT_MAX background
             long b = row / ((long)L * L);
-            return (x - a.stats[b]) * a.aux0[ij * L + k] * sg;
+            return (x - a.stats[b]) * (a.aux0 ? a.aux0[ij * L + k] : 1.0f) * sg;
         }
     }
     return x;
@@ -295,20 +295,40 @@ __global__ void __launch_bounds__(Shape<L>::THREADS) k_c2r_rows(const float2* __
 }
 
 // ------------------------------------------------------------------ host launchers
+// profile record names: k_r2c_rows<L>, k_fft_strided<L>, k_c2r_rows<L>
+template <int L>
+static const char* prof_name(int which) {
+    static const std::string n[3] = {"k_r2c_rows<" + std::to_string(L) + ">", "k_fft_strided<" + std::to_string(L) + ">",
+                                     "k_c2r_rows<" + std::to_string(L) + ">"};
+    return n[which].c_str();
+}
+
 template <int L>
 static void launch_r2c(const FftTables& T, const R2CArgs& a, float2* scratch, hipStream_t s) {
     using S = Shape<L>;
     constexpr int H = L / 2 + 1;
     const float2* tw = T.tw.p + T.offset[fft_len_index(L)];
     long npairs = (long)a.batch * L * L / 2;
+    const double vol_c = (double)a.batch * L * L * H * sizeof(float2);  // one pass over the half spectrum
+    const double vol_r = a.pro == PRO_PAD_MASKED ? (double)a.batch * a.ns * a.ns * a.ns * sizeof(float)
+                                                 : (double)a.batch * L * L * L * sizeof(float);
     // pass k: real rows -> a.out (half rows)
-    k_r2c_rows<L><<<(unsigned)((npairs + S::P - 1) / S::P), S::THREADS, 0, s>>>(a, tw);
+    {
+        ProfScope ps(T.prof, prof_name<L>(0), vol_r + vol_c, s);
+        k_r2c_rows<L><<<(unsigned)((npairs + S::P - 1) / S::P), S::THREADS, 0, s>>>(a, tw);
+    }
     // pass j: pencils over (b, i, k), element stride H
     StridedArgs sj{a.out, a.out, (long)a.batch * L * H, (long)H, (long)L * H, (long)H, 0};
-    k_fft_strided<L><<<(unsigned)((sj.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(sj, tw);
+    {
+        ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
+        k_fft_strided<L><<<(unsigned)((sj.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(sj, tw);
+    }
     // pass i: pencils over (b, j, k), element stride L*H
     StridedArgs si{a.out, a.out, (long)a.batch * L * H, (long)L * H, (long)L * L * H, (long)L * H, 0};
-    k_fft_strided<L><<<(unsigned)((si.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(si, tw);
+    {
+        ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
+        k_fft_strided<L><<<(unsigned)((si.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(si, tw);
+    }
     (void)scratch;
 }
 
@@ -317,12 +337,22 @@ static void launch_c2r(const FftTables& T, const float2* in, float2* tmp, float*
     using S = Shape<L>;
     constexpr int H = L / 2 + 1;
     const float2* tw = T.tw.p + T.offset[fft_len_index(L)];
+    const double vol_c = (double)batch * L * L * H * sizeof(float2), vol_r = (double)batch * L * L * L * sizeof(float);
     StridedArgs si{in, tmp, (long)batch * L * H, (long)L * H, (long)L * L * H, (long)L * H, 1};
-    k_fft_strided<L><<<(unsigned)((si.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(si, tw);
+    {
+        ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
+        k_fft_strided<L><<<(unsigned)((si.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(si, tw);
+    }
     StridedArgs sj{tmp, tmp, (long)batch * L * H, (long)H, (long)L * H, (long)H, 1};
-    k_fft_strided<L><<<(unsigned)((sj.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(sj, tw);
+    {
+        ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
+        k_fft_strided<L><<<(unsigned)((sj.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(sj, tw);
+    }
     long npairs = (long)batch * L * L / 2;
-    k_c2r_rows<L><<<(unsigned)((npairs + S::P - 1) / S::P), S::THREADS, 0, s>>>(tmp, out, npairs, tw);
+    {
+        ProfScope ps(T.prof, prof_name<L>(2), vol_c + vol_r, s);
+        k_c2r_rows<L><<<(unsigned)((npairs + S::P - 1) / S::P), S::THREADS, 0, s>>>(tmp, out, npairs, tw);
+    }
 }
 
 void fft3_r2c(const FftTables& T, const R2CArgs& a, float2* scratch, hipStream_t s) {

@@ -1,5 +1,6 @@
 // host.cpp -- plan set-up, the constrained-CCC batch driver and the C ABI entry points of libettk_align.
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <mutex>
 
@@ -234,13 +235,14 @@ void ensure_ccc_workspace(ettk_plan* P, int cap) {
     const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * H;
     P->nb_real = (int)((n3 + 1023) / 1024);
     P->nb_spec = (int)((nh + 1023) / 1024);
+    P->nb_res = reslice_blocks(N);
     P->R1.ensure(n3 * cap);
     P->R2.ensure(n3 * cap);
     P->S1.ensure(nh * cap);
     P->S2.ensure(nh * cap);
     P->S3.ensure(nh * cap);
     P->G1.ensure((size_t)P->z.E * N * H * cap);
-    P->p_res.ensure((size_t)2 * P->nb_real * cap);
+    P->p_res.ensure((size_t)2 * P->nb_res * cap);
     P->p_n1.ensure((size_t)P->nb_spec * cap);
     P->p_n2.ensure((size_t)P->nb_spec * cap);
     P->p_wf.ensure((size_t)P->nb_spec * cap);
@@ -278,6 +280,7 @@ EvalGeom make_geom_public(const ettk_plan* P, const ettk_particle& d, const Mat4
     }
     g.full = w.full || !P->p.missing_wedge;
     g.particle = particle;
+    g.cut = d.cut_offset;
     return g;
 }
 
@@ -289,6 +292,14 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
     hipStream_t s = P->ctx->stream;
     const int CH = 128;
     ensure_ccc_workspace(P, std::min(n, CH));
+    auto tw0 = std::chrono::steady_clock::now();
+    struct WallRec {
+        KernelProf* kp;
+        std::chrono::steady_clock::time_point t0;
+        ~WallRec() {
+            if (kp) kp->add_host("ccc_batch_wall", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+    } wall_rec{P->ctx->fft.prof, tw0};
     std::vector<EvalGeom> g;
     for (int e0 = 0; e0 < n; e0 += P->cap) {
         int E = std::min(P->cap, n - e0);
@@ -298,27 +309,48 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
         ETTK_HIP(hipEventRecord(P->ev0, s));
         const float* W1 = P->p.missing_wedge ? P->W1.p : nullptr;
         int apply = P->dummy_filter ? 0 : 1;
-        // reference side: F1 = mcf(F1raw) * filter * W1 W2 -> r1 = c2r -> k1_m, r1 m1 -> F1m
-        launch_spec_filter(P->F1mcf.p, 0, P->S1.p, P->filter.p, W1, P->geom.p, E, N, 0, apply, P->p_n1.p, P->p_wf.p,
-                           P->nb_spec, s);
+        KernelProf* kp = P->ctx->fft.prof;
+        const double Rb = (double)E * N * N * N * sizeof(float), Hb = (double)E * nh * sizeof(float2);
+        const double Gb = (double)E * P->z.E * N * H * sizeof(float2);
+        // reference side: F1 = mcf(r2c(I1 maskfft shift)) * filter * W1 W2 -> r1 = c2r -> k1_m, r1 m1 -> F1m
+        {
+            ProfScope ps(kp, "k_spec_filter", 2 * Hb, s);
+            launch_spec_filter(P->F1mcf.p, 0, P->S1.p, P->filter.p, W1, P->geom.p, E, N, 0, apply, P->p_n1.p, P->p_wf.p,
+                               P->nb_spec, s);
+        }
         fft3_c2r(P->ctx->fft, P->S1.p, P->S1.p, P->R1.p, N, E, s);
-        launch_mask_ref(P->R1.p, P->m1.p, E, N, P->p_k1.p, P->nb_real, s);
+        {
+            ProfScope ps(kp, "k_mask_ref", 2 * Rb, s);
+            launch_mask_ref(P->R1.p, P->m1.p, E, N, P->p_k1.p, P->nb_real, s);
+        }
         R2CArgs a1{P->R1.p, P->S1.p, N, E, PRO_NONE, nullptr, nullptr};
         fft3_r2c(P->ctx->fft, a1, nullptr, s);
         // particle side
-        launch_reslice(d_parts, P->geom.p, E, N, P->R2.p, P->p_res.p, P->nb_real, s);
-        launch_reslice_mean(P->p_res.p, P->nb_real, E, P->stats.p, s);
+        {
+            ProfScope ps(kp, "k_reslice", 2 * Rb, s);
+            launch_reslice(d_parts, P->geom.p, E, N, P->R2.p, P->p_res.p, P->nb_res, s);
+        }
+        launch_reslice_mean(P->p_res.p, P->nb_res, E, P->stats.p, s);
         R2CArgs a2{P->R2.p, P->S2.p, N, E, PRO_RESLICED, P->maskfft.p, P->stats.p};
         fft3_r2c(P->ctx->fft, a2, nullptr, s);
-        launch_spec_filter(P->S2.p, (int)nh, P->S2.p, P->filter.p, W1, P->geom.p, E, N, P->p.mcf, apply, P->p_n2.p,
-                           nullptr, P->nb_spec, s);
+        {
+            ProfScope ps(kp, "k_spec_filter", 2 * Hb, s);
+            launch_spec_filter(P->S2.p, (int)nh, P->S2.p, P->filter.p, W1, P->geom.p, E, N, P->p.mcf, apply, P->p_n2.p,
+                               nullptr, P->nb_spec, s);
+        }
         fft3_c2r(P->ctx->fft, P->S2.p, P->S3.p, P->R2.p, N, E, s);
         R2CArgs a3{P->R2.p, P->S3.p, N, E, PRO_SQUARE_SHIFT, nullptr, nullptr};
         fft3_r2c(P->ctx->fft, a3, nullptr, s);
         // combine, upsampled correlation on the search window, peak
         launch_ccc_scalars(P->p_n1.p, P->p_n2.p, P->p_k1.p, P->p_wf.p, P->nb_spec, P->nb_real, E, N, P->scalars.p, s);
-        launch_combine_dftA(P->S1.p, P->S2.p, P->S3.p, P->Mfft.p, P->scalars.p, P->tabA.p, P->z, E, P->G1.p, s);
-        launch_dftBC_argmax(P->G1.p, P->tabA.p, P->tabK.p, P->z, E, P->bval.p, P->bidx.p, s);
+        {
+            ProfScope ps(kp, "k_combine_dftA", 3 * Hb + Gb, s);
+            launch_combine_dftA(P->S1.p, P->S2.p, P->S3.p, P->Mfft.p, P->scalars.p, P->tabA.p, P->z, E, P->G1.p, s);
+        }
+        {
+            ProfScope ps(kp, "k_dftBC_argmax", Gb, s);
+            launch_dftBC_argmax(P->G1.p, P->tabA.p, P->tabK.p, P->z, E, P->bval.p, P->bidx.p, s);
+        }
         launch_peak_final(P->bval.p, P->bidx.p, P->scalars.p, (float)P->sum_filter,
                           (P->p.overlap_norm && P->p.missing_wedge) ? 1 : 0, P->z, E, P->out4.p, s);
         ETTK_HIP(hipEventRecord(P->ev1, s));
@@ -379,6 +411,11 @@ int ettk_ctx_destroy(ettk_ctx* ctx) {
     if (ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         ctx->fft.tw.release();
+        if (ctx->fft.prof) {
+            ctx->fft.prof->reset();
+            for (hipEvent_t e : ctx->fft.prof->pool) (void)hipEventDestroy(e);
+            delete ctx->fft.prof;
+        }
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
     }
@@ -443,11 +480,21 @@ int ettk_reference_set(ettk_plan* P, const float* d_ref, const ettk_particle* re
     Mat4 fin = euler_of(*ref_desc) * Mat4::from(ref_desc->matrix);
     wedge_normals(ref_desc->lwedge, ref_desc->uwedge, mat_inverse(fin), w);
     launch_wedge_ref(w, N, P->W1.p, s);
-    // F1raw = r2c(I1 * maskfft * shift), MCF applied once (executeHalf order: MCF, then filter x wedge)
+    P->wref = w;
+    // F1 = r2c(I1 * maskfft * shift), MCF applied once (executeHalf order: MCF, then filter x wedge)
     P->F1mcf.ensure(nh);
     R2CArgs a{P->ref.p, P->F1mcf.p, N, 1, PRO_MASK_SHIFT, P->maskfft.p, nullptr};
     fft3_r2c(P->ctx->fft, a, nullptr, s);
     if (P->p.mcf) launch_mcf_inplace(P->F1mcf.p, (long)nh, s);
+    // continuous refinement: I1 - mean(I1), x fastest (set-up path: one host round trip)
+    std::vector<float> h(n3);
+    ETTK_HIP(hipMemcpyAsync(h.data(), d_ref, n3 * sizeof(float), hipMemcpyDeviceToHost, s));
+    ETTK_HIP(hipStreamSynchronize(s));
+    double mean = 0;
+    for (float v : h) mean += v;
+    mean /= (double)n3;
+    for (float& v : h) v = (float)((double)v - mean);
+    P->ref_x.upload(h.data(), n3, s);
     ETTK_HIP(hipStreamSynchronize(s));
     P->have_ref = true;
     so3_reference_set(P);
@@ -470,6 +517,37 @@ int ettk_ccc_eval_batch(ettk_plan* P, const float* d_particles, int n_particles,
     P->timing = Timing{};
     ccc_batch(P, d_particles, descs, n_evals, particle_index, t.data(), out_distance);
     for (int e = 0; e < n_evals; e++) std::memcpy(out_transforms + (size_t)e * 16, t[e].a, sizeof(t[e].a));
+    ETTK_API_END
+}
+
+int ettk_profile(ettk_ctx* ctx, int enable) {
+    ETTK_API_BEGIN
+    ETTK_REQUIRE(ctx, ETTK_ERR_ARG, "ettk_profile: null context");
+    ETTK_HIP(hipStreamSynchronize(ctx->stream));
+    if (!ctx->fft.prof) ctx->fft.prof = new KernelProf();
+    ctx->fft.prof->reset();
+    ctx->fft.prof->on = enable != 0;
+    ETTK_API_END
+}
+
+int ettk_profile_query(ettk_ctx* ctx, const char* kernel, double* total_ms, long long* launches, double* bytes) {
+    ETTK_API_BEGIN
+    ETTK_REQUIRE(ctx && kernel, ETTK_ERR_ARG, "ettk_profile_query: null argument");
+    ETTK_HIP(hipStreamSynchronize(ctx->stream));
+    double ms = 0, by = 0;
+    long long nl = 0;
+    if (ctx->fft.prof) {
+        ctx->fft.prof->collect();
+        for (auto& r : ctx->fft.prof->recs)
+            if (r.first == kernel) {
+                ms = r.second.ms;
+                nl = r.second.launches;
+                by = r.second.bytes;
+            }
+    }
+    if (total_ms) *total_ms = ms;
+    if (launches) *launches = nl;
+    if (bytes) *bytes = by;
     ETTK_API_END
 }
 

@@ -11,6 +11,7 @@ struct EvalGeom {
     float v2[3];
     int full;         // particle wedge spans >= 180 degrees (W2 == 1)
     int particle;     // index of the particle volume
+    float cut;        // cut offset of the sub-volume along z (nbfWedgedSubImage3D::setCutOffset)
 };
 
 // smoothed analytic wedge parameters of the reference
@@ -26,7 +27,9 @@ struct CccSizes {
     int ub2;              // (E-1)/2
 };
 
-// reslice (tricubic Keys a=-1/2, clamp border, -FLT_MAX outside): out [e][i][j][k]
+// reslice (tricubic Keys a=-1/2, clamp border, -FLT_MAX outside): out [e][i][j][k]; `nb` must equal
+// reslice_blocks(N) (one workgroup per 8^3 output tile), partials hold 2 * nb floats per evaluation
+int reslice_blocks(int N);
 void launch_reslice(const float* parts, const EvalGeom* geom, int n_eval, int N, float* out, float* partials,
                     int nb, hipStream_t s);
 // mean of the valid voxels per evaluation, from the reslice partials (nb per eval): stats[e] = mean
@@ -48,6 +51,25 @@ void launch_combine_dftA(const float2* F1m, const float2* F2, const float2* Q, c
 // G2 = sum_j A[y][j] G1, T = sum_k Re(G2 K[z][k]) on the crop, spherical mask, first argmax per (e, x)
 void launch_dftBC_argmax(const float2* G1, const float2* tabA, const float2* tabK, const CccSizes& z, int n_eval,
                          float* best_val, int* best_idx, hipStream_t s);
+// continuous CCC and adjoint in the particle frame (refine.hip)
+struct RefineGeom {
+    float G[12];          // q = G[r*4 + 0..2] (x, y, z) - N/2 + G[r*4 + 3]: reference index of particle voxel (x,y,z)
+    float n1[3], n2[3];   // particle wedge normals, lab frame
+    float u1[3], u2[3];   // reference wedge normals in the particle frame at the seed pose
+    int pw, rw;           // wedge tests on (particle, reference)
+    int spec;             // slot of the particle's spectrum
+    int pad;
+};
+int refine_moment_blocks(int N);
+int refine_dot_blocks(int N);
+void launch_vol_mean(const float* vols, int count, int N, float* mean, hipStream_t s);
+void launch_reslice_ref(const float* ref, const RefineGeom* geom, int E, int N, float* out, hipStream_t s);
+void launch_refine_dot(const float2* X1, const float2* FP, const float* filter, const RefineGeom* geom, int E, int N,
+                       double* part, double* sc, hipStream_t s);
+void launch_refine_adj(float2* X1, const float2* FP, const float* filter, const RefineGeom* geom, int E, int N,
+                       const double* sc, hipStream_t s);
+void launch_refine_moments(const float* ref, const RefineGeom* geom, const float* G, const float* maskfft, int E, int N,
+                           float* mpart, double* mom, hipStream_t s);
 void launch_wedge_ref(const WedgeRef& w, int N, float* W, hipStream_t s);
 void launch_xfast_to_internal(const float* in, float* out, int N, hipStream_t s);
 void launch_internal_to_xfast(const float* in, float* out, int N, hipStream_t s);

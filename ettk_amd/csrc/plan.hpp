@@ -70,7 +70,7 @@ struct ettk_plan {
     ettk::DevBuf<float> ref;
     // CCC workspace
     int cap = 0;
-    int nb_real = 0, nb_spec = 0;
+    int nb_real = 0, nb_spec = 0, nb_res = 0;
     ettk::DevBuf<float> R1, R2, p_res, p_n1, p_n2, p_k1, p_wf, stats, scalars, bval, out4;
     ettk::DevBuf<int> bidx;
     ettk::DevBuf<float2> S1, S2, S3, G1;
@@ -81,7 +81,15 @@ struct ettk_plan {
     // S^2 / SO(3) state (so3.hip)
     ettk::So3State* so3 = nullptr;
     // averaging accumulators
-    ettk::DevBuf<float> avg_re, avg_im, avg_acc;
+    ettk::DevBuf<double> avg_re, avg_im, avg_acc;
+    // continuous refinement (refine.cpp): reference minus its mean (x fastest), its wedge normals, particle
+    // spectra and per-evaluation workspace
+    ettk::DevBuf<float> ref_x;
+    ettk::WedgeRef wref{};
+    ettk::DevBuf<float2> r_fp;
+    ettk::DevBuf<float> r_mean, r_mpart;
+    ettk::DevBuf<double> r_pdot, r_sc, r_mom;
+    ettk::DevBuf<ettk::RefineGeom> r_geom;
 };
 
 namespace ettk {

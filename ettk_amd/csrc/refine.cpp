@@ -1,0 +1,290 @@
+// refine.cpp -- batched evaluation of the continuous CCC and its gradient in the particle frame, and the
+// lock-step gradient ascent that replaces Powell in the batched + adjoint mode.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+#include "plan.hpp"
+
+namespace ettk {
+
+// pose of the refinement parameters: t(x) = [R0 Rx(x0) Ry(x1) Rz(x2) | tau0 + (x3, x4, x5)], angles in degrees
+static Mat4 pose(const Mat4& seed, const double* x) {
+    Mat4 R = strip_translation(seed);
+    Mat4 t = R * mat_rot(x[0], 1, 0, 0) * mat_rot(x[1], 0, 1, 0) * mat_rot(x[2], 0, 0, 1);
+    t(0, 3) = seed(0, 3) + x[3];
+    t(1, 3) = seed(1, 3) + x[4];
+    t(2, 3) = seed(2, 3) + x[5];
+    return t;
+}
+// derivative of the rotation about axis `ax` by `deg` degrees, per degree
+static Mat4 drot(int ax, double deg) {
+    double a = deg * M_PI / 180.0, c = std::cos(a), s = std::sin(a), k = M_PI / 180.0;
+    Mat4 D;
+    for (double& v : D.a) v = 0.0;
+    int i = (ax + 1) % 3, j = (ax + 2) % 3;  // rotation in the (i, j) plane
+    D(i, i) = -s * k;
+    D(i, j) = -c * k;
+    D(j, i) = c * k;
+    D(j, j) = -s * k;
+    return D;
+}
+// lab-frame wedge normals (nbfWedge3D::set, Wedge.h:143-184); false when nothing is missing
+static bool lab_normals(float lw, float uw, double n1[3], double n2[3]) {
+    double l = lw < -90 ? -90.0 : lw, u = uw > 90 ? 90.0 : uw;
+    if (u - l >= 180) return false;
+    n1[0] = std::cos(-(90.0 - u) * M_PI / 180.0);
+    n1[1] = 0.0;
+    n1[2] = -std::sin(-(90.0 - u) * M_PI / 180.0);
+    n2[0] = std::cos(-(90.0 - l) * M_PI / 180.0);
+    n2[1] = 0.0;
+    n2[2] = -std::sin(-(90.0 - l) * M_PI / 180.0);
+    return true;
+}
+
+// Spectra r2c((I2 - mean I2) maskfft shift) of particles [p0, p1) into P->r_fp (slot = particle - p0).
+static void particle_spectra(ettk_plan* P, const float* d_parts, int p0, int p1) {
+    const int N = P->N, H = P->H;
+    const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * H;
+    const int cnt = p1 - p0;
+    hipStream_t s = P->ctx->stream;
+    P->r_fp.ensure((size_t)cnt * nh);
+    P->r_mean.ensure(cnt);
+    launch_vol_mean(d_parts + (size_t)p0 * n3, cnt, N, P->r_mean.p, s);
+    const int CH = 64;
+    for (int c0 = 0; c0 < cnt; c0 += CH) {
+        int c = std::min(CH, cnt - c0);
+        R2CArgs a{d_parts + (size_t)(p0 + c0) * n3, P->r_fp.p + (size_t)c0 * nh, N, c, PRO_RESLICED, P->maskfft.p,
+                  P->r_mean.p + c0};
+        fft3_r2c(P->ctx->fft, a, nullptr, s);
+    }
+}
+
+// Score and gradient for n evaluations (particle pidx[q], seed seeds[q], parameters x6[q]).  The particle
+// spectra of [p0, p1) must be in P->r_fp.
+static void refine_eval_batch(ettk_plan* P, const ettk_particle* descs, int n, const int* pidx, int p0,
+                              const Mat4* seeds, const double* x6, double* S, double* grad6) {
+    const int N = P->N, H = P->H;
+    const long nh = (long)N * N * H;
+    hipStream_t s = P->ctx->stream;
+    ensure_ccc_workspace(P, std::min(n, 128));
+    const int cap = P->cap;
+    const int nbd = refine_dot_blocks(N), nbm = refine_moment_blocks(N);
+    P->r_pdot.ensure((size_t)3 * nbd * cap);
+    P->r_sc.ensure((size_t)4 * cap);
+    P->r_mpart.ensure((size_t)12 * nbm * cap);
+    P->r_mom.ensure((size_t)12 * cap);
+    P->r_geom.ensure(cap);
+    std::vector<RefineGeom> g(cap);
+    std::vector<Mat4> Ginv(cap), EMs(cap);
+    std::vector<double> sc((size_t)4 * cap), mom((size_t)12 * cap);
+    KernelProf* kp = P->ctx->fft.prof;
+    for (int e0 = 0; e0 < n; e0 += cap) {
+        const int E = std::min(cap, n - e0);
+        for (int e = 0; e < E; e++) {
+            const int q = e0 + e;
+            const ettk_particle& d = descs[pidx[q]];
+            const Mat4 EM = euler_of(d) * Mat4::from(d.matrix);
+            Mat4 fin = EM * pose(seeds[q], x6 + (size_t)q * 6);
+            if (mat_det(fin) == 0.0) fin = Mat4::ident();
+            const Mat4 Gi = mat_inverse(fin);
+            EMs[e] = EM;
+            Ginv[e] = Gi;
+            RefineGeom& r = g[e];
+            r = RefineGeom{};
+            const double c[3] = {N / 2.0, N / 2.0, N / 2.0 + d.cut_offset};
+            for (int a = 0; a < 3; a++) {
+                for (int b = 0; b < 3; b++) r.G[a * 4 + b] = (float)Gi(a, b);
+                r.G[a * 4 + 3] = (float)(Gi(a, 3) + c[a]);
+            }
+            double n1[3], n2[3];
+            r.pw = (P->p.missing_wedge && lab_normals(d.lwedge, d.uwedge, n1, n2)) ? 1 : 0;
+            if (r.pw)
+                for (int a = 0; a < 3; a++) {
+                    r.n1[a] = (float)n1[a];
+                    r.n2[a] = (float)n2[a];
+                }
+            r.rw = (P->p.missing_wedge && !P->wref.full) ? 1 : 0;
+            if (r.rw) {
+                // reference wedge normals (reference frame) into the particle frame at the seed: u = A^T v
+                Mat4 fs = EM * seeds[q];
+                if (mat_det(fs) == 0.0) fs = Mat4::ident();
+                const Mat4 A = mat_inverse(fs);
+                for (int cc = 0; cc < 3; cc++) {
+                    double a1 = 0, a2 = 0;
+                    for (int rr = 0; rr < 3; rr++) {
+                        a1 += A(rr, cc) * P->wref.v1[rr];
+                        a2 += A(rr, cc) * P->wref.v2[rr];
+                    }
+                    r.u1[cc] = (float)a1;
+                    r.u2[cc] = (float)a2;
+                }
+            }
+            r.spec = pidx[q] - p0;
+        }
+        ETTK_HIP(hipMemcpyAsync(P->r_geom.p, g.data(), E * sizeof(RefineGeom), hipMemcpyHostToDevice, s));
+        const double Rb = (double)E * N * N * N * sizeof(float), Hb = (double)E * nh * sizeof(float2);
+        {
+            ProfScope ps(kp, "k_reslice_ref", Rb, s);
+            launch_reslice_ref(P->ref_x.p, P->r_geom.p, E, N, P->R2.p, s);
+        }
+        R2CArgs a{P->R2.p, P->S2.p, N, E, PRO_MASK_SHIFT, P->maskfft.p, nullptr};
+        fft3_r2c(P->ctx->fft, a, nullptr, s);
+        {
+            ProfScope ps(kp, "k_refine_dot", 2 * Hb, s);
+            launch_refine_dot(P->S2.p, P->r_fp.p, P->filter.p, P->r_geom.p, E, N, P->r_pdot.p, P->r_sc.p, s);
+        }
+        {
+            ProfScope ps(kp, "k_refine_adj", 3 * Hb, s);
+            launch_refine_adj(P->S2.p, P->r_fp.p, P->filter.p, P->r_geom.p, E, N, P->r_sc.p, s);
+        }
+        fft3_c2r(P->ctx->fft, P->S2.p, P->S3.p, P->R1.p, N, E, s);
+        {
+            ProfScope ps(kp, "k_refine_moments", Rb, s);
+            launch_refine_moments(P->ref_x.p, P->r_geom.p, P->R1.p, P->maskfft.p, E, N, P->r_mpart.p, P->r_mom.p, s);
+        }
+        ETTK_HIP(hipMemcpyAsync(sc.data(), P->r_sc.p, (size_t)4 * E * sizeof(double), hipMemcpyDeviceToHost, s));
+        ETTK_HIP(hipMemcpyAsync(mom.data(), P->r_mom.p, (size_t)12 * E * sizeof(double), hipMemcpyDeviceToHost, s));
+        ETTK_HIP(hipStreamSynchronize(s));
+        for (int e = 0; e < E; e++) {
+            const int q = e0 + e;
+            const double* x = x6 + (size_t)q * 6;
+            S[q] = sc[(size_t)e * 4];
+            const Mat4 R0 = strip_translation(seeds[q]);
+            const Mat4 Rx = mat_rot(x[0], 1, 0, 0), Ry = mat_rot(x[1], 0, 1, 0), Rz = mat_rot(x[2], 0, 0, 1);
+            Mat4 dt[6];
+            dt[0] = R0 * drot(0, x[0]) * Ry * Rz;
+            dt[1] = R0 * Rx * drot(1, x[1]) * Rz;
+            dt[2] = R0 * Rx * Ry * drot(2, x[2]);
+            for (int r = 0; r < 3; r++) {
+                dt[r](3, 3) = 0.0;
+                dt[r](0, 3) = dt[r](1, 3) = dt[r](2, 3) = 0.0;
+            }
+            for (int r = 3; r < 6; r++) {
+                for (double& v : dt[r].a) v = 0.0;
+                dt[r](r - 3, 3) = 1.0;
+            }
+            for (int r = 0; r < 6; r++) {
+                // d(fin^-1) = -fin^-1 dfin fin^-1
+                const Mat4 dG = Ginv[e] * (EMs[e] * dt[r]) * Ginv[e];
+                double acc = 0;
+                for (int aa = 0; aa < 3; aa++)
+                    for (int b = 0; b < 4; b++) acc -= dG(aa, b) * mom[(size_t)e * 12 + aa * 4 + b];
+                grad6[(size_t)q * 6 + r] = acc;
+            }
+        }
+    }
+}
+
+// Lock-step gradient ascent for every (particle, seed) job, n_iter evaluations per job: a step of s voxels
+// (start 0.5) along the gradient in the metric where one degree counts as c = (N/4) pi/180 voxels of
+// displacement; an improving step is accepted and s grows by 1.25, otherwise s halves.
+static void refine_jobs(ettk_plan* P, const ettk_particle* descs, int n, const int* pidx, int p0, const Mat4* seeds,
+                        int n_iter, Mat4* out_pose, double* out_score) {
+    const double c = P->N / 4.0 * M_PI / 180.0;
+    std::vector<double> x((size_t)6 * n, 0.0), xt((size_t)6 * n, 0.0), g((size_t)6 * n), gt((size_t)6 * n);
+    std::vector<double> Sc(n), St(n), step(n, 0.5);
+    refine_eval_batch(P, descs, n, pidx, p0, seeds, x.data(), Sc.data(), g.data());
+    for (int it = 1; it < n_iter; it++) {
+        for (int q = 0; q < n; q++) {
+            double dir[6], nrm = 0;
+            for (int r = 0; r < 6; r++) {
+                dir[r] = r < 3 ? g[q * 6 + r] / (c * c) : g[q * 6 + r];
+                nrm += r < 3 ? c * c * dir[r] * dir[r] : dir[r] * dir[r];
+            }
+            nrm = std::sqrt(nrm);
+            for (int r = 0; r < 6; r++) xt[q * 6 + r] = x[q * 6 + r] + (nrm > 0 ? step[q] * dir[r] / nrm : 0.0);
+        }
+        refine_eval_batch(P, descs, n, pidx, p0, seeds, xt.data(), St.data(), gt.data());
+        for (int q = 0; q < n; q++) {
+            if (St[q] > Sc[q]) {
+                Sc[q] = St[q];
+                for (int r = 0; r < 6; r++) {
+                    x[q * 6 + r] = xt[q * 6 + r];
+                    g[q * 6 + r] = gt[q * 6 + r];
+                }
+                step[q] *= 1.25;
+            } else {
+                step[q] *= 0.5;
+            }
+        }
+    }
+    for (int q = 0; q < n; q++) {
+        out_pose[q] = pose(seeds[q], x.data() + (size_t)q * 6);
+        out_score[q] = Sc[q];
+    }
+}
+
+// Range of particles referenced by a batch of jobs
+static void particle_range(int n, const int* pidx, int& p0, int& p1) {
+    p0 = 0;
+    p1 = 0;
+    if (n == 0) return;
+    p0 = *std::min_element(pidx, pidx + n);
+    p1 = *std::max_element(pidx, pidx + n) + 1;
+}
+
+void refine_with_seeds(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
+                       const Mat4* seeds, int n_iter, Mat4* out_pose, double* out_score) {
+    ETTK_REQUIRE(P->have_ref, ETTK_ERR_STATE, "reference not set");
+    if (n == 0) return;
+    int p0, p1;
+    particle_range(n, pidx, p0, p1);
+    particle_spectra(P, d_parts, p0, p1);
+    refine_jobs(P, descs, n, pidx, p0, seeds, n_iter, out_pose, out_score);
+}
+
+}  // namespace ettk
+
+using namespace ettk;
+
+extern "C" int ettk_refine_batch(ettk_plan* P, const float* d_particles, int n_particles, const ettk_particle* descs,
+                                 int n, const int* particle_index, const double* seeds, int n_iter, double* out_poses,
+                                 double* out_scores) {
+    try {
+        ETTK_REQUIRE(P && d_particles && descs && particle_index && seeds && out_poses && out_scores && n >= 0 &&
+                         n_iter >= 1,
+                     ETTK_ERR_ARG, "ettk_refine_batch: bad argument");
+        for (int e = 0; e < n; e++)
+            ETTK_REQUIRE(particle_index[e] >= 0 && particle_index[e] < n_particles, ETTK_ERR_ARG,
+                         "ettk_refine_batch: particle index out of range");
+        ETTK_HIP(hipSetDevice(P->ctx->device));
+        std::vector<Mat4> sd(n), ps(n);
+        for (int e = 0; e < n; e++) sd[e] = Mat4::from(seeds + (size_t)e * 16);
+        refine_with_seeds(P, d_particles, descs, n, particle_index, sd.data(), n_iter, ps.data(), out_scores);
+        for (int e = 0; e < n; e++) std::memcpy(out_poses + (size_t)e * 16, ps[e].a, sizeof(ps[e].a));
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    }
+    set_error("");
+    return ETTK_OK;
+}
+
+extern "C" int ettk_ccc_grad_batch(ettk_plan* P, const float* d_particles, int n_particles, const ettk_particle* descs,
+                                   int n_evals, const int* particle_index, const double* seeds, const double* params6,
+                                   double* out_value, double* out_grad) {
+    try {
+        ETTK_REQUIRE(P && d_particles && descs && particle_index && seeds && params6 && out_value && out_grad &&
+                         n_evals >= 0,
+                     ETTK_ERR_ARG, "ettk_ccc_grad_batch: bad argument");
+        for (int e = 0; e < n_evals; e++)
+            ETTK_REQUIRE(particle_index[e] >= 0 && particle_index[e] < n_particles, ETTK_ERR_ARG,
+                         "ettk_ccc_grad_batch: particle index out of range");
+        ETTK_REQUIRE(P->have_ref, ETTK_ERR_STATE, "reference not set");
+        ETTK_HIP(hipSetDevice(P->ctx->device));
+        if (n_evals > 0) {
+            std::vector<Mat4> sd(n_evals);
+            for (int e = 0; e < n_evals; e++) sd[e] = Mat4::from(seeds + (size_t)e * 16);
+            int p0, p1;
+            particle_range(n_evals, particle_index, p0, p1);
+            particle_spectra(P, d_particles, p0, p1);
+            refine_eval_batch(P, descs, n_evals, particle_index, p0, sd.data(), params6, out_value, out_grad);
+        }
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    }
+    set_error("");
+    return ETTK_OK;
+}

@@ -1,0 +1,286 @@
+// refine.hip -- continuous constrained CCC and its adjoint for local pose refinement (gfx950).
+//
+// No reference counterpart: this replaces the Powell line searches of nbfFourierImageMetric::refine
+// (inc/em/nbfFourierImageMetric.h:1702-1809), whose every function value is a full constrained CCC on a
+// half-voxel shift grid, by gradient ascent on a differentiable score of the six pose parameters.
+//
+// The score lives in the particle's own frame, where the particle's missing wedge is fixed:
+//   X2 = H r2c((I2 - mean I2) maskfft (-1)^(x+y+z))               particle, fixed: spectra computed once
+//   X1 = H r2c(I1'(theta) maskfft (-1)^(x+y+z))                    reference resliced into the particle frame
+//   I1'(p) = (I1 - mean I1)(q(p)),  q = G' (p - N/2, 1),  G' = fin(theta)^-1 plus (N/2, N/2, N/2 + cut)
+//            (trilinear with clamped taps: the reference is extended by its border values, so that I1' and S
+//            are continuous in theta)
+//   S  = <X1, X2> / (|X1| |X2|),  <a, b> = sum_h c_h Re(a_h conj(b_h))  (c_h: exact Hermitian weights)
+// with H = band-pass x binary particle wedge (lab frame) x binary reference wedge (rotated by the seed).
+// Volumes stay in the particle's memory order [z][y][x]; the half axis of every spectrum is x.
+// Adjoint: dS = <Z, dX1> with Z = X2/(n1 n2) - S X1/n1^2, so dS/dI1'(p) = maskfft (-1)^(x+y+z) c2r(H Z)(p),
+// and through the interpolator dS/dtheta_r =
+// sum_ab dG'_r[a][b] Q[a][b], Q[a][b] = sum_p gamma(p) dI1/dq_a(q(p)) w_b(p), w = (x-N/2, y-N/2, z-N/2, 1):
+// twelve moments per evaluation, contracted with dG'_r = -G dfin_r G on the host in FP64.
+#include <cfloat>
+
+#include "kernels.hpp"
+
+namespace ettk {
+
+static constexpr int kT = 256;
+
+__device__ __forceinline__ float wsum64(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double dsum64(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+// trilinear sample of an x-fastest volume with clamped taps, and its gradient in (x, y, z)
+__device__ __forceinline__ float trilin(const float* __restrict__ vol, int N, float px, float py, float pz, float* gr) {
+    float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+    float ax = px - fx, ay = py - fy, az = pz - fz;
+    int x0 = min(max((int)fx, 0), N - 1), x1 = min(max((int)fx + 1, 0), N - 1);
+    int y0 = min(max((int)fy, 0), N - 1), y1 = min(max((int)fy + 1, 0), N - 1);
+    int z0 = min(max((int)fz, 0), N - 1), z1 = min(max((int)fz + 1, 0), N - 1);
+    float v000 = vol[((size_t)z0 * N + y0) * N + x0], v100 = vol[((size_t)z0 * N + y0) * N + x1];
+    float v010 = vol[((size_t)z0 * N + y1) * N + x0], v110 = vol[((size_t)z0 * N + y1) * N + x1];
+    float v001 = vol[((size_t)z1 * N + y0) * N + x0], v101 = vol[((size_t)z1 * N + y0) * N + x1];
+    float v011 = vol[((size_t)z1 * N + y1) * N + x0], v111 = vol[((size_t)z1 * N + y1) * N + x1];
+    float c00 = v000 + ax * (v100 - v000), c10 = v010 + ax * (v110 - v010);
+    float c01 = v001 + ax * (v101 - v001), c11 = v011 + ax * (v111 - v011);
+    float c0 = c00 + ay * (c10 - c00), c1 = c01 + ay * (c11 - c01);
+    if (gr) {
+        float dx00 = v100 - v000, dx10 = v110 - v010, dx01 = v101 - v001, dx11 = v111 - v011;
+        float dx0 = dx00 + ay * (dx10 - dx00), dx1 = dx01 + ay * (dx11 - dx01);
+        gr[0] = dx0 + az * (dx1 - dx0);
+        float dy0 = c10 - c00, dy1 = c11 - c01;
+        gr[1] = dy0 + az * (dy1 - dy0);
+        gr[2] = c1 - c0;
+    }
+    return c0 + az * (c1 - c0);
+}
+
+// sampling position in the reference of voxel (x, y, z) of the particle frame
+__device__ __forceinline__ void ref_pos(const RefineGeom& g, int N, int x, int y, int z, float* q) {
+    const float h = 0.5f * (float)N;
+    const float wx = (float)x - h, wy = (float)y - h, wz = (float)z - h;
+    q[0] = g.G[0] * wx + g.G[1] * wy + g.G[2] * wz + g.G[3];
+    q[1] = g.G[4] * wx + g.G[5] * wy + g.G[6] * wz + g.G[7];
+    q[2] = g.G[8] * wx + g.G[9] * wy + g.G[10] * wz + g.G[11];
+}
+
+// H at half-spectrum element (a, b, c) = (z, y, x) of the particle frame: filter x binary wedges
+__device__ __forceinline__ float hweight(const float* __restrict__ filter, const RefineGeom& g, long h, int a, int b,
+                                         int c, int N, int H) {
+    float w = filter[h];  // radial table: identical in any axis order
+    const float kx = (float)(c - (H - 1)), ky = (float)(b - N / 2), kz = (float)(a - N / 2);
+    if (g.pw) {
+        float s1 = g.n1[0] * kx + g.n1[1] * ky + g.n1[2] * kz, s2 = g.n2[0] * kx + g.n2[1] * ky + g.n2[2] * kz;
+        if (s1 * s2 > 0.f) w = 0.f;
+    }
+    if (g.rw) {
+        float s1 = g.u1[0] * kx + g.u1[1] * ky + g.u1[2] * kz, s2 = g.u2[0] * kx + g.u2[1] * ky + g.u2[2] * kz;
+        if (s1 * s2 > 0.f) w = 0.f;
+    }
+    return w;
+}
+
+// mean of each particle volume (FP64 accumulation): one workgroup per particle
+__global__ void __launch_bounds__(kT) k_vol_mean(const float* __restrict__ vols, long n3, float* __restrict__ mean) {
+    __shared__ double sh[kT / 64];
+    const float* v = vols + (size_t)blockIdx.x * n3;
+    double s = 0.0;
+    for (long o = threadIdx.x; o < n3; o += kT) s += v[o];
+    s = dsum64(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kT / 64; w++) t += sh[w];
+        mean[blockIdx.x] = (float)(t / (double)n3);
+    }
+}
+
+// I1'(p) for every evaluation: out[e][z][y][x]; `ref` holds I1 - mean I1, x fastest
+__global__ void __launch_bounds__(kT) k_reslice_ref(const float* __restrict__ ref, const RefineGeom* __restrict__ geom,
+                                                     int N, float* __restrict__ out) {
+    const int e = blockIdx.y;
+    const RefineGeom g = geom[e];
+    const long n3 = (long)N * N * N;
+    for (long o = (long)blockIdx.x * kT + threadIdx.x; o < n3; o += (long)gridDim.x * kT) {
+        int x = (int)(o % N), y = (int)((o / N) % N), z = (int)(o / ((long)N * N));
+        float q[3];
+        ref_pos(g, N, x, y, z, q);
+        out[(size_t)e * n3 + o] = trilin(ref, N, q[0], q[1], q[2], nullptr);
+    }
+}
+
+// per-block FP64 partials of <H X1, H X2>, |H X1|^2, |H X2|^2 (Hermitian weights): X1 = S1[e], X2 = FP[spec]
+__global__ void __launch_bounds__(kT) k_refine_dot(const float2* __restrict__ X1, const float2* __restrict__ FP,
+                                                    const float* __restrict__ filter,
+                                                    const RefineGeom* __restrict__ geom, int N,
+                                                    double* __restrict__ part) {
+    __shared__ double sh[3][kT / 64];
+    const int e = blockIdx.y, H = N / 2 + 1;
+    const long nh = (long)N * N * H;
+    const RefineGeom g = geom[e];
+    const float2* x1p = X1 + (size_t)e * nh;
+    const float2* x2p = FP + (size_t)g.spec * nh;
+    double sd = 0.0, s1 = 0.0, s2 = 0.0;
+    for (long h = (long)blockIdx.x * kT + threadIdx.x; h < nh; h += (long)gridDim.x * kT) {
+        const int c = (int)(h % H), b = (int)((h / H) % N), a = (int)(h / ((long)N * H));
+        const float w = hweight(filter, g, h, a, b, c, N, H);
+        const float2 u = x1p[h], v = x2p[h];
+        const double ww = (c == 0 || c == H - 1) ? (double)w * w : 2.0 * (double)w * w;
+        sd += ww * ((double)u.x * v.x + (double)u.y * v.y);
+        s1 += ww * ((double)u.x * u.x + (double)u.y * u.y);
+        s2 += ww * ((double)v.x * v.x + (double)v.y * v.y);
+    }
+    sd = dsum64(sd);
+    s1 = dsum64(s1);
+    s2 = dsum64(s2);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][w] = sd;
+        sh[1][w] = s1;
+        sh[2][w] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0.0;
+        for (int i = 0; i < kT / 64; i++) t += sh[threadIdx.x][i];
+        part[((size_t)e * gridDim.x + blockIdx.x) * 3 + threadIdx.x] = t;
+    }
+}
+
+// sc[e] = {S, n1, n2} in FP64
+__global__ void k_refine_scalars(const double* __restrict__ part, int nb, int n_eval, double* __restrict__ sc) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_eval) return;
+    double d = 0, q1 = 0, q2 = 0;
+    for (int b = 0; b < nb; b++) {
+        d += part[((size_t)e * nb + b) * 3];
+        q1 += part[((size_t)e * nb + b) * 3 + 1];
+        q2 += part[((size_t)e * nb + b) * 3 + 2];
+    }
+    const double n1 = sqrt(q1), n2 = sqrt(q2);
+    sc[e * 4 + 0] = (n1 > 0 && n2 > 0) ? d / (n1 * n2) : 0.0;
+    sc[e * 4 + 1] = n1;
+    sc[e * 4 + 2] = n2;
+}
+
+// adjoint spectrum in place: X1 <- H (H X2 / (n1 n2) - S H X1 / n1^2)
+__global__ void __launch_bounds__(kT) k_refine_adj(float2* __restrict__ X1, const float2* __restrict__ FP,
+                                                    const float* __restrict__ filter,
+                                                    const RefineGeom* __restrict__ geom, int N,
+                                                    const double* __restrict__ sc) {
+    const int e = blockIdx.y, H = N / 2 + 1;
+    const long nh = (long)N * N * H;
+    const RefineGeom g = geom[e];
+    const double S = sc[e * 4], n1 = sc[e * 4 + 1], n2 = sc[e * 4 + 2];
+    const float ca = (n1 > 0 && n2 > 0) ? (float)(1.0 / (n1 * n2)) : 0.f;
+    const float cb = n1 > 0 ? (float)(S / (n1 * n1)) : 0.f;
+    float2* x1p = X1 + (size_t)e * nh;
+    const float2* x2p = FP + (size_t)g.spec * nh;
+    for (long h = (long)blockIdx.x * kT + threadIdx.x; h < nh; h += (long)gridDim.x * kT) {
+        const int c = (int)(h % H), b = (int)((h / H) % N), a = (int)(h / ((long)N * H));
+        const float w = hweight(filter, g, h, a, b, c, N, H);
+        const float2 u = x1p[h], v = x2p[h];
+        const float w2 = w * w;
+        x1p[h] = make_float2(w2 * (ca * v.x - cb * u.x), w2 * (ca * v.y - cb * u.y));
+    }
+}
+
+// twelve moments per evaluation: per-block partial sums of gamma(p) dI1/dq_a w_b(p) over `vpb` voxels
+__global__ void __launch_bounds__(kT) k_refine_moments(const float* __restrict__ ref, const RefineGeom* __restrict__ geom,
+                                                        const float* __restrict__ G, const float* __restrict__ maskfft,
+                                                        int N, int vpb, float* __restrict__ part) {
+    __shared__ float sh[12][kT / 64];
+    const int e = blockIdx.y;
+    const RefineGeom g = geom[e];
+    const long n3 = (long)N * N * N;
+    const float* Ge = G + (size_t)e * n3;
+    const float half = 0.5f * (float)N;
+    float q12[12];
+#pragma unroll
+    for (int r = 0; r < 12; r++) q12[r] = 0.f;
+    const long o0 = (long)blockIdx.x * vpb;
+    for (int v = threadIdx.x; v < vpb; v += kT) {
+        const long o = o0 + v;
+        const int x = (int)(o % N), y = (int)((o / N) % N), z = (int)(o / ((long)N * N));
+        float q[3];
+        ref_pos(g, N, x, y, z, q);
+        const float gam = maskfft[o] * (((x + y + z) & 1) ? -Ge[o] : Ge[o]);
+        float gr[3];
+        trilin(ref, N, q[0], q[1], q[2], gr);
+        const float w[4] = {(float)x - half, (float)y - half, (float)z - half, 1.f};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float ga = gam * gr[a];
+#pragma unroll
+            for (int b = 0; b < 4; b++) q12[a * 4 + b] += ga * w[b];
+        }
+    }
+    const int wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < 12; r++) {
+        const float t = wsum64(q12[r]);
+        if ((threadIdx.x & 63) == 0) sh[r][wv] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        float t = 0.f;
+        for (int i = 0; i < kT / 64; i++) t += sh[threadIdx.x][i];
+        part[((size_t)e * gridDim.x + blockIdx.x) * 12 + threadIdx.x] = t;
+    }
+}
+__global__ void k_refine_reduce_moments(const float* __restrict__ part, int nb, int n_eval, double* __restrict__ out) {
+    const int e = blockIdx.x, r = threadIdx.x;
+    if (e >= n_eval || r >= 12) return;
+    double acc = 0;
+    for (int b = 0; b < nb; b++) acc += part[((size_t)e * nb + b) * 12 + r];
+    out[e * 12 + r] = acc;
+}
+
+int refine_moment_blocks(int N) {
+    const long n3 = (long)N * N * N;
+    return (int)(n3 >= 4096 ? n3 / 4096 : 1);
+}
+int refine_dot_blocks(int N) {
+    const long nh = (long)N * N * (N / 2 + 1);
+    return (int)std::min<long>(64, (nh + 4 * kT - 1) / (4 * kT));
+}
+
+void launch_vol_mean(const float* vols, int count, int N, float* mean, hipStream_t s) {
+    k_vol_mean<<<count, kT, 0, s>>>(vols, (long)N * N * N, mean);
+    ETTK_HIP(hipGetLastError());
+}
+void launch_reslice_ref(const float* ref, const RefineGeom* geom, int E, int N, float* out, hipStream_t s) {
+    const long n3 = (long)N * N * N;
+    const int nb = (int)std::min<long>(1024, (n3 + 4 * kT - 1) / (4 * kT));
+    k_reslice_ref<<<dim3(nb, E), kT, 0, s>>>(ref, geom, N, out);
+    ETTK_HIP(hipGetLastError());
+}
+void launch_refine_dot(const float2* X1, const float2* FP, const float* filter, const RefineGeom* geom, int E, int N,
+                       double* part, double* sc, hipStream_t s) {
+    const int nb = refine_dot_blocks(N);
+    k_refine_dot<<<dim3(nb, E), kT, 0, s>>>(X1, FP, filter, geom, N, part);
+    k_refine_scalars<<<(E + 127) / 128, 128, 0, s>>>(part, nb, E, sc);
+    ETTK_HIP(hipGetLastError());
+}
+void launch_refine_adj(float2* X1, const float2* FP, const float* filter, const RefineGeom* geom, int E, int N,
+                       const double* sc, hipStream_t s) {
+    const long nh = (long)N * N * (N / 2 + 1);
+    const int nb = (int)std::min<long>(512, (nh + 4 * kT - 1) / (4 * kT));
+    k_refine_adj<<<dim3(nb, E), kT, 0, s>>>(X1, FP, filter, geom, N, sc);
+    ETTK_HIP(hipGetLastError());
+}
+void launch_refine_moments(const float* ref, const RefineGeom* geom, const float* G, const float* maskfft, int E, int N,
+                           float* mpart, double* mom, hipStream_t s) {
+    const long n3 = (long)N * N * N;
+    const int nb = refine_moment_blocks(N);
+    k_refine_moments<<<dim3(nb, E), kT, 0, s>>>(ref, geom, G, maskfft, N, (int)(n3 / nb), mpart);
+    k_refine_reduce_moments<<<E, 64, 0, s>>>(mpart, nb, E, mom);
+    ETTK_HIP(hipGetLastError());
+}
+
+}  // namespace ettk

@@ -637,7 +637,7 @@ void project_images(ettk_plan* P, const float* d_parts, const ettk_particle* des
     S.projF.ensure(nbig * CH);
     S.projMag.ensure(nbig * CH);
     S.projStats.ensure(4 * CH);
-    int nb_res = (int)((n3 + 1023) / 1024);
+    int nb_res = reslice_blocks(N);
     S.projPartials.ensure((size_t)2 * nb_res * CH);
     S.projGeom.ensure(CH);
     std::vector<EvalGeom> g(CH);

@@ -39,6 +39,9 @@ def load() -> C.CDLL:
             "ettk_avg_accumulate": ([vp, vp, C.c_int, C.POINTER(Particle), fp, dp], C.c_int),
             "ettk_avg_finalize": ([vp, vp], C.c_int),
             "ettk_last_timing": ([vp, dp, C.POINTER(C.c_longlong), dp], C.c_int),
+            "ettk_refine_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, ip, dp, C.c_int, dp, dp], C.c_int),
+            "ettk_profile": ([vp, C.c_int], C.c_int),
+            "ettk_profile_query": ([vp, C.c_char_p, dp, C.POINTER(C.c_longlong), dp], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(lib, name)
@@ -79,6 +82,17 @@ class Context:
         if self.h:
             check(self.lib.ettk_ctx_destroy(self.h))
             self.h = C.c_void_p()
+
+    def profile(self, enable: bool) -> None:
+        check(self.lib.ettk_profile(self.h, int(enable)))
+
+    def profile_query(self, kernel: str):
+        """(total ms, launches, algorithmic bytes) of one instrumented kernel since ettk_profile(ctx, 1)."""
+        ms = C.c_double()
+        n = C.c_longlong()
+        by = C.c_double()
+        check(self.lib.ettk_profile_query(self.h, kernel.encode(), C.byref(ms), C.byref(n), C.byref(by)))
+        return ms.value, n.value, by.value
 
 
 class Plan:
@@ -146,6 +160,16 @@ class Plan:
                                            particle_array(descs), n, _ip(pidx), _dp(seeds), _dp(x), _dp(val),
                                            _dp(grad)))
         return val, grad
+
+    def refine(self, d_parts, descs, pidx, seeds, n_iter: int = 20):
+        n = len(pidx)
+        pidx = np.ascontiguousarray(pidx, dtype=np.int32)
+        seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.float64).reshape(n, 16))
+        poses = np.zeros((n, 16), np.float64)
+        scores = np.zeros(n, np.float64)
+        check(self.lib.ettk_refine_batch(self.h, C.c_void_p(d_parts.data_ptr()), d_parts.shape[0], particle_array(descs),
+                                         n, _ip(pidx), _dp(seeds), int(n_iter), _dp(poses), _dp(scores)))
+        return poses.reshape(n, 4, 4), scores
 
     def average(self, d_parts, descs, weights, matrices, d_out):
         n = d_parts.shape[0]

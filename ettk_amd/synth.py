@@ -115,6 +115,65 @@ def make_particles(ref: np.ndarray, n: int, seed: int, half_wedge=(60.0, 60.0), 
     return ParticleSet(vols, descs, truth, snrs)
 
 
+def make_particles_device(ref: np.ndarray, n: int, seed: int, half_wedge=(45.0, 70.0), snr=(0.05, 0.2),
+                          max_shift: float = 4.0, chunk: int = 64):
+    """Same generator as make_particles (same rotations, shifts, wedges and SNRs from the same Philox
+    streams), evaluated on the GPU with torch: trilinear rotation by grid_sample, wedge by torch.fft, noise
+    from a seeded torch generator.  Returns (device tensor (n, N, N, N) float32, descs, truth)."""
+    import torch
+    import torch.nn.functional as F
+
+    N = ref.shape[0]
+    c = N / 2.0
+    dev = torch.device("cuda")
+    out = torch.empty((n, N, N, N), dtype=torch.float32, device=dev)
+    refd = torch.from_numpy(ref.astype(np.float32)).to(dev)[None, None]
+    ar = torch.arange(N, dtype=torch.float64, device=dev)
+    zz, yy, xx = torch.meshgrid(ar, ar, ar, indexing="ij")
+    xyz = torch.stack([xx, yy, zz], -1).reshape(-1, 3)  # (N^3, 3) x, y, z
+    f = torch.fft.fftfreq(N, device=dev, dtype=torch.float64) * N
+    kz, ky, kx = torch.meshgrid(f, f, f, indexing="ij")
+    gen = torch.Generator(device=dev)
+    truth = np.zeros((n, 4, 4))
+    descs = []
+    params = []
+    for p in range(n):
+        g = _rng(seed, 1000 + p)
+        R = random_rotation(g)
+        s = g.uniform(-max_shift, max_shift, 3)
+        h = g.uniform(half_wedge[0], half_wedge[1])
+        snr_p = g.uniform(snr[0], snr[1])
+        params.append((R, s, h, snr_p))
+        T = np.eye(4)
+        T[:3, :3] = R
+        T[:3, 3] = s
+        truth[p] = T
+        descs.append(Particle(lwedge=-h, uwedge=h))
+    for c0 in range(0, n, chunk):
+        cn = min(chunk, n - c0)
+        grids = []
+        for p in range(c0, c0 + cn):
+            R, s, _, _ = params[p]
+            Rt = torch.from_numpy(R.T.copy()).to(dev)
+            src = c + (xyz - c - torch.from_numpy(s).to(dev)) @ Rt.T  # (N^3, 3) source x, y, z
+            grids.append((2.0 * src / (N - 1) - 1.0).reshape(N, N, N, 3))
+        grid = torch.stack(grids).to(torch.float32)
+        vol = F.grid_sample(refd.expand(cn, 1, N, N, N), grid, mode="bilinear", padding_mode="zeros",
+                            align_corners=True)[:, 0].to(torch.float64)
+        for q in range(cn):
+            _, _, h, snr_p = params[c0 + q]
+            l, u = -h, h
+            n1 = (math.cos(-math.radians(90.0 - u)), 0.0, -math.sin(-math.radians(90.0 - u)))
+            n2 = (math.cos(-math.radians(90.0 - l)), 0.0, -math.sin(-math.radians(90.0 - l)))
+            W = ((n1[0] * kx + n1[2] * kz) * (n2[0] * kx + n2[2] * kz) <= 0).to(torch.float64)
+            v = torch.fft.ifftn(torch.fft.fftn(vol[q]) * W).real
+            gen.manual_seed(int(seed) * 1000003 + c0 + q)
+            noise = torch.randn((N, N, N), generator=gen, device=dev, dtype=torch.float64)
+            out[c0 + q] = (v + noise * math.sqrt(float(v.var()) / snr_p)).to(torch.float32)
+    torch.cuda.synchronize()
+    return out, descs, truth
+
+
 def rotation_error_deg(A: np.ndarray, B: np.ndarray) -> float:
     """Geodesic angle between the rotation parts of two 4x4 transforms."""
     R = A[:3, :3].T @ B[:3, :3]

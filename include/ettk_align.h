@@ -139,6 +139,12 @@ int ettk_ccc_grad_batch(ettk_plan* plan, const float* d_particles, int n_particl
                         const ettk_particle* descs, int n_evals, const int* particle_index,
                         const double* seeds, const double* params6, double* out_value, double* out_grad);
 
+/* Local refinement of n seed poses by n_iter evaluations of the continuous score of ettk_ccc_grad_batch
+   (accept/reject gradient ascent, all seeds in lock-step batches).  out_poses[e*16], out_scores[e]. */
+int ettk_refine_batch(ettk_plan* plan, const float* d_particles, int n_particles, const ettk_particle* descs,
+                      int n, const int* particle_index, const double* seeds, int n_iter, double* out_poses,
+                      double* out_scores);
+
 /* Amplitude-weighted Fourier average (WAvg.h:261-356 and CorrMetric.h:927-980).
    ettk_avg_reset zeroes the (re, im, acc) half-spectrum accumulators; ettk_avg_accumulate adds n particles
    resliced with matrices[i*16] and weighted by weights[i]; ettk_avg_finalize writes the RMS-normalised
@@ -152,6 +158,13 @@ int ettk_avg_finalize(ettk_plan* plan, float* d_out);
    the plan's stream: total milliseconds spent in the constrained-CCC pipeline and number of CCC
    evaluations it ran. */
 int ettk_last_timing(ettk_plan* plan, double* ccc_ms, long long* n_ccc, double* total_ms);
+
+/* Measurement hooks (no reference counterpart).  ettk_profile(ctx, 1) resets and enables per-kernel HIP-event
+   timing on the context's stream; ettk_profile_query returns, for one kernel name (e.g. "k_fft_strided<64>",
+   "k_reslice", "k_combine_dftA"), the summed launch time in ms, the launch count and the algorithmic bytes
+   those launches move (one read of every input and one write of every output). */
+int ettk_profile(ettk_ctx* ctx, int enable);
+int ettk_profile_query(ettk_ctx* ctx, const char* kernel, double* total_ms, long long* launches, double* bytes);
 
 #ifdef __cplusplus
 }

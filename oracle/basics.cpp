@@ -172,10 +172,59 @@ void fft_rec(const cd* in, cd* out, int n, int is, const int* fac, const FftPlan
 }
 }  // namespace
 
+// iterative Stockham autosort (radix 4, 2 and odd factors), ping-pong between x and a scratch buffer
+static void stockham(cd* x, int n, int sign, const FftPlan& P) {
+    thread_local std::vector<cd> scratch;
+    if ((int)scratch.size() < n) scratch.resize(n);
+    cd* a = x;
+    cd* b = scratch.data();
+    int Ns = 1;
+    for (size_t f = 0; f + 1 < P.fac.size(); f++) {
+        const int R = P.fac[f], NB = n / R;
+        for (int j = 0; j < NB; j++) {
+            const int k = j % Ns;
+            cd v[16];
+            for (int m = 0; m < R; m++) {
+                cd z = a[j + m * NB];
+                if (m > 0) {
+                    cd w = P.tw[(long)m * k * (n / (Ns * R))];
+                    if (sign > 0) w = std::conj(w);
+                    z *= w;
+                }
+                v[m] = z;
+            }
+            const int base = (j / Ns) * Ns * R + k;
+            if (R == 2) {
+                b[base] = v[0] + v[1];
+                b[base + Ns] = v[0] - v[1];
+            } else if (R == 4) {
+                cd s0 = v[0] + v[2], d0 = v[0] - v[2], s1 = v[1] + v[3], d1 = v[1] - v[3];
+                cd jd1 = sign < 0 ? cd(d1.imag(), -d1.real()) : cd(-d1.imag(), d1.real());
+                b[base] = s0 + s1;
+                b[base + 2 * Ns] = s0 - s1;
+                b[base + Ns] = d0 + jd1;
+                b[base + 3 * Ns] = d0 - jd1;
+            } else {
+                for (int q = 0; q < R; q++) {
+                    cd s = 0;
+                    for (int m = 0; m < R; m++) {
+                        cd w = P.tw[(long)((m * q) % R) * (n / R)];
+                        if (sign > 0) w = std::conj(w);
+                        s += v[m] * w;
+                    }
+                    b[base + q * Ns] = s;
+                }
+            }
+        }
+        std::swap(a, b);
+        Ns *= R;
+    }
+    if (a != x) std::copy(a, a + n, x);
+}
+
 void fft1d(cd* x, int n, int sign) {
     const FftPlan& P = get_plan(n);
-    std::vector<cd> tmp(x, x + n);
-    fft_rec(tmp.data(), x, n, 1, P.fac.data(), P, 1, sign);
+    stockham(x, n, sign, P);
 }
 
 void r2c3(const double* in, cd* out, int n0, int n1, int n2) {
