@@ -598,4 +598,28 @@ void orc_average(void* h, int n, const float* parts, const ettk_particle* d, con
             for (int k = 0; k < N; k++) out[((size_t)k * N + j) * N + i] = A[((size_t)i * N + j) * N + k] / var_inside;
 }
 
+// ---------------------------------------------------------------------------- checks of the restated pieces
+// Powell (maxfev = 1, the library's settings) on f(x) = sum_i a_i (x_i - c_i)^2 + k (x_0 - c_0)(x_1 - c_1);
+// returns the number of function evaluations, writes the minimiser and its value.
+int orc_powell_quadratic(const double* a, const double* c, double k, const double* x0, double step, double xtol,
+                         double ftol, double* xmin, double* fmin) {
+    PowellState S;
+    S.f = [&](const double* x) -> double {
+        double v = k * (x[0] - c[0]) * (x[1] - c[1]);
+        for (int i = 0; i < 3; i++) v += a[i] * (x[i] - c[i]) * (x[i] - c[i]);
+        return v;
+    };
+    double p[3] = {x0[0], x0[1], x0[2]};
+    powell_minimize(S, p, step, xtol, ftol);
+    for (int i = 0; i < 3; i++) xmin[i] = p[i];
+    *fmin = S.end_error;
+    return (int)S.nevals;
+}
+// 26-neighbour local maxima of an n^3 volume with a -FLT_MAX border (localMaxima3D3x3 + BordStrategyConst)
+void orc_local_maxima(const double* C, int n, double* out) {
+    std::vector<double> c(C, C + (size_t)n * n * n), o;
+    local_maxima(c, n, o);
+    std::memcpy(out, o.data(), o.size() * sizeof(double));
+}
+
 }  // extern "C"

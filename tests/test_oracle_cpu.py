@@ -1,0 +1,325 @@
+"""CPU checks of the FP64 oracle: golden vectors of soft-1.0, the compiled soft-1.0 checker (when present),
+and closed forms for the parts no fixture covers.  No GPU is used."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from ettk_amd.abi import Particle, default_params
+from oracle_lib import OraclePlan, oracle, s2_forward, so3_correlate, softref, vp
+
+GOLD = Path(__file__).resolve().parent / "golden" / "soft"
+
+
+def _wigner_D_grid(B: int, l: int, m1: int, m2: int) -> np.ndarray:
+    """Normalized Wigner D sqrt((2l+1)/8pi^2) e^{-i m1 a} d^l_{m1 m2}(b) e^{-i m2 g} on the SO(3) grid,
+    [beta][alpha][gamma], from the oracle's d^l tables."""
+    o = oracle()
+    n, W = 2 * B, 2 * B - 1
+    a = 2 * np.pi * np.arange(n) / n
+    out = np.zeros((n, n, n), complex)
+    for kb in range(n):
+        d = np.zeros(B * W * W)
+        o.orc_wigner_d(B, math.pi * (2 * kb + 1) / (4 * B), vp(d))
+        dv = d[(l * W + (m1 + B - 1)) * W + (m2 + B - 1)]
+        out[kb] = np.exp(-1j * m1 * a)[:, None] * dv * np.exp(-1j * m2 * a)[None, :]
+    return out * math.sqrt((2 * l + 1) / (8 * math.pi ** 2))
+
+
+def _load_cx(stem: str) -> np.ndarray:
+    return (np.loadtxt(GOLD / f"{stem}real_bw4.dat") + 1j * np.loadtxt(GOLD / f"{stem}imag_bw4.dat")).reshape(8, 8, 8)
+
+
+# ------------------------------------------------------------------------------------- soft-1.0 fixtures
+def test_wigner_d_matches_soft_fixtures():
+    """soft-1.0's sampled D^1_{0,1} (x (2+i)) and D^3_{-1,1} at bw = 4 pin the sign convention, the
+    normalisation sqrt((2l+1)/8 pi^2) and the [beta][alpha][gamma] sample order of the oracle's Wigner-d."""
+    assert np.abs(_load_cx("D101") - (2 + 1j) * _wigner_D_grid(4, 1, 0, 1)).max() < 1e-14
+    assert np.abs(_load_cx("D3-11") - _wigner_D_grid(4, 3, -1, 1)).max() < 1e-14
+
+
+def test_wigner_sum_matches_soft_fixture():
+    """dSum = (2 + sqrt2 i) D^1_{1,0} + (-sqrt5 + 11 i) D^2_{2,2} + (7 + sqrt3 i) D^3_{0,-2}."""
+    want = ((2 + math.sqrt(2) * 1j) * _wigner_D_grid(4, 1, 1, 0) + (-math.sqrt(5) + 11j) * _wigner_D_grid(4, 2, 2, 2)
+            + (7 + math.sqrt(3) * 1j) * _wigner_D_grid(4, 3, 0, -2))
+    assert np.abs(_load_cx("dSum_") - want).max() < 1e-12
+
+
+def test_so3_correlation_matches_soft_output():
+    """The oracle's SO(3) correlation of soft-1.0's two random bw = 8 signals equals soft-1.0's own output."""
+    sig = np.loadtxt(GOLD / "randomS2sig_bw8.dat")[0::2]
+    pat = np.loadtxt(GOLD / "randomS2sigA_bw8.dat")[0::2]
+    want = np.load(GOLD / "correlate_bw8.npy")  # [beta][alpha][gamma]
+    got = so3_correlate(8, sig, pat).transpose(1, 0, 2)  # oracle: [alpha][beta][gamma]
+    assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max()
+    assert np.argmax(got) == np.argmax(want)
+
+
+# ------------------------------------------------------------------------------------- compiled soft-1.0
+def _bandlimited(B: int, seed: int) -> np.ndarray:
+    o = oracle()
+    g = np.random.default_rng(seed)
+    re, im = g.standard_normal(B * B), g.standard_normal(B * B)
+    # real signal: coefficient (l, -m) = (-1)^m conj(l, m); enforce through a forward/inverse pass
+    f = np.zeros((2 * B) * (2 * B))
+    o.orc_s2_inverse(B, vp(re), vp(im), vp(f))
+    return np.ascontiguousarray(f)
+
+
+@pytest.mark.skipif(softref() is None, reason="soft-1.0 checker not built (reference tree absent)")
+@pytest.mark.parametrize("B", [4, 8, 16])
+def test_so3_correlation_matches_compiled_soft(B):
+    r = softref()
+    sig, pat = _bandlimited(B, 1 + B), _bandlimited(B, 2 + B)
+    n = 2 * B
+    want = np.zeros(n ** 3)
+    r.ref_soft_correlate(B, vp(sig), vp(pat), vp(want))
+    got = so3_correlate(B, sig, pat).transpose(1, 0, 2).ravel()
+    assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+
+
+@pytest.mark.skipif(softref() is None, reason="soft-1.0 checker not built (reference tree absent)")
+def test_s2_rotation_matches_compiled_soft():
+    r = softref()
+    o = oracle()
+    B = 8
+    f = _bandlimited(B, 7)
+    want = np.zeros_like(f)
+    got = np.zeros_like(f)
+    r.ref_rotate(B, vp(f), vp(want), 0.3, 1.1, -0.7)
+    o.orc_s2_rotate(B, vp(f), vp(got), 0.3, 1.1, -0.7)
+    assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+
+
+# ------------------------------------------------------------------------------------- closed forms
+def test_s2_transform_round_trip_and_orthonormality():
+    """Forward(inverse(e_lm)) on the Driscoll-Healy grid returns the coefficients of Re(Y_l^m)
+    = (Y_l^m + (-1)^m Y_l^-m) / 2 (the synthesis keeps the real part); coefficients are stored at l^2 + l + m."""
+    o = oracle()
+    B = 8
+    ix = lambda l, m: l * l + l + m  # noqa: E731
+    for (l, m) in [(0, 0), (4, 0), (3, 2), (7, -5), (5, 1)]:
+        re, im = np.zeros(B * B), np.zeros(B * B)
+        re[ix(l, m)] = 1.0
+        f = np.zeros(4 * B * B)
+        o.orc_s2_inverse(B, vp(re), vp(im), vp(f))
+        c = s2_forward(B, f)
+        e = np.zeros(B * B, complex)
+        if m == 0:
+            e[ix(l, 0)] = 1.0
+        else:
+            e[ix(l, m)] = 0.5
+            e[ix(l, -m)] = 0.5 * (-1) ** m
+        assert np.abs(c - e).max() < 1e-12, (l, m)
+
+
+def test_s2_rotation_preserves_norm_and_inverts():
+    o = oracle()
+    B = 8
+    f = _bandlimited(B, 11)
+    g = np.zeros_like(f)
+    h = np.zeros_like(f)
+    a, b, c = 0.4, 0.9, -1.3
+    o.orc_s2_rotate(B, vp(f), vp(g), a, b, c)
+    o.orc_s2_rotate(B, vp(g), vp(h), -c, -b, -a)  # R(a,b,c)^-1 = R(-c,-b,-a)
+    assert np.abs(h - f).max() < 1e-10 * np.abs(f).max()
+    cf, cg = s2_forward(B, f), s2_forward(B, g)
+    assert abs(np.linalg.norm(cf) - np.linalg.norm(cg)) < 1e-10 * np.linalg.norm(cf)
+
+
+def test_so3_correlation_peaks_at_the_rotation():
+    """Correlating f with f rotated by a grid rotation g0 peaks at the bin of g0."""
+    o = oracle()
+    B = 8
+    n = 2 * B
+    f = _bandlimited(B, 21)
+    ia, ib, ig = 3, 5, 12
+    al, be, ga = 2 * np.pi * ia / n, np.pi * (2 * ib + 1) / (4 * B), 2 * np.pi * ig / n
+    g = np.zeros_like(f)
+    o.orc_s2_rotate(B, vp(f), vp(g), al, be, ga)
+    Cc = so3_correlate(B, g, f)  # [alpha][beta][gamma]
+    assert np.unravel_index(np.argmax(Cc), Cc.shape) == (ia, ib, ig)
+
+
+def test_local_maxima_brute_force():
+    o = oracle()
+    n = 10
+    g = np.random.default_rng(3)
+    Cv = g.uniform(-1, 0, (n, n, n))
+    peaks = [(1, 2, 3), (5, 5, 5), (9, 0, 9), (0, 9, 4)]
+    for k, p in enumerate(peaks):
+        Cv[p] = 1.0 + k
+    out = np.zeros(n ** 3)
+    o.orc_local_maxima(vp(np.ascontiguousarray(Cv.ravel())), n, vp(out))
+    out = out.reshape(n, n, n)
+    # brute force: v >= all 26 neighbours (border -FLT_MAX)
+    P = np.pad(Cv, 1, constant_values=-np.finfo(np.float32).max)
+    want = np.zeros_like(Cv)
+    for a in range(n):
+        for b in range(n):
+            for c in range(n):
+                blk = P[a:a + 3, b:b + 3, c:c + 3]
+                if Cv[a, b, c] >= blk.max():
+                    want[a, b, c] = Cv[a, b, c]
+    assert np.array_equal(out, want)
+    for p in peaks:
+        assert out[p] == Cv[p]
+
+
+def test_powell_restatement_on_quadratics():
+    """One Powell sweep (maxfev = 1) with the library's line-search settings minimises a separable quadratic
+    to the line-search tolerance; a coupled one still decreases strictly."""
+    o = oracle()
+    o.orc_powell_quadratic.restype = C.c_int
+    o.orc_powell_quadratic.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_double, C.c_double,
+                                       C.c_double, C.c_void_p, C.c_void_p]
+    a = np.array([1.0, 2.0, 0.5])
+    c = np.array([3.0, -2.0, 7.5])
+    x0 = np.zeros(3)
+    xm = np.zeros(3)
+    fm = C.c_double()
+    nev = o.orc_powell_quadratic(vp(a), vp(c), 0.0, vp(x0), 5.0, 1e-2, 1e-1, vp(xm), C.byref(fm))
+    assert np.abs(xm - c).max() < 5e-2
+    assert fm.value < 1e-3
+    assert 6 <= nev <= 80
+    f0 = float(np.sum(a * c * c) + 0.8 * c[0] * c[1])
+    o.orc_powell_quadratic(vp(a), vp(c), 0.8, vp(x0), 5.0, 1e-2, 1e-1, vp(xm), C.byref(fm))
+    assert fm.value < 0.05 * f0
+
+
+def _quadratic_volume(N: int) -> np.ndarray:
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    return (0.01 * x * x - 0.02 * y * y + 0.015 * z * z + 0.005 * x * y - 0.3 * z + 0.2 * x + 1.0).astype(np.float32)
+
+
+def _reslice(vol, t):
+    o = oracle()
+    N = vol.shape[0]
+    out = np.zeros(N ** 3)
+    tt = np.ascontiguousarray(np.asarray(t, np.float64).ravel())
+    o.orc_reslice(vp(np.ascontiguousarray(vol)), N, C.byref(Particle()), vp(tt), vp(out))
+    return out.reshape(N, N, N)  # [x][y][z]
+
+
+def test_keys_reslice_reproduces_quadratics():
+    N = 16
+    v = _quadratic_volume(N)
+    t = np.eye(4)
+    t[:3, 3] = [0.3, -0.7, 0.45]
+    out = _reslice(v, t)
+    f = lambda x, y, z: 0.01 * x * x - 0.02 * y * y + 0.015 * z * z + 0.005 * x * y - 0.3 * z + 0.2 * x + 1.0  # noqa
+    sl = slice(2, N - 3)
+    xi, yi, zi = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")  # output [x][y][z]
+    ref = f(xi + 0.3, yi - 0.7, zi + 0.45)
+    d = (out - ref)[sl, sl, sl]
+    assert np.ptp(d) < 2e-5  # equal up to the mean-of-valid offset and float storage
+
+
+def test_reslice_identity_and_quarter_turn():
+    N = 16
+    g = np.random.default_rng(5)
+    v = g.standard_normal((N, N, N)).astype(np.float32)
+    out = _reslice(v, np.eye(4))
+    ref = v.transpose(2, 1, 0).astype(np.float64)
+    # the reslice floors with a tolerance of 2^-17 (vtkInterpolationMath::Floor) that stays in the fraction:
+    # the identity map samples at +7.6e-6 voxel, an error of that order times the local slope
+    assert np.abs(out - (ref - np.float32(ref.astype(np.float32).mean(dtype=np.float64)))).max() < 2e-4
+    # quarter turn about z about voxel N/2: input (x, y) = (N/2 - (y - N/2), N/2 + (x - N/2))
+    t = np.eye(4)
+    t[:2, :2] = [[0, -1], [1, 0]]
+    out = _reslice(v, t)
+    xi, yi = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+    px, py = N - yi, xi
+    ok = (px >= 0) & (px < N)
+    vals = np.zeros((N, N, N))
+    vals[ok] = v.transpose(2, 1, 0)[px[ok], py[ok], :]
+    inside = ok[:, :, None] & np.ones((1, 1, N), bool)
+    mean = vals[inside].astype(np.float32).mean(dtype=np.float64)
+    assert np.abs(out[inside] - (vals[inside] - mean)).max() < 2e-4
+
+
+def test_wedge_full_and_plane():
+    o = oracle()
+    N = 16
+    H = N // 2 + 1
+    w = np.zeros(N * N * H, np.float32)
+    o.orc_wedge_half(N, C.byref(Particle(lwedge=-90, uwedge=90)), None, vp(w))
+    assert np.all(w == 1.0)
+    o.orc_wedge_half(N, C.byref(Particle(lwedge=-60, uwedge=60)), None, vp(w))
+    w = w.reshape(N, N, H)
+    # smoothed binary planes: the tilt axis ky and the kx axis (0 deg) are measured, the beam axis kz is not
+    assert np.all(w[N // 2, :, H - 1] > 0.5)
+    assert np.all(w[:, N // 2, H - 1] > 0.5)
+    assert w[N // 2, N // 2, 0] < 0.01
+    assert np.all(w[N // 2, N // 2, : H - 3] < 0.5)
+
+
+def test_ccc_self_and_shift_recovery():
+    N = 32
+    p = default_params(N, 16)
+    p.missing_wedge = 0
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    blob = lambda cx, cy, cz: (np.exp(-((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2) / (2 * 2.5 ** 2))  # noqa
+                               + 0.6 * np.exp(-((x - cx - 4) ** 2 + (y - cy + 3) ** 2 + (z - cz - 2) ** 2) / 8.0))
+    ref = blob(16, 16, 16).astype(np.float32)
+    orc = OraclePlan(p)
+    orc.set_reference(ref)
+    d0, t0 = orc.ccc(ref, Particle(), np.eye(4))
+    assert np.abs(t0[:3, 3]).max() == 0.0
+    for s in [(2.0, -1.5, 0.5), (-3.0, 1.0, 2.5)]:
+        part = blob(16 + s[0], 16 + s[1], 16 + s[2]).astype(np.float32)
+        d, t = orc.ccc(part, Particle(), np.eye(4))
+        assert np.allclose(t[:3, 3], s), (s, t[:3, 3])
+        assert d > d0
+
+
+def test_average_of_identical_copies():
+    """The average of aligned identical copies is the RMS-normalised copy, whatever the (positive) weights."""
+    o = oracle()
+    N = 16
+    p = default_params(N, 8)
+    orc = OraclePlan(p)
+    g = np.random.default_rng(9)
+    v = g.standard_normal((N, N, N)).astype(np.float32)
+    vols = np.ascontiguousarray(np.stack([v, v, v]))
+    from ettk_amd.abi import particle_array
+    mats = np.ascontiguousarray(np.tile(np.eye(4).ravel(), (3, 1)))
+    out1 = np.zeros((N, N, N))
+    out2 = np.zeros((N, N, N))
+    o.orc_average(orc.h, 3, vp(vols), particle_array([Particle()] * 3), vp(np.array([1, 1, 1], np.float32)),
+                  vp(mats), vp(out1))
+    o.orc_average(orc.h, 3, vp(vols), particle_array([Particle()] * 3), vp(np.array([0.2, 3, 1.5], np.float32)),
+                  vp(mats), vp(out2))
+    assert np.abs(out1 - out2).max() < 1e-9
+    assert abs(np.sqrt(np.mean(out1 ** 2)) - 1.0) < 1e-6
+    # the copy itself, mean removed (DC := 0) and RMS-normalised
+    c = v.astype(np.float64) - v.astype(np.float32).mean(dtype=np.float64)
+    c = c - c.mean()
+    c /= np.sqrt(np.mean(c ** 2))
+    assert np.abs(out1 - c).max() < 1e-4
+
+
+def test_refine_score_is_one_at_the_identity():
+    """The continuous score of the reference against itself at the identity pose is 1 and stationary."""
+    o = oracle()
+    o.orc_refine_eval.restype = C.c_double
+    o.orc_refine_eval.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
+    from ettk_amd.synth import blob_reference
+    N = 16
+    p = default_params(N, 8)
+    p.missing_wedge = 0
+    ref = blob_reference(N, seed=3)
+    orc = OraclePlan(p)
+    orc.set_reference(ref)
+    g = np.zeros(6)
+    S = o.orc_refine_eval(orc.h, vp(ref), C.byref(Particle()), vp(np.eye(4).ravel().copy()), vp(np.zeros(6)), 1e-4,
+                          vp(g))
+    assert abs(S - 1.0) < 1e-12
+    assert np.abs(g).max() < 1e-6
+    S2 = o.orc_refine_eval(orc.h, vp(ref), C.byref(Particle()), vp(np.eye(4).ravel().copy()),
+                           vp(np.array([1.0, 0, 0, 0.2, 0, 0])), 1e-4, None)
+    assert S2 < S
