@@ -46,6 +46,20 @@ def build_lib(verbose: bool = False) -> Path:
     return LIB
 
 
+def build_cpp_tests() -> Path:
+    """The C++ drop-in's test driver (tests/cpp/test_metric), linked against libettk_align.so."""
+    src = ROOT / "tests" / "cpp" / "test_metric.cpp"
+    exe = src.with_suffix("")
+    cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I", str(ROOT / "include"), "-I", "/opt/rocm/include",
+           str(src), "-o", str(exe), "-L", str(PKG), "-lettk_align", "-L", "/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/../../ettk_amd", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build of tests/cpp/test_metric failed")
+    return exe
+
+
 def build_oracle() -> None:
     """The FP64 CPU oracle (test infrastructure) and, where the reference tree exists, the soft-1.0 checker."""
     subprocess.run(["make", "-C", str(ROOT / "oracle"), "-s", "liboracle.so"], check=True)
@@ -55,6 +69,7 @@ def build_oracle() -> None:
 
 if __name__ == "__main__":
     build_lib(verbose="-v" in sys.argv)
+    build_cpp_tests()
     if "--no-oracle" not in sys.argv:
         build_oracle()
     print(LIB)

@@ -704,7 +704,114 @@ void refine_adjoint_batch(ettk_plan* P, const float* d_parts, const ettk_particl
     }
 }
 
+// nbfCorrelationImageMetric::getDistances worker semantics (CorrMetric.h:1911-1990) for one reference:
+//   type 0: S^2/SO(3) search + refinement (refinementOn), type 3: search without refinement (refinementOff);
+//   type 1: nbfFourierImageMetric::execute (FIM.h:1812-1840): refine from the volume's stored transform, with
+//           the volume's own transform set to NULL while it runs;
+//   type 2: nbfFourierImageMetricCore::execute (Core.h:35-66): one constrained CCC at the identity (the
+//           volume's stored transform applies), the shifted transform kept only when it is valid.
+// `form` selects plain (0), batched (1) or batched + adjoint refinement (2) execution.
+void distances_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, int type, int form,
+                   ettk_result* out) {
+    ETTK_REQUIRE(P->have_ref, ETTK_ERR_STATE, "reference not set");
+    ETTK_REQUIRE(type >= 0 && type <= 3, ETTK_ERR_ARG, "ettk_distances_batch: type must be 0..3");
+    ETTK_REQUIRE(form >= 0 && form <= 2, ETTK_ERR_ARG, "ettk_distances_batch: form must be 0, 1 or 2");
+    if (type == 0 || type == 3) {
+        const int saved = P->p.refine;
+        P->p.refine = type == 0 ? 1 : 0;
+        try {
+            align_run(P, d_parts, descs, n, (type == 3 && form == 2) ? 1 : form, out);
+        } catch (...) {
+            P->p.refine = saved;
+            throw;
+        }
+        P->p.refine = saved;
+        return;
+    }
+    std::memset(out, 0, sizeof(ettk_result) * n);
+    std::vector<int> pidx(n);
+    for (int i = 0; i < n; i++) pidx[i] = i;
+    if (type == 2) {
+        std::vector<Mat4> t(n, Mat4::ident());
+        std::vector<float> d(n);
+        if (form == 0) {
+            for (int i = 0; i < n; i++) ccc_batch(P, d_parts, descs, 1, &pidx[i], &t[i], &d[i]);
+        } else {
+            ccc_batch(P, d_parts, descs, n, pidx.data(), t.data(), d.data());
+        }
+        for (int i = 0; i < n; i++) {
+            Mat4 r = transform_ok(t[i], P->trans_restrict, P->rot_restrict) ? t[i] : Mat4::ident();
+            out[i].distance = d[i];
+            out[i].cand_distance[0] = d[i];
+            std::memcpy(out[i].matrix, r.a, sizeof(r.a));
+            std::memcpy(out[i].cand_matrix[0], r.a, sizeof(r.a));
+            out[i].n_cand = 1;
+            out[i].n_ccc = 1;
+        }
+        return;
+    }
+    // type 1: the stored transform becomes the starting point, the volumes are resliced without it
+    std::vector<ettk_particle> dz(descs, descs + n);
+    std::vector<Mat4> t(n);
+    for (int i = 0; i < n; i++) {
+        t[i] = Mat4::from(descs[i].matrix);
+        const Mat4 I = Mat4::ident();
+        std::memcpy(dz[i].matrix, I.a, sizeof(I.a));
+    }
+    std::vector<float> dist(n);
+    std::vector<int> nccc(n, 0);
+    if (form == 2) {
+        std::vector<Mat4> pose(n), fin(n);
+        std::vector<double> score(n);
+        refine_with_seeds(P, d_parts, dz.data(), n, pidx.data(), t.data(), 20, pose.data(), score.data());
+        fin = pose;
+        ccc_batch(P, d_parts, dz.data(), n, pidx.data(), fin.data(), dist.data());
+        t = pose;
+        for (int i = 0; i < n; i++) nccc[i] = 1;
+    } else {
+        auto run_one = [&](int i, const EvalFn& ev) {
+            auto counted = [&](int p, Mat4& tt) {
+                nccc[p]++;
+                return ev(p, tt);
+            };
+            dist[i] = refine_powell(P, i, t[i], true, counted);
+        };
+        if (form == 0) {
+            EvalFn ev = [&](int pi, Mat4& tt) {
+                float d;
+                ccc_batch(P, d_parts, dz.data(), 1, &pi, &tt, &d);
+                return d;
+            };
+            for (int i = 0; i < n; i++) run_one(i, ev);
+        } else {
+            int T = std::min(n, 1024);
+            BatchEval be(P, d_parts, dz.data(), T);
+            EvalFn ev = [&](int pi, Mat4& tt) { return be.eval(pi, tt); };
+            std::vector<std::thread> th;
+            for (int w = 0; w < T; w++)
+                th.emplace_back([&, w] {
+                    try {
+                        for (int i = w; i < n; i += T) run_one(i, ev);
+                        be.done();
+                    } catch (const Error& e) {
+                        be.fail(e);
+                    }
+                });
+            for (auto& x : th) x.join();
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        out[i].distance = dist[i];
+        out[i].cand_distance[0] = dist[i];
+        std::memcpy(out[i].matrix, t[i].a, sizeof(t[i].a));
+        std::memcpy(out[i].cand_matrix[0], t[i].a, sizeof(t[i].a));
+        out[i].n_cand = 1;
+        out[i].n_ccc = nccc[i];
+    }
+}
+
 }  // namespace ettk
+
 
 // ============================================================== C ABI
 using namespace ettk;
@@ -740,6 +847,20 @@ int ettk_so3_correlate_batch(ettk_plan* P, int n, const double* sig, const doubl
         soft_given(P, n, sig, pat, S.C.p);
         ETTK_HIP(hipMemcpyAsync(out_c, S.C.p, n3 * n * sizeof(double), hipMemcpyDeviceToHost, P->ctx->stream));
         ETTK_HIP(hipStreamSynchronize(P->ctx->stream));
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    }
+    set_error("");
+    return ETTK_OK;
+}
+
+int ettk_distances_batch(ettk_plan* P, const float* d_particles, int n, const ettk_particle* descs, int type,
+                         int form, ettk_result* out) {
+    try {
+        ETTK_REQUIRE(P && d_particles && descs && out && n > 0, ETTK_ERR_ARG, "ettk_distances_batch: bad argument");
+        ETTK_HIP(hipSetDevice(P->ctx->device));
+        distances_run(P, d_particles, descs, n, type, form, out);
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;

@@ -127,6 +127,49 @@ static void wedge_normals(float lw, float uw, const Mat4& t, WedgeRef& w) {
     }
 }
 
+// search window of the upsampled correlation and its partial-DFT tables (FIM.h:1222-1247); rebuilt when
+// the translation restriction changes
+static void search_window(ettk_plan* P) {
+    const int N = P->N, H = P->H;
+    hipStream_t s = P->ctx->stream;
+    double center = N;  // (2N)/2
+    double restr = center;
+    if (P->trans_restrict < (double)FLT_MAX) restr = std::max(P->trans_restrict * 2.0, 1.0);
+    int I0 = (int)std::max(0.0, center - std::floor(restr));
+    int I1 = (int)std::min(center + std::floor(restr), (double)(2 * N - 1));
+    int E = I1 - I0 + 1;
+    P->z.N = N;
+    P->z.H = H;
+    P->z.E = E;
+    P->z.I0 = I0;
+    P->z.ub2 = (E - 1) / 2;
+    P->z.restr2 = (float)restr * (float)restr;
+    size_t lds_a = (size_t)(N * H + E * N) * sizeof(float2);
+    size_t lds_b = (size_t)(N * H + E * N + 2 * E * H) * sizeof(float2) + 2048;
+    ETTK_REQUIRE(lds_a <= 160 * 1024 && lds_b <= 160 * 1024, ETTK_ERR_UNSUPPORTED,
+                 "translation search window too large for the LDS-resident upsampled DFT (reduce trans_restrict)");
+    std::vector<float2> tA((size_t)E * N), tK((size_t)E * H);
+    for (int x = 0; x < E; x++) {
+        int u = I0 + x - N;
+        for (int i = 0; i < N; i++) {
+            int fr = i - N / 2;
+            double ang = M_PI * (double)fr * (double)u / (double)N;
+            double sg = (fr & 1) ? -1.0 : 1.0;
+            tA[(size_t)x * N + i] = make_float2((float)(sg * std::cos(ang)), (float)(sg * std::sin(ang)));
+        }
+        for (int k = 0; k < H; k++) {
+            int fr = k - N / 2;
+            double ang = M_PI * (double)fr * (double)u / (double)N;
+            double sg = (fr & 1) ? -1.0 : 1.0;
+            double w = (k == H - 1) ? 1.0 : 2.0;
+            tK[(size_t)x * H + k] = make_float2((float)(w * sg * std::cos(ang)), (float)(w * sg * std::sin(ang)));
+        }
+    }
+    P->tabA.upload(tA.data(), tA.size(), s);
+    P->tabK.upload(tK.data(), tK.size(), s);
+    ETTK_HIP(hipStreamSynchronize(s));
+}
+
 static void plan_constants(ettk_plan* P) {
     const ettk_params& p = P->p;
     const int N = P->N, H = P->H;
@@ -190,47 +233,13 @@ static void plan_constants(ettk_plan* P) {
     P->Mfft.ensure(nh);
     R2CArgs a{P->m1.p, P->Mfft.p, N, 1, PRO_SHIFT, nullptr, nullptr};
     fft3_r2c(P->ctx->fft, a, nullptr, s);
-    // search window of the upsampled correlation and its partial-DFT tables (FIM.h:1222-1247)
-    double center = N;  // (2N)/2
-    double restr = center;
-    if (P->trans_restrict < (double)FLT_MAX) restr = std::max(P->trans_restrict * 2.0, 1.0);
-    int I0 = (int)std::max(0.0, center - std::floor(restr));
-    int I1 = (int)std::min(center + std::floor(restr), (double)(2 * N - 1));
-    int E = I1 - I0 + 1;
-    P->z.N = N;
-    P->z.H = H;
-    P->z.E = E;
-    P->z.I0 = I0;
-    P->z.ub2 = (E - 1) / 2;
-    P->z.restr2 = (float)restr * (float)restr;
-    size_t lds_a = (size_t)(N * H + E * N) * sizeof(float2);
-    size_t lds_b = (size_t)(N * H + E * N + 2 * E * H) * sizeof(float2) + 2048;
-    ETTK_REQUIRE(lds_a <= 160 * 1024 && lds_b <= 160 * 1024, ETTK_ERR_UNSUPPORTED,
-                 "translation search window too large for the LDS-resident upsampled DFT (reduce trans_restrict)");
-    std::vector<float2> tA((size_t)E * N), tK((size_t)E * H);
-    for (int x = 0; x < E; x++) {
-        int u = I0 + x - N;
-        for (int i = 0; i < N; i++) {
-            int fr = i - N / 2;
-            double ang = M_PI * (double)fr * (double)u / (double)N;
-            double sg = (fr & 1) ? -1.0 : 1.0;
-            tA[(size_t)x * N + i] = make_float2((float)(sg * std::cos(ang)), (float)(sg * std::sin(ang)));
-        }
-        for (int k = 0; k < H; k++) {
-            int fr = k - N / 2;
-            double ang = M_PI * (double)fr * (double)u / (double)N;
-            double sg = (fr & 1) ? -1.0 : 1.0;
-            double w = (k == H - 1) ? 1.0 : 2.0;
-            tK[(size_t)x * H + k] = make_float2((float)(w * sg * std::cos(ang)), (float)(w * sg * std::sin(ang)));
-        }
-    }
-    P->tabA.upload(tA.data(), tA.size(), s);
-    P->tabK.upload(tK.data(), tK.size(), s);
-    ETTK_HIP(hipStreamSynchronize(s));
+    search_window(P);
 }
 
 void ensure_ccc_workspace(ettk_plan* P, int cap) {
-    if (cap <= P->cap) return;
+    if (cap <= P->cap && P->z.E <= P->ws_E) return;
+    cap = std::max(cap, P->cap);
+    P->ws_E = P->z.E;
     const int N = P->N, H = P->H;
     const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * H;
     P->nb_real = (int)((n3 + 1023) / 1024);
@@ -460,6 +469,31 @@ int ettk_plan_destroy(ettk_plan* P) {
         if (P->ev0) (void)hipEventDestroy(P->ev0);
         if (P->ev1) (void)hipEventDestroy(P->ev1);
         delete P;
+    }
+    ETTK_API_END
+}
+
+int ettk_plan_set_restrictions(ettk_plan* P, float rot_restrict, float trans_restrict) {
+    ETTK_API_BEGIN
+    ETTK_REQUIRE(P, ETTK_ERR_ARG, "ettk_plan_set_restrictions: null plan");
+    ETTK_HIP(hipSetDevice(P->ctx->device));
+    ETTK_HIP(hipStreamSynchronize(P->ctx->stream));
+    const ettk_params saved = P->p;
+    const double tr0 = P->trans_restrict, rot0 = P->rot_restrict;
+    const CccSizes z0 = P->z;
+    P->p.rot_restrict = rot_restrict;
+    P->p.trans_restrict = trans_restrict;
+    P->trans_restrict = trans_restrict < 0 ? (double)FLT_MAX : (double)trans_restrict;
+    P->rot_restrict = std::max((double)rot_restrict, 1.0);
+    try {
+        search_window(P);
+    } catch (...) {
+        P->p = saved;
+        P->trans_restrict = tr0;
+        P->rot_restrict = rot0;
+        P->z = z0;
+        search_window(P);
+        throw;
     }
     ETTK_API_END
 }

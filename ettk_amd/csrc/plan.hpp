@@ -69,7 +69,7 @@ struct ettk_plan {
     ettk_particle refd{};
     ettk::DevBuf<float> ref;
     // CCC workspace
-    int cap = 0;
+    int cap = 0, ws_E = 0;
     int nb_real = 0, nb_spec = 0, nb_res = 0;
     ettk::DevBuf<float> R1, R2, p_res, p_n1, p_n2, p_k1, p_wf, stats, scalars, bval, out4;
     ettk::DevBuf<int> bidx;

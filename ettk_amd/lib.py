@@ -29,11 +29,14 @@ def load() -> C.CDLL:
             "ettk_ctx_destroy": ([vp], C.c_int),
             "ettk_plan_create": ([vp, C.POINTER(Params), C.POINTER(vp)], C.c_int),
             "ettk_plan_destroy": ([vp], C.c_int),
+            "ettk_plan_set_restrictions": ([vp, C.c_float, C.c_float], C.c_int),
             "ettk_reference_set": ([vp, vp, C.POINTER(Particle)], C.c_int),
             "ettk_ccc_eval_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, ip, dp, fp, dp], C.c_int),
             "ettk_project_batch": ([vp, vp, C.c_int, C.POINTER(Particle), fp], C.c_int),
             "ettk_so3_correlate_batch": ([vp, C.c_int, dp, dp, dp], C.c_int),
             "ettk_align_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, C.POINTER(Result)], C.c_int),
+            "ettk_distances_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, C.c_int, C.POINTER(Result)],
+                                     C.c_int),
             "ettk_ccc_grad_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, ip, dp, dp, dp, dp], C.c_int),
             "ettk_avg_reset": ([vp], C.c_int),
             "ettk_avg_accumulate": ([vp, vp, C.c_int, C.POINTER(Particle), fp, dp], C.c_int),
@@ -147,6 +150,19 @@ class Plan:
         n = d_parts.shape[0]
         res = (Result * n)()
         check(self.lib.ettk_align_batch(self.h, C.c_void_p(d_parts.data_ptr()), n, particle_array(descs), mode, res))
+        return list(res)
+
+    def set_restrictions(self, rot_restrict: float, trans_restrict: float):
+        check(self.lib.ettk_plan_set_restrictions(self.h, rot_restrict, trans_restrict))
+        self.params.rot_restrict = rot_restrict
+        self.params.trans_restrict = trans_restrict
+
+    def distances(self, d_parts, descs, type: int = 0, form: int = 1):
+        """getDistances semantics by metric type (0 search+refine, 1 refine only, 2 shift only, 3 search)."""
+        n = d_parts.shape[0]
+        res = (Result * n)()
+        check(self.lib.ettk_distances_batch(self.h, C.c_void_p(d_parts.data_ptr()), n, particle_array(descs), type,
+                                            form, res))
         return list(res)
 
     def ccc_grad(self, d_parts, descs, pidx, seeds, params6):

@@ -97,6 +97,12 @@ int ettk_ctx_destroy(ettk_ctx* ctx);
 int ettk_plan_create(ettk_ctx* ctx, const ettk_params* params, ettk_plan** out);
 int ettk_plan_destroy(ettk_plan* plan);
 
+/* setRotationSearchRestriction / setTranslationSearchRestriction (nbfImageMetric.h:81-95): the rotation
+   restriction is clamped to >= 1 degree, a negative translation restriction means unrestricted; the
+   upsampled search window and its DFT tables are rebuilt.  Restriction 0 evaluates the CCC at zero shift
+   (as the multi-reference assignment does, nbfLoopClustering.h:1560-1563). */
+int ettk_plan_set_restrictions(ettk_plan* plan, float rot_restrict, float trans_restrict);
+
 /* Reference volume (input1 of the metric): setInput1 (nbfImageMetric.h:184-189).  The volume is used as
    is (an average is already RMS-normalised, WAvg.h:241-243); its wedge is analytic from desc->l/uwedge.
    Caches the reference side of the CCC (F1m, k1_m: FIM.h:1123-1153) and its S^2 projection Pp1
@@ -130,11 +136,24 @@ int ettk_so3_correlate_batch(ettk_plan* plan, int n, const double* sig, const do
 int ettk_align_batch(ettk_plan* plan, const float* d_particles, int n, const ettk_particle* descs,
                      int mode, ettk_result* out);
 
+/* nbfCorrelationImageMetric::getDistances (CorrMetric.h:659-704, worker CorrMetric.h:1911-2008) for the
+   reference against n particles, by metric type:
+   type 0 = S^2/SO(3) search + refinement (refinementOn);  type 3 = search without refinement;
+   type 1 = refinement only, from each particle's stored transform (nbfFourierImageMetric::execute,
+            FIM.h:1812-1840: the stored transform seeds refine() and is not applied while it runs);
+   type 2 = translation search only at the stored alignment (nbfFourierImageMetricCore::execute,
+            Core.h:35-66: one constrained CCC at the identity, its shift kept when the result is valid).
+   form as `mode` of ettk_align_batch (0 plain, 1 batched, 2 batched + adjoint refinement). */
+int ettk_distances_batch(ettk_plan* plan, const float* d_particles, int n, const ettk_particle* descs,
+                         int type, int form, ettk_result* out);
+
 /* Continuous constrained CCC and its gradient with respect to (rotX, rotY, rotZ, tx, ty, tz) around the
-   seed transforms (degrees, voxels), through a trilinear rotation of the particle.  No reference
-   counterpart: it replaces the Powell line searches of nbfFourierImageMetric::refine (FIM.h:1702-1809).
-   params6[e*6] = (rx, ry, rz, tx, ty, tz); seeds[e*16]; out_value[e] = masked CCC score (higher is better);
-   out_grad[e*6]. */
+   seed transforms (degrees, voxels).  No reference counterpart: it replaces the Powell line searches of
+   nbfFourierImageMetric::refine (FIM.h:1702-1809).  The score is taken in the particle's own frame: the
+   reference (minus its mean) is resliced trilinearly through fin(x)^-1, fin = Euler M t(x), and correlated
+   with the particle over band-pass x particle wedge x reference wedge (binary, the particle's wedge fixed in
+   its frame).  params6[e*6] = (rx, ry, rz, tx, ty, tz) with t(x) = [R0 Rx Ry Rz | tau0 + (tx, ty, tz)];
+   seeds[e*16]; out_value[e] = normalised score (higher is better); out_grad[e*6] its exact gradient. */
 int ettk_ccc_grad_batch(ettk_plan* plan, const float* d_particles, int n_particles,
                         const ettk_particle* descs, int n_evals, const int* particle_index,
                         const double* seeds, const double* params6, double* out_value, double* out_grad);

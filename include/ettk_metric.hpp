@@ -1,0 +1,336 @@
+/* ettk_metric.hpp -- C++ drop-in for the alignment path of nbfCorrelationImageMetric<float,3>
+ * (inc/em/nbfCorrelationImageMetric.h) on top of the C ABI of libettk_align (ettk_align.h).
+ *
+ * The class keeps the reference's method names, argument meaning and result layout so that a caller such as
+ * MPI_Classification.cpp:954/1087 or nbfLoopClustering.h:1560-1563 switches by changing the type it
+ * instantiates:
+ *   - volumes are device arrays (N^3 float, x fastest, the VTK/MRC order) plus the sub-volume descriptor
+ *     (wedge, Euler normal, cut offset, stored alignment) that nbfWedgedSubImage3D carries;
+ *   - getDistances(refs, vols, D, type) fills D(i, j, 0..16, k) = {distance, M00..M33} of candidate k,
+ *     computing only the entries whose D(i, j, 0, 0) == -1 (resume semantics, CorrMetric.h:1950);
+ *   - getImage(average) builds the amplitude-weighted Fourier average of the average's volumes under their
+ *     alignments [weight, M16] (WAvg.h:182-214, 261-356);
+ *   - mraAssignment(...) is the best-reference assignment of nbfLoopClustering::doMra (:1523-1592).
+ * Errors: the reference prints to cerr and returns sentinels; here every failure of the C ABI throws
+ * ettk::MetricError carrying the code and ettk_last_error() text (no sentinel distances are invented).
+ * Device memory: the metric never takes ownership of caller volumes; when the volumes of one call are not
+ * contiguous in one allocation it gathers them into a scratch buffer it owns.
+ */
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ettk_align.h"
+
+namespace ettk {
+
+struct MetricError : std::runtime_error {
+    int code;
+    MetricError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc, const char* what) {
+    if (rc != ETTK_OK) throw MetricError(rc, std::string(what) + ": " + ettk_last_error());
+}
+inline void check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw MetricError(ETTK_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+/* nbfWedgedSubImage3D<float>: one particle held as a fixed image. */
+struct WedgedSubImage3D {
+    const float* d_volume = nullptr;  // device, N^3 floats, x fastest
+    ettk_particle desc{-90.0f, 90.0f, {0, 0, 0}, 0.0f, {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}};
+
+    void setWedge(float lwedge, float uwedge) {  // nbfWedge3D::set (Wedge.h:143-184)
+        desc.lwedge = lwedge;
+        desc.uwedge = uwedge;
+    }
+    void setTransform(const double* m16) {  // NULL resets to the identity
+        static const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+        std::memcpy(desc.matrix, m16 ? m16 : I, sizeof(desc.matrix));
+    }
+    void getTransform(double* m16) const { std::memcpy(m16, desc.matrix, sizeof(desc.matrix)); }
+    void setCutOffset(float c) { desc.cut_offset = c; }
+    void setNormal(float n0, float n1, float n2) {
+        desc.normal[0] = n0;
+        desc.normal[1] = n1;
+        desc.normal[2] = n2;
+    }
+};
+
+/* nbfWedgedAverageImage3D<float>: a reference.  `d_average` holds the current average (input of the
+   alignments, output of getImage); `volumes` and `alignments` ([n][17][K]: weight, M16 per candidate,
+   setAlignments WAvg.h:182-214) define the average getImage builds. */
+struct WedgedAverageImage3D {
+    float* d_average = nullptr;  // device, N^3 floats, x fastest
+    ettk_particle desc{-90.0f, 90.0f, {0, 0, 0}, 0.0f, {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}};
+    std::vector<WedgedSubImage3D>* volumes = nullptr;
+    std::vector<float> alignments;  // n * 17 * K
+    int K = 1;
+
+    void setVolumes(std::vector<WedgedSubImage3D>* v) { volumes = v; }
+    void setAlignments(const std::vector<float>& a, int k) {
+        alignments = a;
+        K = k;
+    }
+    float& alignment(int i, int c, int k) { return alignments[((size_t)i * 17 + c) * K + k]; }
+};
+
+/* blitz::Array<float, 4> D(nRef, nVol, depth, K) of getDistances; depth 17 = {distance, M00..M33}. */
+class DistanceArray {
+   public:
+    DistanceArray(int rows, int cols, int depth = 17, int K = 1)
+        : rows_(rows), cols_(cols), depth_(depth), K_(K), v_((size_t)rows * cols * depth * K, -1.0f) {}
+    float& operator()(int i, int j, int c, int k) { return v_[(((size_t)i * cols_ + j) * depth_ + c) * K_ + k]; }
+    float operator()(int i, int j, int c, int k) const { return v_[(((size_t)i * cols_ + j) * depth_ + c) * K_ + k]; }
+    int rows() const { return rows_; }
+    int cols() const { return cols_; }
+    int depth() const { return depth_; }
+    int extent3() const { return K_; }
+
+   private:
+    int rows_, cols_, depth_, K_;
+    std::vector<float> v_;
+};
+
+class CorrelationImageMetric {
+   public:
+    /* The plan is built for box size N; B is the spherical-harmonic bandwidth (the reference's public B). */
+    CorrelationImageMetric(int N, int B = 32, int device = 0) : B(B) {
+        check(ettk_ctx_create(device, &ctx_), "ettk_ctx_create");
+        p_.N = N;
+        p_.B = B;
+        p_.bp_lo = 0.0f;
+        p_.bp_hi = 1.0f;
+        p_.bp_lo_decay = p_.bp_hi_decay = 0.01f;
+        p_.bfactor = 0.0f;
+        p_.mask_radius = 0.0f;
+        p_.mask_sigma = 0.0f;
+        p_.square_mask = 0;
+        p_.mcf = 0;
+        p_.overlap_norm = 0;
+        p_.missing_wedge = 0;
+        p_.n_peaks = 5;  // ProjRot.h:209
+        p_.n_candidates = 1;  // CorrMetric.h:185
+        p_.rot_restrict = 180.0f;
+        p_.trans_restrict = -1.0f;
+        p_.refine = 1;
+    }
+    ~CorrelationImageMetric() {
+        if (plan_) ettk_plan_destroy(plan_);
+        if (scratch_) (void)hipFree(scratch_);
+        if (ctx_) ettk_ctx_destroy(ctx_);
+    }
+    CorrelationImageMetric(const CorrelationImageMetric&) = delete;
+    CorrelationImageMetric& operator=(const CorrelationImageMetric&) = delete;
+
+    int B;
+
+    // ---------------------------------------------------------------- configuration (reference names)
+    void setRotationSearchRestriction(float deg) { set_(p_.rot_restrict, std::max(deg, 1.0f), true); }
+    void setTranslationSearchRestriction(float vox) { set_(p_.trans_restrict, vox, true); }
+    float getTranslationSearchRestriction() const { return p_.trans_restrict; }
+    float getRotationSearchRestriction() const { return p_.rot_restrict; }
+    void setToUseMutualCorrelation(bool on) { set_(p_.mcf, on ? 1 : 0); }
+    void setToComputeOverlapNormalizedDistances(bool on) { set_(p_.overlap_norm, on ? 1 : 0); }
+    void setMissingWedgeCompensation(bool on) { set_(p_.missing_wedge, on ? 1 : 0); }
+    void setNumberOfCandidatePeaksToSearch(int k) { set_(p_.n_peaks, k); }
+    void setNumberOfCandidates(int k) { set_(p_.n_candidates, k); }
+    int getNumberOfCandidates() const { return p_.n_candidates; }
+    void refinementOn() { set_(p_.refine, 1); }
+    void refinementOff() { set_(p_.refine, 0); }
+    // nbfFourierFilter::bandPassOn (FFilt.h:700-712) and bfactorOn
+    void bandPassOn(float lo, float hi, float lo_decay, float hi_decay) {
+        set_(p_.bp_lo, lo);
+        set_(p_.bp_hi, hi);
+        set_(p_.bp_lo_decay, lo_decay);
+        set_(p_.bp_hi_decay, hi_decay);
+    }
+    void bfactorOn(float b) { set_(p_.bfactor, b); }
+    // nbfImageFilter::maskOn with an analytic spherical mask (IFilt.h:384-472) and the square FFT window size
+    void maskOn(float radius, float sigma) {
+        set_(p_.mask_radius, radius);
+        set_(p_.mask_sigma, sigma);
+    }
+    void setSquareMaskSize(int sq) { set_(p_.square_mask, sq); }
+    /* Execution form, no reference counterpart: 0 plain (one evaluation per launch, the reference's control
+       flow), 1 batched, 2 batched with adjoint-gradient refinement. */
+    void setExecutionForm(int form) { form_ = form; }
+
+    // ---------------------------------------------------------------- batch alignment
+    /* getDistances(refs, vols, D, type) (CorrMetric.h:659-704): type 0 search + refine, 1 refine only,
+       2 translation only, 3 search without refinement. */
+    void getDistances(std::vector<WedgedAverageImage3D>& refs, std::vector<WedgedSubImage3D>& vols,
+                      DistanceArray& D, int type) {
+        if (D.rows() != (int)refs.size() || D.cols() != (int)vols.size())
+            throw MetricError(ETTK_ERR_ARG, "getDistances: D must be refs x vols");
+        if (D.depth() != 17)
+            throw MetricError(ETTK_ERR_UNSUPPORTED, "getDistances: only the 17-column layout {distance, M16}");
+        const int K = D.extent3();
+        if (K < 1 || K > ETTK_MAX_CANDIDATES) throw MetricError(ETTK_ERR_ARG, "getDistances: bad candidate count");
+        if (type != 1 && type != 2 && K != p_.n_candidates)
+            throw MetricError(ETTK_ERR_ARG, "getDistances: D's last extent must equal the number of candidates");
+        ensure_plan_();
+        for (int i = 0; i < (int)refs.size(); i++) {
+            std::vector<int> todo;
+            for (int j = 0; j < (int)vols.size(); j++)
+                if (D(i, j, 0, 0) == -1.0f) todo.push_back(j);
+            if (todo.empty()) continue;
+            check(ettk_reference_set(plan_, refs[i].d_average, &refs[i].desc), "ettk_reference_set");
+            std::vector<ettk_particle> descs;
+            const float* base = gather_(vols, todo, descs);
+            std::vector<ettk_result> res(todo.size());
+            check(ettk_distances_batch(plan_, base, (int)todo.size(), descs.data(), type, form_, res.data()),
+                  "ettk_distances_batch");
+            for (size_t q = 0; q < todo.size(); q++) {
+                const int j = todo[q];
+                const ettk_result& r = res[q];
+                if (K == 1) {
+                    D(i, j, 0, 0) = r.distance;
+                    for (int c = 0; c < 16; c++) D(i, j, 1 + c, 0) = (float)r.matrix[c];
+                } else {
+                    for (int k = 0; k < K; k++) D(i, j, 0, k) = std::numeric_limits<float>::max();
+                    for (int k = 0; k < std::min(K, r.n_cand); k++) {
+                        D(i, j, 0, k) = r.cand_distance[k];
+                        for (int c = 0; c < 16; c++) D(i, j, 1 + c, k) = (float)r.cand_matrix[k][c];
+                    }
+                }
+            }
+        }
+    }
+
+    // ---------------------------------------------------------------- averaging
+    /* getImage(average) (CorrMetric.h:803-984 with WAvg.h:261-356): sum over volumes and candidates with
+       non-zero weight of the amplitude-weighted spectra, then the RMS-normalised average into d_average. */
+    void getImage(WedgedAverageImage3D& avg) {
+        if (!avg.volumes || avg.volumes->empty()) throw MetricError(ETTK_ERR_ARG, "getImage: no volumes");
+        const int n = (int)avg.volumes->size();
+        if (avg.alignments.size() != (size_t)n * 17 * avg.K) throw MetricError(ETTK_ERR_ARG, "getImage: alignments");
+        ensure_plan_();
+        std::vector<int> all(n);
+        for (int j = 0; j < n; j++) all[j] = j;
+        std::vector<ettk_particle> descs;
+        const float* base = gather_(*avg.volumes, all, descs);
+        check(ettk_avg_reset(plan_), "ettk_avg_reset");
+        std::vector<float> w(n);
+        std::vector<double> m((size_t)n * 16);
+        for (int k = 0; k < avg.K; k++) {
+            bool any = false;
+            for (int j = 0; j < n; j++) {
+                w[j] = avg.alignment(j, 0, k);
+                any = any || w[j] != 0.0f;
+                for (int c = 0; c < 16; c++) m[(size_t)j * 16 + c] = avg.alignment(j, 1 + c, k);
+            }
+            if (any) check(ettk_avg_accumulate(plan_, base, n, descs.data(), w.data(), m.data()), "ettk_avg_accumulate");
+        }
+        check(ettk_avg_finalize(plan_, avg.d_average), "ettk_avg_finalize");
+    }
+
+    // ---------------------------------------------------------------- multi-reference assignment
+    /* nbfLoopClustering::doMra best-reference selection (nbfLoopClustering.h:1523-1592): each volume is
+       evaluated under each of its alignments to the references (A(j, i, 1..16, 0)) against every reference
+       at zero shift (type 2, translation restriction 0); the alignment whose distances sum lowest over the
+       references is marked by negating A(j, i, 0, 0).  Returns the chosen alignment per volume. */
+    std::vector<int> mraAssignment(std::vector<WedgedAverageImage3D>& refs, std::vector<WedgedSubImage3D>& vols,
+                                   DistanceArray& A) {
+        const int R = (int)refs.size(), n = (int)vols.size();
+        if (A.rows() != R || A.cols() != n || A.depth() != 17)
+            throw MetricError(ETTK_ERR_ARG, "mraAssignment: A must be refs x vols x 17");
+        std::vector<int> best(n, 0);
+        if (R < 2) return best;
+        std::vector<WedgedSubImage3D> dup((size_t)n * R);
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < R; j++) {
+                double m[16];
+                for (int c = 0; c < 16; c++) m[c] = A(j, i, 1 + c, 0);
+                dup[(size_t)i * R + j] = vols[i];
+                dup[(size_t)i * R + j].setTransform(m);
+            }
+        const float saved = p_.trans_restrict;
+        setTranslationSearchRestriction(0.0f);
+        DistanceArray E(R, n * R, 17, 1);
+        try {
+            getDistances(refs, dup, E, 2);
+        } catch (...) {
+            setTranslationSearchRestriction(saved);
+            throw;
+        }
+        setTranslationSearchRestriction(saved);
+        for (int i = 0; i < n; i++) {
+            double bestv = 0;
+            for (int j = 0; j < R; j++) {
+                double s = 0;
+                for (int r = 0; r < R; r++) s += E(r, i * R + j, 0, 0);
+                if (j == 0 || s < bestv) {  // first minimum (blitz minIndex)
+                    bestv = s;
+                    best[i] = j;
+                }
+            }
+            A(best[i], i, 0, 0) *= -1.0f;
+        }
+        return best;
+    }
+
+    ettk_plan* plan() {
+        ensure_plan_();
+        return plan_;
+    }
+
+   private:
+    template <class T>
+    void set_(T& field, T v, bool restriction = false) {
+        if (field == v) return;
+        field = v;
+        if (!plan_) return;
+        if (restriction) {
+            check(ettk_plan_set_restrictions(plan_, p_.rot_restrict, p_.trans_restrict), "ettk_plan_set_restrictions");
+        } else {
+            ettk_plan_destroy(plan_);  // the masks and filters are plan constants: rebuilt on next use
+            plan_ = nullptr;
+        }
+    }
+    void ensure_plan_() {
+        if (plan_) return;
+        p_.B = B;
+        check(ettk_plan_create(ctx_, &p_, &plan_), "ettk_plan_create");
+    }
+    // device base of the selected volumes, contiguous in the given order (gathered when they are not)
+    const float* gather_(const std::vector<WedgedSubImage3D>& vols, const std::vector<int>& idx,
+                         std::vector<ettk_particle>& descs) {
+        const size_t n3 = (size_t)p_.N * p_.N * p_.N;
+        descs.resize(idx.size());
+        bool contiguous = true;
+        for (size_t q = 0; q < idx.size(); q++) {
+            descs[q] = vols[idx[q]].desc;
+            if (vols[idx[q]].d_volume != vols[idx[0]].d_volume + q * n3) contiguous = false;
+        }
+        if (contiguous) return vols[idx[0]].d_volume;
+        const size_t bytes = idx.size() * n3 * sizeof(float);
+        if (bytes > scratch_bytes_) {
+            if (scratch_) check_hip(hipFree(scratch_), "hipFree");
+            scratch_ = nullptr;
+            check_hip(hipMalloc(&scratch_, bytes), "hipMalloc");
+            scratch_bytes_ = bytes;
+        }
+        for (size_t q = 0; q < idx.size(); q++)
+            check_hip(hipMemcpy(scratch_ + q * n3, vols[idx[q]].d_volume, n3 * sizeof(float), hipMemcpyDeviceToDevice),
+                      "hipMemcpy");
+        return scratch_;
+    }
+
+    ettk_ctx* ctx_ = nullptr;
+    ettk_plan* plan_ = nullptr;
+    ettk_params p_{};
+    int form_ = 1;
+    float* scratch_ = nullptr;
+    size_t scratch_bytes_ = 0;
+};
+
+}  // namespace ettk

@@ -622,4 +622,40 @@ void orc_local_maxima(const double* C, int n, double* out) {
     std::memcpy(out, o.data(), o.size() * sizeof(double));
 }
 
+// getDistances worker semantics by metric type (CorrMetric.h:1911-1990): 0 search + refine, 3 search only,
+// 1 refine from the stored transform (FIM.h:1812-1840), 2 one CCC at the identity (Core.h:35-66)
+void orc_distance(void* h, const float* part, const ettk_particle* d, int type, ettk_result* out) {
+    Plan& P = *(Plan*)h;
+    if (type == 0 || type == 3) {
+        const int saved = P.p.refine;
+        P.p.refine = type == 0 ? 1 : 0;
+        align_pair(P, part, *d, *out);
+        P.p.refine = saved;
+        return;
+    }
+    std::memset(out, 0, sizeof(*out));
+    const double rot = std::max((double)P.p.rot_restrict, 1.0);
+    const double tr = P.p.trans_restrict < 0 ? (double)FLT_MAX : (double)P.p.trans_restrict;
+    const int c0 = P.ccc_count;
+    M4 r;
+    float D;
+    if (type == 2) {
+        M4 t = m4_ident();
+        D = ccc_eval(P, part, *d, t);
+        r = transform_valid(t, tr, rot) ? t : m4_ident();
+    } else {
+        ettk_particle dz = *d;
+        for (int k = 0; k < 16; k++) dz.matrix[k] = (k % 5 == 0) ? 1.0 : 0.0;
+        r = m4_from(d->matrix);
+        powell_refine(P, part, dz, r, true);
+        D = P.last_peak;
+    }
+    out->distance = D;
+    out->cand_distance[0] = D;
+    std::memcpy(out->matrix, r.a, sizeof(r.a));
+    std::memcpy(out->cand_matrix[0], r.a, sizeof(r.a));
+    out->n_cand = 1;
+    out->n_ccc = P.ccc_count - c0;
+}
+
 }  // extern "C"

@@ -40,6 +40,7 @@ def oracle() -> C.CDLL:
         o.orc_wedge_half.argtypes = [C.c_int, C.POINTER(Particle), C.c_void_p, C.c_void_p]
         o.orc_spherical_wedge.argtypes = [C.c_int, C.POINTER(Particle), C.c_int, C.c_void_p]
         o.orc_masks.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        o.orc_distance.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Particle), C.c_int, C.POINTER(Result)]
         o.orc_ccc_count.restype = C.c_int
         o.orc_ccc_count.argtypes = [C.c_void_p]
         o.orc_average.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(Particle), C.c_void_p, C.c_void_p,
@@ -100,6 +101,12 @@ class OraclePlan:
         r = Result()
         vol = np.ascontiguousarray(vol, dtype=np.float32)
         self.o.orc_align(self.h, vp(vol), C.byref(desc), C.byref(r))
+        return r
+
+    def distance(self, vol: np.ndarray, desc: Particle, type: int) -> Result:
+        r = Result()
+        vol = np.ascontiguousarray(vol, dtype=np.float32)
+        self.o.orc_distance(self.h, vp(vol), C.byref(desc), int(type), C.byref(r))
         return r
 
     def align_many(self, vols: np.ndarray, descs, nthreads: int = 0):

@@ -1,0 +1,104 @@
+"""getDistances metric types through ettk_distances_batch against the FP64 oracle.
+
+type 2 (translation search at the stored alignment), type 1 (Powell refinement from the stored transform),
+type 3 (SO(3) search without refinement); plain vs batched forms give identical results.
+"""
+import numpy as np
+import pytest
+
+from conftest import to_dev
+from ettk_amd.abi import Particle, default_params
+from ettk_amd.synth import blob_reference, make_particles, rotation_error_deg
+from oracle_lib import OraclePlan
+
+
+def _setup(torch_gpu, ctx, n=4, seed=71, N=32, B=16, **kw):
+    from ettk_amd.lib import Plan
+
+    p = default_params(N, B, **kw)
+    ref = blob_reference(N, seed=seed)
+    ps = make_particles(ref, n, seed=seed + 1, half_wedge=(50, 65), snr=(0.5, 0.5))
+    plan = Plan(ctx, p)
+    plan.set_reference(to_dev(torch_gpu, ref))
+    orc = OraclePlan(p)
+    orc.set_reference(ref)
+    return p, ps, plan, orc, to_dev(torch_gpu, ps.vols)
+
+
+def _with_stored(ps, mats):
+    """Descriptors carrying a stored alignment (nbfWedgedSubImage3D::setTransform)."""
+    out = []
+    for d, m in zip(ps.descs, mats):
+        out.append(Particle(lwedge=d.lwedge, uwedge=d.uwedge, matrix=m))
+    return out
+
+
+@pytest.mark.gpu
+def test_type2_shift_only_matches_oracle(torch_gpu, ctx):
+    print("[distances] type 2", flush=True)
+    p, ps, plan, orc, d = _setup(torch_gpu, ctx)
+    try:
+        # stored alignment = the true rotation with a shift off by a few voxels
+        mats = ps.truth.copy()
+        mats[:, :3, 3] += np.array([[1.5, -2.0, 0.5], [0.0, 3.0, -1.0], [-2.5, 0.5, 2.0], [1.0, 1.0, 1.0]])
+        descs = _with_stored(ps, mats)
+        r0 = plan.distances(d, descs, type=2, form=0)
+        r1 = plan.distances(d, descs, type=2, form=1)
+        for i in range(len(descs)):
+            ro = orc.distance(ps.vols[i], descs[i], 2)
+            assert abs(r0[i].distance - ro.distance) <= 2e-3 * abs(ro.distance) + 1e-7
+            assert np.max(np.abs(r0[i].transform() - ro.transform())) <= 0.5 + 1e-9
+            assert r0[i].distance == r1[i].distance and np.array_equal(r0[i].transform(), r1[i].transform())
+            assert r0[i].n_ccc == 1
+    finally:
+        plan.close()
+
+
+@pytest.mark.gpu
+def test_type1_refine_from_stored_matches_oracle(torch_gpu, ctx):
+    print("[distances] type 1", flush=True)
+    p, ps, plan, orc, d = _setup(torch_gpu, ctx, n=3, seed=81)
+    try:
+        g = np.random.default_rng(3)
+        mats = ps.truth.copy()
+        for m in mats:  # a few degrees off about z, shift kept
+            a = np.radians(g.uniform(2, 4))
+            Rz = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+            m[:3, :3] = m[:3, :3] @ Rz
+        descs = _with_stored(ps, mats)
+        r0 = plan.distances(d, descs, type=1, form=0)
+        r1 = plan.distances(d, descs, type=1, form=1)
+        exact = 0
+        for i in range(len(descs)):
+            ro = orc.distance(ps.vols[i], descs[i], 1)
+            assert rotation_error_deg(r0[i].transform(), ro.transform()) <= 0.5
+            assert np.max(np.abs(r0[i].transform()[:3, 3] - ro.transform()[:3, 3])) <= 0.5 + 1e-6
+            assert abs(r0[i].distance - ro.distance) <= 2e-3 * abs(ro.distance) + 1e-7
+            exact += r0[i].n_ccc == ro.n_ccc
+            assert r0[i].distance == r1[i].distance and np.array_equal(r0[i].transform(), r1[i].transform())
+        assert exact >= len(descs) - 1
+        # the adjoint form ascends the continuous score from the same stored transforms
+        r2 = plan.distances(d, descs, type=1, form=2)
+        plain = [Particle(lwedge=x.lwedge, uwedge=x.uwedge) for x in descs]
+        idx = list(range(len(descs)))
+        s_stored, _ = plan.ccc_grad(d, plain, idx, mats, np.zeros((len(descs), 6)))
+        s_refined, _ = plan.ccc_grad(d, plain, idx, np.array([x.transform() for x in r2]), np.zeros((len(descs), 6)))
+        assert np.all(s_refined >= s_stored), (s_refined, s_stored)
+        assert all(x.n_ccc == 1 for x in r2)
+    finally:
+        plan.close()
+
+
+@pytest.mark.gpu
+def test_type3_search_only_matches_oracle(torch_gpu, ctx):
+    print("[distances] type 3", flush=True)
+    p, ps, plan, orc, d = _setup(torch_gpu, ctx, n=3, seed=91)
+    try:
+        r = plan.distances(d, ps.descs, type=3, form=1)
+        for i in range(len(r)):
+            ro = orc.distance(ps.vols[i], ps.descs[i], 3)
+            assert r[i].n_ccc == ro.n_ccc
+            assert rotation_error_deg(r[i].transform(), ro.transform()) <= 0.5
+            assert abs(r[i].distance - ro.distance) <= 2e-3 * abs(ro.distance) + 1e-7
+    finally:
+        plan.close()

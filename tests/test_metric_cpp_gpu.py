@@ -1,0 +1,83 @@
+"""The C++ drop-in (include/ettk_metric.hpp) driven like nbfCorrelationImageMetric by its callers:
+getDistances with a resumed entry, getImage and the multi-reference assignment, checked against the same
+operations through the Python binding of the C ABI."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import to_dev
+from ettk_amd.abi import Particle, default_params
+from ettk_amd.synth import blob_reference, make_particles
+
+EXE = Path(__file__).resolve().parent / "cpp" / "test_metric"
+
+
+@pytest.mark.gpu
+def test_cpp_metric_drop_in(torch_gpu, ctx, tmp_path):
+    from ettk_amd.lib import Plan
+
+    if not EXE.exists():
+        pytest.fail("tests/cpp/test_metric is not built: run the build entry point")
+    print("[cpp metric] getDistances / getImage / mraAssignment", flush=True)
+    N, B = 32, 16
+    refs = [blob_reference(N, seed=101), blob_reference(N, seed=202)]
+    pa = make_particles(refs[0], 4, seed=103, half_wedge=(50, 65), snr=(0.5, 0.5))
+    pb = make_particles(refs[1], 2, seed=203, half_wedge=(50, 65), snr=(0.5, 0.5))
+    vols = np.concatenate([pa.vols, pb.vols])
+    descs = pa.descs + pb.descs
+    n, R = len(descs), 2
+    np.stack(refs).astype(np.float32).tofile(tmp_path / "refs.bin")
+    vols.astype(np.float32).tofile(tmp_path / "parts.bin")
+    with open(tmp_path / "in.txt", "w") as f:
+        f.write(f"{N} {B} {n} {R}\n")
+        for d in descs:
+            f.write(f"{d.lwedge} {d.uwedge}\n")
+    r = subprocess.run([str(EXE), str(tmp_path)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    D0 = np.loadtxt(tmp_path / "out_D0.txt").astype(np.float32).reshape(R, n, 17)  # 9 digits: exact float32
+    p = default_params(N, B, n_candidates=1, refine=1)
+    plan = Plan(ctx, p)
+    try:
+        d = to_dev(torch_gpu, vols)
+        for ri in range(R):
+            plan.set_reference(to_dev(torch_gpu, refs[ri]))
+            res = plan.distances(d, descs, type=0, form=1)
+            for i in range(n):
+                if ri == 0 and i == 1:  # resumed entry: kept, not recomputed
+                    assert D0[ri, i, 0] == 5.0 and np.all(D0[ri, i, 1:] == -1.0)
+                    continue
+                assert D0[ri, i, 0] == np.float32(res[i].distance)
+                assert np.array_equal(D0[ri, i, 1:], res[i].transform().ravel().astype(np.float32))
+
+        # getImage: unit weights on the type-0 alignments to reference 0, the resumed entry weighted 0
+        w = np.ones(n, np.float32)
+        w[1] = 0.0
+        mats = D0[0, :, 1:].astype(np.float32).astype(np.float64).reshape(n, 4, 4)
+        mats[1] = np.eye(4)
+        out = torch_gpu.zeros((N, N, N), dtype=torch_gpu.float32, device="cuda")
+        plan.average(d, descs, w, mats, out)
+        avg = np.fromfile(tmp_path / "out_avg.bin", dtype=np.float32).reshape(N, N, N)
+        assert np.abs(avg - out.cpu().numpy()).max() <= 1e-5
+
+        # multi-reference assignment: sums over references of zero-shift distances under each alignment
+        lines = open(tmp_path / "out_mra.txt").read().split("\n")
+        best = np.array([int(x) for x in lines[:n]])
+        A = np.array([[float(v) for v in ln.split()] for ln in lines[n:] if ln.strip()], np.float32).reshape(R, n, 17)
+        plan.set_restrictions(180.0, 0.0)
+        sums = np.zeros((n, R))
+        for j in range(R):
+            dj = [Particle(lwedge=descs[i].lwedge, uwedge=descs[i].uwedge,
+                           matrix=A[j, i, 1:].astype(np.float64).reshape(4, 4))
+                  for i in range(n)]
+            for ri in range(R):
+                plan.set_reference(to_dev(torch_gpu, refs[ri]))
+                rr = plan.distances(d, dj, type=2, form=1)
+                sums[:, j] += [x.distance for x in rr]
+        assert np.array_equal(best, np.argmin(sums, axis=1)), (best, sums)
+        for i in range(n):
+            assert A[best[i], i, 0] < 0
+    finally:
+        plan.close()
