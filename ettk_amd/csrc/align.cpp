@@ -15,6 +15,8 @@
 #include <map>
 #include <mutex>
 #include <queue>
+#include <future>
+#include <memory>
 #include <thread>
 
 #include "so3.hpp"
@@ -143,38 +145,86 @@ struct SearchOut {
     int use_window = 0;
 };
 
-// projections + SO(3) correlations + local maxima for particles [p0, p0+cn)
-static void search_chunk(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int p0, int cn,
-                         std::vector<SearchOut>& out) {
+// Host-side inputs of the search for every particle of a call, computed on all host threads before the first
+// launch: smoothed spherical wedge W2 (nbfWedge3D::getSphericalImage + conditionSphericalWindow), whether it
+// is "effective" (sum <= 0.95 size, ProjRot.h:526-540), and the reslice geometry of the projection.
+struct SearchInputs {
+    std::vector<float> W2;
+    std::vector<int> uw2;
+    std::vector<EvalGeom> geom;
+};
+
+static void search_inputs(const ettk_plan* P, const ettk_particle* descs, int n, SearchInputs& in) {
+    const int B = P->p.B, nn = 4 * B * B;
+    in.W2.assign((size_t)n * nn, 0.0f);
+    in.uw2.assign(n, 0);
+    in.geom.resize(n);
+    auto work = [&](int i) {
+        std::vector<float> w = spherical_wedge_host(B, descs[i], P->p.missing_wedge != 0);
+        double sw = 0;
+        for (float v : w) sw += v;
+        in.uw2[i] = P->p.missing_wedge && (sw <= 0.95f * (float)nn);
+        std::copy(w.begin(), w.end(), in.W2.begin() + (size_t)i * nn);
+        in.geom[i] = make_geom_public(P, descs[i], Mat4::ident(), i);
+    };
+    const int nthr = std::max(1, std::min<int>(n, (int)std::thread::hardware_concurrency()));
+    if (nthr <= 1 || n < 4) {
+        for (int i = 0; i < n; i++) work(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int w = 0; w < nthr; w++)
+        th.emplace_back([&, w] {
+            for (int i = w; i < n; i += nthr) work(i);
+        });
+    for (auto& t : th) t.join();
+}
+
+// One chunk of the search in flight: the host sources of its uploads (kept alive until the chunk is done),
+// its local-maximum buffers on the device (a slot of the plan's pool) and, on the host, the downloaded lists.
+struct SearchLists {
+    int p0 = 0, cn = 0;
+    bool uw1 = false;
+    std::vector<int> uw2;
+    std::vector<SoftInst> inst;
+    std::vector<ComboOp> ops;
+    LmBuffers* lm = nullptr;
+    hipEvent_t done = nullptr;
+    ~SearchLists() {
+        if (done) (void)hipEventDestroy(done);
+    }
+};
+static constexpr int kLmCap = 16384;  // local maxima kept per correlation volume
+
+// GPU part of the search for particles [p0, p0+cn): projections, SO(3) correlations, local maxima.  Nothing
+// waits here: L.done marks the end of the chunk's kernels, the next chunk is enqueued right behind it.
+static void search_gpu(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int p0, int cn,
+                       const SearchInputs& in, int slot, SearchLists& L) {
     So3State& S = so3_state(P);
     const int B = P->p.B, n = 2 * B, nn = n * n;
     const long n3 = (long)n * n * n;
     hipStream_t s = P->ctx->stream;
     KernelProf* kp = P->ctx->fft.prof;
-    auto tg0 = std::chrono::steady_clock::now();
+    L.p0 = p0;
+    L.cn = cn;
+    L.uw2.assign(in.uw2.begin() + p0, in.uw2.begin() + p0 + cn);
+    L.uw1 = S.use_window1;
+    L.lm = &lm_slot(S, slot, 2 * cn, kLmCap);
     S.Pp2.ensure((size_t)cn * nn);
-    S.W2.ensure((size_t)cn * nn);
     S.Mx.ensure(cn);
-    std::vector<int> pidx(cn);
-    for (int q = 0; q < cn; q++) pidx[q] = p0 + q;
-    project_images(P, d_parts, descs, pidx.data(), cn, false, S.Pp2.p);
+    project_images(P, d_parts, descs, nullptr, cn, false, S.Pp2.p, in.geom.data() + p0);
     launch_ppmax(S.Pp1.p, S.Pp2.p, cn, B, S.Mx.p, s);
-    std::vector<float> W2all((size_t)cn * nn);
-    std::vector<int> uw2(cn);
-    for (int q = 0; q < cn; q++) {
-        std::vector<float> w = spherical_wedge_host(B, descs[p0 + q], P->p.missing_wedge != 0);
-        double sw = 0;
-        for (float v : w) sw += v;
-        uw2[q] = P->p.missing_wedge && (sw <= 0.95f * (float)nn);
-        std::copy(w.begin(), w.end(), W2all.begin() + (size_t)q * nn);
-    }
-    S.W2.upload(W2all.data(), W2all.size(), s);
-    const bool uw1 = S.use_window1;
-    std::vector<SoftInst> inst;
-    std::vector<ComboOp> ops(2 * cn, ComboOp{CO_COPY, {0, 0, 0, 0}});
+    S.W2.ensure((size_t)cn * nn);
+    ETTK_HIP(hipMemcpyAsync(S.W2.p, in.W2.data() + (size_t)p0 * nn, (size_t)cn * nn * sizeof(float),
+                            hipMemcpyHostToDevice, s));
+    const bool uw1 = L.uw1;
+    std::vector<SoftInst>& inst = L.inst;
+    std::vector<ComboOp>& ops = L.ops;
+    inst.clear();
+    ops.assign(2 * cn, ComboOp{CO_COPY, {0, 0, 0, 0}});
     for (int q = 0; q < cn; q++) {
         int base = (int)inst.size();
-        if (uw1 && uw2[q]) {
+        if (uw1 && L.uw2[q]) {
             inst.push_back({q, RC_P1SQ_W1, RC_W2, 1});
             inst.push_back({q, RC_W1, RC_P2SQ_W2, 1});
             inst.push_back({q, RC_P1_W1, RC_P2_W2, 1});
@@ -188,7 +238,7 @@ static void search_chunk(ettk_plan* P, const float* d_parts, const ettk_particle
             inst.push_back({q, RC_P1, RC_P2, 0});
             ops[2 * q] = ComboOp{CO_DIV_SQRT, {base + 1, base, 0, 0}};
             ops[2 * q + 1] = ComboOp{CO_COPY, {base + 2, 0, 0, 0}};
-        } else if (uw2[q]) {
+        } else if (L.uw2[q]) {
             inst.push_back({q, RC_P1SQ, RC_W2, 1});
             inst.push_back({q, RC_P1, RC_P2_W2, 1});
             inst.push_back({q, RC_P1, RC_P2, 0});
@@ -204,43 +254,53 @@ static void search_chunk(ettk_plan* P, const float* d_parts, const ettk_particle
     S.C.ensure((size_t)nq * n3);
     S.Cc.ensure((size_t)2 * cn * n3);
     run_soft(P, inst, S.Pp2.p, S.W2.p, S.Mx.p, S.C.p);
-    S.ops.upload(ops.data(), ops.size(), s);
+    S.ops.ensure(ops.size());
+    ETTK_HIP(hipMemcpyAsync(S.ops.p, ops.data(), ops.size() * sizeof(ComboOp), hipMemcpyHostToDevice, s));
     double f = std::floor(P->rot_restrict * B / 90.0 - 1.0);
     int b0 = (int)std::max(1.0, std::min(f, (double)(n - 1)));
-    launch_combine(S.C.p, S.ops.p, 2 * cn, B, b0, S.Cc.p, s);
-    const int cap = 16384;
-    S.lm_count.ensure(2 * cn);
-    S.lm_idx.ensure((size_t)2 * cn * cap);
-    S.lm_val.ensure((size_t)2 * cn * cap);
-    ETTK_HIP(hipMemsetAsync(S.lm_count.p, 0, 2 * cn * sizeof(int), s));
-    launch_localmax(S.Cc.p, 2 * cn, B, cap, S.lm_count.p, S.lm_idx.p, S.lm_val.p, s);
-    std::vector<int> cnt(2 * cn);
-    ETTK_HIP(hipMemcpyAsync(cnt.data(), S.lm_count.p, 2 * cn * sizeof(int), hipMemcpyDeviceToHost, s));
-    ETTK_HIP(hipStreamSynchronize(s));
+    {
+        ProfScope ps(kp, "k_combine", (double)(nq + 2 * cn) * n3 * 8, s);
+        launch_combine(S.C.p, S.ops.p, 2 * cn, B, b0, S.Cc.p, s);
+    }
+    ETTK_HIP(hipMemsetAsync(L.lm->count.p, 0, 2 * cn * sizeof(int), s));
+    {
+        ProfScope ps(kp, "k_localmax", 2.0 * cn * n3 * 8, s);
+        launch_localmax(S.Cc.p, 2 * cn, B, kLmCap, L.lm->count.p, L.lm->idx.p, L.lm->val.p, s);
+    }
+    ETTK_HIP(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+    ETTK_HIP(hipEventRecord(L.done, s));
+}
+
+// Host part: download the chunk's maxima (after its kernels, on the null stream: the plan's stream does not
+// block on it) and extract the candidate lists of its volumes on several host threads.
+static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOut>& out) {
+    KernelProf* kp = P->ctx->fft.prof;
+    ETTK_HIP(hipEventSynchronize(L.done));
     auto th0 = std::chrono::steady_clock::now();
-    if (kp) kp->add_host("search_gpu_wall", std::chrono::duration<double, std::milli>(th0 - tg0).count());
+    const int p0 = L.p0, cn = L.cn;
+    const bool uw1 = L.uw1;
+    std::vector<int> cnt(2 * cn);
+    ETTK_HIP(hipMemcpy(cnt.data(), L.lm->count.p, 2 * cn * sizeof(int), hipMemcpyDeviceToHost));
     std::vector<std::vector<int>> idx(2 * cn);
     std::vector<std::vector<double>> val(2 * cn);
     for (int v = 0; v < 2 * cn; v++) {
         int q = v / 2;
-        bool two = uw1 || uw2[q];
+        bool two = uw1 || L.uw2[q];
         if (v % 2 == 1 && !two) continue;
-        ETTK_REQUIRE(cnt[v] <= cap, ETTK_ERR_UNSUPPORTED, "more SO(3) local maxima than the candidate list holds");
+        ETTK_REQUIRE(cnt[v] <= kLmCap, ETTK_ERR_UNSUPPORTED, "more SO(3) local maxima than the candidate list holds");
         idx[v].resize(cnt[v]);
         val[v].resize(cnt[v]);
         if (cnt[v] > 0) {
-            ETTK_HIP(hipMemcpyAsync(idx[v].data(), S.lm_idx.p + (size_t)v * cap, cnt[v] * sizeof(int), hipMemcpyDeviceToHost, s));
-            ETTK_HIP(hipMemcpyAsync(val[v].data(), S.lm_val.p + (size_t)v * cap, cnt[v] * sizeof(double), hipMemcpyDeviceToHost, s));
+            ETTK_HIP(hipMemcpy(idx[v].data(), L.lm->idx.p + (size_t)v * kLmCap, cnt[v] * sizeof(int), hipMemcpyDeviceToHost));
+            ETTK_HIP(hipMemcpy(val[v].data(), L.lm->val.p + (size_t)v * kLmCap, cnt[v] * sizeof(double), hipMemcpyDeviceToHost));
         }
         if (kp && kp->on) kp->add_host("lm_count", (double)cnt[v]);
     }
-    ETTK_HIP(hipStreamSynchronize(s));
     for (int q = 0; q < cn; q++) {
         SearchOut& o = out[p0 + q];
-        o.two = uw1 || uw2[q];
-        o.use_window = (uw1 ? 1 : 0) | (uw2[q] ? 2 : 0);
+        o.two = uw1 || L.uw2[q];
+        o.use_window = (uw1 ? 1 : 0) | (L.uw2[q] ? 2 : 0);
     }
-    // candidate lists of the volumes are independent: extract them on several host threads
     auto work = [&](int v) {
         int q = v / 2;
         if (v % 2 == 1 && !out[p0 + q].two) return;
@@ -261,6 +321,38 @@ static void search_chunk(ettk_plan* P, const float* d_parts, const ettk_particle
         for (auto& t : th) t.join();
     }
     if (kp) kp->add_host("search_host", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count());
+}
+
+// The search of particles [0, n): chunks of `chunk` particles enqueued back to back; each chunk's lists are
+// processed on a host thread as soon as its kernels are done, while later chunks run on the GPU.
+static void search_all(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, int chunk,
+                       std::vector<SearchOut>& out) {
+    KernelProf* kp = P->ctx->fft.prof;
+    auto ts0 = std::chrono::steady_clock::now();
+    SearchInputs in;
+    search_inputs(P, descs, n, in);
+    std::vector<std::unique_ptr<SearchLists>> lists;
+    std::vector<std::future<void>> pending;
+    std::exception_ptr err;
+    try {
+        for (int p0 = 0, k = 0; p0 < n; p0 += chunk, k++) {
+            lists.emplace_back(new SearchLists());
+            SearchLists* L = lists.back().get();
+            search_gpu(P, d_parts, descs, p0, std::min(chunk, n - p0), in, k, *L);
+            pending.push_back(std::async(std::launch::async, [P, L, &out] { search_host(P, *L, out); }));
+        }
+    } catch (...) {
+        err = std::current_exception();
+    }
+    for (auto& f : pending) {
+        try {
+            f.get();
+        } catch (...) {
+            if (!err) err = std::current_exception();
+        }
+    }
+    if (err) std::rethrow_exception(err);
+    if (kp) kp->add_host("search_stage_wall", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts0).count());
 }
 
 // ------------------------------------------------------------------ Powell (vnl_powell, maxfev = 1)
@@ -613,13 +705,16 @@ void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
             ccc_batch(P, d_parts, descs, 1, &pi, &t, &d);
             return d;
         };
+        SearchInputs in;
+        search_inputs(P, descs, n, in);
         for (int i = 0; i < n; i++) {
-            search_chunk(P, d_parts, descs, i, 1, so);
+            SearchLists L;
+            search_gpu(P, d_parts, descs, i, 1, in, 0, L);
+            search_host(P, L, so);
             candidate_stage(P, i, so[i], ev, out[i]);
         }
     } else {
-        const int CH = 32;
-        for (int p0 = 0; p0 < n; p0 += CH) search_chunk(P, d_parts, descs, p0, std::min(CH, n - p0), so);
+        search_all(P, d_parts, descs, n, 32, so);
         if (mode == 1) {
             int T = std::min(n, 1024);
             BatchEval be(P, d_parts, descs, T);
@@ -658,6 +753,8 @@ void refine_with_seeds(ettk_plan* P, const float* d_parts, const ettk_particle* 
 void refine_adjoint_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n,
                           std::vector<SearchOut*>& so, ettk_result* out) {
     std::vector<std::vector<Mat4>> seeds(n);
+    KernelProf* kp = P->ctx->fft.prof;
+    auto tc0 = std::chrono::steady_clock::now();
     {
         int T = std::min(n, 1024);
         BatchEval be(P, d_parts, descs, T);
@@ -686,7 +783,12 @@ void refine_adjoint_batch(ettk_plan* P, const float* d_parts, const ettk_particl
     if (nj == 0) return;
     std::vector<Mat4> pose(nj), fin(nj);
     std::vector<double> score(nj);
+    auto tr0 = std::chrono::steady_clock::now();
+    if (kp) kp->add_host("candidate_stage_wall", std::chrono::duration<double, std::milli>(tr0 - tc0).count());
     refine_with_seeds(P, d_parts, descs, nj, pidx.data(), sd.data(), 20, pose.data(), score.data());
+    if (kp)
+        kp->add_host("refine_wall",
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tr0).count());
     std::vector<float> dist(nj);
     fin = pose;
     ccc_batch(P, d_parts, descs, nj, pidx.data(), fin.data(), dist.data());

@@ -4,6 +4,8 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <list>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -74,15 +76,16 @@ struct KernelProf {
         double ms = 0;
         long long launches = 0;
     };
-    std::vector<std::pair<std::string, Rec>> recs;
+    std::mutex mu;  // host stages are recorded from worker threads while launches are being recorded
+    std::list<std::pair<std::string, Rec>> recs;  // stable addresses: scopes keep a pointer to their record
     std::vector<hipEvent_t> pool;
-    Rec& rec(const char* name) {
+    Rec& rec_locked(const char* name) {
         for (auto& r : recs)
             if (r.first == name) return r.second;
         recs.emplace_back(name, Rec{});
         return recs.back().second;
     }
-    hipEvent_t take() {
+    hipEvent_t take_locked() {
         if (!pool.empty()) {
             hipEvent_t e = pool.back();
             pool.pop_back();
@@ -94,6 +97,7 @@ struct KernelProf {
     }
     // fold recorded event pairs into totals (call after the stream has been synchronised)
     void collect() {
+        std::lock_guard<std::mutex> g(mu);
         for (auto& r : recs) {
             for (auto& p : r.second.ev) {
                 float ms = 0;
@@ -107,13 +111,27 @@ struct KernelProf {
     }
     void reset() {
         collect();
+        std::lock_guard<std::mutex> g(mu);
         recs.clear();
     }
     void add_host(const char* name, double ms) {
         if (!on) return;
-        Rec& r = rec(name);
+        std::lock_guard<std::mutex> g(mu);
+        Rec& r = rec_locked(name);
         r.ms += ms;
         r.launches++;
+    }
+    bool query(const char* name, double& ms, long long& launches, double& bytes) {
+        collect();
+        std::lock_guard<std::mutex> g(mu);
+        for (auto& r : recs)
+            if (r.first == name) {
+                ms = r.second.ms;
+                launches = r.second.launches;
+                bytes = r.second.bytes;
+                return true;
+            }
+        return false;
     }
 };
 struct ProfScope {
@@ -123,17 +141,27 @@ struct ProfScope {
     hipStream_t s;
     ProfScope(KernelProf* prof, const char* name, double bytes, hipStream_t st) : P(prof), s(st) {
         if (P && P->on) {
-            r = &P->rec(name);
-            r->bytes += bytes;
-            r->launches++;
-            b = P->take();
+            {
+                std::lock_guard<std::mutex> g(P->mu);
+                r = &P->rec_locked(name);
+                r->bytes += bytes;
+                r->launches++;
+                b = P->take_locked();
+            }
             ETTK_HIP(hipEventRecord(b, s));
         }
     }
     ~ProfScope() {
         if (r) {
-            hipEvent_t e = P->take();
-            if (hipEventRecord(e, s) == hipSuccess) r->ev.emplace_back(b, e);
+            hipEvent_t e;
+            {
+                std::lock_guard<std::mutex> g(P->mu);
+                e = P->take_locked();
+            }
+            if (hipEventRecord(e, s) == hipSuccess) {
+                std::lock_guard<std::mutex> g(P->mu);
+                r->ev.emplace_back(b, e);
+            }
         }
     }
 };

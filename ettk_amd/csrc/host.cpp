@@ -100,6 +100,8 @@ void smooth3(std::vector<float>& A, int n0, int n1, int n2, double sd, double rf
     int dims[3] = {n0, n1, n2};
     long st[3] = {(long)n1 * n2, n2, 1};
     int rad = (int)(sd * rf);
+    std::vector<double> g(2 * rad + 1);  // kernel taps, the same values for every voxel
+    for (int x = -rad; x <= rad; x++) g[x + rad] = std::exp(-(double)(x * x) / (sd * sd * 2.0));
     std::vector<float> o(A.size());
     for (int ax = 2; ax >= 0; ax--) {
         int n = dims[ax];
@@ -107,8 +109,8 @@ void smooth3(std::vector<float>& A, int n0, int n1, int n2, double sd, double rf
             int idx = (int)((b / st[ax]) % n);
             int lo = std::max(-rad, -idx), hi = std::min(rad, n - 1 - idx);
             double ks = 0, s = 0;
-            for (int x = lo; x <= hi; x++) ks += std::exp(-(double)(x * x) / (sd * sd * 2.0));
-            for (int x = lo; x <= hi; x++) s += std::exp(-(double)(x * x) / (sd * sd * 2.0)) / ks * A[b + x * st[ax]];
+            for (int x = lo; x <= hi; x++) ks += g[x + rad];
+            for (int x = lo; x <= hi; x++) s += g[x + rad] / ks * A[b + x * st[ax]];
             o[b] = (float)s;
         }
         A.swap(o);
@@ -570,15 +572,7 @@ int ettk_profile_query(ettk_ctx* ctx, const char* kernel, double* total_ms, long
     ETTK_HIP(hipStreamSynchronize(ctx->stream));
     double ms = 0, by = 0;
     long long nl = 0;
-    if (ctx->fft.prof) {
-        ctx->fft.prof->collect();
-        for (auto& r : ctx->fft.prof->recs)
-            if (r.first == kernel) {
-                ms = r.second.ms;
-                nl = r.second.launches;
-                by = r.second.bytes;
-            }
-    }
+    if (ctx->fft.prof) ctx->fft.prof->query(kernel, ms, nl, by);
     if (total_ms) *total_ms = ms;
     if (launches) *launches = nl;
     if (bytes) *bytes = by;

@@ -19,6 +19,9 @@ namespace ettk {
 
 static constexpr int kT = 256;
 
+// the fused synthesis + LDS FFT path holds one (2B)^2 FP64 complex slice in LDS: 2B in {16, 32, 64}
+static inline bool fused_synthesis(int B) { return B == 8 || B == 16 || B == 32; }
+
 // ============================================================== tables (host, FP64)
 static double lfact(int n) { return std::lgamma((double)n + 1.0); }
 
@@ -110,6 +113,21 @@ void so3_tables(ettk_plan* P) {
     }
     S.dtab.upload(D.data(), D.size(), s);
     S.doff.upload(off.data(), off.size(), s);
+    if (fused_synthesis(B)) {
+        // [beta][c1][l][c2] with wrapped order indices c (m = c < B ? c : c - n), zero where l < l0 or c == B:
+        // consecutive c2 are consecutive addresses for the synthesis loop over l
+        std::vector<double> D2((size_t)n * n * B * n, 0.0);
+        for (int kb = 0; kb < n; kb++)
+            for (int c1 = 0; c1 < n; c1++)
+                for (int c2 = 0; c2 < n; c2++) {
+                    if (c1 == B || c2 == B) continue;
+                    int m1 = c1 < B ? c1 : c1 - n, m2 = c2 < B ? c2 : c2 - n;
+                    int l0 = std::max(std::abs(m1), std::abs(m2));
+                    size_t o = (size_t)kb * tot + off[(size_t)(m1 + B - 1) * W + (m2 + B - 1)];
+                    for (int l = l0; l < B; l++) D2[(((size_t)kb * n + c1) * B + l) * n + c2] = D[o + (l - l0)];
+                }
+        S.dtab2.upload(D2.data(), D2.size(), s);
+    }
     std::vector<double2> tw(n);
     for (int t = 0; t < n; t++) tw[t] = make_double2(std::cos(2 * M_PI * t / n), -std::sin(2 * M_PI * t / n));
     S.tw.upload(tw.data(), tw.size(), s);
@@ -178,13 +196,20 @@ __device__ __forceinline__ float mag_at(const float* __restrict__ mag, int M, in
     return mag[((size_t)(M - i) * M + (M - j)) * Hm + (M - k)];
 }
 
+// One S^2 sample per group of kLanes consecutive threads; the radial shells r = bandL..bandU are split over the
+// lanes and summed with wavefront shuffles.  Tricubic Keys weights (a = -1/2) with the 2^-17 floor tolerance
+// and wrap-around taps, FP64 like the double image of the reference.
+static constexpr int kLanes = 8;
+__device__ __forceinline__ int wrap_idx(int x, int M) { return x < 0 ? x + M : (x >= M ? x - M : x); }
+
 __global__ void __launch_bounds__(kT) k_sph_sample(const float* __restrict__ mag, const float* __restrict__ st, int M,
                                                     int B, int bandL, int bandU, float* __restrict__ Pp) {
     const int b = blockIdx.y;
     const int n = 2 * B;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * n) return;
-    const int jp = t / n, kt = t % n;
+    const int t = blockIdx.x * (kT / kLanes) + threadIdx.x / kLanes;
+    const int lane = threadIdx.x % kLanes;
+    const bool live = t < n * n;
+    const int jp = live ? t / n : 0, kt = live ? t % n : 0;
     const float* mg = mag + (size_t)b * M * M * (M / 2 + 1);
     const float padc = st[b * 4 + 2];
     const float Bf = (float)B;
@@ -193,7 +218,7 @@ __global__ void __launch_bounds__(kT) k_sph_sample(const float* __restrict__ mag
     const double sp = sin(phi), cp = cos(phi), ct = cos(th), stt = sin(th);
     const double origin = -(M - 1) / 2.0 - 0.5;
     double sum = 0;
-    for (int r = bandL; r <= bandU; r++) {
+    for (int r = bandL + lane; live && r <= bandU; r += kLanes) {
         double p[3] = {r * sp * ct - origin, r * sp * stt - origin, r * cp - origin};
         int idx[3];
         double w[3][4];
@@ -209,22 +234,20 @@ __global__ void __launch_bounds__(kT) k_sph_sample(const float* __restrict__ mag
         }
         double v = 0;
         for (int cz = 0; cz < 4; cz++) {
-            int kz = ((idx[2] - 1 + cz) % M + M) % M;
+            const int kz = wrap_idx(idx[2] - 1 + cz, M);
             double vy = 0;
             for (int cy = 0; cy < 4; cy++) {
-                int jy = ((idx[1] - 1 + cy) % M + M) % M;
+                const int jy = wrap_idx(idx[1] - 1 + cy, M);
                 double vx = 0;
-                for (int cx = 0; cx < 4; cx++) {
-                    int ix = ((idx[0] - 1 + cx) % M + M) % M;
-                    vx += w[0][cx] * (double)mag_at(mg, M, ix, jy, kz, padc);
-                }
+                for (int cx = 0; cx < 4; cx++) vx += w[0][cx] * (double)mag_at(mg, M, wrap_idx(idx[0] - 1 + cx, M), jy, kz, padc);
                 vy += w[1][cy] * vx;
             }
             v += w[2][cz] * vy;
         }
         sum += v;
     }
-    Pp[(size_t)b * n * n + t] = (float)sum;
+    for (int o = kLanes / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, kLanes);
+    if (live && lane == 0) Pp[(size_t)b * n * n + t] = (float)sum;
 }
 
 // ============================================================== soft instances: assemble S^2 samples
@@ -420,6 +443,83 @@ __global__ void k_dft_m2(const double2* __restrict__ T2, const double2* __restri
             re += z.x * w.x - z.y * w.y;
         }
         C[(((size_t)q * n + a) * n + kb) * n + g] = re;
+    }
+}
+
+// In-place radix-2 FFT (e^{-2 pi i jk/n}) of n vectors of length n = 2^LOGN held in LDS: element e of vector v
+// at X[v * vs + e * es]; consecutive threads take consecutive vectors.
+template <int LOGN>
+__device__ __forceinline__ void fft_lds_f64(double2* X, int es, int vs, const double2* tw) {
+    constexpr int n = 1 << LOGN;
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+        const int v = t % n, i = t / n;
+        const int r = (int)(__brev((unsigned)i) >> (32 - LOGN));
+        if (r > i) {
+            double2 a = X[v * vs + i * es], b = X[v * vs + r * es];
+            X[v * vs + i * es] = b;
+            X[v * vs + r * es] = a;
+        }
+    }
+    __syncthreads();
+    for (int st = 1; st <= LOGN; st++) {
+        const int half = 1 << (st - 1);
+        for (int t = threadIdx.x; t < n * n / 2; t += blockDim.x) {
+            const int v = t % n, b = t / n;
+            const int j = b % half, k = (b / half) * 2 * half;
+            const double2 w = tw[j * (n >> st)];
+            const int i0 = v * vs + (k + j) * es, i1 = i0 + half * es;
+            const double2 u = X[i0], x = X[i1];
+            const double2 y = make_double2(w.x * x.x - w.y * x.y, w.x * x.y + w.y * x.x);
+            X[i0] = make_double2(u.x + y.x, u.y + y.y);
+            X[i1] = make_double2(u.x - y.x, u.y - y.y);
+        }
+        __syncthreads();
+    }
+}
+
+// Synthesis of one beta slice of one instance and its two DFTs, all in LDS (n = 2B <= 64):
+//   T1[m1][m2] = sum_l sig(l,-m1) (-1)^(m1+m2) conj(pat(l,-m2)) sqrt((2l+1)/8pi^2) d^l_{m1m2}(beta)
+//   C[q][alpha][beta][gamma] = Re sum_{m1,m2} T1[m1][m2] e^{-i m1 alpha} e^{-i m2 gamma}
+template <int LOGN>
+__global__ void __launch_bounds__(kT) k_so3_synth_fft(const double2* __restrict__ cs, const double2* __restrict__ cp,
+                                                       const double* __restrict__ dtab2, int B,
+                                                       const double2* __restrict__ tw, double* __restrict__ C) {
+    constexpr int n = 1 << LOGN, LD = n + 1;
+    extern __shared__ double2 X[];  // n * LD
+    __shared__ double2 stw[n];
+    const int q = blockIdx.x, kb = blockIdx.y;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = tw[t];
+    const double2* S = cs + (size_t)q * B * B;
+    const double2* Pt = cp + (size_t)q * B * B;
+    const double* D = dtab2 + (size_t)kb * n * B * n;
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+        const int c1 = t / n, c2 = t % n;
+        double re = 0, im = 0;
+        if (c1 != B && c2 != B) {
+            const int m1 = c1 < B ? c1 : c1 - n, m2 = c2 < B ? c2 : c2 - n;
+            const int am1 = m1 < 0 ? -m1 : m1, am2 = m2 < 0 ? -m2 : m2;
+            const int l0 = am1 > am2 ? am1 : am2;
+            const double* d = D + (size_t)c1 * B * n + c2;
+            for (int l = l0; l < B; l++) {
+                const double2 a = S[l * l + l - m1], b = Pt[l * l + l - m2];
+                const double pr = a.x * b.x + a.y * b.y, pi = a.y * b.x - a.x * b.y;
+                const double dv = d[(size_t)l * n];
+                re += pr * dv;
+                im += pi * dv;
+            }
+            if ((m1 + m2) & 1) {
+                re = -re;
+                im = -im;
+            }
+        }
+        X[c1 * LD + c2] = make_double2(re, im);
+    }
+    __syncthreads();
+    fft_lds_f64<LOGN>(X, LD, 1, stw);  // over m1 for every m2 -> alpha
+    fft_lds_f64<LOGN>(X, 1, LD, stw);  // over m2 for every alpha -> gamma
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+        const int a = t / n, g = t % n;
+        C[(((size_t)q * n + a) * n + kb) * n + g] = X[a * LD + g].x;
     }
 }
 
@@ -627,7 +727,7 @@ static int band_hi(const ettk_plan* P) {
 
 // projections of `n` images: is_ref -> R already holds the internal image (no reslice)
 void project_images(ettk_plan* P, const float* d_parts, const ettk_particle* descs, const int* pidx, int n, bool is_ref,
-                    float* d_Pp) {
+                    float* d_Pp, const EvalGeom* h_geom) {
     So3State& S = so3(P);
     const int N = P->N, M = 2 * N, B = S.B, Hm = N + 1;
     const size_t n3 = (size_t)N * N * N, nbig = (size_t)M * M * Hm;
@@ -646,11 +746,15 @@ void project_images(ettk_plan* P, const float* d_parts, const ettk_particle* des
         if (is_ref) {
             ETTK_HIP(hipMemcpyAsync(S.projR.p, P->ref.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, s));
         } else {
-            for (int q = 0; q < cn; q++) {
-                int pi = pidx ? pidx[c0 + q] : c0 + q;
-                g[q] = make_geom_public(P, descs[pi], Mat4::ident(), pi);
+            const EvalGeom* src = h_geom ? h_geom + c0 : g.data();
+            if (!h_geom) {
+                for (int q = 0; q < cn; q++) {
+                    int pi = pidx ? pidx[c0 + q] : c0 + q;
+                    g[q] = make_geom_public(P, descs[pi], Mat4::ident(), pi);
+                }
             }
-            ETTK_HIP(hipMemcpyAsync(S.projGeom.p, g.data(), cn * sizeof(EvalGeom), hipMemcpyHostToDevice, s));
+            ETTK_HIP(hipMemcpyAsync(S.projGeom.p, src, cn * sizeof(EvalGeom), hipMemcpyHostToDevice, s));
+            if (!h_geom) ETTK_HIP(hipStreamSynchronize(s));  // g is reused by the next sub-chunk
             launch_reslice(d_parts, S.projGeom.p, cn, N, S.projR.p, S.projPartials.p, nb_res, s);
         }
         k_proj_stats<<<cn, kT, 0, s>>>(S.projR.p, S.projPartials.p, nb_res, P->m1.p, N, P->p.mask_radius != 0.0f ? 1 : 0,
@@ -660,7 +764,8 @@ void project_images(ettk_plan* P, const float* d_parts, const ettk_particle* des
         fft3_r2c(P->ctx->fft, a, nullptr, s);
         k_magnitude<<<2048, kT, 0, s>>>(S.projF.p, S.projMag.p, (long)nbig * cn, P->p.mcf);
         ETTK_HIP(hipGetLastError());
-        dim3 grid((4 * B * B + kT - 1) / kT, cn);
+        dim3 grid((4 * B * B + kT / kLanes - 1) / (kT / kLanes), cn);
+        ProfScope ps(P->ctx->fft.prof, "k_sph_sample", (double)cn * ((double)nbig * 4 + 16.0 * B * B), s);
         k_sph_sample<<<grid, kT, 0, s>>>(S.projMag.p, S.projStats.p, M, B, band_lo(P), band_hi(P),
                                          d_Pp + (size_t)c0 * 4 * B * B);
         ETTK_HIP(hipGetLastError());
@@ -690,8 +795,6 @@ void soft_core(ettk_plan* P, const std::vector<SoftInst>& inst, double* d_C) {
     S.fm.ensure((size_t)nq * n * n);
     S.cs.ensure((size_t)nq * B * B);
     S.cp.ensure((size_t)nq * B * B);
-    S.T1.ensure((size_t)nq * n3);
-    S.T2.ensure((size_t)nq * n3);
     S.vals.ensure((size_t)2 * nq);
     S.pos.ensure((size_t)2 * nq);
     S.rot.ensure((size_t)2 * nq * B * B);
@@ -702,14 +805,29 @@ void soft_core(ettk_plan* P, const std::vector<SoftInst>& inst, double* d_C) {
     k_legendre_fwd<<<dim3(8, nq), kT, 0, s>>>(S.fm.p, S.legF.p, B, S.cs.p);
     k_ring_dft<<<dim3(16, nq), kT, 0, s>>>(S.pat.p, S.tw.p, B, S.fm.p);
     k_legendre_fwd<<<dim3(8, nq), kT, 0, s>>>(S.fm.p, S.legF.p, B, S.cp.p);
-    k_so3_synth<<<dim3(n, nq), kT, 0, s>>>(S.cs.p, S.cp.p, S.dtab.p, S.doff.p, S.d_slice, B, S.T1.p);
-    k_dft_m1<<<dim3(n, nq), kT, 0, s>>>(S.T1.p, S.tw.p, B, S.T2.p);
-    k_dft_m2<<<dim3(n, nq), kT, 0, s>>>(S.T2.p, S.tw.p, B, d_C);
+    if (fused_synthesis(B)) {
+        ProfScope ps(P->ctx->fft.prof, "k_so3_synth_fft", (double)nq * ((double)n3 * 8 + 32.0 * B * B), s);
+        const size_t shm = (size_t)n * (n + 1) * sizeof(double2);
+        switch (n) {
+            case 16: k_so3_synth_fft<4><<<dim3(nq, n), kT, shm, s>>>(S.cs.p, S.cp.p, S.dtab2.p, B, S.tw.p, d_C); break;
+            case 32: k_so3_synth_fft<5><<<dim3(nq, n), kT, shm, s>>>(S.cs.p, S.cp.p, S.dtab2.p, B, S.tw.p, d_C); break;
+            default: k_so3_synth_fft<6><<<dim3(nq, n), kT, shm, s>>>(S.cs.p, S.cp.p, S.dtab2.p, B, S.tw.p, d_C); break;
+        }
+    } else {
+        S.T1.ensure((size_t)nq * n3);
+        S.T2.ensure((size_t)nq * n3);
+        k_so3_synth<<<dim3(n, nq), kT, 0, s>>>(S.cs.p, S.cp.p, S.dtab.p, S.doff.p, S.d_slice, B, S.T1.p);
+        k_dft_m1<<<dim3(n, nq), kT, 0, s>>>(S.T1.p, S.tw.p, B, S.T2.p);
+        k_dft_m2<<<dim3(n, nq), kT, 0, s>>>(S.T2.p, S.tw.p, B, d_C);
+    }
     ETTK_HIP(hipGetLastError());
     bool any_scaled = false;
     for (auto& q : inst) any_scaled |= q.scaled != 0;
     if (any_scaled) {
-        k_extrema<<<nq, kT, 0, s>>>(d_C, n3, S.vals.p, S.pos.p);
+        {
+            ProfScope ps(P->ctx->fft.prof, "k_extrema", (double)nq * n3 * 8, s);
+            k_extrema<<<nq, kT, 0, s>>>(d_C, n3, S.vals.p, S.pos.p);
+        }
         size_t shm = (size_t)B * (2 * B - 1) * sizeof(double2);
         k_rotate_coef<<<dim3(nq, 2), kT, shm, s>>>(S.cp.p, S.pos.p, B, S.rot.p);
         k_legendre_inv<<<dim3(16, 2 * nq), kT, 0, s>>>(S.rot.p, S.legI.p, B, S.g.p);
@@ -749,6 +867,12 @@ void soft_given(ettk_plan* P, int nq, const double* h_sig, const double* h_pat, 
 static void so3_kernel_attributes(int B) {
     int shm = B * (2 * B - 1) * (int)sizeof(double2);
     ETTK_HIP(hipFuncSetAttribute((const void*)k_rotate_coef, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+    if (fused_synthesis(B)) {
+        const int n = 2 * B, fs = n * (n + 1) * (int)sizeof(double2);
+        const void* f = n == 16 ? (const void*)k_so3_synth_fft<4> : n == 32 ? (const void*)k_so3_synth_fft<5>
+                                                                            : (const void*)k_so3_synth_fft<6>;
+        ETTK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, fs));
+    }
 }
 
 void launch_combine(const double* C, const ComboOp* d_ops, int np, int B, int b0, double* out, hipStream_t s) {

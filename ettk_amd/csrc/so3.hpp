@@ -1,5 +1,7 @@
 // so3.hpp -- S^2 projection / SO(3) search state and launch wrappers.
 #pragma once
+#include <memory>
+
 #include "plan.hpp"
 
 namespace ettk {
@@ -19,11 +21,17 @@ struct ComboOp {
     int i[4];      // instance indices
 };
 
+// local-maximum lists of one search chunk (count per volume, then up to `cap` indices and values per volume)
+struct LmBuffers {
+    DevBuf<int> count, idx;
+    DevBuf<double> val;
+};
+
 struct So3State {
     int B = 0;
     int d_slice = 0;
     bool use_window1 = false;
-    DevBuf<double> legF, legI, dtab;
+    DevBuf<double> legF, legI, dtab, dtab2;
     DevBuf<int> doff;
     DevBuf<double2> tw;
     // reference
@@ -41,15 +49,25 @@ struct So3State {
     DevBuf<float> Pp2, W2, Mx;
     DevBuf<double> C, Cc;
     DevBuf<ComboOp> ops;
-    DevBuf<int> lm_count, lm_idx;
-    DevBuf<double> lm_val;
+    std::vector<std::unique_ptr<LmBuffers>> lm_pool;  // one slot per chunk in flight
 };
+
+inline LmBuffers& lm_slot(So3State& S, int slot, int nvol, int cap) {
+    while ((int)S.lm_pool.size() <= slot) S.lm_pool.emplace_back(new LmBuffers());
+    LmBuffers& b = *S.lm_pool[slot];
+    b.count.ensure(nvol);
+    b.idx.ensure((size_t)nvol * cap);
+    b.val.ensure((size_t)nvol * cap);
+    return b;
+}
 
 EvalGeom make_geom_public(const ettk_plan* P, const ettk_particle& d, const Mat4& t, int particle);
 // smoothed analytic spherical wedge (nbfWedge3D::getSphericalImage + conditionSphericalWindow), host FP64
 std::vector<float> spherical_wedge_host(int B, const ettk_particle& d, bool wedge_on);
+// `h_geom` (optional): reslice geometry of the n images on the host, kept alive by the caller until the stream
+// has consumed it; computed here when null
 void project_images(ettk_plan* P, const float* d_parts, const ettk_particle* descs, const int* pidx, int n, bool is_ref,
-                    float* d_Pp);
+                    float* d_Pp, const EvalGeom* h_geom = nullptr);
 void run_soft(ettk_plan* P, const std::vector<SoftInst>& inst, const float* d_Pp2, const float* d_W2, const float* d_Mx,
               double* d_C);
 void soft_given(ettk_plan* P, int nq, const double* h_sig, const double* h_pat, double* d_C);
