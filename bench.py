@@ -29,6 +29,25 @@ METRIC = "aligned particles/s, 64^3 B=32: batched+adjoint vs plain GPU at ≤0.5
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic_ratio(kernel: str):
+    """HBM traffic / algorithmic bytes of `kernel` from the committed rocprof counter summaries (FETCH_SIZE and
+    WRITE_SIZE passes, calibrated on a copy kernel); (None, None) when no summary covers the kernel."""
+    prof = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(prof):
+        return None, None
+    for f in sorted(os.listdir(prof)):
+        if not (f.startswith("pmc_traffic") and f.endswith(".md")):
+            continue
+        for line in open(os.path.join(prof, f)):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 6 and cells[0] == kernel:
+                try:
+                    return float(cells[5]), f"profiles/{f}"
+                except ValueError:
+                    pass
+    return None, None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -38,7 +57,12 @@ def main() -> int:
     ap.add_argument("--plain-n", type=int, default=16, help="particles timed in the plain form")
     ap.add_argument("--cpu-n", type=int, default=8, help="particles of the CPU oracle sample (0 = skip)")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--forms", default="plain,batched,adjoint",
+                    help="forms to run (the headline form 'adjoint' always runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one 256 MiB device copy (a known-size kernel for rocprof counter calibration)")
     args = ap.parse_args()
+    forms = set(args.forms.split(","))
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only (DESIGN.md)"}))
         return 2
@@ -48,6 +72,14 @@ def main() -> int:
     from ettk_amd.abi import default_params
     from ettk_amd.lib import Context, Plan
     from ettk_amd.synth import blob_reference, make_particles_device, rotation_error_deg
+
+    if args.copy_calibration:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     N, B = 64, 32
     p = default_params(N, B)
@@ -63,22 +95,25 @@ def main() -> int:
     plan.set_reference(d_ref)
 
     # ---------------------------------------------------------------- plain form on a subset
-    sub = d_parts[: args.plain_n].contiguous()
-    torch.cuda.synchronize()
-    plan.align(sub, descs[: args.plain_n], mode=0)  # warm-up
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    res_plain = plan.align(sub, descs[: args.plain_n], mode=0)
-    torch.cuda.synchronize()
-    plain_s = time.perf_counter() - t0
-    plain_pps = args.plain_n / plain_s
+    plain_pps, res_plain = None, []
+    if "plain" in forms:
+        sub = d_parts[: args.plain_n].contiguous()
+        torch.cuda.synchronize()
+        plan.align(sub, descs[: args.plain_n], mode=0)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res_plain = plan.align(sub, descs[: args.plain_n], mode=0)
+        torch.cuda.synchronize()
+        plain_pps = args.plain_n / (time.perf_counter() - t0)
 
     # ---------------------------------------------------------------- batched form (Powell, the reference method)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    res_b = plan.align(d_parts, descs, mode=1)
-    torch.cuda.synchronize()
-    batched_pps = args.n / (time.perf_counter() - t0)
+    batched_pps, res_b = None, []
+    if "batched" in forms:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res_b = plan.align(d_parts, descs, mode=1)
+        torch.cuda.synchronize()
+        batched_pps = args.n / (time.perf_counter() - t0)
 
     # ---------------------------------------------------------------- batched + adjoint form (headline)
     for _ in range(args.warmup):
@@ -94,7 +129,8 @@ def main() -> int:
         torch.cuda.synchronize()
         step_ms.append((time.perf_counter() - t0) * 1e3)
     kernels = {}
-    for name in ("k_fft_strided<64>", "k_r2c_rows<64>", "k_c2r_rows<64>", "k_reslice", "k_spec_filter", "k_mask_ref",
+    for name in ("k_fft_strided<64>", "k_r2c_plane<64>", "k_c2r_plane<64>", "k_r2c_rows<64>", "k_c2r_rows<64>",
+                 "k_reslice", "k_spec_filter", "k_mask_ref",
                  "k_combine_dftA", "k_dftBC_argmax", "k_fft_strided<128>", "k_r2c_rows<128>",
                  "k_reslice_ref", "k_refine_dot", "k_refine_adj", "k_refine_moments", "k_sph_sample",
                  "k_so3_synth_fft", "k_extrema", "k_combine", "k_localmax"):
@@ -114,19 +150,27 @@ def main() -> int:
     same = sum(1 for a, b in zip(res_plain, res_b[: args.plain_n])
                if a.n_ccc == b.n_ccc and np.array_equal(a.transform(), b.transform()))
     rot_err = np.array([rotation_error_deg(r.transform(), truth[i]) for i, r in enumerate(res)])
-    rot_err_b = np.array([rotation_error_deg(r.transform(), truth[i]) for i, r in enumerate(res_b)])
     sh_err = np.array([np.abs(r.transform()[:3, 3] - truth[i][:3, 3]).max() for i, r in enumerate(res)])
-    sh_err_b = np.array([np.abs(r.transform()[:3, 3] - truth[i][:3, 3]).max() for i, r in enumerate(res_b)])
     n_ccc = np.array([r.n_ccc for r in res])
-    n_ccc_b = np.array([r.n_ccc for r in res_b])
+    batched = None
+    if res_b:
+        rot_err_b = np.array([rotation_error_deg(r.transform(), truth[i]) for i, r in enumerate(res_b)])
+        sh_err_b = np.array([np.abs(r.transform()[:3, 3] - truth[i][:3, 3]).max() for i, r in enumerate(res_b)])
+        batched = {"particles_per_s": round(batched_pps, 3), "particles": args.n,
+                   "ccc_per_particle": round(float(np.mean([r.n_ccc for r in res_b])), 2),
+                   "rot_err_deg_median": round(float(np.median(rot_err_b)), 3),
+                   "shift_err_vox_median": round(float(np.median(sh_err_b)), 3)}
 
     # ---------------------------------------------------------------- roofline of the dominant kernel
     dom = max(kernels.items(), key=lambda kv: kv[1]["ms"])
     avg_ms = dom[1]["ms"] / dom[1]["launches"]
     bytes_per_launch = dom[1]["bytes"] / dom[1]["launches"]
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
+    ratio, traffic_src = measured_traffic_ratio(dom[0])
     roofline = {"bound": "hbm", "kernel": dom[0], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": round(ratio * bytes_per_launch) if ratio else None,
+                "traffic_source": traffic_src,
                 "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": bytes_per_launch}
 
     # ---------------------------------------------------------------- CPU baseline: FP64 oracle on host cores
@@ -163,20 +207,17 @@ def main() -> int:
         "config": {"workload": "plain GPU scale-up: 1024 particles 64^3, B=32, wedges +-U[45,70], SNR U[0.05,0.2]",
                    "particles": args.n, "box": N, "bandwidth": B, "form": "batched+adjoint",
                    "Kp": p.n_peaks, "Kc": p.n_candidates, "trans_restrict": p.trans_restrict},
-        "forms": {"plain": {"particles_per_s": round(plain_pps, 3), "particles": args.plain_n},
-                  "batched": {"particles_per_s": round(batched_pps, 3), "particles": args.n,
-                              "ccc_per_particle": round(float(n_ccc_b.mean()), 2),
-                              "rot_err_deg_median": round(float(np.median(rot_err_b)), 3),
-                              "shift_err_vox_median": round(float(np.median(sh_err_b)), 3)},
+        "forms": {"plain": {"particles_per_s": round(plain_pps, 3), "particles": args.plain_n} if plain_pps else None,
+                  "batched": batched,
                   "batched+adjoint": {"particles_per_s": round(value, 3), "particles": args.n,
                                       "ccc_per_particle": round(float(n_ccc.mean()), 2),
                                       "refine_iterations": 20,
                                       "rot_err_deg_median": round(float(np.median(rot_err)), 3),
                                       "shift_err_vox_median": round(float(np.median(sh_err)), 3),
                                       "frac_rot_err_below_5deg": round(float(np.mean(rot_err < 5.0)), 3)},
-                  "batched+adjoint_over_plain": round(value / plain_pps, 2),
-                  "batched_over_plain": round(batched_pps / plain_pps, 2),
-                  "plain_batched_identical": f"{same}/{args.plain_n}"},
+                  "batched+adjoint_over_plain": round(value / plain_pps, 2) if plain_pps else None,
+                  "batched_over_plain": round(batched_pps / plain_pps, 2) if (plain_pps and batched_pps) else None,
+                  "plain_batched_identical": f"{same}/{args.plain_n}" if (res_plain and res_b) else None},
         "kernels": kernels,
         "host_stages": host,
         "roofline": roofline,

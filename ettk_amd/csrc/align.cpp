@@ -551,18 +551,16 @@ static float refine_powell(const ettk_plan* P, int pi, Mat4& t, bool run, const 
     return last;
 }
 
-// candidate loop of nbfProjectionRotationMetric3D::execute (ProjRot.h:678-901).  With `seeds` set (batched
-// + adjoint mode) each of the Kc best candidates gets the refine(t, 0) evaluation only (one CCC at the candidate
-// rotation, FIM.h:1758-1766) and its transform is handed back as a seed of the continuous refinement.
-static void candidate_stage(const ettk_plan* P, int pi, SearchOut& so, const EvalFn& eval, ettk_result& out,
-                            std::vector<Mat4>* seeds = nullptr) {
+// candidate loop of nbfProjectionRotationMetric3D::execute (ProjRot.h:678-901), split in two parts:
+// peak_loop evaluates the direct and mirrored transform of the SO(3) peaks of each queue until Kp (Kp/3 for
+// the second queue) of them beat their mirror, then eliminates equivalent angles and returns the Kc best;
+// final_candidates refines them (Powell) or, with `seeds` set (batched + adjoint mode), gives each the
+// refine(t, 0) evaluation only (one CCC at the candidate rotation, FIM.h:1758-1766) and hands the transform
+// back as a seed of the continuous refinement.
+static void peak_loop(const ettk_plan* P, int pi, SearchOut& so, const EvalFn& eval, std::vector<Cand>& selected,
+                      int& nccc) {
     std::vector<Cand> combined;
     int nq = so.two ? 2 : 1;
-    int nccc = 0;
-    auto counted = [&](int p, Mat4& t) {
-        nccc++;
-        return eval(p, t);
-    };
     for (int q = 0; q < nq; q++) {
         CandQueue& pq = q == 0 ? so.pq1 : so.pq2;
         int peaks = P->p.n_peaks;
@@ -572,10 +570,12 @@ static void candidate_stage(const ettk_plan* P, int pi, SearchOut& so, const Eva
             Cand p = pq.top();
             if (p.v == FLT_MAX) break;
             Mat4 t1 = zyz(p.a, p.b, p.g), td = t1;
-            float D = counted(pi, td);
+            nccc++;
+            float D = eval(pi, td);
             if (transform_ok(td, P->trans_restrict, P->rot_restrict)) {
                 Mat4 mir = mat_scale(-1, -1, -1) * t1;
-                float Dm = counted(pi, mir);
+                nccc++;
+                float Dm = eval(pi, mir);
                 if (Dm >= D) index++;
                 combined.push_back(Cand{p.a, p.b, p.g, D});
             }
@@ -585,11 +585,23 @@ static void candidate_stage(const ettk_plan* P, int pi, SearchOut& so, const Eva
     if (combined.empty()) combined.push_back(Cand{0, 0, 0, 1.0f});
     CandQueue pqc;
     eliminate_equivalent(combined, pqc, 2.5f);
+    selected.clear();
+    while (!pqc.empty() && (int)selected.size() < P->p.n_candidates) {
+        selected.push_back(pqc.top());
+        pqc.pop();
+    }
+}
+
+static void final_candidates(const ettk_plan* P, int pi, const std::vector<Cand>& selected, SearchOut& so,
+                             const EvalFn& eval, ettk_result& out, int nccc, std::vector<Mat4>* seeds = nullptr) {
+    auto counted = [&](int p, Mat4& t) {
+        nccc++;
+        return eval(p, t);
+    };
     float best = FLT_MAX;
     Mat4 bestT = Mat4::ident();
     int index = 0;
-    while (!pqc.empty() && index < P->p.n_candidates) {
-        Cand p = pqc.top();
+    for (const Cand& p : selected) {
         Mat4 t = zyz(p.a, p.b, p.g);
         float dist = refine_powell(P, pi, t, seeds == nullptr && P->p.refine != 0, counted);
         if (seeds) seeds->push_back(t);
@@ -602,7 +614,6 @@ static void candidate_stage(const ettk_plan* P, int pi, SearchOut& so, const Eva
         out.so3_angles[index][0] = p.a;
         out.so3_angles[index][1] = p.b;
         out.so3_angles[index][2] = p.g;
-        pqc.pop();
         index++;
     }
     out.n_cand = index;
@@ -612,10 +623,104 @@ static void candidate_stage(const ettk_plan* P, int pi, SearchOut& so, const Eva
     out.use_window = so.use_window;
 }
 
-// ------------------------------------------------------------------ batched evaluation service
-// Every particle's control flow runs in its own host thread; eval() parks the request, and the last thread
-// to arrive runs one ccc_batch over all parked requests.  Each evaluation is independent of the others in
-// the batch, so every particle follows exactly the trajectory of the plain mode.
+static void candidate_stage(const ettk_plan* P, int pi, SearchOut& so, const EvalFn& eval, ettk_result& out,
+                            std::vector<Mat4>* seeds = nullptr) {
+    std::vector<Cand> selected;
+    int nccc = 0;
+    peak_loop(P, pi, so, eval, selected, nccc);
+    final_candidates(P, pi, selected, so, eval, out, nccc, seeds);
+}
+
+// The peak loop of every particle in lock-step rounds (batched modes).  A queue that still needs `need`
+// peaks beating their mirror will evaluate at least its next `need` peaks (each raises the count by at most
+// one), so each round evaluates those peaks, direct and mirrored, for all particles in one batch; the results
+// are then consumed in the order of the sequential loop.  A mirror whose direct transform turns out invalid
+// is discarded (and not counted), so every particle ends with the same candidates, distances and CCC count as
+// peak_loop.
+static void peak_rounds(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n,
+                        std::vector<SearchOut*>& so, std::vector<std::vector<Cand>>& selected, std::vector<int>& nccc) {
+    struct State {
+        int q = 0, index = 0, peaks = 0;
+        bool done = false;
+        std::vector<Cand> combined, popped;
+    };
+    std::vector<State> st(n);
+    for (int i = 0; i < n; i++) st[i].peaks = P->p.n_peaks;
+    nccc.assign(n, 0);
+    selected.assign(n, {});
+    auto queue = [&](int i) -> CandQueue& { return st[i].q == 0 ? so[i]->pq1 : so[i]->pq2; };
+    auto advance = [&](int i) {  // current queue finished: next queue or done
+        State& s = st[i];
+        if (s.q == 0 && so[i]->two) {
+            s.q = 1;
+            s.index = 0;
+            s.peaks = (int)std::max(1.0, std::floor(P->p.n_peaks / 3.0));
+        } else {
+            s.done = true;
+        }
+    };
+    std::vector<int> pidx;
+    std::vector<Mat4> t;
+    std::vector<float> d;
+    for (;;) {
+        pidx.clear();
+        t.clear();
+        for (int i = 0; i < n; i++) {
+            State& s = st[i];
+            s.popped.clear();
+            while (!s.done) {
+                CandQueue& pq = queue(i);
+                const int need = s.peaks - s.index;
+                while ((int)s.popped.size() < need && !pq.empty() && pq.top().v != FLT_MAX) {
+                    s.popped.push_back(pq.top());
+                    pq.pop();
+                }
+                if (!s.popped.empty()) break;
+                advance(i);  // nothing left to evaluate in this queue
+            }
+            for (const Cand& p : s.popped) {
+                Mat4 t1 = zyz(p.a, p.b, p.g);
+                pidx.push_back(i);
+                t.push_back(t1);
+                pidx.push_back(i);
+                t.push_back(mat_scale(-1, -1, -1) * t1);
+            }
+        }
+        if (pidx.empty()) break;
+        d.resize(pidx.size());
+        ccc_batch(P, d_parts, descs, (int)pidx.size(), pidx.data(), t.data(), d.data());
+        size_t r = 0;
+        for (int i = 0; i < n; i++) {
+            State& s = st[i];
+            if (s.popped.empty()) continue;
+            for (const Cand& p : s.popped) {
+                const float D = d[r];
+                const Mat4& td = t[r];
+                const float Dm = d[r + 1];
+                r += 2;
+                nccc[i]++;
+                if (transform_ok(td, P->trans_restrict, P->rot_restrict)) {
+                    nccc[i]++;
+                    if (Dm >= D) s.index++;
+                    s.combined.push_back(Cand{p.a, p.b, p.g, D});
+                }
+            }
+            CandQueue& pq = queue(i);
+            if (s.index >= s.peaks || pq.empty() || pq.top().v == FLT_MAX) advance(i);
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        std::vector<Cand>& combined = st[i].combined;
+        if (combined.empty()) combined.push_back(Cand{0, 0, 0, 1.0f});
+        CandQueue pqc;
+        eliminate_equivalent(combined, pqc, 2.5f);
+        while (!pqc.empty() && (int)selected[i].size() < P->p.n_candidates) {
+            selected[i].push_back(pqc.top());
+            pqc.pop();
+        }
+    }
+}
+
 class BatchEval {
    public:
     BatchEval(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n_threads)
@@ -716,6 +821,11 @@ void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
     } else {
         search_all(P, d_parts, descs, n, 32, so);
         if (mode == 1) {
+            std::vector<SearchOut*> sp(n);
+            for (int i = 0; i < n; i++) sp[i] = &so[i];
+            std::vector<std::vector<Cand>> sel;
+            std::vector<int> nccc;
+            peak_rounds(P, d_parts, descs, n, sp, sel, nccc);
             int T = std::min(n, 1024);
             BatchEval be(P, d_parts, descs, T);
             EvalFn ev = [&](int pi, Mat4& t) { return be.eval(pi, t); };
@@ -723,7 +833,7 @@ void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
             for (int w = 0; w < T; w++)
                 th.emplace_back([&, w] {
                     try {
-                        for (int i = w; i < n; i += T) candidate_stage(P, i, so[i], ev, out[i]);
+                        for (int i = w; i < n; i += T) final_candidates(P, i, sel[i], so[i], ev, out[i], nccc[i]);
                         be.done();
                     } catch (const Error& e) {
                         be.fail(e);
@@ -756,20 +866,32 @@ void refine_adjoint_batch(ettk_plan* P, const float* d_parts, const ettk_particl
     KernelProf* kp = P->ctx->fft.prof;
     auto tc0 = std::chrono::steady_clock::now();
     {
-        int T = std::min(n, 1024);
-        BatchEval be(P, d_parts, descs, T);
-        EvalFn ev = [&](int pi, Mat4& t) { return be.eval(pi, t); };
-        std::vector<std::thread> th;
-        for (int w = 0; w < T; w++)
-            th.emplace_back([&, w] {
-                try {
-                    for (int i = w; i < n; i += T) candidate_stage(P, i, *so[i], ev, out[i], &seeds[i]);
-                    be.done();
-                } catch (const Error& e) {
-                    be.fail(e);
-                }
-            });
-        for (auto& t : th) t.join();
+        std::vector<std::vector<Cand>> sel;
+        std::vector<int> nccc;
+        peak_rounds(P, d_parts, descs, n, so, sel, nccc);
+        // refine(t, 0) of every selected candidate in one batch
+        std::vector<int> pidx;
+        std::vector<Mat4> t;
+        for (int i = 0; i < n; i++)
+            for (const Cand& p : sel[i]) {
+                pidx.push_back(i);
+                t.push_back(strip_translation(zyz(p.a, p.b, p.g)));
+            }
+        std::vector<float> d(pidx.size());
+        if (!pidx.empty()) ccc_batch(P, d_parts, descs, (int)pidx.size(), pidx.data(), t.data(), d.data());
+        size_t r = 0;
+        for (int i = 0; i < n; i++) {
+            std::vector<size_t> mine;
+            for (size_t c = 0; c < sel[i].size(); c++) mine.push_back(r + c);
+            EvalFn ev = [&](int, Mat4& tt) {
+                const size_t k = mine.front();
+                mine.erase(mine.begin());
+                tt = t[k];
+                return d[k];
+            };
+            final_candidates(P, i, sel[i], *so[i], ev, out[i], nccc[i], &seeds[i]);
+            r += sel[i].size();
+        }
     }
     std::vector<int> pidx, cand;
     std::vector<Mat4> sd;

@@ -193,10 +193,38 @@ __device__ __forceinline__ void tap3(int idx, int n, float* w) {
     w[2] = r * inv;
 }
 __device__ float wedge_smooth(const float* v1, const float* v2, int i, int j, int k, int N, int H) {
+    // Away from both planes the 27 binary samples all equal the centre's: the smoothed value is then 0 or the
+    // kernel's sum, accumulated in the same order as the full stencil below (so the value is identical).
+    const float di = (float)(i - N / 2), dj = (float)(j - N / 2), dk = (float)(k - (H - 1));
+    const float a0 = v1[0] * di + v1[1] * dj + v1[2] * dk, b0 = v2[0] * di + v2[1] * dj + v2[2] * dk;
+    const float r1 = fabsf(v1[0]) + fabsf(v1[1]) + fabsf(v1[2]), r2 = fabsf(v2[0]) + fabsf(v2[1]) + fabsf(v2[2]);
+    const bool uniform = fabsf(a0) > r1 * 1.001f + 1e-3f && fabsf(b0) > r2 * 1.001f + 1e-3f;
+    if (uniform && a0 * b0 > 0.f) return 0.f;
     float wi[3], wj[3], wk[3];
     tap3(i, N, wi);
     tap3(j, N, wj);
     tap3(k, H, wk);
+    if (uniform) {
+        float acc = 0.f;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            if (wi[a] == 0.f) continue;
+            float sj = 0.f;
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                if (wj[b] == 0.f) continue;
+                float sk = 0.f;
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    if (wk[c] == 0.f) continue;
+                    sk += wk[c] * 1.f;
+                }
+                sj += wj[b] * sk;
+            }
+            acc += wi[a] * sj;
+        }
+        return acc;
+    }
     float acc = 0.f;
 #pragma unroll
     for (int a = -1; a <= 1; a++) {

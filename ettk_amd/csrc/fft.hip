@@ -294,8 +294,101 @@ __global__ void __launch_bounds__(Shape<L>::THREADS) k_c2r_rows(const float2* __
     }
 }
 
+// ------------------------------------------------------------------ fused plane passes (L <= 64)
+// One workgroup per (batch, i) plane of L x L reals: the k pass (pairs of real rows packed as complex rows) and
+// the j pass run back to back in LDS, so an r2c moves each element through HBM twice (plane pass + i pass)
+// instead of three times.  H pencil groups of TPP threads; the spare group of the k pass works on a dummy row.
+template <int L> struct Plane {
+    static constexpr int H = L / 2 + 1;
+    static constexpr int TPP = Shape<L>::TPP;
+    static constexpr int THREADS = H * TPP;
+    static constexpr int LD = L + 1;
+};
+
+template <int L>
+__global__ void __launch_bounds__(Plane<L>::THREADS) k_r2c_plane(R2CArgs a, const float2* __restrict__ twg) {
+    using PL = Plane<L>;
+    constexpr int H = PL::H, LD = PL::LD;
+    __shared__ float2 rows[H * LD];  // k pass: L/2 packed row pairs (+1 dummy)
+    __shared__ float2 cols[H * LD];  // j pass: H columns of length L
+    __shared__ float2 tw[L];
+    const long plane = blockIdx.x;   // b * L + i
+    for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
+    for (int t = threadIdx.x; t < L * L; t += blockDim.x) {
+        const int j = t / L, x = t % L;
+        float* f = reinterpret_cast<float*>(&rows[(j >> 1) * LD + x]);
+        f[j & 1] = r2c_load<L>(a, plane * L + j, x);
+    }
+    __syncthreads();
+    const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
+    lds_fft<L>(&rows[q * LD], lane, tw, false);
+    for (int t = threadIdx.x; t < (L / 2) * 2 * H; t += blockDim.x) {
+        const int pp = t / (2 * H), r = t % (2 * H), which = r / H, k = r % H;
+        const float2 z = rows[pp * LD + k], zc = cconj(rows[pp * LD + ((L - k) % L)]);
+        const float2 o = which == 0 ? cscale(cadd(z, zc), 0.5f) : make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
+        cols[k * LD + 2 * pp + which] = o;
+    }
+    __syncthreads();
+    lds_fft<L>(&cols[q * LD], lane, tw, false);
+    float2* o = a.out + plane * L * H;
+    for (int t = threadIdx.x; t < L * H; t += blockDim.x) {
+        const int j = t / H, k = t % H;
+        o[(long)j * H + k] = cols[k * LD + j];
+    }
+}
+
+template <int L>
+__global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_plane(const float2* __restrict__ in, float* __restrict__ out,
+                                                                   const float2* __restrict__ twg) {
+    using PL = Plane<L>;
+    constexpr int H = PL::H, LD = PL::LD;
+    __shared__ float2 cols[H * LD];
+    __shared__ float2 rows[H * LD];
+    __shared__ float2 tw[L];
+    const long plane = blockIdx.x;
+    for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
+    const float2* ip = in + plane * L * H;
+    for (int t = threadIdx.x; t < L * H; t += blockDim.x) {
+        const int j = t / H, k = t % H;
+        cols[k * LD + j] = ip[(long)j * H + k];
+    }
+    __syncthreads();
+    const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
+    lds_fft<L>(&cols[q * LD], lane, tw, true);
+    // Z(k) = A(k) + i B(k) of rows 2p, 2p+1 over the full length, A(L-k) = conj(A(k)), real A(0), A(L/2)
+    for (int t = threadIdx.x; t < (L / 2) * L; t += blockDim.x) {
+        const int pp = t / L, k = t % L;
+        float2 A, Bv;
+        if (k <= L / 2) {
+            A = cols[k * LD + 2 * pp];
+            Bv = cols[k * LD + 2 * pp + 1];
+            if (k == 0 || k == L / 2) {
+                A.y = 0.f;
+                Bv.y = 0.f;
+            }
+        } else {
+            A = cconj(cols[(L - k) * LD + 2 * pp]);
+            Bv = cconj(cols[(L - k) * LD + 2 * pp + 1]);
+        }
+        rows[pp * LD + k] = make_float2(A.x - Bv.y, A.y + Bv.x);
+    }
+    __syncthreads();
+    lds_fft<L>(&rows[q * LD], lane, tw, true);
+    float* op = out + plane * L * L;
+    for (int t = threadIdx.x; t < L * L; t += blockDim.x) {
+        const int j = t / L, x = t % L;
+        const float2 z = rows[(j >> 1) * LD + x];
+        op[t] = (j & 1) ? z.y : z.x;
+    }
+}
+
 // ------------------------------------------------------------------ host launchers
 // profile record names: k_r2c_rows<L>, k_fft_strided<L>, k_c2r_rows<L>
+template <int L>
+static const char* plane_name(int which) {
+    static const std::string n[2] = {"k_r2c_plane<" + std::to_string(L) + ">", "k_c2r_plane<" + std::to_string(L) + ">"};
+    return n[which].c_str();
+}
 template <int L>
 static const char* prof_name(int which) {
     static const std::string n[3] = {"k_r2c_rows<" + std::to_string(L) + ">", "k_fft_strided<" + std::to_string(L) + ">",
@@ -312,16 +405,22 @@ static void launch_r2c(const FftTables& T, const R2CArgs& a, float2* scratch, hi
     const double vol_c = (double)a.batch * L * L * H * sizeof(float2);  // one pass over the half spectrum
     const double vol_r = a.pro == PRO_PAD_MASKED ? (double)a.batch * a.ns * a.ns * a.ns * sizeof(float)
                                                  : (double)a.batch * L * L * L * sizeof(float);
-    // pass k: real rows -> a.out (half rows)
-    {
-        ProfScope ps(T.prof, prof_name<L>(0), vol_r + vol_c, s);
-        k_r2c_rows<L><<<(unsigned)((npairs + S::P - 1) / S::P), S::THREADS, 0, s>>>(a, tw);
-    }
-    // pass j: pencils over (b, i, k), element stride H
-    StridedArgs sj{a.out, a.out, (long)a.batch * L * H, (long)H, (long)L * H, (long)H, 0};
-    {
-        ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
-        k_fft_strided<L><<<(unsigned)((sj.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(sj, tw);
+    if constexpr (L <= 64) {
+        // passes k and j fused per (b, i) plane
+        ProfScope ps(T.prof, plane_name<L>(0), vol_r + vol_c, s);
+        k_r2c_plane<L><<<(unsigned)((long)a.batch * L), Plane<L>::THREADS, 0, s>>>(a, tw);
+    } else {
+        // pass k: real rows -> a.out (half rows)
+        {
+            ProfScope ps(T.prof, prof_name<L>(0), vol_r + vol_c, s);
+            k_r2c_rows<L><<<(unsigned)((npairs + S::P - 1) / S::P), S::THREADS, 0, s>>>(a, tw);
+        }
+        // pass j: pencils over (b, i, k), element stride H
+        StridedArgs sj{a.out, a.out, (long)a.batch * L * H, (long)H, (long)L * H, (long)H, 0};
+        {
+            ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
+            k_fft_strided<L><<<(unsigned)((sj.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(sj, tw);
+        }
     }
     // pass i: pencils over (b, j, k), element stride L*H
     StridedArgs si{a.out, a.out, (long)a.batch * L * H, (long)L * H, (long)L * L * H, (long)L * H, 0};
@@ -343,15 +442,20 @@ static void launch_c2r(const FftTables& T, const float2* in, float2* tmp, float*
         ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
         k_fft_strided<L><<<(unsigned)((si.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(si, tw);
     }
-    StridedArgs sj{tmp, tmp, (long)batch * L * H, (long)H, (long)L * H, (long)H, 1};
-    {
-        ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
-        k_fft_strided<L><<<(unsigned)((sj.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(sj, tw);
-    }
-    long npairs = (long)batch * L * L / 2;
-    {
-        ProfScope ps(T.prof, prof_name<L>(2), vol_c + vol_r, s);
-        k_c2r_rows<L><<<(unsigned)((npairs + S::P - 1) / S::P), S::THREADS, 0, s>>>(tmp, out, npairs, tw);
+    if constexpr (L <= 64) {
+        ProfScope ps(T.prof, plane_name<L>(1), vol_c + vol_r, s);
+        k_c2r_plane<L><<<(unsigned)((long)batch * L), Plane<L>::THREADS, 0, s>>>(tmp, out, tw);
+    } else {
+        StridedArgs sj{tmp, tmp, (long)batch * L * H, (long)H, (long)L * H, (long)H, 1};
+        {
+            ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
+            k_fft_strided<L><<<(unsigned)((sj.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(sj, tw);
+        }
+        long npairs = (long)batch * L * L / 2;
+        {
+            ProfScope ps(T.prof, prof_name<L>(2), vol_c + vol_r, s);
+            k_c2r_rows<L><<<(unsigned)((npairs + S::P - 1) / S::P), S::THREADS, 0, s>>>(tmp, out, npairs, tw);
+        }
     }
 }
 

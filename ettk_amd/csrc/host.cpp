@@ -3,6 +3,7 @@
 #include <chrono>
 #include <cmath>
 #include <mutex>
+#include <thread>
 
 #include "plan.hpp"
 
@@ -311,22 +312,37 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
             if (kp) kp->add_host("ccc_batch_wall", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
         }
     } wall_rec{P->ctx->fft.prof, tw0};
-    std::vector<EvalGeom> g;
+    // geometry of every evaluation up front (one upload), then all chunks back to back without host waits;
+    // one download of the results at the end
+    std::vector<EvalGeom> g(n);
+    auto geom_of = [&](int e) { g[e] = make_geom_public(P, descs[pidx[e]], t[e], pidx[e]); };
+    const int nthr = std::max(1, std::min<int>(n / 256, (int)std::thread::hardware_concurrency()));
+    if (nthr <= 1) {
+        for (int e = 0; e < n; e++) geom_of(e);
+    } else {
+        std::vector<std::thread> th;
+        for (int w = 0; w < nthr; w++)
+            th.emplace_back([&, w] {
+                for (int e = w; e < n; e += nthr) geom_of(e);
+            });
+        for (auto& x : th) x.join();
+    }
+    P->geom_all.ensure(n);
+    P->out_all.ensure((size_t)4 * n);
+    ETTK_HIP(hipMemcpyAsync(P->geom_all.p, g.data(), (size_t)n * sizeof(EvalGeom), hipMemcpyHostToDevice, s));
+    ETTK_HIP(hipEventRecord(P->ev0, s));
+    const float* W1 = P->p.missing_wedge ? P->W1.p : nullptr;
+    const int apply = P->dummy_filter ? 0 : 1;
+    KernelProf* kp = P->ctx->fft.prof;
     for (int e0 = 0; e0 < n; e0 += P->cap) {
-        int E = std::min(P->cap, n - e0);
-        g.resize(E);
-        for (int e = 0; e < E; e++) g[e] = make_geom_public(P, descs[pidx[e0 + e]], t[e0 + e], pidx[e0 + e]);
-        ETTK_HIP(hipMemcpyAsync(P->geom.p, g.data(), E * sizeof(EvalGeom), hipMemcpyHostToDevice, s));
-        ETTK_HIP(hipEventRecord(P->ev0, s));
-        const float* W1 = P->p.missing_wedge ? P->W1.p : nullptr;
-        int apply = P->dummy_filter ? 0 : 1;
-        KernelProf* kp = P->ctx->fft.prof;
+        const int E = std::min(P->cap, n - e0);
+        const EvalGeom* geo = P->geom_all.p + e0;
         const double Rb = (double)E * N * N * N * sizeof(float), Hb = (double)E * nh * sizeof(float2);
         const double Gb = (double)E * P->z.E * N * H * sizeof(float2);
         // reference side: F1 = mcf(r2c(I1 maskfft shift)) * filter * W1 W2 -> r1 = c2r -> k1_m, r1 m1 -> F1m
         {
             ProfScope ps(kp, "k_spec_filter", 2 * Hb, s);
-            launch_spec_filter(P->F1mcf.p, 0, P->S1.p, P->filter.p, W1, P->geom.p, E, N, 0, apply, P->p_n1.p, P->p_wf.p,
+            launch_spec_filter(P->F1mcf.p, 0, P->S1.p, P->filter.p, W1, geo, E, N, 0, apply, P->p_n1.p, P->p_wf.p,
                                P->nb_spec, s);
         }
         fft3_c2r(P->ctx->fft, P->S1.p, P->S1.p, P->R1.p, N, E, s);
@@ -339,14 +355,14 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
         // particle side
         {
             ProfScope ps(kp, "k_reslice", 2 * Rb, s);
-            launch_reslice(d_parts, P->geom.p, E, N, P->R2.p, P->p_res.p, P->nb_res, s);
+            launch_reslice(d_parts, geo, E, N, P->R2.p, P->p_res.p, P->nb_res, s);
         }
         launch_reslice_mean(P->p_res.p, P->nb_res, E, P->stats.p, s);
         R2CArgs a2{P->R2.p, P->S2.p, N, E, PRO_RESLICED, P->maskfft.p, P->stats.p};
         fft3_r2c(P->ctx->fft, a2, nullptr, s);
         {
             ProfScope ps(kp, "k_spec_filter", 2 * Hb, s);
-            launch_spec_filter(P->S2.p, (int)nh, P->S2.p, P->filter.p, W1, P->geom.p, E, N, P->p.mcf, apply, P->p_n2.p,
+            launch_spec_filter(P->S2.p, (int)nh, P->S2.p, P->filter.p, W1, geo, E, N, P->p.mcf, apply, P->p_n2.p,
                                nullptr, P->nb_spec, s);
         }
         fft3_c2r(P->ctx->fft, P->S2.p, P->S3.p, P->R2.p, N, E, s);
@@ -363,19 +379,20 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
             launch_dftBC_argmax(P->G1.p, P->tabA.p, P->tabK.p, P->z, E, P->bval.p, P->bidx.p, s);
         }
         launch_peak_final(P->bval.p, P->bidx.p, P->scalars.p, (float)P->sum_filter,
-                          (P->p.overlap_norm && P->p.missing_wedge) ? 1 : 0, P->z, E, P->out4.p, s);
-        ETTK_HIP(hipEventRecord(P->ev1, s));
-        ETTK_HIP(hipMemcpyAsync(P->h_out4.data(), P->out4.p, (size_t)4 * E * sizeof(float), hipMemcpyDeviceToHost, s));
-        ETTK_HIP(hipStreamSynchronize(s));
-        float ms = 0;
-        ETTK_HIP(hipEventElapsedTime(&ms, P->ev0, P->ev1));
-        P->timing.ccc_ms += ms;
-        P->timing.n_ccc += E;
-        for (int e = 0; e < E; e++) {
-            dist[e0 + e] = P->h_out4[e * 4];
-            double sx = P->h_out4[e * 4 + 1], sy = P->h_out4[e * 4 + 2], sz = P->h_out4[e * 4 + 3];
-            t[e0 + e] = t[e0 + e] * mat_translate(-sx, -sy, -sz);
-        }
+                          (P->p.overlap_norm && P->p.missing_wedge) ? 1 : 0, P->z, E, P->out_all.p + (size_t)4 * e0, s);
+    }
+    ETTK_HIP(hipEventRecord(P->ev1, s));
+    P->h_out4.resize((size_t)4 * n);
+    ETTK_HIP(hipMemcpyAsync(P->h_out4.data(), P->out_all.p, (size_t)4 * n * sizeof(float), hipMemcpyDeviceToHost, s));
+    ETTK_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    ETTK_HIP(hipEventElapsedTime(&ms, P->ev0, P->ev1));
+    P->timing.ccc_ms += ms;
+    P->timing.n_ccc += n;
+    for (int e = 0; e < n; e++) {
+        dist[e] = P->h_out4[(size_t)e * 4];
+        double sx = P->h_out4[(size_t)e * 4 + 1], sy = P->h_out4[(size_t)e * 4 + 2], sz = P->h_out4[(size_t)e * 4 + 3];
+        t[e] = t[e] * mat_translate(-sx, -sy, -sz);
     }
 }
 

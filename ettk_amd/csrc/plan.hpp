@@ -74,7 +74,8 @@ struct ettk_plan {
     ettk::DevBuf<float> R1, R2, p_res, p_n1, p_n2, p_k1, p_wf, stats, scalars, bval, out4;
     ettk::DevBuf<int> bidx;
     ettk::DevBuf<float2> S1, S2, S3, G1;
-    ettk::DevBuf<ettk::EvalGeom> geom;
+    ettk::DevBuf<ettk::EvalGeom> geom, geom_all;
+    ettk::DevBuf<float> out_all;
     std::vector<float> h_out4;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ettk::Timing timing;

@@ -71,107 +71,109 @@ static void refine_eval_batch(ettk_plan* P, const ettk_particle* descs, int n, c
     const int cap = P->cap;
     const int nbd = refine_dot_blocks(N), nbm = refine_moment_blocks(N);
     P->r_pdot.ensure((size_t)3 * nbd * cap);
-    P->r_sc.ensure((size_t)4 * cap);
     P->r_mpart.ensure((size_t)12 * nbm * cap);
-    P->r_mom.ensure((size_t)12 * cap);
-    P->r_geom.ensure(cap);
-    std::vector<RefineGeom> g(cap);
-    std::vector<Mat4> Ginv(cap), EMs(cap);
-    std::vector<double> sc((size_t)4 * cap), mom((size_t)12 * cap);
+    P->r_sc.ensure((size_t)4 * n);
+    P->r_mom.ensure((size_t)12 * n);
+    P->r_geom.ensure(n);
+    // geometry of every evaluation on the host (FP64), one upload, all chunks back to back, one download
+    std::vector<RefineGeom> g(n);
+    std::vector<Mat4> Ginv(n), EMs(n);
+    for (int q = 0; q < n; q++) {
+        const ettk_particle& d = descs[pidx[q]];
+        const Mat4 EM = euler_of(d) * Mat4::from(d.matrix);
+        Mat4 fin = EM * pose(seeds[q], x6 + (size_t)q * 6);
+        if (mat_det(fin) == 0.0) fin = Mat4::ident();
+        const Mat4 Gi = mat_inverse(fin);
+        EMs[q] = EM;
+        Ginv[q] = Gi;
+        RefineGeom& r = g[q];
+        r = RefineGeom{};
+        const double c[3] = {N / 2.0, N / 2.0, N / 2.0 + d.cut_offset};
+        for (int a = 0; a < 3; a++) {
+            for (int b = 0; b < 3; b++) r.G[a * 4 + b] = (float)Gi(a, b);
+            r.G[a * 4 + 3] = (float)(Gi(a, 3) + c[a]);
+        }
+        double n1[3], n2[3];
+        r.pw = (P->p.missing_wedge && lab_normals(d.lwedge, d.uwedge, n1, n2)) ? 1 : 0;
+        if (r.pw)
+            for (int a = 0; a < 3; a++) {
+                r.n1[a] = (float)n1[a];
+                r.n2[a] = (float)n2[a];
+            }
+        r.rw = (P->p.missing_wedge && !P->wref.full) ? 1 : 0;
+        if (r.rw) {
+            // reference wedge normals (reference frame) into the particle frame at the seed: u = A^T v
+            Mat4 fs = EM * seeds[q];
+            if (mat_det(fs) == 0.0) fs = Mat4::ident();
+            const Mat4 A = mat_inverse(fs);
+            for (int cc = 0; cc < 3; cc++) {
+                double a1 = 0, a2 = 0;
+                for (int rr = 0; rr < 3; rr++) {
+                    a1 += A(rr, cc) * P->wref.v1[rr];
+                    a2 += A(rr, cc) * P->wref.v2[rr];
+                }
+                r.u1[cc] = (float)a1;
+                r.u2[cc] = (float)a2;
+            }
+        }
+        r.spec = pidx[q] - p0;
+    }
+    ETTK_HIP(hipMemcpyAsync(P->r_geom.p, g.data(), (size_t)n * sizeof(RefineGeom), hipMemcpyHostToDevice, s));
     KernelProf* kp = P->ctx->fft.prof;
     for (int e0 = 0; e0 < n; e0 += cap) {
         const int E = std::min(cap, n - e0);
-        for (int e = 0; e < E; e++) {
-            const int q = e0 + e;
-            const ettk_particle& d = descs[pidx[q]];
-            const Mat4 EM = euler_of(d) * Mat4::from(d.matrix);
-            Mat4 fin = EM * pose(seeds[q], x6 + (size_t)q * 6);
-            if (mat_det(fin) == 0.0) fin = Mat4::ident();
-            const Mat4 Gi = mat_inverse(fin);
-            EMs[e] = EM;
-            Ginv[e] = Gi;
-            RefineGeom& r = g[e];
-            r = RefineGeom{};
-            const double c[3] = {N / 2.0, N / 2.0, N / 2.0 + d.cut_offset};
-            for (int a = 0; a < 3; a++) {
-                for (int b = 0; b < 3; b++) r.G[a * 4 + b] = (float)Gi(a, b);
-                r.G[a * 4 + 3] = (float)(Gi(a, 3) + c[a]);
-            }
-            double n1[3], n2[3];
-            r.pw = (P->p.missing_wedge && lab_normals(d.lwedge, d.uwedge, n1, n2)) ? 1 : 0;
-            if (r.pw)
-                for (int a = 0; a < 3; a++) {
-                    r.n1[a] = (float)n1[a];
-                    r.n2[a] = (float)n2[a];
-                }
-            r.rw = (P->p.missing_wedge && !P->wref.full) ? 1 : 0;
-            if (r.rw) {
-                // reference wedge normals (reference frame) into the particle frame at the seed: u = A^T v
-                Mat4 fs = EM * seeds[q];
-                if (mat_det(fs) == 0.0) fs = Mat4::ident();
-                const Mat4 A = mat_inverse(fs);
-                for (int cc = 0; cc < 3; cc++) {
-                    double a1 = 0, a2 = 0;
-                    for (int rr = 0; rr < 3; rr++) {
-                        a1 += A(rr, cc) * P->wref.v1[rr];
-                        a2 += A(rr, cc) * P->wref.v2[rr];
-                    }
-                    r.u1[cc] = (float)a1;
-                    r.u2[cc] = (float)a2;
-                }
-            }
-            r.spec = pidx[q] - p0;
-        }
-        ETTK_HIP(hipMemcpyAsync(P->r_geom.p, g.data(), E * sizeof(RefineGeom), hipMemcpyHostToDevice, s));
+        const RefineGeom* geo = P->r_geom.p + e0;
         const double Rb = (double)E * N * N * N * sizeof(float), Hb = (double)E * nh * sizeof(float2);
         {
             ProfScope ps(kp, "k_reslice_ref", Rb, s);
-            launch_reslice_ref(P->ref_x.p, P->r_geom.p, E, N, P->R2.p, s);
+            launch_reslice_ref(P->ref_x.p, geo, E, N, P->R2.p, s);
         }
         R2CArgs a{P->R2.p, P->S2.p, N, E, PRO_MASK_SHIFT, P->maskfft.p, nullptr};
         fft3_r2c(P->ctx->fft, a, nullptr, s);
+        double* sc = P->r_sc.p + (size_t)4 * e0;
         {
             ProfScope ps(kp, "k_refine_dot", 2 * Hb, s);
-            launch_refine_dot(P->S2.p, P->r_fp.p, P->filter.p, P->r_geom.p, E, N, P->r_pdot.p, P->r_sc.p, s);
+            launch_refine_dot(P->S2.p, P->r_fp.p, P->filter.p, geo, E, N, P->r_pdot.p, sc, s);
         }
         {
             ProfScope ps(kp, "k_refine_adj", 3 * Hb, s);
-            launch_refine_adj(P->S2.p, P->r_fp.p, P->filter.p, P->r_geom.p, E, N, P->r_sc.p, s);
+            launch_refine_adj(P->S2.p, P->r_fp.p, P->filter.p, geo, E, N, sc, s);
         }
         fft3_c2r(P->ctx->fft, P->S2.p, P->S3.p, P->R1.p, N, E, s);
         {
             ProfScope ps(kp, "k_refine_moments", Rb, s);
-            launch_refine_moments(P->ref_x.p, P->r_geom.p, P->R1.p, P->maskfft.p, E, N, P->r_mpart.p, P->r_mom.p, s);
+            launch_refine_moments(P->ref_x.p, geo, P->R1.p, P->maskfft.p, E, N, P->r_mpart.p,
+                                  P->r_mom.p + (size_t)12 * e0, s);
         }
-        ETTK_HIP(hipMemcpyAsync(sc.data(), P->r_sc.p, (size_t)4 * E * sizeof(double), hipMemcpyDeviceToHost, s));
-        ETTK_HIP(hipMemcpyAsync(mom.data(), P->r_mom.p, (size_t)12 * E * sizeof(double), hipMemcpyDeviceToHost, s));
-        ETTK_HIP(hipStreamSynchronize(s));
-        for (int e = 0; e < E; e++) {
-            const int q = e0 + e;
-            const double* x = x6 + (size_t)q * 6;
-            S[q] = sc[(size_t)e * 4];
-            const Mat4 R0 = strip_translation(seeds[q]);
-            const Mat4 Rx = mat_rot(x[0], 1, 0, 0), Ry = mat_rot(x[1], 0, 1, 0), Rz = mat_rot(x[2], 0, 0, 1);
-            Mat4 dt[6];
-            dt[0] = R0 * drot(0, x[0]) * Ry * Rz;
-            dt[1] = R0 * Rx * drot(1, x[1]) * Rz;
-            dt[2] = R0 * Rx * Ry * drot(2, x[2]);
-            for (int r = 0; r < 3; r++) {
-                dt[r](3, 3) = 0.0;
-                dt[r](0, 3) = dt[r](1, 3) = dt[r](2, 3) = 0.0;
-            }
-            for (int r = 3; r < 6; r++) {
-                for (double& v : dt[r].a) v = 0.0;
-                dt[r](r - 3, 3) = 1.0;
-            }
-            for (int r = 0; r < 6; r++) {
-                // d(fin^-1) = -fin^-1 dfin fin^-1
-                const Mat4 dG = Ginv[e] * (EMs[e] * dt[r]) * Ginv[e];
-                double acc = 0;
-                for (int aa = 0; aa < 3; aa++)
-                    for (int b = 0; b < 4; b++) acc -= dG(aa, b) * mom[(size_t)e * 12 + aa * 4 + b];
-                grad6[(size_t)q * 6 + r] = acc;
-            }
+    }
+    std::vector<double> sc((size_t)4 * n), mom((size_t)12 * n);
+    ETTK_HIP(hipMemcpyAsync(sc.data(), P->r_sc.p, (size_t)4 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+    ETTK_HIP(hipMemcpyAsync(mom.data(), P->r_mom.p, (size_t)12 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+    ETTK_HIP(hipStreamSynchronize(s));
+    for (int q = 0; q < n; q++) {
+        const double* x = x6 + (size_t)q * 6;
+        S[q] = sc[(size_t)q * 4];
+        const Mat4 R0 = strip_translation(seeds[q]);
+        const Mat4 Rx = mat_rot(x[0], 1, 0, 0), Ry = mat_rot(x[1], 0, 1, 0), Rz = mat_rot(x[2], 0, 0, 1);
+        Mat4 dt[6];
+        dt[0] = R0 * drot(0, x[0]) * Ry * Rz;
+        dt[1] = R0 * Rx * drot(1, x[1]) * Rz;
+        dt[2] = R0 * Rx * Ry * drot(2, x[2]);
+        for (int r = 0; r < 3; r++) {
+            dt[r](3, 3) = 0.0;
+            dt[r](0, 3) = dt[r](1, 3) = dt[r](2, 3) = 0.0;
+        }
+        for (int r = 3; r < 6; r++) {
+            for (double& v : dt[r].a) v = 0.0;
+            dt[r](r - 3, 3) = 1.0;
+        }
+        for (int r = 0; r < 6; r++) {
+            // d(fin^-1) = -fin^-1 dfin fin^-1
+            const Mat4 dG = Ginv[q] * (EMs[q] * dt[r]) * Ginv[q];
+            double acc = 0;
+            for (int aa = 0; aa < 3; aa++)
+                for (int b = 0; b < 4; b++) acc -= dG(aa, b) * mom[(size_t)q * 12 + aa * 4 + b];
+            grad6[(size_t)q * 6 + r] = acc;
         }
     }
 }

@@ -32,7 +32,8 @@ def _cases(ps, n_rand, seed):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N,mcf,wedge,overlap", [(32, 1, 1, 1), (32, 0, 1, 0), (32, 1, 0, 0), (64, 1, 1, 1)])
+@pytest.mark.parametrize("N,mcf,wedge,overlap", [(32, 1, 1, 1), (32, 0, 1, 0), (32, 1, 0, 0), (64, 1, 1, 1), (48, 1, 1, 1),
+                                                  (96, 1, 1, 0)])
 def test_ccc_matches_oracle(torch_gpu, ctx, N, mcf, wedge, overlap):
     from ettk_amd.lib import Plan
 
