@@ -46,9 +46,10 @@ __device__ __forceinline__ void keys_w(float f, float* w) {  // vtkTricubicInter
 }
 
 // One workgroup reslices an 8x8x8 tile of the output.  The input voxels its 64-tap stencils can touch lie in
-// the clamped bounding box of the tile's transformed corners (at most 17 voxels per axis for a rigid map);
-// that box is staged in LDS once and every tap is read from LDS.  A non-rigid map whose box does not fit falls
-// back to reading the taps from global memory.
+// the bounding box of the tile's transformed corners (at most 17 voxels per axis for a rigid map); that box,
+// with the volume's border replicated where it reaches outside (the taps clamp to the border), is staged in
+// LDS at fixed strides, so every tap is a plain LDS read and the 4 taps of a row are consecutive words.  A
+// non-rigid map whose box does not fit reads the clamped taps from global memory instead.
 static constexpr int kTile = 8, kBox = 18;
 __global__ void __launch_bounds__(kThreads) k_reslice(const float* __restrict__ parts, const EvalGeom* __restrict__ geom,
                                                        int N, float* __restrict__ out, float* __restrict__ partials) {
@@ -62,25 +63,36 @@ __global__ void __launch_bounds__(kThreads) k_reslice(const float* __restrict__ 
     const int ti = blockIdx.x / (nt * nt), tj = (blockIdx.x / nt) % nt, tk = blockIdx.x % nt;
     const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * kTile;
     const float tol = 7.62939453125e-06f;
-    // bounding box of the stencils in input index space (per axis: x, y, z)
-    int lo[3], dim[3];
+    // box origin per input axis (x, y, z); it may reach outside the volume
+    int lo[3];
     bool fits = true;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         float c0 = g.m[a] * i0 + g.m[3 + a] * j0 + g.m[6 + a] * k0 + g.m[9 + a];
         float ext = (kTile - 1) * (fabsf(g.m[a]) + fabsf(g.m[3 + a]) + fabsf(g.m[6 + a]));
         float mn = c0 + (kTile - 1) * (fminf(g.m[a], 0.f) + fminf(g.m[3 + a], 0.f) + fminf(g.m[6 + a], 0.f));
-        int l = min(max((int)floorf(mn + tol) - 1, 0), N - 1);
-        int h = min(max((int)floorf(mn + ext + tol) + 2, 0), N - 1);
+        int l = (int)floorf(mn + tol) - 1;
+        int h = (int)floorf(mn + ext + tol) + 2;
         lo[a] = l;
-        dim[a] = h - l + 1;
-        if (dim[a] > kBox) fits = false;
+        if (h - l + 1 > kBox) fits = false;
     }
     if (fits) {
-        const int nbox = dim[0] * dim[1] * dim[2];
-        for (int t = threadIdx.x; t < nbox; t += blockDim.x) {
-            int x = t % dim[0], y = (t / dim[0]) % dim[1], z = t / (dim[0] * dim[1]);
-            box[t] = vol[((size_t)(lo[2] + z) * N + (lo[1] + y)) * N + (lo[0] + x)];
+        // 2-D thread map over (y, x) of the box, loop over z; coordinates clamped to the volume
+        const int t = threadIdx.x;
+        if (t < kBox * 14) {
+            const int x = t % kBox, y0 = t / kBox;
+            const int gx = min(max(lo[0] + x, 0), N - 1);
+#pragma unroll
+            for (int yy = 0; yy < 2; yy++) {
+                const int y = y0 + 14 * yy;
+                if (y >= kBox) break;
+                const int gy = min(max(lo[1] + y, 0), N - 1);
+                const float* src = vol + (size_t)gy * N + gx;
+                for (int z = 0; z < kBox; z++) {
+                    const int gz = min(max(lo[2] + z, 0), N - 1);
+                    box[(z * kBox + y) * kBox + x] = src[(size_t)gz * N * N];
+                }
+            }
         }
     }
     __syncthreads();
@@ -99,32 +111,27 @@ __global__ void __launch_bounds__(kThreads) k_reslice(const float* __restrict__ 
             keys_w(xx - ix, wx);
             keys_w(yy - iy, wy);
             keys_w(zz - iz, wz);
-            int cx[4], cy[4], cz[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                cx[q] = min(max(ix - 1 + q, 0), N - 1);
-                cy[q] = min(max(iy - 1 + q, 0), N - 1);
-                cz[q] = min(max(iz - 1 + q, 0), N - 1);
-            }
             float acc = 0.f;
             if (fits) {
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    cx[q] -= lo[0];
-                    cy[q] -= lo[1];
-                    cz[q] -= lo[2];
-                }
+                const float* b0 = box + ((iz - 1 - lo[2]) * kBox + (iy - 1 - lo[1])) * kBox + (ix - 1 - lo[0]);
 #pragma unroll
                 for (int a = 0; a < 4; a++) {
                     float ay = 0.f;
 #pragma unroll
                     for (int b = 0; b < 4; b++) {
-                        const float* row = box + (cz[a] * dim[1] + cy[b]) * dim[0];
-                        ay += wy[b] * (wx[0] * row[cx[0]] + wx[1] * row[cx[1]] + wx[2] * row[cx[2]] + wx[3] * row[cx[3]]);
+                        const float* row = b0 + (a * kBox + b) * kBox;
+                        ay += wy[b] * (wx[0] * row[0] + wx[1] * row[1] + wx[2] * row[2] + wx[3] * row[3]);
                     }
                     acc += wz[a] * ay;
                 }
             } else {
+                int cx[4], cy[4], cz[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    cx[q] = min(max(ix - 1 + q, 0), N - 1);
+                    cy[q] = min(max(iy - 1 + q, 0), N - 1);
+                    cz[q] = min(max(iz - 1 + q, 0), N - 1);
+                }
 #pragma unroll
                 for (int a = 0; a < 4; a++) {
                     float ay = 0.f;

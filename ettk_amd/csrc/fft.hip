@@ -139,6 +139,70 @@ __device__ __forceinline__ void lds_fft(float2* pen, int lane, const float2* tw,
     run_stage<L, 3>(pen, lane, Ns, tw, inv);
 }
 
+// ------------------------------------------------------------------ length 64 in registers (8 x 8)
+// 64 = 8 x 8 Cooley-Tukey: each of 8 threads holds x[t + 8m] (m = 0..7), runs an 8-point DFT in registers,
+// applies W64^(t k1), and the 8 x 8 block is transposed through the pencil's own LDS row; an 8-point DFT per
+// thread finishes X[k1 + 8 k2].  One LDS exchange instead of three radix-4 stages.
+__device__ __forceinline__ float2 mul_w8(float2 z, int n, bool inv) {  // z * exp(-+2 pi i n / 8), n = 1..3
+    const float r = 0.70710678118654752440f;
+    switch (n) {
+        case 1: return inv ? make_float2(r * (z.x - z.y), r * (z.x + z.y)) : make_float2(r * (z.x + z.y), r * (z.y - z.x));
+        case 2: return mul_i(z, inv);
+        default: return inv ? make_float2(-r * (z.x + z.y), r * (z.x - z.y)) : make_float2(r * (z.y - z.x), -r * (z.x + z.y));
+    }
+}
+// in place, natural order in and out: a[k] = sum_n a[n] exp(-+2 pi i n k / 8)
+__device__ __forceinline__ void dft8(float2* a, bool inv) {
+    float2 b0 = cadd(a[0], a[4]), b4 = csub(a[0], a[4]);
+    float2 b1 = cadd(a[1], a[5]), b5 = mul_w8(csub(a[1], a[5]), 1, inv);
+    float2 b2 = cadd(a[2], a[6]), b6 = mul_i(csub(a[2], a[6]), inv);
+    float2 b3 = cadd(a[3], a[7]), b7 = mul_w8(csub(a[3], a[7]), 3, inv);
+    float2 c0 = cadd(b0, b2), c2 = csub(b0, b2), c1 = cadd(b1, b3), c3 = mul_i(csub(b1, b3), inv);
+    float2 c4 = cadd(b4, b6), c6 = csub(b4, b6), c5 = cadd(b5, b7), c7 = mul_i(csub(b5, b7), inv);
+    a[0] = cadd(c0, c1);
+    a[4] = csub(c0, c1);
+    a[2] = cadd(c2, c3);
+    a[6] = csub(c2, c3);
+    a[1] = cadd(c4, c5);
+    a[5] = csub(c4, c5);
+    a[3] = cadd(c6, c7);
+    a[7] = csub(c6, c7);
+}
+// every thread of the workgroup must call it (it synchronises); `tw` holds exp(-2 pi i t / 64)
+__device__ __forceinline__ void fft64_reg(float2* pen, int lane, const float2* tw, bool inv) {
+    float2 x[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) x[m] = pen[lane + 8 * m];
+    dft8(x, inv);
+#pragma unroll
+    for (int k = 1; k < 8; k++) {
+        float2 w = tw[(lane * k) & 63];
+        if (inv) w.y = -w.y;
+        x[k] = cmul(x[k], w);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; k++) pen[k * 8 + lane] = x[k];
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 8; t++) x[t] = pen[lane * 8 + t];
+    dft8(x, inv);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; k++) pen[lane + 8 * k] = x[k];
+    __syncthreads();
+}
+
+// threads per pencil of the pencil FFT used by the fused and strided passes
+template <int L> struct PenT {
+    static constexpr int TPP = (L == 64) ? 8 : Shape<L>::TPP;
+};
+template <int L>
+__device__ __forceinline__ void pencil_fft(float2* pen, int lane, const float2* tw, bool inv) {
+    if constexpr (L == 64) fft64_reg(pen, lane, tw, inv);
+    else lds_fft<L>(pen, lane, tw, inv);
+}
+
 // ------------------------------------------------------------------ r2c, pass along k (real rows)
 template <int L>
 __device__ __forceinline__ float r2c_load(const R2CArgs& a, long row, int k) {
@@ -222,9 +286,16 @@ struct StridedArgs {
     int inv;
 };
 
+template <int L> struct StridedShape {
+    static constexpr int TPP = PenT<L>::TPP;
+    static constexpr int P = Shape<L>::THREADS / TPP;  // pencils per workgroup
+    static constexpr int THREADS = Shape<L>::THREADS;
+    static constexpr int LD = L + 1;
+};
+
 template <int L>
-__global__ void __launch_bounds__(Shape<L>::THREADS) k_fft_strided(StridedArgs a, const float2* __restrict__ twg) {
-    using S = Shape<L>;
+__global__ void __launch_bounds__(StridedShape<L>::THREADS) k_fft_strided(StridedArgs a, const float2* __restrict__ twg) {
+    using S = StridedShape<L>;
     __shared__ float2 lds[S::P * S::LD];
     __shared__ float2 tw[L];
     const long q0 = (long)blockIdx.x * S::P;
@@ -241,7 +312,7 @@ __global__ void __launch_bounds__(Shape<L>::THREADS) k_fft_strided(StridedArgs a
     }
     __syncthreads();
     const int q = threadIdx.x / S::TPP, lane = threadIdx.x % S::TPP;
-    lds_fft<L>(&lds[q * S::LD], lane, tw, a.inv != 0);
+    pencil_fft<L>(&lds[q * S::LD], lane, tw, a.inv != 0);
     for (int t = threadIdx.x; t < S::P * L; t += blockDim.x) {
         int qq = t % S::P, l = t / S::P;
         long pen = q0 + qq;
@@ -300,7 +371,7 @@ __global__ void __launch_bounds__(Shape<L>::THREADS) k_c2r_rows(const float2* __
 // instead of three times.  H pencil groups of TPP threads; the spare group of the k pass works on a dummy row.
 template <int L> struct Plane {
     static constexpr int H = L / 2 + 1;
-    static constexpr int TPP = Shape<L>::TPP;
+    static constexpr int TPP = PenT<L>::TPP;
     static constexpr int THREADS = H * TPP;
     static constexpr int LD = L + 1;
 };
@@ -309,31 +380,45 @@ template <int L>
 __global__ void __launch_bounds__(Plane<L>::THREADS) k_r2c_plane(R2CArgs a, const float2* __restrict__ twg) {
     using PL = Plane<L>;
     constexpr int H = PL::H, LD = PL::LD;
-    __shared__ float2 rows[H * LD];  // k pass: L/2 packed row pairs (+1 dummy)
-    __shared__ float2 cols[H * LD];  // j pass: H columns of length L
+    constexpr int NS = ((L / 2) * 2 * H + PL::THREADS - 1) / PL::THREADS;  // split outputs per thread
+    __shared__ float2 buf[H * LD];  // k pass: L/2 packed row pairs (+1 dummy); then j pass: H columns
     __shared__ float2 tw[L];
-    const long plane = blockIdx.x;   // b * L + i
+    const long plane = blockIdx.x;  // b * L + i
     for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
     for (int t = threadIdx.x; t < L * L; t += blockDim.x) {
         const int j = t / L, x = t % L;
-        float* f = reinterpret_cast<float*>(&rows[(j >> 1) * LD + x]);
+        float* f = reinterpret_cast<float*>(&buf[(j >> 1) * LD + x]);
         f[j & 1] = r2c_load<L>(a, plane * L + j, x);
     }
     __syncthreads();
     const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
-    lds_fft<L>(&rows[q * LD], lane, tw, false);
-    for (int t = threadIdx.x; t < (L / 2) * 2 * H; t += blockDim.x) {
-        const int pp = t / (2 * H), r = t % (2 * H), which = r / H, k = r % H;
-        const float2 z = rows[pp * LD + k], zc = cconj(rows[pp * LD + ((L - k) % L)]);
-        const float2 o = which == 0 ? cscale(cadd(z, zc), 0.5f) : make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
-        cols[k * LD + 2 * pp + which] = o;
+    pencil_fft<L>(&buf[q * LD], lane, tw, false);
+    // split Z = A + iB of each pair into the half spectra of its two rows, through registers
+    float2 o[NS];
+#pragma unroll
+    for (int u = 0; u < NS; u++) {
+        const int t = threadIdx.x + u * PL::THREADS;
+        if (t < (L / 2) * 2 * H) {
+            const int pp = t / (2 * H), r = t % (2 * H), which = r / H, k = r % H;
+            const float2 z = buf[pp * LD + k], zc = cconj(buf[pp * LD + ((L - k) % L)]);
+            o[u] = which == 0 ? cscale(cadd(z, zc), 0.5f) : make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
+        }
     }
     __syncthreads();
-    lds_fft<L>(&cols[q * LD], lane, tw, false);
-    float2* o = a.out + plane * L * H;
+#pragma unroll
+    for (int u = 0; u < NS; u++) {
+        const int t = threadIdx.x + u * PL::THREADS;
+        if (t < (L / 2) * 2 * H) {
+            const int pp = t / (2 * H), r = t % (2 * H), which = r / H, k = r % H;
+            buf[k * LD + 2 * pp + which] = o[u];
+        }
+    }
+    __syncthreads();
+    pencil_fft<L>(&buf[q * LD], lane, tw, false);
+    float2* op = a.out + plane * L * H;
     for (int t = threadIdx.x; t < L * H; t += blockDim.x) {
         const int j = t / H, k = t % H;
-        o[(long)j * H + k] = cols[k * LD + j];
+        op[(long)j * H + k] = buf[k * LD + j];
     }
 }
 
@@ -342,42 +427,53 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_plane(const float2* _
                                                                    const float2* __restrict__ twg) {
     using PL = Plane<L>;
     constexpr int H = PL::H, LD = PL::LD;
-    __shared__ float2 cols[H * LD];
-    __shared__ float2 rows[H * LD];
+    constexpr int NZ = ((L / 2) * L + PL::THREADS - 1) / PL::THREADS;
+    __shared__ float2 buf[H * LD];
     __shared__ float2 tw[L];
     const long plane = blockIdx.x;
     for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
     const float2* ip = in + plane * L * H;
     for (int t = threadIdx.x; t < L * H; t += blockDim.x) {
         const int j = t / H, k = t % H;
-        cols[k * LD + j] = ip[(long)j * H + k];
+        buf[k * LD + j] = ip[(long)j * H + k];
     }
     __syncthreads();
     const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
-    lds_fft<L>(&cols[q * LD], lane, tw, true);
+    pencil_fft<L>(&buf[q * LD], lane, tw, true);
     // Z(k) = A(k) + i B(k) of rows 2p, 2p+1 over the full length, A(L-k) = conj(A(k)), real A(0), A(L/2)
-    for (int t = threadIdx.x; t < (L / 2) * L; t += blockDim.x) {
-        const int pp = t / L, k = t % L;
-        float2 A, Bv;
-        if (k <= L / 2) {
-            A = cols[k * LD + 2 * pp];
-            Bv = cols[k * LD + 2 * pp + 1];
-            if (k == 0 || k == L / 2) {
-                A.y = 0.f;
-                Bv.y = 0.f;
+    float2 zr[NZ];
+#pragma unroll
+    for (int u = 0; u < NZ; u++) {
+        const int t = threadIdx.x + u * PL::THREADS;
+        if (t < (L / 2) * L) {
+            const int pp = t / L, k = t % L;
+            float2 A, Bv;
+            if (k <= L / 2) {
+                A = buf[k * LD + 2 * pp];
+                Bv = buf[k * LD + 2 * pp + 1];
+                if (k == 0 || k == L / 2) {
+                    A.y = 0.f;
+                    Bv.y = 0.f;
+                }
+            } else {
+                A = cconj(buf[(L - k) * LD + 2 * pp]);
+                Bv = cconj(buf[(L - k) * LD + 2 * pp + 1]);
             }
-        } else {
-            A = cconj(cols[(L - k) * LD + 2 * pp]);
-            Bv = cconj(cols[(L - k) * LD + 2 * pp + 1]);
+            zr[u] = make_float2(A.x - Bv.y, A.y + Bv.x);
         }
-        rows[pp * LD + k] = make_float2(A.x - Bv.y, A.y + Bv.x);
     }
     __syncthreads();
-    lds_fft<L>(&rows[q * LD], lane, tw, true);
+#pragma unroll
+    for (int u = 0; u < NZ; u++) {
+        const int t = threadIdx.x + u * PL::THREADS;
+        if (t < (L / 2) * L) buf[(t / L) * LD + (t % L)] = zr[u];
+    }
+    __syncthreads();
+    pencil_fft<L>(&buf[q * LD], lane, tw, true);
     float* op = out + plane * L * L;
     for (int t = threadIdx.x; t < L * L; t += blockDim.x) {
         const int j = t / L, x = t % L;
-        const float2 z = rows[(j >> 1) * LD + x];
+        const float2 z = buf[(j >> 1) * LD + x];
         op[t] = (j & 1) ? z.y : z.x;
     }
 }
@@ -419,14 +515,14 @@ static void launch_r2c(const FftTables& T, const R2CArgs& a, float2* scratch, hi
         StridedArgs sj{a.out, a.out, (long)a.batch * L * H, (long)H, (long)L * H, (long)H, 0};
         {
             ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
-            k_fft_strided<L><<<(unsigned)((sj.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(sj, tw);
+            k_fft_strided<L><<<(unsigned)((sj.npencils + StridedShape<L>::P - 1) / StridedShape<L>::P), StridedShape<L>::THREADS, 0, s>>>(sj, tw);
         }
     }
     // pass i: pencils over (b, j, k), element stride L*H
     StridedArgs si{a.out, a.out, (long)a.batch * L * H, (long)L * H, (long)L * L * H, (long)L * H, 0};
     {
         ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
-        k_fft_strided<L><<<(unsigned)((si.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(si, tw);
+        k_fft_strided<L><<<(unsigned)((si.npencils + StridedShape<L>::P - 1) / StridedShape<L>::P), StridedShape<L>::THREADS, 0, s>>>(si, tw);
     }
     (void)scratch;
 }
@@ -440,7 +536,7 @@ static void launch_c2r(const FftTables& T, const float2* in, float2* tmp, float*
     StridedArgs si{in, tmp, (long)batch * L * H, (long)L * H, (long)L * L * H, (long)L * H, 1};
     {
         ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
-        k_fft_strided<L><<<(unsigned)((si.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(si, tw);
+        k_fft_strided<L><<<(unsigned)((si.npencils + StridedShape<L>::P - 1) / StridedShape<L>::P), StridedShape<L>::THREADS, 0, s>>>(si, tw);
     }
     if constexpr (L <= 64) {
         ProfScope ps(T.prof, plane_name<L>(1), vol_c + vol_r, s);
@@ -449,7 +545,7 @@ static void launch_c2r(const FftTables& T, const float2* in, float2* tmp, float*
         StridedArgs sj{tmp, tmp, (long)batch * L * H, (long)H, (long)L * H, (long)H, 1};
         {
             ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
-            k_fft_strided<L><<<(unsigned)((sj.npencils + S::P - 1) / S::P), S::THREADS, 0, s>>>(sj, tw);
+            k_fft_strided<L><<<(unsigned)((sj.npencils + StridedShape<L>::P - 1) / StridedShape<L>::P), StridedShape<L>::THREADS, 0, s>>>(sj, tw);
         }
         long npairs = (long)batch * L * L / 2;
         {
