@@ -46,7 +46,8 @@ __global__ void k_avg_normalise(const float* __restrict__ A, int N, float* __res
     const double inv = 1.0 / (double)n3;
     double s = 0;
     for (long o = threadIdx.x; o < n3; o += blockDim.x) {
-        int i = (int)(o / ((long)N * N)), j = (int)((o / N) % N), k = (int)(o % N);
+        int i, j, k;
+        split3((unsigned)o, (unsigned)N, (unsigned)N, i, j, k);
         double v = A[o] * inv * (((i + j + k) & 1) ? -1.0 : 1.0);
         s += v * v;
     }
@@ -58,7 +59,8 @@ __global__ void k_avg_normalise(const float* __restrict__ A, int N, float* __res
     }
     const float rms = (float)sqrt(sh[0] / (double)n3);  // varianceInside is a Pixel (float)
     for (long o = threadIdx.x; o < n3; o += blockDim.x) {
-        int x = (int)(o % N), y = (int)((o / N) % N), z = (int)(o / ((long)N * N));
+        int z, y, x;
+        split3((unsigned)o, (unsigned)N, (unsigned)N, z, y, x);
         long src = ((long)x * N + y) * N + z;
         double v = A[src] * inv * (((x + y + z) & 1) ? -1.0 : 1.0);
         out[o] = (float)(v / rms);

@@ -257,7 +257,8 @@ __global__ void k_wedge_ref(WedgeRef w, int N, float* __restrict__ W) {
     const int H = N / 2 + 1;
     long nh = (long)N * N * H;
     for (long h = blockIdx.x * (long)blockDim.x + threadIdx.x; h < nh; h += (long)gridDim.x * blockDim.x) {
-        int i = (int)(h / ((long)N * H)), j = (int)((h / H) % N), k = (int)(h % H);
+        int i, j, k;
+        split3((unsigned)h, (unsigned)N, (unsigned)H, i, j, k);
         W[h] = w.full ? 1.f : wedge_smooth(w.v1, w.v2, i, j, k, N, H);
     }
 }
@@ -281,7 +282,8 @@ __global__ void __launch_bounds__(kThreads) k_spec_filter(const float2* __restri
     const long dc = ((long)(N / 2) * N + N / 2) * H + (H - 1);
     float s = 0.f, swf = 0.f;
     for (long h = h0 + threadIdx.x; h < h1; h += blockDim.x) {
-        int i = (int)(h / ((long)N * H)), j = (int)((h / H) % N), k = (int)(h % H);
+        int i, j, k;
+        split3((unsigned)h, (unsigned)N, (unsigned)H, i, j, k);
         float2 z = F[(size_t)e * f_stride + h];
         if (mcf) {
             float m2 = z.x * z.x + z.y * z.y;
@@ -528,7 +530,8 @@ void launch_peak_final(const float* best_val, const int* best_idx, const float* 
 __global__ void k_xfast_to_internal(const float* __restrict__ in, float* __restrict__ out, int N) {
     long n3 = (long)N * N * N;
     for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < n3; o += (long)gridDim.x * blockDim.x) {
-        int i = (int)(o / ((long)N * N)), j = (int)((o / N) % N), k = (int)(o % N);
+        int i, j, k;
+        split3((unsigned)o, (unsigned)N, (unsigned)N, i, j, k);
         out[o] = in[((size_t)k * N + j) * N + i];
     }
 }
@@ -539,7 +542,8 @@ void launch_xfast_to_internal(const float* in, float* out, int N, hipStream_t s)
 __global__ void k_internal_to_xfast(const float* __restrict__ in, float* __restrict__ out, int N) {
     long n3 = (long)N * N * N;
     for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < n3; o += (long)gridDim.x * blockDim.x) {
-        int x = (int)(o % N), y = (int)((o / N) % N), zz = (int)(o / ((long)N * N));
+        int zz, y, x;
+        split3((unsigned)o, (unsigned)N, (unsigned)N, zz, y, x);
         out[o] = in[((size_t)x * N + y) * N + zz];
     }
 }

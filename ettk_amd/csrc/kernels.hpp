@@ -4,6 +4,15 @@
 
 namespace ettk {
 
+// (a, b, c) of the flat index h into an [n0][n1][n2] array (h < 2^32): 32-bit divisions only
+__device__ __forceinline__ void split3(unsigned h, unsigned n1, unsigned n2, int& a, int& b, int& c) {
+    const unsigned q = h / n2;
+    c = (int)(h - q * n2);
+    const unsigned a_ = q / n1;
+    b = (int)(q - a_ * n1);
+    a = (int)a_;
+}
+
 // per-evaluation geometry computed on the host in FP64
 struct EvalGeom {
     float m[12];      // reslice map: input index = m[0..2]*i + m[3..5]*j + m[6..8]*k + m[9..11]  (rows x,y,z)

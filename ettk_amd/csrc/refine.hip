@@ -41,10 +41,11 @@ __device__ __forceinline__ float trilin(const float* __restrict__ vol, int N, fl
     int x0 = min(max((int)fx, 0), N - 1), x1 = min(max((int)fx + 1, 0), N - 1);
     int y0 = min(max((int)fy, 0), N - 1), y1 = min(max((int)fy + 1, 0), N - 1);
     int z0 = min(max((int)fz, 0), N - 1), z1 = min(max((int)fz + 1, 0), N - 1);
-    float v000 = vol[((size_t)z0 * N + y0) * N + x0], v100 = vol[((size_t)z0 * N + y0) * N + x1];
-    float v010 = vol[((size_t)z0 * N + y1) * N + x0], v110 = vol[((size_t)z0 * N + y1) * N + x1];
-    float v001 = vol[((size_t)z1 * N + y0) * N + x0], v101 = vol[((size_t)z1 * N + y0) * N + x1];
-    float v011 = vol[((size_t)z1 * N + y1) * N + x0], v111 = vol[((size_t)z1 * N + y1) * N + x1];
+    const int r00 = (z0 * N + y0) * N, r01 = (z0 * N + y1) * N, r10 = (z1 * N + y0) * N, r11 = (z1 * N + y1) * N;
+    float v000 = vol[r00 + x0], v100 = vol[r00 + x1];
+    float v010 = vol[r01 + x0], v110 = vol[r01 + x1];
+    float v001 = vol[r10 + x0], v101 = vol[r10 + x1];
+    float v011 = vol[r11 + x0], v111 = vol[r11 + x1];
     float c00 = v000 + ax * (v100 - v000), c10 = v010 + ax * (v110 - v010);
     float c01 = v001 + ax * (v101 - v001), c11 = v011 + ax * (v111 - v011);
     float c0 = c00 + ay * (c10 - c00), c1 = c01 + ay * (c11 - c01);
@@ -107,7 +108,8 @@ __global__ void __launch_bounds__(kT) k_reslice_ref(const float* __restrict__ re
     const RefineGeom g = geom[e];
     const long n3 = (long)N * N * N;
     for (long o = (long)blockIdx.x * kT + threadIdx.x; o < n3; o += (long)gridDim.x * kT) {
-        int x = (int)(o % N), y = (int)((o / N) % N), z = (int)(o / ((long)N * N));
+        int z, y, x;
+        split3((unsigned)o, (unsigned)N, (unsigned)N, z, y, x);
         float q[3];
         ref_pos(g, N, x, y, z, q);
         out[(size_t)e * n3 + o] = trilin(ref, N, q[0], q[1], q[2], nullptr);
@@ -127,7 +129,8 @@ __global__ void __launch_bounds__(kT) k_refine_dot(const float2* __restrict__ X1
     const float2* x2p = FP + (size_t)g.spec * nh;
     double sd = 0.0, s1 = 0.0, s2 = 0.0;
     for (long h = (long)blockIdx.x * kT + threadIdx.x; h < nh; h += (long)gridDim.x * kT) {
-        const int c = (int)(h % H), b = (int)((h / H) % N), a = (int)(h / ((long)N * H));
+        int a, b, c;
+        split3((unsigned)h, (unsigned)N, (unsigned)H, a, b, c);
         const float w = hweight(filter, g, h, a, b, c, N, H);
         const float2 u = x1p[h], v = x2p[h];
         const double ww = (c == 0 || c == H - 1) ? (double)w * w : 2.0 * (double)w * w;
@@ -182,7 +185,8 @@ __global__ void __launch_bounds__(kT) k_refine_adj(float2* __restrict__ X1, cons
     float2* x1p = X1 + (size_t)e * nh;
     const float2* x2p = FP + (size_t)g.spec * nh;
     for (long h = (long)blockIdx.x * kT + threadIdx.x; h < nh; h += (long)gridDim.x * kT) {
-        const int c = (int)(h % H), b = (int)((h / H) % N), a = (int)(h / ((long)N * H));
+        int a, b, c;
+        split3((unsigned)h, (unsigned)N, (unsigned)H, a, b, c);
         const float w = hweight(filter, g, h, a, b, c, N, H);
         const float2 u = x1p[h], v = x2p[h];
         const float w2 = w * w;
@@ -206,7 +210,8 @@ __global__ void __launch_bounds__(kT) k_refine_moments(const float* __restrict__
     const long o0 = (long)blockIdx.x * vpb;
     for (int v = threadIdx.x; v < vpb; v += kT) {
         const long o = o0 + v;
-        const int x = (int)(o % N), y = (int)((o / N) % N), z = (int)(o / ((long)N * N));
+        int z, y, x;
+        split3((unsigned)o, (unsigned)N, (unsigned)N, z, y, x);
         float q[3];
         ref_pos(g, N, x, y, z, q);
         const float gam = maskfft[o] * (((x + y + z) & 1) ? -Ge[o] : Ge[o]);
