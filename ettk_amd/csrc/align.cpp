@@ -24,14 +24,13 @@
 namespace ettk {
 
 // ------------------------------------------------------------------ spherical wedge (host, FP64)
-// nbfWedgedSubImage3D::getSphericalWedgeImage -> nbfWedge3D::getSphericalImage (Wedge.h:364-429) on the
-// grid theta_i = i pi/(2B-1), phi_j = j 2pi/(2B-1) (Pixel arithmetic), then conditionSphericalWindow
-// (ProjRot.h:1322-1345: Gaussian sigma 2, radius factor 1)
-std::vector<float> spherical_wedge_host(int B, const ettk_particle& d, bool wedge_on) {
+// nbfWedgedSubImage3D::getSphericalWedgeImage -> nbfWedge3D::getSphericalImage (Wedge.h:364-429): the binary
+// wedge on the grid theta_i = i pi/(2B-1), phi_j = j 2pi/(2B-1) (Pixel arithmetic), all ones without a wedge
+std::vector<float> spherical_wedge_bin(int B, const ettk_particle& d) {
     const int n = 2 * B;
     std::vector<float> W((size_t)n * n, 1.0f);
     double l = d.lwedge < -90 ? -90.0 : d.lwedge, u = d.uwedge > 90 ? 90.0 : d.uwedge;
-    if (wedge_on && u - l < 180) {
+    if (u - l < 180) {
         Mat4 R = strip_translation(mat_inverse(euler_of(d) * Mat4::from(d.matrix)));
         double n1[3] = {std::cos(-(90.0 - u) * M_PI / 180.0), 0.0, -std::sin(-(90.0 - u) * M_PI / 180.0)};
         double n2[3] = {std::cos(-(90.0 - l) * M_PI / 180.0), 0.0, -std::sin(-(90.0 - l) * M_PI / 180.0)};
@@ -50,7 +49,15 @@ std::vector<float> spherical_wedge_host(int B, const ettk_particle& d, bool wedg
                 W[(size_t)i * n + j] = (a * b <= 0.0) ? 1.0f : 0.0f;
             }
     }
-    if (wedge_on) smooth3(W, n, n, 1, 2.0, 1.0);
+    return W;
+}
+
+// the window of the search: the binary wedge conditioned by conditionSphericalWindow (ProjRot.h:1322-1345:
+// Gaussian sigma 2, radius factor 1) when missing-wedge compensation is on, all ones otherwise
+std::vector<float> spherical_wedge_host(int B, const ettk_particle& d, bool wedge_on) {
+    if (!wedge_on) return std::vector<float>((size_t)4 * B * B, 1.0f);
+    std::vector<float> W = spherical_wedge_bin(B, d);
+    smooth3(W, 2 * B, 2 * B, 1, 2.0, 1.0);
     return W;
 }
 

@@ -118,7 +118,7 @@ void smooth3(std::vector<float>& A, int n0, int n1, int n2, double sd, double rf
     }
 }
 
-static void wedge_normals(float lw, float uw, const Mat4& t, WedgeRef& w) {
+void wedge_normals(float lw, float uw, const Mat4& t, WedgeRef& w) {
     double l = lw < -90 ? -90.0 : lw, u = uw > 90 ? 90.0 : uw;
     w.full = (u - l >= 180) ? 1 : 0;
     double n1[3] = {std::cos(-(90.0 - u) * M_PI / 180.0), 0.0, -std::sin(-(90.0 - u) * M_PI / 180.0)};
@@ -517,22 +517,38 @@ int ettk_plan_set_restrictions(ettk_plan* P, float rot_restrict, float trans_res
     ETTK_API_END
 }
 
-int ettk_reference_set(ettk_plan* P, const float* d_ref, const ettk_particle* ref_desc) {
-    ETTK_API_BEGIN
-    ETTK_REQUIRE(P && d_ref && ref_desc, ETTK_ERR_ARG, "ettk_reference_set: null argument");
+}  // extern "C"
+
+// input1 of the metric (setInput1, nbfImageMetric.h:184-189): the volume, its wedge W1, the cached reference
+// side of the CCC (FIM.h:1123-1153) and of the search (ProjRot.h:445-453).  wedge_kind 0: W1 is the analytic
+// wedge of `desc` at its own alignment (FIM.h:1101-1105); 1: the accumulated wedge of an average
+// (P->wsm_half, WAvg.h:555-611); 2: an average whose accumulated wedge is not effective (W1 = 1).
+static void reference_load(ettk_plan* P, const float* d_ref, const ettk_particle& desc, int wedge_kind) {
     ETTK_HIP(hipSetDevice(P->ctx->device));
     const int N = P->N, H = P->H;
     const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * H;
     hipStream_t s = P->ctx->stream;
-    P->refd = *ref_desc;
+    P->have_ref = false;
+    P->refd = desc;
+    P->ref_wedge_kind = wedge_kind;
     P->ref.ensure(n3);
     launch_xfast_to_internal(d_ref, P->ref.p, N, s);
-    // W1: wedge of the reference at its own alignment (FIM.h:1101-1105)
     P->W1.ensure(nh);
     WedgeRef w{};
-    Mat4 fin = euler_of(*ref_desc) * Mat4::from(ref_desc->matrix);
-    wedge_normals(ref_desc->lwedge, ref_desc->uwedge, mat_inverse(fin), w);
-    launch_wedge_ref(w, N, P->W1.p, s);
+    if (wedge_kind == 0) {
+        Mat4 fin = euler_of(desc) * Mat4::from(desc.matrix);
+        wedge_normals(desc.lwedge, desc.uwedge, mat_inverse(fin), w);
+        launch_wedge_ref(w, N, P->W1.p, s);
+    } else {
+        w.full = 1;
+        if (wedge_kind == 1) {
+            ETTK_HIP(hipMemcpyAsync(P->W1.p, P->wsm_half.p, nh * sizeof(float), hipMemcpyDeviceToDevice, s));
+            P->ref_wbin.ensure(n3);
+            ETTK_HIP(hipMemcpyAsync(P->ref_wbin.p, P->wbin.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+        } else {
+            launch_wedge_ref(w, N, P->W1.p, s);
+        }
+    }
     P->wref = w;
     // F1 = r2c(I1 * maskfft * shift), MCF applied once (executeHalf order: MCF, then filter x wedge)
     P->F1mcf.ensure(nh);
@@ -551,6 +567,29 @@ int ettk_reference_set(ettk_plan* P, const float* d_ref, const ettk_particle* re
     ETTK_HIP(hipStreamSynchronize(s));
     P->have_ref = true;
     so3_reference_set(P);
+}
+
+extern "C" {
+
+int ettk_reference_set(ettk_plan* P, const float* d_ref, const ettk_particle* ref_desc) {
+    ETTK_API_BEGIN
+    ETTK_REQUIRE(P && d_ref && ref_desc, ETTK_ERR_ARG, "ettk_reference_set: null argument");
+    reference_load(P, d_ref, *ref_desc, 0);
+    ETTK_API_END
+}
+
+// setInput1 with an nbfWedgedAverageImage3D: its wedge is the accumulated wedge of the particles it averages
+int ettk_reference_set_average(ettk_plan* P, const float* d_ref, int n, const ettk_particle* descs,
+                               const float* weights, const double* matrices) {
+    ETTK_API_BEGIN
+    ETTK_REQUIRE(P && d_ref && descs && weights && matrices && n > 0, ETTK_ERR_ARG,
+                 "ettk_reference_set_average: bad argument");
+    avg_wedge_compute(P, n, descs, weights, matrices);
+    ettk_particle full{};
+    full.lwedge = -90.0f;
+    full.uwedge = 90.0f;
+    for (int k = 0; k < 16; k++) full.matrix[k] = (k % 5 == 0) ? 1.0 : 0.0;
+    reference_load(P, d_ref, full, P->wacc_effective ? 1 : 2);
     ETTK_API_END
 }
 

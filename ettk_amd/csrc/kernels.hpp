@@ -65,18 +65,20 @@ struct RefineGeom {
     float G[12];          // q = G[r*4 + 0..2] (x, y, z) - N/2 + G[r*4 + 3]: reference index of particle voxel (x,y,z)
     float n1[3], n2[3];   // particle wedge normals, lab frame
     float u1[3], u2[3];   // reference wedge normals in the particle frame at the seed pose
-    int pw, rw;           // wedge tests on (particle, reference)
+    float A[9];           // particle-frame frequency -> reference-frame frequency at the seed (rows), rw == 2
+    int pw, rw;           // wedge tests on (particle, reference): rw 1 = analytic normals u1/u2, 2 = lookup of
+                          // the binarised accumulated wedge (full grid, k fastest) at A k
     int spec;             // slot of the particle's spectrum
-    int pad;
+    int pad[3];
 };
 int refine_moment_blocks(int N);
 int refine_dot_blocks(int N);
 void launch_vol_mean(const float* vols, int count, int N, float* mean, hipStream_t s);
 void launch_reslice_ref(const float* ref, const RefineGeom* geom, int E, int N, float* out, hipStream_t s);
-void launch_refine_dot(const float2* X1, const float2* FP, const float* filter, const RefineGeom* geom, int E, int N,
-                       double* part, double* sc, hipStream_t s);
-void launch_refine_adj(float2* X1, const float2* FP, const float* filter, const RefineGeom* geom, int E, int N,
-                       const double* sc, hipStream_t s);
+void launch_refine_dot(const float2* X1, const float2* FP, const float* filter, const float* wbin, const RefineGeom* geom,
+                       int E, int N, double* part, double* sc, hipStream_t s);
+void launch_refine_adj(float2* X1, const float2* FP, const float* filter, const float* wbin, const RefineGeom* geom,
+                       int E, int N, const double* sc, hipStream_t s);
 void launch_refine_moments(const float* ref, const RefineGeom* geom, const float* G, const float* maskfft, int E, int N,
                            float* mpart, double* mom, hipStream_t s);
 void launch_wedge_ref(const WedgeRef& w, int N, float* W, hipStream_t s);

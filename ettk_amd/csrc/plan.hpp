@@ -87,6 +87,16 @@ struct ettk_plan {
     // spectra and per-evaluation workspace
     ettk::DevBuf<float> ref_x;
     ettk::WedgeRef wref{};
+    // accumulated wedge of an average (WAvg.h:49-70, 496-538, 654-667), full grid A(i,j,k) with k fastest:
+    // sum, binarised sum, smoothed binarised sum and its half grid; the binarised spherical sum (2B x 2B).
+    // ref_wedge_kind: 0 = the reference's wedge is analytic (refd), 1 = accumulated and effective,
+    // 2 = accumulated and not effective (all ones)
+    int ref_wedge_kind = 0;
+    bool wacc_ready = false, wacc_effective = false;
+    ettk::DevBuf<float> wacc, wbin, wsm, wsm_half;
+    ettk::DevBuf<int> wcount;
+    std::vector<float> sph_acc_bin;
+    ettk::DevBuf<float> ref_wbin;  // the reference's copy of wbin (kind 1), read by the refinement
     ettk::DevBuf<float2> r_fp;
     ettk::DevBuf<float> r_mean, r_mpart;
     ettk::DevBuf<double> r_pdot, r_sc, r_mom;
@@ -100,4 +110,8 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
 void ensure_ccc_workspace(ettk_plan* P, int cap);
 void so3_state_destroy(ettk_plan* P);
 void so3_reference_set(ettk_plan* P);
+void wedge_normals(float lw, float uw, const Mat4& t, WedgeRef& w);
+std::vector<float> spherical_wedge_bin(int B, const ettk_particle& d);
+// accumulated wedge of the entries with nonzero weight into P->wacc/wbin/wsm/wsm_half/sph_acc_bin
+void avg_wedge_compute(ettk_plan* P, int n, const ettk_particle* descs, const float* weights, const double* matrices);
 }  // namespace ettk

@@ -100,21 +100,25 @@ static void refine_eval_batch(ettk_plan* P, const ettk_particle* descs, int n, c
                 r.n1[a] = (float)n1[a];
                 r.n2[a] = (float)n2[a];
             }
-        r.rw = (P->p.missing_wedge && !P->wref.full) ? 1 : 0;
+        r.rw = !P->p.missing_wedge ? 0 : P->ref_wedge_kind == 1 ? 2 : (P->ref_wedge_kind == 0 && !P->wref.full) ? 1 : 0;
         if (r.rw) {
-            // reference wedge normals (reference frame) into the particle frame at the seed: u = A^T v
+            // particle-frame frequency k -> reference-frame frequency A k at the seed; the analytic reference
+            // wedge normals v become u = A^T v
             Mat4 fs = EM * seeds[q];
             if (mat_det(fs) == 0.0) fs = Mat4::ident();
             const Mat4 A = mat_inverse(fs);
-            for (int cc = 0; cc < 3; cc++) {
-                double a1 = 0, a2 = 0;
-                for (int rr = 0; rr < 3; rr++) {
-                    a1 += A(rr, cc) * P->wref.v1[rr];
-                    a2 += A(rr, cc) * P->wref.v2[rr];
+            for (int rr = 0; rr < 3; rr++)
+                for (int cc = 0; cc < 3; cc++) r.A[rr * 3 + cc] = (float)A(rr, cc);
+            if (r.rw == 1)
+                for (int cc = 0; cc < 3; cc++) {
+                    double a1 = 0, a2 = 0;
+                    for (int rr = 0; rr < 3; rr++) {
+                        a1 += A(rr, cc) * P->wref.v1[rr];
+                        a2 += A(rr, cc) * P->wref.v2[rr];
+                    }
+                    r.u1[cc] = (float)a1;
+                    r.u2[cc] = (float)a2;
                 }
-                r.u1[cc] = (float)a1;
-                r.u2[cc] = (float)a2;
-            }
         }
         r.spec = pidx[q] - p0;
     }
@@ -133,11 +137,11 @@ static void refine_eval_batch(ettk_plan* P, const ettk_particle* descs, int n, c
         double* sc = P->r_sc.p + (size_t)4 * e0;
         {
             ProfScope ps(kp, "k_refine_dot", 2 * Hb, s);
-            launch_refine_dot(P->S2.p, P->r_fp.p, P->filter.p, geo, E, N, P->r_pdot.p, sc, s);
+            launch_refine_dot(P->S2.p, P->r_fp.p, P->filter.p, P->ref_wbin.p, geo, E, N, P->r_pdot.p, sc, s);
         }
         {
             ProfScope ps(kp, "k_refine_adj", 3 * Hb, s);
-            launch_refine_adj(P->S2.p, P->r_fp.p, P->filter.p, geo, E, N, sc, s);
+            launch_refine_adj(P->S2.p, P->r_fp.p, P->filter.p, P->ref_wbin.p, geo, E, N, sc, s);
         }
         fft3_c2r(P->ctx->fft, P->S2.p, P->S3.p, P->R1.p, N, E, s);
         {

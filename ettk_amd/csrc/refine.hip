@@ -70,17 +70,26 @@ __device__ __forceinline__ void ref_pos(const RefineGeom& g, int N, int x, int y
 }
 
 // H at half-spectrum element (a, b, c) = (z, y, x) of the particle frame: filter x binary wedges
-__device__ __forceinline__ float hweight(const float* __restrict__ filter, const RefineGeom& g, long h, int a, int b,
-                                         int c, int N, int H) {
+__device__ __forceinline__ float hweight(const float* __restrict__ filter, const float* __restrict__ wbin,
+                                         const RefineGeom& g, long h, int a, int b, int c, int N, int H) {
     float w = filter[h];  // radial table: identical in any axis order
     const float kx = (float)(c - (H - 1)), ky = (float)(b - N / 2), kz = (float)(a - N / 2);
     if (g.pw) {
         float s1 = g.n1[0] * kx + g.n1[1] * ky + g.n1[2] * kz, s2 = g.n2[0] * kx + g.n2[1] * ky + g.n2[2] * kz;
         if (s1 * s2 > 0.f) w = 0.f;
     }
-    if (g.rw) {
+    if (g.rw == 1) {
         float s1 = g.u1[0] * kx + g.u1[1] * ky + g.u1[2] * kz, s2 = g.u2[0] * kx + g.u2[1] * ky + g.u2[2] * kz;
         if (s1 * s2 > 0.f) w = 0.f;
+    } else if (g.rw == 2) {
+        // nearest voxel of the reference-frame frequency, clamped to the grid
+        int r[3];
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const float p = g.A[t * 3] * kx + g.A[t * 3 + 1] * ky + g.A[t * 3 + 2] * kz;
+            r[t] = min(max((int)rintf(p) + N / 2, 0), N - 1);
+        }
+        if (wbin[(r[0] * N + r[1]) * N + r[2]] == 0.f) w = 0.f;
     }
     return w;
 }
@@ -118,7 +127,7 @@ __global__ void __launch_bounds__(kT) k_reslice_ref(const float* __restrict__ re
 
 // per-block FP64 partials of <H X1, H X2>, |H X1|^2, |H X2|^2 (Hermitian weights): X1 = S1[e], X2 = FP[spec]
 __global__ void __launch_bounds__(kT) k_refine_dot(const float2* __restrict__ X1, const float2* __restrict__ FP,
-                                                    const float* __restrict__ filter,
+                                                    const float* __restrict__ filter, const float* __restrict__ wbin,
                                                     const RefineGeom* __restrict__ geom, int N,
                                                     double* __restrict__ part) {
     __shared__ double sh[3][kT / 64];
@@ -131,7 +140,7 @@ __global__ void __launch_bounds__(kT) k_refine_dot(const float2* __restrict__ X1
     for (long h = (long)blockIdx.x * kT + threadIdx.x; h < nh; h += (long)gridDim.x * kT) {
         int a, b, c;
         split3((unsigned)h, (unsigned)N, (unsigned)H, a, b, c);
-        const float w = hweight(filter, g, h, a, b, c, N, H);
+        const float w = hweight(filter, wbin, g, h, a, b, c, N, H);
         const float2 u = x1p[h], v = x2p[h];
         const double ww = (c == 0 || c == H - 1) ? (double)w * w : 2.0 * (double)w * w;
         sd += ww * ((double)u.x * v.x + (double)u.y * v.y);
@@ -173,7 +182,7 @@ __global__ void k_refine_scalars(const double* __restrict__ part, int nb, int n_
 
 // adjoint spectrum in place: X1 <- H (H X2 / (n1 n2) - S H X1 / n1^2)
 __global__ void __launch_bounds__(kT) k_refine_adj(float2* __restrict__ X1, const float2* __restrict__ FP,
-                                                    const float* __restrict__ filter,
+                                                    const float* __restrict__ filter, const float* __restrict__ wbin,
                                                     const RefineGeom* __restrict__ geom, int N,
                                                     const double* __restrict__ sc) {
     const int e = blockIdx.y, H = N / 2 + 1;
@@ -187,7 +196,7 @@ __global__ void __launch_bounds__(kT) k_refine_adj(float2* __restrict__ X1, cons
     for (long h = (long)blockIdx.x * kT + threadIdx.x; h < nh; h += (long)gridDim.x * kT) {
         int a, b, c;
         split3((unsigned)h, (unsigned)N, (unsigned)H, a, b, c);
-        const float w = hweight(filter, g, h, a, b, c, N, H);
+        const float w = hweight(filter, wbin, g, h, a, b, c, N, H);
         const float2 u = x1p[h], v = x2p[h];
         const float w2 = w * w;
         x1p[h] = make_float2(w2 * (ca * v.x - cb * u.x), w2 * (ca * v.y - cb * u.y));
@@ -265,18 +274,20 @@ void launch_reslice_ref(const float* ref, const RefineGeom* geom, int E, int N, 
     k_reslice_ref<<<dim3(nb, E), kT, 0, s>>>(ref, geom, N, out);
     ETTK_HIP(hipGetLastError());
 }
-void launch_refine_dot(const float2* X1, const float2* FP, const float* filter, const RefineGeom* geom, int E, int N,
+void launch_refine_dot(const float2* X1, const float2* FP, const float* filter, const float* wbin, const RefineGeom* geom,
+                       int E, int N,
                        double* part, double* sc, hipStream_t s) {
     const int nb = refine_dot_blocks(N);
-    k_refine_dot<<<dim3(nb, E), kT, 0, s>>>(X1, FP, filter, geom, N, part);
+    k_refine_dot<<<dim3(nb, E), kT, 0, s>>>(X1, FP, filter, wbin, geom, N, part);
     k_refine_scalars<<<(E + 127) / 128, 128, 0, s>>>(part, nb, E, sc);
     ETTK_HIP(hipGetLastError());
 }
-void launch_refine_adj(float2* X1, const float2* FP, const float* filter, const RefineGeom* geom, int E, int N,
+void launch_refine_adj(float2* X1, const float2* FP, const float* filter, const float* wbin, const RefineGeom* geom,
+                       int E, int N,
                        const double* sc, hipStream_t s) {
     const long nh = (long)N * N * (N / 2 + 1);
     const int nb = (int)std::min<long>(512, (nh + 4 * kT - 1) / (4 * kT));
-    k_refine_adj<<<dim3(nb, E), kT, 0, s>>>(X1, FP, filter, geom, N, sc);
+    k_refine_adj<<<dim3(nb, E), kT, 0, s>>>(X1, FP, filter, wbin, geom, N, sc);
     ETTK_HIP(hipGetLastError());
 }
 void launch_refine_moments(const float* ref, const RefineGeom* geom, const float* G, const float* maskfft, int E, int N,

@@ -777,8 +777,15 @@ void so3_reference_set(ettk_plan* P) {
     const int B = S.B;
     S.Pp1.ensure((size_t)4 * B * B);
     project_images(P, nullptr, nullptr, nullptr, 1, true, S.Pp1.p);
-    // spherical wedge of the reference (host, FP64), smoothed
-    std::vector<float> W1 = spherical_wedge_host(B, P->refd, P->p.missing_wedge != 0);
+    // spherical wedge of the reference (host, FP64), conditioned: analytic, or for an average its binarised
+    // accumulated spherical wedge (all ones when the accumulated wedge is not effective, WAvg.h:670-690)
+    std::vector<float> W1;
+    if (P->ref_wedge_kind == 0) {
+        W1 = spherical_wedge_host(B, P->refd, P->p.missing_wedge != 0);
+    } else {
+        W1 = P->ref_wedge_kind == 1 ? P->sph_acc_bin : std::vector<float>((size_t)4 * B * B, 1.0f);
+        if (P->p.missing_wedge) smooth3(W1, 2 * B, 2 * B, 1, 2.0, 1.0);
+    }
     S.W1.upload(W1.data(), W1.size(), P->ctx->stream);
     double sw = 0;
     for (float v : W1) sw += v;

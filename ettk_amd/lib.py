@@ -41,6 +41,8 @@ def load() -> C.CDLL:
             "ettk_avg_reset": ([vp], C.c_int),
             "ettk_avg_accumulate": ([vp, vp, C.c_int, C.POINTER(Particle), fp, dp], C.c_int),
             "ettk_avg_finalize": ([vp, vp], C.c_int),
+            "ettk_avg_wedge": ([vp, C.c_int, C.POINTER(Particle), fp, dp, vp, vp, fp, ip], C.c_int),
+            "ettk_reference_set_average": ([vp, vp, C.c_int, C.POINTER(Particle), fp, dp], C.c_int),
             "ettk_last_timing": ([vp, dp, C.POINTER(C.c_longlong), dp], C.c_int),
             "ettk_refine_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, ip, dp, C.c_int, dp, dp], C.c_int),
             "ettk_profile": ([vp, C.c_int], C.c_int),
@@ -195,6 +197,29 @@ class Plan:
         check(self.lib.ettk_avg_accumulate(self.h, C.c_void_p(d_parts.data_ptr()), n, particle_array(descs), _fp(w),
                                            _dp(m)))
         check(self.lib.ettk_avg_finalize(self.h, C.c_void_p(d_out.data_ptr())))
+
+    def avg_wedge(self, descs, weights, matrices, d_accum=None, d_wedge=None):
+        """Accumulated wedge of an average (ettk_avg_wedge): fills the optional device arrays (N^3 float,
+        k fastest) and returns (spherical (2B, 2B) window, effective)."""
+        n = len(descs)
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        m = np.ascontiguousarray(np.asarray(matrices, dtype=np.float64).reshape(n, 16))
+        B = self.params.B
+        sph = np.zeros((2 * B, 2 * B), np.float32)
+        eff = C.c_int()
+        ptr = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+        check(self.lib.ettk_avg_wedge(self.h, n, particle_array(descs), _fp(w), _dp(m), ptr(d_accum), ptr(d_wedge),
+                                      _fp(sph), C.byref(eff)))
+        return sph, bool(eff.value)
+
+    def set_reference_average(self, d_ref, descs, weights, matrices):
+        """Reference that is an average of the given aligned particles (ettk_reference_set_average)."""
+        n = len(descs)
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        m = np.ascontiguousarray(np.asarray(matrices, dtype=np.float64).reshape(n, 16))
+        self._keep = [d_ref]
+        check(self.lib.ettk_reference_set_average(self.h, C.c_void_p(d_ref.data_ptr()), n, particle_array(descs),
+                                                  _fp(w), _dp(m)))
 
     def last_timing(self):
         ms = C.c_double()

@@ -173,6 +173,29 @@ int ettk_avg_accumulate(ettk_plan* plan, const float* d_particles, int n, const 
                         const float* weights, const double* matrices);
 int ettk_avg_finalize(ettk_plan* plan, float* d_out);
 
+/* Accumulated missing wedge of an average (nbfWedgedAverageImage3D::updateAccumulatedWedgeImage and
+   setAccumulatedWedgeImage, WAvg.h:496-538 and :49-70; spherical counterpart updateSphericalWedgeImage,
+   :614-667).  Entry e of n (the reference's (particle, candidate) order) adds the wedge of descs[e] at the
+   alignment matrices[e*16]; the particle's own matrix is not used (setTransform(NULL) precedes getWedgeImage);
+   entries with weights[e] == 0 are skipped, and at least one must remain.  Outputs, each optional (NULL):
+   d_accum   N^3 floats, the sum of the smoothed full-grid wedges (getAccumulatedWedgeImage), array order
+             A(i,j,k) with i the x frequency and k fastest, frequency (i,j,k) - N/2;
+   d_wedge   N^3 floats, getWedgeImage: the sum binarised at max/2 and smoothed when the wedge is effective,
+             all ones otherwise;
+   out_spherical (2B)^2 host floats, getSphericalWedgeImage: the binarised spherical sum, all ones when the
+             wedge is not effective;
+   out_effective 1 when sum(binarised) < 0.95 N^3 (wedgeEffective). */
+int ettk_avg_wedge(ettk_plan* plan, int n, const ettk_particle* descs, const float* weights,
+                   const double* matrices, float* d_accum, float* d_wedge, float* out_spherical,
+                   int* out_effective);
+
+/* Reference that is an average (setInput1 with an nbfWedgedAverageImage3D): as ettk_reference_set, with the
+   reference's wedge taken from its particles as ettk_avg_wedge computes it, in the CCC
+   (getWedgeImageHalf, WAvg.h:602-611), in the S^2 search (getSphericalWedgeImage, ProjRot.h:481) and, binarised,
+   in the continuous refinement. */
+int ettk_reference_set_average(ettk_plan* plan, const float* d_ref, int n, const ettk_particle* descs,
+                               const float* weights, const double* matrices);
+
 /* Per-kernel timing of the last ettk_align_batch / ettk_ccc_eval_batch call, measured with HIP events on
    the plan's stream: total milliseconds spent in the constrained-CCC pipeline and number of CCC
    evaluations it ran. */

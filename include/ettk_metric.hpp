@@ -183,7 +183,7 @@ class CorrelationImageMetric {
             for (int j = 0; j < (int)vols.size(); j++)
                 if (D(i, j, 0, 0) == -1.0f) todo.push_back(j);
             if (todo.empty()) continue;
-            check(ettk_reference_set(plan_, refs[i].d_average, &refs[i].desc), "ettk_reference_set");
+            set_reference_(refs[i]);
             std::vector<ettk_particle> descs;
             const float* base = gather_(vols, todo, descs);
             std::vector<ettk_result> res(todo.size());
@@ -302,6 +302,32 @@ class CorrelationImageMetric {
         check(ettk_plan_create(ctx_, &p_, &plan_), "ettk_plan_create");
     }
     // device base of the selected volumes, contiguous in the given order (gathered when they are not)
+    /* setInput1(average): an average that carries its volumes and alignments uses their accumulated wedge
+       (updateAccumulatedWedgeImage, WAvg.h:496-611, entries in (volume, candidate) order); one without them,
+       or whose weights are all zero, uses its own descriptor. */
+    void set_reference_(WedgedAverageImage3D& a) {
+        const int n = a.volumes ? (int)a.volumes->size() : 0;
+        if (n > 0 && a.alignments.size() == (size_t)n * 17 * a.K) {
+            std::vector<ettk_particle> d;
+            std::vector<float> w;
+            std::vector<double> m;
+            bool any = false;
+            for (int j = 0; j < n; j++)
+                for (int k = 0; k < a.K; k++) {
+                    d.push_back((*a.volumes)[j].desc);
+                    w.push_back(a.alignment(j, 0, k));
+                    any = any || w.back() != 0.0f;
+                    for (int c = 0; c < 16; c++) m.push_back(a.alignment(j, 1 + c, k));
+                }
+            if (any) {
+                check(ettk_reference_set_average(plan_, a.d_average, (int)d.size(), d.data(), w.data(), m.data()),
+                      "ettk_reference_set_average");
+                return;
+            }
+        }
+        check(ettk_reference_set(plan_, a.d_average, &a.desc), "ettk_reference_set");
+    }
+
     const float* gather_(const std::vector<WedgedSubImage3D>& vols, const std::vector<int>& idx,
                          std::vector<ettk_particle>& descs) {
         const size_t n3 = (size_t)p_.N * p_.N * p_.N;

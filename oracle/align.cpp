@@ -335,7 +335,7 @@ void align_pair(Plan& P, const float* part, const ettk_particle& d, ettk_result&
     std::vector<float> W1(nn), W2(nn);
     bool uw1 = false, uw2 = false;
     if (P.p.missing_wedge) {
-        W1 = spherical_wedge(B, P.refd);
+        W1 = P.ref_wedge_kind == 0 ? spherical_wedge(B, P.refd) : P.W1s_acc;  // average: WAvg.h:670-690
         W2 = spherical_wedge(B, d);
         condition_spherical_window(W1, B);
         condition_spherical_window(W2, B);
@@ -552,6 +552,25 @@ void orc_spherical_wedge(int B, const ettk_particle* d, int condition, float* ou
     if (condition) condition_spherical_window(w, B);
     std::memcpy(out, w.data(), w.size() * sizeof(float));
 }
+// accumulated wedge of an average (WAvg.h:49-70, 496-538, 614-667); outputs may be null
+void orc_avg_wedge(int N, int B, int n, const ettk_particle* d, const float* w, const double* mats, float* accum,
+                   float* bin, float* wedge, float* sph, int* effective) {
+    AvgWedge aw = avg_wedge(N, B, n, d, w, mats);
+    if (accum) std::memcpy(accum, aw.accum.data(), aw.accum.size() * sizeof(float));
+    if (bin) std::memcpy(bin, aw.bin.data(), aw.bin.size() * sizeof(float));
+    if (wedge) std::memcpy(wedge, aw.wedge.data(), aw.wedge.size() * sizeof(float));
+    if (sph)
+        for (size_t o = 0; o < aw.sph.size(); o++) sph[o] = aw.effective ? aw.sph[o] : 1.0f;
+    if (effective) *effective = aw.effective ? 1 : 0;
+}
+void orc_set_reference_average(void* h, const float* ref, int n, const ettk_particle* d, const float* w,
+                               const double* mats) {
+    Plan& P = *(Plan*)h;
+    AvgWedge aw = avg_wedge(P.N, P.p.B, n, d, w, mats);
+    plan_set_reference_average(P, ref, aw);
+    project_image(P, P.ref, P.Pp1);
+}
+
 void orc_masks(void* h, float* maskfft, float* refmask, float* filter) {
     Plan& P = *(Plan*)h;
     std::memcpy(maskfft, P.maskfft.data(), P.maskfft.size() * sizeof(float));

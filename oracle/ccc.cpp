@@ -8,6 +8,7 @@
 // (nbfWedge3D.h:313-360) with nbfWedgedSubImage3D::getWedgeImageHalf (nbfWedgedSubImage3D.h:941-979).
 #include "oracle_core.hpp"
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -77,6 +78,75 @@ std::vector<float> particle_wedge_half(int N, const ettk_particle& d, const M4* 
     return wedge_half(N, d.lwedge, d.uwedge, m4_inv(fin));
 }
 
+// nbfWedge3D::getImage (Wedge.h:195-290) without a stored wedge image: the binary wedge on the full grid,
+// centre floor(shape / 2), then smoothWedgeImage; all ones when the wedge spans 180 degrees
+std::vector<float> wedge_full(int N, float lw, float uw, const M4& t) {
+    std::vector<float> W((size_t)N * N * N, 1.0f);
+    double l = lw < -90 ? -90.0 : lw;
+    double u = uw > 90 ? 90.0 : uw;
+    if (u - l >= 180) return W;
+    double n1[3] = {std::cos(-(90.0 - u) * M_PI / 180.0), 0.0, -std::sin(-(90.0 - u) * M_PI / 180.0)};
+    double n2[3] = {std::cos(-(90.0 - l) * M_PI / 180.0), 0.0, -std::sin(-(90.0 - l) * M_PI / 180.0)};
+    M4 R = m4_strip_translation(t);
+    double v1[3], v2[3];
+    for (int r = 0; r < 3; r++) {
+        v1[r] = R(r, 0) * n1[0] + R(r, 1) * n1[1] + R(r, 2) * n1[2];
+        v2[r] = R(r, 0) * n2[0] + R(r, 1) * n2[1] + R(r, 2) * n2[2];
+    }
+    const int c = N / 2;
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++)
+            for (int k = 0; k < N; k++) {
+                double a = v1[0] * (i - c) + v1[1] * (j - c) + v1[2] * (k - c);
+                double b = v2[0] * (i - c) + v2[1] * (j - c) + v2[2] * (k - c);
+                W[((size_t)i * N + j) * N + k] = (a * b <= 0.0) ? 1.0f : 0.0f;
+            }
+    gauss_smooth3(W, N, N, N, 2.0, 0.5);
+    return W;
+}
+
+// updateAccumulatedWedgeImage (WAvg.h:496-538): for every entry with |weight| > 0, setTransform(NULL) then
+// getWedgeImage(current, store) (WSub.h:841-870: fin = Euler(normal) * store, wedge of inverse(fin)),
+// accum += current; setAccumulatedWedgeImage (:49-70): binarise at max/2, effective if the count is below
+// 0.95 of the size, then smoothWedgeImage.  Spherical: updateAccumulatedSphericalWedgeImageMPI (:614-651) sums
+// getSphericalWedgeImage(A, store) (WSub.h:1007-1041), updateSphericalWedgeImage (:654-667) binarises at max/2.
+AvgWedge avg_wedge(int N, int B, int n, const ettk_particle* d, const float* w, const double* mats) {
+    AvgWedge out;
+    const size_t n3 = (size_t)N * N * N, nn = (size_t)4 * B * B;
+    out.accum.assign(n3, 0.0f);
+    std::vector<float> sacc(nn, 0.0f);
+    for (int e = 0; e < n; e++) {
+        if (!(std::fabs(w[e]) > 0)) continue;
+        ettk_particle q = d[e];
+        std::memcpy(q.matrix, mats + (size_t)e * 16, sizeof(q.matrix));
+        M4 euler = m4_ident();
+        pre(euler, m4_rz(-q.normal[2]));
+        pre(euler, m4_rx(-q.normal[0]));
+        pre(euler, m4_rz(-q.normal[1]));
+        std::vector<float> cur = wedge_full(N, q.lwedge, q.uwedge, m4_inv(m4_mul(euler, m4_from(q.matrix))));
+        for (size_t o = 0; o < n3; o++) out.accum[o] += cur[o];
+        std::vector<float> sw = spherical_wedge(B, q);
+        for (size_t o = 0; o < nn; o++) sacc[o] += sw[o];
+    }
+    float mx = -FLT_MAX;
+    for (float v : out.accum) mx = std::max(mx, v);
+    out.bin.resize(n3);
+    double cnt = 0;
+    for (size_t o = 0; o < n3; o++) {
+        out.bin[o] = (double)out.accum[o] > (double)mx / 2.0 ? 1.0f : 0.0f;
+        cnt += out.bin[o];
+    }
+    out.effective = cnt < 0.95 * (double)n3;
+    out.wedge = out.bin;
+    if (out.effective) gauss_smooth3(out.wedge, N, N, N, 2.0, 0.5);
+    else std::fill(out.wedge.begin(), out.wedge.end(), 1.0f);
+    float smx = -FLT_MAX;
+    for (float v : sacc) smx = std::max(smx, v);
+    out.sph.resize(nn);
+    for (size_t o = 0; o < nn; o++) out.sph[o] = (double)sacc[o] > (double)smx / 2.0 ? 1.0f : 0.0f;
+    return out;
+}
+
 void plan_init(Plan& P, const ettk_params& p) {
     P.p = p;
     int N = p.N;
@@ -138,6 +208,30 @@ void plan_set_reference(Plan& P, const float* ref, const ettk_particle& d) {
     // (FIM.h:1123-1153) depends on W1*W2(t), so it is evaluated inside every ccc_eval call, as the reference does.
     if (P.p.missing_wedge) P.W1 = particle_wedge_half(P.N, d, nullptr);
     else P.W1.clear();
+    P.ref_wedge_kind = 0;
+    P.Wbin_acc.clear();
+    P.W1s_acc.clear();
+}
+
+// setInput1 with an average: its wedge is the accumulated one (getWedgeImageHalf takes k <= N/2 of
+// getWedgeImage, WAvg.h:555-611); the spherical window is its binarised spherical sum (ones when not effective)
+void plan_set_reference_average(Plan& P, const float* ref, const AvgWedge& aw) {
+    ettk_particle full{};
+    full.lwedge = -90.0f;
+    full.uwedge = 90.0f;
+    for (int k = 0; k < 16; k++) full.matrix[k] = (k % 5 == 0) ? 1.0 : 0.0;
+    plan_set_reference(P, ref, full);
+    const int N = P.N, H = P.H;
+    P.ref_wedge_kind = aw.effective ? 1 : 2;
+    if (P.p.missing_wedge) {
+        P.W1.assign((size_t)N * N * H, 1.0f);
+        if (aw.effective)
+            for (int i = 0; i < N; i++)
+                for (int j = 0; j < N; j++)
+                    for (int k = 0; k < H; k++) P.W1[((size_t)i * N + j) * H + k] = aw.wedge[((size_t)i * N + j) * N + k];
+    }
+    P.Wbin_acc = aw.bin;
+    P.W1s_acc = aw.effective ? aw.sph : std::vector<float>(aw.sph.size(), 1.0f);
 }
 
 // locateMaxima with blitz maxIndex (first maximum in row-major order)

@@ -76,6 +76,10 @@ struct Plan {
     std::vector<double> ref;      // I1
     ettk_particle refd;
     std::vector<float> W1;        // half-spectrum wedge of the reference
+    // reference that is an average: 0 = analytic wedge of refd, 1 = accumulated wedge (effective),
+    // 2 = accumulated wedge not effective (ones); binarised full grid and spherical window of kind 1
+    int ref_wedge_kind = 0;
+    std::vector<float> Wbin_acc, W1s_acc;
     std::vector<cd> F1m;
     double k1_m;
     std::vector<float> Pp1;       // S^2 projection of the reference
@@ -86,6 +90,14 @@ void plan_init(Plan& P, const ettk_params& p);
 void plan_set_reference(Plan& P, const float* ref, const ettk_particle& d);
 std::vector<float> wedge_half(int N, float lw, float uw, const M4& rot);   // Wedge.h:313-360
 std::vector<float> particle_wedge_half(int N, const ettk_particle& d, const M4* t);  // WSub.h:941-979
+std::vector<float> wedge_full(int N, float lw, float uw, const M4& rot);               // Wedge.h:195-290
+// accumulated wedge of an average (WAvg.h:49-70, 496-538, 614-667)
+struct AvgWedge {
+    std::vector<float> accum, bin, wedge, sph;  // N^3 sum, binarised, getWedgeImage; (2B)^2 binarised
+    bool effective = false;
+};
+AvgWedge avg_wedge(int N, int B, int n, const ettk_particle* d, const float* w, const double* mats);
+void plan_set_reference_average(Plan& P, const float* ref, const AvgWedge& aw);
 // executeFourierNewHalf: returns correlationPeak, updates t (t <- t * Translate(-shift))
 float ccc_eval(Plan& P, const float* part, const ettk_particle& d, M4& t);
 

@@ -14,6 +14,7 @@
 //   t(x) = [R0 Rx(x0) Ry(x1) Rz(x2) | tau0 + x3..5]  (degrees, voxels)
 //   H = band-pass filter x binary particle wedge (lab frame) x binary reference wedge (rotated by the seed)
 // Volumes are handled in the particle's memory order (x fastest): the half axis of the spectra is x.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -95,8 +96,15 @@ static std::vector<double> frame_weights(const Plan& P, const ettk_particle& d, 
     std::vector<double> Wt((size_t)N * N * H);
     double n1[3] = {0, 0, 0}, n2[3] = {0, 0, 0}, r1[3] = {0, 0, 0}, r2[3] = {0, 0, 0};
     bool pw = P.p.missing_wedge && raw_normals(d.lwedge, d.uwedge, n1, n2);
-    bool rw = false;
-    if (P.p.missing_wedge) {
+    bool rw = false, rlook = false;
+    double A3[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (P.p.missing_wedge && P.ref_wedge_kind == 1) {
+        // average: binarised accumulated wedge at the reference-frame frequency A k (nearest voxel, clamped)
+        M4 A = m4_strip_translation(m4_inv(m4_mul(euler_local(d), seed)));
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) A3[r * 3 + c] = A(r, c);
+        rlook = true;
+    } else if (P.p.missing_wedge && P.ref_wedge_kind == 0) {
         double m1[3], m2[3];
         if (raw_normals(P.refd.lwedge, P.refd.uwedge, m1, m2)) {
             // reference wedge normals in the reference frame (v = inverse(fin_ref) n), then in the particle
@@ -123,6 +131,14 @@ static std::vector<double> frame_weights(const Plan& P, const ettk_particle& d, 
                 double w = P.filter[h];  // radial: the same table in any axis order
                 if (pw && (n1[0] * kx + n1[1] * ky + n1[2] * kz) * (n2[0] * kx + n2[1] * ky + n2[2] * kz) > 0) w = 0;
                 if (rw && (r1[0] * kx + r1[1] * ky + r1[2] * kz) * (r2[0] * kx + r2[1] * ky + r2[2] * kz) > 0) w = 0;
+                if (rlook) {
+                    int q[3];
+                    for (int t = 0; t < 3; t++) {
+                        double pt = A3[t * 3] * kx + A3[t * 3 + 1] * ky + A3[t * 3 + 2] * kz;
+                        q[t] = std::min(std::max((int)std::nearbyint(pt) + N / 2, 0), N - 1);
+                    }
+                    if (P.Wbin_acc[((size_t)q[0] * N + q[1]) * N + q[2]] == 0.0f) w = 0;
+                }
                 Wt[h] = w;
             }
     return Wt;

@@ -45,6 +45,10 @@ def oracle() -> C.CDLL:
         o.orc_ccc_count.argtypes = [C.c_void_p]
         o.orc_average.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(Particle), C.c_void_p, C.c_void_p,
                                   C.c_void_p]
+        o.orc_avg_wedge.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(Particle), C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        o.orc_set_reference_average.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(Particle), C.c_void_p,
+                                                C.c_void_p]
         _o = o
     return _o
 
@@ -84,6 +88,14 @@ class OraclePlan:
         self._ref = np.ascontiguousarray(ref, dtype=np.float32)
         self.o.orc_set_reference(self.h, vp(self._ref), C.byref(desc or Particle()))
 
+    def set_reference_average(self, ref: np.ndarray, descs, weights, matrices):
+        from ettk_amd.abi import particle_array
+        self._ref = np.ascontiguousarray(ref, dtype=np.float32)
+        n = len(descs)
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        m = np.ascontiguousarray(np.asarray(matrices, dtype=np.float64).reshape(n, 16))
+        self.o.orc_set_reference_average(self.h, vp(self._ref), n, particle_array(descs), vp(w), vp(m))
+
     def ccc(self, vol: np.ndarray, desc: Particle, t: np.ndarray):
         vol = np.ascontiguousarray(vol, dtype=np.float32)
         tt = np.ascontiguousarray(np.asarray(t, dtype=np.float64).reshape(16)).copy()
@@ -119,6 +131,23 @@ class OraclePlan:
 
     def ccc_count(self) -> int:
         return self.o.orc_ccc_count(self.h)
+
+
+def avg_wedge(N: int, B: int, descs, weights, matrices):
+    """Accumulated wedge of an average: dict of accum, bin, wedge (N^3, k fastest), sph ((2B)^2), effective."""
+    from ettk_amd.abi import particle_array
+    o = oracle()
+    n = len(descs)
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    m = np.ascontiguousarray(np.asarray(matrices, dtype=np.float64).reshape(n, 16))
+    out = {k: np.zeros((N, N, N), np.float32) for k in ("accum", "bin", "wedge")}
+    sph = np.zeros((2 * B, 2 * B), np.float32)
+    eff = C.c_int()
+    o.orc_avg_wedge(N, B, n, particle_array(descs), vp(w), vp(m), vp(out["accum"]), vp(out["bin"]),
+                    vp(out["wedge"]), vp(sph), C.byref(eff))
+    out["sph"] = sph
+    out["effective"] = bool(eff.value)
+    return out
 
 
 def so3_correlate(B: int, sig: np.ndarray, pat: np.ndarray, scaled: bool = False) -> np.ndarray:

@@ -73,7 +73,10 @@ def test_cpp_metric_drop_in(torch_gpu, ctx, tmp_path):
                            matrix=A[j, i, 1:].astype(np.float64).reshape(4, 4))
                   for i in range(n)]
             for ri in range(R):
-                plan.set_reference(to_dev(torch_gpu, refs[ri]))
+                if ri == 0:  # after getImage reference 0 carries its volumes: its wedge is their accumulated one
+                    plan.set_reference_average(to_dev(torch_gpu, refs[0]), descs, w, mats)
+                else:
+                    plan.set_reference(to_dev(torch_gpu, refs[ri]))
                 rr = plan.distances(d, dj, type=2, form=1)
                 sums[:, j] += [x.distance for x in rr]
         assert np.array_equal(best, np.argmin(sums, axis=1)), (best, sums)
