@@ -1086,6 +1086,20 @@ int ettk_so3_correlate_batch(ettk_plan* P, int n, const double* sig, const doubl
     return ETTK_OK;
 }
 
+int ettk_sph_rotate_batch(ettk_plan* P, int n, const float* images, const double* transforms, float* out) {
+    try {
+        ETTK_REQUIRE(P && images && transforms && out && n > 0, ETTK_ERR_ARG, "ettk_sph_rotate_batch: bad argument");
+        ETTK_HIP(hipSetDevice(P->ctx->device));
+        so3_state(P);
+        sph_rotate(P, n, images, transforms, out);
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    }
+    set_error("");
+    return ETTK_OK;
+}
+
 int ettk_distances_batch(ettk_plan* P, const float* d_particles, int n, const ettk_particle* descs, int type,
                          int form, ettk_result* out) {
     try {

@@ -560,19 +560,11 @@ __global__ void k_extrema(const double* __restrict__ C, long n3, double* __restr
     }
 }
 
-// rotation of the pattern's coefficients by the Euler angles of the extrema (rotateFct, rotate_so3.c:695-751):
-// r(l,m) = sum_m' e^{-i m alpha} d^l_{m m'}(beta) e^{-i m' gamma} c(l,m').  Block (q, which): which = 0 min, 1 max.
-__global__ void k_rotate_coef(const double2* __restrict__ c, const int* __restrict__ pos, int B, double2* __restrict__ r) {
-    extern __shared__ double2 acc[];  // [l][m + B - 1]
-    const int q = blockIdx.x, which = blockIdx.y;
-    const int n = 2 * B, W = 2 * B - 1;
-    const int p = pos[q * 2 + which];
-    const int a = p / (n * n), b = (p / n) % n, g = p % n;
-    // angles are Pixel (float) in the reference
-    const double alpha = (double)(float)(a / (double)B * M_PI);
-    const double beta = (double)(float)((2 * b + 1.0) / (4. * B) * M_PI);
-    const double gamma = (double)(float)(g / (double)B * M_PI);
-    const double2* cc = c + (size_t)q * B * B;
+// rotateFct (rotate_so3.c:695-751) on the coefficients of one S^2 function:
+// r(l,m) = sum_m' e^{-i m alpha} d^l_{m m'}(beta) e^{-i m' gamma} c(l,m').  `acc` is B (2B-1) doubles2 of LDS.
+__device__ void rotate_coef_block(const double2* __restrict__ cc, double alpha, double beta, double gamma, int B,
+                                  double2* __restrict__ out, double2* acc) {
+    const int W = 2 * B - 1;
     for (int t = threadIdx.x; t < B * W; t += blockDim.x) acc[t] = make_double2(0, 0);
     __syncthreads();
     const double cb = cos(beta);
@@ -596,7 +588,6 @@ __global__ void k_rotate_coef(const double2* __restrict__ c, const int* __restri
         }
     }
     __syncthreads();
-    double2* out = r + ((size_t)q * 2 + which) * B * B;
     for (int t = threadIdx.x; t < B * B; t += blockDim.x) {
         int l = (int)sqrt((double)t);
         while (l * l > t) l--;
@@ -604,6 +595,30 @@ __global__ void k_rotate_coef(const double2* __restrict__ c, const int* __restri
         int m = t - l * l - l;
         out[t] = acc[l * W + (m + B - 1)];
     }
+}
+
+// the pattern's coefficients rotated by the Euler angles of the extrema of C (rotatedSphericalImageMinMax,
+// ProjRot.h:1558-1589).  Block (q, which): which = 0 min, 1 max.
+__global__ void k_rotate_coef(const double2* __restrict__ c, const int* __restrict__ pos, int B, double2* __restrict__ r) {
+    extern __shared__ double2 acc[];  // [l][m + B - 1]
+    const int q = blockIdx.x, which = blockIdx.y;
+    const int n = 2 * B;
+    const int p = pos[q * 2 + which];
+    const int a = p / (n * n), b = (p / n) % n, g = p % n;
+    // angles are Pixel (float) in the reference
+    const double alpha = (double)(float)(a / (double)B * M_PI);
+    const double beta = (double)(float)((2 * b + 1.0) / (4. * B) * M_PI);
+    const double gamma = (double)(float)(g / (double)B * M_PI);
+    rotate_coef_block(c + (size_t)q * B * B, alpha, beta, gamma, B, r + ((size_t)q * 2 + which) * B * B, acc);
+}
+
+// coefficients of image q rotated by the angles ang[q*3 .. q*3+2] (harmonics(A, alpha, beta, gamma, R),
+// ProjRot.h:1457-1554)
+__global__ void k_rotate_coef_angles(const double2* __restrict__ c, const double* __restrict__ ang, int B,
+                                     double2* __restrict__ r) {
+    extern __shared__ double2 acc[];
+    const int q = blockIdx.x;
+    rotate_coef_block(c + (size_t)q * B * B, ang[q * 3], ang[q * 3 + 1], ang[q * 3 + 2], B, r + (size_t)q * B * B, acc);
 }
 
 // wMin / wMax = sum F1 * R * sin(theta_i) and the affine rescale of C (ProjRot.h:1303-1318)
@@ -859,6 +874,74 @@ void run_soft(ettk_plan* P, const std::vector<SoftInst>& inst, const float* d_Pp
     soft_core(P, inst, d_C);
 }
 
+// Euler angles of harmonics(A, t, R) (ProjRot.h:1359-1448): psi, phi from atan of element ratios of the rotation
+// part of t, theta = acos(M22), the quadrant and the sign of theta chosen from the signs of M21, M12 and
+// cos(psi) M20, in Pixel (float) arithmetic; returns the rotateFct angles (-psi, -theta, -phi)
+static void harmonics_angles(const double* t, double ang[3]) {
+    const double m20 = t[8], m21 = t[9], m12 = t[6], m02 = t[2], m22 = t[10];
+    const double pmin = (double)FLT_MIN;
+    float psi, phi, theta;
+    if (std::fabs(m20) < pmin) {
+        if (std::fabs(m21) < pmin) psi = 0;
+        else if (m21 < -pmin) psi = (float)(M_PI / 2.0);
+        else psi = (float)(-M_PI / 2.0);
+    } else {
+        psi = (float)std::atan(-m21 / m20);
+    }
+    if (std::fabs(m02) < pmin) {
+        if (std::fabs(m12) < pmin) phi = 0;
+        else if (m12 > pmin) phi = (float)(M_PI / 2.0);
+        else phi = (float)(-M_PI / 2.0);
+    } else {
+        phi = (float)std::atan(m12 / m02);
+    }
+    theta = (float)std::acos(m22);
+    const float psi_1 = psi, psi_2 = psi > 0 ? (float)(psi - M_PI) : (float)(psi + M_PI);
+    const float phi_1 = phi, phi_2 = phi > 0 ? (float)(phi - M_PI) : (float)(phi + M_PI);
+    psi = m21 * psi_1 > 0 ? psi_1 : psi_2;
+    phi = m12 * phi_1 > 0 ? phi_1 : phi_2;
+    if (std::cos(psi) * m20 > 0) {
+        theta = -theta;
+        psi = m21 * psi_1 < 0 ? psi_1 : psi_2;
+        phi = m12 * phi_1 < 0 ? phi_1 : phi_2;
+    }
+    ang[0] = -(double)psi;
+    ang[1] = -(double)theta;
+    ang[2] = -(double)phi;
+}
+
+// harmonics(A, t, R) for n (2B x 2B) float images: S^2 forward transform, rotateFct(B, B, B-1) of the
+// coefficients, inverse transform; the real part is returned as float.  (The reference's t == NULL case, R = A,
+// is left to the caller: every image here goes through the transform, an identity t by zero angles.)
+void sph_rotate(ettk_plan* P, int n, const float* h_img, const double* h_t, float* h_out) {
+    So3State& S = so3(P);
+    const int B = S.B, ns = 2 * B;
+    const size_t nn = (size_t)ns * ns;
+    hipStream_t s = P->ctx->stream;
+    std::vector<double> img((size_t)n * nn), ang((size_t)n * 3);
+    for (size_t q = 0; q < img.size(); q++) img[q] = (double)h_img[q];
+    for (int i = 0; i < n; i++) harmonics_angles(h_t + (size_t)i * 16, &ang[(size_t)i * 3]);
+    DevBuf<double> f, a, out;
+    DevBuf<double2> fm, c, r, g;
+    f.upload(img.data(), img.size(), s);
+    a.upload(ang.data(), ang.size(), s);
+    fm.ensure((size_t)n * nn);
+    c.ensure((size_t)n * B * B);
+    r.ensure((size_t)n * B * B);
+    g.ensure((size_t)n * nn);
+    out.ensure((size_t)n * nn);
+    k_ring_dft<<<dim3(16, n), kT, 0, s>>>(f.p, S.tw.p, B, fm.p);
+    k_legendre_fwd<<<dim3(8, n), kT, 0, s>>>(fm.p, S.legF.p, B, c.p);
+    k_rotate_coef_angles<<<n, kT, (size_t)B * (2 * B - 1) * sizeof(double2), s>>>(c.p, a.p, B, r.p);
+    k_legendre_inv<<<dim3(16, n), kT, 0, s>>>(r.p, S.legI.p, B, g.p);
+    k_ring_idft<<<dim3(16, n), kT, 0, s>>>(g.p, S.tw.p, B, out.p);
+    ETTK_HIP(hipGetLastError());
+    std::vector<double> h((size_t)n * nn);
+    ETTK_HIP(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    ETTK_HIP(hipStreamSynchronize(s));
+    for (size_t q = 0; q < h.size(); q++) h_out[q] = (float)h[q];
+}
+
 void soft_given(ettk_plan* P, int nq, const double* h_sig, const double* h_pat, double* d_C) {
     So3State& S = so3(P);
     const int B = S.B, n = 2 * B;
@@ -874,6 +957,7 @@ void soft_given(ettk_plan* P, int nq, const double* h_sig, const double* h_pat, 
 static void so3_kernel_attributes(int B) {
     int shm = B * (2 * B - 1) * (int)sizeof(double2);
     ETTK_HIP(hipFuncSetAttribute((const void*)k_rotate_coef, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+    ETTK_HIP(hipFuncSetAttribute((const void*)k_rotate_coef_angles, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
     if (fused_synthesis(B)) {
         const int n = 2 * B, fs = n * (n + 1) * (int)sizeof(double2);
         const void* f = n == 16 ? (const void*)k_so3_synth_fft<4> : n == 32 ? (const void*)k_so3_synth_fft<5>

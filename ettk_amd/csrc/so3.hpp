@@ -71,6 +71,8 @@ void project_images(ettk_plan* P, const float* d_parts, const ettk_particle* des
 void run_soft(ettk_plan* P, const std::vector<SoftInst>& inst, const float* d_Pp2, const float* d_W2, const float* d_Mx,
               double* d_C);
 void soft_given(ettk_plan* P, int nq, const double* h_sig, const double* h_pat, double* d_C);
+// harmonics(A, t, R) (ProjRot.h:1348-1454) of n (2B x 2B) host images by the rotation parts of h_t[i*16]
+void sph_rotate(ettk_plan* P, int n, const float* h_img, const double* h_t, float* h_out);
 So3State& so3_state(ettk_plan* P);
 void launch_combine(const double* C, const ComboOp* d_ops, int np, int B, int b0, double* out, hipStream_t s);
 void launch_localmax(const double* C, int np, int B, int cap, int* count, int* idx, double* val, hipStream_t s);

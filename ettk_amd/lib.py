@@ -34,6 +34,7 @@ def load() -> C.CDLL:
             "ettk_ccc_eval_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, ip, dp, fp, dp], C.c_int),
             "ettk_project_batch": ([vp, vp, C.c_int, C.POINTER(Particle), fp], C.c_int),
             "ettk_so3_correlate_batch": ([vp, C.c_int, dp, dp, dp], C.c_int),
+            "ettk_sph_rotate_batch": ([vp, C.c_int, fp, dp, fp], C.c_int),
             "ettk_align_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, C.POINTER(Result)], C.c_int),
             "ettk_distances_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, C.c_int, C.POINTER(Result)],
                                      C.c_int),
@@ -146,6 +147,15 @@ class Plan:
         n = sig.shape[0]
         out = np.zeros((n, 2 * B, 2 * B, 2 * B), np.float64)
         check(self.lib.ettk_so3_correlate_batch(self.h, n, _dp(sig), _dp(pat), _dp(out)))
+        return out
+
+    def sph_rotate(self, images: np.ndarray, transforms) -> np.ndarray:
+        """harmonics(A, t, R) of n (2B, 2B) spherical images (ettk_sph_rotate_batch)."""
+        img = np.ascontiguousarray(images, dtype=np.float32)
+        n = img.shape[0]
+        t = np.ascontiguousarray(np.asarray(transforms, dtype=np.float64).reshape(n, 16))
+        out = np.zeros_like(img)
+        check(self.lib.ettk_sph_rotate_batch(self.h, n, _fp(img), _dp(t), _fp(out)))
         return out
 
     def align(self, d_parts, descs, mode: int = 0):

@@ -127,6 +127,14 @@ int ettk_project_batch(ettk_plan* plan, const float* d_particles, int n, const e
    pat[i] (2B x 2B doubles, host) give out_c[i] ((2B)^3 doubles, layout [alpha][beta][gamma]). */
 int ettk_so3_correlate_batch(ettk_plan* plan, int n, const double* sig, const double* pat, double* out_c);
 
+/* nbfProjectionRotationMetric3D::harmonics(A, t, R) (ProjRot.h:1348-1454): each (2B x 2B) spherical image
+   images[i] (host floats, row-major [polar][azimuth]) rotated on the sphere by the rotation part of
+   transforms[i*16]: Euler angles (psi, theta, phi) taken from the matrix as the reference does (including its
+   choice psi = phi = 0 when the rotation is about z alone), then rotateFct(B, B, B-1) by (-psi, -theta, -phi)
+   (soft-1.0 rotate_so3.c:695-751); out[i] receives the real part.  Used by the reference for stored spherical
+   wedges (nbfWedge3D::getSphericalImage, Wedge.h:364-383) and averages' spherical wedges (WAvg.h:680-683). */
+int ettk_sph_rotate_batch(ettk_plan* plan, int n, const float* images, const double* transforms, float* out);
+
 /* Full pairwise alignment of n particles against the reference: nbfProjectionRotationMetric3D::execute
    (ProjRot.h:416-1052) as called from getDistances type 0/3 (CorrMetric.h:1874-2008).
    mode 0 = plain: particle by particle, Powell refinement driven from the host (the reference method);

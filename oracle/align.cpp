@@ -11,6 +11,8 @@
 // vnl_brent_minimizer.cxx) and nbfWedgedAverageImage3D::getImageMPI/getImage (WAvg.h:217-356).
 #include "oracle_core.hpp"
 
+#include <limits>
+
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -523,6 +525,33 @@ void orc_so3_correlate_scaled(int B, const double* sig, const double* pat, doubl
     std::memcpy(C, c.data(), c.size() * sizeof(double));
 }
 void orc_s2_rotate(int B, const double* in, double* out, double a, double b, double g) { s2_rotate(B, in, out, a, b, g); }
+
+// harmonics(A, t, R) (ProjRot.h:1348-1454): Euler angles of the rotation part of t in Pixel (float) arithmetic
+// (:1369-1448), rotateFct by (-psi, -theta, -phi) of the image as REAL (double), result cast to Pixel
+void orc_harmonics_t(int B, const float* A, const double* t, float* R) {
+    const double m20 = t[8], m21 = t[9], m12 = t[6], m02 = t[2], m22 = t[10];
+    const double fmin = (double)std::numeric_limits<float>::min();
+    float psi, phi, theta;
+    if (std::fabs(m20) < fmin) psi = std::fabs(m21) < fmin ? 0.0f : (m21 < -fmin ? (float)(M_PI / 2.0) : (float)(-M_PI / 2.0));
+    else psi = (float)std::atan(-m21 / m20);
+    if (std::fabs(m02) < fmin) phi = std::fabs(m12) < fmin ? 0.0f : (m12 > fmin ? (float)(M_PI / 2.0) : (float)(-M_PI / 2.0));
+    else phi = (float)std::atan(m12 / m02);
+    theta = (float)std::acos(m22);
+    float psi_1 = psi, psi_2 = psi > 0 ? (float)(psi - M_PI) : (float)(psi + M_PI);
+    float phi_1 = phi, phi_2 = phi > 0 ? (float)(phi - M_PI) : (float)(phi + M_PI);
+    psi = (m21 * psi_1 > 0) ? psi_1 : psi_2;   // theta > 0
+    phi = (m12 * phi_1 > 0) ? phi_1 : phi_2;
+    if (std::cos(psi) * m20 > 0) {              // theta < 0
+        theta = -theta;
+        psi = (m21 * psi_1 < 0) ? psi_1 : psi_2;
+        phi = (m12 * phi_1 < 0) ? phi_1 : phi_2;
+    }
+    const int n = 2 * B;
+    std::vector<double> a((size_t)n * n), r((size_t)n * n);
+    for (size_t i = 0; i < a.size(); i++) a[i] = (double)A[i];
+    s2_rotate(B, a.data(), r.data(), -(double)psi, -(double)theta, -(double)phi);
+    for (size_t i = 0; i < r.size(); i++) R[i] = (float)r[i];
+}
 void orc_wigner_d(int B, double beta, double* d) {
     std::vector<double> v;
     wigner_d_all(B, beta, v);

@@ -45,6 +45,7 @@ def oracle() -> C.CDLL:
         o.orc_ccc_count.argtypes = [C.c_void_p]
         o.orc_average.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(Particle), C.c_void_p, C.c_void_p,
                                   C.c_void_p]
+        o.orc_harmonics_t.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         o.orc_avg_wedge.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(Particle), C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         o.orc_set_reference_average.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(Particle), C.c_void_p,

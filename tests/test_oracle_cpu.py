@@ -323,3 +323,60 @@ def test_refine_score_is_one_at_the_identity():
     S2 = o.orc_refine_eval(orc.h, vp(ref), C.byref(Particle()), vp(np.eye(4).ravel().copy()),
                            vp(np.array([1.0, 0, 0, 0.2, 0, 0])), 1e-4, None)
     assert S2 < S
+
+
+# ------------------------------------------------------------------------------------- harmonics(A, t, R)
+def _rz(a):
+    c, s = math.cos(a), math.sin(a)
+    M = np.eye(4)
+    M[:2, :2] = [[c, -s], [s, c]]
+    return M
+
+
+def _ry(b):
+    c, s = math.cos(b), math.sin(b)
+    M = np.eye(4)
+    M[0, 0], M[0, 2], M[2, 0], M[2, 2] = c, s, -s, c
+    return M
+
+
+def _harmonics_t(B, A, t):
+    out = np.zeros((2 * B, 2 * B), np.float32)
+    a32 = np.ascontiguousarray(A, dtype=np.float32)
+    oracle().orc_harmonics_t(B, vp(a32), vp(np.ascontiguousarray(t, dtype=np.float64)), vp(out))
+    return out
+
+
+def _rotate(B, A, a, b, g):
+    f = np.ascontiguousarray(np.asarray(A, dtype=np.float32).astype(np.float64))
+    out = np.zeros_like(f)
+    oracle().orc_s2_rotate(B, vp(f), vp(out), a, b, g)
+    return out.astype(np.float32)
+
+
+@pytest.mark.parametrize("a,b,c", [(0.4, 1.1, -0.7), (2.5, 0.6, 1.9), (-2.2, 2.3, -2.8), (1.0, -0.8, 0.3),
+                                   (-1.3, -2.0, 2.6), (0.4, 1.1, 0.0), (-2.0, 0.7, 0.0), (0.0, 0.9, 1.2)])
+def test_harmonics_of_a_transform_is_the_inverse_zyz_rotation(a, b, c):
+    """For t = Rz(a) Ry(b) Rz(c) the Euler angles read back from t give rotateFct(-c, -b, -a): the rotation by
+    the inverse of t, whichever branch the sign tests select (c = 0 makes M21 vanish and takes the theta < 0
+    branch).  With a = 0 exactly, M12 = 0 and the reference's tests pick phi = pi instead of 0
+    (phi_1 = 0 is not > 0, ProjRot.h:1409-1424): the image is then turned by an extra pi about z."""
+    B = 8
+    A = _bandlimited(B, 21).reshape(2 * B, 2 * B)
+    t = _rz(a) @ _ry(b) @ _rz(c)
+    t[:3, 3] = (3.0, -1.0, 2.0)  # the translation is removed first
+    got = _harmonics_t(B, A, t)
+    want = _rotate(B, A, -c, -b, -math.pi if a == 0.0 else -a)
+    assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+    # and it is not the rotation by t itself
+    assert np.abs(got - _rotate(B, A, a, b, c)).max() > 1e-2 * np.abs(want).max()
+
+
+def test_harmonics_degenerate_transforms_keep_the_image():
+    """Identity, and a rotation about z alone: every ratio's denominator and numerator vanish, psi = phi = 0 and
+    theta = 0, so the reference leaves the image unrotated (ProjRot.h:1371-1397)."""
+    B = 8
+    A = _bandlimited(B, 22).reshape(2 * B, 2 * B).astype(np.float32)
+    for t in (np.eye(4), _rz(0.9)):
+        got = _harmonics_t(B, A, t)
+        assert np.abs(got - A).max() <= 1e-5 * np.abs(A).max()

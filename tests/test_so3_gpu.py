@@ -31,6 +31,40 @@ def test_so3_correlation_matches_oracle(torch_gpu, ctx, B):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("B", [16, 32])
+def test_spherical_rotation_matches_oracle(torch_gpu, ctx, B):
+    """harmonics(A, t, R): Euler angles read from t on the host, FP64 S^2 transform / rotateFct / inverse on the
+    GPU, against the oracle (its rotation pinned to soft-1.0, its angle reading to the ZYZ closed form)."""
+    import ctypes as C
+
+    from ettk_amd.lib import Plan
+    from ettk_amd.synth import random_rotation
+    from oracle_lib import oracle, vp
+
+    print(f"[sph rotate] B={B}", flush=True)
+    plan = Plan(ctx, default_params(32, B))
+    try:
+        g = np.random.default_rng(B + 1)
+        n = 2 * B
+        imgs = (g.random((5, n, n)) > 0.4).astype(np.float32)  # binary wedge-like images
+        ts = []
+        for i in range(5):
+            t = np.eye(4)
+            t[:3, :3] = random_rotation(g)
+            t[:3, 3] = g.uniform(-2, 2, 3)
+            ts.append(t)
+        ts[4] = np.eye(4)  # degenerate branch: zero angles
+        out = plan.sph_rotate(imgs, ts)
+        o = oracle()
+        for i in range(5):
+            want = np.zeros((n, n), np.float32)
+            o.orc_harmonics_t(B, vp(np.ascontiguousarray(imgs[i])), vp(np.ascontiguousarray(ts[i])), vp(want))
+            assert np.max(np.abs(out[i] - want)) <= 1e-5 * max(1.0, np.max(np.abs(want))), i
+    finally:
+        plan.close()
+
+
+@pytest.mark.gpu
 def test_projection_matches_oracle(torch_gpu, ctx):
     """cartesianToSpherical: FP32 (2N)^3 FFT + tricubic S^2 sampling against the FP64 oracle."""
     from ettk_amd.lib import Plan
