@@ -54,7 +54,7 @@ class Result(C.Structure):
         ("cand_distance", C.c_float * ETTK_MAX_CANDIDATES),
         ("cand_matrix", (C.c_double * 16) * ETTK_MAX_CANDIDATES),
         ("so3_angles", (C.c_float * 3) * ETTK_MAX_CANDIDATES),
-        ("n_ccc", C.c_int), ("use_window", C.c_int),
+        ("n_ccc", C.c_int), ("use_window", C.c_int), ("overlap", C.c_float),
     ]
 
     def transform(self) -> np.ndarray:

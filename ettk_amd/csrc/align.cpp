@@ -801,6 +801,15 @@ class BatchEval {
 void refine_adjoint_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n,
                           std::vector<SearchOut*>& so, ettk_result* out);
 
+// result.overlap: getWedgeOverlap at the reported transform of each result (particle descriptors as the run used)
+static void set_overlaps(ettk_plan* P, const ettk_particle* descs, int n, ettk_result* out) {
+    std::vector<Mat4> tf(n);
+    std::vector<float> ov(n);
+    for (int i = 0; i < n; i++) tf[i] = Mat4::from(out[i].matrix);
+    wedge_overlaps(P, descs, tf.data(), n, ov.data());
+    for (int i = 0; i < n; i++) out[i].overlap = ov[i];
+}
+
 void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, int mode, ettk_result* out) {
     ETTK_REQUIRE(P->have_ref, ETTK_ERR_STATE, "reference not set");
     ETTK_REQUIRE(mode >= 0 && mode <= 2, ETTK_ERR_ARG, "ettk_align_batch: mode must be 0, 1 or 2");
@@ -853,6 +862,7 @@ void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
             refine_adjoint_batch(P, d_parts, descs, n, sp, out);
         }
     }
+    set_overlaps(P, descs, n, out);
     ETTK_HIP(hipEventRecord(b, P->ctx->stream));
     ETTK_HIP(hipEventSynchronize(b));
     float ms = 0;
@@ -979,6 +989,7 @@ void distances_run(ettk_plan* P, const float* d_parts, const ettk_particle* desc
             out[i].n_cand = 1;
             out[i].n_ccc = 1;
         }
+        set_overlaps(P, descs, n, out);
         return;
     }
     // type 1: the stored transform becomes the starting point, the volumes are resliced without it
@@ -1039,6 +1050,7 @@ void distances_run(ettk_plan* P, const float* d_parts, const ettk_particle* desc
         out[i].n_cand = 1;
         out[i].n_ccc = nccc[i];
     }
+    set_overlaps(P, dz.data(), n, out);
 }
 
 }  // namespace ettk

@@ -246,6 +246,48 @@ void launch_spec_filter(const float2* F, int f_batch_stride, float2* X, const fl
     ETTK_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ wedge overlap
+// nbfCorrelationImageMetric::getWedgeOverlap (CorrMetric.h:533-566): sum(W1 W2(t) filter) over the half grid,
+// W2 the particle's smoothed wedge at the evaluation's transform (all ones when geom.full).  Per-block FP64
+// partials, then one sum per evaluation.
+__global__ void __launch_bounds__(kThreads) k_wedge_overlap(const float* __restrict__ W1, const float* __restrict__ filter,
+                                                             const EvalGeom* __restrict__ geom, int N,
+                                                             double* __restrict__ part) {
+    __shared__ double sh[kThreads / 64];
+    const int e = blockIdx.y, H = N / 2 + 1;
+    const unsigned nh = (unsigned)N * N * H;
+    const EvalGeom& g = geom[e];
+    double acc = 0.0;
+    for (unsigned h = blockIdx.x * kThreads + threadIdx.x; h < nh; h += gridDim.x * kThreads) {
+        int i, j, k;
+        split3(h, (unsigned)N, (unsigned)H, i, j, k);
+        float w = W1[h] * (g.full ? 1.f : wedge_smooth(g.v1, g.v2, i, j, k, N, H));
+        acc += (double)(w * filter[h]);
+    }
+    for (int d = 32; d > 0; d >>= 1) acc += __shfl_xor(acc, d);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int q = 0; q < kThreads / 64; q++) t += sh[q];
+        part[(size_t)e * gridDim.x + blockIdx.x] = t;
+    }
+}
+__global__ void k_sum_partials(const double* __restrict__ part, int nb, int n_eval, double* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_eval) return;
+    double t = 0.0;
+    for (int b = 0; b < nb; b++) t += part[(size_t)e * nb + b];
+    out[e] = t;
+}
+void launch_wedge_overlap(const float* W1, const float* filter, const EvalGeom* geom, int E, int N, double* part,
+                          double* out, hipStream_t s) {
+    const int nb = 64;
+    k_wedge_overlap<<<dim3(nb, E), kThreads, 0, s>>>(W1, filter, geom, N, part);
+    k_sum_partials<<<(E + 127) / 128, 128, 0, s>>>(part, nb, E, out);
+    ETTK_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ reference-side real-space mask
 __global__ void __launch_bounds__(kThreads) k_mask_ref(float* __restrict__ R, const float* __restrict__ m1, int N,
                                                         float* __restrict__ partials) {

@@ -296,6 +296,47 @@ EvalGeom make_geom_public(const ettk_plan* P, const ettk_particle& d, const Mat4
     return g;
 }
 
+// getWedgeOverlap (CorrMetric.h:533-566) of the reference against particle descs[i] at transform t[i]: the
+// wedges are used whatever the missing-wedge setting; 1 when neither wedge is effective (nbfWedge3D::isEffective,
+// Wedge.h:70: limits other than exactly -90 / 90 after the clamp of set())
+void wedge_overlaps(ettk_plan* P, const ettk_particle* descs, const Mat4* t, int n, float* out) {
+    if (n <= 0) return;
+    hipStream_t s = P->ctx->stream;
+    const int N = P->N;
+    auto effective = [](float lw, float uw) {
+        const float l = lw < -90 ? -90.0f : lw, u = uw > 90 ? 90.0f : uw;
+        return l != -90.0f || u != 90.0f;
+    };
+    const bool eff1 = P->ref_wedge_kind == 0 ? effective(P->refd.lwedge, P->refd.uwedge) : P->ref_wedge_kind == 1;
+    std::vector<EvalGeom> g(n);
+    std::vector<int> eff2(n);
+    for (int i = 0; i < n; i++) {
+        const ettk_particle& d = descs[i];
+        Mat4 fin = euler_of(d) * Mat4::from(d.matrix) * t[i];
+        if (mat_det(fin) == 0.0) fin = Mat4::ident();
+        WedgeRef w{};
+        wedge_normals(d.lwedge, d.uwedge, mat_inverse(fin), w);
+        EvalGeom& e = g[i];
+        e = EvalGeom{};
+        for (int r = 0; r < 3; r++) {
+            e.v1[r] = w.v1[r];
+            e.v2[r] = w.v2[r];
+        }
+        eff2[i] = effective(d.lwedge, d.uwedge);
+        e.full = (w.full || !eff2[i]) ? 1 : 0;
+    }
+    DevBuf<EvalGeom> dg;
+    DevBuf<double> part, sums;
+    dg.upload(g.data(), n, s);
+    part.ensure((size_t)n * 64);
+    sums.ensure(n);
+    launch_wedge_overlap(P->W1.p, P->filter.p, dg.p, n, N, part.p, sums.p, s);
+    std::vector<double> h(n);
+    ETTK_HIP(hipMemcpyAsync(h.data(), sums.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    ETTK_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < n; i++) out[i] = (eff1 || eff2[i]) ? (float)(h[i] / P->sum_filter) : 1.0f;
+}
+
 void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx, Mat4* t,
                float* dist) {
     ETTK_REQUIRE(P->have_ref, ETTK_ERR_STATE, "reference not set");

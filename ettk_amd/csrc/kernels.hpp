@@ -82,6 +82,9 @@ void launch_refine_adj(float2* X1, const float2* FP, const float* filter, const 
 void launch_refine_moments(const float* ref, const RefineGeom* geom, const float* G, const float* maskfft, int E, int N,
                            float* mpart, double* mom, hipStream_t s);
 void launch_wedge_ref(const WedgeRef& w, int N, float* W, hipStream_t s);
+// out[e] = sum over the half grid of W1 W2(e) filter (FP64); `part` holds 64 doubles per evaluation
+void launch_wedge_overlap(const float* W1, const float* filter, const EvalGeom* geom, int E, int N, double* part,
+                          double* out, hipStream_t s);
 void launch_xfast_to_internal(const float* in, float* out, int N, hipStream_t s);
 void launch_internal_to_xfast(const float* in, float* out, int N, hipStream_t s);
 void launch_mcf_inplace(float2* F, long n, hipStream_t s);

@@ -108,6 +108,8 @@ namespace ettk {
 void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
                Mat4* t, float* dist);
 void ensure_ccc_workspace(ettk_plan* P, int cap);
+// getWedgeOverlap of the reference against descs[i] at t[i] (CorrMetric.h:533-566)
+void wedge_overlaps(ettk_plan* P, const ettk_particle* descs, const Mat4* t, int n, float* out);
 void so3_state_destroy(ettk_plan* P);
 void so3_reference_set(ettk_plan* P);
 void wedge_normals(float lw, float uw, const Mat4& t, WedgeRef& w);

@@ -78,6 +78,10 @@ typedef struct ettk_result {
     float  so3_angles[ETTK_MAX_CANDIDATES][3];       /* (alpha, beta, gamma) of each refined candidate */
     int    n_ccc;                                    /* constrained-CCC evaluations spent on the pair  */
     int    use_window;                               /* bit0: reference window, bit1: particle window  */
+    float  overlap;                                  /* getWedgeOverlap(getTransform()), CorrMetric.h:
+                                                        533-566: sum(W1 W2 filter) / sum(filter) at the
+                                                        best transform, 1 when neither wedge is effective;
+                                                        the correlation scale of these metrics is 1      */
 } ettk_result;
 
 typedef struct ettk_ctx  ettk_ctx;

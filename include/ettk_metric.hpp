@@ -171,8 +171,10 @@ class CorrelationImageMetric {
                       DistanceArray& D, int type) {
         if (D.rows() != (int)refs.size() || D.cols() != (int)vols.size())
             throw MetricError(ETTK_ERR_ARG, "getDistances: D must be refs x vols");
-        if (D.depth() != 17)
-            throw MetricError(ETTK_ERR_UNSUPPORTED, "getDistances: only the 17-column layout {distance, M16}");
+        if (D.depth() < 17 || D.depth() > 20)
+            throw MetricError(ETTK_ERR_ARG, "getDistances: D rows hold 17 to 20 columns");
+        // column layout of CorrMetric.h:1966-2000: distance, [correlation scale], [wedge overlap], M16
+        const int off = 1 + (D.depth() > 17) + (D.depth() > 18);
         const int K = D.extent3();
         if (K < 1 || K > ETTK_MAX_CANDIDATES) throw MetricError(ETTK_ERR_ARG, "getDistances: bad candidate count");
         if (type != 1 && type != 2 && K != p_.n_candidates)
@@ -194,12 +196,16 @@ class CorrelationImageMetric {
                 const ettk_result& r = res[q];
                 if (K == 1) {
                     D(i, j, 0, 0) = r.distance;
-                    for (int c = 0; c < 16; c++) D(i, j, 1 + c, 0) = (float)r.matrix[c];
+                    if (off > 1) D(i, j, 1, 0) = 1.0f;        // getCorrelationScale: 1 for these metrics
+                    if (off > 2) D(i, j, 2, 0) = r.overlap;   // getWedgeOverlap(getTransform())
+                    for (int c = 0; c < 16; c++) D(i, j, off + c, 0) = (float)r.matrix[c];
                 } else {
                     for (int k = 0; k < K; k++) D(i, j, 0, k) = std::numeric_limits<float>::max();
                     for (int k = 0; k < std::min(K, r.n_cand); k++) {
                         D(i, j, 0, k) = r.cand_distance[k];
-                        for (int c = 0; c < 16; c++) D(i, j, 1 + c, k) = (float)r.cand_matrix[k][c];
+                        if (off > 1) D(i, j, 1, k) = 1.0f;    // candidate scales and overlaps are stored
+                        if (off > 2) D(i, j, 2, k) = 1.0f;    //   as 1 (ProjRot.h:879-880, FIM.h:1872-1876)
+                        for (int c = 0; c < 16; c++) D(i, j, off + c, k) = (float)r.cand_matrix[k][c];
                     }
                 }
             }

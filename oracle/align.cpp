@@ -600,6 +600,10 @@ void orc_set_reference_average(void* h, const float* ref, int n, const ettk_part
     project_image(P, P.ref, P.Pp1);
 }
 
+float orc_wedge_overlap(void* h, const ettk_particle* d, const double* t) {
+    return wedge_overlap(*(Plan*)h, *d, m4_from(t));
+}
+
 void orc_masks(void* h, float* maskfft, float* refmask, float* filter) {
     Plan& P = *(Plan*)h;
     std::memcpy(maskfft, P.maskfft.data(), P.maskfft.size() * sizeof(float));

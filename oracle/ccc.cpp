@@ -230,8 +230,35 @@ void plan_set_reference_average(Plan& P, const float* ref, const AvgWedge& aw) {
                 for (int j = 0; j < N; j++)
                     for (int k = 0; k < H; k++) P.W1[((size_t)i * N + j) * H + k] = aw.wedge[((size_t)i * N + j) * N + k];
     }
+    P.W1acc_half.assign((size_t)N * N * H, 1.0f);
+    if (aw.effective)
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < N; j++)
+                for (int k = 0; k < H; k++) P.W1acc_half[((size_t)i * N + j) * H + k] = aw.wedge[((size_t)i * N + j) * N + k];
     P.Wbin_acc = aw.bin;
     P.W1s_acc = aw.effective ? aw.sph : std::vector<float>(aw.sph.size(), 1.0f);
+}
+
+// getWedgeOverlap (CorrMetric.h:533-566): wedge images of the inputs whose wedge is effective
+// (nbfWedge3D::isEffective, Wedge.h:70), sum(w1 w2 filter) / sum(filter) over the half grid, 1 when neither is
+float wedge_overlap(const Plan& P, const ettk_particle& d, const M4& t) {
+    auto eff = [](float lw, float uw) {
+        const float l = lw < -90 ? -90.0f : lw, u = uw > 90 ? 90.0f : uw;
+        return l != -90.0f || u != 90.0f;
+    };
+    const bool e1 = P.ref_wedge_kind == 0 ? eff(P.refd.lwedge, P.refd.uwedge) : P.ref_wedge_kind == 1;
+    const bool e2 = eff(d.lwedge, d.uwedge);
+    if (!e1 && !e2) return 1.0f;
+    const size_t nh = (size_t)P.N * P.N * P.H;
+    std::vector<float> w1(nh, 1.0f), w2(nh, 1.0f);
+    if (e1) w1 = P.ref_wedge_kind == 0 ? particle_wedge_half(P.N, P.refd, nullptr) : P.W1acc_half;
+    if (e2) w2 = particle_wedge_half(P.N, d, &t);
+    double a = 0, b = 0;
+    for (size_t h = 0; h < nh; h++) {
+        a += (double)(w1[h] * w2[h] * P.filter[h]);
+        b += P.filter[h];
+    }
+    return (float)(a / b);
 }
 
 // locateMaxima with blitz maxIndex (first maximum in row-major order)

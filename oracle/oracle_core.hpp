@@ -79,7 +79,7 @@ struct Plan {
     // reference that is an average: 0 = analytic wedge of refd, 1 = accumulated wedge (effective),
     // 2 = accumulated wedge not effective (ones); binarised full grid and spherical window of kind 1
     int ref_wedge_kind = 0;
-    std::vector<float> Wbin_acc, W1s_acc;
+    std::vector<float> Wbin_acc, W1s_acc, W1acc_half;
     std::vector<cd> F1m;
     double k1_m;
     std::vector<float> Pp1;       // S^2 projection of the reference
@@ -98,6 +98,8 @@ struct AvgWedge {
 };
 AvgWedge avg_wedge(int N, int B, int n, const ettk_particle* d, const float* w, const double* mats);
 void plan_set_reference_average(Plan& P, const float* ref, const AvgWedge& aw);
+// nbfCorrelationImageMetric::getWedgeOverlap(t) (CorrMetric.h:533-566) of the reference against particle d
+float wedge_overlap(const Plan& P, const ettk_particle& d, const M4& t);
 // executeFourierNewHalf: returns correlationPeak, updates t (t <- t * Translate(-shift))
 float ccc_eval(Plan& P, const float* part, const ettk_particle& d, M4& t);
 

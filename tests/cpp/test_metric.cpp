@@ -64,6 +64,16 @@ int main(int argc, char** argv) {
                     for (int c = 0; c < 17; c++) o << D(r, i, c, 0) << (c < 16 ? " " : "\n");
                 }
         }
+        // the 20-column row of type 2: distance, correlation scale, wedge overlap, M16, one spare column
+        {
+            ettk::DistanceArray D20(R, n, 20, 1);
+            metric.getDistances(refs, vols, D20, 2);
+            std::ofstream o(dir + "/out_D20.txt");
+            o.precision(9);
+            for (int r = 0; r < R; r++)
+                for (int i = 0; i < n; i++)
+                    for (int c = 0; c < 20; c++) o << D20(r, i, c, 0) << (c < 19 ? " " : "\n");
+        }
         // average of the volumes under their type-0 alignments to reference 0, unit weights
         refs[0].setVolumes(&vols);
         std::vector<float> al((size_t)n * 17, 0.0f);

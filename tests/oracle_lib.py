@@ -45,6 +45,8 @@ def oracle() -> C.CDLL:
         o.orc_ccc_count.argtypes = [C.c_void_p]
         o.orc_average.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(Particle), C.c_void_p, C.c_void_p,
                                   C.c_void_p]
+        o.orc_wedge_overlap.restype = C.c_float
+        o.orc_wedge_overlap.argtypes = [C.c_void_p, C.POINTER(Particle), C.c_void_p]
         o.orc_harmonics_t.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         o.orc_avg_wedge.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(Particle), C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -96,6 +98,10 @@ class OraclePlan:
         w = np.ascontiguousarray(weights, dtype=np.float32)
         m = np.ascontiguousarray(np.asarray(matrices, dtype=np.float64).reshape(n, 16))
         self.o.orc_set_reference_average(self.h, vp(self._ref), n, particle_array(descs), vp(w), vp(m))
+
+    def wedge_overlap(self, desc: Particle, t: np.ndarray) -> float:
+        tt = np.ascontiguousarray(np.asarray(t, dtype=np.float64).reshape(16))
+        return float(self.o.orc_wedge_overlap(self.h, C.byref(desc), vp(tt)))
 
     def ccc(self, vol: np.ndarray, desc: Particle, t: np.ndarray):
         vol = np.ascontiguousarray(vol, dtype=np.float32)

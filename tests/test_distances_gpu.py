@@ -50,6 +50,8 @@ def test_type2_shift_only_matches_oracle(torch_gpu, ctx):
             assert np.max(np.abs(r0[i].transform() - ro.transform())) <= 0.5 + 1e-9
             assert r0[i].distance == r1[i].distance and np.array_equal(r0[i].transform(), r1[i].transform())
             assert r0[i].n_ccc == 1
+            # getWedgeOverlap at the reported transform (the stored alignment is part of the particle's wedge)
+            assert abs(r0[i].overlap - orc.wedge_overlap(descs[i], r0[i].transform())) <= 1e-5
     finally:
         plan.close()
 
@@ -101,6 +103,7 @@ def test_type3_search_only_matches_oracle(torch_gpu, ctx):
         r = plan.distances(d, ps.descs, type=3, form=1)
         for i in range(len(r)):
             ro = orc.distance(ps.vols[i], ps.descs[i], 3)
+            assert abs(r[i].overlap - orc.wedge_overlap(ps.descs[i], r[i].transform())) <= 1e-5
             assert r[i].n_ccc == ro.n_ccc
             assert rotation_error_deg(r[i].transform(), ro.transform()) <= 0.5
             assert abs(r[i].distance - ro.distance) <= 2e-3 * abs(ro.distance) + 1e-7

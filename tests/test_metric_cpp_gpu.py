@@ -10,6 +10,7 @@ import pytest
 from conftest import to_dev
 from ettk_amd.abi import Particle, default_params
 from ettk_amd.synth import blob_reference, make_particles
+from oracle_lib import OraclePlan
 
 EXE = Path(__file__).resolve().parent / "cpp" / "test_metric"
 
@@ -51,6 +52,20 @@ def test_cpp_metric_drop_in(torch_gpu, ctx, tmp_path):
                     continue
                 assert D0[ri, i, 0] == np.float32(res[i].distance)
                 assert np.array_equal(D0[ri, i, 1:], res[i].transform().ravel().astype(np.float32))
+
+        # 20-column rows of type 2: scale 1, the wedge overlap at the reported transform, the matrix, a spare -1
+        D20 = np.loadtxt(tmp_path / "out_D20.txt").astype(np.float32).reshape(R, n, 20)
+        orc = OraclePlan(p)
+        for ri in range(R):
+            plan.set_reference(to_dev(torch_gpu, refs[ri]))
+            orc.set_reference(refs[ri])
+            res = plan.distances(d, descs, type=2, form=1)
+            for i in range(n):
+                assert D20[ri, i, 0] == np.float32(res[i].distance) and D20[ri, i, 1] == 1.0
+                assert D20[ri, i, 2] == np.float32(res[i].overlap)
+                assert np.array_equal(D20[ri, i, 3:19], res[i].transform().ravel().astype(np.float32))
+                assert D20[ri, i, 19] == -1.0
+                assert abs(res[i].overlap - orc.wedge_overlap(descs[i], res[i].transform())) <= 1e-5
 
         # getImage: unit weights on the type-0 alignments to reference 0, the resumed entry weighted 0
         w = np.ones(n, np.float32)

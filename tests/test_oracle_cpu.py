@@ -380,3 +380,32 @@ def test_harmonics_degenerate_transforms_keep_the_image():
     for t in (np.eye(4), _rz(0.9)):
         got = _harmonics_t(B, A, t)
         assert np.abs(got - A).max() <= 1e-5 * np.abs(A).max()
+
+
+# ------------------------------------------------------------------------------------- wedge overlap
+def test_wedge_overlap_closed_forms():
+    """getWedgeOverlap: 1 without effective wedges; a +-60 degree wedge keeps about 2/3 of a radial band
+    (the angular fraction 120/180); swapping the roles of two unrotated wedges leaves the product unchanged;
+    turning the particle's wedge by 90 degrees about z moves its missing region away from the reference's."""
+    N = 32
+    p = default_params(N, 8, bp_lo=0.1, bp_hi=0.45, bp_lo_decay=0.02, bp_hi_decay=0.02)
+    full, w60 = Particle(lwedge=-90, uwedge=90), Particle(lwedge=-60, uwedge=60)
+    I = np.eye(4)
+    o = OraclePlan(p)
+    o.set_reference(np.zeros((N, N, N), np.float32), full)
+    assert o.wedge_overlap(full, I) == 1.0
+    assert o.wedge_overlap(Particle(lwedge=-120, uwedge=130), I) == 1.0  # clamped to -90 / 90
+    one = o.wedge_overlap(w60, I)
+    assert abs(one - 2.0 / 3.0) < 0.03, one
+    w45 = Particle(lwedge=-45, uwedge=50)
+    a = o.wedge_overlap(w45, I)
+    o2 = OraclePlan(p)
+    o2.set_reference(np.zeros((N, N, N), np.float32), w60)
+    b = o2.wedge_overlap(w45, I)
+    o3 = OraclePlan(p)
+    o3.set_reference(np.zeros((N, N, N), np.float32), w45)
+    c = o3.wedge_overlap(w60, I)
+    assert abs(b - c) < 1e-6 and b < a
+    rz = np.eye(4)
+    rz[:2, :2] = [[0, -1], [1, 0]]
+    assert o2.wedge_overlap(w60, rz) < o2.wedge_overlap(w60, I) - 0.05
