@@ -9,6 +9,9 @@ EvalGeom make_geom_public(const ettk_plan* P, const ettk_particle& d, const Mat4
 void launch_avg_acc(const float2* F, const float* w, int E, long nh, double* re, double* im, double* acc, hipStream_t s);
 void launch_avg_final(const double* re, const double* im, const double* acc, int N, float2* F, hipStream_t s);
 void launch_avg_normalise(const float* A, int N, float* out, hipStream_t s);
+void launch_var_acc(const float* R, const float* mean, const float* w, int E, double wsum, const float* avg, int N,
+                    double* var, hipStream_t s);
+void launch_var_out(const double* var, int N, float* out, hipStream_t s);
 void launch_wedge_accum(const WedgeRef* w, int m, int N, float* acc, hipStream_t s);
 void launch_wedge_binarize(const float* acc, int N, float* bin, int* count, hipStream_t s);
 void launch_wedge_smooth_grid(const float* bin, int N, float* full, float* half, hipStream_t s);
@@ -125,6 +128,59 @@ int ettk_avg_accumulate(ettk_plan* P, const float* d_particles, int n, const ett
             launch_avg_acc(P->S2.p, dw.p, E, nh, acc_buf(P, 0).p, acc_buf(P, 1).p, acc_buf(P, 2).p, s);
             ETTK_HIP(hipStreamSynchronize(s));
         }
+    } catch (const Error& e) {
+        set_error(e.msg);
+        return e.code;
+    }
+    set_error("");
+    return ETTK_OK;
+}
+
+// nbfCorrelationImageMetric::getImageVariance -> nbfWedgedAverageImage3D::getImageVarianceMPI (CorrMetric.h:
+// 987-1003, WAvg.h:440-478): reslice each entry with |weight| > 0 at its alignment (setTransform(NULL),
+// getImage(image, store, false): valid voxels minus their mean, 0 outside), var += |w| (T - avg)^2 / sum|w|
+int ettk_avg_variance(ettk_plan* P, const float* d_particles, int n, const ettk_particle* descs, const float* weights,
+                      const double* matrices, const float* d_average, float* d_out) {
+    try {
+        ETTK_REQUIRE(P && d_particles && descs && weights && matrices && d_average && d_out && n > 0, ETTK_ERR_ARG,
+                     "ettk_avg_variance: bad argument");
+        ETTK_HIP(hipSetDevice(P->ctx->device));
+        const int N = P->N;
+        const size_t n3 = (size_t)N * N * N;
+        hipStream_t s = P->ctx->stream;
+        ensure_ccc_workspace(P, std::min(std::max(n, 1), 128));
+        double wsum = 0;
+        std::vector<int> sel;
+        for (int i = 0; i < n; i++) {
+            wsum += std::fabs((double)weights[i]);
+            if (std::fabs(weights[i]) > 0) sel.push_back(i);
+        }
+        ETTK_REQUIRE(!sel.empty(), ETTK_ERR_ARG, "ettk_avg_variance: no entry has a nonzero weight");
+        DevBuf<double> var;
+        var.ensure(n3);
+        ETTK_HIP(hipMemsetAsync(var.p, 0, n3 * sizeof(double), s));
+        std::vector<EvalGeom> g(P->cap);
+        std::vector<float> w(P->cap);
+        DevBuf<float> dw;
+        dw.ensure(P->cap);
+        for (size_t c0 = 0; c0 < sel.size(); c0 += P->cap) {
+            const int E = (int)std::min<size_t>(P->cap, sel.size() - c0);
+            for (int e = 0; e < E; e++) {
+                const int i = sel[c0 + e];
+                ettk_particle d = descs[i];
+                for (int k = 0; k < 16; k++) d.matrix[k] = (k % 5 == 0) ? 1.0 : 0.0;  // setTransform(NULL)
+                g[e] = make_geom_public(P, d, Mat4::from(matrices + (size_t)i * 16), i);
+                w[e] = weights[i];
+            }
+            ETTK_HIP(hipMemcpyAsync(P->geom.p, g.data(), E * sizeof(EvalGeom), hipMemcpyHostToDevice, s));
+            ETTK_HIP(hipMemcpyAsync(dw.p, w.data(), E * sizeof(float), hipMemcpyHostToDevice, s));
+            launch_reslice(d_particles, P->geom.p, E, N, P->R2.p, P->p_res.p, P->nb_res, s);
+            launch_reslice_mean(P->p_res.p, P->nb_res, E, P->stats.p, s);
+            launch_var_acc(P->R2.p, P->stats.p, dw.p, E, wsum, d_average, N, var.p, s);
+            ETTK_HIP(hipStreamSynchronize(s));
+        }
+        launch_var_out(var.p, N, d_out, s);
+        ETTK_HIP(hipStreamSynchronize(s));
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;

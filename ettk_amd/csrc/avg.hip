@@ -81,6 +81,48 @@ void launch_avg_normalise(const float* A, int N, float* out, hipStream_t s) {
     ETTK_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ variance map of an average
+// nbfWedgedAverageImage3D::getImageVarianceMPI (WAvg.h:440-478): var += |w| (T - average)^2 / sum|w| over the
+// entries in order, T the resliced particle minus its valid-voxel mean (0 outside the particle box).  R holds E
+// resliced volumes in A(i,j,k) order (k fastest), `avg` and the output are x fastest.
+__global__ void __launch_bounds__(256) k_var_acc(const float* __restrict__ R, const float* __restrict__ mean,
+                                                 const float* __restrict__ w, int E, double wsum, const float* __restrict__ avg,
+                                                 int N, double* __restrict__ var) {
+    const unsigned n3 = (unsigned)N * N * N;
+    for (unsigned o = blockIdx.x * blockDim.x + threadIdx.x; o < n3; o += gridDim.x * blockDim.x) {
+        int i, j, k;
+        split3(o, (unsigned)N, (unsigned)N, i, j, k);
+        const double a = avg[((unsigned)k * N + j) * N + i];
+        double v = var[o];
+        for (int e = 0; e < E; e++) {
+            const float x = R[(size_t)e * n3 + o];
+            const double T = (x == -FLT_MAX) ? 0.0 : (double)x - (double)mean[e];
+            const double d = T - a;
+            v += fabs((double)w[e]) * (d * d) / wsum;
+        }
+        var[o] = v;
+    }
+}
+__global__ void __launch_bounds__(256) k_var_out(const double* __restrict__ var, int N, float* __restrict__ out) {
+    const unsigned n3 = (unsigned)N * N * N;
+    for (unsigned o = blockIdx.x * blockDim.x + threadIdx.x; o < n3; o += gridDim.x * blockDim.x) {
+        int z, y, x;
+        split3(o, (unsigned)N, (unsigned)N, z, y, x);
+        out[o] = (float)var[((unsigned)x * N + y) * N + z];
+    }
+}
+void launch_var_acc(const float* R, const float* mean, const float* w, int E, double wsum, const float* avg, int N,
+                    double* var, hipStream_t s) {
+    const unsigned n3 = (unsigned)N * N * N;
+    k_var_acc<<<(n3 + 255) / 256, 256, 0, s>>>(R, mean, w, E, wsum, avg, N, var);
+    ETTK_HIP(hipGetLastError());
+}
+void launch_var_out(const double* var, int N, float* out, hipStream_t s) {
+    const unsigned n3 = (unsigned)N * N * N;
+    k_var_out<<<(n3 + 255) / 256, 256, 0, s>>>(var, N, out);
+    ETTK_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ accumulated wedge of an average
 // nbfWedgedAverageImage3D::updateAccumulatedWedgeImage (WAvg.h:496-538): accum += getWedgeImage(current, store)
 // over the (particle, candidate) entries in order, each the smoothed full-grid wedge of nbfWedge3D::getImage

@@ -42,6 +42,7 @@ def load() -> C.CDLL:
             "ettk_avg_reset": ([vp], C.c_int),
             "ettk_avg_accumulate": ([vp, vp, C.c_int, C.POINTER(Particle), fp, dp], C.c_int),
             "ettk_avg_finalize": ([vp, vp], C.c_int),
+            "ettk_avg_variance": ([vp, vp, C.c_int, C.POINTER(Particle), fp, dp, vp, vp], C.c_int),
             "ettk_avg_wedge": ([vp, C.c_int, C.POINTER(Particle), fp, dp, vp, vp, fp, ip], C.c_int),
             "ettk_reference_set_average": ([vp, vp, C.c_int, C.POINTER(Particle), fp, dp], C.c_int),
             "ettk_last_timing": ([vp, dp, C.POINTER(C.c_longlong), dp], C.c_int),
@@ -207,6 +208,14 @@ class Plan:
         check(self.lib.ettk_avg_accumulate(self.h, C.c_void_p(d_parts.data_ptr()), n, particle_array(descs), _fp(w),
                                            _dp(m)))
         check(self.lib.ettk_avg_finalize(self.h, C.c_void_p(d_out.data_ptr())))
+
+    def variance(self, d_parts, descs, weights, matrices, d_average, d_out):
+        """Variance map of an average around d_average (ettk_avg_variance)."""
+        n = d_parts.shape[0]
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        m = np.ascontiguousarray(np.asarray(matrices, dtype=np.float64).reshape(n, 16))
+        check(self.lib.ettk_avg_variance(self.h, C.c_void_p(d_parts.data_ptr()), n, particle_array(descs), _fp(w),
+                                         _dp(m), C.c_void_p(d_average.data_ptr()), C.c_void_p(d_out.data_ptr())))
 
     def avg_wedge(self, descs, weights, matrices, d_accum=None, d_wedge=None):
         """Accumulated wedge of an average (ettk_avg_wedge): fills the optional device arrays (N^3 float,

@@ -185,6 +185,14 @@ int ettk_avg_accumulate(ettk_plan* plan, const float* d_particles, int n, const 
                         const float* weights, const double* matrices);
 int ettk_avg_finalize(ettk_plan* plan, float* d_out);
 
+/* Variance map of an average: nbfCorrelationImageMetric::getImageVariance (CorrMetric.h:987-1003) through
+   nbfWedgedAverageImage3D::getImageVarianceMPI (WAvg.h:440-478).  Each entry with weights[i] != 0 is
+   resliced at matrices[i*16] (its own matrix reset, as for the average), T = valid voxels minus their mean, and
+   var += |w_i| (T - average)^2 / sum_i |w_i| in FP64 in entry order.  d_average is the average the variance is
+   taken around (N^3 floats, x fastest, e.g. the output of ettk_avg_finalize); d_out receives the map, x fastest. */
+int ettk_avg_variance(ettk_plan* plan, const float* d_particles, int n, const ettk_particle* descs,
+                      const float* weights, const double* matrices, const float* d_average, float* d_out);
+
 /* Accumulated missing wedge of an average (nbfWedgedAverageImage3D::updateAccumulatedWedgeImage and
    setAccumulatedWedgeImage, WAvg.h:496-538 and :49-70; spherical counterpart updateSphericalWedgeImage,
    :614-667).  Entry e of n (the reference's (particle, candidate) order) adds the wedge of descs[e] at the

@@ -239,6 +239,44 @@ class CorrelationImageMetric {
         check(ettk_avg_finalize(plan_, avg.d_average), "ettk_avg_finalize");
     }
 
+    /* getImageVariance(average) (CorrMetric.h:987-1003, WAvg.h:440-478): the weighted variance of the aligned
+       volumes around avg.d_average (computed by getImage first), into d_variance (N^3 floats, x fastest). */
+    void getImageVariance(WedgedAverageImage3D& avg, float* d_variance) {
+        if (!avg.volumes || avg.volumes->empty()) throw MetricError(ETTK_ERR_ARG, "getImageVariance: no volumes");
+        const int n = (int)avg.volumes->size();
+        if (avg.alignments.size() != (size_t)n * 17 * avg.K)
+            throw MetricError(ETTK_ERR_ARG, "getImageVariance: alignments");
+        ensure_plan_();
+        std::vector<int> all(n);
+        for (int j = 0; j < n; j++) all[j] = j;
+        std::vector<ettk_particle> d0;
+        const float* base = gather_(*avg.volumes, all, d0);
+        // entries in the reference's (volume, candidate) order; the volumes are repeated per candidate
+        std::vector<int> idx;
+        std::vector<ettk_particle> descs;
+        std::vector<float> w;
+        std::vector<double> m;
+        for (int j = 0; j < n; j++)
+            for (int k = 0; k < avg.K; k++) {
+                idx.push_back(j);
+                descs.push_back(d0[j]);
+                w.push_back(avg.alignment(j, 0, k));
+                for (int c = 0; c < 16; c++) m.push_back(avg.alignment(j, 1 + c, k));
+            }
+        const float* vols = base;
+        if (avg.K > 1) {
+            std::vector<WedgedSubImage3D> rep;
+            for (int j : idx) rep.push_back((*avg.volumes)[j]);
+            std::vector<int> all2(rep.size());
+            for (size_t q = 0; q < rep.size(); q++) all2[q] = (int)q;
+            std::vector<ettk_particle> dd;
+            vols = gather_(rep, all2, dd);
+        }
+        check(ettk_avg_variance(plan_, vols, (int)descs.size(), descs.data(), w.data(), m.data(), avg.d_average,
+                                d_variance),
+              "ettk_avg_variance");
+    }
+
     // ---------------------------------------------------------------- multi-reference assignment
     /* nbfLoopClustering::doMra best-reference selection (nbfLoopClustering.h:1523-1592): each volume is
        evaluated under each of its alignments to the references (A(j, i, 1..16, 0)) against every reference

@@ -650,6 +650,35 @@ void orc_average(void* h, int n, const float* parts, const ettk_particle* d, con
             for (int k = 0; k < N; k++) out[((size_t)k * N + j) * N + i] = A[((size_t)i * N + j) * N + k] / var_inside;
 }
 
+// Variance map (WAvg.h:440-478): var += |w| (T - avg)^2 / sum|w| over the entries with |w| > 0, T the resliced
+// particle (getImage(image, store, false): valid voxels minus their mean, 0 outside); avg and out are x fastest.
+void orc_variance(void* h, int n, const float* parts, const ettk_particle* d, const float* weights,
+                  const double* mats, const double* avg, double* out) {
+    Plan& P = *(Plan*)h;
+    const int N = P.N;
+    const size_t n3 = (size_t)N * N * N;
+    double wsum = 0;
+    for (int i = 0; i < n; i++) wsum += std::fabs((double)weights[i]);
+    std::vector<double> var(n3, 0.0), T;
+    for (int i = 0; i < n; i++) {
+        if (!(std::fabs(weights[i]) > 0)) continue;
+        ettk_particle di = d[i];
+        for (int k = 0; k < 16; k++) di.matrix[k] = (k % 5 == 0) ? 1.0 : 0.0;  // setTransform(NULL)
+        M4 store = m4_from(mats + (size_t)i * 16);
+        reslice_image(parts + (size_t)i * n3, N, di, &store, T);
+        for (int a = 0; a < N; a++)
+            for (int b = 0; b < N; b++)
+                for (int c = 0; c < N; c++) {
+                    const size_t o = ((size_t)a * N + b) * N + c;  // A(i,j,k), i = x
+                    const double dv = T[o] - avg[((size_t)c * N + b) * N + a];
+                    var[o] += std::fabs((double)weights[i]) * (dv * dv) / wsum;
+                }
+    }
+    for (int a = 0; a < N; a++)
+        for (int b = 0; b < N; b++)
+            for (int c = 0; c < N; c++) out[((size_t)c * N + b) * N + a] = var[((size_t)a * N + b) * N + c];
+}
+
 // ---------------------------------------------------------------------------- checks of the restated pieces
 // Powell (maxfev = 1, the library's settings) on f(x) = sum_i a_i (x_i - c_i)^2 + k (x_0 - c_0)(x_1 - c_1);
 // returns the number of function evaluations, writes the minimiser and its value.

@@ -45,6 +45,8 @@ def oracle() -> C.CDLL:
         o.orc_ccc_count.argtypes = [C.c_void_p]
         o.orc_average.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(Particle), C.c_void_p, C.c_void_p,
                                   C.c_void_p]
+        o.orc_variance.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(Particle), C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]
         o.orc_wedge_overlap.restype = C.c_float
         o.orc_wedge_overlap.argtypes = [C.c_void_p, C.POINTER(Particle), C.c_void_p]
         o.orc_harmonics_t.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -98,6 +100,17 @@ class OraclePlan:
         w = np.ascontiguousarray(weights, dtype=np.float32)
         m = np.ascontiguousarray(np.asarray(matrices, dtype=np.float64).reshape(n, 16))
         self.o.orc_set_reference_average(self.h, vp(self._ref), n, particle_array(descs), vp(w), vp(m))
+
+    def variance(self, vols: np.ndarray, descs, weights, matrices, avg: np.ndarray) -> np.ndarray:
+        from ettk_amd.abi import particle_array
+        n, N = vols.shape[0], vols.shape[1]
+        v = np.ascontiguousarray(vols, dtype=np.float32)
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        m = np.ascontiguousarray(np.asarray(matrices, dtype=np.float64).reshape(n, 16))
+        a = np.ascontiguousarray(avg, dtype=np.float64)
+        out = np.zeros((N, N, N))
+        self.o.orc_variance(self.h, n, vp(v), particle_array(descs), vp(w), vp(m), vp(a), vp(out))
+        return out
 
     def wedge_overlap(self, desc: Particle, t: np.ndarray) -> float:
         tt = np.ascontiguousarray(np.asarray(t, dtype=np.float64).reshape(16))

@@ -409,3 +409,23 @@ def test_wedge_overlap_closed_forms():
     rz = np.eye(4)
     rz[:2, :2] = [[0, -1], [1, 0]]
     assert o2.wedge_overlap(w60, rz) < o2.wedge_overlap(w60, I) - 0.05
+
+
+def test_variance_map_closed_forms():
+    """getImageVarianceMPI at the identity alignment, where the resliced particle is the input minus its mean:
+    around that image itself the variance vanishes; around zero it is the |w|-weighted mean of T^2 (negative
+    weights count by magnitude, zero weights not at all)."""
+    N = 16
+    orc = OraclePlan(default_params(N, 8))
+    g = np.random.default_rng(10)
+    v1, v2 = g.standard_normal((2, N, N, N)).astype(np.float32)
+    T1 = v1.astype(np.float64) - v1.mean(dtype=np.float64)
+    T2 = v2.astype(np.float64) - v2.mean(dtype=np.float64)
+    mats = np.tile(np.eye(4), (3, 1, 1))
+    descs = [Particle()] * 3
+    var0 = orc.variance(np.stack([v1, v1, v1]), descs, [0.2, 3.0, -1.5], mats, T1)
+    assert np.abs(var0).max() < 1e-8
+    var = orc.variance(np.stack([v1, v2, v2]), descs, [2.0, -0.5, 0.0], mats, np.zeros((N, N, N)))
+    want = (2.0 * T1 ** 2 + 0.5 * T2 ** 2) / 2.5
+    # the identity reslice reproduces the input to the 2^-17 floor tolerance of VTK's index arithmetic
+    assert np.abs(var - want).max() <= 2e-4 * want.max()

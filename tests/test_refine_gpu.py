@@ -200,3 +200,30 @@ def test_average_matches_oracle(torch_gpu, ctx):
         assert c_all > max(c_one), (c_all, c_one)
     finally:
         plan.close()
+
+
+@pytest.mark.gpu
+def test_variance_map_matches_oracle(torch_gpu, ctx):
+    """getImageVariance around the GPU average, against the FP64 restatement around the same average."""
+    from ettk_amd.lib import Plan
+
+    print("[variance] vs FP64 oracle", flush=True)
+    N = 32
+    p = default_params(N, 16)
+    ref = blob_reference(N, seed=63)
+    ps = make_particles(ref, 5, seed=64, half_wedge=(50, 65), snr=(0.5, 0.5))
+    w = np.array([1.0, 0.5, -0.25, 0.0, 2.0], np.float32)
+    plan = Plan(ctx, p)
+    orc = OraclePlan(p)
+    try:
+        d = to_dev(torch_gpu, ps.vols)
+        avg = torch_gpu.zeros((N, N, N), dtype=torch_gpu.float32, device="cuda")
+        plan.average(d, ps.descs, w, ps.truth, avg)
+        var = torch_gpu.zeros_like(avg)
+        plan.variance(d, ps.descs, w, ps.truth, avg, var)
+        got = var.cpu().numpy()
+        want = orc.variance(ps.vols, ps.descs, w, ps.truth, avg.cpu().numpy().astype(np.float64))
+        assert np.max(np.abs(got - want)) <= 2e-4 * np.max(want), np.max(np.abs(got - want))
+        assert got.min() >= 0.0
+    finally:
+        plan.close()
