@@ -274,34 +274,44 @@ static void search_gpu(ettk_plan* P, const float* d_parts, const ettk_particle* 
         ProfScope ps(kp, "k_localmax", 2.0 * cn * n3 * 8, s);
         launch_localmax(S.Cc.p, 2 * cn, B, kLmCap, L.lm->count.p, L.lm->idx.p, L.lm->val.p, s);
     }
+    launch_lm_compact(L.lm->count.p, L.lm->idx.p, L.lm->val.p, 2 * cn, kLmCap, L.lm->head.p, L.lm->pidx.p,
+                      L.lm->pval.p, s);
     ETTK_HIP(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
     ETTK_HIP(hipEventRecord(L.done, s));
 }
 
-// Host part: download the chunk's maxima (after its kernels, on the null stream: the plan's stream does not
-// block on it) and extract the candidate lists of its volumes on several host threads.
+// Host part: download the chunk's compacted maxima (after its kernels, on the context's copy stream: the plan's
+// stream does not wait for it) and extract the candidate lists of its volumes on several host threads.
 static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOut>& out) {
     KernelProf* kp = P->ctx->fft.prof;
     ETTK_HIP(hipEventSynchronize(L.done));
     auto th0 = std::chrono::steady_clock::now();
     const int p0 = L.p0, cn = L.cn;
     const bool uw1 = L.uw1;
-    std::vector<int> cnt(2 * cn);
-    ETTK_HIP(hipMemcpy(cnt.data(), L.lm->count.p, 2 * cn * sizeof(int), hipMemcpyDeviceToHost));
-    std::vector<std::vector<int>> idx(2 * cn);
-    std::vector<std::vector<double>> val(2 * cn);
-    for (int v = 0; v < 2 * cn; v++) {
+    const int nv = 2 * cn;
+    hipStream_t cs = P->ctx->copy_stream;
+    std::vector<int> head(2 * nv + 1);
+    ETTK_HIP(hipMemcpyAsync(head.data(), L.lm->head.p, head.size() * sizeof(int), hipMemcpyDeviceToHost, cs));
+    ETTK_HIP(hipStreamSynchronize(cs));
+    const int total = head[nv];
+    std::vector<int> pidx(total);
+    std::vector<double> pval(total);
+    if (total > 0) {
+        ETTK_HIP(hipMemcpyAsync(pidx.data(), L.lm->pidx.p, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, cs));
+        ETTK_HIP(hipMemcpyAsync(pval.data(), L.lm->pval.p, (size_t)total * sizeof(double), hipMemcpyDeviceToHost, cs));
+        ETTK_HIP(hipStreamSynchronize(cs));
+    }
+    std::vector<std::vector<int>> idx(nv);
+    std::vector<std::vector<double>> val(nv);
+    for (int v = 0; v < nv; v++) {
         int q = v / 2;
         bool two = uw1 || L.uw2[q];
         if (v % 2 == 1 && !two) continue;
-        ETTK_REQUIRE(cnt[v] <= kLmCap, ETTK_ERR_UNSUPPORTED, "more SO(3) local maxima than the candidate list holds");
-        idx[v].resize(cnt[v]);
-        val[v].resize(cnt[v]);
-        if (cnt[v] > 0) {
-            ETTK_HIP(hipMemcpy(idx[v].data(), L.lm->idx.p + (size_t)v * kLmCap, cnt[v] * sizeof(int), hipMemcpyDeviceToHost));
-            ETTK_HIP(hipMemcpy(val[v].data(), L.lm->val.p + (size_t)v * kLmCap, cnt[v] * sizeof(double), hipMemcpyDeviceToHost));
-        }
-        if (kp && kp->on) kp->add_host("lm_count", (double)cnt[v]);
+        const int c = head[nv + 1 + v];
+        ETTK_REQUIRE(c <= kLmCap, ETTK_ERR_UNSUPPORTED, "more SO(3) local maxima than the candidate list holds");
+        idx[v].assign(pidx.begin() + head[v], pidx.begin() + head[v] + c);
+        val[v].assign(pval.begin() + head[v], pval.begin() + head[v] + c);
+        if (kp && kp->on) kp->add_host("lm_count", (double)c);
     }
     for (int q = 0; q < cn; q++) {
         SearchOut& o = out[p0 + q];

@@ -61,7 +61,8 @@ __global__ void __launch_bounds__(kThreads) k_reslice(const float* __restrict__ 
     const float* vol = parts + (size_t)g.particle * N * N * N;
     const long n3 = (long)N * N * N;
     const int nt = N / kTile;
-    const int ti = blockIdx.x / (nt * nt), tj = (blockIdx.x / nt) % nt, tk = blockIdx.x % nt;
+    const int tb = xcd_tile(blockIdx.x, gridDim.x);
+    const int ti = tb / (nt * nt), tj = (tb / nt) % nt, tk = tb % nt;
     const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * kTile;
     const float tol = 7.62939453125e-06f;
     // box origin per input axis (x, y, z); it may reach outside the volume
@@ -153,20 +154,27 @@ __global__ void __launch_bounds__(kThreads) k_reslice(const float* __restrict__ 
     float ts = block_sum(s, sh);
     float tc = block_sum(c, sh);
     if (threadIdx.x == 0) {
-        partials[((size_t)e * gridDim.x + blockIdx.x) * 2 + 0] = ts;
-        partials[((size_t)e * gridDim.x + blockIdx.x) * 2 + 1] = tc;
+        partials[((size_t)e * gridDim.x + tb) * 2 + 0] = ts;  // in tile order: the same sum order for any mapping
+        partials[((size_t)e * gridDim.x + tb) * 2 + 1] = tc;
     }
 }
 
-__global__ void k_reslice_mean(const float* __restrict__ partials, int nb, int n_eval, float* __restrict__ stats) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n_eval) return;
+// one wave per evaluation: lane l sums the partials b = l, l + 64, ... in FP64, then a fixed butterfly
+__device__ __forceinline__ double wave_dsum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__global__ void __launch_bounds__(64) k_reslice_mean(const float* __restrict__ partials, int nb, int n_eval,
+                                                     float* __restrict__ stats) {
+    const int e = blockIdx.x, l = threadIdx.x;
     double s = 0, c = 0;
-    for (int b = 0; b < nb; b++) {
+    for (int b = l; b < nb; b += 64) {
         s += partials[((size_t)e * nb + b) * 2];
         c += partials[((size_t)e * nb + b) * 2 + 1];
     }
-    stats[e] = (float)(s / (double)(float)c);  // meanInside is a float (WSub.h:794-797)
+    s = wave_dsum(s);
+    c = wave_dsum(c);
+    if (l == 0) stats[e] = (float)(s / (double)(float)c);  // meanInside is a float (WSub.h:794-797)
 }
 
 int reslice_blocks(int N) { return (N / kTile) * (N / kTile) * (N / kTile); }
@@ -179,7 +187,7 @@ void launch_reslice(const float* parts, const EvalGeom* geom, int n_eval, int N,
     ETTK_HIP(hipGetLastError());
 }
 void launch_reslice_mean(const float* partials, int nb, int n_eval, float* stats, hipStream_t s) {
-    k_reslice_mean<<<(n_eval + 127) / 128, 128, 0, s>>>(partials, nb, n_eval, stats);
+    k_reslice_mean<<<n_eval, 64, 0, s>>>(partials, nb, n_eval, stats);
     ETTK_HIP(hipGetLastError());
 }
 
@@ -312,17 +320,21 @@ void launch_mask_ref(float* R, const float* m1, int n_eval, int N, float* partia
 }
 
 // ------------------------------------------------------------------ per-evaluation scalars
-__global__ void k_ccc_scalars(const float* p_n1, const float* p_n2, const float* p_k1, const float* p_wf, int nb,
-                              int nbr, int n_eval, int N, float* scalars) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n_eval) return;
+__global__ void __launch_bounds__(64) k_ccc_scalars(const float* p_n1, const float* p_n2, const float* p_k1,
+                                                    const float* p_wf, int nb, int nbr, int n_eval, int N, float* scalars) {
+    const int e = blockIdx.x, l = threadIdx.x;
     double s1 = 0, s2 = 0, k1 = 0, wf = 0;
-    for (int b = 0; b < nb; b++) {
+    for (int b = l; b < nb; b += 64) {
         s1 += p_n1[(size_t)e * nb + b];
         s2 += p_n2[(size_t)e * nb + b];
-        wf += p_wf[(size_t)e * nb + b];
+        if (p_wf) wf += p_wf[(size_t)e * nb + b];
     }
-    for (int b = 0; b < nbr; b++) k1 += p_k1[(size_t)e * nbr + b];
+    for (int b = l; b < nbr; b += 64) k1 += p_k1[(size_t)e * nbr + b];
+    s1 = wave_dsum(s1);
+    s2 = wave_dsum(s2);
+    k1 = wave_dsum(k1);
+    wf = wave_dsum(wf);
+    if (l != 0) return;
     double n3 = (double)N * N * N;
     float n1 = (float)sqrt(s1 / n3), n2 = (float)sqrt(s2 / n3);
     scalars[e * 4 + 0] = n1;
@@ -332,7 +344,7 @@ __global__ void k_ccc_scalars(const float* p_n1, const float* p_n2, const float*
 }
 void launch_ccc_scalars(const float* p_n1, const float* p_n2, const float* p_k1, const float* p_wf, int nb,
                         int nbr, int n_eval, int N, float* scalars, hipStream_t s) {
-    k_ccc_scalars<<<(n_eval + 127) / 128, 128, 0, s>>>(p_n1, p_n2, p_k1, p_wf, nb, nbr, n_eval, N, scalars);
+    k_ccc_scalars<<<n_eval, 64, 0, s>>>(p_n1, p_n2, p_k1, p_wf, nb, nbr, n_eval, N, scalars);
     ETTK_HIP(hipGetLastError());
 }
 

@@ -470,6 +470,7 @@ int ettk_ctx_create(int device, ettk_ctx** out) {
     auto* c = new ettk_ctx();
     c->device = device;
     ETTK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    ETTK_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
     fft_tables_init(c->fft, c->stream);
     *out = c;
     ETTK_API_END
@@ -486,6 +487,7 @@ int ettk_ctx_destroy(ettk_ctx* ctx) {
             delete ctx->fft.prof;
         }
         (void)hipStreamDestroy(ctx->stream);
+        if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
         delete ctx;
     }
     ETTK_API_END

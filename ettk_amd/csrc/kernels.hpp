@@ -13,6 +13,14 @@ __device__ __forceinline__ void split3(unsigned h, unsigned n1, unsigned n2, int
     a = (int)a_;
 }
 
+// Tile of workgroup `b` out of `nb` per evaluation, grouped by XCD: the dispatcher deals consecutive workgroups
+// of a grid round-robin to the 8 XCDs (each with its own L2), so workgroup b runs on XCD b % 8.  Giving XCD k
+// the contiguous tiles [k nb/8, (k+1) nb/8) keeps the tiles that share cache lines (neighbours along the
+// fastest axis) in one L2.  nb must be a multiple of 8 (otherwise the identity).
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+}
+
 // per-evaluation geometry computed on the host in FP64
 struct EvalGeom {
     float m[12];      // reslice map: input index = m[0..2]*i + m[3..5]*j + m[6..8]*k + m[9..11]  (rows x,y,z)

@@ -49,6 +49,7 @@ struct Timing {
 struct ettk_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;  // host downloads issued from worker threads (non-blocking)
     ettk::FftTables fft;
 };
 

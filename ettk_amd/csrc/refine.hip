@@ -69,6 +69,75 @@ __device__ __forceinline__ void ref_pos(const RefineGeom& g, int N, int x, int y
     q[2] = g.G[8] * wx + g.G[9] * wy + g.G[10] * wz + g.G[11];
 }
 
+// ---------------------------------------------------------------- LDS-staged trilinear sampling
+// One workgroup handles an 8x8x8 tile of the particle frame.  The reference voxels its trilinear taps can
+// touch lie in the bounding box of the tile's mapped corners (at most 7 sqrt(3) + 2 voxels per axis for a
+// rigid map); that box plus one voxel of margin on each side, with the reference's border replicated where
+// it reaches outside (the taps clamp to the border), is staged in LDS at fixed strides, so the divergent
+// gathers become LDS reads and every sample equals the clamped-tap global one.  A map whose box does not fit
+// samples from global memory instead.
+static constexpr int kRT = 8, kRB = 18;
+
+__device__ __forceinline__ bool stage_ref_box(const float* __restrict__ vol, int N, const RefineGeom& g, int x0,
+                                              int y0, int z0, float* box, int lo[3]) {
+    const float h = 0.5f * (float)N;
+    bool fits = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float* r = g.G + a * 4;
+        const float c0 = r[0] * ((float)x0 - h) + r[1] * ((float)y0 - h) + r[2] * ((float)z0 - h) + r[3];
+        const float mn = c0 + (kRT - 1) * (fminf(r[0], 0.f) + fminf(r[1], 0.f) + fminf(r[2], 0.f));
+        const float ext = (kRT - 1) * (fabsf(r[0]) + fabsf(r[1]) + fabsf(r[2]));
+        lo[a] = (int)floorf(mn) - 1;
+        const int hi = (int)floorf(mn + ext) + 2;
+        if (!(hi - lo[a] + 1 <= kRB) || !(fabsf(mn) < 1e6f)) fits = false;
+    }
+    if (fits) {
+        const int t = threadIdx.x;
+        if (t < kRB * 14) {
+            const int x = t % kRB, yb = t / kRB;
+            const int gx = min(max(lo[0] + x, 0), N - 1);
+#pragma unroll
+            for (int yy = 0; yy < 2; yy++) {
+                const int y = yb + 14 * yy;
+                if (y >= kRB) break;
+                const int gy = min(max(lo[1] + y, 0), N - 1);
+                const float* src = vol + (size_t)gy * N + gx;
+                for (int z = 0; z < kRB; z++) {
+                    const int gz = min(max(lo[2] + z, 0), N - 1);
+                    box[(z * kRB + y) * kRB + x] = src[(size_t)gz * N * N];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    return fits;
+}
+
+// trilin() on the staged box: the same arithmetic, taps read from LDS
+__device__ __forceinline__ float trilin_box(const float* box, const int lo[3], float px, float py, float pz, float* gr) {
+    float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
+    float ax = px - fx, ay = py - fy, az = pz - fz;
+    const int bx = (int)fx - lo[0], by = (int)fy - lo[1], bz = (int)fz - lo[2];
+    const float* p = box + (bz * kRB + by) * kRB + bx;
+    float v000 = p[0], v100 = p[1];
+    float v010 = p[kRB], v110 = p[kRB + 1];
+    float v001 = p[kRB * kRB], v101 = p[kRB * kRB + 1];
+    float v011 = p[kRB * kRB + kRB], v111 = p[kRB * kRB + kRB + 1];
+    float c00 = v000 + ax * (v100 - v000), c10 = v010 + ax * (v110 - v010);
+    float c01 = v001 + ax * (v101 - v001), c11 = v011 + ax * (v111 - v011);
+    float c0 = c00 + ay * (c10 - c00), c1 = c01 + ay * (c11 - c01);
+    if (gr) {
+        float dx00 = v100 - v000, dx10 = v110 - v010, dx01 = v101 - v001, dx11 = v111 - v011;
+        float dx0 = dx00 + ay * (dx10 - dx00), dx1 = dx01 + ay * (dx11 - dx01);
+        gr[0] = dx0 + az * (dx1 - dx0);
+        float dy0 = c10 - c00, dy1 = c11 - c01;
+        gr[1] = dy0 + az * (dy1 - dy0);
+        gr[2] = c1 - c0;
+    }
+    return c0 + az * (c1 - c0);
+}
+
 // H at half-spectrum element (a, b, c) = (z, y, x) of the particle frame: filter x binary wedges
 __device__ __forceinline__ float hweight(const float* __restrict__ filter, const float* __restrict__ wbin,
                                          const RefineGeom& g, long h, int a, int b, int c, int N, int H) {
@@ -113,15 +182,21 @@ __global__ void __launch_bounds__(kT) k_vol_mean(const float* __restrict__ vols,
 // I1'(p) for every evaluation: out[e][z][y][x]; `ref` holds I1 - mean I1, x fastest
 __global__ void __launch_bounds__(kT) k_reslice_ref(const float* __restrict__ ref, const RefineGeom* __restrict__ geom,
                                                      int N, float* __restrict__ out) {
+    __shared__ float box[kRB * kRB * kRB];
     const int e = blockIdx.y;
     const RefineGeom g = geom[e];
-    const long n3 = (long)N * N * N;
-    for (long o = (long)blockIdx.x * kT + threadIdx.x; o < n3; o += (long)gridDim.x * kT) {
-        int z, y, x;
-        split3((unsigned)o, (unsigned)N, (unsigned)N, z, y, x);
+    const size_t n3 = (size_t)N * N * N;
+    const int nt = N / kRT;
+    const int tb = xcd_tile(blockIdx.x, gridDim.x);
+    const int z0 = (tb / (nt * nt)) * kRT, y0 = ((tb / nt) % nt) * kRT, x0 = (tb % nt) * kRT;
+    int lo[3];
+    const bool fits = stage_ref_box(ref, N, g, x0, y0, z0, box, lo);
+    for (int t = threadIdx.x; t < kRT * kRT * kRT; t += kT) {
+        const int x = x0 + (t & 7), y = y0 + ((t >> 3) & 7), z = z0 + (t >> 6);
         float q[3];
         ref_pos(g, N, x, y, z, q);
-        out[(size_t)e * n3 + o] = trilin(ref, N, q[0], q[1], q[2], nullptr);
+        const float v = fits ? trilin_box(box, lo, q[0], q[1], q[2], nullptr) : trilin(ref, N, q[0], q[1], q[2], nullptr);
+        out[(size_t)e * n3 + ((size_t)z * N + y) * N + x] = v;
     }
 }
 
@@ -165,15 +240,21 @@ __global__ void __launch_bounds__(kT) k_refine_dot(const float2* __restrict__ X1
 }
 
 // sc[e] = {S, n1, n2} in FP64
-__global__ void k_refine_scalars(const double* __restrict__ part, int nb, int n_eval, double* __restrict__ sc) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n_eval) return;
+__global__ void __launch_bounds__(64) k_refine_scalars(const double* __restrict__ part, int nb, int n_eval,
+                                                       double* __restrict__ sc) {
+    const int e = blockIdx.x, l = threadIdx.x;
     double d = 0, q1 = 0, q2 = 0;
-    for (int b = 0; b < nb; b++) {
+    for (int b = l; b < nb; b += 64) {
         d += part[((size_t)e * nb + b) * 3];
         q1 += part[((size_t)e * nb + b) * 3 + 1];
         q2 += part[((size_t)e * nb + b) * 3 + 2];
     }
+    for (int o = 32; o > 0; o >>= 1) {
+        d += __shfl_xor(d, o, 64);
+        q1 += __shfl_xor(q1, o, 64);
+        q2 += __shfl_xor(q2, o, 64);
+    }
+    if (l != 0) return;
     const double n1 = sqrt(q1), n2 = sqrt(q2);
     sc[e * 4 + 0] = (n1 > 0 && n2 > 0) ? d / (n1 * n2) : 0.0;
     sc[e * 4 + 1] = n1;
@@ -203,29 +284,34 @@ __global__ void __launch_bounds__(kT) k_refine_adj(float2* __restrict__ X1, cons
     }
 }
 
-// twelve moments per evaluation: per-block partial sums of gamma(p) dI1/dq_a w_b(p) over `vpb` voxels
+// twelve moments per evaluation: per-tile partial sums of gamma(p) dI1/dq_a w_b(p) over an 8x8x8 tile
 __global__ void __launch_bounds__(kT) k_refine_moments(const float* __restrict__ ref, const RefineGeom* __restrict__ geom,
                                                         const float* __restrict__ G, const float* __restrict__ maskfft,
-                                                        int N, int vpb, float* __restrict__ part) {
+                                                        int N, float* __restrict__ part) {
+    __shared__ float box[kRB * kRB * kRB];
     __shared__ float sh[12][kT / 64];
     const int e = blockIdx.y;
     const RefineGeom g = geom[e];
-    const long n3 = (long)N * N * N;
+    const size_t n3 = (size_t)N * N * N;
     const float* Ge = G + (size_t)e * n3;
     const float half = 0.5f * (float)N;
+    const int nt = N / kRT;
+    const int tb = xcd_tile(blockIdx.x, gridDim.x);
+    const int z0 = (tb / (nt * nt)) * kRT, y0 = ((tb / nt) % nt) * kRT, x0 = (tb % nt) * kRT;
+    int lo[3];
+    const bool fits = stage_ref_box(ref, N, g, x0, y0, z0, box, lo);
     float q12[12];
 #pragma unroll
     for (int r = 0; r < 12; r++) q12[r] = 0.f;
-    const long o0 = (long)blockIdx.x * vpb;
-    for (int v = threadIdx.x; v < vpb; v += kT) {
-        const long o = o0 + v;
-        int z, y, x;
-        split3((unsigned)o, (unsigned)N, (unsigned)N, z, y, x);
+    for (int t = threadIdx.x; t < kRT * kRT * kRT; t += kT) {
+        const int x = x0 + (t & 7), y = y0 + ((t >> 3) & 7), z = z0 + (t >> 6);
+        const size_t o = ((size_t)z * N + y) * N + x;
         float q[3];
         ref_pos(g, N, x, y, z, q);
         const float gam = maskfft[o] * (((x + y + z) & 1) ? -Ge[o] : Ge[o]);
         float gr[3];
-        trilin(ref, N, q[0], q[1], q[2], gr);
+        if (fits) trilin_box(box, lo, q[0], q[1], q[2], gr);
+        else trilin(ref, N, q[0], q[1], q[2], gr);
         const float w[4] = {(float)x - half, (float)y - half, (float)z - half, 1.f};
 #pragma unroll
         for (int a = 0; a < 3; a++) {
@@ -244,20 +330,31 @@ __global__ void __launch_bounds__(kT) k_refine_moments(const float* __restrict__
     if (threadIdx.x < 12) {
         float t = 0.f;
         for (int i = 0; i < kT / 64; i++) t += sh[threadIdx.x][i];
-        part[((size_t)e * gridDim.x + blockIdx.x) * 12 + threadIdx.x] = t;
+        part[((size_t)e * gridDim.x + tb) * 12 + threadIdx.x] = t;
     }
 }
-__global__ void k_refine_reduce_moments(const float* __restrict__ part, int nb, int n_eval, double* __restrict__ out) {
-    const int e = blockIdx.x, r = threadIdx.x;
-    if (e >= n_eval || r >= 12) return;
-    double acc = 0;
-    for (int b = 0; b < nb; b++) acc += part[((size_t)e * nb + b) * 12 + r];
-    out[e * 12 + r] = acc;
+__global__ void __launch_bounds__(64) k_refine_reduce_moments(const float* __restrict__ part, int nb, int n_eval,
+                                                              double* __restrict__ out) {
+    const int e = blockIdx.x, l = threadIdx.x;
+    double acc[12];
+#pragma unroll
+    for (int r = 0; r < 12; r++) acc[r] = 0.0;
+    for (int b = l; b < nb; b += 64) {
+        const float* p = part + ((size_t)e * nb + b) * 12;
+#pragma unroll
+        for (int r = 0; r < 12; r++) acc[r] += p[r];
+    }
+#pragma unroll
+    for (int r = 0; r < 12; r++) {
+        double v = acc[r];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (l == 0) out[e * 12 + r] = v;
+    }
 }
 
 int refine_moment_blocks(int N) {
-    const long n3 = (long)N * N * N;
-    return (int)(n3 >= 4096 ? n3 / 4096 : 1);
+    const int nt = N / kRT;
+    return nt * nt * nt;
 }
 int refine_dot_blocks(int N) {
     const long nh = (long)N * N * (N / 2 + 1);
@@ -269,9 +366,9 @@ void launch_vol_mean(const float* vols, int count, int N, float* mean, hipStream
     ETTK_HIP(hipGetLastError());
 }
 void launch_reslice_ref(const float* ref, const RefineGeom* geom, int E, int N, float* out, hipStream_t s) {
-    const long n3 = (long)N * N * N;
-    const int nb = (int)std::min<long>(1024, (n3 + 4 * kT - 1) / (4 * kT));
-    k_reslice_ref<<<dim3(nb, E), kT, 0, s>>>(ref, geom, N, out);
+    ETTK_REQUIRE(N % kRT == 0, ETTK_ERR_ARG, "launch_reslice_ref: box size must be a multiple of 8");
+    const int nt = N / kRT;
+    k_reslice_ref<<<dim3(nt * nt * nt, E), kT, 0, s>>>(ref, geom, N, out);
     ETTK_HIP(hipGetLastError());
 }
 void launch_refine_dot(const float2* X1, const float2* FP, const float* filter, const float* wbin, const RefineGeom* geom,
@@ -279,7 +376,7 @@ void launch_refine_dot(const float2* X1, const float2* FP, const float* filter, 
                        double* part, double* sc, hipStream_t s) {
     const int nb = refine_dot_blocks(N);
     k_refine_dot<<<dim3(nb, E), kT, 0, s>>>(X1, FP, filter, wbin, geom, N, part);
-    k_refine_scalars<<<(E + 127) / 128, 128, 0, s>>>(part, nb, E, sc);
+    k_refine_scalars<<<E, 64, 0, s>>>(part, nb, E, sc);
     ETTK_HIP(hipGetLastError());
 }
 void launch_refine_adj(float2* X1, const float2* FP, const float* filter, const float* wbin, const RefineGeom* geom,
@@ -292,9 +389,9 @@ void launch_refine_adj(float2* X1, const float2* FP, const float* filter, const 
 }
 void launch_refine_moments(const float* ref, const RefineGeom* geom, const float* G, const float* maskfft, int E, int N,
                            float* mpart, double* mom, hipStream_t s) {
-    const long n3 = (long)N * N * N;
+    ETTK_REQUIRE(N % kRT == 0, ETTK_ERR_ARG, "launch_refine_moments: box size must be a multiple of 8");
     const int nb = refine_moment_blocks(N);
-    k_refine_moments<<<dim3(nb, E), kT, 0, s>>>(ref, geom, G, maskfft, N, (int)(n3 / nb), mpart);
+    k_refine_moments<<<dim3(nb, E), kT, 0, s>>>(ref, geom, G, maskfft, N, mpart);
     k_refine_reduce_moments<<<E, 64, 0, s>>>(mpart, nb, E, mom);
     ETTK_HIP(hipGetLastError());
 }

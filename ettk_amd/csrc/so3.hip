@@ -976,6 +976,28 @@ void launch_localmax(const double* C, int np, int B, int cap, int* count, int* i
     k_localmax<<<dim3(256, np), kT, 0, s>>>(C, n, cap, count, idx, val);
     ETTK_HIP(hipGetLastError());
 }
+// block v: offset of list v = sum of the clamped counts before it; copies its entries in slot order
+__global__ void k_lm_compact(const int* __restrict__ count, const int* __restrict__ idx, const double* __restrict__ val,
+                             int np, int cap, int* __restrict__ head, int* __restrict__ pidx, double* __restrict__ pval) {
+    const int v = blockIdx.x;
+    int off = 0;
+    for (int u = 0; u < v; u++) off += min(count[u], cap);
+    const int c = min(count[v], cap);
+    if (threadIdx.x == 0) {
+        head[v] = off;
+        head[np + 1 + v] = count[v];
+        if (v == np - 1) head[np] = off + c;
+    }
+    for (int k = threadIdx.x; k < c; k += blockDim.x) {
+        pidx[off + k] = idx[(size_t)v * cap + k];
+        pval[off + k] = val[(size_t)v * cap + k];
+    }
+}
+void launch_lm_compact(const int* count, const int* idx, const double* val, int np, int cap, int* head, int* pidx,
+                       double* pval, hipStream_t s) {
+    k_lm_compact<<<np, kT, 0, s>>>(count, idx, val, np, cap, head, pidx, pval);
+    ETTK_HIP(hipGetLastError());
+}
 void launch_ppmax(const float* Pp1, const float* Pp2, int np, int B, float* Mx, hipStream_t s) {
     k_ppmax<<<np, kT, 0, s>>>(Pp1, Pp2, 4 * B * B, Mx);
     ETTK_HIP(hipGetLastError());

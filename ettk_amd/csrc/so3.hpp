@@ -25,6 +25,9 @@ struct ComboOp {
 struct LmBuffers {
     DevBuf<int> count, idx;
     DevBuf<double> val;
+    // compacted lists: head = [offsets (nvol + 1), counts (nvol)], then the packed indices and values
+    DevBuf<int> head, pidx;
+    DevBuf<double> pval;
 };
 
 struct So3State {
@@ -58,6 +61,9 @@ inline LmBuffers& lm_slot(So3State& S, int slot, int nvol, int cap) {
     b.count.ensure(nvol);
     b.idx.ensure((size_t)nvol * cap);
     b.val.ensure((size_t)nvol * cap);
+    b.head.ensure((size_t)2 * nvol + 1);
+    b.pidx.ensure((size_t)nvol * cap);
+    b.pval.ensure((size_t)nvol * cap);
     return b;
 }
 
@@ -76,6 +82,9 @@ void sph_rotate(ettk_plan* P, int n, const float* h_img, const double* h_t, floa
 So3State& so3_state(ettk_plan* P);
 void launch_combine(const double* C, const ComboOp* d_ops, int np, int B, int b0, double* out, hipStream_t s);
 void launch_localmax(const double* C, int np, int B, int cap, int* count, int* idx, double* val, hipStream_t s);
+// packs the np lists (at most `cap` entries each) back to back: head = [offsets (np + 1), counts (np)]
+void launch_lm_compact(const int* count, const int* idx, const double* val, int np, int cap, int* head, int* pidx,
+                       double* pval, hipStream_t s);
 void launch_ppmax(const float* Pp1, const float* Pp2, int np, int B, float* Mx, hipStream_t s);
 
 }  // namespace ettk
