@@ -201,5 +201,11 @@ struct R2CArgs {
 void fft3_r2c(const FftTables& T, const R2CArgs& a, float2* scratch, hipStream_t s);
 // c2r: in is preserved when `in_scratch` != in (the i/j passes run out of place into in_scratch)
 void fft3_c2r(const FftTables& T, const float2* in, float2* in_scratch, float* out, int n, int batch, hipStream_t s);
+// r2c(op(c2r(in))) without the real volume in HBM (n <= 64): the inverse i pass reads `in` and writes `io`
+// (in == io allowed), the fused plane kernel and the forward i pass work in place on `io`.
+enum : int { PLANE_OP_MASK = 0, PLANE_OP_SQUARE_SHIFT = 1 };
+bool fft3_fused_supported(int n);
+void fft3_c2r_op_r2c(const FftTables& T, const float2* in, float2* io, int n, int batch, int op, const float* aux,
+                     float* partials, hipStream_t s);
 
 }  // namespace ettk

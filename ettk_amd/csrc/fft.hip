@@ -478,6 +478,130 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_plane(const float2* _
     }
 }
 
+// ------------------------------------------------------------------ fused c2r -> pointwise op -> r2c (L <= 64)
+// Between the inverse and the forward transform of one volume, a (b, i) plane is complete in LDS after the c2r
+// plane passes: the pointwise operation and the r2c plane passes run there, so the real volume never goes
+// through HBM.  `io` holds the half spectrum after the inverse i pass and receives the forward j/k passes.
+//   PLANE_OP_MASK:          r <- r m (m = aux, N^3), partials[plane] = sum r^2 m over the plane
+//   PLANE_OP_SQUARE_SHIFT:  r <- r^2 (-1)^(i+j+k)
+template <int L>
+__global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_r2c_plane(float2* __restrict__ io, int op,
+                                                                       const float* __restrict__ aux,
+                                                                       float* __restrict__ partials,
+                                                                       const float2* __restrict__ twg) {
+    using PL = Plane<L>;
+    constexpr int H = PL::H, LD = PL::LD, T = PL::THREADS;
+    constexpr int NZ = ((L / 2) * L + T - 1) / T;
+    constexpr int NS = ((L / 2) * 2 * H + T - 1) / T;
+    constexpr int NFW = T / 64, REM = T % 64;  // full waves, threads of the partial wave
+    __shared__ float2 buf[H * LD];
+    __shared__ float2 tw[L];
+    __shared__ float sred[NFW + REM + 1];
+    const long plane = blockIdx.x;
+    const int pi = (int)(plane % L);
+    for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
+    float2* ip = io + plane * L * H;
+    for (int t = threadIdx.x; t < L * H; t += blockDim.x) {
+        const int j = t / H, k = t % H;
+        buf[k * LD + j] = ip[(long)j * H + k];
+    }
+    __syncthreads();
+    const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
+    pencil_fft<L>(&buf[q * LD], lane, tw, true);
+    // c2r along k: Z(k) = A(k) + i B(k) of rows 2p, 2p+1 (as k_c2r_plane)
+    float2 zr[NZ];
+#pragma unroll
+    for (int u = 0; u < NZ; u++) {
+        const int t = threadIdx.x + u * T;
+        if (t < (L / 2) * L) {
+            const int pp = t / L, k = t % L;
+            float2 A, Bv;
+            if (k <= L / 2) {
+                A = buf[k * LD + 2 * pp];
+                Bv = buf[k * LD + 2 * pp + 1];
+                if (k == 0 || k == L / 2) {
+                    A.y = 0.f;
+                    Bv.y = 0.f;
+                }
+            } else {
+                A = cconj(buf[(L - k) * LD + 2 * pp]);
+                Bv = cconj(buf[(L - k) * LD + 2 * pp + 1]);
+            }
+            zr[u] = make_float2(A.x - Bv.y, A.y + Bv.x);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NZ; u++) {
+        const int t = threadIdx.x + u * T;
+        if (t < (L / 2) * L) buf[(t / L) * LD + (t % L)] = zr[u];
+    }
+    __syncthreads();
+    pencil_fft<L>(&buf[q * LD], lane, tw, true);
+    __syncthreads();
+    // buf[p LD + x] = (r(row 2p, x), r(row 2p + 1, x)): the pointwise operation in place
+    float sacc = 0.f;
+    for (int t = threadIdx.x; t < (L / 2) * L; t += T) {
+        const int pp = t / L, x = t % L, j0 = 2 * pp;
+        float2 z = buf[pp * LD + x];
+        if (op == PLANE_OP_MASK) {
+            const float m0 = aux[((size_t)pi * L + j0) * L + x], m1 = aux[((size_t)pi * L + j0 + 1) * L + x];
+            sacc += z.x * z.x * m0;
+            sacc += z.y * z.y * m1;
+            z.x *= m0;
+            z.y *= m1;
+        } else {
+            const float sg = ((pi + j0 + x) & 1) ? -1.0f : 1.0f;
+            z.x = z.x * z.x * sg;
+            z.y = z.y * z.y * (-sg);
+        }
+        buf[pp * LD + x] = z;
+    }
+    if (op == PLANE_OP_MASK) {
+        // deterministic block sum: shuffles inside the full waves, the partial wave's values through LDS
+        const int w = threadIdx.x >> 6;
+        if (w < NFW) {
+            for (int o = 32; o > 0; o >>= 1) sacc += __shfl_down(sacc, o, 64);
+            if ((threadIdx.x & 63) == 0) sred[w] = sacc;
+        } else {
+            sred[NFW + (threadIdx.x & 63)] = sacc;
+        }
+    }
+    __syncthreads();
+    if (op == PLANE_OP_MASK && threadIdx.x == 0) {
+        float tot = 0.f;
+        for (int i = 0; i < NFW + REM; i++) tot += sred[i];
+        partials[plane] = tot;
+    }
+    // r2c along k then j (as k_r2c_plane after its load)
+    pencil_fft<L>(&buf[q * LD], lane, tw, false);
+    float2 o[NS];
+#pragma unroll
+    for (int u = 0; u < NS; u++) {
+        const int t = threadIdx.x + u * T;
+        if (t < (L / 2) * 2 * H) {
+            const int pp = t / (2 * H), r = t % (2 * H), which = r / H, k = r % H;
+            const float2 z = buf[pp * LD + k], zc = cconj(buf[pp * LD + ((L - k) % L)]);
+            o[u] = which == 0 ? cscale(cadd(z, zc), 0.5f) : make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NS; u++) {
+        const int t = threadIdx.x + u * T;
+        if (t < (L / 2) * 2 * H) {
+            const int pp = t / (2 * H), r = t % (2 * H), which = r / H, k = r % H;
+            buf[k * LD + 2 * pp + which] = o[u];
+        }
+    }
+    __syncthreads();
+    pencil_fft<L>(&buf[q * LD], lane, tw, false);
+    for (int t = threadIdx.x; t < L * H; t += blockDim.x) {
+        const int j = t / H, k = t % H;
+        ip[(long)j * H + k] = buf[k * LD + j];
+    }
+}
+
 // ------------------------------------------------------------------ host launchers
 // profile record names: k_r2c_rows<L>, k_fft_strided<L>, k_c2r_rows<L>
 template <int L>
@@ -553,6 +677,43 @@ static void launch_c2r(const FftTables& T, const float2* in, float2* tmp, float*
             k_c2r_rows<L><<<(unsigned)((npairs + S::P - 1) / S::P), S::THREADS, 0, s>>>(tmp, out, npairs, tw);
         }
     }
+}
+
+template <int L>
+static void launch_c2r_op_r2c(const FftTables& T, const float2* in, float2* io, int batch, int op, const float* aux,
+                              float* partials, hipStream_t s) {
+    constexpr int H = L / 2 + 1;
+    const float2* tw = T.tw.p + T.offset[fft_len_index(L)];
+    const double vol_c = (double)batch * L * L * H * sizeof(float2);
+    StridedArgs si{in, io, (long)batch * L * H, (long)L * H, (long)L * L * H, (long)L * H, 1};
+    {
+        ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
+        k_fft_strided<L><<<(unsigned)((si.npencils + StridedShape<L>::P - 1) / StridedShape<L>::P), StridedShape<L>::THREADS, 0, s>>>(si, tw);
+    }
+    {
+        static const std::string name = "k_c2r_r2c_plane<" + std::to_string(L) + ">";
+        ProfScope ps(T.prof, name.c_str(), 2 * vol_c, s);
+        k_c2r_r2c_plane<L><<<(unsigned)((long)batch * L), Plane<L>::THREADS, 0, s>>>(io, op, aux, partials, tw);
+    }
+    StridedArgs sf{io, io, (long)batch * L * H, (long)L * H, (long)L * L * H, (long)L * H, 0};
+    {
+        ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
+        k_fft_strided<L><<<(unsigned)((sf.npencils + StridedShape<L>::P - 1) / StridedShape<L>::P), StridedShape<L>::THREADS, 0, s>>>(sf, tw);
+    }
+}
+
+bool fft3_fused_supported(int n) { return n == 16 || n == 32 || n == 48 || n == 64; }
+
+void fft3_c2r_op_r2c(const FftTables& T, const float2* in, float2* io, int n, int batch, int op, const float* aux,
+                     float* partials, hipStream_t s) {
+    switch (n) {
+        case 16: launch_c2r_op_r2c<16>(T, in, io, batch, op, aux, partials, s); break;
+        case 32: launch_c2r_op_r2c<32>(T, in, io, batch, op, aux, partials, s); break;
+        case 48: launch_c2r_op_r2c<48>(T, in, io, batch, op, aux, partials, s); break;
+        case 64: launch_c2r_op_r2c<64>(T, in, io, batch, op, aux, partials, s); break;
+        default: throw Error{ETTK_ERR_UNSUPPORTED, "fft3_c2r_op_r2c: unsupported size " + std::to_string(n)};
+    }
+    ETTK_HIP(hipGetLastError());
 }
 
 void fft3_r2c(const FftTables& T, const R2CArgs& a, float2* scratch, hipStream_t s) {

@@ -258,7 +258,7 @@ void ensure_ccc_workspace(ettk_plan* P, int cap) {
     P->p_n1.ensure((size_t)P->nb_spec * cap);
     P->p_n2.ensure((size_t)P->nb_spec * cap);
     P->p_wf.ensure((size_t)P->nb_spec * cap);
-    P->p_k1.ensure((size_t)P->nb_real * cap);
+    P->p_k1.ensure((size_t)std::max(P->nb_real, N) * cap);  // mask partials: per block, or per plane (fused)
     P->stats.ensure(cap);
     P->scalars.ensure((size_t)4 * cap);
     P->bval.ensure((size_t)P->z.E * cap);
@@ -386,13 +386,19 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
             launch_spec_filter(P->F1mcf.p, 0, P->S1.p, P->filter.p, W1, geo, E, N, 0, apply, P->p_n1.p, P->p_wf.p,
                                P->nb_spec, s);
         }
-        fft3_c2r(P->ctx->fft, P->S1.p, P->S1.p, P->R1.p, N, E, s);
-        {
-            ProfScope ps(kp, "k_mask_ref", 2 * Rb, s);
-            launch_mask_ref(P->R1.p, P->m1.p, E, N, P->p_k1.p, P->nb_real, s);
+        const bool fused = fft3_fused_supported(N);
+        if (fused) {
+            // c2r -> r1 m1 (with the k1_m partials) -> r2c, one partial per (evaluation, plane)
+            fft3_c2r_op_r2c(P->ctx->fft, P->S1.p, P->S1.p, N, E, PLANE_OP_MASK, P->m1.p, P->p_k1.p, s);
+        } else {
+            fft3_c2r(P->ctx->fft, P->S1.p, P->S1.p, P->R1.p, N, E, s);
+            {
+                ProfScope ps(kp, "k_mask_ref", 2 * Rb, s);
+                launch_mask_ref(P->R1.p, P->m1.p, E, N, P->p_k1.p, P->nb_real, s);
+            }
+            R2CArgs a1{P->R1.p, P->S1.p, N, E, PRO_NONE, nullptr, nullptr};
+            fft3_r2c(P->ctx->fft, a1, nullptr, s);
         }
-        R2CArgs a1{P->R1.p, P->S1.p, N, E, PRO_NONE, nullptr, nullptr};
-        fft3_r2c(P->ctx->fft, a1, nullptr, s);
         // particle side
         {
             ProfScope ps(kp, "k_reslice", 2 * Rb, s);
@@ -406,11 +412,17 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
             launch_spec_filter(P->S2.p, (int)nh, P->S2.p, P->filter.p, W1, geo, E, N, P->p.mcf, apply, P->p_n2.p,
                                nullptr, P->nb_spec, s);
         }
-        fft3_c2r(P->ctx->fft, P->S2.p, P->S3.p, P->R2.p, N, E, s);
-        R2CArgs a3{P->R2.p, P->S3.p, N, E, PRO_SQUARE_SHIFT, nullptr, nullptr};
-        fft3_r2c(P->ctx->fft, a3, nullptr, s);
+        if (fused) {
+            // Q = r2c(c2r(F2)^2 shift); F2 (S2) is kept for the combine
+            fft3_c2r_op_r2c(P->ctx->fft, P->S2.p, P->S3.p, N, E, PLANE_OP_SQUARE_SHIFT, nullptr, nullptr, s);
+        } else {
+            fft3_c2r(P->ctx->fft, P->S2.p, P->S3.p, P->R2.p, N, E, s);
+            R2CArgs a3{P->R2.p, P->S3.p, N, E, PRO_SQUARE_SHIFT, nullptr, nullptr};
+            fft3_r2c(P->ctx->fft, a3, nullptr, s);
+        }
         // combine, upsampled correlation on the search window, peak
-        launch_ccc_scalars(P->p_n1.p, P->p_n2.p, P->p_k1.p, P->p_wf.p, P->nb_spec, P->nb_real, E, N, P->scalars.p, s);
+        launch_ccc_scalars(P->p_n1.p, P->p_n2.p, P->p_k1.p, P->p_wf.p, P->nb_spec, fused ? N : P->nb_real, E, N,
+                           P->scalars.p, s);
         {
             ProfScope ps(kp, "k_combine_dftA", 3 * Hb + Gb, s);
             launch_combine_dftA(P->S1.p, P->S2.p, P->S3.p, P->Mfft.p, P->scalars.p, P->tabA.p, P->z, E, P->G1.p, s);
