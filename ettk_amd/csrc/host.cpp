@@ -380,16 +380,26 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
         const EvalGeom* geo = P->geom_all.p + e0;
         const double Rb = (double)E * N * N * N * sizeof(float), Hb = (double)E * nh * sizeof(float2);
         const double Gb = (double)E * P->z.E * N * H * sizeof(float2);
-        // reference side: F1 = mcf(r2c(I1 maskfft shift)) * filter * W1 W2 -> r1 = c2r -> k1_m, r1 m1 -> F1m
+        // particle side: F2 = r2c((reslice - mean) maskfft shift)
         {
-            ProfScope ps(kp, "k_spec_filter", 2 * Hb, s);
-            launch_spec_filter(P->F1mcf.p, 0, P->S1.p, P->filter.p, W1, geo, E, N, 0, apply, P->p_n1.p, P->p_wf.p,
-                               P->nb_spec, s);
+            ProfScope ps(kp, "k_reslice", 2 * Rb, s);
+            launch_reslice(d_parts, geo, E, N, P->R2.p, P->p_res.p, P->nb_res, s);
+        }
+        launch_reslice_mean(P->p_res.p, P->nb_res, E, P->stats.p, s);
+        R2CArgs a2{P->R2.p, P->S2.p, N, E, PRO_RESLICED, P->maskfft.p, P->stats.p};
+        fft3_r2c(P->ctx->fft, a2, nullptr, s);
+        // both spectra weighted by filter x W1 W2 in one pass: X1 = F1mcf (reference, MCF applied once), X2 = mcf(F2)
+        {
+            ProfScope ps(kp, "k_spec_filter", 4 * Hb, s);
+            launch_spec_filter2(P->F1mcf.p, P->S1.p, P->S2.p, P->filter.p, W1, geo, E, N, P->p.mcf, apply, P->p_n1.p,
+                                P->p_wf.p, P->p_n2.p, P->nb_spec, s);
         }
         const bool fused = fft3_fused_supported(N);
         if (fused) {
-            // c2r -> r1 m1 (with the k1_m partials) -> r2c, one partial per (evaluation, plane)
+            // reference side: c2r -> r1 m1 (with the k1_m partials) -> r2c = F1m, one partial per (evaluation, plane)
             fft3_c2r_op_r2c(P->ctx->fft, P->S1.p, P->S1.p, N, E, PLANE_OP_MASK, P->m1.p, P->p_k1.p, s);
+            // particle side: Q = r2c(c2r(X2)^2 shift); X2 (S2) is kept for the combine
+            fft3_c2r_op_r2c(P->ctx->fft, P->S2.p, P->S3.p, N, E, PLANE_OP_SQUARE_SHIFT, nullptr, nullptr, s);
         } else {
             fft3_c2r(P->ctx->fft, P->S1.p, P->S1.p, P->R1.p, N, E, s);
             {
@@ -398,24 +408,6 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
             }
             R2CArgs a1{P->R1.p, P->S1.p, N, E, PRO_NONE, nullptr, nullptr};
             fft3_r2c(P->ctx->fft, a1, nullptr, s);
-        }
-        // particle side
-        {
-            ProfScope ps(kp, "k_reslice", 2 * Rb, s);
-            launch_reslice(d_parts, geo, E, N, P->R2.p, P->p_res.p, P->nb_res, s);
-        }
-        launch_reslice_mean(P->p_res.p, P->nb_res, E, P->stats.p, s);
-        R2CArgs a2{P->R2.p, P->S2.p, N, E, PRO_RESLICED, P->maskfft.p, P->stats.p};
-        fft3_r2c(P->ctx->fft, a2, nullptr, s);
-        {
-            ProfScope ps(kp, "k_spec_filter", 2 * Hb, s);
-            launch_spec_filter(P->S2.p, (int)nh, P->S2.p, P->filter.p, W1, geo, E, N, P->p.mcf, apply, P->p_n2.p,
-                               nullptr, P->nb_spec, s);
-        }
-        if (fused) {
-            // Q = r2c(c2r(F2)^2 shift); F2 (S2) is kept for the combine
-            fft3_c2r_op_r2c(P->ctx->fft, P->S2.p, P->S3.p, N, E, PLANE_OP_SQUARE_SHIFT, nullptr, nullptr, s);
-        } else {
             fft3_c2r(P->ctx->fft, P->S2.p, P->S3.p, P->R2.p, N, E, s);
             R2CArgs a3{P->R2.p, P->S3.p, N, E, PRO_SQUARE_SHIFT, nullptr, nullptr};
             fft3_r2c(P->ctx->fft, a3, nullptr, s);

@@ -56,6 +56,11 @@ void launch_reslice_mean(const float* partials, int nb, int n_eval, float* stats
 void launch_spec_filter(const float2* F, int f_batch_stride, float2* X, const float* filter, const float* W1,
                         const EvalGeom* geom, int n_eval, int N, int mcf, int apply_filter, float* partials,
                         float* wf_partials, int nb, hipStream_t s);
+// both sides at once: X1 = F1 filter W1 W2(e) (F1 shared), X2 = mcf(X2) filter W1 W2(e) in place; partials of
+// |X1|^2, sum(W filter), |X2|^2 exactly as two launch_spec_filter calls produce them
+void launch_spec_filter2(const float2* F1, float2* X1, float2* X2, const float* filter, const float* W1,
+                         const EvalGeom* geom, int n_eval, int N, int mcf, int apply, float* p_n1, float* p_wf,
+                         float* p_n2, int nb, hipStream_t s);
 // r1 *= m1, partial sums of r1^2 m1
 void launch_mask_ref(float* R, const float* m1, int n_eval, int N, float* partials, int nb, hipStream_t s);
 // per-evaluation scalars: norm1, norm2, k1_m, sum(W filter)

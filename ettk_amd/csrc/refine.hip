@@ -284,7 +284,9 @@ __global__ void __launch_bounds__(kT) k_refine_adj(float2* __restrict__ X1, cons
     }
 }
 
-// twelve moments per evaluation: per-tile partial sums of gamma(p) dI1/dq_a w_b(p) over an 8x8x8 tile
+// twelve moments per evaluation: per-workgroup partial sums of gamma(p) dI1/dq_a w_b(p) over kMT consecutive
+// 8x8x8 tiles (one staged box per tile, one 12-value reduction per workgroup)
+static constexpr int kMT = 4;
 __global__ void __launch_bounds__(kT) k_refine_moments(const float* __restrict__ ref, const RefineGeom* __restrict__ geom,
                                                         const float* __restrict__ G, const float* __restrict__ maskfft,
                                                         int N, float* __restrict__ part) {
@@ -295,29 +297,34 @@ __global__ void __launch_bounds__(kT) k_refine_moments(const float* __restrict__
     const size_t n3 = (size_t)N * N * N;
     const float* Ge = G + (size_t)e * n3;
     const float half = 0.5f * (float)N;
-    const int nt = N / kRT;
-    const int tb = xcd_tile(blockIdx.x, gridDim.x);
-    const int z0 = (tb / (nt * nt)) * kRT, y0 = ((tb / nt) % nt) * kRT, x0 = (tb % nt) * kRT;
-    int lo[3];
-    const bool fits = stage_ref_box(ref, N, g, x0, y0, z0, box, lo);
+    const int nt = N / kRT, ntile = nt * nt * nt;
+    const int wb = xcd_tile(blockIdx.x, gridDim.x);
     float q12[12];
 #pragma unroll
     for (int r = 0; r < 12; r++) q12[r] = 0.f;
-    for (int t = threadIdx.x; t < kRT * kRT * kRT; t += kT) {
-        const int x = x0 + (t & 7), y = y0 + ((t >> 3) & 7), z = z0 + (t >> 6);
-        const size_t o = ((size_t)z * N + y) * N + x;
-        float q[3];
-        ref_pos(g, N, x, y, z, q);
-        const float gam = maskfft[o] * (((x + y + z) & 1) ? -Ge[o] : Ge[o]);
-        float gr[3];
-        if (fits) trilin_box(box, lo, q[0], q[1], q[2], gr);
-        else trilin(ref, N, q[0], q[1], q[2], gr);
-        const float w[4] = {(float)x - half, (float)y - half, (float)z - half, 1.f};
+    for (int u = 0; u < kMT; u++) {
+        const int tb = wb * kMT + u;
+        if (tb >= ntile) break;
+        const int z0 = (tb / (nt * nt)) * kRT, y0 = ((tb / nt) % nt) * kRT, x0 = (tb % nt) * kRT;
+        if (u > 0) __syncthreads();  // the previous tile's box is no longer read
+        int lo[3];
+        const bool fits = stage_ref_box(ref, N, g, x0, y0, z0, box, lo);
+        for (int t = threadIdx.x; t < kRT * kRT * kRT; t += kT) {
+            const int x = x0 + (t & 7), y = y0 + ((t >> 3) & 7), z = z0 + (t >> 6);
+            const size_t o = ((size_t)z * N + y) * N + x;
+            float q[3];
+            ref_pos(g, N, x, y, z, q);
+            const float gam = maskfft[o] * (((x + y + z) & 1) ? -Ge[o] : Ge[o]);
+            float gr[3];
+            if (fits) trilin_box(box, lo, q[0], q[1], q[2], gr);
+            else trilin(ref, N, q[0], q[1], q[2], gr);
+            const float w[4] = {(float)x - half, (float)y - half, (float)z - half, 1.f};
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-            const float ga = gam * gr[a];
+            for (int a = 0; a < 3; a++) {
+                const float ga = gam * gr[a];
 #pragma unroll
-            for (int b = 0; b < 4; b++) q12[a * 4 + b] += ga * w[b];
+                for (int b = 0; b < 4; b++) q12[a * 4 + b] += ga * w[b];
+            }
         }
     }
     const int wv = threadIdx.x >> 6;
@@ -330,9 +337,10 @@ __global__ void __launch_bounds__(kT) k_refine_moments(const float* __restrict__
     if (threadIdx.x < 12) {
         float t = 0.f;
         for (int i = 0; i < kT / 64; i++) t += sh[threadIdx.x][i];
-        part[((size_t)e * gridDim.x + tb) * 12 + threadIdx.x] = t;
+        part[((size_t)e * gridDim.x + wb) * 12 + threadIdx.x] = t;
     }
 }
+
 __global__ void __launch_bounds__(64) k_refine_reduce_moments(const float* __restrict__ part, int nb, int n_eval,
                                                               double* __restrict__ out) {
     const int e = blockIdx.x, l = threadIdx.x;
@@ -354,7 +362,7 @@ __global__ void __launch_bounds__(64) k_refine_reduce_moments(const float* __res
 
 int refine_moment_blocks(int N) {
     const int nt = N / kRT;
-    return nt * nt * nt;
+    return (nt * nt * nt + kMT - 1) / kMT;
 }
 int refine_dot_blocks(int N) {
     const long nh = (long)N * N * (N / 2 + 1);
