@@ -300,26 +300,19 @@ __global__ void __launch_bounds__(StridedShape<L>::THREADS) k_fft_strided(Stride
     __shared__ float2 tw[L];
     const long q0 = (long)blockIdx.x * S::P;
     for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
-    for (int t = threadIdx.x; t < S::P * L; t += blockDim.x) {
-        int q = t % S::P, l = t / S::P;
-        long pen = q0 + q;
-        float2 x = make_float2(0.f, 0.f);
-        if (pen < a.npencils) {
-            long base = (pen / a.inner) * a.outer_stride + (pen % a.inner);
-            x = a.in[base + l * a.es];
-        }
-        lds[q * S::LD + l] = x;
-    }
+    // THREADS is a multiple of P: a thread moves the elements l = t / P, t / P + THREADS / P, ... of the one
+    // pencil t % P, so the pencil's base address is computed once
+    const int qm = threadIdx.x % S::P;
+    const long pen = q0 + qm;
+    const bool valid = pen < a.npencils;
+    const long base = valid ? (pen / a.inner) * a.outer_stride + (pen % a.inner) : 0;
+    for (int l = threadIdx.x / S::P; l < L; l += S::THREADS / S::P)
+        lds[qm * S::LD + l] = valid ? a.in[base + l * a.es] : make_float2(0.f, 0.f);
     __syncthreads();
     const int q = threadIdx.x / S::TPP, lane = threadIdx.x % S::TPP;
     pencil_fft<L>(&lds[q * S::LD], lane, tw, a.inv != 0);
-    for (int t = threadIdx.x; t < S::P * L; t += blockDim.x) {
-        int qq = t % S::P, l = t / S::P;
-        long pen = q0 + qq;
-        if (pen >= a.npencils) continue;
-        long base = (pen / a.inner) * a.outer_stride + (pen % a.inner);
-        a.out[base + l * a.es] = lds[qq * S::LD + l];
-    }
+    if (valid)
+        for (int l = threadIdx.x / S::P; l < L; l += S::THREADS / S::P) a.out[base + l * a.es] = lds[qm * S::LD + l];
 }
 
 // ------------------------------------------------------------------ c2r, pass along k (half rows)
@@ -376,6 +369,29 @@ template <int L> struct Plane {
     static constexpr int LD = L + 1;
 };
 
+// r2c_load for the rows of one (b, i) plane: batch, plane offset and the per-batch statistics are uniform
+template <int L>
+__device__ __forceinline__ float r2c_load_plane(const R2CArgs& a, int b, int i, const float* __restrict__ src, float mean,
+                                                int j, int k) {
+    const float sg = ((i + j + k) & 1) ? -1.0f : 1.0f;
+    const int o = (i * L + j) * L + k;  // index inside one volume
+    switch (a.pro) {
+        case PRO_NONE: return src[j * L + k];
+        case PRO_SQUARE_SHIFT: {
+            const float x = src[j * L + k];
+            return x * x * sg;
+        }
+        case PRO_SHIFT: return src[j * L + k] * sg;
+        case PRO_MASK_SHIFT: return src[j * L + k] * a.aux0[o] * sg;
+        case PRO_RESLICED: {
+            const float x = src[j * L + k];
+            if (x == -3.402823466e+38f) return 0.0f;  // outside the particle box: -FLT_MAX background
+            return (x - mean) * (a.aux0 ? a.aux0[o] : 1.0f) * sg;
+        }
+        default: return r2c_load<L>(a, ((long)b * L + i) * L + j, k);
+    }
+}
+
 template <int L>
 __global__ void __launch_bounds__(Plane<L>::THREADS) k_r2c_plane(R2CArgs a, const float2* __restrict__ twg) {
     using PL = Plane<L>;
@@ -384,11 +400,14 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_r2c_plane(R2CArgs a, cons
     __shared__ float2 buf[H * LD];  // k pass: L/2 packed row pairs (+1 dummy); then j pass: H columns
     __shared__ float2 tw[L];
     const long plane = blockIdx.x;  // b * L + i
+    const int pb = (int)(plane / L), pi = (int)(plane % L);
+    const float* src = a.in + plane * L * L;
+    const float mean = a.pro == PRO_RESLICED ? a.stats[pb] : 0.0f;
     for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
     for (int t = threadIdx.x; t < L * L; t += blockDim.x) {
         const int j = t / L, x = t % L;
         float* f = reinterpret_cast<float*>(&buf[(j >> 1) * LD + x]);
-        f[j & 1] = r2c_load<L>(a, plane * L + j, x);
+        f[j & 1] = r2c_load_plane<L>(a, pb, pi, src, mean, j, x);
     }
     __syncthreads();
     const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
