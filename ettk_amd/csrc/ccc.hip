@@ -56,12 +56,12 @@ __global__ void __launch_bounds__(kThreads) k_reslice(const float* __restrict__ 
                                                        int N, float* __restrict__ out, float* __restrict__ partials) {
     __shared__ float sh[8];
     __shared__ float box[kBox * kBox * kBox];
-    const int e = blockIdx.y;
+    int e, tb;
+    xcd_eval_tile(blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, e, tb);
     const EvalGeom g = geom[e];
     const float* vol = parts + (size_t)g.particle * N * N * N;
     const long n3 = (long)N * N * N;
     const int nt = N / kTile;
-    const int tb = xcd_tile(blockIdx.x, gridDim.x);
     const int ti = tb / (nt * nt), tj = (tb / nt) % nt, tk = tb % nt;
     const int i0 = ti * kTile, j0 = tj * kTile, k0 = tk * kTile;
     const float tol = 7.62939453125e-06f;

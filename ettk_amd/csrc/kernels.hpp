@@ -21,6 +21,19 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
 }
 
+// (evaluation, tile) of workgroup (bx, by) of an nb x ne grid of per-evaluation tiles, one evaluation per XCD:
+// when both extents are multiples of 8, the workgroups dealt to XCD k (linear id = k mod 8) cover whole
+// evaluations (e = k mod 8 within each group of 8), so a particle volume is fetched into one L2 only.
+__device__ __forceinline__ void xcd_eval_tile(int bx, int by, int nb, int ne, int& e, int& tile) {
+    if ((nb & 7) == 0 && (ne & 7) == 0) {
+        e = (by & ~7) | (bx & 7);
+        tile = (by & 7) * (nb >> 3) + (bx >> 3);
+    } else {
+        e = by;
+        tile = xcd_tile(bx, nb);
+    }
+}
+
 // per-evaluation geometry computed on the host in FP64
 struct EvalGeom {
     float m[12];      // reslice map: input index = m[0..2]*i + m[3..5]*j + m[6..8]*k + m[9..11]  (rows x,y,z)
