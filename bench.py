@@ -172,6 +172,15 @@ def main() -> int:
                 "traffic": round(ratio * bytes_per_launch) if ratio else None,
                 "traffic_source": traffic_src,
                 "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": bytes_per_launch}
+    if dom[0] == "k_reslice":
+        # the tricubic reslice is arithmetic-bound: 64 taps x 2 FLOP per output voxel (33.5 MFLOP per
+        # evaluation at 64^3); evaluations per launch from its algorithmic bytes (2 R per evaluation)
+        n3 = 64 ** 3
+        evals = bytes_per_launch / (2 * n3 * 4)
+        flops = evals * n3 * 128
+        roofline["compute"] = {"achieved": round(flops / (avg_ms / 1e3) / 1e12, 2), "peak": 157.3,
+                               "unit": "TFLOP/s", "frac": round(flops / (avg_ms / 1e3) / 157.3e12, 4),
+                               "flops_per_launch": flops, "basis": "64-tap tricubic, 128 FLOP per output voxel, FP32 VALU"}
 
     # ---------------------------------------------------------------- CPU baseline: FP64 oracle on host cores
     cpu = None
