@@ -168,24 +168,43 @@ __device__ __forceinline__ void dft8(float2* a, bool inv) {
     a[3] = cadd(c6, c7);
     a[7] = csub(c6, c7);
 }
-// every thread of the workgroup must call it (it synchronises); `tw` holds exp(-2 pi i t / 64)
-__device__ __forceinline__ void fft64_reg(float2* pen, int lane, const float2* tw, bool inv) {
+// the twiddles a thread of a length-64 pencil applies, w[k] = exp(-2 pi i lane k / 64), held in registers
+// for all the pencil transforms of a kernel (length 64 only; other lengths read the LDS table)
+template <int L> struct TwCache {
+    float2 w[L == 64 ? 8 : 1];
+};
+template <int L>
+__device__ __forceinline__ TwCache<L> tw_cache(const float2* tw, int lane) {
+    TwCache<L> c;
+    if constexpr (L == 64) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) c.w[k] = tw[(lane * k) & 63];
+    } else {
+        c.w[0] = make_float2(1.f, 0.f);
+    }
+    return c;
+}
+
+// every thread of the workgroup must call it (it synchronises).  The 8 x 8 exchange goes through rows of 9
+// (pen[k * 9 + t] = element k of thread t), so neither its writes nor its reads collide in LDS banks; the
+// pencil's storage therefore spans 72 entries.
+__device__ __forceinline__ void fft64_reg(float2* pen, int lane, const TwCache<64>& tc, bool inv) {
     float2 x[8];
 #pragma unroll
     for (int m = 0; m < 8; m++) x[m] = pen[lane + 8 * m];
     dft8(x, inv);
 #pragma unroll
     for (int k = 1; k < 8; k++) {
-        float2 w = tw[(lane * k) & 63];
+        float2 w = tc.w[k];
         if (inv) w.y = -w.y;
         x[k] = cmul(x[k], w);
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 8; k++) pen[k * 8 + lane] = x[k];
+    for (int k = 0; k < 8; k++) pen[k * 9 + lane] = x[k];
     __syncthreads();
 #pragma unroll
-    for (int t = 0; t < 8; t++) x[t] = pen[lane * 8 + t];
+    for (int t = 0; t < 8; t++) x[t] = pen[lane * 9 + t];
     dft8(x, inv);
     __syncthreads();
 #pragma unroll
@@ -193,13 +212,14 @@ __device__ __forceinline__ void fft64_reg(float2* pen, int lane, const float2* t
     __syncthreads();
 }
 
-// threads per pencil of the pencil FFT used by the fused and strided passes
+// threads per pencil of the pencil FFT used by the fused and strided passes, and the LDS stride of a pencil
 template <int L> struct PenT {
     static constexpr int TPP = (L == 64) ? 8 : Shape<L>::TPP;
+    static constexpr int LD = (L == 64) ? 73 : L + 1;
 };
 template <int L>
-__device__ __forceinline__ void pencil_fft(float2* pen, int lane, const float2* tw, bool inv) {
-    if constexpr (L == 64) fft64_reg(pen, lane, tw, inv);
+__device__ __forceinline__ void pencil_fft(float2* pen, int lane, const float2* tw, const TwCache<L>& tc, bool inv) {
+    if constexpr (L == 64) fft64_reg(pen, lane, tc, inv);
     else lds_fft<L>(pen, lane, tw, inv);
 }
 
@@ -290,7 +310,7 @@ template <int L> struct StridedShape {
     static constexpr int TPP = PenT<L>::TPP;
     static constexpr int P = Shape<L>::THREADS / TPP;  // pencils per workgroup
     static constexpr int THREADS = Shape<L>::THREADS;
-    static constexpr int LD = L + 1;
+    static constexpr int LD = PenT<L>::LD;
 };
 
 template <int L>
@@ -310,7 +330,7 @@ __global__ void __launch_bounds__(StridedShape<L>::THREADS) k_fft_strided(Stride
         lds[qm * S::LD + l] = valid ? a.in[base + l * a.es] : make_float2(0.f, 0.f);
     __syncthreads();
     const int q = threadIdx.x / S::TPP, lane = threadIdx.x % S::TPP;
-    pencil_fft<L>(&lds[q * S::LD], lane, tw, a.inv != 0);
+    pencil_fft<L>(&lds[q * S::LD], lane, tw, tw_cache<L>(tw, lane), a.inv != 0);
     if (valid)
         for (int l = threadIdx.x / S::P; l < L; l += S::THREADS / S::P) a.out[base + l * a.es] = lds[qm * S::LD + l];
 }
@@ -366,7 +386,7 @@ template <int L> struct Plane {
     static constexpr int H = L / 2 + 1;
     static constexpr int TPP = PenT<L>::TPP;
     static constexpr int THREADS = H * TPP;
-    static constexpr int LD = L + 1;
+    static constexpr int LD = PenT<L>::LD;
 };
 
 // r2c_load for the rows of one (b, i) plane: batch, plane offset and the per-batch statistics are uniform
@@ -411,7 +431,8 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_r2c_plane(R2CArgs a, cons
     }
     __syncthreads();
     const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
-    pencil_fft<L>(&buf[q * LD], lane, tw, false);
+    const TwCache<L> twc = tw_cache<L>(tw, lane);
+    pencil_fft<L>(&buf[q * LD], lane, tw, twc, false);
     // split Z = A + iB of each pair into the half spectra of its two rows, through registers
     float2 o[NS];
 #pragma unroll
@@ -433,7 +454,7 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_r2c_plane(R2CArgs a, cons
         }
     }
     __syncthreads();
-    pencil_fft<L>(&buf[q * LD], lane, tw, false);
+    pencil_fft<L>(&buf[q * LD], lane, tw, twc, false);
     float2* op = a.out + plane * L * H;
     for (int t = threadIdx.x; t < L * H; t += blockDim.x) {
         const int j = t / H, k = t % H;
@@ -458,7 +479,8 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_plane(const float2* _
     }
     __syncthreads();
     const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
-    pencil_fft<L>(&buf[q * LD], lane, tw, true);
+    const TwCache<L> twc = tw_cache<L>(tw, lane);
+    pencil_fft<L>(&buf[q * LD], lane, tw, twc, true);
     // Z(k) = A(k) + i B(k) of rows 2p, 2p+1 over the full length, A(L-k) = conj(A(k)), real A(0), A(L/2)
     float2 zr[NZ];
 #pragma unroll
@@ -488,7 +510,7 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_plane(const float2* _
         if (t < (L / 2) * L) buf[(t / L) * LD + (t % L)] = zr[u];
     }
     __syncthreads();
-    pencil_fft<L>(&buf[q * LD], lane, tw, true);
+    pencil_fft<L>(&buf[q * LD], lane, tw, twc, true);
     float* op = out + plane * L * L;
     for (int t = threadIdx.x; t < L * L; t += blockDim.x) {
         const int j = t / L, x = t % L;
@@ -526,7 +548,8 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_r2c_plane(float2* __r
     }
     __syncthreads();
     const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
-    pencil_fft<L>(&buf[q * LD], lane, tw, true);
+    const TwCache<L> twc = tw_cache<L>(tw, lane);
+    pencil_fft<L>(&buf[q * LD], lane, tw, twc, true);
     // c2r along k: Z(k) = A(k) + i B(k) of rows 2p, 2p+1 (as k_c2r_plane)
     float2 zr[NZ];
 #pragma unroll
@@ -556,7 +579,7 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_r2c_plane(float2* __r
         if (t < (L / 2) * L) buf[(t / L) * LD + (t % L)] = zr[u];
     }
     __syncthreads();
-    pencil_fft<L>(&buf[q * LD], lane, tw, true);
+    pencil_fft<L>(&buf[q * LD], lane, tw, twc, true);
     __syncthreads();
     // buf[p LD + x] = (r(row 2p, x), r(row 2p + 1, x)): the pointwise operation in place
     float sacc = 0.f;
@@ -593,7 +616,7 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_r2c_plane(float2* __r
         partials[plane] = tot;
     }
     // r2c along k then j (as k_r2c_plane after its load)
-    pencil_fft<L>(&buf[q * LD], lane, tw, false);
+    pencil_fft<L>(&buf[q * LD], lane, tw, twc, false);
     float2 o[NS];
 #pragma unroll
     for (int u = 0; u < NS; u++) {
@@ -614,7 +637,7 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_r2c_plane(float2* __r
         }
     }
     __syncthreads();
-    pencil_fft<L>(&buf[q * LD], lane, tw, false);
+    pencil_fft<L>(&buf[q * LD], lane, tw, twc, false);
     for (int t = threadIdx.x; t < L * H; t += blockDim.x) {
         const int j = t / H, k = t % H;
         ip[(long)j * H + k] = buf[k * LD + j];
