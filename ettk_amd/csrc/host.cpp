@@ -343,8 +343,9 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
     const int N = P->N, H = P->H;
     const long nh = (long)N * N * H;
     hipStream_t s = P->ctx->stream;
-    const int CH = 128;
+    const int CH = 256;  // evaluations per chunk
     ensure_ccc_workspace(P, std::min(n, CH));
+    const int chunk = std::min(P->cap, CH);
     auto tw0 = std::chrono::steady_clock::now();
     struct WallRec {
         KernelProf* kp;
@@ -375,8 +376,8 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
     const float* W1 = P->p.missing_wedge ? P->W1.p : nullptr;
     const int apply = P->dummy_filter ? 0 : 1;
     KernelProf* kp = P->ctx->fft.prof;
-    for (int e0 = 0; e0 < n; e0 += P->cap) {
-        const int E = std::min(P->cap, n - e0);
+    for (int e0 = 0; e0 < n; e0 += chunk) {
+        const int E = std::min(chunk, n - e0);
         const EvalGeom* geo = P->geom_all.p + e0;
         const double Rb = (double)E * N * N * N * sizeof(float), Hb = (double)E * nh * sizeof(float2);
         const double Gb = (double)E * P->z.E * N * H * sizeof(float2);

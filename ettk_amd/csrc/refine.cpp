@@ -68,7 +68,7 @@ static void refine_eval_batch(ettk_plan* P, const ettk_particle* descs, int n, c
     const long nh = (long)N * N * H;
     hipStream_t s = P->ctx->stream;
     ensure_ccc_workspace(P, std::min(n, 128));
-    const int cap = P->cap;
+    const int cap = std::min(P->cap, 128);  // evaluations per chunk of the refinement
     const int nbd = refine_dot_blocks(N), nbm = refine_moment_blocks(N);
     P->r_pdot.ensure((size_t)3 * nbd * cap);
     P->r_mpart.ensure((size_t)12 * nbm * cap);
