@@ -597,6 +597,68 @@ __device__ void rotate_coef_block(const double2* __restrict__ cc, double alpha, 
     }
 }
 
+// d^l_{m m'}(beta_b) for l < B, |m|, |m'| <= l at the angles rotateFct receives for the extrema's beta bins,
+// beta_b = (double)(float)((2b + 1) pi / 4B) (Pixel angles in the reference); stored [b][l][m + l][m' + l]
+// after sum_{l' < l} (2l' + 1)^2 entries per l.  The same recurrence as the direct evaluation.
+__device__ __forceinline__ size_t rot_tab_l(int l) { return (size_t)l * (2 * l - 1) * (2 * l + 1) / 3; }
+__global__ void __launch_bounds__(kT) k_wigner_rot_table(int B, double* __restrict__ tab) {
+    const int b = blockIdx.x, W = 2 * B - 1;
+    const double beta = (double)(float)((2 * b + 1.0) / (4. * B) * M_PI);
+    const double cb = cos(beta);
+    double* T = tab + (size_t)b * rot_tab_l(B);
+    for (int t = threadIdx.x; t < W * W; t += blockDim.x) {
+        const int m = t / W - (B - 1), mp = t % W - (B - 1);
+        const int l0 = max(abs(m), abs(mp));
+        double dprev = 0, dcur = wig_seed(m, mp, beta);
+        for (int l = l0; l < B; l++) {
+            T[rot_tab_l(l) + (size_t)(m + l) * (2 * l + 1) + (mp + l)] = dcur;
+            double dn = wig_next(l, m, mp, cb, dcur, dprev);
+            dprev = dcur;
+            dcur = dn;
+        }
+    }
+}
+
+// k_rotate_coef with the d values from the table: one thread per output (l, m), the sum over m' in increasing
+// order as the direct evaluation accumulates it; the phases (cos, sin) of -(m alpha + m' gamma) in LDS
+__global__ void __launch_bounds__(kT) k_rotate_coef_tab(const double2* __restrict__ c, const int* __restrict__ pos,
+                                                         int B, const double* __restrict__ tab, double2* __restrict__ r) {
+    extern __shared__ double2 cs[];  // [m + B - 1][m' + B - 1]
+    const int q = blockIdx.x, which = blockIdx.y;
+    const int n = 2 * B, W = 2 * B - 1;
+    const int p = pos[q * 2 + which];
+    const int a = p / (n * n), b = (p / n) % n, g = p % n;
+    const double alpha = (double)(float)(a / (double)B * M_PI);
+    const double gamma = (double)(float)(g / (double)B * M_PI);
+    for (int t = threadIdx.x; t < W * W; t += blockDim.x) {
+        const int m = t / W - (B - 1), mp = t % W - (B - 1);
+        const double ph = -(m * alpha + mp * gamma);
+        cs[t] = make_double2(cos(ph), sin(ph));
+    }
+    __syncthreads();
+    const double* T = tab + (size_t)b * rot_tab_l(B);
+    const double2* cc = c + (size_t)q * B * B;
+    double2* out = r + ((size_t)q * 2 + which) * B * B;
+    for (int t = threadIdx.x; t < B * B; t += blockDim.x) {
+        int l = (int)sqrt((double)t);
+        while (l * l > t) l--;
+        while ((l + 1) * (l + 1) <= t) l++;
+        const int m = t - l * l - l;
+        const double* d = T + rot_tab_l(l) + (size_t)(m + l) * (2 * l + 1) + l;
+        const double2* e = cs + (m + B - 1) * W + (B - 1);
+        double2 o = make_double2(0, 0);
+        for (int mp = -l; mp <= l; mp++) {
+            const double dcur = d[mp];
+            const double2 ph = e[mp];
+            const double2 z = cc[l * l + l + mp];
+            double Dr = ph.x * dcur, Di = ph.y * dcur;
+            o.x += Dr * z.x - Di * z.y;
+            o.y += Dr * z.y + Di * z.x;
+        }
+        out[t] = o;
+    }
+}
+
 // the pattern's coefficients rotated by the Euler angles of the extrema of C (rotatedSphericalImageMinMax,
 // ProjRot.h:1558-1589).  Block (q, which): which = 0 min, 1 max.
 __global__ void k_rotate_coef(const double2* __restrict__ c, const int* __restrict__ pos, int B, double2* __restrict__ r) {
@@ -850,8 +912,22 @@ void soft_core(ettk_plan* P, const std::vector<SoftInst>& inst, double* d_C) {
             ProfScope ps(P->ctx->fft.prof, "k_extrema", (double)nq * n3 * 8, s);
             k_extrema<<<nq, kT, 0, s>>>(d_C, n3, S.vals.p, S.pos.p);
         }
-        size_t shm = (size_t)B * (2 * B - 1) * sizeof(double2);
-        k_rotate_coef<<<dim3(nq, 2), kT, shm, s>>>(S.cp.p, S.pos.p, B, S.rot.p);
+        {
+            ProfScope ps(P->ctx->fft.prof, "k_rotate_coef", (double)2 * nq * (B * B * 32.0), s);
+            if (B <= 32) {
+                // table of d at the 2B beta bins (22 MB at B = 32), phases in LDS ((2B-1)^2 complex)
+                if (S.rtab.n == 0) {
+                    const size_t per_b = (size_t)B * (2 * B - 1) * (2 * B + 1) / 3;
+                    S.rtab.ensure(per_b * 2 * B);
+                    k_wigner_rot_table<<<2 * B, kT, 0, s>>>(B, S.rtab.p);
+                }
+                const size_t shm = (size_t)(2 * B - 1) * (2 * B - 1) * sizeof(double2);
+                k_rotate_coef_tab<<<dim3(nq, 2), kT, shm, s>>>(S.cp.p, S.pos.p, B, S.rtab.p, S.rot.p);
+            } else {
+                const size_t shm = (size_t)B * (2 * B - 1) * sizeof(double2);
+                k_rotate_coef<<<dim3(nq, 2), kT, shm, s>>>(S.cp.p, S.pos.p, B, S.rot.p);
+            }
+        }
         k_legendre_inv<<<dim3(16, 2 * nq), kT, 0, s>>>(S.rot.p, S.legI.p, B, S.g.p);
         k_ring_idft<<<dim3(16, 2 * nq), kT, 0, s>>>(S.g.p, S.tw.p, B, S.Rr.p);
         k_wsum<<<dim3(nq, 2), kT, 0, s>>>(S.sig.p, S.Rr.p, B, S.wv.p);
@@ -958,6 +1034,10 @@ static void so3_kernel_attributes(int B) {
     int shm = B * (2 * B - 1) * (int)sizeof(double2);
     ETTK_HIP(hipFuncSetAttribute((const void*)k_rotate_coef, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
     ETTK_HIP(hipFuncSetAttribute((const void*)k_rotate_coef_angles, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+    if (B <= 32) {
+        const int shm2 = (2 * B - 1) * (2 * B - 1) * (int)sizeof(double2);
+        ETTK_HIP(hipFuncSetAttribute((const void*)k_rotate_coef_tab, hipFuncAttributeMaxDynamicSharedMemorySize, shm2));
+    }
     if (fused_synthesis(B)) {
         const int n = 2 * B, fs = n * (n + 1) * (int)sizeof(double2);
         const void* f = n == 16 ? (const void*)k_so3_synth_fft<4> : n == 32 ? (const void*)k_so3_synth_fft<5>

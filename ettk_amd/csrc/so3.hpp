@@ -35,6 +35,7 @@ struct So3State {
     int d_slice = 0;
     bool use_window1 = false;
     DevBuf<double> legF, legI, dtab, dtab2;
+    DevBuf<double> rtab;  // d^l_{m m'} at the softScaled angles beta_b = (float)((2b+1) pi / 4B), [b][l][m][m']
     DevBuf<int> doff;
     DevBuf<double2> tw;
     // reference
