@@ -232,17 +232,44 @@ __global__ void __launch_bounds__(kT) k_sph_sample(const float* __restrict__ mag
             w[a][2] = -((ft3 - 4) * f - 1) * fd2;
             w[a][3] = f * fd2 * fm1;
         }
+        // the 4x4x4 stencil lies inside the stored half (k <= M/2) or entirely in the mirrored half, away from the
+        // i = 0 plane, the j = 0 column of the mirror and the wrap: then its taps sit at fixed strides (the values
+        // mag_at would return); otherwise every tap goes through mag_at
+        const int i0 = idx[0] - 1, j0 = idx[1] - 1, k0 = idx[2] - 1, Hm = M / 2 + 1;
+        const bool ij_in = i0 >= 1 && i0 + 3 <= M - 1 && j0 >= 0 && j0 + 3 <= M - 1;
+        const bool direct = ij_in && k0 >= 0 && k0 + 3 <= M / 2;
+        const bool mirror = ij_in && j0 >= 1 && k0 >= M / 2 + 1 && k0 + 3 <= M - 1;
         double v = 0;
-        for (int cz = 0; cz < 4; cz++) {
-            const int kz = wrap_idx(idx[2] - 1 + cz, M);
-            double vy = 0;
-            for (int cy = 0; cy < 4; cy++) {
-                const int jy = wrap_idx(idx[1] - 1 + cy, M);
-                double vx = 0;
-                for (int cx = 0; cx < 4; cx++) vx += w[0][cx] * (double)mag_at(mg, M, wrap_idx(idx[0] - 1 + cx, M), jy, kz, padc);
-                vy += w[1][cy] * vx;
+        if (direct || mirror) {
+            const long si = direct ? (long)M * Hm : -(long)M * Hm, sj = direct ? Hm : -Hm, sk = direct ? 1 : -1;
+            const float* pb = direct ? mg + ((long)i0 * M + j0) * Hm + k0
+                                     : mg + ((long)(M - i0) * M + (M - j0)) * Hm + (M - k0);
+#pragma unroll
+            for (int cz = 0; cz < 4; cz++) {
+                double vy = 0;
+#pragma unroll
+                for (int cy = 0; cy < 4; cy++) {
+                    const float* pr = pb + cz * sk + cy * sj;
+                    double vx = 0;
+#pragma unroll
+                    for (int cx = 0; cx < 4; cx++) vx += w[0][cx] * (double)pr[cx * si];
+                    vy += w[1][cy] * vx;
+                }
+                v += w[2][cz] * vy;
             }
-            v += w[2][cz] * vy;
+        } else {
+            for (int cz = 0; cz < 4; cz++) {
+                const int kz = wrap_idx(idx[2] - 1 + cz, M);
+                double vy = 0;
+                for (int cy = 0; cy < 4; cy++) {
+                    const int jy = wrap_idx(idx[1] - 1 + cy, M);
+                    double vx = 0;
+                    for (int cx = 0; cx < 4; cx++)
+                        vx += w[0][cx] * (double)mag_at(mg, M, wrap_idx(idx[0] - 1 + cx, M), jy, kz, padc);
+                    vy += w[1][cy] * vx;
+                }
+                v += w[2][cz] * vy;
+            }
         }
         sum += v;
     }
