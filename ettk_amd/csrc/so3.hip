@@ -552,14 +552,18 @@ __global__ void __launch_bounds__(kT) k_so3_synth_fft(const double2* __restrict_
 
 // ============================================================== softScaled pieces
 // first maximum and first minimum in row-major order (ProjRot.h:1562-1576)
-__global__ void k_extrema(const double* __restrict__ C, long n3, double* __restrict__ vals, int* __restrict__ pos) {
+// min / max of each correlation volume and the first position of each (blitz minIndex / maxIndex order):
+// kExtParts workgroups per volume scan contiguous parts, k_extrema_final combines them with the same rule
+static constexpr int kExtParts = 16;
+__global__ void k_extrema(const double* __restrict__ C, long n3, double* __restrict__ pvals, int* __restrict__ ppos) {
     __shared__ double smax[kT], smin[kT];
     __shared__ int imax[kT], imin[kT];
-    const int q = blockIdx.x;
+    const int q = blockIdx.x, part = blockIdx.y;
     const double* c = C + (size_t)q * n3;
+    const long o0 = n3 * part / kExtParts, o1 = n3 * (part + 1) / kExtParts;
     double bx = -DBL_MAX, bn = DBL_MAX;
     int ix = 0x7fffffff, in = 0x7fffffff;
-    for (long o = threadIdx.x; o < n3; o += blockDim.x) {
+    for (long o = o0 + threadIdx.x; o < o1; o += blockDim.x) {
         double v = c[o];
         if (v > bx) { bx = v; ix = (int)o; }
         if (v < bn) { bn = v; in = (int)o; }
@@ -580,11 +584,30 @@ __global__ void k_extrema(const double* __restrict__ C, long n3, double* __restr
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        vals[q * 2 + 0] = smin[0];
-        vals[q * 2 + 1] = smax[0];
-        pos[q * 2 + 0] = imin[0];
-        pos[q * 2 + 1] = imax[0];
+        const size_t k = (size_t)q * kExtParts + part;
+        pvals[k * 2 + 0] = smin[0];
+        pvals[k * 2 + 1] = smax[0];
+        ppos[k * 2 + 0] = imin[0];
+        ppos[k * 2 + 1] = imax[0];
     }
+}
+__global__ void k_extrema_final(const double* __restrict__ pvals, const int* __restrict__ ppos, int nq,
+                                double* __restrict__ vals, int* __restrict__ pos) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    double bn = DBL_MAX, bx = -DBL_MAX;
+    int in = 0x7fffffff, ix = 0x7fffffff;
+    for (int part = 0; part < kExtParts; part++) {
+        const size_t k = (size_t)q * kExtParts + part;
+        const double vn = pvals[k * 2 + 0], vx = pvals[k * 2 + 1];
+        const int jn = ppos[k * 2 + 0], jx = ppos[k * 2 + 1];
+        if (vn < bn || (vn == bn && jn < in)) { bn = vn; in = jn; }
+        if (vx > bx || (vx == bx && jx < ix)) { bx = vx; ix = jx; }
+    }
+    vals[q * 2 + 0] = bn;
+    vals[q * 2 + 1] = bx;
+    pos[q * 2 + 0] = in;
+    pos[q * 2 + 1] = ix;
 }
 
 // rotateFct (rotate_so3.c:695-751) on the coefficients of one S^2 function:
@@ -937,7 +960,10 @@ void soft_core(ettk_plan* P, const std::vector<SoftInst>& inst, double* d_C) {
     if (any_scaled) {
         {
             ProfScope ps(P->ctx->fft.prof, "k_extrema", (double)nq * n3 * 8, s);
-            k_extrema<<<nq, kT, 0, s>>>(d_C, n3, S.vals.p, S.pos.p);
+            S.pvals.ensure((size_t)2 * kExtParts * nq);
+            S.ppos.ensure((size_t)2 * kExtParts * nq);
+            k_extrema<<<dim3(nq, kExtParts), kT, 0, s>>>(d_C, n3, S.pvals.p, S.ppos.p);
+            k_extrema_final<<<(nq + 63) / 64, 64, 0, s>>>(S.pvals.p, S.ppos.p, nq, S.vals.p, S.pos.p);
         }
         {
             ProfScope ps(P->ctx->fft.prof, "k_rotate_coef", (double)2 * nq * (B * B * 32.0), s);

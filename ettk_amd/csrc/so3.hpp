@@ -46,7 +46,8 @@ struct So3State {
     DevBuf<EvalGeom> projGeom;
     // soft workspace
     DevBuf<SoftInst> inst;
-    DevBuf<double> sig, pat, vals, Rr, wv;
+    DevBuf<double> sig, pat, vals, Rr, wv, pvals;
+    DevBuf<int> ppos;
     DevBuf<double2> fm, cs, cp, T1, T2, rot, g;
     DevBuf<int> pos;
     // search workspace (align driver)
