@@ -26,6 +26,29 @@ namespace ettk {
 // ------------------------------------------------------------------ spherical wedge (host, FP64)
 // nbfWedgedSubImage3D::getSphericalWedgeImage -> nbfWedge3D::getSphericalImage (Wedge.h:364-429): the binary
 // wedge on the grid theta_i = i pi/(2B-1), phi_j = j 2pi/(2B-1) (Pixel arithmetic), all ones without a wedge
+// grid directions (cos theta_j sin phi_i, sin theta_j sin phi_i, cos phi_i), phi_i = i pi/(2B-1),
+// theta_j = j 2pi/(2B-1), in the Pixel (float) arithmetic of nbfWedge3D::getSphericalImage; one table per B
+static const std::vector<double>& sph_dirs(int B) {
+    static std::mutex mu;
+    static std::map<int, std::vector<double>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(B);
+    if (it != cache.end()) return it->second;
+    const int n = 2 * B;
+    std::vector<double> t((size_t)3 * n * n);
+    float phiK = (float)(M_PI / (n - 1)), thetaK = (float)(2.0 * M_PI / (n - 1));
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            float si = std::sin((float)(i * phiK)), ci = std::cos((float)(i * phiK));
+            float sj = std::sin((float)(j * thetaK)), cj = std::cos((float)(j * thetaK));
+            double* d3 = &t[3 * ((size_t)i * n + j)];
+            d3[0] = (double)(cj * si);
+            d3[1] = (double)(sj * si);
+            d3[2] = (double)ci;
+        }
+    return cache.emplace(B, std::move(t)).first->second;
+}
+
 std::vector<float> spherical_wedge_bin(int B, const ettk_particle& d) {
     const int n = 2 * B;
     std::vector<float> W((size_t)n * n, 1.0f);
@@ -39,15 +62,13 @@ std::vector<float> spherical_wedge_bin(int B, const ettk_particle& d) {
             v1[r] = R(r, 0) * n1[0] + R(r, 1) * n1[1] + R(r, 2) * n1[2];
             v2[r] = R(r, 0) * n2[0] + R(r, 1) * n2[1] + R(r, 2) * n2[2];
         }
-        float phiK = (float)(M_PI / (n - 1)), thetaK = (float)(2.0 * M_PI / (n - 1));
-        for (int i = 0; i < n; i++)
-            for (int j = 0; j < n; j++) {
-                float si = std::sin((float)(i * phiK)), ci = std::cos((float)(i * phiK));
-                float sj = std::sin((float)(j * thetaK)), cj = std::cos((float)(j * thetaK));
-                double a = v1[0] * (double)(cj * si) + v1[1] * (double)(sj * si) + v1[2] * (double)ci;
-                double b = v2[0] * (double)(cj * si) + v2[1] * (double)(sj * si) + v2[2] * (double)ci;
-                W[(size_t)i * n + j] = (a * b <= 0.0) ? 1.0f : 0.0f;
-            }
+        const std::vector<double>& dir = sph_dirs(B);
+        for (size_t o = 0; o < (size_t)n * n; o++) {
+            const double* d3 = &dir[3 * o];
+            double a = v1[0] * d3[0] + v1[1] * d3[1] + v1[2] * d3[2];
+            double b = v2[0] * d3[0] + v2[1] * d3[1] + v2[2] * d3[2];
+            W[o] = (a * b <= 0.0) ? 1.0f : 0.0f;
+        }
     }
     return W;
 }
@@ -152,20 +173,23 @@ struct SearchOut {
     int use_window = 0;
 };
 
-// Host-side inputs of the search for every particle of a call, computed on all host threads before the first
-// launch: smoothed spherical wedge W2 (nbfWedge3D::getSphericalImage + conditionSphericalWindow), whether it
-// is "effective" (sum <= 0.95 size, ProjRot.h:526-540), and the reslice geometry of the projection.
+// Host-side inputs of the search, per chunk of particles (computed on the host threads while the previous
+// chunk runs on the GPU): smoothed spherical wedge W2 (nbfWedge3D::getSphericalImage +
+// conditionSphericalWindow), whether it is "effective" (sum <= 0.95 size, ProjRot.h:526-540), and the reslice
+// geometry of the projection.  The arrays are sized for the whole call and stay alive until it ends.
 struct SearchInputs {
     std::vector<float> W2;
     std::vector<int> uw2;
     std::vector<EvalGeom> geom;
 };
 
-static void search_inputs(const ettk_plan* P, const ettk_particle* descs, int n, SearchInputs& in) {
+static void search_inputs(const ettk_plan* P, const ettk_particle* descs, int n, int p0, int cn, SearchInputs& in) {
     const int B = P->p.B, nn = 4 * B * B;
-    in.W2.assign((size_t)n * nn, 0.0f);
-    in.uw2.assign(n, 0);
-    in.geom.resize(n);
+    if (in.uw2.size() != (size_t)n) {
+        in.W2.assign((size_t)n * nn, 0.0f);
+        in.uw2.assign(n, 0);
+        in.geom.resize(n);
+    }
     auto work = [&](int i) {
         std::vector<float> w = spherical_wedge_host(B, descs[i], P->p.missing_wedge != 0);
         double sw = 0;
@@ -174,15 +198,15 @@ static void search_inputs(const ettk_plan* P, const ettk_particle* descs, int n,
         std::copy(w.begin(), w.end(), in.W2.begin() + (size_t)i * nn);
         in.geom[i] = make_geom_public(P, descs[i], Mat4::ident(), i);
     };
-    const int nthr = std::max(1, std::min<int>(n, (int)std::thread::hardware_concurrency()));
-    if (nthr <= 1 || n < 4) {
-        for (int i = 0; i < n; i++) work(i);
+    const int nthr = std::max(1, std::min<int>(cn / 4, (int)std::thread::hardware_concurrency()));
+    if (nthr <= 1) {
+        for (int i = p0; i < p0 + cn; i++) work(i);
         return;
     }
     std::vector<std::thread> th;
     for (int w = 0; w < nthr; w++)
         th.emplace_back([&, w] {
-            for (int i = w; i < n; i += nthr) work(i);
+            for (int i = p0 + w; i < p0 + cn; i += nthr) work(i);
         });
     for (auto& t : th) t.join();
 }
@@ -347,15 +371,16 @@ static void search_all(ettk_plan* P, const float* d_parts, const ettk_particle* 
     KernelProf* kp = P->ctx->fft.prof;
     auto ts0 = std::chrono::steady_clock::now();
     SearchInputs in;
-    search_inputs(P, descs, n, in);
     std::vector<std::unique_ptr<SearchLists>> lists;
     std::vector<std::future<void>> pending;
     std::exception_ptr err;
     try {
         for (int p0 = 0, k = 0; p0 < n; p0 += chunk, k++) {
+            const int cn = std::min(chunk, n - p0);
+            search_inputs(P, descs, n, p0, cn, in);
             lists.emplace_back(new SearchLists());
             SearchLists* L = lists.back().get();
-            search_gpu(P, d_parts, descs, p0, std::min(chunk, n - p0), in, k, *L);
+            search_gpu(P, d_parts, descs, p0, cn, in, k, *L);
             pending.push_back(std::async(std::launch::async, [P, L, &out] { search_host(P, *L, out); }));
         }
     } catch (...) {
@@ -837,8 +862,8 @@ void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
             return d;
         };
         SearchInputs in;
-        search_inputs(P, descs, n, in);
         for (int i = 0; i < n; i++) {
+            search_inputs(P, descs, n, i, 1, in);
             SearchLists L;
             search_gpu(P, d_parts, descs, i, 1, in, 0, L);
             search_host(P, L, so);
