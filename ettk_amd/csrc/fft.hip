@@ -223,6 +223,25 @@ __device__ __forceinline__ void pencil_fft(float2* pen, int lane, const float2* 
     else lds_fft<L>(pen, lane, tw, inv);
 }
 
+// A workgroup-wide copy of NE elements by NT threads: thread t moves elements t, t + NT, ...  Every global load
+// of the thread is issued before the first value is stored (out-of-range slots re-read the last element and are
+// dropped), so one memory latency covers the whole copy instead of one per element.
+template <int NT, int NE, class Ld, class St>
+__device__ __forceinline__ void block_copy(Ld ld, St st) {
+    constexpr int NI = (NE + NT - 1) / NT;
+    decltype(ld(0)) v[NI];
+#pragma unroll
+    for (int u = 0; u < NI; u++) {
+        const int t = (int)threadIdx.x + u * NT;
+        v[u] = ld((NE % NT == 0 || t < NE) ? t : NE - 1);
+    }
+#pragma unroll
+    for (int u = 0; u < NI; u++) {
+        const int t = (int)threadIdx.x + u * NT;
+        if (NE % NT == 0 || t < NE) st(t, v[u]);
+    }
+}
+
 // ------------------------------------------------------------------ r2c, pass along k (real rows)
 template <int L>
 __device__ __forceinline__ float r2c_load(const R2CArgs& a, long row, int k) {
@@ -326,13 +345,22 @@ __global__ void __launch_bounds__(StridedShape<L>::THREADS) k_fft_strided(Stride
     const long pen = q0 + qm;
     const bool valid = pen < a.npencils;
     const long base = valid ? (pen / a.inner) * a.outer_stride + (pen % a.inner) : 0;
-    for (int l = threadIdx.x / S::P; l < L; l += S::THREADS / S::P)
-        lds[qm * S::LD + l] = valid ? a.in[base + l * a.es] : make_float2(0.f, 0.f);
+    constexpr int STEP = S::THREADS / S::P, NL = L / STEP;
+    static_assert(L % STEP == 0, "pencil elements per thread");
+    const int l0 = threadIdx.x / S::P;
+    {
+        float2 v[NL];  // all loads in flight before the first LDS store (an invalid pencil re-reads pencil 0)
+#pragma unroll
+        for (int u = 0; u < NL; u++) v[u] = a.in[base + (long)(l0 + u * STEP) * a.es];
+#pragma unroll
+        for (int u = 0; u < NL; u++) lds[qm * S::LD + l0 + u * STEP] = valid ? v[u] : make_float2(0.f, 0.f);
+    }
     __syncthreads();
     const int q = threadIdx.x / S::TPP, lane = threadIdx.x % S::TPP;
     pencil_fft<L>(&lds[q * S::LD], lane, tw, tw_cache<L>(tw, lane), a.inv != 0);
     if (valid)
-        for (int l = threadIdx.x / S::P; l < L; l += S::THREADS / S::P) a.out[base + l * a.es] = lds[qm * S::LD + l];
+#pragma unroll
+        for (int u = 0; u < NL; u++) a.out[base + (long)(l0 + u * STEP) * a.es] = lds[qm * S::LD + l0 + u * STEP];
 }
 
 // ------------------------------------------------------------------ c2r, pass along k (half rows)
@@ -389,27 +417,26 @@ template <int L> struct Plane {
     static constexpr int LD = PenT<L>::LD;
 };
 
-// r2c_load for the rows of one (b, i) plane: batch, plane offset and the per-batch statistics are uniform
-template <int L>
-__device__ __forceinline__ float r2c_load_plane(const R2CArgs& a, int b, int i, const float* __restrict__ src, float mean,
-                                                int j, int k) {
-    const float sg = ((i + j + k) & 1) ? -1.0f : 1.0f;
-    const int o = (i * L + j) * L + k;  // index inside one volume
-    switch (a.pro) {
-        case PRO_NONE: return src[j * L + k];
-        case PRO_SQUARE_SHIFT: {
-            const float x = src[j * L + k];
-            return x * x * sg;
-        }
-        case PRO_SHIFT: return src[j * L + k] * sg;
-        case PRO_MASK_SHIFT: return src[j * L + k] * a.aux0[o] * sg;
-        case PRO_RESLICED: {
-            const float x = src[j * L + k];
-            if (x == -3.402823466e+38f) return 0.0f;  // outside the particle box: -FLT_MAX background
-            return (x - mean) * (a.aux0 ? a.aux0[o] : 1.0f) * sg;
-        }
-        default: return r2c_load<L>(a, ((long)b * L + i) * L + j, k);
-    }
+// the k-pass input of plane i for one preprocessing mode, staged as packed row pairs buf[(j/2) LD + x]
+// (.x row j even, .y row j odd); MSK: aux0 holds the N^3 mask
+template <int L, int PRO, bool MSK>
+__device__ __forceinline__ void r2c_stage_plane(const R2CArgs& a, int pi, const float* __restrict__ src, float mean,
+                                                float2* buf) {
+    constexpr int LD = Plane<L>::LD;
+    const float* __restrict__ msk = MSK ? a.aux0 + (size_t)pi * L * L : nullptr;
+    block_copy<Plane<L>::THREADS, L * L>(
+        [&](int t) { return make_float2(src[t], MSK ? msk[t] : 1.0f); },
+        [&](int t, float2 v) {
+            const int j = t / L, x = t % L;
+            const float sg = ((pi + j + x) & 1) ? -1.0f : 1.0f;
+            float r;
+            if constexpr (PRO == PRO_NONE) r = v.x;
+            else if constexpr (PRO == PRO_SQUARE_SHIFT) r = v.x * v.x * sg;
+            else if constexpr (PRO == PRO_SHIFT) r = v.x * sg;
+            else if constexpr (PRO == PRO_MASK_SHIFT) r = v.x * v.y * sg;
+            else r = (v.x == -3.402823466e+38f) ? 0.0f : (v.x - mean) * v.y * sg;  // PRO_RESLICED
+            reinterpret_cast<float*>(&buf[(j >> 1) * LD + x])[j & 1] = r;
+        });
 }
 
 template <int L>
@@ -424,10 +451,21 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_r2c_plane(R2CArgs a, cons
     const float* src = a.in + plane * L * L;
     const float mean = a.pro == PRO_RESLICED ? a.stats[pb] : 0.0f;
     for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
-    for (int t = threadIdx.x; t < L * L; t += blockDim.x) {
-        const int j = t / L, x = t % L;
-        float* f = reinterpret_cast<float*>(&buf[(j >> 1) * LD + x]);
-        f[j & 1] = r2c_load_plane<L>(a, pb, pi, src, mean, j, x);
+    switch (a.pro) {
+        case PRO_NONE: r2c_stage_plane<L, PRO_NONE, false>(a, pi, src, mean, buf); break;
+        case PRO_SQUARE_SHIFT: r2c_stage_plane<L, PRO_SQUARE_SHIFT, false>(a, pi, src, mean, buf); break;
+        case PRO_SHIFT: r2c_stage_plane<L, PRO_SHIFT, false>(a, pi, src, mean, buf); break;
+        case PRO_MASK_SHIFT: r2c_stage_plane<L, PRO_MASK_SHIFT, true>(a, pi, src, mean, buf); break;
+        case PRO_RESLICED:
+            if (a.aux0) r2c_stage_plane<L, PRO_RESLICED, true>(a, pi, src, mean, buf);
+            else r2c_stage_plane<L, PRO_RESLICED, false>(a, pi, src, mean, buf);
+            break;
+        default:
+            for (int t = threadIdx.x; t < L * L; t += blockDim.x) {
+                const int j = t / L, x = t % L;
+                float* f = reinterpret_cast<float*>(&buf[(j >> 1) * LD + x]);
+                f[j & 1] = r2c_load<L>(a, ((long)pb * L + pi) * L + j, x);
+            }
     }
     __syncthreads();
     const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
@@ -473,10 +511,7 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_plane(const float2* _
     const long plane = blockIdx.x;
     for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
     const float2* ip = in + plane * L * H;
-    for (int t = threadIdx.x; t < L * H; t += blockDim.x) {
-        const int j = t / H, k = t % H;
-        buf[k * LD + j] = ip[(long)j * H + k];
-    }
+    block_copy<PL::THREADS, L * H>([&](int t) { return ip[t]; }, [&](int t, float2 v) { buf[(t % H) * LD + t / H] = v; });
     __syncthreads();
     const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
     const TwCache<L> twc = tw_cache<L>(tw, lane);
@@ -542,10 +577,7 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_r2c_plane(float2* __r
     const int pi = (int)(plane % L);
     for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
     float2* ip = io + plane * L * H;
-    for (int t = threadIdx.x; t < L * H; t += blockDim.x) {
-        const int j = t / H, k = t % H;
-        buf[k * LD + j] = ip[(long)j * H + k];
-    }
+    block_copy<T, L * H>([&](int t) { return ip[t]; }, [&](int t, float2 v) { buf[(t % H) * LD + t / H] = v; });
     __syncthreads();
     const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
     const TwCache<L> twc = tw_cache<L>(tw, lane);
@@ -583,21 +615,29 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_r2c_plane(float2* __r
     __syncthreads();
     // buf[p LD + x] = (r(row 2p, x), r(row 2p + 1, x)): the pointwise operation in place
     float sacc = 0.f;
-    for (int t = threadIdx.x; t < (L / 2) * L; t += T) {
-        const int pp = t / L, x = t % L, j0 = 2 * pp;
-        float2 z = buf[pp * LD + x];
-        if (op == PLANE_OP_MASK) {
-            const float m0 = aux[((size_t)pi * L + j0) * L + x], m1 = aux[((size_t)pi * L + j0 + 1) * L + x];
-            sacc += z.x * z.x * m0;
-            sacc += z.y * z.y * m1;
-            z.x *= m0;
-            z.y *= m1;
-        } else {
+    if (op == PLANE_OP_MASK) {
+        // mask values of rows j0, j0 + 1 at x: all loads in flight before the first use
+        const float* __restrict__ mp = aux + (size_t)pi * L * L;
+        block_copy<T, (L / 2) * L>(
+            [&](int t) { return make_float2(mp[(2 * (t / L)) * L + t % L], mp[(2 * (t / L) + 1) * L + t % L]); },
+            [&](int t, float2 m) {
+                const int pp = t / L, x = t % L;
+                float2 z = buf[pp * LD + x];
+                sacc += z.x * z.x * m.x;
+                sacc += z.y * z.y * m.y;
+                z.x *= m.x;
+                z.y *= m.y;
+                buf[pp * LD + x] = z;
+            });
+    } else {
+        for (int t = threadIdx.x; t < (L / 2) * L; t += T) {
+            const int pp = t / L, x = t % L, j0 = 2 * pp;
+            float2 z = buf[pp * LD + x];
             const float sg = ((pi + j0 + x) & 1) ? -1.0f : 1.0f;
             z.x = z.x * z.x * sg;
             z.y = z.y * z.y * (-sg);
+            buf[pp * LD + x] = z;
         }
-        buf[pp * LD + x] = z;
     }
     if (op == PLANE_OP_MASK) {
         // deterministic block sum: shuffles inside the full waves, the partial wave's values through LDS
