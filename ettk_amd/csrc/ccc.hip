@@ -423,15 +423,25 @@ __global__ void __launch_bounds__(kThreads) k_combine_dftA(const float2* __restr
     const float n3 = (float)N * N * N;
     const float a = 2.f * n3 / (n1 * n2);
     const float b = 1.f / (n2 * n2);
-    for (int t = threadIdx.x; t < N * H; t += blockDim.x) {
-        int i = t / H, k = t % H;
-        long h = ((long)i * N + j) * H + k;
-        float2 f1 = F1m[(size_t)e * nh + h], f2 = F2[(size_t)e * nh + h], q = Q[(size_t)e * nh + h], m = Mfft[h];
-        float2 c = csub(cscale(cmulc(f1, f2), a), cscale(cmulc(m, q), b));
-        if ((i + j + k) & 1) c = make_float2(-c.x, -c.y);
-        C[t] = c;
-    }
-    for (int t = threadIdx.x; t < E * N; t += blockDim.x) A[t] = tabA[t];
+    struct Quad {
+        float2 f1, f2, q, m;
+    };
+    const float2* __restrict__ f1p = F1m + (size_t)e * nh;
+    const float2* __restrict__ f2p = F2 + (size_t)e * nh;
+    const float2* __restrict__ qp = Q + (size_t)e * nh;
+    block_copy_rt<5>(
+        N * H,
+        [&](int t) {
+            const long h = ((long)(t / H) * N + j) * H + t % H;
+            return Quad{f1p[h], f2p[h], qp[h], Mfft[h]};
+        },
+        [&](int t, const Quad& v) {
+            const int i = t / H, k = t % H;
+            float2 c = csub(cscale(cmulc(v.f1, v.f2), a), cscale(cmulc(v.m, v.q), b));
+            if ((i + j + k) & 1) c = make_float2(-c.x, -c.y);
+            C[t] = c;
+        });
+    block_copy_rt<9>(E * N, [&](int t) { return tabA[t]; }, [&](int t, float2 v) { A[t] = v; });
     __syncthreads();
     for (int t = threadIdx.x; t < E * H; t += blockDim.x) {
         int x = t / H, k = t % H;
@@ -468,9 +478,9 @@ __global__ void __launch_bounds__(kThreads) k_dftBC_argmax(const float2* __restr
     __shared__ int si[kThreads];
     const int x = blockIdx.x, e = blockIdx.y;
     const float2* src = G1 + ((size_t)e * E + x) * N * H;
-    for (int t = threadIdx.x; t < N * H; t += blockDim.x) g1[t] = src[t];
-    for (int t = threadIdx.x; t < E * N; t += blockDim.x) A[t] = tabA[t];
-    for (int t = threadIdx.x; t < E * H; t += blockDim.x) K[t] = tabK[t];
+    block_copy_rt<9>(N * H, [&](int t) { return src[t]; }, [&](int t, float2 v) { g1[t] = v; });
+    block_copy_rt<9>(E * N, [&](int t) { return tabA[t]; }, [&](int t, float2 v) { A[t] = v; });
+    block_copy_rt<5>(E * H, [&](int t) { return tabK[t]; }, [&](int t, float2 v) { K[t] = v; });
     __syncthreads();
     for (int t = threadIdx.x; t < E * H; t += blockDim.x) {
         int y = t / H, k = t % H;

@@ -39,6 +39,44 @@ __host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(
 __host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 __host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
+// Workgroup-wide copies in which every global load of a round is issued before the first loaded value is used,
+// so a round costs one memory latency instead of one per element.  Thread t moves elements t, t + NT, ...;
+// out-of-range slots re-read the last element and are dropped.
+//   block_copy<NT, NE>: NE elements, NT threads, one round
+//   block_copy_rt<U>:   ne elements (runtime), blockDim.x threads, U elements per thread per round
+template <int NT, int NE, class Ld, class St>
+__device__ __forceinline__ void block_copy(Ld ld, St st) {
+    constexpr int NI = (NE + NT - 1) / NT;
+    decltype(ld(0)) v[NI];
+#pragma unroll
+    for (int u = 0; u < NI; u++) {
+        const int t = (int)threadIdx.x + u * NT;
+        v[u] = ld((NE % NT == 0 || t < NE) ? t : NE - 1);
+    }
+#pragma unroll
+    for (int u = 0; u < NI; u++) {
+        const int t = (int)threadIdx.x + u * NT;
+        if (NE % NT == 0 || t < NE) st(t, v[u]);
+    }
+}
+template <int U, class Ld, class St>
+__device__ __forceinline__ void block_copy_rt(int ne, Ld ld, St st) {
+    const int nt = (int)blockDim.x;
+    for (int t0 = (int)threadIdx.x; t0 < ne; t0 += U * nt) {
+        decltype(ld(0)) v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u * nt;
+            v[u] = ld(t < ne ? t : ne - 1);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u * nt;
+            if (t < ne) st(t, v[u]);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ device buffers
 template <class T>
 struct DevBuf {

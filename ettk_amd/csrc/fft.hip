@@ -50,6 +50,7 @@ template <int L> struct Shape {
     static constexpr int P = (1024 / L) > 0 ? 1024 / L : 1; // pencils per workgroup
     static constexpr int THREADS = P * TPP;
     static constexpr int LD = L + 1;                        // padded LDS row
+    static_assert(THREADS >= L, "one twiddle per thread");
 };
 
 __device__ __forceinline__ float2 mul_i(float2 z, bool inv) {  // forward: -i z, inverse: +i z
@@ -223,25 +224,6 @@ __device__ __forceinline__ void pencil_fft(float2* pen, int lane, const float2* 
     else lds_fft<L>(pen, lane, tw, inv);
 }
 
-// A workgroup-wide copy of NE elements by NT threads: thread t moves elements t, t + NT, ...  Every global load
-// of the thread is issued before the first value is stored (out-of-range slots re-read the last element and are
-// dropped), so one memory latency covers the whole copy instead of one per element.
-template <int NT, int NE, class Ld, class St>
-__device__ __forceinline__ void block_copy(Ld ld, St st) {
-    constexpr int NI = (NE + NT - 1) / NT;
-    decltype(ld(0)) v[NI];
-#pragma unroll
-    for (int u = 0; u < NI; u++) {
-        const int t = (int)threadIdx.x + u * NT;
-        v[u] = ld((NE % NT == 0 || t < NE) ? t : NE - 1);
-    }
-#pragma unroll
-    for (int u = 0; u < NI; u++) {
-        const int t = (int)threadIdx.x + u * NT;
-        if (NE % NT == 0 || t < NE) st(t, v[u]);
-    }
-}
-
 // ------------------------------------------------------------------ r2c, pass along k (real rows)
 template <int L>
 __device__ __forceinline__ float r2c_load(const R2CArgs& a, long row, int k) {
@@ -286,7 +268,7 @@ __global__ void __launch_bounds__(Shape<L>::THREADS) k_r2c_rows(R2CArgs a, const
     __shared__ float2 tw[L];
     const long npairs = (long)a.batch * L * L / 2;
     const long p0 = (long)blockIdx.x * S::P;
-    for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
+    const float2 twv = twg[threadIdx.x % L];  // stored after the staging loads are in flight
     // load P pairs of rows (2L contiguous floats each) and pack row 2q + i*row 2q+1
     for (int t = threadIdx.x; t < S::P * 2 * L; t += blockDim.x) {
         int q = t / (2 * L), r = t % (2 * L);
@@ -299,6 +281,7 @@ __global__ void __launch_bounds__(Shape<L>::THREADS) k_r2c_rows(R2CArgs a, const
         float* f = reinterpret_cast<float*>(&lds[q * S::LD + (r % L)]);
         f[r / L] = x;
     }
+    if (threadIdx.x < L) tw[threadIdx.x] = twv;
     __syncthreads();
     const int q = threadIdx.x / S::TPP, lane = threadIdx.x % S::TPP;
     lds_fft<L>(&lds[q * S::LD], lane, tw, false);
@@ -338,7 +321,7 @@ __global__ void __launch_bounds__(StridedShape<L>::THREADS) k_fft_strided(Stride
     __shared__ float2 lds[S::P * S::LD];
     __shared__ float2 tw[L];
     const long q0 = (long)blockIdx.x * S::P;
-    for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
+    const float2 twv = twg[threadIdx.x % L];  // stored after the staging loads are in flight
     // THREADS is a multiple of P: a thread moves the elements l = t / P, t / P + THREADS / P, ... of the one
     // pencil t % P, so the pencil's base address is computed once
     const int qm = threadIdx.x % S::P;
@@ -355,6 +338,7 @@ __global__ void __launch_bounds__(StridedShape<L>::THREADS) k_fft_strided(Stride
 #pragma unroll
         for (int u = 0; u < NL; u++) lds[qm * S::LD + l0 + u * STEP] = valid ? v[u] : make_float2(0.f, 0.f);
     }
+    if (threadIdx.x < L) tw[threadIdx.x] = twv;
     __syncthreads();
     const int q = threadIdx.x / S::TPP, lane = threadIdx.x % S::TPP;
     pencil_fft<L>(&lds[q * S::LD], lane, tw, tw_cache<L>(tw, lane), a.inv != 0);
@@ -372,7 +356,7 @@ __global__ void __launch_bounds__(Shape<L>::THREADS) k_c2r_rows(const float2* __
     __shared__ float2 lds[S::P * S::LD];
     __shared__ float2 tw[L];
     const long p0 = (long)blockIdx.x * S::P;
-    for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
+    const float2 twv = twg[threadIdx.x % L];  // stored after the staging loads are in flight
     // Z(k) = A(k) + i B(k) over the full length, with A(L-k) = conj(A(k)) and real A(0), A(L/2)
     for (int t = threadIdx.x; t < S::P * L; t += blockDim.x) {
         int q = t / L, k = t % L;
@@ -394,6 +378,7 @@ __global__ void __launch_bounds__(Shape<L>::THREADS) k_c2r_rows(const float2* __
         }
         lds[q * S::LD + k] = z;
     }
+    if (threadIdx.x < L) tw[threadIdx.x] = twv;
     __syncthreads();
     const int q = threadIdx.x / S::TPP, lane = threadIdx.x % S::TPP;
     lds_fft<L>(&lds[q * S::LD], lane, tw, true);
@@ -415,6 +400,7 @@ template <int L> struct Plane {
     static constexpr int TPP = PenT<L>::TPP;
     static constexpr int THREADS = H * TPP;
     static constexpr int LD = PenT<L>::LD;
+    static_assert(THREADS >= L, "one twiddle per thread");
 };
 
 // the k-pass input of plane i for one preprocessing mode, staged as packed row pairs buf[(j/2) LD + x]
@@ -450,7 +436,7 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_r2c_plane(R2CArgs a, cons
     const int pb = (int)(plane / L), pi = (int)(plane % L);
     const float* src = a.in + plane * L * L;
     const float mean = a.pro == PRO_RESLICED ? a.stats[pb] : 0.0f;
-    for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
+    const float2 twv = twg[threadIdx.x % L];  // stored after the staging loads are in flight
     switch (a.pro) {
         case PRO_NONE: r2c_stage_plane<L, PRO_NONE, false>(a, pi, src, mean, buf); break;
         case PRO_SQUARE_SHIFT: r2c_stage_plane<L, PRO_SQUARE_SHIFT, false>(a, pi, src, mean, buf); break;
@@ -467,6 +453,7 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_r2c_plane(R2CArgs a, cons
                 f[j & 1] = r2c_load<L>(a, ((long)pb * L + pi) * L + j, x);
             }
     }
+    if (threadIdx.x < L) tw[threadIdx.x] = twv;
     __syncthreads();
     const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
     const TwCache<L> twc = tw_cache<L>(tw, lane);
@@ -509,9 +496,10 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_plane(const float2* _
     __shared__ float2 buf[H * LD];
     __shared__ float2 tw[L];
     const long plane = blockIdx.x;
-    for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
+    const float2 twv = twg[threadIdx.x % L];  // stored after the staging loads are in flight
     const float2* ip = in + plane * L * H;
     block_copy<PL::THREADS, L * H>([&](int t) { return ip[t]; }, [&](int t, float2 v) { buf[(t % H) * LD + t / H] = v; });
+    if (threadIdx.x < L) tw[threadIdx.x] = twv;
     __syncthreads();
     const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
     const TwCache<L> twc = tw_cache<L>(tw, lane);
@@ -575,9 +563,10 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_r2c_plane(float2* __r
     __shared__ float sred[NFW + REM + 1];
     const long plane = blockIdx.x;
     const int pi = (int)(plane % L);
-    for (int t = threadIdx.x; t < L; t += blockDim.x) tw[t] = twg[t];
+    const float2 twv = twg[threadIdx.x % L];  // stored after the staging loads are in flight
     float2* ip = io + plane * L * H;
     block_copy<T, L * H>([&](int t) { return ip[t]; }, [&](int t, float2 v) { buf[(t % H) * LD + t / H] = v; });
+    if (threadIdx.x < L) tw[threadIdx.x] = twv;
     __syncthreads();
     const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
     const TwCache<L> twc = tw_cache<L>(tw, lane);
