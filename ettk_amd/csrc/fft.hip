@@ -6,7 +6,8 @@
 // Stockham stages there and writes them back, so every pass reads and writes each element of HBM once.
 //   r2c: pass k (pairs of real rows packed as one complex row, split after the transform), pass j, pass i
 //   c2r: pass i, pass j (complex, backward), then pass k (pairs of half rows -> two real rows)
-#include "common.hpp"
+#include "kernels.hpp"
+#include "wedge.hpp"
 
 #include <cmath>
 
@@ -345,6 +346,132 @@ __global__ void __launch_bounds__(StridedShape<L>::THREADS) k_fft_strided(Stride
     if (valid)
 #pragma unroll
         for (int u = 0; u < NL; u++) a.out[base + (long)(l0 + u * STEP) * a.es] = lds[qm * S::LD + l0 + u * STEP];
+}
+
+// ------------------------------------------------------------------ CCC spectra: one pass along i (L <= 64)
+// deterministic sum over the NT threads of a workgroup (NT need not be a multiple of 64), valid in thread 0
+template <int NT>
+__device__ __forceinline__ float block_sum_nt(float v, float* sred) {
+    constexpr int NFW = NT / 64, REM = NT % 64;
+    const int w = threadIdx.x >> 6;
+    if (w < NFW) {
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if ((threadIdx.x & 63) == 0) sred[w] = v;
+    } else {
+        sred[NFW + (threadIdx.x & 63)] = v;
+    }
+    __syncthreads();
+    float tot = 0.f;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NFW + REM; i++) tot += sred[i];
+    __syncthreads();
+    return tot;
+}
+
+// Between the k/j plane passes of the particle's r2c and the fused plane kernels, every step of the CCC runs on
+// pencils along i, so one kernel does them all per pencil (j, k) of an evaluation:
+//   F2 = forward i-transform of the plane-pass output (completes the particle's r2c)
+//   fw = filter * W1 * W2(e) (the smoothed particle wedge at the evaluation's transform, computed once)
+//   X2 = mcf(F2) * fw, DC term zeroed -> S2 (the combine's F2), and its inverse i-transform -> S3
+//   X1 = F1mcf * fw, DC term zeroed -> inverse i-transform -> S1
+//   per-block partials of sum hw |X1|^2, sum fw, sum hw |X2|^2 (hw = 2 off the last half plane)
+// The element-wise arithmetic is that of k_spec_filter2 (nbfFourierFilter::executeHalf, FFilt.h:587-606, and
+// normalizeFourierHalf, CorrMetric.h:498-530); the transforms are those of the separate i passes.
+template <int L>
+__global__ void __launch_bounds__(StridedShape<L>::THREADS) k_ccc_ipass(CccIpassArgs a, const float2* __restrict__ twg) {
+    using S = StridedShape<L>;
+    constexpr int H = L / 2 + 1, LH = L * H, STEP = S::THREADS / S::P, NL = L / STEP;
+    static_assert(L % STEP == 0, "pencil elements per thread");
+    __shared__ float2 lds[S::P * S::LD];
+    __shared__ float2 tw[L];
+    __shared__ float sred[S::THREADS / 64 + S::THREADS % 64 + 1];
+    const int e = blockIdx.y;
+    const float2 twv = twg[threadIdx.x % L];
+    const int qm = threadIdx.x % S::P, l0 = threadIdx.x / S::P;
+    const int p = blockIdx.x * S::P + qm;  // pencil (j, k) = (p / H, p % H) of evaluation e
+    const bool valid = p < LH;
+    const int pc = valid ? p : 0, pj = pc / H, pk = pc % H;
+    const size_t vol = (size_t)e * L * LH;
+    float2* __restrict__ pen = &lds[qm * S::LD];
+    {
+        float2 v[NL];
+#pragma unroll
+        for (int u = 0; u < NL; u++) v[u] = a.S2[vol + (size_t)(l0 + u * STEP) * LH + pc];
+#pragma unroll
+        for (int u = 0; u < NL; u++) pen[l0 + u * STEP] = valid ? v[u] : make_float2(0.f, 0.f);
+    }
+    if (threadIdx.x < L) tw[threadIdx.x] = twv;
+    __syncthreads();
+    const int q = threadIdx.x / S::TPP, lane = threadIdx.x % S::TPP;
+    const TwCache<L> twc = tw_cache<L>(tw, lane);
+    pencil_fft<L>(&lds[q * S::LD], lane, tw, twc, false);
+    // filter weights of this thread's elements (i = l0 + u STEP of pencil p), the particle side in place
+    const EvalGeom& g = a.geom[e];
+    const int dc_i = L / 2, dc_j = L / 2, dc_k = H - 1;
+    const float hw = pk < H - 1 ? 2.f : 1.f;
+    float fw[NL], flt[NL], w1[NL];
+#pragma unroll
+    for (int u = 0; u < NL; u++) {
+        const size_t h = (size_t)(l0 + u * STEP) * LH + pc;
+        flt[u] = a.filter[h];
+        w1[u] = a.W1 ? a.W1[h] : 1.f;
+    }
+    float s2 = 0.f, swf = 0.f;
+#pragma unroll
+    for (int u = 0; u < NL; u++) {
+        const int i = l0 + u * STEP;
+        float w = w1[u];
+        if (!g.full) w *= wedge_smooth(g.v1, g.v2, i, pj, pk, L, H);
+        fw[u] = flt[u] * w;
+        const bool dc = i == dc_i && pj == dc_j && pk == dc_k;
+        float2 z = pen[i];
+        if (a.mcf) {
+            const float m2 = z.x * z.x + z.y * z.y;
+            const float C = sqrtf(sqrtf(m2));
+            if (C > 0.f) z = make_float2(z.x / C, z.y / C);
+        }
+        if (a.apply) z = cscale(z, fw[u]);
+        if (dc) z = make_float2(0.f, 0.f);
+        if (valid) {
+            a.S2[vol + (size_t)i * LH + pc] = z;
+            s2 += hw * (z.x * z.x + z.y * z.y);
+            swf += fw[u];
+        }
+        pen[i] = z;
+    }
+    __syncthreads();
+    pencil_fft<L>(&lds[q * S::LD], lane, tw, twc, true);
+    // reference side: its spectrum loads are issued before the particle side's inverse transform is stored
+    float2 f1[NL];
+#pragma unroll
+    for (int u = 0; u < NL; u++) f1[u] = a.F1mcf[(size_t)(l0 + u * STEP) * LH + pc];
+    if (valid)
+#pragma unroll
+        for (int u = 0; u < NL; u++) a.S3[vol + (size_t)(l0 + u * STEP) * LH + pc] = pen[l0 + u * STEP];
+    float s1 = 0.f;
+#pragma unroll
+    for (int u = 0; u < NL; u++) {
+        const int i = l0 + u * STEP;
+        float2 z = f1[u];
+        if (a.apply) z = cscale(z, fw[u]);
+        if (i == dc_i && pj == dc_j && pk == dc_k) z = make_float2(0.f, 0.f);
+        if (valid) s1 += hw * (z.x * z.x + z.y * z.y);
+        pen[i] = z;
+    }
+    __syncthreads();
+    pencil_fft<L>(&lds[q * S::LD], lane, tw, twc, true);
+    if (valid)
+#pragma unroll
+        for (int u = 0; u < NL; u++) a.S1[vol + (size_t)(l0 + u * STEP) * LH + pc] = pen[l0 + u * STEP];
+    const float t1 = block_sum_nt<S::THREADS>(s1, sred);
+    const float tf = block_sum_nt<S::THREADS>(swf, sred);
+    const float t2 = block_sum_nt<S::THREADS>(s2, sred);
+    if (threadIdx.x == 0) {
+        const size_t o = (size_t)e * gridDim.x + blockIdx.x;
+        a.p_n1[o] = t1;
+        a.p_wf[o] = tf;
+        a.p_n2[o] = t2;
+    }
 }
 
 // ------------------------------------------------------------------ c2r, pass along k (half rows)
@@ -757,7 +884,7 @@ static void launch_c2r_op_r2c(const FftTables& T, const float2* in, float2* io, 
     const float2* tw = T.tw.p + T.offset[fft_len_index(L)];
     const double vol_c = (double)batch * L * L * H * sizeof(float2);
     StridedArgs si{in, io, (long)batch * L * H, (long)L * H, (long)L * L * H, (long)L * H, 1};
-    {
+    if (in) {
         ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
         k_fft_strided<L><<<(unsigned)((si.npencils + StridedShape<L>::P - 1) / StridedShape<L>::P), StridedShape<L>::THREADS, 0, s>>>(si, tw);
     }
@@ -775,8 +902,75 @@ static void launch_c2r_op_r2c(const FftTables& T, const float2* in, float2* io, 
 
 bool fft3_fused_supported(int n) { return n == 16 || n == 32 || n == 48 || n == 64; }
 
+template <int L>
+static int ccc_ipass_blocks_t() {
+    return (L * (L / 2 + 1) + StridedShape<L>::P - 1) / StridedShape<L>::P;
+}
+int fft3_ccc_ipass_blocks(int n) {
+    switch (n) {
+        case 16: return ccc_ipass_blocks_t<16>();
+        case 32: return ccc_ipass_blocks_t<32>();
+        case 48: return ccc_ipass_blocks_t<48>();
+        case 64: return ccc_ipass_blocks_t<64>();
+        default: throw Error{ETTK_ERR_UNSUPPORTED, "fft3_ccc_ipass: unsupported size " + std::to_string(n)};
+    }
+}
+
+template <int L>
+static void launch_ccc_ipass(const FftTables& T, const CccIpassArgs& a, int batch, hipStream_t s) {
+    constexpr int H = L / 2 + 1;
+    const float2* tw = T.tw.p + T.offset[fft_len_index(L)];
+    const double vol_c = (double)batch * L * L * H * sizeof(float2);
+    static const std::string name = "k_ccc_ipass<" + std::to_string(L) + ">";
+    ProfScope ps(T.prof, name.c_str(), 4 * vol_c, s);  // S2 read and written, S3 and S1 written
+    k_ccc_ipass<L><<<dim3((unsigned)ccc_ipass_blocks_t<L>(), (unsigned)batch), StridedShape<L>::THREADS, 0, s>>>(a, tw);
+}
+void fft3_ccc_ipass(const FftTables& T, const CccIpassArgs& a, int n, int batch, hipStream_t s) {
+    switch (n) {
+        case 16: launch_ccc_ipass<16>(T, a, batch, s); break;
+        case 32: launch_ccc_ipass<32>(T, a, batch, s); break;
+        case 48: launch_ccc_ipass<48>(T, a, batch, s); break;
+        case 64: launch_ccc_ipass<64>(T, a, batch, s); break;
+        default: throw Error{ETTK_ERR_UNSUPPORTED, "fft3_ccc_ipass: unsupported size " + std::to_string(n)};
+    }
+    ETTK_HIP(hipGetLastError());
+}
+
+template <int L>
+static void launch_r2c_planes(const FftTables& T, const R2CArgs& a, hipStream_t s) {
+    constexpr int H = L / 2 + 1;
+    const float2* tw = T.tw.p + T.offset[fft_len_index(L)];
+    const double vol_c = (double)a.batch * L * L * H * sizeof(float2), vol_r = (double)a.batch * L * L * L * sizeof(float);
+    ProfScope ps(T.prof, plane_name<L>(0), vol_r + vol_c, s);
+    k_r2c_plane<L><<<(unsigned)((long)a.batch * L), Plane<L>::THREADS, 0, s>>>(a, tw);
+}
+void fft3_r2c_planes(const FftTables& T, const R2CArgs& a, hipStream_t s) {
+    ETTK_REQUIRE(a.pro != PRO_PAD_MASKED, ETTK_ERR_ARG, "fft3_r2c_planes: padded prologue not supported");
+    switch (a.n) {
+        case 16: launch_r2c_planes<16>(T, a, s); break;
+        case 32: launch_r2c_planes<32>(T, a, s); break;
+        case 48: launch_r2c_planes<48>(T, a, s); break;
+        case 64: launch_r2c_planes<64>(T, a, s); break;
+        default: throw Error{ETTK_ERR_UNSUPPORTED, "fft3_r2c_planes: unsupported size " + std::to_string(a.n)};
+    }
+    ETTK_HIP(hipGetLastError());
+}
+
+void fft3_op_r2c_after_ipass(const FftTables& T, float2* io, int n, int batch, int op, const float* aux, float* partials,
+                             hipStream_t s) {
+    switch (n) {
+        case 16: launch_c2r_op_r2c<16>(T, nullptr, io, batch, op, aux, partials, s); break;
+        case 32: launch_c2r_op_r2c<32>(T, nullptr, io, batch, op, aux, partials, s); break;
+        case 48: launch_c2r_op_r2c<48>(T, nullptr, io, batch, op, aux, partials, s); break;
+        case 64: launch_c2r_op_r2c<64>(T, nullptr, io, batch, op, aux, partials, s); break;
+        default: throw Error{ETTK_ERR_UNSUPPORTED, "fft3_op_r2c_after_ipass: unsupported size " + std::to_string(n)};
+    }
+    ETTK_HIP(hipGetLastError());
+}
+
 void fft3_c2r_op_r2c(const FftTables& T, const float2* in, float2* io, int n, int batch, int op, const float* aux,
                      float* partials, hipStream_t s) {
+    ETTK_REQUIRE(in != nullptr, ETTK_ERR_ARG, "fft3_c2r_op_r2c: null input");
     switch (n) {
         case 16: launch_c2r_op_r2c<16>(T, in, io, batch, op, aux, partials, s); break;
         case 32: launch_c2r_op_r2c<32>(T, in, io, batch, op, aux, partials, s); break;

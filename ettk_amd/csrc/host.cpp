@@ -255,9 +255,11 @@ void ensure_ccc_workspace(ettk_plan* P, int cap) {
     P->S3.ensure(nh * cap);
     P->G1.ensure((size_t)P->z.E * N * H * cap);
     P->p_res.ensure((size_t)2 * P->nb_res * cap);
-    P->p_n1.ensure((size_t)P->nb_spec * cap);
-    P->p_n2.ensure((size_t)P->nb_spec * cap);
-    P->p_wf.ensure((size_t)P->nb_spec * cap);
+    // spectral partials: per k_spec_filter2 block, or per k_ccc_ipass block (fused transforms)
+    const size_t nb_sp = (size_t)std::max(P->nb_spec, fft3_fused_supported(N) ? fft3_ccc_ipass_blocks(N) : 0);
+    P->p_n1.ensure(nb_sp * cap);
+    P->p_n2.ensure(nb_sp * cap);
+    P->p_wf.ensure(nb_sp * cap);
     P->p_k1.ensure((size_t)std::max(P->nb_real, N) * cap);  // mask partials: per block, or per plane (fused)
     P->stats.ensure(cap);
     P->scalars.ensure((size_t)4 * cap);
@@ -388,20 +390,29 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
         }
         launch_reslice_mean(P->p_res.p, P->nb_res, E, P->stats.p, s);
         R2CArgs a2{P->R2.p, P->S2.p, N, E, PRO_RESLICED, P->maskfft.p, P->stats.p};
-        fft3_r2c(P->ctx->fft, a2, nullptr, s);
-        // both spectra weighted by filter x W1 W2 in one pass: X1 = F1mcf (reference, MCF applied once), X2 = mcf(F2)
-        {
-            ProfScope ps(kp, "k_spec_filter", 4 * Hb, s);
-            launch_spec_filter2(P->F1mcf.p, P->S1.p, P->S2.p, P->filter.p, W1, geo, E, N, P->p.mcf, apply, P->p_n1.p,
-                                P->p_wf.p, P->p_n2.p, P->nb_spec, s);
-        }
         const bool fused = fft3_fused_supported(N);
+        int nb_sp = P->nb_spec;
         if (fused) {
+            // F2 = r2c((reslice - mean) maskfft shift) up to its i pass; then on pencils along i: the i pass, both
+            // spectra weighted by filter x W1 W2 (X1 = F1mcf, the reference with MCF applied once; X2 = mcf(F2)),
+            // X2 kept for the combine, and both inverse i passes of the transforms below
+            fft3_r2c_planes(P->ctx->fft, a2, s);
+            CccIpassArgs ia{P->S2.p, P->S3.p, P->S1.p, P->F1mcf.p, P->filter.p, W1, geo, P->p.mcf, apply,
+                            P->p_n1.p, P->p_wf.p, P->p_n2.p};
+            fft3_ccc_ipass(P->ctx->fft, ia, N, E, s);
+            nb_sp = fft3_ccc_ipass_blocks(N);
             // reference side: c2r -> r1 m1 (with the k1_m partials) -> r2c = F1m, one partial per (evaluation, plane)
-            fft3_c2r_op_r2c(P->ctx->fft, P->S1.p, P->S1.p, N, E, PLANE_OP_MASK, P->m1.p, P->p_k1.p, s);
-            // particle side: Q = r2c(c2r(X2)^2 shift); X2 (S2) is kept for the combine
-            fft3_c2r_op_r2c(P->ctx->fft, P->S2.p, P->S3.p, N, E, PLANE_OP_SQUARE_SHIFT, nullptr, nullptr, s);
+            fft3_op_r2c_after_ipass(P->ctx->fft, P->S1.p, N, E, PLANE_OP_MASK, P->m1.p, P->p_k1.p, s);
+            // particle side: Q = r2c(c2r(X2)^2 shift)
+            fft3_op_r2c_after_ipass(P->ctx->fft, P->S3.p, N, E, PLANE_OP_SQUARE_SHIFT, nullptr, nullptr, s);
         } else {
+            fft3_r2c(P->ctx->fft, a2, nullptr, s);
+            // both spectra weighted by filter x W1 W2 in one pass: X1 = F1mcf (reference, MCF applied once), X2 = mcf(F2)
+            {
+                ProfScope ps(kp, "k_spec_filter", 4 * Hb, s);
+                launch_spec_filter2(P->F1mcf.p, P->S1.p, P->S2.p, P->filter.p, W1, geo, E, N, P->p.mcf, apply, P->p_n1.p,
+                                    P->p_wf.p, P->p_n2.p, P->nb_spec, s);
+            }
             fft3_c2r(P->ctx->fft, P->S1.p, P->S1.p, P->R1.p, N, E, s);
             {
                 ProfScope ps(kp, "k_mask_ref", 2 * Rb, s);
@@ -414,7 +425,7 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
             fft3_r2c(P->ctx->fft, a3, nullptr, s);
         }
         // combine, upsampled correlation on the search window, peak
-        launch_ccc_scalars(P->p_n1.p, P->p_n2.p, P->p_k1.p, P->p_wf.p, P->nb_spec, fused ? N : P->nb_real, E, N,
+        launch_ccc_scalars(P->p_n1.p, P->p_n2.p, P->p_k1.p, P->p_wf.p, nb_sp, fused ? N : P->nb_real, E, N,
                            P->scalars.p, s);
         {
             ProfScope ps(kp, "k_combine_dftA", 3 * Hb + Gb, s);

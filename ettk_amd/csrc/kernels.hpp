@@ -44,6 +44,31 @@ struct EvalGeom {
     float cut;        // cut offset of the sub-volume along z (nbfWedgedSubImage3D::setCutOffset)
 };
 
+// The CCC steps between the particle's r2c plane passes and the fused plane kernels, on pencils along i
+// (n <= 64, fft3_fused_supported): S2 holds the plane-pass output of the resliced particles and receives the
+// filtered particle spectra X2; S3 receives X2 and S1 the filtered reference spectrum X1, both inverse-transformed
+// along i (the inputs of fft3_op_r2c_after_ipass).  Partials: fft3_ccc_ipass_blocks(n) per evaluation.
+struct CccIpassArgs {
+    float2* S2;
+    float2* S3;
+    float2* S1;
+    const float2* F1mcf;   // reference spectrum, MCF applied (one volume)
+    const float* filter;   // half grid
+    const float* W1;       // reference wedge on the half grid, or null (no missing wedge)
+    const EvalGeom* geom;  // per evaluation: the particle wedge at its transform
+    int mcf, apply;
+    float* p_n1;
+    float* p_wf;
+    float* p_n2;
+};
+int fft3_ccc_ipass_blocks(int n);
+void fft3_ccc_ipass(const FftTables& T, const CccIpassArgs& a, int n, int batch, hipStream_t s);
+// the k/j plane passes of an r2c only (n <= 64): the i pass is left to fft3_ccc_ipass
+void fft3_r2c_planes(const FftTables& T, const R2CArgs& a, hipStream_t s);
+// fft3_c2r_op_r2c whose inverse i pass has already run (io holds the i-inverse-transformed spectra)
+void fft3_op_r2c_after_ipass(const FftTables& T, float2* io, int n, int batch, int op, const float* aux, float* partials,
+                             hipStream_t s);
+
 // smoothed analytic wedge parameters of the reference
 struct WedgeRef {
     float v1[3], v2[3];
