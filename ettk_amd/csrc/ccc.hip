@@ -407,6 +407,15 @@ void launch_ccc_scalars(const float* p_n1, const float* p_n2, const float* p_k1,
     ETTK_HIP(hipGetLastError());
 }
 
+// acc + a b as four chained FMAs (the partial DFTs' inner product)
+__device__ __forceinline__ float2 cmac(float2 acc, float2 a, float2 b) {
+    acc.x = fmaf(a.x, b.x, acc.x);
+    acc.x = fmaf(-a.y, b.y, acc.x);
+    acc.y = fmaf(a.x, b.y, acc.y);
+    acc.y = fmaf(a.y, b.x, acc.y);
+    return acc;
+}
+
 // ------------------------------------------------------------------ combine + first partial DFT (along i)
 __global__ void __launch_bounds__(kThreads) k_combine_dftA(const float2* __restrict__ F1m, const float2* __restrict__ F2,
                                                             const float2* __restrict__ Q, const float2* __restrict__ Mfft,
@@ -446,7 +455,7 @@ __global__ void __launch_bounds__(kThreads) k_combine_dftA(const float2* __restr
     for (int t = threadIdx.x; t < E * H; t += blockDim.x) {
         int x = t / H, k = t % H;
         float2 acc = make_float2(0.f, 0.f);
-        for (int i = 0; i < N; i++) acc = cadd(acc, cmul(A[x * N + i], C[i * H + k]));
+        for (int i = 0; i < N; i++) acc = cmac(acc, A[x * N + i], C[i * H + k]);
         G1[(((size_t)e * E + x) * N + j) * H + k] = acc;
     }
 }
@@ -485,7 +494,7 @@ __global__ void __launch_bounds__(kThreads) k_dftBC_argmax(const float2* __restr
     for (int t = threadIdx.x; t < E * H; t += blockDim.x) {
         int y = t / H, k = t % H;
         float2 acc = make_float2(0.f, 0.f);
-        for (int j = 0; j < N; j++) acc = cadd(acc, cmul(A[y * N + j], g1[j * H + k]));
+        for (int j = 0; j < N; j++) acc = cmac(acc, A[y * N + j], g1[j * H + k]);
         g2[t] = acc;
     }
     __syncthreads();
@@ -500,8 +509,9 @@ __global__ void __launch_bounds__(kThreads) k_dftBC_argmax(const float2* __restr
         if (!(rad < z.restr2)) continue;
         float acc = 0.f;
         for (int k = 0; k < H; k++) {
-            float2 p = g2[y * H + k], q = K[zz * H + k];
-            acc += p.x * q.x - p.y * q.y;
+            const float2 p = g2[y * H + k], q = K[zz * H + k];
+            acc = fmaf(p.x, q.x, acc);
+            acc = fmaf(-p.y, q.y, acc);
         }
         acc *= scale;
         if (acc > bv || (acc == bv && t < bi)) { bv = acc; bi = t; }
