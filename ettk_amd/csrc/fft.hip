@@ -187,7 +187,17 @@ __device__ __forceinline__ TwCache<L> tw_cache(const float2* tw, int lane) {
     return c;
 }
 
-// every thread of the workgroup must call it (it synchronises).  The 8 x 8 exchange goes through rows of 9
+// Orders the LDS accesses of one wavefront: the LDS instructions of a wavefront execute in order, so threads of
+// one wavefront exchanging data through LDS only need the compiler kept from reordering those accesses.
+__device__ __forceinline__ void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// every thread of the workgroup must call it (it ends with a workgroup barrier, after which the whole pencil is
+// visible to every thread).  The 8 threads of a pencil (lanes 8q .. 8q+7 of the workgroup) share a wavefront,
+// so the exchange inside needs only wavefront ordering.  The 8 x 8 exchange goes through rows of 9
 // (pen[k * 9 + t] = element k of thread t), so neither its writes nor its reads collide in LDS banks; the
 // pencil's storage therefore spans 72 entries.
 __device__ __forceinline__ void fft64_reg(float2* pen, int lane, const TwCache<64>& tc, bool inv) {
@@ -201,14 +211,14 @@ __device__ __forceinline__ void fft64_reg(float2* pen, int lane, const TwCache<6
         if (inv) w.y = -w.y;
         x[k] = cmul(x[k], w);
     }
-    __syncthreads();
+    wave_lds_order();
 #pragma unroll
     for (int k = 0; k < 8; k++) pen[k * 9 + lane] = x[k];
-    __syncthreads();
+    wave_lds_order();
 #pragma unroll
     for (int t = 0; t < 8; t++) x[t] = pen[lane * 9 + t];
     dft8(x, inv);
-    __syncthreads();
+    wave_lds_order();
 #pragma unroll
     for (int k = 0; k < 8; k++) pen[lane + 8 * k] = x[k];
     __syncthreads();
