@@ -137,39 +137,74 @@ static void so3_kernel_attributes(int B);
 
 // ============================================================== projection kernels
 // statistics of the masked image: st[0] = valid mean (from reslice), st[1] = mean of the image, st[2] = pad
-// constant (value of voxel (0,0,0) after masking), st[3] = 1 when the image mask is on
-__global__ void k_proj_stats(const float* __restrict__ R, const float* __restrict__ res_partials, int nb_res,
-                             const float* __restrict__ mask, int N, int mask_on, int is_ref, float* __restrict__ st) {
-    __shared__ double sh[kT];
-    const int b = blockIdx.x;
+// constant (value of voxel (0,0,0) after masking), st[3] = 1 when the image mask is on.  Two kernels: kProjParts
+// workgroups per image sum (x - valid mean) over contiguous parts in FP64, one wave per image adds the parts in
+// a fixed order.
+static constexpr int kProjParts = 64;
+__device__ __forceinline__ double wave_dsum64(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+// mean of the valid voxels from the reslice partials (sum, count per tile), the same for every thread
+__device__ __forceinline__ float proj_valid_mean(const float* __restrict__ res_partials, int b, int nb_res, double* sh) {
+    double s = 0, c = 0;
+    for (int q = threadIdx.x; q < nb_res; q += kT) {
+        s += res_partials[((size_t)b * nb_res + q) * 2];
+        c += res_partials[((size_t)b * nb_res + q) * 2 + 1];
+    }
+    s = wave_dsum64(s);
+    c = wave_dsum64(c);
+    if ((threadIdx.x & 63) == 0) {
+        sh[threadIdx.x >> 6] = s;
+        sh[kT / 64 + (threadIdx.x >> 6)] = c;
+    }
+    __syncthreads();
+    double ts = 0, tc = 0;
+    for (int w = 0; w < kT / 64; w++) {
+        ts += sh[w];
+        tc += sh[kT / 64 + w];
+    }
+    __syncthreads();
+    return (float)(ts / (double)(float)tc);
+}
+__global__ void __launch_bounds__(kT) k_proj_partial(const float* __restrict__ R, const float* __restrict__ res_partials,
+                                                     int nb_res, int N, int is_ref, double* __restrict__ part,
+                                                     float* __restrict__ st) {
+    __shared__ double sh[2 * (kT / 64)];
+    const int q = blockIdx.x, b = blockIdx.y;
     const long n3 = (long)N * N * N;
     const float* r = R + (size_t)b * n3;
-    float mv = 0.f;
-    if (!is_ref) {
-        double s = 0, c = 0;
-        for (int q = 0; q < nb_res; q++) {
-            s += res_partials[((size_t)b * nb_res + q) * 2];
-            c += res_partials[((size_t)b * nb_res + q) * 2 + 1];
-        }
-        mv = (float)(s / (double)(float)c);
-    }
+    const float mv = is_ref ? 0.f : proj_valid_mean(res_partials, b, nb_res, sh);
+    const long chunk = (n3 + kProjParts - 1) / kProjParts, o0 = q * chunk, o1 = min(n3, o0 + chunk);
     double acc = 0;
-    for (long o = threadIdx.x; o < n3; o += blockDim.x) {
-        float x = r[o];
+    for (long o = o0 + threadIdx.x; o < o1; o += kT) {
+        const float x = r[o];
         acc += (x == -FLT_MAX) ? 0.0 : (double)(x - mv);
     }
-    sh[threadIdx.x] = acc;
+    acc = wave_dsum64(acc);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
     __syncthreads();
-    for (int st2 = blockDim.x / 2; st2 > 0; st2 >>= 1) {
-        if (threadIdx.x < st2) sh[threadIdx.x] += sh[threadIdx.x + st2];
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        double mean = sh[0] / (double)n3;
-        float x0 = r[0];
-        double img0 = (x0 == -FLT_MAX) ? 0.0 : (double)(x0 - mv);
-        double c = mask_on ? (img0 - mean) * mask[0] + mean : img0;
-        st[b * 4 + 0] = mv;
+        double t = 0;
+        for (int w = 0; w < kT / 64; w++) t += sh[w];
+        part[(size_t)b * kProjParts + q] = t;
+        if (q == 0) st[b * 4 + 0] = mv;
+    }
+}
+__global__ void __launch_bounds__(64) k_proj_final(const float* __restrict__ R, const double* __restrict__ part,
+                                                   const float* __restrict__ mask, int N, int mask_on,
+                                                   float* __restrict__ st) {
+    const int b = blockIdx.x;
+    const long n3 = (long)N * N * N;
+    double acc = 0;
+    for (int q = threadIdx.x; q < kProjParts; q += 64) acc += part[(size_t)b * kProjParts + q];
+    acc = wave_dsum64(acc);
+    if (threadIdx.x == 0) {
+        const float mv = st[b * 4 + 0];
+        const double mean = acc / (double)n3;
+        const float x0 = R[(size_t)b * n3];
+        const double img0 = (x0 == -FLT_MAX) ? 0.0 : (double)(x0 - mv);
+        const double c = mask_on ? (img0 - mean) * mask[0] + mean : img0;
         st[b * 4 + 1] = (float)mean;
         st[b * 4 + 2] = (float)c;
         st[b * 4 + 3] = mask_on ? 1.f : 0.f;
@@ -864,6 +899,7 @@ void project_images(ettk_plan* P, const float* d_parts, const ettk_particle* des
     S.projF.ensure(nbig * CH);
     S.projMag.ensure(nbig * CH);
     S.projStats.ensure(4 * CH);
+    S.projSum.ensure((size_t)kProjParts * CH);
     int nb_res = reslice_blocks(N);
     S.projPartials.ensure((size_t)2 * nb_res * CH);
     S.projGeom.ensure(CH);
@@ -884,8 +920,10 @@ void project_images(ettk_plan* P, const float* d_parts, const ettk_particle* des
             if (!h_geom) ETTK_HIP(hipStreamSynchronize(s));  // g is reused by the next sub-chunk
             launch_reslice(d_parts, S.projGeom.p, cn, N, S.projR.p, S.projPartials.p, nb_res, s);
         }
-        k_proj_stats<<<cn, kT, 0, s>>>(S.projR.p, S.projPartials.p, nb_res, P->m1.p, N, P->p.mask_radius != 0.0f ? 1 : 0,
-                                       is_ref ? 1 : 0, S.projStats.p);
+        k_proj_partial<<<dim3(kProjParts, cn), kT, 0, s>>>(S.projR.p, S.projPartials.p, nb_res, N, is_ref ? 1 : 0,
+                                                           S.projSum.p, S.projStats.p);
+        k_proj_final<<<cn, 64, 0, s>>>(S.projR.p, S.projSum.p, P->m1.p, N, P->p.mask_radius != 0.0f ? 1 : 0,
+                                       S.projStats.p);
         ETTK_HIP(hipGetLastError());
         R2CArgs a{S.projR.p, S.projF.p, M, cn, PRO_PAD_MASKED, P->m1.p, S.projStats.p, N};
         fft3_r2c(P->ctx->fft, a, nullptr, s);

@@ -42,6 +42,7 @@ struct So3State {
     DevBuf<float> Pp1, W1;
     // projection workspace
     DevBuf<float> projR, projMag, projStats, projPartials;
+    DevBuf<double> projSum;  // per (particle, part) sums of the image statistics
     DevBuf<float2> projF;
     DevBuf<EvalGeom> projGeom;
     // soft workspace
