@@ -562,7 +562,26 @@ __global__ void __launch_bounds__(kT) k_so3_synth_fft(const double2* __restrict_
             const int am1 = m1 < 0 ? -m1 : m1, am2 = m2 < 0 ? -m2 : m2;
             const int l0 = am1 > am2 ? am1 : am2;
             const double* d = D + (size_t)c1 * B * n + c2;
-            for (int l = l0; l < B; l++) {
+            int l = l0;
+            // four degrees at a time: their twelve loads in flight together, the sum kept in order of l
+            for (; l + 3 < B; l += 4) {
+                double2 a[4], b[4];
+                double dv[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int lu = l + u;
+                    a[u] = S[lu * lu + lu - m1];
+                    b[u] = Pt[lu * lu + lu - m2];
+                    dv[u] = d[(size_t)lu * n];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const double pr = a[u].x * b[u].x + a[u].y * b[u].y, pi = a[u].y * b[u].x - a[u].x * b[u].y;
+                    re += pr * dv[u];
+                    im += pi * dv[u];
+                }
+            }
+            for (; l < B; l++) {
                 const double2 a = S[l * l + l - m1], b = Pt[l * l + l - m2];
                 const double pr = a.x * b.x + a.y * b.y, pi = a.y * b.x - a.x * b.y;
                 const double dv = d[(size_t)l * n];
