@@ -476,6 +476,43 @@ extern "C" {
 
 const char* ettk_last_error(void) { return g_err.c_str(); }
 
+int ettk_fft3(ettk_ctx* ctx, int n, int batch, int kind, const void* in, void* out, const float* aux) {
+    ETTK_API_BEGIN
+    ETTK_REQUIRE(ctx && in && out && batch > 0, ETTK_ERR_ARG, "ettk_fft3: bad argument");
+    ETTK_REQUIRE(fft_len_index(n) >= 0, ETTK_ERR_UNSUPPORTED, "ettk_fft3: unsupported size " + std::to_string(n));
+    ETTK_REQUIRE(kind >= ETTK_FFT_R2C && kind <= ETTK_FFT_SQUARE, ETTK_ERR_ARG, "ettk_fft3: unknown kind");
+    ETTK_REQUIRE(kind < ETTK_FFT_MASK || fft3_fused_supported(n), ETTK_ERR_UNSUPPORTED,
+                 "ettk_fft3: the fused transforms need n <= 64");
+    ETTK_REQUIRE(kind != ETTK_FFT_MASK || aux, ETTK_ERR_ARG, "ettk_fft3: ETTK_FFT_MASK needs the mask");
+    ETTK_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const size_t nh = (size_t)n * n * (n / 2 + 1);
+    DevBuf<float2> tmp;
+    DevBuf<float> part;
+    switch (kind) {
+        case ETTK_FFT_R2C: {
+            R2CArgs a{static_cast<const float*>(in), static_cast<float2*>(out), n, batch, PRO_NONE, nullptr, nullptr};
+            fft3_r2c(ctx->fft, a, nullptr, s);
+            break;
+        }
+        case ETTK_FFT_C2R:
+            tmp.ensure(nh * batch);
+            fft3_c2r(ctx->fft, static_cast<const float2*>(in), tmp.p, static_cast<float*>(out), n, batch, s);
+            break;
+        case ETTK_FFT_MASK:
+            part.ensure((size_t)n * batch);
+            fft3_c2r_op_r2c(ctx->fft, static_cast<const float2*>(in), static_cast<float2*>(out), n, batch, PLANE_OP_MASK,
+                            aux, part.p, s);
+            break;
+        case ETTK_FFT_SQUARE:
+            fft3_c2r_op_r2c(ctx->fft, static_cast<const float2*>(in), static_cast<float2*>(out), n, batch,
+                            PLANE_OP_SQUARE_SHIFT, nullptr, nullptr, s);
+            break;
+    }
+    ETTK_HIP(hipStreamSynchronize(s));
+    ETTK_API_END
+}
+
 int ettk_ctx_create(int device, ettk_ctx** out) {
     ETTK_API_BEGIN
     ETTK_REQUIRE(out != nullptr, ETTK_ERR_ARG, "ettk_ctx_create: null output");

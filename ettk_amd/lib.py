@@ -47,6 +47,7 @@ def load() -> C.CDLL:
             "ettk_reference_set_average": ([vp, vp, C.c_int, C.POINTER(Particle), fp, dp], C.c_int),
             "ettk_last_timing": ([vp, dp, C.POINTER(C.c_longlong), dp], C.c_int),
             "ettk_refine_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, ip, dp, C.c_int, dp, dp], C.c_int),
+            "ettk_fft3": ([vp, C.c_int, C.c_int, C.c_int, vp, vp, vp], C.c_int),
             "ettk_profile": ([vp, C.c_int], C.c_int),
             "ettk_profile_query": ([vp, C.c_char_p, dp, C.POINTER(C.c_longlong), dp], C.c_int),
         }
@@ -89,6 +90,15 @@ class Context:
         if self.h:
             check(self.lib.ettk_ctx_destroy(self.h))
             self.h = C.c_void_p()
+
+    FFT_R2C, FFT_C2R, FFT_MASK, FFT_SQUARE = 0, 1, 2, 3
+
+    def fft3(self, kind: int, n: int, batch: int, d_in, d_out, d_aux=None) -> None:
+        """ettk_fft3 on device tensors: real volumes [batch][n][n][n] float32, half spectra
+        [batch][n][n][n/2+1] complex64 (as the kind expects), d_aux the n^3 float32 mask of FFT_MASK."""
+        aux = C.c_void_p(d_aux.data_ptr()) if d_aux is not None else None
+        check(self.lib.ettk_fft3(self.h, n, batch, kind, C.c_void_p(d_in.data_ptr()), C.c_void_p(d_out.data_ptr()),
+                                 aux))
 
     def profile(self, enable: bool) -> None:
         check(self.lib.ettk_profile(self.h, int(enable)))

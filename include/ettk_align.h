@@ -221,6 +221,20 @@ int ettk_reference_set_average(ettk_plan* plan, const float* d_ref, int n, const
    evaluations it ran. */
 int ettk_last_timing(ettk_plan* plan, double* ccc_ms, long long* n_ccc, double* total_ms);
 
+/* The path's 3-D transforms on device buffers, FFTW conventions: the r2c/c2r plans of
+   nbfFourierFilter::initializeFFTWhalf (nbfFourierFilter.h:270-358) on [batch][i][j][k] volumes with the half
+   spectrum along k ([batch][n][n][n/2+1] complex float, interleaved), r2c forward (e^-i), c2r backward (e^+i)
+   and unnormalised.  Kinds: ETTK_FFT_R2C (in real, out complex), ETTK_FFT_C2R (in complex and preserved, out
+   real), and the two transforms of the constrained CCC that never materialise the real volume (in and out
+   complex, n <= 64): ETTK_FFT_MASK (r2c(c2r(in) * aux), aux an n^3 real mask in the same [i][j][k] order,
+   FIM.h:1123-1153) and ETTK_FFT_SQUARE (r2c(c2r(in)^2 (-1)^(i+j+k)), FIM.h:1176-1187).  n in {16, 32, 48, 64,
+   96, 128, 192, 256}.  Runs on the context's stream and returns when the result is complete. */
+#define ETTK_FFT_R2C     0
+#define ETTK_FFT_C2R     1
+#define ETTK_FFT_MASK    2
+#define ETTK_FFT_SQUARE  3
+int ettk_fft3(ettk_ctx* ctx, int n, int batch, int kind, const void* in, void* out, const float* aux);
+
 /* Measurement hooks (no reference counterpart).  ettk_profile(ctx, 1) resets and enables per-kernel HIP-event
    timing on the context's stream; ettk_profile_query returns, for one kernel name (e.g. "k_fft_strided<64>",
    "k_reslice", "k_combine_dftA"), the summed launch time in ms, the launch count and the algorithmic bytes

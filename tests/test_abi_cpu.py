@@ -54,6 +54,7 @@ def test_null_arguments_are_rejected_without_a_device(lib):
     assert lib.ettk_reference_set_average(None, None, 0, None, None, None) == err_arg
     assert lib.ettk_sph_rotate_batch(None, 0, None, None, None) == err_arg
     assert lib.ettk_avg_variance(None, None, 0, None, None, None, None, None) == err_arg
+    assert lib.ettk_fft3(None, 64, 1, 0, None, None, None) == err_arg
     lib.ettk_last_error.restype = C.c_char_p
     assert b"null" in lib.ettk_last_error() or b"bad" in lib.ettk_last_error()
 
