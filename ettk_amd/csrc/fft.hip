@@ -532,13 +532,74 @@ __global__ void __launch_bounds__(Shape<L>::THREADS) k_c2r_rows(const float2* __
 // One workgroup per (batch, i) plane of L x L reals: the k pass (pairs of real rows packed as complex rows) and
 // the j pass run back to back in LDS, so an r2c moves each element through HBM twice (plane pass + i pass)
 // instead of three times.  H pencil groups of TPP threads; the spare group of the k pass works on a dummy row.
+// The k pass works on the L/2 row pairs.  The j pass works on L/2 pencils as well: the DC (k = 0) and Nyquist
+// (k = L/2) columns of a real plane's row transforms are real, so they travel as one complex pencil
+// DC + i Nyquist (pencil 0) and are separated after the transform by Hermitian symmetry; the other pencils
+// are the columns k = 1 .. L/2 - 1.  Slot L/2 of the buffer holds the Nyquist column while a c2r packs it.
 template <int L> struct Plane {
     static constexpr int H = L / 2 + 1;
     static constexpr int TPP = PenT<L>::TPP;
-    static constexpr int THREADS = H * TPP;
+    static constexpr int THREADS = (L / 2) * TPP;
     static constexpr int LD = PenT<L>::LD;
     static_assert(THREADS >= L, "one twiddle per thread");
 };
+
+// c2r: pencil 0 <- h(column 0) + i h(column L/2), h(c)(j) = (c(j) + conj c(-j)) / 2.  The c2r keeps only the real
+// part of the inverse i/j transforms on these two planes (FFTW's c2r; a spectrum that is not exactly Hermitian
+// there, such as one weighted by a border-clipped smoothed wedge, has an imaginary part that is dropped).  The
+// inverse transform of h(c) is exactly that real part, so the two columns share one transform: its real and
+// imaginary parts.  Every thread of the workgroup must call it (it synchronises).
+template <int L>
+__device__ __forceinline__ void pack_dc_nyquist(float2* buf) {
+    constexpr int LD = Plane<L>::LD;
+    const int t = threadIdx.x, tm = (L - t) % L;
+    float2 z = make_float2(0.f, 0.f);
+    if (t < L) {
+        const float2 c0 = buf[t], c0m = cconj(buf[tm]), cn = buf[(L / 2) * LD + t], cnm = cconj(buf[(L / 2) * LD + tm]);
+        const float2 h0 = make_float2(0.5f * (c0.x + c0m.x), 0.5f * (c0.y + c0m.y));
+        const float2 hn = make_float2(0.5f * (cn.x + cnm.x), 0.5f * (cn.y + cnm.y));
+        z = make_float2(h0.x - hn.y, h0.y + hn.x);
+    }
+    __syncthreads();
+    if (t < L) buf[t] = z;
+}
+// c2r along k: Z(k) = A(k) + i B(k) of rows 2pp, 2pp+1 over the full length from the inverse-j-transformed
+// pencils, A(L-k) = conj(A(k)), A(0) and A(L/2) real (the real and imaginary parts of pencil 0)
+template <int L>
+__device__ __forceinline__ float2 c2r_row_pair(const float2* buf, int pp, int k) {
+    constexpr int LD = Plane<L>::LD;
+    float2 A, Bv;
+    if (k == 0 || k == L / 2) {
+        const float2 a = buf[2 * pp], b = buf[2 * pp + 1];
+        A = make_float2(k == 0 ? a.x : a.y, 0.f);
+        Bv = make_float2(k == 0 ? b.x : b.y, 0.f);
+    } else if (k < L / 2) {
+        A = buf[k * LD + 2 * pp];
+        Bv = buf[k * LD + 2 * pp + 1];
+    } else {
+        A = cconj(buf[(L - k) * LD + 2 * pp]);
+        Bv = cconj(buf[(L - k) * LD + 2 * pp + 1]);
+    }
+    return make_float2(A.x - Bv.y, A.y + Bv.x);
+}
+// r2c: value o of column k of row 2pp + which into the j-pass pencils; at k = 0 and k = L/2 the row transform of
+// a real row is real (o.y is exactly 0) and goes to the real or imaginary part of pencil 0
+template <int L>
+__device__ __forceinline__ void r2c_split_store(float2* buf, int pp, int which, int k, float2 o) {
+    constexpr int LD = Plane<L>::LD;
+    if (k > 0 && k < L / 2) buf[k * LD + 2 * pp + which] = o;
+    else reinterpret_cast<float*>(&buf[2 * pp + which])[k == 0 ? 0 : 1] = o.x;
+}
+// r2c: element (j, k) of the half plane after the forward j pass; DC and Nyquist from pencil 0 Z:
+// X0(j) = (Z(j) + conj Z(-j)) / 2, XN(j) = (Z(j) - conj Z(-j)) / 2i
+template <int L>
+__device__ __forceinline__ float2 r2c_half_at(const float2* buf, int j, int k) {
+    constexpr int LD = Plane<L>::LD;
+    if (k > 0 && k < L / 2) return buf[k * LD + j];
+    const float2 z = buf[j], zc = cconj(buf[(L - j) % L]);
+    if (k == 0) return make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+    return make_float2(0.5f * (z.y - zc.y), -0.5f * (z.x - zc.x));
+}
 
 // the k-pass input of plane i for one preprocessing mode, staged as packed row pairs buf[(j/2) LD + x]
 // (.x row j even, .y row j odd); MSK: aux0 holds the N^3 mask
@@ -567,7 +628,7 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_r2c_plane(R2CArgs a, cons
     using PL = Plane<L>;
     constexpr int H = PL::H, LD = PL::LD;
     constexpr int NS = ((L / 2) * 2 * H + PL::THREADS - 1) / PL::THREADS;  // split outputs per thread
-    __shared__ float2 buf[H * LD];  // k pass: L/2 packed row pairs (+1 dummy); then j pass: H columns
+    __shared__ float2 buf[H * LD];  // k pass: L/2 packed row pairs; then j pass: L/2 column pencils
     __shared__ float2 tw[L];
     const long plane = blockIdx.x;  // b * L + i
     const int pb = (int)(plane / L), pi = (int)(plane % L);
@@ -612,16 +673,13 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_r2c_plane(R2CArgs a, cons
         const int t = threadIdx.x + u * PL::THREADS;
         if (t < (L / 2) * 2 * H) {
             const int pp = t / (2 * H), r = t % (2 * H), which = r / H, k = r % H;
-            buf[k * LD + 2 * pp + which] = o[u];
+            r2c_split_store<L>(buf, pp, which, k, o[u]);
         }
     }
     __syncthreads();
     pencil_fft<L>(&buf[q * LD], lane, tw, twc, false);
     float2* op = a.out + plane * L * H;
-    for (int t = threadIdx.x; t < L * H; t += blockDim.x) {
-        const int j = t / H, k = t % H;
-        op[(long)j * H + k] = buf[k * LD + j];
-    }
+    for (int t = threadIdx.x; t < L * H; t += blockDim.x) op[t] = r2c_half_at<L>(buf, t / H, t % H);
 }
 
 template <int L>
@@ -638,30 +696,16 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_plane(const float2* _
     block_copy<PL::THREADS, L * H>([&](int t) { return ip[t]; }, [&](int t, float2 v) { buf[(t % H) * LD + t / H] = v; });
     if (threadIdx.x < L) tw[threadIdx.x] = twv;
     __syncthreads();
+    pack_dc_nyquist<L>(buf);
+    __syncthreads();
     const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
     const TwCache<L> twc = tw_cache<L>(tw, lane);
     pencil_fft<L>(&buf[q * LD], lane, tw, twc, true);
-    // Z(k) = A(k) + i B(k) of rows 2p, 2p+1 over the full length, A(L-k) = conj(A(k)), real A(0), A(L/2)
     float2 zr[NZ];
 #pragma unroll
     for (int u = 0; u < NZ; u++) {
         const int t = threadIdx.x + u * PL::THREADS;
-        if (t < (L / 2) * L) {
-            const int pp = t / L, k = t % L;
-            float2 A, Bv;
-            if (k <= L / 2) {
-                A = buf[k * LD + 2 * pp];
-                Bv = buf[k * LD + 2 * pp + 1];
-                if (k == 0 || k == L / 2) {
-                    A.y = 0.f;
-                    Bv.y = 0.f;
-                }
-            } else {
-                A = cconj(buf[(L - k) * LD + 2 * pp]);
-                Bv = cconj(buf[(L - k) * LD + 2 * pp + 1]);
-            }
-            zr[u] = make_float2(A.x - Bv.y, A.y + Bv.x);
-        }
+        if (t < (L / 2) * L) zr[u] = c2r_row_pair<L>(buf, t / L, t % L);
     }
     __syncthreads();
 #pragma unroll
@@ -705,30 +749,17 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_r2c_plane(float2* __r
     block_copy<T, L * H>([&](int t) { return ip[t]; }, [&](int t, float2 v) { buf[(t % H) * LD + t / H] = v; });
     if (threadIdx.x < L) tw[threadIdx.x] = twv;
     __syncthreads();
+    pack_dc_nyquist<L>(buf);
+    __syncthreads();
     const int q = threadIdx.x / PL::TPP, lane = threadIdx.x % PL::TPP;
     const TwCache<L> twc = tw_cache<L>(tw, lane);
     pencil_fft<L>(&buf[q * LD], lane, tw, twc, true);
-    // c2r along k: Z(k) = A(k) + i B(k) of rows 2p, 2p+1 (as k_c2r_plane)
+    // c2r along k (as k_c2r_plane)
     float2 zr[NZ];
 #pragma unroll
     for (int u = 0; u < NZ; u++) {
         const int t = threadIdx.x + u * T;
-        if (t < (L / 2) * L) {
-            const int pp = t / L, k = t % L;
-            float2 A, Bv;
-            if (k <= L / 2) {
-                A = buf[k * LD + 2 * pp];
-                Bv = buf[k * LD + 2 * pp + 1];
-                if (k == 0 || k == L / 2) {
-                    A.y = 0.f;
-                    Bv.y = 0.f;
-                }
-            } else {
-                A = cconj(buf[(L - k) * LD + 2 * pp]);
-                Bv = cconj(buf[(L - k) * LD + 2 * pp + 1]);
-            }
-            zr[u] = make_float2(A.x - Bv.y, A.y + Bv.x);
-        }
+        if (t < (L / 2) * L) zr[u] = c2r_row_pair<L>(buf, t / L, t % L);
     }
     __syncthreads();
 #pragma unroll
@@ -799,15 +830,12 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_r2c_plane(float2* __r
         const int t = threadIdx.x + u * T;
         if (t < (L / 2) * 2 * H) {
             const int pp = t / (2 * H), r = t % (2 * H), which = r / H, k = r % H;
-            buf[k * LD + 2 * pp + which] = o[u];
+            r2c_split_store<L>(buf, pp, which, k, o[u]);
         }
     }
     __syncthreads();
     pencil_fft<L>(&buf[q * LD], lane, tw, twc, false);
-    for (int t = threadIdx.x; t < L * H; t += blockDim.x) {
-        const int j = t / H, k = t % H;
-        ip[(long)j * H + k] = buf[k * LD + j];
-    }
+    for (int t = threadIdx.x; t < L * H; t += blockDim.x) ip[t] = r2c_half_at<L>(buf, t / H, t % H);
 }
 
 // ------------------------------------------------------------------ host launchers

@@ -77,6 +77,31 @@ def test_fused_transforms_match_numpy(torch_gpu, ctx, n):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n", [16, 48, 64, 128])
+def test_c2r_of_a_non_hermitian_spectrum_keeps_the_real_part(torch_gpu, ctx, n):
+    """A half spectrum that is not exactly Hermitian on its k = 0 and k = n/2 planes (as the CCC's wedge-weighted
+    spectra are near the borders) transforms as FFTW's and numpy's c2r do: complex inverse transforms along i and
+    j, then the half-complex transform along k with the imaginary parts of those two planes dropped."""
+    torch = torch_gpu
+    print(f"[fft] c2r of a non-Hermitian spectrum n={n}", flush=True)
+    g = np.random.default_rng(200 + n)
+    h = n // 2 + 1
+    X = (g.standard_normal((1, n, n, h)) + 1j * g.standard_normal((1, n, n, h))).astype(np.complex64)
+    r = np.fft.irfftn(X.astype(np.complex128), s=(n, n, n), axes=(1, 2, 3)) * float(n) ** 3
+    dX = torch.from_numpy(X).cuda()
+    dy = torch.empty((1, n, n, n), dtype=torch.float32, device="cuda")
+    ctx.fft3(ctx.FFT_C2R, n, 1, dX, dy)
+    assert _rel(dy.cpu().numpy(), r) < BAR
+    if n <= 64:
+        out = torch.empty_like(dX)
+        ctx.fft3(ctx.FFT_SQUARE, n, 1, dX, out)
+        _check_half(out.cpu().numpy(), np.fft.rfftn(r * r * _checker(n), axes=(1, 2, 3)))
+        m = g.uniform(0.0, 1.0, (n, n, n)).astype(np.float32)
+        ctx.fft3(ctx.FFT_MASK, n, 1, dX, out, torch.from_numpy(m).cuda())
+        _check_half(out.cpu().numpy(), np.fft.rfftn(r * m.astype(np.float64), axes=(1, 2, 3)))
+
+
+@pytest.mark.gpu
 def test_unsupported_requests_are_rejected(torch_gpu, ctx):
     torch = torch_gpu
     from ettk_amd.lib import EttkError
