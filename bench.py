@@ -138,7 +138,8 @@ def main() -> int:
         if nl:
             kernels[name] = {"ms": ms, "launches": nl, "bytes": by}
     host = {}
-    for name in ("search_stage_wall", "search_gpu_wall", "search_host", "candidate_stage_wall", "refine_wall",
+    for name in ("search_stage_wall", "search_inputs", "search_enqueue", "search_host", "search_download", "search_tail",
+                 "candidate_stage_wall", "refine_wall",
                  "ccc_batch_wall", "lm_count"):
         ms, nl, _ = ctx.profile_query(name)
         host[name] = {"total": round(ms, 2), "count": nl}

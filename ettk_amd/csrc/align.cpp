@@ -95,15 +95,16 @@ using CandQueue = std::priority_queue<Cand, std::vector<Cand>, CandCmp>;
 // matrices are within `tol` in L1 norm.  A pair within tol differs by less than tol in every matrix element, so
 // its third columns fall in the same or adjacent cells of a grid of pitch tol: only those pairs are visited,
 // in the same (i, then increasing j) order, which leaves every decision of the exhaustive loop unchanged.
-static void eliminate_equivalent(std::vector<Cand>& pos, CandQueue& pq, float tol) {
+static void eliminate_equivalent(std::vector<Cand>& pos, const std::vector<Mat4>& mats, CandQueue& pq, float tol) {
     const float NEG = -FLT_MAX;
     const size_t n = pos.size();
-    std::vector<Mat4> mats(n);
-    for (size_t i = 0; i < n; i++) mats[i] = zyz(pos[i].a, pos[i].b, pos[i].g);
-    auto close = [&](size_t i, size_t j) {
+    auto close = [&](size_t i, size_t j) {  // the partial sums never decrease: stop once they reach tol
         double err = 0;
-        for (int m = 0; m < 16; m++) err += std::fabs(mats[i].a[m] - mats[j].a[m]);
-        return err < tol;
+        for (int m = 0; m < 16; m++) {
+            err += std::fabs(mats[i].a[m] - mats[j].a[m]);
+            if (!(err < tol)) return false;
+        }
+        return true;
     };
     auto visit = [&](size_t i, size_t j) {
         if (i != j && pos[j].v > NEG && close(i, j)) {
@@ -131,6 +132,8 @@ static void eliminate_equivalent(std::vector<Cand>& pos, CandQueue& pq, float to
             next[i] = head[k];
             head[k] = (int)i;
         }
+        // closeness does not depend on the visiting order: the close partners of i are found first, then visited
+        // in increasing index order as the exhaustive loop would
         std::vector<size_t> js;
         for (size_t i = 0; i < n; i++) {
             if (pos[i].v == NEG) continue;
@@ -138,11 +141,15 @@ static void eliminate_equivalent(std::vector<Cand>& pos, CandQueue& pq, float to
             for (int a = -1; a <= 1; a++)
                 for (int b = -1; b <= 1; b++)
                     for (int c = -1; c <= 1; c++)
-                        for (int j = head[key[i] + (a * G + b) * G + c]; j >= 0; j = next[j]) js.push_back((size_t)j);
+                        for (int j = head[key[i] + (a * G + b) * G + c]; j >= 0; j = next[j])
+                            if ((size_t)j != i && close(i, (size_t)j)) js.push_back((size_t)j);
             std::sort(js.begin(), js.end());
             for (size_t j : js) {
                 if (pos[i].v == NEG) break;
-                visit(i, j);
+                if (pos[j].v > NEG) {
+                    if (pos[i].v > pos[j].v) pos[i].v = NEG;
+                    else pos[j].v = NEG;
+                }
             }
         }
     }
@@ -150,21 +157,57 @@ static void eliminate_equivalent(std::vector<Cand>& pos, CandQueue& pq, float to
         if (p.v > NEG) pq.push(p);
 }
 
+static void eliminate_equivalent(std::vector<Cand>& pos, CandQueue& pq, float tol) {
+    std::vector<Mat4> mats(pos.size());
+    for (size_t i = 0; i < pos.size(); i++) mats[i] = zyz(pos[i].a, pos[i].b, pos[i].g);
+    eliminate_equivalent(pos, mats, pq, tol);
+}
+
+// the Euler angles of the (2B)^3 correlation grid (ProjRot.h:1620-1640) and the three rotations zyz multiplies for
+// each of their 2B values: zyz(alpha, beta, gamma) = (rg[g] rb[b]) ra[a], the same products as computed directly
+struct ZyzTables {
+    std::vector<float> alpha, beta, gamma;
+    std::vector<Mat4> ra, rb, rg;
+};
+static const ZyzTables& zyz_tables(int B) {
+    static std::mutex mu;
+    static std::map<int, ZyzTables> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(B);
+    if (it != cache.end()) return it->second;
+    const int n = 2 * B;
+    ZyzTables z;
+    for (int i = 0; i < n; i++) {
+        const float alpha = (float)(i / (double)B * 180.0);
+        float beta = (float)((2 * i + 1.0) / (4. * B) * 180.0);
+        if ((float)B <= 32) beta = (float)((2 * i + 0.0) / (4. * B) * 180.0);
+        const float gamma = (float)(i / (double)B * 180.0);
+        z.alpha.push_back(alpha);
+        z.beta.push_back(beta);
+        z.gamma.push_back(gamma);
+        z.ra.push_back(mat_rot(-(double)alpha, 0, 0, 1));
+        z.rb.push_back(mat_rot(-(double)beta, 0, 1, 0));
+        z.rg.push_back(mat_rot(-(double)gamma, 0, 0, 1));
+    }
+    return cache.emplace(B, std::move(z)).first->second;
+}
+
 static void candidates_from_maxima(const ettk_plan* P, std::vector<std::pair<int, double>>& lm, CandQueue& pq) {
     const int B = P->p.B, n = 2 * B;
+    const ZyzTables& Z = zyz_tables(B);
     std::sort(lm.begin(), lm.end());  // blitz iteration order (alpha slowest)
     std::vector<Cand> pos;
+    std::vector<Mat4> mats;
     for (auto& e : lm) {
         int t = e.first;
         int a = t / (n * n), b = (t / n) % n, g = t % n;
-        float alpha = (float)(a / (double)B * 180.0);
-        float beta = (float)((2 * b + 1.0) / (4. * B) * 180.0);
-        if ((float)B <= 32) beta = (float)((2 * b + 0.0) / (4. * B) * 180.0);
-        float gamma = (float)(g / (double)B * 180.0);
-        if (transform_ok(zyz(alpha, beta, gamma), P->trans_restrict, P->rot_restrict))
-            pos.push_back(Cand{alpha, beta, gamma, (float)(-e.second)});
+        const Mat4 m = Z.rg[g] * Z.rb[b] * Z.ra[a];
+        if (transform_ok(m, P->trans_restrict, P->rot_restrict)) {
+            pos.push_back(Cand{Z.alpha[a], Z.beta[b], Z.gamma[g], (float)(-e.second)});
+            mats.push_back(m);
+        }
     }
-    eliminate_equivalent(pos, pq, 0.25f);
+    eliminate_equivalent(pos, mats, pq, 0.25f);
 }
 
 struct SearchOut {
@@ -176,29 +219,38 @@ struct SearchOut {
 // Host-side inputs of the search, per chunk of particles (computed on the host threads while the previous
 // chunk runs on the GPU): smoothed spherical wedge W2 (nbfWedge3D::getSphericalImage +
 // conditionSphericalWindow), whether it is "effective" (sum <= 0.95 size, ProjRot.h:526-540), and the reslice
-// geometry of the projection.  The arrays are sized for the whole call and stay alive until it ends.
+// geometry of the projection.  The arrays are sized for the whole call (W2 and geom in the plan's pinned buffers,
+// uploaded asynchronously) and stay unchanged until it ends.
 struct SearchInputs {
-    std::vector<float> W2;
+    float* W2 = nullptr;
+    EvalGeom* geom = nullptr;
     std::vector<int> uw2;
-    std::vector<EvalGeom> geom;
 };
+
+// sizes the inputs of a call with n particles and every device buffer of its chunks (up to `chunk` particles)
+static void search_inputs_init(ettk_plan* P, int n, int chunk, SearchInputs& in) {
+    So3State& S = so3_state(P);
+    const size_t nn = (size_t)4 * P->p.B * P->p.B;
+    S.hW2.ensure((size_t)n * nn);
+    S.hGeom.ensure(n);
+    in.W2 = S.hW2.p;
+    in.geom = S.hGeom.p;
+    in.uw2.assign(n, 0);
+    so3_search_reserve(P, chunk);
+}
 
 static void search_inputs(const ettk_plan* P, const ettk_particle* descs, int n, int p0, int cn, SearchInputs& in) {
     const int B = P->p.B, nn = 4 * B * B;
-    if (in.uw2.size() != (size_t)n) {
-        in.W2.assign((size_t)n * nn, 0.0f);
-        in.uw2.assign(n, 0);
-        in.geom.resize(n);
-    }
+    (void)n;
     auto work = [&](int i) {
         std::vector<float> w = spherical_wedge_host(B, descs[i], P->p.missing_wedge != 0);
         double sw = 0;
         for (float v : w) sw += v;
         in.uw2[i] = P->p.missing_wedge && (sw <= 0.95f * (float)nn);
-        std::copy(w.begin(), w.end(), in.W2.begin() + (size_t)i * nn);
+        std::copy(w.begin(), w.end(), in.W2 + (size_t)i * nn);
         in.geom[i] = make_geom_public(P, descs[i], Mat4::ident(), i);
     };
-    const int nthr = std::max(1, std::min<int>(cn / 4, (int)std::thread::hardware_concurrency()));
+    const int nthr = std::max(1, std::min<int>(cn / 4, host_threads() / 2));
     if (nthr <= 1) {
         for (int i = p0; i < p0 + cn; i++) work(i);
         return;
@@ -224,6 +276,7 @@ struct SearchLists {
     ~SearchLists() {
         if (done) (void)hipEventDestroy(done);
     }
+    std::chrono::steady_clock::time_point t_ready;  // when its kernels had completed
 };
 static constexpr int kLmCap = 16384;  // local maxima kept per correlation volume
 
@@ -243,10 +296,10 @@ static void search_gpu(ettk_plan* P, const float* d_parts, const ettk_particle* 
     L.lm = &lm_slot(S, slot, 2 * cn, kLmCap);
     S.Pp2.ensure((size_t)cn * nn);
     S.Mx.ensure(cn);
-    project_images(P, d_parts, descs, nullptr, cn, false, S.Pp2.p, in.geom.data() + p0);
+    project_images(P, d_parts, descs, nullptr, cn, false, S.Pp2.p, in.geom + p0);
     launch_ppmax(S.Pp1.p, S.Pp2.p, cn, B, S.Mx.p, s);
     S.W2.ensure((size_t)cn * nn);
-    ETTK_HIP(hipMemcpyAsync(S.W2.p, in.W2.data() + (size_t)p0 * nn, (size_t)cn * nn * sizeof(float),
+    ETTK_HIP(hipMemcpyAsync(S.W2.p, in.W2 + (size_t)p0 * nn, (size_t)cn * nn * sizeof(float),
                             hipMemcpyHostToDevice, s));
     const bool uw1 = L.uw1;
     std::vector<SoftInst>& inst = L.inst;
@@ -284,16 +337,20 @@ static void search_gpu(ettk_plan* P, const float* d_parts, const ettk_particle* 
     const int nq = (int)inst.size();
     S.C.ensure((size_t)nq * n3);
     S.Cc.ensure((size_t)2 * cn * n3);
-    run_soft(P, inst, S.Pp2.p, S.W2.p, S.Mx.p, S.C.p);
+    L.lm->hinst.ensure(inst.size());
+    L.lm->hops.ensure(ops.size());
+    std::copy(inst.begin(), inst.end(), L.lm->hinst.p);
+    std::copy(ops.begin(), ops.end(), L.lm->hops.p);
+    run_soft(P, inst, S.Pp2.p, S.W2.p, S.Mx.p, S.C.p, L.lm->hinst.p);
     S.ops.ensure(ops.size());
-    ETTK_HIP(hipMemcpyAsync(S.ops.p, ops.data(), ops.size() * sizeof(ComboOp), hipMemcpyHostToDevice, s));
+    ETTK_HIP(hipMemcpyAsync(S.ops.p, L.lm->hops.p, ops.size() * sizeof(ComboOp), hipMemcpyHostToDevice, s));
     double f = std::floor(P->rot_restrict * B / 90.0 - 1.0);
     int b0 = (int)std::max(1.0, std::min(f, (double)(n - 1)));
     {
         ProfScope ps(kp, "k_combine", (double)(nq + 2 * cn) * n3 * 8, s);
         launch_combine(S.C.p, S.ops.p, 2 * cn, B, b0, S.Cc.p, s);
     }
-    ETTK_HIP(hipMemsetAsync(L.lm->count.p, 0, 2 * cn * sizeof(int), s));
+    ETTK_HIP(hipMemsetAsync(L.lm->count.p, 0, (size_t)2 * cn * kLmCountStride * sizeof(int), s));
     {
         ProfScope ps(kp, "k_localmax", 2.0 * cn * n3 * 8, s);
         launch_localmax(S.Cc.p, 2 * cn, B, kLmCap, L.lm->count.p, L.lm->idx.p, L.lm->val.p, s);
@@ -306,10 +363,11 @@ static void search_gpu(ettk_plan* P, const float* d_parts, const ettk_particle* 
 
 // Host part: download the chunk's compacted maxima (after its kernels, on the context's copy stream: the plan's
 // stream does not wait for it) and extract the candidate lists of its volumes on several host threads.
-static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOut>& out) {
+static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOut>& out, bool last = false) {
     KernelProf* kp = P->ctx->fft.prof;
     ETTK_HIP(hipEventSynchronize(L.done));
     auto th0 = std::chrono::steady_clock::now();
+    L.t_ready = th0;
     const int p0 = L.p0, cn = L.cn;
     const bool uw1 = L.uw1;
     const int nv = 2 * cn;
@@ -325,6 +383,7 @@ static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOu
         ETTK_HIP(hipMemcpyAsync(pval.data(), L.lm->pval.p, (size_t)total * sizeof(double), hipMemcpyDeviceToHost, cs));
         ETTK_HIP(hipStreamSynchronize(cs));
     }
+    if (kp) kp->add_host("search_download", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count());
     std::vector<std::vector<int>> idx(nv);
     std::vector<std::vector<double>> val(nv);
     for (int v = 0; v < nv; v++) {
@@ -350,7 +409,9 @@ static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOu
         SearchOut& o = out[p0 + q];
         candidates_from_maxima(P, lm, v % 2 == 0 ? o.pq1 : o.pq2);
     };
-    int nthr = std::max(1, std::min<int>(2 * cn, (int)std::thread::hardware_concurrency()));
+    // the next chunk's extraction and input preparation run at the same time: half of the threads each, all of
+    // them for the last chunk of a call
+    int nthr = std::max(1, std::min<int>(2 * cn, last ? host_threads() : host_threads() / 2));
     if (nthr <= 1) {
         for (int v = 0; v < 2 * cn; v++) work(v);
     } else {
@@ -374,17 +435,41 @@ static void search_all(ettk_plan* P, const float* d_parts, const ettk_particle* 
     std::vector<std::unique_ptr<SearchLists>> lists;
     std::vector<std::future<void>> pending;
     std::exception_ptr err;
+    // the inputs of chunk k + 1 are prepared on a second host thread while chunk k is enqueued (disjoint ranges of
+    // arrays sized for the whole call by the first, synchronous preparation)
+    std::future<void> next;
     try {
+        search_inputs_init(P, n, chunk, in);
+        search_inputs(P, descs, n, 0, std::min(chunk, n), in);
         for (int p0 = 0, k = 0; p0 < n; p0 += chunk, k++) {
             const int cn = std::min(chunk, n - p0);
-            search_inputs(P, descs, n, p0, cn, in);
+            auto ti = std::chrono::steady_clock::now();
+            if (next.valid()) next.get();
+            auto tg = std::chrono::steady_clock::now();
+            if (p0 + chunk < n) {
+                const int q0 = p0 + chunk, qn = std::min(chunk, n - q0);
+                next = std::async(std::launch::async, [P, descs, n, q0, qn, &in] { search_inputs(P, descs, n, q0, qn, in); });
+            }
             lists.emplace_back(new SearchLists());
             SearchLists* L = lists.back().get();
             search_gpu(P, d_parts, descs, p0, cn, in, k, *L);
-            pending.push_back(std::async(std::launch::async, [P, L, &out] { search_host(P, *L, out); }));
+            if (kp) {
+                auto te = std::chrono::steady_clock::now();
+                kp->add_host("search_inputs", std::chrono::duration<double, std::milli>(tg - ti).count());
+                kp->add_host("search_enqueue", std::chrono::duration<double, std::milli>(te - tg).count());
+            }
+            const bool last = p0 + chunk >= n;
+            pending.push_back(std::async(std::launch::async, [P, L, &out, last] { search_host(P, *L, out, last); }));
         }
     } catch (...) {
         err = std::current_exception();
+    }
+    if (next.valid()) {
+        try {
+            next.get();
+        } catch (...) {
+            if (!err) err = std::current_exception();
+        }
     }
     for (auto& f : pending) {
         try {
@@ -394,7 +479,12 @@ static void search_all(ettk_plan* P, const float* d_parts, const ettk_particle* 
         }
     }
     if (err) std::rethrow_exception(err);
-    if (kp) kp->add_host("search_stage_wall", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ts0).count());
+    if (kp) {
+        const auto te = std::chrono::steady_clock::now();
+        kp->add_host("search_stage_wall", std::chrono::duration<double, std::milli>(te - ts0).count());
+        if (!lists.empty())  // host time after the last chunk's kernels
+            kp->add_host("search_tail", std::chrono::duration<double, std::milli>(te - lists.back()->t_ready).count());
+    }
 }
 
 // ------------------------------------------------------------------ Powell (vnl_powell, maxfev = 1)
@@ -862,6 +952,7 @@ void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
             return d;
         };
         SearchInputs in;
+        search_inputs_init(P, n, 1, in);
         for (int i = 0; i < n; i++) {
             search_inputs(P, descs, n, i, 1, in);
             SearchLists L;

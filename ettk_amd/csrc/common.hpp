@@ -15,6 +15,10 @@ namespace ettk {
 
 // ------------------------------------------------------------------ errors
 void set_error(const std::string& msg);
+
+// host worker threads for one parallel section: OMP_NUM_THREADS when set (the process's share of the CPU),
+// otherwise the hardware thread count, at most 16
+int host_threads();
 struct Error {
     int code;
     std::string msg;
@@ -100,6 +104,28 @@ struct DevBuf {
     void upload(const T* h, size_t count, hipStream_t s) {
         ensure(count);
         ETTK_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+// pinned host buffer (hipHostMalloc): host-to-device copies from it are asynchronous with respect to the host
+template <class T>
+struct PinnedBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() { release(); }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void ensure(size_t count) {
+        if (count <= n) return;
+        release();
+        ETTK_HIP(hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault));
+        n = count;
     }
 };
 

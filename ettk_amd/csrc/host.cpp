@@ -3,6 +3,7 @@
 #include <chrono>
 #include <cmath>
 #include <mutex>
+#include <cstdlib>
 #include <thread>
 
 #include "plan.hpp"
@@ -97,6 +98,18 @@ Mat4 euler_of(const ettk_particle& d) {
 
 // ============================================================== constant arrays of a plan
 // separable vtkImageGaussianSmooth on an A(i,j,k) float array: axes 2, 1, 0, kernel clipped and renormalised
+int host_threads() {
+    static const int n = [] {
+        int t = (int)std::thread::hardware_concurrency();
+        if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+            const int v = std::atoi(e);
+            if (v > 0) t = v;
+        }
+        return std::max(1, std::min(t, 16));
+    }();
+    return n;
+}
+
 void smooth3(std::vector<float>& A, int n0, int n1, int n2, double sd, double rf) {
     int dims[3] = {n0, n1, n2};
     long st[3] = {(long)n1 * n2, n2, 1};
@@ -104,16 +117,34 @@ void smooth3(std::vector<float>& A, int n0, int n1, int n2, double sd, double rf
     std::vector<double> g(2 * rad + 1);  // kernel taps, the same values for every voxel
     for (int x = -rad; x <= rad; x++) g[x + rad] = std::exp(-(double)(x * x) / (sd * sd * 2.0));
     std::vector<float> o(A.size());
+    const int nt = 2 * rad + 1;
     for (int ax = 2; ax >= 0; ax--) {
-        int n = dims[ax];
-        for (long b = 0; b < (long)A.size(); b++) {
-            int idx = (int)((b / st[ax]) % n);
-            int lo = std::max(-rad, -idx), hi = std::min(rad, n - 1 - idx);
-            double ks = 0, s = 0;
+        const int n = dims[ax];
+        const long sa = st[ax];
+        // the clipped, renormalised taps g[x] / sum_{lo..hi} g of each position along the axis (per position, not
+        // per voxel: the same doubles the voxel loop multiplies by)
+        std::vector<double> w((size_t)n * nt, 0.0);
+        std::vector<int> wlo(n), whi(n);
+        for (int idx = 0; idx < n; idx++) {
+            const int lo = std::max(-rad, -idx), hi = std::min(rad, n - 1 - idx);
+            double ks = 0;
             for (int x = lo; x <= hi; x++) ks += g[x + rad];
-            for (int x = lo; x <= hi; x++) s += g[x + rad] / ks * A[b + x * st[ax]];
-            o[b] = (float)s;
+            for (int x = lo; x <= hi; x++) w[(size_t)idx * nt + x + rad] = g[x + rad] / ks;
+            wlo[idx] = lo;
+            whi[idx] = hi;
         }
+        const long outer = (long)A.size() / ((long)n * sa);
+        for (long ob = 0; ob < outer; ob++)
+            for (int idx = 0; idx < n; idx++) {
+                const double* wi = &w[(size_t)idx * nt + rad];
+                const int lo = wlo[idx], hi = whi[idx];
+                for (long in = 0; in < sa; in++) {
+                    const long b = (ob * n + idx) * sa + in;
+                    double s = 0;
+                    for (int x = lo; x <= hi; x++) s += wi[x] * A[b + x * sa];
+                    o[b] = (float)s;
+                }
+            }
         A.swap(o);
     }
 }
@@ -360,7 +391,7 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
     // one download of the results at the end
     std::vector<EvalGeom> g(n);
     auto geom_of = [&](int e) { g[e] = make_geom_public(P, descs[pidx[e]], t[e], pidx[e]); };
-    const int nthr = std::max(1, std::min<int>(n / 256, (int)std::thread::hardware_concurrency()));
+    const int nthr = std::max(1, std::min<int>(n / 256, host_threads()));
     if (nthr <= 1) {
         for (int e = 0; e < n; e++) geom_of(e);
     } else {

@@ -846,6 +846,63 @@ __global__ void k_combine(const double* __restrict__ C, const ComboOp* __restric
 
 // 26-neighbour >= test with a -FLT_MAX border (localMaxima3D3x3 + BordStrategyConst, ProjRot.h:67-79, 605-610);
 // nonzero results are appended to a per-volume list (order restored on the host by sorting the index)
+// local maxima of each correlation volume (26 neighbours, none outside the volume; ProjRot.h:67-79): one 8^3 tile
+// per workgroup, staged with its one-voxel halo in LDS (a neighbour outside the volume reads -FLT_MAX)
+static constexpr int kLmT = 8, kLmH = kLmT + 2;
+__global__ void __launch_bounds__(kT) k_localmax_tiled(const double* __restrict__ C, int n, int cap,
+                                                       int* __restrict__ count, int* __restrict__ idx,
+                                                       double* __restrict__ val) {
+    __shared__ double tile[kLmH * kLmH * kLmH];
+    __shared__ int s_idx[kLmT * kLmT * kLmT];
+    __shared__ double s_val[kLmT * kLmT * kLmT];
+    __shared__ int s_n, s_base;
+    const int p = blockIdx.y, nt = n / kLmT, tb = blockIdx.x;
+    if (threadIdx.x == 0) s_n = 0;
+    const int a0 = (tb / (nt * nt)) * kLmT, b0 = ((tb / nt) % nt) * kLmT, g0 = (tb % nt) * kLmT;
+    const double* c = C + (size_t)p * n * n * n;
+    block_copy<kT, kLmH * kLmH * kLmH>(
+        [&](int t) {
+            const int aa = a0 + t / (kLmH * kLmH) - 1, bb = b0 + (t / kLmH) % kLmH - 1, gg = g0 + t % kLmH - 1;
+            const bool in = aa >= 0 && bb >= 0 && gg >= 0 && aa < n && bb < n && gg < n;
+            const double x = c[in ? ((size_t)aa * n + bb) * n + gg : ((size_t)a0 * n + b0) * n + g0];
+            return in ? x : -(double)FLT_MAX;
+        },
+        [&](int t, double x) { tile[t] = x; });
+    __syncthreads();
+    for (int t = threadIdx.x; t < kLmT * kLmT * kLmT; t += kT) {
+        const int la = t / (kLmT * kLmT), lb = (t / kLmT) % kLmT, lg = t % kLmT;
+        const int ci = ((la + 1) * kLmH + lb + 1) * kLmH + lg + 1;
+        const double v = tile[ci];
+        bool ok = true;
+#pragma unroll
+        for (int da = -1; da <= 1; da++)
+#pragma unroll
+            for (int db = -1; db <= 1; db++)
+#pragma unroll
+                for (int dg = -1; dg <= 1; dg++) {
+                    if (!da && !db && !dg) continue;
+                    ok = ok && (v >= tile[ci + (da * kLmH + db) * kLmH + dg]);
+                }
+        if (ok && v != 0.0) {
+            const int k = atomicAdd(&s_n, 1);
+            s_idx[k] = ((a0 + la) * n + b0 + lb) * n + g0 + lg;
+            s_val[k] = v;
+        }
+    }
+    // one global append per workgroup
+    __syncthreads();
+    if (threadIdx.x == 0 && s_n > 0) s_base = atomicAdd(&count[(size_t)p * kLmCountStride], s_n);
+    __syncthreads();
+    for (int k = threadIdx.x; k < s_n; k += kT) {
+        const int slot = s_base + k;
+        if (slot < cap) {
+            idx[(size_t)p * cap + slot] = s_idx[k];
+            val[(size_t)p * cap + slot] = s_val[k];
+        }
+    }
+}
+
+// the same predicate for volume edges that are not a multiple of the tile
 __global__ void k_localmax(const double* __restrict__ C, int n, int cap, int* __restrict__ count,
                            int* __restrict__ idx, double* __restrict__ val) {
     const int p = blockIdx.y;
@@ -853,20 +910,33 @@ __global__ void k_localmax(const double* __restrict__ C, int n, int cap, int* __
     const double* c = C + (size_t)p * n3;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n3; t += (long)gridDim.x * blockDim.x) {
         int a = (int)(t / ((long)n * n)), b = (int)((t / n) % n), g = (int)(t % n);
-        double v = c[t];
-        bool ok = true;
-        for (int da = -1; da <= 1 && ok; da++)
-            for (int db = -1; db <= 1 && ok; db++)
-                for (int dg = -1; dg <= 1 && ok; dg++) {
+        const double v = c[t];
+        // all 26 neighbours loaded before the first comparison (one memory latency); a neighbour outside the
+        // volume counts as -FLT_MAX (its load reads the centre and is discarded)
+        double nv[26];
+        unsigned inb = 0;
+        int q = 0;
+#pragma unroll
+        for (int da = -1; da <= 1; da++)
+#pragma unroll
+            for (int db = -1; db <= 1; db++)
+#pragma unroll
+                for (int dg = -1; dg <= 1; dg++) {
                     if (!da && !db && !dg) continue;
-                    int aa = a + da, bb = b + db, gg = g + dg;
-                    double nv = (aa < 0 || bb < 0 || gg < 0 || aa >= n || bb >= n || gg >= n)
-                                    ? -(double)FLT_MAX
-                                    : c[((size_t)aa * n + bb) * n + gg];
-                    if (!(v >= nv)) ok = false;
+                    const int aa = a + da, bb = b + db, gg = g + dg;
+                    const bool in = aa >= 0 && bb >= 0 && gg >= 0 && aa < n && bb < n && gg < n;
+                    nv[q] = c[in ? ((size_t)aa * n + bb) * n + gg : (size_t)t];
+                    inb |= (in ? 1u : 0u) << q;
+                    q++;
                 }
+        bool ok = true;
+#pragma unroll
+        for (int r = 0; r < 26; r++) {
+            const double w = ((inb >> r) & 1u) ? nv[r] : -(double)FLT_MAX;
+            ok = ok && (v >= w);
+        }
         if (ok && v != 0.0) {
-            int slot = atomicAdd(&count[p], 1);
+            int slot = atomicAdd(&count[(size_t)p * kLmCountStride], 1);
             if (slot < cap) {
                 idx[(size_t)p * cap + slot] = (int)t;
                 val[(size_t)p * cap + slot] = v;
@@ -1046,13 +1116,43 @@ void soft_core(ettk_plan* P, const std::vector<SoftInst>& inst, double* d_C) {
     }
 }
 
+void so3_search_reserve(ettk_plan* P, int cn) {
+    So3State& S = so3(P);
+    const int B = S.B, n = 2 * B;
+    const size_t n3 = (size_t)n * n * n, nq = (size_t)5 * cn, nn = (size_t)n * n;
+    S.inst.ensure(nq);
+    S.sig.ensure(nq * nn);
+    S.pat.ensure(nq * nn);
+    S.fm.ensure(nq * nn);
+    S.cs.ensure(nq * B * B);
+    S.cp.ensure(nq * B * B);
+    S.vals.ensure(2 * nq);
+    S.pos.ensure(2 * nq);
+    S.rot.ensure(2 * nq * B * B);
+    S.g.ensure(2 * nq * nn);
+    S.Rr.ensure(2 * nq * nn);
+    S.wv.ensure(2 * nq);
+    S.pvals.ensure((size_t)2 * kExtParts * nq);
+    S.ppos.ensure((size_t)2 * kExtParts * nq);
+    if (!fused_synthesis(B)) {
+        S.T1.ensure(nq * n3);
+        S.T2.ensure(nq * n3);
+    }
+    S.Pp2.ensure((size_t)cn * nn);
+    S.Mx.ensure(cn);
+    S.W2.ensure((size_t)cn * nn);
+    S.C.ensure(nq * n3);
+    S.Cc.ensure((size_t)2 * cn * n3);
+    S.ops.ensure((size_t)2 * cn);
+}
+
 void run_soft(ettk_plan* P, const std::vector<SoftInst>& inst, const float* d_Pp2, const float* d_W2, const float* d_Mx,
-              double* d_C) {
+              double* d_C, const SoftInst* h_inst) {
     So3State& S = so3(P);
     const int B = S.B, n = 2 * B, nq = (int)inst.size();
     hipStream_t s = P->ctx->stream;
     S.inst.ensure(nq);
-    ETTK_HIP(hipMemcpyAsync(S.inst.p, inst.data(), nq * sizeof(SoftInst), hipMemcpyHostToDevice, s));
+    ETTK_HIP(hipMemcpyAsync(S.inst.p, h_inst ? h_inst : inst.data(), nq * sizeof(SoftInst), hipMemcpyHostToDevice, s));
     S.sig.ensure((size_t)nq * n * n);
     S.pat.ensure((size_t)nq * n * n);
     k_assemble<<<dim3(16, nq), kT, 0, s>>>(S.inst.p, S.Pp1.p, d_Pp2, S.W1.p, d_W2, d_Mx, B, S.sig.p, S.pat.p);
@@ -1163,7 +1263,12 @@ void launch_combine(const double* C, const ComboOp* d_ops, int np, int B, int b0
 }
 void launch_localmax(const double* C, int np, int B, int cap, int* count, int* idx, double* val, hipStream_t s) {
     const int n = 2 * B;
-    k_localmax<<<dim3(256, np), kT, 0, s>>>(C, n, cap, count, idx, val);
+    if (n % kLmT == 0) {
+        const int nt = n / kLmT;
+        k_localmax_tiled<<<dim3(nt * nt * nt, np), kT, 0, s>>>(C, n, cap, count, idx, val);
+    } else {
+        k_localmax<<<dim3(256, np), kT, 0, s>>>(C, n, cap, count, idx, val);
+    }
     ETTK_HIP(hipGetLastError());
 }
 // block v: offset of list v = sum of the clamped counts before it; copies its entries in slot order
@@ -1171,11 +1276,11 @@ __global__ void k_lm_compact(const int* __restrict__ count, const int* __restric
                              int np, int cap, int* __restrict__ head, int* __restrict__ pidx, double* __restrict__ pval) {
     const int v = blockIdx.x;
     int off = 0;
-    for (int u = 0; u < v; u++) off += min(count[u], cap);
-    const int c = min(count[v], cap);
+    for (int u = 0; u < v; u++) off += min(count[(size_t)u * kLmCountStride], cap);
+    const int c = min(count[(size_t)v * kLmCountStride], cap);
     if (threadIdx.x == 0) {
         head[v] = off;
-        head[np + 1 + v] = count[v];
+        head[np + 1 + v] = count[(size_t)v * kLmCountStride];
         if (v == np - 1) head[np] = off + c;
     }
     for (int k = threadIdx.x; k < c; k += blockDim.x) {

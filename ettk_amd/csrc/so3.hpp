@@ -21,13 +21,19 @@ struct ComboOp {
     int i[4];      // instance indices
 };
 
-// local-maximum lists of one search chunk (count per volume, then up to `cap` indices and values per volume)
+// local-maximum lists of one search chunk (count per volume, then up to `cap` indices and values per volume); the
+// per-volume counters sit kLmCountStride ints apart, one 128-byte line each, so that appends to different
+// volumes do not serialise on one line
+constexpr int kLmCountStride = 32;
 struct LmBuffers {
     DevBuf<int> count, idx;
     DevBuf<double> val;
     // compacted lists: head = [offsets (nvol + 1), counts (nvol)], then the packed indices and values
     DevBuf<int> head, pidx;
     DevBuf<double> pval;
+    // the chunk's SOFT instances and combine operations, staged in pinned memory for asynchronous uploads
+    PinnedBuf<SoftInst> hinst;
+    PinnedBuf<ComboOp> hops;
 };
 
 struct So3State {
@@ -56,12 +62,15 @@ struct So3State {
     DevBuf<double> C, Cc;
     DevBuf<ComboOp> ops;
     std::vector<std::unique_ptr<LmBuffers>> lm_pool;  // one slot per chunk in flight
+    // host inputs of a search call (spherical wedges, reslice geometry), pinned for asynchronous uploads
+    PinnedBuf<float> hW2;
+    PinnedBuf<EvalGeom> hGeom;
 };
 
 inline LmBuffers& lm_slot(So3State& S, int slot, int nvol, int cap) {
     while ((int)S.lm_pool.size() <= slot) S.lm_pool.emplace_back(new LmBuffers());
     LmBuffers& b = *S.lm_pool[slot];
-    b.count.ensure(nvol);
+    b.count.ensure((size_t)nvol * kLmCountStride);
     b.idx.ensure((size_t)nvol * cap);
     b.val.ensure((size_t)nvol * cap);
     b.head.ensure((size_t)2 * nvol + 1);
@@ -77,8 +86,12 @@ std::vector<float> spherical_wedge_host(int B, const ettk_particle& d, bool wedg
 // has consumed it; computed here when null
 void project_images(ettk_plan* P, const float* d_parts, const ettk_particle* descs, const int* pidx, int n, bool is_ref,
                     float* d_Pp, const EvalGeom* h_geom = nullptr);
+// `h_inst` (optional): a copy of `inst` in pinned memory, kept unchanged until the stream has consumed it
 void run_soft(ettk_plan* P, const std::vector<SoftInst>& inst, const float* d_Pp2, const float* d_W2, const float* d_Mx,
-              double* d_C);
+              double* d_C, const SoftInst* h_inst = nullptr);
+// sizes every device buffer of the search for chunks of up to `cn` particles (at most five SOFT instances each),
+// so that none is reallocated (a free synchronises the device) while chunks are in flight
+void so3_search_reserve(ettk_plan* P, int cn);
 void soft_given(ettk_plan* P, int nq, const double* h_sig, const double* h_pat, double* d_C);
 // harmonics(A, t, R) (ProjRot.h:1348-1454) of n (2B x 2B) host images by the rotation parts of h_t[i*16]
 void sph_rotate(ettk_plan* P, int n, const float* h_img, const double* h_t, float* h_out);
