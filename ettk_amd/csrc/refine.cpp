@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <memory>
 
 #include "plan.hpp"
 
@@ -60,38 +61,69 @@ static void particle_spectra(ettk_plan* P, const float* d_parts, int p0, int p1)
     }
 }
 
-// Score and gradient for n evaluations (particle pidx[q], seed seeds[q], parameters x6[q]).  The particle
-// spectra of [p0, p1) must be in P->r_fp.
-static void refine_eval_batch(ettk_plan* P, const ettk_particle* descs, int n, const int* pidx, int p0,
-                              const Mat4* seeds, const double* x6, double* S, double* grad6) {
-    const int N = P->N, H = P->H;
-    const long nh = (long)N * N * H;
-    hipStream_t s = P->ctx->stream;
+// One chunk of evaluations (at most 128, the CCC workspace) of the score and gradient: its geometry and results in
+// pinned host memory (asynchronous copies), the host matrices its gradient needs, and the event that marks its
+// kernels and downloads done.  Chunks share the device workspaces in stream order; their geometry, score and
+// moment regions of the device arrays are their own.
+struct RefineChunk {
+    int e0 = 0, E = 0;
+    PinnedBuf<RefineGeom> hg;
+    PinnedBuf<double> hsc, hmom;  // 4 and 12 per evaluation
+    std::vector<Mat4> Ginv, EMs;
+    hipEvent_t done = nullptr;
+    RefineChunk() = default;
+    RefineChunk(const RefineChunk&) = delete;
+    RefineChunk& operator=(const RefineChunk&) = delete;
+    ~RefineChunk() {
+        if (done) (void)hipEventDestroy(done);
+    }
+};
+
+// device arrays for n evaluations in chunks of `cap`, and the chunks themselves
+static void refine_prepare(ettk_plan* P, int n, std::vector<std::unique_ptr<RefineChunk>>& chunks) {
+    const int N = P->N;
     ensure_ccc_workspace(P, std::min(n, 128));
-    const int cap = std::min(P->cap, 128);  // evaluations per chunk of the refinement
+    const int cap = std::min(P->cap, 128);
     const int nbd = refine_dot_blocks(N), nbm = refine_moment_blocks(N);
     P->r_pdot.ensure((size_t)3 * nbd * cap);
     P->r_mpart.ensure((size_t)12 * nbm * cap);
     P->r_sc.ensure((size_t)4 * n);
     P->r_mom.ensure((size_t)12 * n);
     P->r_geom.ensure(n);
-    // geometry of every evaluation on the host (FP64), one upload, all chunks back to back, one download
-    std::vector<RefineGeom> g(n);
-    std::vector<Mat4> Ginv(n), EMs(n);
-    for (int q = 0; q < n; q++) {
+    chunks.clear();
+    for (int e0 = 0; e0 < n; e0 += cap) {
+        chunks.emplace_back(new RefineChunk());
+        RefineChunk& c = *chunks.back();
+        c.e0 = e0;
+        c.E = std::min(cap, n - e0);
+        c.hg.ensure(c.E);
+        c.hsc.ensure((size_t)4 * c.E);
+        c.hmom.ensure((size_t)12 * c.E);
+        c.Ginv.resize(c.E);
+        c.EMs.resize(c.E);
+        ETTK_HIP(hipEventCreateWithFlags(&c.done, hipEventDisableTiming));
+    }
+}
+
+// host geometry (FP64) of the chunk's evaluations q (particle pidx[q], seed seeds[q], parameters x6[q])
+static void refine_geometry(const ettk_plan* P, const ettk_particle* descs, const int* pidx, int p0, const Mat4* seeds,
+                            const double* x6, RefineChunk& c) {
+    const int N = P->N;
+    for (int k = 0; k < c.E; k++) {
+        const int q = c.e0 + k;
         const ettk_particle& d = descs[pidx[q]];
         const Mat4 EM = euler_of(d) * Mat4::from(d.matrix);
         Mat4 fin = EM * pose(seeds[q], x6 + (size_t)q * 6);
         if (mat_det(fin) == 0.0) fin = Mat4::ident();
         const Mat4 Gi = mat_inverse(fin);
-        EMs[q] = EM;
-        Ginv[q] = Gi;
-        RefineGeom& r = g[q];
+        c.EMs[k] = EM;
+        c.Ginv[k] = Gi;
+        RefineGeom& r = c.hg.p[k];
         r = RefineGeom{};
-        const double c[3] = {N / 2.0, N / 2.0, N / 2.0 + d.cut_offset};
+        const double cc[3] = {N / 2.0, N / 2.0, N / 2.0 + d.cut_offset};
         for (int a = 0; a < 3; a++) {
             for (int b = 0; b < 3; b++) r.G[a * 4 + b] = (float)Gi(a, b);
-            r.G[a * 4 + 3] = (float)(Gi(a, 3) + c[a]);
+            r.G[a * 4 + 3] = (float)(Gi(a, 3) + cc[a]);
         }
         double n1[3], n2[3];
         r.pw = (P->p.missing_wedge && lab_normals(d.lwedge, d.uwedge, n1, n2)) ? 1 : 0;
@@ -108,55 +140,66 @@ static void refine_eval_batch(ettk_plan* P, const ettk_particle* descs, int n, c
             if (mat_det(fs) == 0.0) fs = Mat4::ident();
             const Mat4 A = mat_inverse(fs);
             for (int rr = 0; rr < 3; rr++)
-                for (int cc = 0; cc < 3; cc++) r.A[rr * 3 + cc] = (float)A(rr, cc);
+                for (int ci = 0; ci < 3; ci++) r.A[rr * 3 + ci] = (float)A(rr, ci);
             if (r.rw == 1)
-                for (int cc = 0; cc < 3; cc++) {
+                for (int ci = 0; ci < 3; ci++) {
                     double a1 = 0, a2 = 0;
                     for (int rr = 0; rr < 3; rr++) {
-                        a1 += A(rr, cc) * P->wref.v1[rr];
-                        a2 += A(rr, cc) * P->wref.v2[rr];
+                        a1 += A(rr, ci) * P->wref.v1[rr];
+                        a2 += A(rr, ci) * P->wref.v2[rr];
                     }
-                    r.u1[cc] = (float)a1;
-                    r.u2[cc] = (float)a2;
+                    r.u1[ci] = (float)a1;
+                    r.u2[ci] = (float)a2;
                 }
         }
         r.spec = pidx[q] - p0;
     }
-    ETTK_HIP(hipMemcpyAsync(P->r_geom.p, g.data(), (size_t)n * sizeof(RefineGeom), hipMemcpyHostToDevice, s));
+}
+
+// upload of the chunk's geometry, its kernels and the download of its scores and moments; returns at once.  The
+// particle spectra of the referenced particles must be in P->r_fp.
+static void refine_enqueue(ettk_plan* P, RefineChunk& c) {
+    const int N = P->N, H = P->H;
+    const long nh = (long)N * N * H;
+    hipStream_t s = P->ctx->stream;
     KernelProf* kp = P->ctx->fft.prof;
-    for (int e0 = 0; e0 < n; e0 += cap) {
-        const int E = std::min(cap, n - e0);
-        const RefineGeom* geo = P->r_geom.p + e0;
-        const double Rb = (double)E * N * N * N * sizeof(float), Hb = (double)E * nh * sizeof(float2);
-        {
-            ProfScope ps(kp, "k_reslice_ref", Rb, s);
-            launch_reslice_ref(P->ref_x.p, geo, E, N, P->R2.p, s);
-        }
-        R2CArgs a{P->R2.p, P->S2.p, N, E, PRO_MASK_SHIFT, P->maskfft.p, nullptr};
-        fft3_r2c(P->ctx->fft, a, nullptr, s);
-        double* sc = P->r_sc.p + (size_t)4 * e0;
-        {
-            ProfScope ps(kp, "k_refine_dot", 2 * Hb, s);
-            launch_refine_dot(P->S2.p, P->r_fp.p, P->filter.p, P->ref_wbin.p, geo, E, N, P->r_pdot.p, sc, s);
-        }
-        {
-            ProfScope ps(kp, "k_refine_adj", 3 * Hb, s);
-            launch_refine_adj(P->S2.p, P->r_fp.p, P->filter.p, P->ref_wbin.p, geo, E, N, sc, s);
-        }
-        fft3_c2r(P->ctx->fft, P->S2.p, P->S3.p, P->R1.p, N, E, s);
-        {
-            ProfScope ps(kp, "k_refine_moments", Rb, s);
-            launch_refine_moments(P->ref_x.p, geo, P->R1.p, P->maskfft.p, E, N, P->r_mpart.p,
-                                  P->r_mom.p + (size_t)12 * e0, s);
-        }
+    const int E = c.E, e0 = c.e0;
+    RefineGeom* geo = P->r_geom.p + e0;
+    ETTK_HIP(hipMemcpyAsync(geo, c.hg.p, (size_t)E * sizeof(RefineGeom), hipMemcpyHostToDevice, s));
+    const double Rb = (double)E * N * N * N * sizeof(float), Hb = (double)E * nh * sizeof(float2);
+    {
+        ProfScope ps(kp, "k_reslice_ref", Rb, s);
+        launch_reslice_ref(P->ref_x.p, geo, E, N, P->R2.p, s);
     }
-    std::vector<double> sc((size_t)4 * n), mom((size_t)12 * n);
-    ETTK_HIP(hipMemcpyAsync(sc.data(), P->r_sc.p, (size_t)4 * n * sizeof(double), hipMemcpyDeviceToHost, s));
-    ETTK_HIP(hipMemcpyAsync(mom.data(), P->r_mom.p, (size_t)12 * n * sizeof(double), hipMemcpyDeviceToHost, s));
-    ETTK_HIP(hipStreamSynchronize(s));
-    for (int q = 0; q < n; q++) {
+    R2CArgs a{P->R2.p, P->S2.p, N, E, PRO_MASK_SHIFT, P->maskfft.p, nullptr};
+    fft3_r2c(P->ctx->fft, a, nullptr, s);
+    double* sc = P->r_sc.p + (size_t)4 * e0;
+    {
+        ProfScope ps(kp, "k_refine_dot", 2 * Hb, s);
+        launch_refine_dot(P->S2.p, P->r_fp.p, P->filter.p, P->ref_wbin.p, geo, E, N, P->r_pdot.p, sc, s);
+    }
+    {
+        ProfScope ps(kp, "k_refine_adj", 3 * Hb, s);
+        launch_refine_adj(P->S2.p, P->r_fp.p, P->filter.p, P->ref_wbin.p, geo, E, N, sc, s);
+    }
+    fft3_c2r(P->ctx->fft, P->S2.p, P->S3.p, P->R1.p, N, E, s);
+    double* mom = P->r_mom.p + (size_t)12 * e0;
+    {
+        ProfScope ps(kp, "k_refine_moments", Rb, s);
+        launch_refine_moments(P->ref_x.p, geo, P->R1.p, P->maskfft.p, E, N, P->r_mpart.p, mom, s);
+    }
+    ETTK_HIP(hipMemcpyAsync(c.hsc.p, sc, (size_t)4 * E * sizeof(double), hipMemcpyDeviceToHost, s));
+    ETTK_HIP(hipMemcpyAsync(c.hmom.p, mom, (size_t)12 * E * sizeof(double), hipMemcpyDeviceToHost, s));
+    ETTK_HIP(hipEventRecord(c.done, s));
+}
+
+// score S[q] and gradient grad6[q * 6] of the chunk's evaluations once its kernels are done
+static void refine_finish(RefineChunk& c, const Mat4* seeds, const double* x6, double* S, double* grad6) {
+    ETTK_HIP(hipEventSynchronize(c.done));
+    for (int k = 0; k < c.E; k++) {
+        const int q = c.e0 + k;
         const double* x = x6 + (size_t)q * 6;
-        S[q] = sc[(size_t)q * 4];
+        S[q] = c.hsc.p[(size_t)k * 4];
         const Mat4 R0 = strip_translation(seeds[q]);
         const Mat4 Rx = mat_rot(x[0], 1, 0, 0), Ry = mat_rot(x[1], 0, 1, 0), Rz = mat_rot(x[2], 0, 0, 1);
         Mat4 dt[6];
@@ -173,46 +216,83 @@ static void refine_eval_batch(ettk_plan* P, const ettk_particle* descs, int n, c
         }
         for (int r = 0; r < 6; r++) {
             // d(fin^-1) = -fin^-1 dfin fin^-1
-            const Mat4 dG = Ginv[q] * (EMs[q] * dt[r]) * Ginv[q];
+            const Mat4 dG = c.Ginv[k] * (c.EMs[k] * dt[r]) * c.Ginv[k];
             double acc = 0;
             for (int aa = 0; aa < 3; aa++)
-                for (int b = 0; b < 4; b++) acc -= dG(aa, b) * mom[(size_t)q * 12 + aa * 4 + b];
+                for (int b = 0; b < 4; b++) acc -= dG(aa, b) * c.hmom.p[(size_t)k * 12 + aa * 4 + b];
             grad6[(size_t)q * 6 + r] = acc;
         }
     }
 }
 
+// Score and gradient for n evaluations (particle pidx[q], seed seeds[q], parameters x6[q]).  The particle
+// spectra of the referenced particles must be in P->r_fp (slot = particle - p0).
+static void refine_eval_batch(ettk_plan* P, const ettk_particle* descs, int n, const int* pidx, int p0,
+                              const Mat4* seeds, const double* x6, double* S, double* grad6) {
+    std::vector<std::unique_ptr<RefineChunk>> chunks;
+    refine_prepare(P, n, chunks);
+    for (auto& c : chunks) {
+        refine_geometry(P, descs, pidx, p0, seeds, x6, *c);
+        refine_enqueue(P, *c);
+    }
+    for (auto& c : chunks) refine_finish(*c, seeds, x6, S, grad6);
+}
+
 // Lock-step gradient ascent for every (particle, seed) job, n_iter evaluations per job: a step of s voxels
 // (start 0.5) along the gradient in the metric where one degree counts as c = (N/4) pi/180 voxels of
-// displacement; an improving step is accepted and s grows by 1.25, otherwise s halves.
+// displacement; an improving step is accepted and s grows by 1.25, otherwise s halves.  Jobs are independent:
+// each chunk of jobs advances on its own, round robin, so the host finishes one chunk's step and enqueues its next
+// evaluation while the GPU evaluates the other chunks.
 static void refine_jobs(ettk_plan* P, const ettk_particle* descs, int n, const int* pidx, int p0, const Mat4* seeds,
                         int n_iter, Mat4* out_pose, double* out_score) {
-    const double c = P->N / 4.0 * M_PI / 180.0;
+    const double cm = P->N / 4.0 * M_PI / 180.0;
     std::vector<double> x((size_t)6 * n, 0.0), xt((size_t)6 * n, 0.0), g((size_t)6 * n), gt((size_t)6 * n);
     std::vector<double> Sc(n), St(n), step(n, 0.5);
-    refine_eval_batch(P, descs, n, pidx, p0, seeds, x.data(), Sc.data(), g.data());
-    for (int it = 1; it < n_iter; it++) {
-        for (int q = 0; q < n; q++) {
-            double dir[6], nrm = 0;
-            for (int r = 0; r < 6; r++) {
-                dir[r] = r < 3 ? g[q * 6 + r] / (c * c) : g[q * 6 + r];
-                nrm += r < 3 ? c * c * dir[r] * dir[r] : dir[r] * dir[r];
-            }
-            nrm = std::sqrt(nrm);
-            for (int r = 0; r < 6; r++) xt[q * 6 + r] = x[q * 6 + r] + (nrm > 0 ? step[q] * dir[r] / nrm : 0.0);
-        }
-        refine_eval_batch(P, descs, n, pidx, p0, seeds, xt.data(), St.data(), gt.data());
-        for (int q = 0; q < n; q++) {
-            if (St[q] > Sc[q]) {
-                Sc[q] = St[q];
-                for (int r = 0; r < 6; r++) {
-                    x[q * 6 + r] = xt[q * 6 + r];
-                    g[q * 6 + r] = gt[q * 6 + r];
-                }
-                step[q] *= 1.25;
+    std::vector<std::unique_ptr<RefineChunk>> chunks;
+    refine_prepare(P, n, chunks);
+    for (auto& c : chunks) {
+        refine_geometry(P, descs, pidx, p0, seeds, x.data(), *c);
+        refine_enqueue(P, *c);
+    }
+    std::vector<int> evals(chunks.size(), 0);
+    size_t active = chunks.size();
+    while (active > 0) {
+        for (size_t ci = 0; ci < chunks.size(); ci++) {
+            if (evals[ci] >= n_iter) continue;
+            RefineChunk& c = *chunks[ci];
+            const int q0 = c.e0, q1 = c.e0 + c.E;
+            if (evals[ci] == 0) {
+                refine_finish(c, seeds, x.data(), Sc.data(), g.data());
             } else {
-                step[q] *= 0.5;
+                refine_finish(c, seeds, xt.data(), St.data(), gt.data());
+                for (int q = q0; q < q1; q++) {
+                    if (St[q] > Sc[q]) {
+                        Sc[q] = St[q];
+                        for (int r = 0; r < 6; r++) {
+                            x[q * 6 + r] = xt[q * 6 + r];
+                            g[q * 6 + r] = gt[q * 6 + r];
+                        }
+                        step[q] *= 1.25;
+                    } else {
+                        step[q] *= 0.5;
+                    }
+                }
             }
+            if (++evals[ci] >= n_iter) {
+                active--;
+                continue;
+            }
+            for (int q = q0; q < q1; q++) {
+                double dir[6], nrm = 0;
+                for (int r = 0; r < 6; r++) {
+                    dir[r] = r < 3 ? g[q * 6 + r] / (cm * cm) : g[q * 6 + r];
+                    nrm += r < 3 ? cm * cm * dir[r] * dir[r] : dir[r] * dir[r];
+                }
+                nrm = std::sqrt(nrm);
+                for (int r = 0; r < 6; r++) xt[q * 6 + r] = x[q * 6 + r] + (nrm > 0 ? step[q] * dir[r] / nrm : 0.0);
+            }
+            refine_geometry(P, descs, pidx, p0, seeds, xt.data(), c);
+            refine_enqueue(P, c);
         }
     }
     for (int q = 0; q < n; q++) {
