@@ -387,30 +387,35 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
             if (kp) kp->add_host("ccc_batch_wall", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
         }
     } wall_rec{P->ctx->fft.prof, tw0};
-    // geometry of every evaluation up front (one upload), then all chunks back to back without host waits;
-    // one download of the results at the end
-    std::vector<EvalGeom> g(n);
-    auto geom_of = [&](int e) { g[e] = make_geom_public(P, descs[pidx[e]], t[e], pidx[e]); };
-    const int nthr = std::max(1, std::min<int>(n / 256, host_threads()));
-    if (nthr <= 1) {
-        for (int e = 0; e < n; e++) geom_of(e);
-    } else {
-        std::vector<std::thread> th;
-        for (int w = 0; w < nthr; w++)
-            th.emplace_back([&, w] {
-                for (int e = w; e < n; e += nthr) geom_of(e);
-            });
-        for (auto& x : th) x.join();
-    }
+    // geometry of each chunk computed on the host just before its kernels are enqueued, in pinned memory (so the
+    // upload does not wait for the earlier chunks' kernels, which run meanwhile); one download of the results at
+    // the end
     P->geom_all.ensure(n);
     P->out_all.ensure((size_t)4 * n);
-    ETTK_HIP(hipMemcpyAsync(P->geom_all.p, g.data(), (size_t)n * sizeof(EvalGeom), hipMemcpyHostToDevice, s));
+    P->h_geom.ensure(n);
+    EvalGeom* g = P->h_geom.p;
+    auto geom_chunk = [&](int e0, int E) {
+        const int nthr = std::max(1, std::min<int>(E / 64, 4));
+        auto geom_of = [&](int e) { g[e] = make_geom_public(P, descs[pidx[e]], t[e], pidx[e]); };
+        if (nthr <= 1) {
+            for (int e = e0; e < e0 + E; e++) geom_of(e);
+        } else {
+            std::vector<std::thread> th;
+            for (int w = 0; w < nthr; w++)
+                th.emplace_back([&, w] {
+                    for (int e = e0 + w; e < e0 + E; e += nthr) geom_of(e);
+                });
+            for (auto& x : th) x.join();
+        }
+        ETTK_HIP(hipMemcpyAsync(P->geom_all.p + e0, g + e0, (size_t)E * sizeof(EvalGeom), hipMemcpyHostToDevice, s));
+    };
     ETTK_HIP(hipEventRecord(P->ev0, s));
     const float* W1 = P->p.missing_wedge ? P->W1.p : nullptr;
     const int apply = P->dummy_filter ? 0 : 1;
     KernelProf* kp = P->ctx->fft.prof;
     for (int e0 = 0; e0 < n; e0 += chunk) {
         const int E = std::min(chunk, n - e0);
+        geom_chunk(e0, E);
         const EvalGeom* geo = P->geom_all.p + e0;
         const double Rb = (double)E * N * N * N * sizeof(float), Hb = (double)E * nh * sizeof(float2);
         const double Gb = (double)E * P->z.E * N * H * sizeof(float2);

@@ -76,6 +76,7 @@ struct ettk_plan {
     ettk::DevBuf<int> bidx;
     ettk::DevBuf<float2> S1, S2, S3, G1;
     ettk::DevBuf<ettk::EvalGeom> geom, geom_all;
+    ettk::PinnedBuf<ettk::EvalGeom> h_geom;  // host geometry of a CCC batch (asynchronous uploads)
     ettk::DevBuf<float> out_all;
     std::vector<float> h_out4;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
