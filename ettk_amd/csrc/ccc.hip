@@ -51,11 +51,13 @@ __device__ __forceinline__ void keys_w(float f, float* w) {  // vtkTricubicInter
 // with the volume's border replicated where it reaches outside (the taps clamp to the border), is staged in
 // LDS at fixed strides, so every tap is a plain LDS read and the 4 taps of a row are consecutive words.  A
 // non-rigid map whose box does not fit reads the clamped taps from global memory instead.
-static constexpr int kTile = 8, kBox = 18;
+// LDS strides of the staged box: odd row and plane strides (19, 343 = 23 mod 64) spread the taps of a
+// wavefront's voxels over the LDS banks better than the packed 18, 324 (= 4 mod 64)
+static constexpr int kTile = 8, kBox = 18, kSY = kBox + 1, kSZ = kSY * kBox + 1;
 __global__ void __launch_bounds__(kThreads) k_reslice(const float* __restrict__ parts, const EvalGeom* __restrict__ geom,
                                                        int N, float* __restrict__ out, float* __restrict__ partials) {
     __shared__ float sh[8];
-    __shared__ float box[kBox * kBox * kBox];
+    __shared__ float box[kBox * kSZ];
     int e, tb;
     xcd_eval_tile(blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, e, tb);
     const EvalGeom g = geom[e];
@@ -92,7 +94,7 @@ __global__ void __launch_bounds__(kThreads) k_reslice(const float* __restrict__ 
                 const float* src = vol + (size_t)gy * N + gx;
                 for (int z = 0; z < kBox; z++) {
                     const int gz = min(max(lo[2] + z, 0), N - 1);
-                    box[(z * kBox + y) * kBox + x] = src[(size_t)gz * N * N];
+                    box[z * kSZ + y * kSY + x] = src[(size_t)gz * N * N];
                 }
             }
         }
@@ -115,13 +117,13 @@ __global__ void __launch_bounds__(kThreads) k_reslice(const float* __restrict__ 
             keys_w(zz - iz, wz);
             float acc = 0.f;
             if (fits) {
-                const float* b0 = box + ((iz - 1 - lo[2]) * kBox + (iy - 1 - lo[1])) * kBox + (ix - 1 - lo[0]);
+                const float* b0 = box + (iz - 1 - lo[2]) * kSZ + (iy - 1 - lo[1]) * kSY + (ix - 1 - lo[0]);
 #pragma unroll
                 for (int a = 0; a < 4; a++) {
                     float ay = 0.f;
 #pragma unroll
                     for (int b = 0; b < 4; b++) {
-                        const float* row = b0 + (a * kBox + b) * kBox;
+                        const float* row = b0 + a * kSZ + b * kSY;
                         ay += wy[b] * (wx[0] * row[0] + wx[1] * row[1] + wx[2] * row[2] + wx[3] * row[3]);
                     }
                     acc += wz[a] * ay;
