@@ -208,49 +208,11 @@ void launch_wedge_ref(const WedgeRef& w, int N, float* W, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ spectral filter
-__global__ void __launch_bounds__(kThreads) k_spec_filter(const float2* __restrict__ F, int f_stride, float2* __restrict__ X,
-                                                           const float* __restrict__ filter, const float* __restrict__ W1,
-                                                           const EvalGeom* __restrict__ geom, int N, int mcf, int apply,
-                                                           float* __restrict__ partials, float* __restrict__ wf_partials) {
-    __shared__ float sh[8];
-    const int e = blockIdx.y;
-    const int H = N / 2 + 1;
-    const long nh = (long)N * N * H;
-    const EvalGeom& g = geom[e];
-    const long chunk = (nh + gridDim.x - 1) / gridDim.x;
-    const long h0 = blockIdx.x * chunk, h1 = min(nh, h0 + chunk);
-    const long dc = ((long)(N / 2) * N + N / 2) * H + (H - 1);
-    float s = 0.f, swf = 0.f;
-    for (long h = h0 + threadIdx.x; h < h1; h += blockDim.x) {
-        int i, j, k;
-        split3((unsigned)h, (unsigned)N, (unsigned)H, i, j, k);
-        float2 z = F[(size_t)e * f_stride + h];
-        if (mcf) {
-            float m2 = z.x * z.x + z.y * z.y;
-            float C = sqrtf(sqrtf(m2));
-            if (C > 0.f) z = make_float2(z.x / C, z.y / C);
-        }
-        float w = W1 ? W1[h] : 1.f;
-        if (!g.full) w *= wedge_smooth(g.v1, g.v2, i, j, k, N, H);
-        float fw = filter[h] * w;
-        if (apply) z = cscale(z, fw);
-        if (h == dc) z = make_float2(0.f, 0.f);
-        X[(size_t)e * nh + h] = z;
-        s += (k < H - 1 ? 2.f : 1.f) * (z.x * z.x + z.y * z.y);
-        swf += fw;
-    }
-    float ts = block_sum(s, sh);
-    float tw = block_sum(swf, sh);
-    if (threadIdx.x == 0) {
-        partials[(size_t)e * gridDim.x + blockIdx.x] = ts;
-        if (wf_partials) wf_partials[(size_t)e * gridDim.x + blockIdx.x] = tw;
-    }
-}
-
 // both sides of one evaluation in one pass, the smoothed wedge computed once per element:
 //   X1 = F1 * filter * W1 W2(e)            (reference side, F1 shared by all evaluations; n1 and sum(W filter))
 //   X2 = mcf(F2[e]) * filter * W1 W2(e)    (particle side, in place; n2)
-// the per-element arithmetic and the per-block partials are those of two k_spec_filter launches
+// (nbfFourierFilter::executeHalf, FFilt.h:587-606, and the Hermitian weighted sums of normalizeFourierHalf,
+// CorrMetric.h:498-530)
 __global__ void __launch_bounds__(kThreads) k_spec_filter2(const float2* __restrict__ F1, float2* __restrict__ X1,
                                                             float2* __restrict__ X2, const float* __restrict__ filter,
                                                             const float* __restrict__ W1, const EvalGeom* __restrict__ geom,
@@ -303,15 +265,6 @@ void launch_spec_filter2(const float2* F1, float2* X1, float2* X2, const float* 
                          float* p_n2, int nb, hipStream_t s) {
     dim3 grid(nb, n_eval);
     k_spec_filter2<<<grid, kThreads, 0, s>>>(F1, X1, X2, filter, W1, geom, N, mcf, apply, p_n1, p_wf, p_n2);
-    ETTK_HIP(hipGetLastError());
-}
-
-void launch_spec_filter(const float2* F, int f_batch_stride, float2* X, const float* filter, const float* W1,
-                        const EvalGeom* geom, int n_eval, int N, int mcf, int apply_filter, float* partials,
-                        float* wf_partials, int nb, hipStream_t s) {
-    dim3 grid(nb, n_eval);
-    k_spec_filter<<<grid, kThreads, 0, s>>>(F, f_batch_stride, X, filter, W1, geom, N, mcf, apply_filter, partials,
-                                            wf_partials);
     ETTK_HIP(hipGetLastError());
 }
 

@@ -89,13 +89,8 @@ void launch_reslice(const float* parts, const EvalGeom* geom, int n_eval, int N,
                     int nb, hipStream_t s);
 // mean of the valid voxels per evaluation, from the reslice partials (nb per eval): stats[e] = mean
 void launch_reslice_mean(const float* partials, int nb, int n_eval, float* stats, hipStream_t s);
-// spectral filter of one side: X = (mcf ? F/|F|^.5 : F) * filter * W1 * W2(e); DC = 0; writes Hermitian
-// weighted |X|^2 partial sums (and sum(W*filter) when `wf_partials`)
-void launch_spec_filter(const float2* F, int f_batch_stride, float2* X, const float* filter, const float* W1,
-                        const EvalGeom* geom, int n_eval, int N, int mcf, int apply_filter, float* partials,
-                        float* wf_partials, int nb, hipStream_t s);
-// both sides at once: X1 = F1 filter W1 W2(e) (F1 shared), X2 = mcf(X2) filter W1 W2(e) in place; partials of
-// |X1|^2, sum(W filter), |X2|^2 exactly as two launch_spec_filter calls produce them
+// spectral weighting of both CCC sides: X1 = F1 filter W1 W2(e) (F1 shared), X2 = mcf(X2) filter W1 W2(e) in
+// place (mcf: F / |F|^.5), DC = 0; per-block partials of the Hermitian weighted |X1|^2, sum(W filter), |X2|^2
 void launch_spec_filter2(const float2* F1, float2* X1, float2* X2, const float* filter, const float* W1,
                          const EvalGeom* geom, int n_eval, int N, int mcf, int apply, float* p_n1, float* p_wf,
                          float* p_n2, int nb, hipStream_t s);
