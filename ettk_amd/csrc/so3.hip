@@ -559,8 +559,11 @@ __device__ __forceinline__ void fft_lds_f64(double2* X, int es, int vs, const do
 // Synthesis of one beta slice of one instance and its two DFTs, all in LDS (n = 2B <= 64):
 //   T1[m1][m2] = sum_l sig(l,-m1) (-1)^(m1+m2) conj(pat(l,-m2)) sqrt((2l+1)/8pi^2) d^l_{m1m2}(beta)
 //   C[q][alpha][beta][gamma] = Re sum_{m1,m2} T1[m1][m2] e^{-i m1 alpha} e^{-i m2 gamma}
+// 512 threads per workgroup: the n x (n+1) FP64 LDS tile limits a CU to two workgroups, so the wavefronts that hide
+// the synthesis's load latency come from the workgroup size
+static constexpr int kSynT = 512;
 template <int LOGN>
-__global__ void __launch_bounds__(kT) k_so3_synth_fft(const double2* __restrict__ cs, const double2* __restrict__ cp,
+__global__ void __launch_bounds__(kSynT) k_so3_synth_fft(const double2* __restrict__ cs, const double2* __restrict__ cp,
                                                        const double* __restrict__ dtab2, int B,
                                                        const double2* __restrict__ tw, double* __restrict__ C) {
     constexpr int n = 1 << LOGN, LD = n + 1;
@@ -1087,9 +1090,9 @@ void soft_core(ettk_plan* P, const std::vector<SoftInst>& inst, double* d_C) {
         ProfScope ps(P->ctx->fft.prof, "k_so3_synth_fft", (double)nq * ((double)n3 * 8 + 32.0 * B * B), s);
         const size_t shm = (size_t)n * (n + 1) * sizeof(double2);
         switch (n) {
-            case 16: k_so3_synth_fft<4><<<dim3(nq, n), kT, shm, s>>>(S.cs.p, S.cp.p, S.dtab2.p, B, S.tw.p, d_C); break;
-            case 32: k_so3_synth_fft<5><<<dim3(nq, n), kT, shm, s>>>(S.cs.p, S.cp.p, S.dtab2.p, B, S.tw.p, d_C); break;
-            default: k_so3_synth_fft<6><<<dim3(nq, n), kT, shm, s>>>(S.cs.p, S.cp.p, S.dtab2.p, B, S.tw.p, d_C); break;
+            case 16: k_so3_synth_fft<4><<<dim3(nq, n), kSynT, shm, s>>>(S.cs.p, S.cp.p, S.dtab2.p, B, S.tw.p, d_C); break;
+            case 32: k_so3_synth_fft<5><<<dim3(nq, n), kSynT, shm, s>>>(S.cs.p, S.cp.p, S.dtab2.p, B, S.tw.p, d_C); break;
+            default: k_so3_synth_fft<6><<<dim3(nq, n), kSynT, shm, s>>>(S.cs.p, S.cp.p, S.dtab2.p, B, S.tw.p, d_C); break;
         }
     } else {
         S.T1.ensure((size_t)nq * n3);
