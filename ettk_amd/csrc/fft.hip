@@ -838,6 +838,64 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_r2c_plane(float2* __r
     for (int t = threadIdx.x; t < L * H; t += blockDim.x) ip[t] = r2c_half_at<L>(buf, t / H, t % H);
 }
 
+// ------------------------------------------------------------------ combine + first DFT axis by FFT (N = 64)
+// The first axis of the zero-padded 2N-point inverse transform on the search window (k_combine_dftA's table
+// A[x][i] = (-1)^f e^{+i pi f u / N}, f = i - N/2, u = I0 + x - N) as two length-N FFTs per pencil: with
+// u = 2m + b, G(u) = sum_f [(-1)^f C(f) e^{+2 pi i f b / 2N}] e^{+2 pi i f m / N}, a backward N-point transform of
+// the twiddled pencil (f periodic mod N) for each parity b, read at m mod N.  One workgroup per (j, evaluation):
+// 2H pencils (k, b) of 8 threads, each a length-64 register FFT.
+static constexpr int kCfP = 2 * 33, kCfT = kCfP * 8;
+__global__ void __launch_bounds__(kCfT) k_combine_fftA(const float2* __restrict__ F1m, const float2* __restrict__ F2,
+                                                       const float2* __restrict__ Q, const float2* __restrict__ Mfft,
+                                                       const float* __restrict__ scalars, CccSizes z,
+                                                       float2* __restrict__ G1, const float2* __restrict__ tw64,
+                                                       const float2* __restrict__ tw128) {
+    constexpr int L = 64, H = L / 2 + 1, LD = PenT<64>::LD;
+    __shared__ float2 pens[kCfP * LD];
+    __shared__ float2 tw[L];
+    __shared__ float2 wb[L];  // e^{+2 pi i (i - L/2) / 2L} for i = 0 .. L-1
+    const int j = blockIdx.x, e = blockIdx.y, E = z.E;
+    const long nh = (long)L * L * H;
+    if (threadIdx.x < L) {
+        tw[threadIdx.x] = tw64[threadIdx.x];
+        wb[threadIdx.x] = cconj(tw128[(threadIdx.x - L / 2 + 2 * L) % (2 * L)]);
+    }
+    const float n1 = scalars[e * 4 + 0], n2 = scalars[e * 4 + 1];
+    const float n3 = (float)L * L * L;
+    const float a = 2.f * n3 / (n1 * n2);
+    const float b = 1.f / (n2 * n2);
+    __syncthreads();
+    struct Quad {
+        float2 f1, f2, q, m;
+    };
+    const float2* __restrict__ f1p = F1m + (size_t)e * nh;
+    const float2* __restrict__ f2p = F2 + (size_t)e * nh;
+    const float2* __restrict__ qp = Q + (size_t)e * nh;
+    block_copy_rt<4>(
+        L * H,
+        [&](int t) {
+            const long h = ((long)(t / H) * L + j) * H + t % H;
+            return Quad{f1p[h], f2p[h], qp[h], Mfft[h]};
+        },
+        [&](int t, const Quad& v) {
+            const int i = t / H, k = t % H;
+            float2 c = csub(cscale(cmulc(v.f1, v.f2), a), cscale(cmulc(v.m, v.q), b));
+            if ((i + j + k) & 1) c = make_float2(-c.x, -c.y);
+            if ((i - L / 2) & 1) c = make_float2(-c.x, -c.y);  // (-1)^f of the table
+            const int n = (i + L / 2) & (L - 1);            // f mod L
+            pens[k * LD + n] = c;
+            pens[(H + k) * LD + n] = cmul(c, wb[i]);
+        });
+    __syncthreads();
+    const int q = threadIdx.x >> 3, lane = threadIdx.x & 7;
+    fft64_reg(&pens[q * LD], lane, tw_cache<64>(tw, lane), true);
+    const int u0 = z.I0 - L;
+    for (int t = threadIdx.x; t < E * H; t += kCfT) {
+        const int x = t / H, k = t % H, u = u0 + x, par = u & 1, m = ((u - par) / 2) & (L - 1);
+        G1[(((size_t)e * E + x) * L + j) * H + k] = pens[(par * H + k) * LD + m];
+    }
+}
+
 // ------------------------------------------------------------------ host launchers
 // profile record names: k_r2c_rows<L>, k_fft_strided<L>, k_c2r_rows<L>
 template <int L>
@@ -939,6 +997,16 @@ static void launch_c2r_op_r2c(const FftTables& T, const float2* in, float2* io, 
 }
 
 bool fft3_fused_supported(int n) { return n == 16 || n == 32 || n == 48 || n == 64; }
+
+bool ccc_combine_fft_supported(const CccSizes& z) { return z.N == 64; }
+void launch_combine_fftA(const FftTables& T, const float2* F1m, const float2* F2, const float2* Q, const float2* Mfft,
+                         const float* scalars, const CccSizes& z, int n_eval, float2* G1, hipStream_t s) {
+    ETTK_REQUIRE(ccc_combine_fft_supported(z), ETTK_ERR_UNSUPPORTED, "launch_combine_fftA: box size must be 64");
+    const float2* tw64 = T.tw.p + T.offset[fft_len_index(64)];
+    const float2* tw128 = T.tw.p + T.offset[fft_len_index(128)];
+    k_combine_fftA<<<dim3(z.N, n_eval), kCfT, 0, s>>>(F1m, F2, Q, Mfft, scalars, z, G1, tw64, tw128);
+    ETTK_HIP(hipGetLastError());
+}
 
 template <int L>
 static int ccc_ipass_blocks_t() {

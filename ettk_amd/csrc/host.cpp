@@ -465,7 +465,10 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
                            P->scalars.p, s);
         {
             ProfScope ps(kp, "k_combine_dftA", 3 * Hb + Gb, s);
-            launch_combine_dftA(P->S1.p, P->S2.p, P->S3.p, P->Mfft.p, P->scalars.p, P->tabA.p, P->z, E, P->G1.p, s);
+            if (ccc_combine_fft_supported(P->z))
+                launch_combine_fftA(P->ctx->fft, P->S1.p, P->S2.p, P->S3.p, P->Mfft.p, P->scalars.p, P->z, E, P->G1.p, s);
+            else
+                launch_combine_dftA(P->S1.p, P->S2.p, P->S3.p, P->Mfft.p, P->scalars.p, P->tabA.p, P->z, E, P->G1.p, s);
         }
         {
             ProfScope ps(kp, "k_dftBC_argmax", Gb, s);
