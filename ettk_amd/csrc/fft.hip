@@ -9,6 +9,7 @@
 #include "kernels.hpp"
 #include "wedge.hpp"
 
+#include <cfloat>
 #include <cmath>
 
 namespace ettk {
@@ -896,6 +897,87 @@ __global__ void __launch_bounds__(kCfT) k_combine_fftA(const float2* __restrict_
     }
 }
 
+// The second axis the same way (k_dftBC_argmax's A along j), then the third axis on the crop, the spherical mask
+// and the first maximum as k_dftBC_argmax computes them.  One workgroup per (x, evaluation).
+__global__ void __launch_bounds__(kCfT) k_dftBC_fft(const float2* __restrict__ G1, const float2* __restrict__ tabK,
+                                                    CccSizes z, float* __restrict__ best_val, int* __restrict__ best_idx,
+                                                    const float2* __restrict__ tw64, const float2* __restrict__ tw128) {
+    constexpr int L = 64, H = L / 2 + 1, LD = PenT<64>::LD;
+    extern __shared__ float2 K[];  // [zz][k], E x H
+    __shared__ float2 pens[kCfP * LD];
+    __shared__ float2 tw[L];
+    __shared__ float2 wb[L];
+    __shared__ float sv[kCfT];
+    __shared__ int si[kCfT];
+    const int x = blockIdx.x, e = blockIdx.y, E = z.E;
+    if (threadIdx.x < L) {
+        tw[threadIdx.x] = tw64[threadIdx.x];
+        wb[threadIdx.x] = cconj(tw128[(threadIdx.x - L / 2 + 2 * L) % (2 * L)]);
+    }
+    __syncthreads();
+    const float2* src = G1 + ((size_t)e * E + x) * L * H;  // [j][k]
+    block_copy_rt<4>(
+        L * H, [&](int t) { return src[t]; },
+        [&](int t, float2 c) {
+            const int jj = t / H, k = t % H;
+            if ((jj - L / 2) & 1) c = make_float2(-c.x, -c.y);
+            const int n = (jj + L / 2) & (L - 1);
+            pens[k * LD + n] = c;
+            pens[(H + k) * LD + n] = cmul(c, wb[jj]);
+        });
+    block_copy_rt<3>(E * H, [&](int t) { return tabK[t]; }, [&](int t, float2 v) { K[t] = v; });
+    __syncthreads();
+    const int q = threadIdx.x >> 3, lane = threadIdx.x & 7;
+    fft64_reg(&pens[q * LD], lane, tw_cache<64>(tw, lane), true);
+    // g2(y, k) = pens[(par H + k) LD + m] with u = I0 + y - L = 2m + par
+    const float scale = 8.f / ((float)(2 * L) * (float)(2 * L) * (float)(2 * L));
+    float bv = -FLT_MAX;
+    int bi = 0x7fffffff;
+    const int dx = z.ub2 - x, u0 = z.I0 - L;
+    for (int t = threadIdx.x; t < E * E; t += kCfT) {
+        const int y = t / E, zz = t % E;
+        const int dy = z.ub2 - y, dz = z.ub2 - zz;
+        const float rad = (float)(dx * dx + dy * dy + dz * dz);
+        if (!(rad < z.restr2)) continue;
+        const int u = u0 + y, par = u & 1, m = ((u - par) / 2) & (L - 1);
+        const float2* gy = &pens[par * H * LD + m];
+        float acc = 0.f;
+        for (int k = 0; k < H; k++) {
+            const float2 p = gy[k * LD], qv = K[zz * H + k];
+            acc = fmaf(p.x, qv.x, acc);
+            acc = fmaf(-p.y, qv.y, acc);
+        }
+        acc *= scale;
+        if (acc > bv || (acc == bv && t < bi)) {
+            bv = acc;
+            bi = t;
+        }
+    }
+    // first maximum over the workgroup (value descending, then index ascending: a total order, so the halving
+    // tree over any count gives the same result as a scan)
+    sv[threadIdx.x] = bv;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int cnt = kCfT; cnt > 1;) {
+        const int st = (cnt + 1) / 2;
+        if ((int)threadIdx.x < cnt - st) {
+            const int r = threadIdx.x;
+            const float v2 = sv[r + st];
+            const int i2 = si[r + st];
+            if (v2 > sv[r] || (v2 == sv[r] && i2 < si[r])) {
+                sv[r] = v2;
+                si[r] = i2;
+            }
+        }
+        __syncthreads();
+        cnt = st;
+    }
+    if (threadIdx.x == 0) {
+        best_val[(size_t)e * E + x] = sv[0];
+        best_idx[(size_t)e * E + x] = si[0];
+    }
+}
+
 // ------------------------------------------------------------------ host launchers
 // profile record names: k_r2c_rows<L>, k_fft_strided<L>, k_c2r_rows<L>
 template <int L>
@@ -999,6 +1081,20 @@ static void launch_c2r_op_r2c(const FftTables& T, const float2* in, float2* io, 
 bool fft3_fused_supported(int n) { return n == 16 || n == 32 || n == 48 || n == 64; }
 
 bool ccc_combine_fft_supported(const CccSizes& z) { return z.N == 64; }
+void launch_dftBC_fft(const FftTables& T, const float2* G1, const float2* tabK, const CccSizes& z, int n_eval,
+                      float* best_val, int* best_idx, hipStream_t s) {
+    ETTK_REQUIRE(ccc_combine_fft_supported(z), ETTK_ERR_UNSUPPORTED, "launch_dftBC_fft: box size must be 64");
+    const float2* tw64 = T.tw.p + T.offset[fft_len_index(64)];
+    const float2* tw128 = T.tw.p + T.offset[fft_len_index(128)];
+    const size_t shm = (size_t)z.E * z.H * sizeof(float2);
+    static size_t set = 0;
+    if (shm > set) {
+        ETTK_HIP(hipFuncSetAttribute((const void*)k_dftBC_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        set = shm;
+    }
+    k_dftBC_fft<<<dim3(z.E, n_eval), kCfT, shm, s>>>(G1, tabK, z, best_val, best_idx, tw64, tw128);
+    ETTK_HIP(hipGetLastError());
+}
 void launch_combine_fftA(const FftTables& T, const float2* F1m, const float2* F2, const float2* Q, const float2* Mfft,
                          const float* scalars, const CccSizes& z, int n_eval, float2* G1, hipStream_t s) {
     ETTK_REQUIRE(ccc_combine_fft_supported(z), ETTK_ERR_UNSUPPORTED, "launch_combine_fftA: box size must be 64");

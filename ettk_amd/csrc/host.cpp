@@ -472,7 +472,10 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
         }
         {
             ProfScope ps(kp, "k_dftBC_argmax", Gb, s);
-            launch_dftBC_argmax(P->G1.p, P->tabA.p, P->tabK.p, P->z, E, P->bval.p, P->bidx.p, s);
+            if (ccc_combine_fft_supported(P->z))
+                launch_dftBC_fft(P->ctx->fft, P->G1.p, P->tabK.p, P->z, E, P->bval.p, P->bidx.p, s);
+            else
+                launch_dftBC_argmax(P->G1.p, P->tabA.p, P->tabK.p, P->z, E, P->bval.p, P->bidx.p, s);
         }
         launch_peak_final(P->bval.p, P->bidx.p, P->scalars.p, (float)P->sum_filter,
                           (P->p.overlap_norm && P->p.missing_wedge) ? 1 : 0, P->z, E, P->out_all.p + (size_t)4 * e0, s);

@@ -85,6 +85,9 @@ struct CccSizes {
 bool ccc_combine_fft_supported(const CccSizes& z);
 void launch_combine_fftA(const FftTables& T, const float2* F1m, const float2* F2, const float2* Q, const float2* Mfft,
                          const float* scalars, const CccSizes& z, int n_eval, float2* G1, hipStream_t s);
+// launch_dftBC_argmax's result with the second DFT axis evaluated by FFTs (box size 64, any window)
+void launch_dftBC_fft(const FftTables& T, const float2* G1, const float2* tabK, const CccSizes& z, int n_eval,
+                      float* best_val, int* best_idx, hipStream_t s);
 
 // reslice (tricubic Keys a=-1/2, clamp border, -FLT_MAX outside): out [e][i][j][k]; `nb` must equal
 // reslice_blocks(N) (one workgroup per 8^3 output tile), partials hold 2 * nb floats per evaluation
