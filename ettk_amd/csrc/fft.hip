@@ -845,13 +845,17 @@ __global__ void __launch_bounds__(Plane<L>::THREADS) k_c2r_r2c_plane(float2* __r
 // u = 2m + b, G(u) = sum_f [(-1)^f C(f) e^{+2 pi i f b / 2N}] e^{+2 pi i f m / N}, a backward N-point transform of
 // the twiddled pencil (f periodic mod N) for each parity b, read at m mod N.  One workgroup per (j, evaluation):
 // 2H pencils (k, b) of 8 threads, each a length-64 register FFT.
+// FWD: F1m and Q arrive before their forward i transform (the fused plane kernels' outputs); the workgroup runs
+// those transforms on its 2H pencils first.
 static constexpr int kCfP = 2 * 33, kCfT = kCfP * 8;
+template <bool FWD>
 __global__ void __launch_bounds__(kCfT) k_combine_fftA(const float2* __restrict__ F1m, const float2* __restrict__ F2,
                                                        const float2* __restrict__ Q, const float2* __restrict__ Mfft,
                                                        const float* __restrict__ scalars, CccSizes z,
                                                        float2* __restrict__ G1, const float2* __restrict__ tw64,
                                                        const float2* __restrict__ tw128) {
-    constexpr int L = 64, H = L / 2 + 1, LD = PenT<64>::LD;
+    constexpr int L = 64, H = L / 2 + 1, LD = PenT<64>::LD, NE = (L * H + kCfT - 1) / kCfT;
+    static_assert(L * H == NE * kCfT, "four elements per thread");
     __shared__ float2 pens[kCfP * LD];
     __shared__ float2 tw[L];
     __shared__ float2 wb[L];  // e^{+2 pi i (i - L/2) / 2L} for i = 0 .. L-1
@@ -865,30 +869,64 @@ __global__ void __launch_bounds__(kCfT) k_combine_fftA(const float2* __restrict_
     const float n3 = (float)L * L * L;
     const float a = 2.f * n3 / (n1 * n2);
     const float b = 1.f / (n2 * n2);
-    __syncthreads();
-    struct Quad {
-        float2 f1, f2, q, m;
-    };
     const float2* __restrict__ f1p = F1m + (size_t)e * nh;
     const float2* __restrict__ f2p = F2 + (size_t)e * nh;
     const float2* __restrict__ qp = Q + (size_t)e * nh;
-    block_copy_rt<4>(
-        L * H,
-        [&](int t) {
+    const int q = threadIdx.x >> 3, lane = threadIdx.x & 7;
+    // this thread's elements t = threadIdx.x + u kCfT of the (i, k) plane
+    float2 c0[NE], c1[NE];
+    if constexpr (FWD) {
+        float2 vf1[NE], vq[NE], vf2[NE], vm[NE];
+#pragma unroll
+        for (int u = 0; u < NE; u++) {
+            const int t = threadIdx.x + u * kCfT;
             const long h = ((long)(t / H) * L + j) * H + t % H;
-            return Quad{f1p[h], f2p[h], qp[h], Mfft[h]};
-        },
-        [&](int t, const Quad& v) {
-            const int i = t / H, k = t % H;
-            float2 c = csub(cscale(cmulc(v.f1, v.f2), a), cscale(cmulc(v.m, v.q), b));
+            vf1[u] = f1p[h];
+            vq[u] = qp[h];
+            vf2[u] = f2p[h];
+            vm[u] = Mfft[h];
+        }
+#pragma unroll
+        for (int u = 0; u < NE; u++) {
+            const int t = threadIdx.x + u * kCfT, i = t / H, k = t % H;
+            pens[k * LD + i] = vf1[u];
+            pens[(H + k) * LD + i] = vq[u];
+        }
+        __syncthreads();
+        fft64_reg(&pens[q * LD], lane, tw_cache<64>(tw, lane), false);  // F1m and Q along i
+#pragma unroll
+        for (int u = 0; u < NE; u++) {
+            const int t = threadIdx.x + u * kCfT, i = t / H, k = t % H;
+            float2 c = csub(cscale(cmulc(pens[k * LD + i], vf2[u]), a), cscale(cmulc(vm[u], pens[(H + k) * LD + i]), b));
             if ((i + j + k) & 1) c = make_float2(-c.x, -c.y);
             if ((i - L / 2) & 1) c = make_float2(-c.x, -c.y);  // (-1)^f of the table
-            const int n = (i + L / 2) & (L - 1);            // f mod L
-            pens[k * LD + n] = c;
-            pens[(H + k) * LD + n] = cmul(c, wb[i]);
-        });
+            c0[u] = c;
+            c1[u] = cmul(c, wb[i]);
+        }
+        __syncthreads();
+    } else {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NE; u++) {
+            const int t = threadIdx.x + u * kCfT;
+            const long h = ((long)(t / H) * L + j) * H + t % H;
+            const float2 f1 = f1p[h], f2 = f2p[h], qv = qp[h], m = Mfft[h];
+            const int i = t / H, k = t % H;
+            float2 c = csub(cscale(cmulc(f1, f2), a), cscale(cmulc(m, qv), b));
+            if ((i + j + k) & 1) c = make_float2(-c.x, -c.y);
+            if ((i - L / 2) & 1) c = make_float2(-c.x, -c.y);
+            c0[u] = c;
+            c1[u] = cmul(c, wb[i]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NE; u++) {
+        const int t = threadIdx.x + u * kCfT, i = t / H, k = t % H;
+        const int n = (i + L / 2) & (L - 1);  // f mod L
+        pens[k * LD + n] = c0[u];
+        pens[(H + k) * LD + n] = c1[u];
+    }
     __syncthreads();
-    const int q = threadIdx.x >> 3, lane = threadIdx.x & 7;
     fft64_reg(&pens[q * LD], lane, tw_cache<64>(tw, lane), true);
     const int u0 = z.I0 - L;
     for (int t = threadIdx.x; t < E * H; t += kCfT) {
@@ -1057,7 +1095,7 @@ static void launch_c2r(const FftTables& T, const float2* in, float2* tmp, float*
 
 template <int L>
 static void launch_c2r_op_r2c(const FftTables& T, const float2* in, float2* io, int batch, int op, const float* aux,
-                              float* partials, hipStream_t s) {
+                              float* partials, hipStream_t s, bool fwd_i = true) {
     constexpr int H = L / 2 + 1;
     const float2* tw = T.tw.p + T.offset[fft_len_index(L)];
     const double vol_c = (double)batch * L * L * H * sizeof(float2);
@@ -1072,7 +1110,7 @@ static void launch_c2r_op_r2c(const FftTables& T, const float2* in, float2* io, 
         k_c2r_r2c_plane<L><<<(unsigned)((long)batch * L), Plane<L>::THREADS, 0, s>>>(io, op, aux, partials, tw);
     }
     StridedArgs sf{io, io, (long)batch * L * H, (long)L * H, (long)L * L * H, (long)L * H, 0};
-    {
+    if (fwd_i) {
         ProfScope ps(T.prof, prof_name<L>(1), 2 * vol_c, s);
         k_fft_strided<L><<<(unsigned)((sf.npencils + StridedShape<L>::P - 1) / StridedShape<L>::P), StridedShape<L>::THREADS, 0, s>>>(sf, tw);
     }
@@ -1096,11 +1134,14 @@ void launch_dftBC_fft(const FftTables& T, const float2* G1, const float2* tabK, 
     ETTK_HIP(hipGetLastError());
 }
 void launch_combine_fftA(const FftTables& T, const float2* F1m, const float2* F2, const float2* Q, const float2* Mfft,
-                         const float* scalars, const CccSizes& z, int n_eval, float2* G1, hipStream_t s) {
+                         const float* scalars, const CccSizes& z, int n_eval, float2* G1, bool forward_i, hipStream_t s) {
     ETTK_REQUIRE(ccc_combine_fft_supported(z), ETTK_ERR_UNSUPPORTED, "launch_combine_fftA: box size must be 64");
     const float2* tw64 = T.tw.p + T.offset[fft_len_index(64)];
     const float2* tw128 = T.tw.p + T.offset[fft_len_index(128)];
-    k_combine_fftA<<<dim3(z.N, n_eval), kCfT, 0, s>>>(F1m, F2, Q, Mfft, scalars, z, G1, tw64, tw128);
+    if (forward_i)
+        k_combine_fftA<true><<<dim3(z.N, n_eval), kCfT, 0, s>>>(F1m, F2, Q, Mfft, scalars, z, G1, tw64, tw128);
+    else
+        k_combine_fftA<false><<<dim3(z.N, n_eval), kCfT, 0, s>>>(F1m, F2, Q, Mfft, scalars, z, G1, tw64, tw128);
     ETTK_HIP(hipGetLastError());
 }
 
@@ -1159,12 +1200,12 @@ void fft3_r2c_planes(const FftTables& T, const R2CArgs& a, hipStream_t s) {
 }
 
 void fft3_op_r2c_after_ipass(const FftTables& T, float2* io, int n, int batch, int op, const float* aux, float* partials,
-                             hipStream_t s) {
+                             bool forward_i, hipStream_t s) {
     switch (n) {
-        case 16: launch_c2r_op_r2c<16>(T, nullptr, io, batch, op, aux, partials, s); break;
-        case 32: launch_c2r_op_r2c<32>(T, nullptr, io, batch, op, aux, partials, s); break;
-        case 48: launch_c2r_op_r2c<48>(T, nullptr, io, batch, op, aux, partials, s); break;
-        case 64: launch_c2r_op_r2c<64>(T, nullptr, io, batch, op, aux, partials, s); break;
+        case 16: launch_c2r_op_r2c<16>(T, nullptr, io, batch, op, aux, partials, s, forward_i); break;
+        case 32: launch_c2r_op_r2c<32>(T, nullptr, io, batch, op, aux, partials, s, forward_i); break;
+        case 48: launch_c2r_op_r2c<48>(T, nullptr, io, batch, op, aux, partials, s, forward_i); break;
+        case 64: launch_c2r_op_r2c<64>(T, nullptr, io, batch, op, aux, partials, s, forward_i); break;
         default: throw Error{ETTK_ERR_UNSUPPORTED, "fft3_op_r2c_after_ipass: unsupported size " + std::to_string(n)};
     }
     ETTK_HIP(hipGetLastError());

@@ -438,9 +438,11 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
             fft3_ccc_ipass(P->ctx->fft, ia, N, E, s);
             nb_sp = fft3_ccc_ipass_blocks(N);
             // reference side: c2r -> r1 m1 (with the k1_m partials) -> r2c = F1m, one partial per (evaluation, plane)
-            fft3_op_r2c_after_ipass(P->ctx->fft, P->S1.p, N, E, PLANE_OP_MASK, P->m1.p, P->p_k1.p, s);
+            // (at 64 the forward i passes of both run inside the combine kernel)
+            const bool fwd = !ccc_combine_fft_supported(P->z);
+            fft3_op_r2c_after_ipass(P->ctx->fft, P->S1.p, N, E, PLANE_OP_MASK, P->m1.p, P->p_k1.p, fwd, s);
             // particle side: Q = r2c(c2r(X2)^2 shift)
-            fft3_op_r2c_after_ipass(P->ctx->fft, P->S3.p, N, E, PLANE_OP_SQUARE_SHIFT, nullptr, nullptr, s);
+            fft3_op_r2c_after_ipass(P->ctx->fft, P->S3.p, N, E, PLANE_OP_SQUARE_SHIFT, nullptr, nullptr, fwd, s);
         } else {
             fft3_r2c(P->ctx->fft, a2, nullptr, s);
             // both spectra weighted by filter x W1 W2 in one pass: X1 = F1mcf (reference, MCF applied once), X2 = mcf(F2)
@@ -466,7 +468,8 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
         {
             ProfScope ps(kp, "k_combine_dftA", 3 * Hb + Gb, s);
             if (ccc_combine_fft_supported(P->z))
-                launch_combine_fftA(P->ctx->fft, P->S1.p, P->S2.p, P->S3.p, P->Mfft.p, P->scalars.p, P->z, E, P->G1.p, s);
+                launch_combine_fftA(P->ctx->fft, P->S1.p, P->S2.p, P->S3.p, P->Mfft.p, P->scalars.p, P->z, E, P->G1.p,
+                                    fused, s);
             else
                 launch_combine_dftA(P->S1.p, P->S2.p, P->S3.p, P->Mfft.p, P->scalars.p, P->tabA.p, P->z, E, P->G1.p, s);
         }

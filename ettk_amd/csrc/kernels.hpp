@@ -65,9 +65,10 @@ int fft3_ccc_ipass_blocks(int n);
 void fft3_ccc_ipass(const FftTables& T, const CccIpassArgs& a, int n, int batch, hipStream_t s);
 // the k/j plane passes of an r2c only (n <= 64): the i pass is left to fft3_ccc_ipass
 void fft3_r2c_planes(const FftTables& T, const R2CArgs& a, hipStream_t s);
-// fft3_c2r_op_r2c whose inverse i pass has already run (io holds the i-inverse-transformed spectra)
+// fft3_c2r_op_r2c whose inverse i pass has already run (io holds the i-inverse-transformed spectra); without
+// `forward_i` its forward i pass is left to launch_combine_fftA
 void fft3_op_r2c_after_ipass(const FftTables& T, float2* io, int n, int batch, int op, const float* aux, float* partials,
-                             hipStream_t s);
+                             bool forward_i, hipStream_t s);
 
 // smoothed analytic wedge parameters of the reference
 struct WedgeRef {
@@ -81,10 +82,11 @@ struct CccSizes {
     float restr2;         // restriction^2 for the spherical search mask
     int ub2;              // (E-1)/2
 };
-// launch_combine_dftA's result with the first DFT axis evaluated by FFTs (box size 64, any window)
+// launch_combine_dftA's result with the first DFT axis evaluated by FFTs (box size 64, any window); with
+// `forward_i`, F1m and Q are passed before their forward i transform, which the kernel runs itself
 bool ccc_combine_fft_supported(const CccSizes& z);
 void launch_combine_fftA(const FftTables& T, const float2* F1m, const float2* F2, const float2* Q, const float2* Mfft,
-                         const float* scalars, const CccSizes& z, int n_eval, float2* G1, hipStream_t s);
+                         const float* scalars, const CccSizes& z, int n_eval, float2* G1, bool forward_i, hipStream_t s);
 // launch_dftBC_argmax's result with the second DFT axis evaluated by FFTs (box size 64, any window)
 void launch_dftBC_fft(const FftTables& T, const float2* G1, const float2* tabK, const CccSizes& z, int n_eval,
                       float* best_val, int* best_idx, hipStream_t s);
