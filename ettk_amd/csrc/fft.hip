@@ -876,24 +876,32 @@ __global__ void __launch_bounds__(kCfT) k_combine_fftA(const float2* __restrict_
     // this thread's elements t = threadIdx.x + u kCfT of the (i, k) plane
     float2 c0[NE], c1[NE];
     if constexpr (FWD) {
-        float2 vf1[NE], vq[NE], vf2[NE], vm[NE];
+        {
+            float2 vf1[NE], vq[NE];
+#pragma unroll
+            for (int u = 0; u < NE; u++) {
+                const int t = threadIdx.x + u * kCfT;
+                const long h = ((long)(t / H) * L + j) * H + t % H;
+                vf1[u] = f1p[h];
+                vq[u] = qp[h];
+            }
+#pragma unroll
+            for (int u = 0; u < NE; u++) {
+                const int t = threadIdx.x + u * kCfT, i = t / H, k = t % H;
+                pens[k * LD + i] = vf1[u];
+                pens[(H + k) * LD + i] = vq[u];
+            }
+        }
+        __syncthreads();
+        fft64_reg(&pens[q * LD], lane, tw_cache<64>(tw, lane), false);  // F1m and Q along i
+        float2 vf2[NE], vm[NE];  // loaded after the transforms: fewer registers held across them
 #pragma unroll
         for (int u = 0; u < NE; u++) {
             const int t = threadIdx.x + u * kCfT;
             const long h = ((long)(t / H) * L + j) * H + t % H;
-            vf1[u] = f1p[h];
-            vq[u] = qp[h];
             vf2[u] = f2p[h];
             vm[u] = Mfft[h];
         }
-#pragma unroll
-        for (int u = 0; u < NE; u++) {
-            const int t = threadIdx.x + u * kCfT, i = t / H, k = t % H;
-            pens[k * LD + i] = vf1[u];
-            pens[(H + k) * LD + i] = vq[u];
-        }
-        __syncthreads();
-        fft64_reg(&pens[q * LD], lane, tw_cache<64>(tw, lane), false);  // F1m and Q along i
 #pragma unroll
         for (int u = 0; u < NE; u++) {
             const int t = threadIdx.x + u * kCfT, i = t / H, k = t % H;
