@@ -465,17 +465,18 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
         // combine, upsampled correlation on the search window, peak
         launch_ccc_scalars(P->p_n1.p, P->p_n2.p, P->p_k1.p, P->p_wf.p, nb_sp, fused ? N : P->nb_real, E, N,
                            P->scalars.p, s);
+        const bool cfft = ccc_combine_fft_supported(P->z);
         {
-            ProfScope ps(kp, "k_combine_dftA", 3 * Hb + Gb, s);
-            if (ccc_combine_fft_supported(P->z))
+            ProfScope ps(kp, cfft ? "k_combine_fftA" : "k_combine_dftA", 3 * Hb + Gb, s);
+            if (cfft)
                 launch_combine_fftA(P->ctx->fft, P->S1.p, P->S2.p, P->S3.p, P->Mfft.p, P->scalars.p, P->z, E, P->G1.p,
                                     fused, s);
             else
                 launch_combine_dftA(P->S1.p, P->S2.p, P->S3.p, P->Mfft.p, P->scalars.p, P->tabA.p, P->z, E, P->G1.p, s);
         }
         {
-            ProfScope ps(kp, "k_dftBC_argmax", Gb, s);
-            if (ccc_combine_fft_supported(P->z))
+            ProfScope ps(kp, cfft ? "k_dftBC_fft" : "k_dftBC_argmax", Gb, s);
+            if (cfft)
                 launch_dftBC_fft(P->ctx->fft, P->G1.p, P->tabK.p, P->z, E, P->bval.p, P->bidx.p, s);
             else
                 launch_dftBC_argmax(P->G1.p, P->tabA.p, P->tabK.p, P->z, E, P->bval.p, P->bidx.p, s);
