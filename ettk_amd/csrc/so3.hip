@@ -239,9 +239,11 @@ __device__ __forceinline__ int wrap_idx(int x, int M) { return x < 0 ? x + M : (
 
 __global__ void __launch_bounds__(kT) k_sph_sample(const float* __restrict__ mag, const float* __restrict__ st, int M,
                                                     int B, int bandL, int bandU, float* __restrict__ Pp) {
-    const int b = blockIdx.y;
+    // one particle per XCD at a time: its magnitude volume is gathered through that XCD's L2 only
+    int b, tile;
+    xcd_eval_tile(blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, b, tile);
     const int n = 2 * B;
-    const int t = blockIdx.x * (kT / kLanes) + threadIdx.x / kLanes;
+    const int t = tile * (kT / kLanes) + threadIdx.x / kLanes;
     const int lane = threadIdx.x % kLanes;
     const bool live = t < n * n;
     const int jp = live ? t / n : 0, kt = live ? t % n : 0;
@@ -579,30 +581,32 @@ __global__ void __launch_bounds__(kSynT) k_so3_synth_fft(const double2* __restri
         double re = 0, im = 0;
         if (c1 != B && c2 != B) {
             const int m1 = c1 < B ? c1 : c1 - n, m2 = c2 < B ? c2 : c2 - n;
-            const int am1 = m1 < 0 ? -m1 : m1, am2 = m2 < 0 ? -m2 : m2;
-            const int l0 = am1 > am2 ? am1 : am2;
+            const int am1 = m1 < 0 ? -m1 : m1;
             const double* d = D + (size_t)c1 * B * n + c2;
-            int l = l0;
-            // four degrees at a time: their twelve loads in flight together, the sum kept in order of l
-            for (; l + 3 < B; l += 4) {
-                double2 a[4], b[4];
-                double dv[4];
+            // degrees from |m1| rounded down to a multiple of 8, eight at a time with their 24 loads in flight:
+            // the table is zero for l < max(|m1|, |m2|), so the extra leading terms add exact zeros (their
+            // coefficient indices clamped into the arrays) and the sum keeps its order of l.  The start depends
+            // on m1 only, which is uniform over a wavefront when n = 64, so its lanes run the same trip count.
+            int l = am1 & ~7;
+            for (; l + 7 < B; l += 8) {
+                double2 a[8], b[8];
+                double dv[8];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < 8; u++) {
                     const int lu = l + u;
-                    a[u] = S[lu * lu + lu - m1];
-                    b[u] = Pt[lu * lu + lu - m2];
+                    a[u] = S[max(lu * lu + lu - m1, 0)];
+                    b[u] = Pt[max(lu * lu + lu - m2, 0)];
                     dv[u] = d[(size_t)lu * n];
                 }
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < 8; u++) {
                     const double pr = a[u].x * b[u].x + a[u].y * b[u].y, pi = a[u].y * b[u].x - a[u].x * b[u].y;
                     re += pr * dv[u];
                     im += pi * dv[u];
                 }
             }
             for (; l < B; l++) {
-                const double2 a = S[l * l + l - m1], b = Pt[l * l + l - m2];
+                const double2 a = S[max(l * l + l - m1, 0)], b = Pt[max(l * l + l - m2, 0)];
                 const double pr = a.x * b.x + a.y * b.y, pi = a.y * b.x - a.x * b.y;
                 const double dv = d[(size_t)l * n];
                 re += pr * dv;
