@@ -588,6 +588,30 @@ __global__ void __launch_bounds__(kSynT) k_so3_synth_fft(const double2* __restri
             // coefficient indices clamped into the arrays) and the sum keeps its order of l.  The start depends
             // on m1 only, which is uniform over a wavefront when n = 64, so its lanes run the same trip count.
             int l = am1 & ~7;
+            if constexpr (LOGN == 6) {
+                // n = 64, 512 threads: c1 = t / 64 is uniform over a wavefront, so the signal coefficients of a
+                // degree are one scalar load per wavefront; sixteen degrees per round trip from |m1| rounded
+                // down to a multiple of 16 (B = 32: at most two round trips per element)
+                const int m1u = __builtin_amdgcn_readfirstlane(m1);
+                const int am1u = m1u < 0 ? -m1u : m1u;
+                for (l = am1u & ~15; l + 15 < B; l += 16) {
+                    double2 a[16], b[16];
+                    double dv[16];
+#pragma unroll
+                    for (int u = 0; u < 16; u++) {
+                        const int lu = l + u;
+                        a[u] = S[max(lu * lu + lu - m1u, 0)];
+                        b[u] = Pt[max(lu * lu + lu - m2, 0)];
+                        dv[u] = d[(size_t)lu * n];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 16; u++) {
+                        const double pr = a[u].x * b[u].x + a[u].y * b[u].y, pi = a[u].y * b[u].x - a[u].x * b[u].y;
+                        re += pr * dv[u];
+                        im += pi * dv[u];
+                    }
+                }
+            }
             for (; l + 7 < B; l += 8) {
                 double2 a[8], b[8];
                 double dv[8];
