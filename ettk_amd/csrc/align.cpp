@@ -372,17 +372,20 @@ static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOu
     const bool uw1 = L.uw1;
     const int nv = 2 * cn;
     hipStream_t cs = P->ctx->copy_stream;
-    std::vector<int> head(2 * nv + 1);
-    ETTK_HIP(hipMemcpyAsync(head.data(), L.lm->head.p, head.size() * sizeof(int), hipMemcpyDeviceToHost, cs));
+    LmBuffers& lb = *L.lm;
+    const int* head = lb.hhead.p;
+    ETTK_HIP(hipMemcpyAsync(lb.hhead.p, lb.head.p, (size_t)(2 * nv + 1) * sizeof(int), hipMemcpyDeviceToHost, cs));
     ETTK_HIP(hipStreamSynchronize(cs));
     const int total = head[nv];
-    std::vector<int> pidx(total);
-    std::vector<double> pval(total);
     if (total > 0) {
-        ETTK_HIP(hipMemcpyAsync(pidx.data(), L.lm->pidx.p, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, cs));
-        ETTK_HIP(hipMemcpyAsync(pval.data(), L.lm->pval.p, (size_t)total * sizeof(double), hipMemcpyDeviceToHost, cs));
+        lb.hpidx.ensure(total);
+        lb.hpval.ensure(total);
+        ETTK_HIP(hipMemcpyAsync(lb.hpidx.p, lb.pidx.p, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, cs));
+        ETTK_HIP(hipMemcpyAsync(lb.hpval.p, lb.pval.p, (size_t)total * sizeof(double), hipMemcpyDeviceToHost, cs));
         ETTK_HIP(hipStreamSynchronize(cs));
     }
+    const int* pidx = lb.hpidx.p;
+    const double* pval = lb.hpval.p;
     if (kp) kp->add_host("search_download", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count());
     std::vector<std::vector<int>> idx(nv);
     std::vector<std::vector<double>> val(nv);
@@ -392,8 +395,8 @@ static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOu
         if (v % 2 == 1 && !two) continue;
         const int c = head[nv + 1 + v];
         ETTK_REQUIRE(c <= kLmCap, ETTK_ERR_UNSUPPORTED, "more SO(3) local maxima than the candidate list holds");
-        idx[v].assign(pidx.begin() + head[v], pidx.begin() + head[v] + c);
-        val[v].assign(pval.begin() + head[v], pval.begin() + head[v] + c);
+        idx[v].assign(pidx + head[v], pidx + head[v] + c);
+        val[v].assign(pval + head[v], pval + head[v] + c);
         if (kp && kp->on) kp->add_host("lm_count", (double)c);
     }
     for (int q = 0; q < cn; q++) {

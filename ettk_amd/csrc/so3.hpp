@@ -1,5 +1,6 @@
 // so3.hpp -- S^2 projection / SO(3) search state and launch wrappers.
 #pragma once
+#include <algorithm>
 #include <memory>
 
 #include "plan.hpp"
@@ -34,6 +35,10 @@ struct LmBuffers {
     // the chunk's SOFT instances and combine operations, staged in pinned memory for asynchronous uploads
     PinnedBuf<SoftInst> hinst;
     PinnedBuf<ComboOp> hops;
+    // host copies of the compacted lists, pinned: downloads from pageable memory pass through one staging buffer
+    // of the runtime, which serialises the chunks' downloads issued from concurrent host threads
+    PinnedBuf<int> hhead, hpidx;
+    PinnedBuf<double> hpval;
 };
 
 struct So3State {
@@ -76,6 +81,10 @@ inline LmBuffers& lm_slot(So3State& S, int slot, int nvol, int cap) {
     b.head.ensure((size_t)2 * nvol + 1);
     b.pidx.ensure((size_t)nvol * cap);
     b.pval.ensure((size_t)nvol * cap);
+    // the host lists start at 2048 entries per volume (grown by the download when a chunk holds more)
+    b.hhead.ensure((size_t)2 * nvol + 1);
+    b.hpidx.ensure((size_t)nvol * std::min(cap, 2048));
+    b.hpval.ensure((size_t)nvol * std::min(cap, 2048));
     return b;
 }
 
