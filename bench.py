@@ -132,7 +132,7 @@ def main() -> int:
     for name in ("k_fft_strided<64>", "k_r2c_plane<64>", "k_c2r_plane<64>", "k_c2r_r2c_plane<64>", "k_ccc_ipass<64>",
                  "k_r2c_rows<64>", "k_c2r_rows<64>", "k_reslice", "k_spec_filter", "k_mask_ref",
                  "k_combine_dftA", "k_dftBC_argmax", "k_combine_fftA", "k_dftBC_fft", "k_fft_strided<128>", "k_r2c_rows<128>",
-                 "k_reslice_ref", "k_refine_dot", "k_refine_adj", "k_refine_moments", "k_sph_sample",
+                 "k_reslice_ref", "k_refine_dot", "k_refine_adj", "k_refine_ipass_dot", "k_refine_ipass_adj", "k_refine_moments", "k_sph_sample",
                  "k_so3_synth_fft", "k_rotate_coef", "k_extrema", "k_combine", "k_localmax"):
         ms, nl, by = ctx.profile_query(name)
         if nl:

@@ -1195,6 +1195,186 @@ static void launch_r2c_planes(const FftTables& T, const R2CArgs& a, hipStream_t 
     ProfScope ps(T.prof, plane_name<L>(0), vol_r + vol_c, s);
     k_r2c_plane<L><<<(unsigned)((long)a.batch * L), Plane<L>::THREADS, 0, s>>>(a, tw);
 }
+// ------------------------------------------------------------------ refinement: score and adjoint in the i passes
+// FP64 sums of the three score terms over a workgroup (halving tree, any thread count), valid in thread 0
+template <int NT>
+__device__ __forceinline__ void block_sum3_f64(double v0, double v1, double v2, double* sred) {
+    sred[threadIdx.x] = v0;
+    sred[NT + threadIdx.x] = v1;
+    sred[2 * NT + threadIdx.x] = v2;
+    __syncthreads();
+    for (int cnt = NT; cnt > 1;) {
+        const int st = (cnt + 1) / 2;
+        if ((int)threadIdx.x < cnt - st)
+#pragma unroll
+            for (int r = 0; r < 3; r++) sred[r * NT + threadIdx.x] += sred[r * NT + threadIdx.x + st];
+        __syncthreads();
+        cnt = st;
+    }
+}
+
+// The forward i pass of X1 (in place, after the plane passes of its r2c) and k_refine_dot's terms on the
+// transformed elements: per workgroup (32 pencils (j, k) of one evaluation) the FP64 partials of <H X1, H X2>,
+// |H X1|^2 and |H X2|^2 with the Hermitian weights (2 off the k = 0 and k = N/2 planes).
+template <int L>
+__global__ void __launch_bounds__(StridedShape<L>::THREADS) k_refine_ipass_dot(
+    float2* __restrict__ X1, const float2* __restrict__ FP, const float* __restrict__ filter,
+    const float* __restrict__ wbin, const RefineGeom* __restrict__ geom, double* __restrict__ part,
+    const float2* __restrict__ twg) {
+    using S = StridedShape<L>;
+    constexpr int H = L / 2 + 1, LH = L * H, STEP = S::THREADS / S::P, NL = L / STEP;
+    static_assert(L % STEP == 0, "pencil elements per thread");
+    __shared__ float2 lds[S::P * S::LD];
+    __shared__ float2 tw[L];
+    __shared__ double sred[3 * S::THREADS];
+    const int e = blockIdx.y;
+    const float2 twv = twg[threadIdx.x % L];
+    const int qm = threadIdx.x % S::P, l0 = threadIdx.x / S::P;
+    const int p = blockIdx.x * S::P + qm;  // pencil (j, k) = (p / H, p % H) of evaluation e
+    const bool valid = p < LH;
+    const int pc = valid ? p : 0, pj = pc / H, pk = pc % H;
+    const RefineGeom g = geom[e];
+    float2* x1p = X1 + (size_t)e * L * LH;
+    const float2* x2p = FP + (size_t)g.spec * L * LH;
+    float2* __restrict__ pen = &lds[qm * S::LD];
+    float2 v2[NL];
+    {
+        float2 v[NL];
+#pragma unroll
+        for (int u = 0; u < NL; u++) {
+            v[u] = x1p[(size_t)(l0 + u * STEP) * LH + pc];
+            v2[u] = x2p[(size_t)(l0 + u * STEP) * LH + pc];
+        }
+#pragma unroll
+        for (int u = 0; u < NL; u++) pen[l0 + u * STEP] = valid ? v[u] : make_float2(0.f, 0.f);
+    }
+    if (threadIdx.x < L) tw[threadIdx.x] = twv;
+    __syncthreads();
+    const int q = threadIdx.x / S::TPP, lane = threadIdx.x % S::TPP;
+    pencil_fft<L>(&lds[q * S::LD], lane, tw, tw_cache<L>(tw, lane), false);
+    __syncthreads();
+    double sd = 0.0, s1 = 0.0, s2 = 0.0;
+    if (valid) {
+#pragma unroll
+        for (int u = 0; u < NL; u++) {
+            const int i = l0 + u * STEP;
+            const size_t h = (size_t)i * LH + pc;
+            const float2 z = pen[i];
+            x1p[h] = z;
+            const float w = refine_hweight(filter, wbin, g, (long)h, i, pj, pk, L, H);
+            const double ww = (pk == 0 || pk == H - 1) ? (double)w * w : 2.0 * (double)w * w;
+            const float2 v = v2[u];
+            sd += ww * ((double)z.x * v.x + (double)z.y * v.y);
+            s1 += ww * ((double)z.x * z.x + (double)z.y * z.y);
+            s2 += ww * ((double)v.x * v.x + (double)v.y * v.y);
+        }
+    }
+    block_sum3_f64<S::THREADS>(sd, s1, s2, sred);
+    if (threadIdx.x < 3) part[((size_t)e * gridDim.x + blockIdx.x) * 3 + threadIdx.x] = sred[threadIdx.x * S::THREADS];
+}
+
+// k_refine_adj's adjoint spectrum H (H X2 / (n1 n2) - S H X1 / n1^2), formed as the pencils are loaded, then
+// the inverse i pass into `out` (X1 is left unchanged)
+template <int L>
+__global__ void __launch_bounds__(StridedShape<L>::THREADS) k_refine_ipass_adj(
+    const float2* __restrict__ X1, const float2* __restrict__ FP, const float* __restrict__ filter,
+    const float* __restrict__ wbin, const RefineGeom* __restrict__ geom, const double* __restrict__ sc,
+    float2* __restrict__ out, const float2* __restrict__ twg) {
+    using S = StridedShape<L>;
+    constexpr int H = L / 2 + 1, LH = L * H, STEP = S::THREADS / S::P, NL = L / STEP;
+    static_assert(L % STEP == 0, "pencil elements per thread");
+    __shared__ float2 lds[S::P * S::LD];
+    __shared__ float2 tw[L];
+    const int e = blockIdx.y;
+    const float2 twv = twg[threadIdx.x % L];
+    const int qm = threadIdx.x % S::P, l0 = threadIdx.x / S::P;
+    const int p = blockIdx.x * S::P + qm;
+    const bool valid = p < LH;
+    const int pc = valid ? p : 0, pj = pc / H, pk = pc % H;
+    const RefineGeom g = geom[e];
+    const double Sv = sc[e * 4], n1 = sc[e * 4 + 1], n2 = sc[e * 4 + 2];
+    const float ca = (n1 > 0 && n2 > 0) ? (float)(1.0 / (n1 * n2)) : 0.f;
+    const float cb = n1 > 0 ? (float)(Sv / (n1 * n1)) : 0.f;
+    const float2* x1p = X1 + (size_t)e * L * LH;
+    const float2* x2p = FP + (size_t)g.spec * L * LH;
+    float2* __restrict__ pen = &lds[qm * S::LD];
+    {
+        float2 x[NL], v[NL];
+#pragma unroll
+        for (int u = 0; u < NL; u++) {
+            x[u] = x1p[(size_t)(l0 + u * STEP) * LH + pc];
+            v[u] = x2p[(size_t)(l0 + u * STEP) * LH + pc];
+        }
+#pragma unroll
+        for (int u = 0; u < NL; u++) {
+            const int i = l0 + u * STEP;
+            const float w = refine_hweight(filter, wbin, g, (long)((size_t)i * LH + pc), i, pj, pk, L, H);
+            const float w2 = w * w;
+            pen[i] = valid ? make_float2(w2 * (ca * v[u].x - cb * x[u].x), w2 * (ca * v[u].y - cb * x[u].y))
+                           : make_float2(0.f, 0.f);
+        }
+    }
+    if (threadIdx.x < L) tw[threadIdx.x] = twv;
+    __syncthreads();
+    const int q = threadIdx.x / S::TPP, lane = threadIdx.x % S::TPP;
+    pencil_fft<L>(&lds[q * S::LD], lane, tw, tw_cache<L>(tw, lane), true);
+    __syncthreads();
+    if (valid) {
+        float2* op = out + (size_t)e * L * LH;
+#pragma unroll
+        for (int u = 0; u < NL; u++) op[(size_t)(l0 + u * STEP) * LH + pc] = pen[l0 + u * STEP];
+    }
+}
+
+template <int L>
+static void launch_refine_dot_t(const FftTables& T, float2* X1, const float2* FP, const float* filter,
+                                const float* wbin, const RefineGeom* geom, int batch, double* part, hipStream_t s) {
+    const float2* tw = T.tw.p + T.offset[fft_len_index(L)];
+    ProfScope ps(T.prof, "k_refine_ipass_dot", 3.0 * batch * L * L * (L / 2 + 1) * sizeof(float2), s);
+    k_refine_ipass_dot<L><<<dim3((unsigned)ccc_ipass_blocks_t<L>(), (unsigned)batch), StridedShape<L>::THREADS, 0, s>>>(
+        X1, FP, filter, wbin, geom, part, tw);
+}
+template <int L>
+static void launch_refine_adj_c2r_t(const FftTables& T, const float2* X1, const float2* FP, const float* filter,
+                                    const float* wbin, const RefineGeom* geom, const double* sc, int batch,
+                                    float2* tmp, float* out, hipStream_t s) {
+    constexpr int H = L / 2 + 1;
+    const float2* tw = T.tw.p + T.offset[fft_len_index(L)];
+    {
+        ProfScope ps(T.prof, "k_refine_ipass_adj", 3.0 * batch * L * L * H * sizeof(float2), s);
+        k_refine_ipass_adj<L><<<dim3((unsigned)ccc_ipass_blocks_t<L>(), (unsigned)batch), StridedShape<L>::THREADS, 0,
+                                s>>>(X1, FP, filter, wbin, geom, sc, tmp, tw);
+    }
+    ETTK_HIP(hipGetLastError());
+    const double vol_c = (double)batch * L * L * H * sizeof(float2), vol_r = (double)batch * L * L * L * sizeof(float);
+    ProfScope ps(T.prof, plane_name<L>(1), vol_c + vol_r, s);
+    k_c2r_plane<L><<<(unsigned)((long)batch * L), Plane<L>::THREADS, 0, s>>>(tmp, out, tw);
+}
+int fft3_refine_ipass_blocks(int n) { return fft3_ccc_ipass_blocks(n); }
+void fft3_refine_dot(const FftTables& T, float2* X1, const float2* FP, const float* filter, const float* wbin,
+                     const RefineGeom* geom, int n, int batch, double* part, hipStream_t s) {
+    switch (n) {
+        case 16: launch_refine_dot_t<16>(T, X1, FP, filter, wbin, geom, batch, part, s); break;
+        case 32: launch_refine_dot_t<32>(T, X1, FP, filter, wbin, geom, batch, part, s); break;
+        case 48: launch_refine_dot_t<48>(T, X1, FP, filter, wbin, geom, batch, part, s); break;
+        case 64: launch_refine_dot_t<64>(T, X1, FP, filter, wbin, geom, batch, part, s); break;
+        default: throw Error{ETTK_ERR_UNSUPPORTED, "fft3_refine_dot: unsupported size " + std::to_string(n)};
+    }
+    ETTK_HIP(hipGetLastError());
+}
+void fft3_refine_adj_c2r(const FftTables& T, const float2* X1, const float2* FP, const float* filter,
+                         const float* wbin, const RefineGeom* geom, const double* sc, int n, int batch, float2* tmp,
+                         float* out, hipStream_t s) {
+    switch (n) {
+        case 16: launch_refine_adj_c2r_t<16>(T, X1, FP, filter, wbin, geom, sc, batch, tmp, out, s); break;
+        case 32: launch_refine_adj_c2r_t<32>(T, X1, FP, filter, wbin, geom, sc, batch, tmp, out, s); break;
+        case 48: launch_refine_adj_c2r_t<48>(T, X1, FP, filter, wbin, geom, sc, batch, tmp, out, s); break;
+        case 64: launch_refine_adj_c2r_t<64>(T, X1, FP, filter, wbin, geom, sc, batch, tmp, out, s); break;
+        default: throw Error{ETTK_ERR_UNSUPPORTED, "fft3_refine_adj_c2r: unsupported size " + std::to_string(n)};
+    }
+    ETTK_HIP(hipGetLastError());
+}
+
 void fft3_r2c_planes(const FftTables& T, const R2CArgs& a, hipStream_t s) {
     ETTK_REQUIRE(a.pro != PRO_PAD_MASKED, ETTK_ERR_ARG, "fft3_r2c_planes: padded prologue not supported");
     switch (a.n) {

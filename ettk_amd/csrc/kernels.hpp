@@ -126,7 +126,45 @@ struct RefineGeom {
     int spec;             // slot of the particle's spectrum
     int pad[3];
 };
+
+// H at half-spectrum element (a, b, c) = (z, y, x) of the particle frame: filter x binary wedges
+__device__ __forceinline__ float refine_hweight(const float* __restrict__ filter, const float* __restrict__ wbin,
+                                         const RefineGeom& g, long h, int a, int b, int c, int N, int H) {
+    float w = filter[h];  // radial table: identical in any axis order
+    const float kx = (float)(c - (H - 1)), ky = (float)(b - N / 2), kz = (float)(a - N / 2);
+    if (g.pw) {
+        float s1 = g.n1[0] * kx + g.n1[1] * ky + g.n1[2] * kz, s2 = g.n2[0] * kx + g.n2[1] * ky + g.n2[2] * kz;
+        if (s1 * s2 > 0.f) w = 0.f;
+    }
+    if (g.rw == 1) {
+        float s1 = g.u1[0] * kx + g.u1[1] * ky + g.u1[2] * kz, s2 = g.u2[0] * kx + g.u2[1] * ky + g.u2[2] * kz;
+        if (s1 * s2 > 0.f) w = 0.f;
+    } else if (g.rw == 2) {
+        // nearest voxel of the reference-frame frequency, clamped to the grid
+        int r[3];
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const float p = g.A[t * 3] * kx + g.A[t * 3 + 1] * ky + g.A[t * 3 + 2] * kz;
+            r[t] = min(max((int)rintf(p) + N / 2, 0), N - 1);
+        }
+        if (wbin[(r[0] * N + r[1]) * N + r[2]] == 0.f) w = 0.f;
+    }
+    return w;
+}
 int refine_moment_blocks(int N);
+void launch_refine_scalars(const double* part, int nb, int E, double* sc, hipStream_t s);
+// the refinement's transforms with the score and adjoint steps in their i passes (box sizes 16, 32, 48, 64):
+//   fft3_refine_dot: forward i pass of X1 in place (after fft3_r2c_planes) with the FP64 score partials of
+//     launch_refine_dot, fft3_refine_ipass_blocks(n) per evaluation;
+//   fft3_refine_adj_c2r: the adjoint spectrum of launch_refine_adj formed on load, its inverse i pass into
+//     `tmp`, then the c2r plane passes into `out` (X1 unchanged)
+int fft3_refine_ipass_blocks(int n);
+void fft3_refine_dot(const FftTables& T, float2* X1, const float2* FP, const float* filter, const float* wbin,
+                     const RefineGeom* geom, int n, int batch, double* part, hipStream_t s);
+void fft3_refine_adj_c2r(const FftTables& T, const float2* X1, const float2* FP, const float* filter,
+                         const float* wbin, const RefineGeom* geom, const double* sc, int n, int batch, float2* tmp,
+                         float* out, hipStream_t s);
+
 int refine_dot_blocks(int N);
 void launch_vol_mean(const float* vols, int count, int N, float* mean, hipStream_t s);
 void launch_reslice_ref(const float* ref, const RefineGeom* geom, int E, int N, float* out, hipStream_t s);

@@ -85,7 +85,7 @@ static void refine_prepare(ettk_plan* P, int n, std::vector<std::unique_ptr<Refi
     ensure_ccc_workspace(P, std::min(n, 128));
     const int cap = std::min(P->cap, 128);
     const int nbd = refine_dot_blocks(N), nbm = refine_moment_blocks(N);
-    P->r_pdot.ensure((size_t)3 * nbd * cap);
+    P->r_pdot.ensure((size_t)3 * std::max(nbd, fft3_fused_supported(N) ? fft3_refine_ipass_blocks(N) : 0) * cap);
     P->r_mpart.ensure((size_t)12 * nbm * cap);
     P->r_sc.ensure((size_t)4 * n);
     P->r_mom.ensure((size_t)12 * n);
@@ -172,17 +172,27 @@ static void refine_enqueue(ettk_plan* P, RefineChunk& c) {
         launch_reslice_ref(P->ref_x.p, geo, E, N, P->R2.p, s);
     }
     R2CArgs a{P->R2.p, P->S2.p, N, E, PRO_MASK_SHIFT, P->maskfft.p, nullptr};
-    fft3_r2c(P->ctx->fft, a, nullptr, s);
     double* sc = P->r_sc.p + (size_t)4 * e0;
-    {
-        ProfScope ps(kp, "k_refine_dot", 2 * Hb, s);
-        launch_refine_dot(P->S2.p, P->r_fp.p, P->filter.p, P->ref_wbin.p, geo, E, N, P->r_pdot.p, sc, s);
+    if (fft3_fused_supported(N)) {
+        // the score terms in the epilogue of the forward i pass, the adjoint spectrum in the prologue of the
+        // inverse one: X1 is read once by each instead of once more by separate score and adjoint passes
+        fft3_r2c_planes(P->ctx->fft, a, s);
+        fft3_refine_dot(P->ctx->fft, P->S2.p, P->r_fp.p, P->filter.p, P->ref_wbin.p, geo, N, E, P->r_pdot.p, s);
+        launch_refine_scalars(P->r_pdot.p, fft3_refine_ipass_blocks(N), E, sc, s);
+        fft3_refine_adj_c2r(P->ctx->fft, P->S2.p, P->r_fp.p, P->filter.p, P->ref_wbin.p, geo, sc, N, E, P->S3.p, P->R1.p,
+                            s);
+    } else {
+        fft3_r2c(P->ctx->fft, a, nullptr, s);
+        {
+            ProfScope ps(kp, "k_refine_dot", 2 * Hb, s);
+            launch_refine_dot(P->S2.p, P->r_fp.p, P->filter.p, P->ref_wbin.p, geo, E, N, P->r_pdot.p, sc, s);
+        }
+        {
+            ProfScope ps(kp, "k_refine_adj", 3 * Hb, s);
+            launch_refine_adj(P->S2.p, P->r_fp.p, P->filter.p, P->ref_wbin.p, geo, E, N, sc, s);
+        }
+        fft3_c2r(P->ctx->fft, P->S2.p, P->S3.p, P->R1.p, N, E, s);
     }
-    {
-        ProfScope ps(kp, "k_refine_adj", 3 * Hb, s);
-        launch_refine_adj(P->S2.p, P->r_fp.p, P->filter.p, P->ref_wbin.p, geo, E, N, sc, s);
-    }
-    fft3_c2r(P->ctx->fft, P->S2.p, P->S3.p, P->R1.p, N, E, s);
     double* mom = P->r_mom.p + (size_t)12 * e0;
     {
         ProfScope ps(kp, "k_refine_moments", Rb, s);

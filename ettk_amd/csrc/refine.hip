@@ -138,30 +138,6 @@ __device__ __forceinline__ float trilin_box(const float* box, const int lo[3], f
     return c0 + az * (c1 - c0);
 }
 
-// H at half-spectrum element (a, b, c) = (z, y, x) of the particle frame: filter x binary wedges
-__device__ __forceinline__ float hweight(const float* __restrict__ filter, const float* __restrict__ wbin,
-                                         const RefineGeom& g, long h, int a, int b, int c, int N, int H) {
-    float w = filter[h];  // radial table: identical in any axis order
-    const float kx = (float)(c - (H - 1)), ky = (float)(b - N / 2), kz = (float)(a - N / 2);
-    if (g.pw) {
-        float s1 = g.n1[0] * kx + g.n1[1] * ky + g.n1[2] * kz, s2 = g.n2[0] * kx + g.n2[1] * ky + g.n2[2] * kz;
-        if (s1 * s2 > 0.f) w = 0.f;
-    }
-    if (g.rw == 1) {
-        float s1 = g.u1[0] * kx + g.u1[1] * ky + g.u1[2] * kz, s2 = g.u2[0] * kx + g.u2[1] * ky + g.u2[2] * kz;
-        if (s1 * s2 > 0.f) w = 0.f;
-    } else if (g.rw == 2) {
-        // nearest voxel of the reference-frame frequency, clamped to the grid
-        int r[3];
-#pragma unroll
-        for (int t = 0; t < 3; t++) {
-            const float p = g.A[t * 3] * kx + g.A[t * 3 + 1] * ky + g.A[t * 3 + 2] * kz;
-            r[t] = min(max((int)rintf(p) + N / 2, 0), N - 1);
-        }
-        if (wbin[(r[0] * N + r[1]) * N + r[2]] == 0.f) w = 0.f;
-    }
-    return w;
-}
 
 // mean of each particle volume (FP64 accumulation): one workgroup per particle
 __global__ void __launch_bounds__(kT) k_vol_mean(const float* __restrict__ vols, long n3, float* __restrict__ mean) {
@@ -215,7 +191,7 @@ __global__ void __launch_bounds__(kT) k_refine_dot(const float2* __restrict__ X1
     for (long h = (long)blockIdx.x * kT + threadIdx.x; h < nh; h += (long)gridDim.x * kT) {
         int a, b, c;
         split3((unsigned)h, (unsigned)N, (unsigned)H, a, b, c);
-        const float w = hweight(filter, wbin, g, h, a, b, c, N, H);
+        const float w = refine_hweight(filter, wbin, g, h, a, b, c, N, H);
         const float2 u = x1p[h], v = x2p[h];
         const double ww = (c == 0 || c == H - 1) ? (double)w * w : 2.0 * (double)w * w;
         sd += ww * ((double)u.x * v.x + (double)u.y * v.y);
@@ -277,7 +253,7 @@ __global__ void __launch_bounds__(kT) k_refine_adj(float2* __restrict__ X1, cons
     for (long h = (long)blockIdx.x * kT + threadIdx.x; h < nh; h += (long)gridDim.x * kT) {
         int a, b, c;
         split3((unsigned)h, (unsigned)N, (unsigned)H, a, b, c);
-        const float w = hweight(filter, wbin, g, h, a, b, c, N, H);
+        const float w = refine_hweight(filter, wbin, g, h, a, b, c, N, H);
         const float2 u = x1p[h], v = x2p[h];
         const float w2 = w * w;
         x1p[h] = make_float2(w2 * (ca * v.x - cb * u.x), w2 * (ca * v.y - cb * u.y));
@@ -384,6 +360,10 @@ void launch_refine_dot(const float2* X1, const float2* FP, const float* filter, 
                        double* part, double* sc, hipStream_t s) {
     const int nb = refine_dot_blocks(N);
     k_refine_dot<<<dim3(nb, E), kT, 0, s>>>(X1, FP, filter, wbin, geom, N, part);
+    k_refine_scalars<<<E, 64, 0, s>>>(part, nb, E, sc);
+    ETTK_HIP(hipGetLastError());
+}
+void launch_refine_scalars(const double* part, int nb, int E, double* sc, hipStream_t s) {
     k_refine_scalars<<<E, 64, 0, s>>>(part, nb, E, sc);
     ETTK_HIP(hipGetLastError());
 }
