@@ -7,6 +7,7 @@
 //            particle and every CCC request of all threads is served by one batched launch);
 //        2 = batched + adjoint refinement (refine.cpp).
 #include <algorithm>
+#include <cstring>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -125,24 +126,51 @@ static void eliminate_equivalent(std::vector<Cand>& pos, const std::vector<Mat4>
             long v = (long)std::floor(mats[i].a[c] / (double)tol) + G / 2;
             return (int)std::min<long>(std::max<long>(v, 1), G - 2);
         };
-        std::vector<int> head((size_t)G * G * G, -1), next(n, -1), key(n);
-        for (size_t i = n; i-- > 0;) {  // each cell's chain in increasing index order
-            int k = (cell(i, 2) * G + cell(i, 6)) * G + cell(i, 10);
-            key[i] = k;
-            next[i] = head[k];
-            head[k] = (int)i;
+        // the points sorted by cell (counting sort, increasing index within a cell), their matrices packed in that
+        // order: the three cells adjacent along the last key axis are one contiguous range
+        const size_t G3 = (size_t)G * G * G;
+        std::vector<int> key(n), start(G3 + 1, 0), order(n);
+        for (size_t i = 0; i < n; i++) {
+            key[i] = (cell(i, 2) * G + cell(i, 6)) * G + cell(i, 10);
+            start[key[i] + 1]++;
+        }
+        for (size_t k = 0; k < G3; k++) start[k + 1] += start[k];
+        std::vector<double> pm(16 * n);
+        {
+            std::vector<int> fill(start.begin(), start.end() - 1);
+            for (size_t i = 0; i < n; i++) {
+                const int s = fill[key[i]]++;
+                order[s] = (int)i;
+                std::memcpy(&pm[(size_t)s * 16], mats[i].a, 16 * sizeof(double));
+            }
         }
         // closeness does not depend on the visiting order: the close partners of i are found first, then visited
-        // in increasing index order as the exhaustive loop would
+        // in increasing index order as the exhaustive loop would (a partner already dropped is skipped there)
         std::vector<size_t> js;
         for (size_t i = 0; i < n; i++) {
             if (pos[i].v == NEG) continue;
             js.clear();
+            const double* mi = mats[i].a;
             for (int a = -1; a <= 1; a++)
-                for (int b = -1; b <= 1; b++)
-                    for (int c = -1; c <= 1; c++)
-                        for (int j = head[key[i] + (a * G + b) * G + c]; j >= 0; j = next[j])
-                            if ((size_t)j != i && close(i, (size_t)j)) js.push_back((size_t)j);
+                for (int b = -1; b <= 1; b++) {
+                    const int k0 = key[i] + (a * G + b) * G - 1;
+                    for (int sj = start[k0]; sj < start[k0 + 3]; sj++) {
+                        const int j = order[sj];
+                        if ((size_t)j == i || !(pos[j].v > NEG)) continue;
+                        const double* mj = &pm[(size_t)sj * 16];
+                        // the partial sum over the first three elements (the same additions in the same order)
+                        // already rejects most neighbours, with a branch that is nearly always taken
+                        const double d3 = std::fabs(mi[0] - mj[0]) + std::fabs(mi[1] - mj[1]) + std::fabs(mi[2] - mj[2]);
+                        if (!(d3 < tol)) continue;
+                        double err = 0;
+                        int m = 0;
+                        for (; m < 16; m++) {
+                            err += std::fabs(mi[m] - mj[m]);
+                            if (!(err < tol)) break;
+                        }
+                        if (m == 16) js.push_back((size_t)j);
+                    }
+                }
             std::sort(js.begin(), js.end());
             for (size_t j : js) {
                 if (pos[i].v == NEG) break;
