@@ -859,7 +859,11 @@ __global__ void __launch_bounds__(kCfT) k_combine_fftA(const float2* __restrict_
     __shared__ float2 pens[kCfP * LD];
     __shared__ float2 tw[L];
     __shared__ float2 wb[L];  // e^{+2 pi i (i - L/2) / 2L} for i = 0 .. L-1
-    const int j = blockIdx.x, e = blockIdx.y, E = z.E;
+    // the j planes of one evaluation on one XCD: a plane's rows of 33 complex values are not aligned to cache
+    // lines, and the lines they share with the neighbouring planes are then fetched into one L2 only
+    int j, e;
+    xcd_eval_tile(blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, e, j);
+    const int E = z.E;
     const long nh = (long)L * L * H;
     if (threadIdx.x < L) {
         tw[threadIdx.x] = tw64[threadIdx.x];
