@@ -70,9 +70,12 @@ def test_adjoint_gradient_vs_finite_differences(torch_gpu, ctx):
 
 
 @pytest.mark.gpu
-def test_score_and_gradient_vs_fp64(torch_gpu, ctx):
-    print("[refine] score/gradient vs FP64 oracle", flush=True)
-    p, ref, ps, plan, d = _setup(torch_gpu, ctx)
+@pytest.mark.parametrize("N", [32, 64])
+def test_score_and_gradient_vs_fp64(torch_gpu, ctx, N):
+    # 32 and 64 run different pencil transforms inside the score and adjoint i passes (LDS radix passes; the
+    # 8 x 8 register transform)
+    print(f"[refine] score/gradient vs FP64 oracle N={N}", flush=True)
+    p, ref, ps, plan, d = _setup(torch_gpu, ctx, N=N)
     orc = OraclePlan(p)
     orc.set_reference(ref)
     o = oracle()
