@@ -9,9 +9,10 @@ from oracle_lib import OraclePlan, so3_correlate
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("B", [8, 16])
+@pytest.mark.parametrize("B", [8, 16, 32])
 def test_so3_correlation_matches_oracle(torch_gpu, ctx, B):
-    """FP64 S^2 transforms + SO(3) synthesis on the GPU against the oracle (itself pinned to soft-1.0)."""
+    """FP64 S^2 transforms + SO(3) synthesis on the GPU against the oracle (itself pinned to soft-1.0); B = 32
+    runs the synthesis with wavefront-uniform coefficient loads and sixteen degrees per round trip."""
     from ettk_amd.lib import Plan
 
     print(f"[so3 correlate] B={B}", flush=True)
