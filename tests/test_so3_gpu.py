@@ -66,20 +66,22 @@ def test_spherical_rotation_matches_oracle(torch_gpu, ctx, B):
 
 
 @pytest.mark.gpu
-def test_projection_matches_oracle(torch_gpu, ctx):
-    """cartesianToSpherical: FP32 (2N)^3 FFT + tricubic S^2 sampling against the FP64 oracle."""
+@pytest.mark.parametrize("n", [3, 16])
+def test_projection_matches_oracle(torch_gpu, ctx, n):
+    """cartesianToSpherical: FP32 (2N)^3 FFT + tricubic S^2 sampling against the FP64 oracle.  With 16 particles
+    in one launch the sampling kernel deals its workgroups one particle per XCD; with 3 it keeps grid order."""
     from ettk_amd.lib import Plan
 
-    print("[projection] N=32 B=16", flush=True)
+    print(f"[projection] N=32 B=16 particles={n}", flush=True)
     p = default_params(32, 16)
     ref = blob_reference(32, seed=3)
-    ps = make_particles(ref, 3, seed=4, half_wedge=(55, 65), snr=(0.2, 0.2))
+    ps = make_particles(ref, n, seed=4, half_wedge=(55, 65), snr=(0.2, 0.2))
     plan = Plan(ctx, p)
     orc = OraclePlan(p)
     try:
         plan.set_reference(to_dev(torch_gpu, ref))
         pp = plan.project(to_dev(torch_gpu, ps.vols), ps.descs)
-        for i in range(3):
+        for i in range(n):
             pr = orc.project(ps.vols[i], ps.descs[i])
             assert np.max(np.abs(pp[i] - pr)) <= 2e-4 * np.max(np.abs(pr)), i
     finally:
