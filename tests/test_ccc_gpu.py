@@ -45,7 +45,8 @@ def test_ccc_matches_oracle(torch_gpu, ctx, N, mcf, wedge, overlap, tr):
     p = default_params(N, 16, mcf=mcf, missing_wedge=wedge, overlap_norm=overlap, trans_restrict=tr)
     ref = blob_reference(N, seed=11)
     ps = make_particles(ref, 2, seed=12, half_wedge=(50, 65), snr=(0.2, 0.2))
-    cases = _cases(ps, 3 if N <= 64 else 1, seed=N)
+    # 16 evaluations at N <= 64 (a multiple of 8: the reslice and FFT combine deal one evaluation per XCD)
+    cases = _cases(ps, 6 if N <= 64 else 1, seed=N)
     orc = OraclePlan(p)
     orc.set_reference(ref)
     plan = Plan(ctx, p)
