@@ -103,6 +103,21 @@ static void true_local_maxima(const Plan& P, const std::vector<double>& C, CandQ
     eliminate_equivalent(pos, pq, tol);
 }
 
+std::vector<Cand> candidates_of(const Plan& P, std::vector<double> C, float tol) {
+    const int n = 2 * P.p.B;
+    restrict_beta(C, P.p.B, std::max((double)P.p.rot_restrict, 1.0));
+    std::vector<double> L;
+    local_maxima(C, n, L);
+    CandQueue pq;
+    true_local_maxima(P, L, pq, tol);
+    std::vector<Cand> out;
+    while (!pq.empty()) {
+        out.push_back(pq.top());
+        pq.pop();
+    }
+    return out;
+}
+
 // --------------------------------------------------------------------------- vnl Powell restatement
 namespace {
 struct PowellState {
@@ -611,6 +626,41 @@ void orc_masks(void* h, float* maskfft, float* refmask, float* filter) {
     std::memcpy(filter, P.filter.data(), P.filter.size() * sizeof(float));
 }
 int orc_ccc_count(void* h) { return ((Plan*)h)->ccc_count; }
+
+// executeFourierNewHalf with its search window: T (ext^3 doubles, [i][j][k], before the spherical mask) and
+// info = {k1_m, ext}; returns correlationPeak, t_inout <- t * Translate(-shift)
+float orc_ccc_trace(void* h, const float* part, const ettk_particle* d, double* t_inout, double* T, double* info) {
+    Plan& P = *(Plan*)h;
+    M4 t = m4_from(t_inout);
+    CccTrace tr;
+    float v = ccc_eval(P, part, *d, t, &tr);
+    std::memcpy(t_inout, t.a, sizeof(t.a));
+    if (T) std::memcpy(T, tr.T.data(), tr.T.size() * sizeof(double));
+    info[0] = tr.k1_m;
+    info[1] = tr.ext;
+    return v;
+}
+
+// the surviving candidates of one correlation volume, in priority order: out[4 i] = (alpha, beta, gamma, -C)
+int orc_candidates(void* h, const double* C, float tol, float* out, int max_out) {
+    Plan& P = *(Plan*)h;
+    const size_t n = (size_t)2 * P.p.B;
+    std::vector<double> c(C, C + n * n * n);
+    std::vector<Cand> v = candidates_of(P, c, tol);
+    const int m = std::min((int)v.size(), max_out);
+    for (int i = 0; i < m; i++) {
+        out[4 * i] = v[i].a;
+        out[4 * i + 1] = v[i].b;
+        out[4 * i + 2] = v[i].g;
+        out[4 * i + 3] = v[i].v;
+    }
+    return (int)v.size();
+}
+
+// nbfImageMetric::isTransformValid (nbfImageMetric.h:416-459); trans_restrict < 0 is unrestricted
+int orc_transform_valid(const double* t, double trans_restrict, double rot_restrict) {
+    return transform_valid(m4_from(t), trans_restrict < 0 ? (double)FLT_MAX : trans_restrict, std::max(rot_restrict, 1.0)) ? 1 : 0;
+}
 
 // Amplitude-weighted Fourier average (WAvg.h:261-356, WAvg.h:217-258); out is N^3 doubles, x fastest.
 void orc_average(void* h, int n, const float* parts, const ettk_particle* d, const float* weights,

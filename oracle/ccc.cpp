@@ -262,7 +262,7 @@ float wedge_overlap(const Plan& P, const ettk_particle& d, const M4& t) {
 }
 
 // locateMaxima with blitz maxIndex (first maximum in row-major order)
-float ccc_eval(Plan& P, const float* part, const ettk_particle& d, M4& t) {
+float ccc_eval(Plan& P, const float* part, const ettk_particle& d, M4& t, CccTrace* trace) {
     const int N = P.N, H = P.H;
     const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * H;
     P.ccc_count++;
@@ -330,6 +330,11 @@ float ccc_eval(Plan& P, const float* part, const ettk_particle& d, M4& t) {
     int ext = I1 - I0 + 1;
     int ub2 = (ext - 1) / 2;
     double big_n3 = (double)N2 * N2 * N2;
+    if (trace) {
+        trace->ext = ext;
+        trace->k1_m = (double)k1_m;
+        trace->T.resize((size_t)ext * ext * ext);
+    }
     double best = -DBL_MAX;
     int bi = -1, bj = -1, bk = -1;
     for (int i = 0; i < ext; i++)
@@ -337,6 +342,7 @@ float ccc_eval(Plan& P, const float* part, const ettk_particle& d, M4& t) {
             for (int k = 0; k < ext; k++) {
                 int gi = I0 + i, gj = I0 + j, gk = I0 + k;
                 double v = Tb[((size_t)gi * N2 + gj) * N2 + gk] * (((gi + gj + gk) & 1) ? -1.0 : 1.0) / big_n3;
+                if (trace) trace->T[((size_t)i * ext + j) * ext + k] = v;
                 float rad = (float)((ub2 - i) * (ub2 - i) + (ub2 - j) * (ub2 - j) + (ub2 - k) * (ub2 - k));
                 if (!(rad < restriction * restriction)) v = -DBL_MAX;
                 if (bi < 0 || v > best) { best = v; bi = i; bj = j; bk = k; }

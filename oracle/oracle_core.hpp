@@ -100,8 +100,15 @@ AvgWedge avg_wedge(int N, int B, int n, const ettk_particle* d, const float* w, 
 void plan_set_reference_average(Plan& P, const float* ref, const AvgWedge& aw);
 // nbfCorrelationImageMetric::getWedgeOverlap(t) (CorrMetric.h:533-566) of the reference against particle d
 float wedge_overlap(const Plan& P, const ettk_particle& d, const M4& t);
+// what ccc_eval leaves behind for the checks of its arithmetic: the search window T of the upsampled
+// correlation (ext^3 values, before the spherical mask) and k1_m
+struct CccTrace {
+    std::vector<double> T;
+    int ext = 0;
+    double k1_m = 0;
+};
 // executeFourierNewHalf: returns correlationPeak, updates t (t <- t * Translate(-shift))
-float ccc_eval(Plan& P, const float* part, const ettk_particle& d, M4& t);
+float ccc_eval(Plan& P, const float* part, const ettk_particle& d, M4& t, CccTrace* trace = nullptr);
 
 // ---------------------------------------------------------------- S^2 / SO(3)
 void project_image(Plan& P, std::vector<double> img, std::vector<float>& Pp);  // cartesianToSpherical
@@ -120,6 +127,11 @@ void wigner_d_all(int B, double beta, std::vector<double>& d);
 
 // ---------------------------------------------------------------- driver
 struct Cand { float a, b, g, v; };
+// the candidate chain of one correlation volume C ((2B)^3, [alpha][beta][gamma]): restriction of beta to the
+// rotation restriction, 26-neighbour local maxima with a -FLT_MAX border, decoding, validity and the
+// elimination of equivalent rotations; the surviving candidates in priority-queue order (ProjRot.h:590-640,
+// :1593-1702)
+std::vector<Cand> candidates_of(const Plan& P, std::vector<double> C, float tol);
 void align_pair(Plan& P, const float* part, const ettk_particle& d, ettk_result& out);
 void powell_refine(Plan& P, const float* part, const ettk_particle& d, M4& t, bool run);
 

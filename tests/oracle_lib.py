@@ -3,6 +3,7 @@ reference sources (oracle/_ref/libsoftref.so).  Test infrastructure only."""
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
@@ -10,7 +11,9 @@ import numpy as np
 from ettk_amd.abi import Params, Particle, Result
 
 ROOT = Path(__file__).resolve().parents[1]
-ORACLE = ROOT / "oracle" / "liboracle.so"
+# ETTK_ORACLE_LIB points the checks at another build of the oracle (a copy with a planted mistake, to show
+# that the checks catch it)
+ORACLE = Path(os.environ.get("ETTK_ORACLE_LIB", str(ROOT / "oracle" / "liboracle.so")))
 SOFTREF = ROOT / "oracle" / "_ref" / "libsoftref.so"
 
 _o = None
