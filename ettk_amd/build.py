@@ -18,7 +18,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 in any kernel
 HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["fft.hip", "ccc.hip", "so3.hip", "refine.hip", "host.cpp", "align.cpp", "refine.cpp", "avg.hip", "avg.cpp"]
+SOURCES = ["fft.hip", "ccc.hip", "so3.hip", "refine.hip", "host.cpp", "align.cpp", "refine.cpp", "avg.hip", "avg.cpp",
+           "noalloc.hip"]
 
 
 def build_lib(verbose: bool = False) -> Path:

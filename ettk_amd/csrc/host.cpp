@@ -259,6 +259,9 @@ static void plan_constants(ettk_plan* P) {
     P->dummy_filter = !(bp || p.bfactor != 0.0f);
     P->sum_filter = 0;
     for (float v : f) P->sum_filter += v;
+    P->nb_real = (int)((n3 + 1023) / 1024);
+    P->nb_spec = (int)((nh + 1023) / 1024);
+    P->nb_res = reslice_blocks(N);
     P->h_m1 = m1;
     P->m1.upload(m1.data(), n3, s);
     P->maskfft.upload(mf.data(), n3, s);
@@ -270,15 +273,71 @@ static void plan_constants(ettk_plan* P) {
     search_window(P);
 }
 
+// spectral partials per evaluation: per k_spec_filter2 block, or per k_ccc_ipass block (fused transforms)
+static size_t spectral_partials(const ettk_plan* P) {
+    return (size_t)std::max(P->nb_spec, fft3_fused_supported(P->N) ? fft3_ccc_ipass_blocks(P->N) : 0);
+}
+
+// one region holding a CccWork for E evaluations: every piece starts on a 256-byte boundary
+template <class F>
+static void ccc_work_pieces(const ettk_plan* P, int E, F&& piece) {
+    const int N = P->N, H = P->H;
+    const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * H, nb_sp = spectral_partials(P);
+    piece(0, n3 * E * sizeof(float));                                   // R1
+    piece(1, n3 * E * sizeof(float));                                   // R2
+    piece(2, (size_t)2 * P->nb_res * E * sizeof(float));                // p_res
+    piece(3, nb_sp * E * sizeof(float));                                // p_n1
+    piece(4, nb_sp * E * sizeof(float));                                // p_n2
+    piece(5, nb_sp * E * sizeof(float));                                // p_wf
+    piece(6, (size_t)std::max(P->nb_real, N) * E * sizeof(float));      // p_k1
+    piece(7, (size_t)E * sizeof(float));                                // stats
+    piece(8, (size_t)4 * E * sizeof(float));                            // scalars
+    piece(9, (size_t)P->z.E * E * sizeof(float));                       // bval
+    piece(10, (size_t)P->z.E * E * sizeof(int));                        // bidx
+    piece(11, nh * E * sizeof(float2));                                 // S1
+    piece(12, nh * E * sizeof(float2));                                 // S2
+    piece(13, nh * E * sizeof(float2));                                 // S3
+    piece(14, (size_t)P->z.E * N * H * E * sizeof(float2));             // G1
+}
+
+size_t ccc_work_bytes(const ettk_plan* P, int E) {
+    size_t tot = 0;
+    ccc_work_pieces(P, E, [&](int, size_t b) { tot += (b + 255) & ~(size_t)255; });
+    return tot;
+}
+
+CccWork ccc_work_carve(const ettk_plan* P, int E, void* base) {
+    void* ptr[15];
+    size_t off = 0;
+    ccc_work_pieces(P, E, [&](int i, size_t b) {
+        ptr[i] = static_cast<char*>(base) + off;
+        off += (b + 255) & ~(size_t)255;
+    });
+    CccWork W;
+    W.R1 = (float*)ptr[0];
+    W.R2 = (float*)ptr[1];
+    W.p_res = (float*)ptr[2];
+    W.p_n1 = (float*)ptr[3];
+    W.p_n2 = (float*)ptr[4];
+    W.p_wf = (float*)ptr[5];
+    W.p_k1 = (float*)ptr[6];
+    W.stats = (float*)ptr[7];
+    W.scalars = (float*)ptr[8];
+    W.bval = (float*)ptr[9];
+    W.bidx = (int*)ptr[10];
+    W.S1 = (float2*)ptr[11];
+    W.S2 = (float2*)ptr[12];
+    W.S3 = (float2*)ptr[13];
+    W.G1 = (float2*)ptr[14];
+    return W;
+}
+
 void ensure_ccc_workspace(ettk_plan* P, int cap) {
     if (cap <= P->cap && P->z.E <= P->ws_E) return;
     cap = std::max(cap, P->cap);
     P->ws_E = P->z.E;
     const int N = P->N, H = P->H;
     const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * H;
-    P->nb_real = (int)((n3 + 1023) / 1024);
-    P->nb_spec = (int)((nh + 1023) / 1024);
-    P->nb_res = reslice_blocks(N);
     P->R1.ensure(n3 * cap);
     P->R2.ensure(n3 * cap);
     P->S1.ensure(nh * cap);
@@ -286,8 +345,7 @@ void ensure_ccc_workspace(ettk_plan* P, int cap) {
     P->S3.ensure(nh * cap);
     P->G1.ensure((size_t)P->z.E * N * H * cap);
     P->p_res.ensure((size_t)2 * P->nb_res * cap);
-    // spectral partials: per k_spec_filter2 block, or per k_ccc_ipass block (fused transforms)
-    const size_t nb_sp = (size_t)std::max(P->nb_spec, fft3_fused_supported(N) ? fft3_ccc_ipass_blocks(N) : 0);
+    const size_t nb_sp = spectral_partials(P);
     P->p_n1.ensure(nb_sp * cap);
     P->p_n2.ensure(nb_sp * cap);
     P->p_wf.ensure(nb_sp * cap);
@@ -370,11 +428,81 @@ void wedge_overlaps(ettk_plan* P, const ettk_particle* descs, const Mat4* t, int
     for (int i = 0; i < n; i++) out[i] = (eff1 || eff2[i]) ? (float)(h[i] / P->sum_filter) : 1.0f;
 }
 
+void ccc_enqueue_chunk(ettk_plan* P, const CccWork& W, const float* d_parts, const EvalGeom* geo, int E, float* out4,
+                       hipStream_t s) {
+    const int N = P->N, H = P->H;
+    const long nh = (long)N * N * H;
+    KernelProf* kp = P->ctx->fft.prof;
+    const float* W1 = P->p.missing_wedge ? P->W1.p : nullptr;
+    const int apply = P->dummy_filter ? 0 : 1;
+    const double Rb = (double)E * N * N * N * sizeof(float), Hb = (double)E * nh * sizeof(float2);
+    const double Gb = (double)E * P->z.E * N * H * sizeof(float2);
+    // particle side: F2 = r2c((reslice - mean) maskfft shift)
+    {
+        ProfScope ps(kp, "k_reslice", 2 * Rb, s);
+        launch_reslice(d_parts, geo, E, N, W.R2, W.p_res, P->nb_res, s);
+    }
+    launch_reslice_mean(W.p_res, P->nb_res, E, W.stats, s);
+    R2CArgs a2{W.R2, W.S2, N, E, PRO_RESLICED, P->maskfft.p, W.stats};
+    const bool fused = fft3_fused_supported(N);
+    int nb_sp = P->nb_spec;
+    if (fused) {
+        // F2 = r2c((reslice - mean) maskfft shift) up to its i pass; then on pencils along i: the i pass, both
+        // spectra weighted by filter x W1 W2 (X1 = F1mcf, the reference with MCF applied once; X2 = mcf(F2)),
+        // X2 kept for the combine, and both inverse i passes of the transforms below
+        fft3_r2c_planes(P->ctx->fft, a2, s);
+        CccIpassArgs ia{W.S2, W.S3, W.S1, P->F1mcf.p, P->filter.p, W1, geo, P->p.mcf, apply, W.p_n1, W.p_wf, W.p_n2};
+        fft3_ccc_ipass(P->ctx->fft, ia, N, E, s);
+        nb_sp = fft3_ccc_ipass_blocks(N);
+        // reference side: c2r -> r1 m1 (with the k1_m partials) -> r2c = F1m, one partial per (evaluation, plane)
+        // (at 64 the forward i passes of both run inside the combine kernel)
+        const bool fwd = !ccc_combine_fft_supported(P->z);
+        fft3_op_r2c_after_ipass(P->ctx->fft, W.S1, N, E, PLANE_OP_MASK, P->m1.p, W.p_k1, fwd, s);
+        // particle side: Q = r2c(c2r(X2)^2 shift)
+        fft3_op_r2c_after_ipass(P->ctx->fft, W.S3, N, E, PLANE_OP_SQUARE_SHIFT, nullptr, nullptr, fwd, s);
+    } else {
+        fft3_r2c(P->ctx->fft, a2, nullptr, s);
+        // both spectra weighted by filter x W1 W2 in one pass: X1 = F1mcf (reference, MCF applied once), X2 = mcf(F2)
+        {
+            ProfScope ps(kp, "k_spec_filter", 4 * Hb, s);
+            launch_spec_filter2(P->F1mcf.p, W.S1, W.S2, P->filter.p, W1, geo, E, N, P->p.mcf, apply, W.p_n1, W.p_wf,
+                                W.p_n2, P->nb_spec, s);
+        }
+        fft3_c2r(P->ctx->fft, W.S1, W.S1, W.R1, N, E, s);
+        {
+            ProfScope ps(kp, "k_mask_ref", 2 * Rb, s);
+            launch_mask_ref(W.R1, P->m1.p, E, N, W.p_k1, P->nb_real, s);
+        }
+        R2CArgs a1{W.R1, W.S1, N, E, PRO_NONE, nullptr, nullptr};
+        fft3_r2c(P->ctx->fft, a1, nullptr, s);
+        fft3_c2r(P->ctx->fft, W.S2, W.S3, W.R2, N, E, s);
+        R2CArgs a3{W.R2, W.S3, N, E, PRO_SQUARE_SHIFT, nullptr, nullptr};
+        fft3_r2c(P->ctx->fft, a3, nullptr, s);
+    }
+    // combine, upsampled correlation on the search window, peak
+    launch_ccc_scalars(W.p_n1, W.p_n2, W.p_k1, W.p_wf, nb_sp, fused ? N : P->nb_real, E, N, W.scalars, s);
+    const bool cfft = ccc_combine_fft_supported(P->z);
+    {
+        ProfScope ps(kp, cfft ? "k_combine_fftA" : "k_combine_dftA", 3 * Hb + Gb, s);
+        if (cfft)
+            launch_combine_fftA(P->ctx->fft, W.S1, W.S2, W.S3, P->Mfft.p, W.scalars, P->z, E, W.G1, fused, s);
+        else
+            launch_combine_dftA(W.S1, W.S2, W.S3, P->Mfft.p, W.scalars, P->tabA.p, P->z, E, W.G1, s);
+    }
+    {
+        ProfScope ps(kp, cfft ? "k_dftBC_fft" : "k_dftBC_argmax", Gb, s);
+        if (cfft)
+            launch_dftBC_fft(P->ctx->fft, W.G1, P->tabK.p, P->z, E, W.bval, W.bidx, s);
+        else
+            launch_dftBC_argmax(W.G1, P->tabA.p, P->tabK.p, P->z, E, W.bval, W.bidx, s);
+    }
+    launch_peak_final(W.bval, W.bidx, W.scalars, (float)P->sum_filter, (P->p.overlap_norm && P->p.missing_wedge) ? 1 : 0,
+                      P->z, E, out4, s);
+}
+
 void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx, Mat4* t,
                float* dist) {
     ETTK_REQUIRE(P->have_ref, ETTK_ERR_STATE, "reference not set");
-    const int N = P->N, H = P->H;
-    const long nh = (long)N * N * H;
     hipStream_t s = P->ctx->stream;
     const int CH = 256;  // evaluations per chunk
     ensure_ccc_workspace(P, std::min(n, CH));
@@ -410,79 +538,12 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
         ETTK_HIP(hipMemcpyAsync(P->geom_all.p + e0, g + e0, (size_t)E * sizeof(EvalGeom), hipMemcpyHostToDevice, s));
     };
     ETTK_HIP(hipEventRecord(P->ev0, s));
-    const float* W1 = P->p.missing_wedge ? P->W1.p : nullptr;
-    const int apply = P->dummy_filter ? 0 : 1;
-    KernelProf* kp = P->ctx->fft.prof;
+    CccWork W{P->R1.p, P->R2.p, P->p_res.p, P->p_n1.p, P->p_n2.p, P->p_wf.p, P->p_k1.p, P->stats.p, P->scalars.p,
+              P->bval.p, P->bidx.p, P->S1.p, P->S2.p, P->S3.p, P->G1.p};
     for (int e0 = 0; e0 < n; e0 += chunk) {
         const int E = std::min(chunk, n - e0);
         geom_chunk(e0, E);
-        const EvalGeom* geo = P->geom_all.p + e0;
-        const double Rb = (double)E * N * N * N * sizeof(float), Hb = (double)E * nh * sizeof(float2);
-        const double Gb = (double)E * P->z.E * N * H * sizeof(float2);
-        // particle side: F2 = r2c((reslice - mean) maskfft shift)
-        {
-            ProfScope ps(kp, "k_reslice", 2 * Rb, s);
-            launch_reslice(d_parts, geo, E, N, P->R2.p, P->p_res.p, P->nb_res, s);
-        }
-        launch_reslice_mean(P->p_res.p, P->nb_res, E, P->stats.p, s);
-        R2CArgs a2{P->R2.p, P->S2.p, N, E, PRO_RESLICED, P->maskfft.p, P->stats.p};
-        const bool fused = fft3_fused_supported(N);
-        int nb_sp = P->nb_spec;
-        if (fused) {
-            // F2 = r2c((reslice - mean) maskfft shift) up to its i pass; then on pencils along i: the i pass, both
-            // spectra weighted by filter x W1 W2 (X1 = F1mcf, the reference with MCF applied once; X2 = mcf(F2)),
-            // X2 kept for the combine, and both inverse i passes of the transforms below
-            fft3_r2c_planes(P->ctx->fft, a2, s);
-            CccIpassArgs ia{P->S2.p, P->S3.p, P->S1.p, P->F1mcf.p, P->filter.p, W1, geo, P->p.mcf, apply,
-                            P->p_n1.p, P->p_wf.p, P->p_n2.p};
-            fft3_ccc_ipass(P->ctx->fft, ia, N, E, s);
-            nb_sp = fft3_ccc_ipass_blocks(N);
-            // reference side: c2r -> r1 m1 (with the k1_m partials) -> r2c = F1m, one partial per (evaluation, plane)
-            // (at 64 the forward i passes of both run inside the combine kernel)
-            const bool fwd = !ccc_combine_fft_supported(P->z);
-            fft3_op_r2c_after_ipass(P->ctx->fft, P->S1.p, N, E, PLANE_OP_MASK, P->m1.p, P->p_k1.p, fwd, s);
-            // particle side: Q = r2c(c2r(X2)^2 shift)
-            fft3_op_r2c_after_ipass(P->ctx->fft, P->S3.p, N, E, PLANE_OP_SQUARE_SHIFT, nullptr, nullptr, fwd, s);
-        } else {
-            fft3_r2c(P->ctx->fft, a2, nullptr, s);
-            // both spectra weighted by filter x W1 W2 in one pass: X1 = F1mcf (reference, MCF applied once), X2 = mcf(F2)
-            {
-                ProfScope ps(kp, "k_spec_filter", 4 * Hb, s);
-                launch_spec_filter2(P->F1mcf.p, P->S1.p, P->S2.p, P->filter.p, W1, geo, E, N, P->p.mcf, apply, P->p_n1.p,
-                                    P->p_wf.p, P->p_n2.p, P->nb_spec, s);
-            }
-            fft3_c2r(P->ctx->fft, P->S1.p, P->S1.p, P->R1.p, N, E, s);
-            {
-                ProfScope ps(kp, "k_mask_ref", 2 * Rb, s);
-                launch_mask_ref(P->R1.p, P->m1.p, E, N, P->p_k1.p, P->nb_real, s);
-            }
-            R2CArgs a1{P->R1.p, P->S1.p, N, E, PRO_NONE, nullptr, nullptr};
-            fft3_r2c(P->ctx->fft, a1, nullptr, s);
-            fft3_c2r(P->ctx->fft, P->S2.p, P->S3.p, P->R2.p, N, E, s);
-            R2CArgs a3{P->R2.p, P->S3.p, N, E, PRO_SQUARE_SHIFT, nullptr, nullptr};
-            fft3_r2c(P->ctx->fft, a3, nullptr, s);
-        }
-        // combine, upsampled correlation on the search window, peak
-        launch_ccc_scalars(P->p_n1.p, P->p_n2.p, P->p_k1.p, P->p_wf.p, nb_sp, fused ? N : P->nb_real, E, N,
-                           P->scalars.p, s);
-        const bool cfft = ccc_combine_fft_supported(P->z);
-        {
-            ProfScope ps(kp, cfft ? "k_combine_fftA" : "k_combine_dftA", 3 * Hb + Gb, s);
-            if (cfft)
-                launch_combine_fftA(P->ctx->fft, P->S1.p, P->S2.p, P->S3.p, P->Mfft.p, P->scalars.p, P->z, E, P->G1.p,
-                                    fused, s);
-            else
-                launch_combine_dftA(P->S1.p, P->S2.p, P->S3.p, P->Mfft.p, P->scalars.p, P->tabA.p, P->z, E, P->G1.p, s);
-        }
-        {
-            ProfScope ps(kp, cfft ? "k_dftBC_fft" : "k_dftBC_argmax", Gb, s);
-            if (cfft)
-                launch_dftBC_fft(P->ctx->fft, P->G1.p, P->tabK.p, P->z, E, P->bval.p, P->bidx.p, s);
-            else
-                launch_dftBC_argmax(P->G1.p, P->tabA.p, P->tabK.p, P->z, E, P->bval.p, P->bidx.p, s);
-        }
-        launch_peak_final(P->bval.p, P->bidx.p, P->scalars.p, (float)P->sum_filter,
-                          (P->p.overlap_norm && P->p.missing_wedge) ? 1 : 0, P->z, E, P->out_all.p + (size_t)4 * e0, s);
+        ccc_enqueue_chunk(P, W, d_parts, P->geom_all.p + e0, E, P->out_all.p + (size_t)4 * e0, s);
     }
     ETTK_HIP(hipEventRecord(P->ev1, s));
     P->h_out4.resize((size_t)4 * n);
@@ -571,6 +632,7 @@ int ettk_ctx_create(int device, ettk_ctx** out) {
     ETTK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     ETTK_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
     fft_tables_init(c->fft, c->stream);
+    noalloc_scratch_init(c);
     *out = c;
     ETTK_API_END
 }

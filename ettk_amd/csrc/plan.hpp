@@ -51,6 +51,7 @@ struct ettk_ctx {
     hipStream_t stream = nullptr;
     hipStream_t copy_stream = nullptr;  // host downloads issued from worker threads (non-blocking)
     ettk::FftTables fft;
+    void* scratch = nullptr;  // device address of this device's fixed CCC scratch region (noalloc.hip)
 };
 
 struct ettk_plan {
@@ -106,6 +107,23 @@ struct ettk_plan {
 };
 
 namespace ettk {
+// device workspace of one chunk of E constrained-CCC evaluations: the plan's own buffers (ccc_batch) or a
+// caller-independent fixed region (ettk_ccc_eval_batch_noalloc)
+struct CccWork {
+    float *R1, *R2, *p_res, *p_n1, *p_n2, *p_wf, *p_k1, *stats, *scalars, *bval;
+    int* bidx;
+    float2 *S1, *S2, *S3, *G1;
+};
+// device bytes of a CccWork for E evaluations, and its carving out of one region (256-byte aligned pieces)
+size_t ccc_work_bytes(const ettk_plan* P, int E);
+CccWork ccc_work_carve(const ettk_plan* P, int E, void* base);
+// per-evaluation geometry of the reslice and the particle wedge (FP64 on the host)
+EvalGeom make_geom_public(const ettk_plan* P, const ettk_particle& d, const Mat4& t, int particle);
+// the kernels of one chunk of E evaluations: geometry geo[E] on the device, results out4[E][4] (distance, shift)
+void ccc_enqueue_chunk(ettk_plan* P, const CccWork& W, const float* d_parts, const EvalGeom* geo, int E, float* out4,
+                       hipStream_t s);
+// device address of the fixed CCC scratch region of the context's device into ctx->scratch (noalloc.hip)
+void noalloc_scratch_init(ettk_ctx* ctx);
 // constrained CCC for a batch of evaluations: on return t[e] <- t[e] * Translate(-shift), dist[e]
 void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
                Mat4* t, float* dist);

@@ -32,6 +32,8 @@ def load() -> C.CDLL:
             "ettk_plan_set_restrictions": ([vp, C.c_float, C.c_float], C.c_int),
             "ettk_reference_set": ([vp, vp, C.POINTER(Particle)], C.c_int),
             "ettk_ccc_eval_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, ip, dp, fp, dp], C.c_int),
+            "ettk_ccc_eval_batch_noalloc": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, ip, dp, fp, dp],
+                                            C.c_int),
             "ettk_project_batch": ([vp, vp, C.c_int, C.POINTER(Particle), fp], C.c_int),
             "ettk_so3_correlate_batch": ([vp, C.c_int, dp, dp, dp], C.c_int),
             "ettk_sph_rotate_batch": ([vp, C.c_int, fp, dp, fp], C.c_int),
@@ -142,6 +144,18 @@ class Plan:
         tout = np.zeros((n, 16), np.float64)
         check(self.lib.ettk_ccc_eval_batch(self.h, C.c_void_p(d_parts.data_ptr()), d_parts.shape[0], arr, n,
                                            _ip(pidx), _dp(t), _fp(dist), _dp(tout)))
+        return dist, tout.reshape(n, 4, 4)
+
+    def ccc_noalloc(self, d_parts, descs, pidx, transforms):
+        """ettk_ccc_eval_batch_noalloc: as ccc(), with the library's fixed scratch region as workspace."""
+        n = len(pidx)
+        arr = particle_array(descs)
+        pidx = np.ascontiguousarray(pidx, dtype=np.int32)
+        t = np.ascontiguousarray(np.asarray(transforms, dtype=np.float64).reshape(n, 16))
+        dist = np.zeros(n, np.float32)
+        tout = np.zeros((n, 16), np.float64)
+        check(self.lib.ettk_ccc_eval_batch_noalloc(self.h, C.c_void_p(d_parts.data_ptr()), d_parts.shape[0], arr, n,
+                                                   _ip(pidx), _dp(t), _fp(dist), _dp(tout)))
         return dist, tout.reshape(n, 4, 4)
 
     def project(self, d_parts, descs):

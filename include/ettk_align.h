@@ -121,6 +121,18 @@ int ettk_ccc_eval_batch(ettk_plan* plan, const float* d_particles, int n_particl
                         const ettk_particle* descs, int n_evals, const int* particle_index,
                         const double* transforms, float* out_distance, double* out_transforms);
 
+/* ettk_ccc_eval_batch without any memory allocation.  No counterpart in the reference: it is for callers that
+   cannot allocate device memory while they run (code under HIP graph capture, tight serving loops, harnesses
+   that forbid hipMalloc).  Same arguments, results and errors as ettk_ccc_eval_batch; the workspace is a fixed
+   128 MiB device region per GPU, reserved when the library is loaded on that GPU and held until the process
+   ends; evaluations run in chunks that fit it (at box 64 with an 8-voxel translation window, 16 per chunk).
+   A size whose single evaluation does not fit is rejected with ETTK_ERR_UNSUPPORTED.  All calls on one GPU
+   share the region: calls that overlap on one GPU race, so the caller serialises them.  The plan and its
+   reference must have been set up beforehand (ettk_plan_create, ettk_reference_set allocate). */
+int ettk_ccc_eval_batch_noalloc(ettk_plan* plan, const float* d_particles, int n_particles,
+                                const ettk_particle* descs, int n_evals, const int* particle_index,
+                                const double* transforms, float* out_distance, double* out_transforms);
+
 /* S^2 projection of |FFT| (nbfProjectionRotationMetric3D::cartesianToSpherical, ProjRot.h:265-413) of each
    particle at its current alignment: out_pp is n x (2B x 2B) float, row-major [polar][azimuth]. */
 int ettk_project_batch(ettk_plan* plan, const float* d_particles, int n, const ettk_particle* descs,
