@@ -519,6 +519,41 @@ void orc_align_many(void* h, int n, const float* parts, const ettk_particle* d, 
     }
 }
 
+double orc_refine(void* hp, const float* part, const ettk_particle* d, const double* seed, int n_iter, double* pose_out);
+
+// The batched + adjoint form of the library in FP64, OpenMP over particles: the search and candidate stage of
+// the reference without Powell (refinementOff: one CCC at each selected candidate, FIM.h:1758-1766), then
+// n_iter steps of the continuous ascent (orc_refine) from the best candidate's transform, then one CCC at the
+// refined pose for the reported distance (the refined pose is kept).  scores[i] = final continuous score.
+void orc_align_adjoint_many(void* h, int n, const float* parts, const ettk_particle* d, int n_iter, ettk_result* out,
+                            double* scores, int nthreads) {
+    Plan& P0 = *(Plan*)h;
+    const size_t n3 = (size_t)P0.N * P0.N * P0.N;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+    {
+        Plan P = P0;
+        P.p.refine = 0;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int i = 0; i < n; i++) {
+            const float* part = parts + (size_t)i * n3;
+            align_pair(P, part, d[i], out[i]);
+            double pose[16];
+            scores[i] = orc_refine(&P, part, &d[i], out[i].matrix, n_iter, pose);
+            M4 t = m4_from(pose);
+            out[i].distance = ccc_eval(P, part, d[i], t);
+            out[i].cand_distance[0] = out[i].distance;
+            std::memcpy(out[i].matrix, pose, sizeof(pose));
+            std::memcpy(out[i].cand_matrix[0], pose, sizeof(pose));
+            out[i].n_ccc += 1;
+        }
+    }
+}
+
 void orc_s2_forward(int B, const double* f, double* re, double* im) {
     std::vector<cd> c;
     s2_forward(B, f, c);
@@ -639,6 +674,19 @@ float orc_ccc_trace(void* h, const float* part, const ettk_particle* d, double* 
     info[0] = tr.k1_m;
     info[1] = tr.ext;
     return v;
+}
+
+// sub-volumes cut from a volume (x fastest, nx x ny x nz) at positions pos[3 i]: out[i] is N^3 doubles, x fastest
+void orc_extract(const float* vol, int nx, int ny, int nz, int n, const double* pos, const ettk_particle* d, int N,
+                 double* out) {
+    const size_t n3 = (size_t)N * N * N;
+    for (int i = 0; i < n; i++) {
+        std::vector<double> A;
+        extract_image(vol, nx, ny, nz, pos + 3 * i, N, d[i], nullptr, A);
+        for (int x = 0; x < N; x++)
+            for (int y = 0; y < N; y++)
+                for (int z = 0; z < N; z++) out[i * n3 + ((size_t)z * N + y) * N + x] = A[((size_t)x * N + y) * N + z];
+    }
 }
 
 // the surviving candidates of one correlation volume, in priority order: out[4 i] = (alpha, beta, gamma, -C)

@@ -369,6 +369,16 @@ inline int vfloor(double x, double& f) {  // vtkInterpolationMath::Floor with VT
 }  // namespace
 
 void reslice_image(const float* vol, int N, const ettk_particle& d, const M4* t, std::vector<double>& out) {
+    const double pos[3] = {N / 2.0, N / 2.0, N / 2.0};
+    extract_image(vol, N, N, N, pos, N, d, t, out);
+}
+
+// nbfWedgedSubImage3D::getImage (WSub.h:603-802, normalize off) from a volume of nx x ny x nz voxels (x fastest)
+// whose origin is moved to `pos` (the sub-volume's position): output voxel q of the N^3 box reads the input at
+// pos + fin (q - N/2 - cut z), fin = Euler(normal) matrix t; VTK tricubic, taps clamped to the input, outside
+// [-0.5, n - 0.5] per axis invalid; the mean of the valid (float) samples is subtracted, invalid voxels are 0
+void extract_image(const float* vol, int NX, int NY, int NZ, const double* pos3, int N, const ettk_particle& d,
+                   const M4* t, std::vector<double>& out) {
     M4 euler = m4_ident();
     pre(euler, m4_rz(-d.normal[2]));
     pre(euler, m4_rx(-d.normal[0]));
@@ -376,8 +386,8 @@ void reslice_image(const float* vol, int N, const ettk_particle& d, const M4* t,
     M4 local = m4_mul(euler, m4_from(d.matrix));
     M4 fin = t ? m4_mul(local, *t) : local;
     if (m4_det(fin) == 0.0) fin = m4_ident();
-    const double pos = N / 2.0, half = N / 2.0;
-    const double lo = -0.5, hi = N - 1 + 0.5;
+    const double half = N / 2.0;
+    const double lo = -0.5, hx = NX - 1 + 0.5, hy = NY - 1 + 0.5, hz = NZ - 1 + 0.5;
     out.assign((size_t)N * N * N, 0.0);
     std::vector<unsigned char> valid((size_t)N * N * N, 0);
     double sum = 0;
@@ -387,24 +397,24 @@ void reslice_image(const float* vol, int N, const ettk_particle& d, const M4* t,
             for (int k = 0; k < N; k++) {
                 double w[3] = {i - half, j - half, k - half - d.cut_offset};
                 double p[3];
-                for (int r = 0; r < 3; r++) p[r] = fin(r, 0) * w[0] + fin(r, 1) * w[1] + fin(r, 2) * w[2] + fin(r, 3) + pos;
+                for (int r = 0; r < 3; r++) p[r] = fin(r, 0) * w[0] + fin(r, 1) * w[1] + fin(r, 2) * w[2] + fin(r, 3) + pos3[r];
                 size_t o = ((size_t)i * N + j) * N + k;
-                if (p[0] < lo || p[0] > hi || p[1] < lo || p[1] > hi || p[2] < lo || p[2] > hi) continue;
+                if (p[0] < lo || p[0] > hx || p[1] < lo || p[1] > hy || p[2] < lo || p[2] > hz) continue;
                 double fx, fy, fz;
                 int ix = vfloor(p[0], fx), iy = vfloor(p[1], fy), iz = vfloor(p[2], fz);
                 double wx[4], wy[4], wz[4];
                 keys_weights(fx, wx); keys_weights(fy, wy); keys_weights(fz, wz);
                 int cx[4], cy[4], cz[4];
                 for (int q = 0; q < 4; q++) {
-                    cx[q] = std::min(std::max(ix - 1 + q, 0), N - 1);
-                    cy[q] = std::min(std::max(iy - 1 + q, 0), N - 1);
-                    cz[q] = std::min(std::max(iz - 1 + q, 0), N - 1);
+                    cx[q] = std::min(std::max(ix - 1 + q, 0), NX - 1);
+                    cy[q] = std::min(std::max(iy - 1 + q, 0), NY - 1);
+                    cz[q] = std::min(std::max(iz - 1 + q, 0), NZ - 1);
                 }
                 double v = 0;
                 for (int c = 0; c < 4; c++) {
                     double vy = 0;
                     for (int b = 0; b < 4; b++) {
-                        const float* row = vol + ((size_t)cz[c] * N + cy[b]) * N;
+                        const float* row = vol + ((size_t)cz[c] * NY + cy[b]) * NX;
                         double vx = wx[0] * row[cx[0]] + wx[1] * row[cx[1]] + wx[2] * row[cx[2]] + wx[3] * row[cx[3]];
                         vy += wy[b] * vx;
                     }

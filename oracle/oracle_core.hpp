@@ -58,6 +58,9 @@ std::vector<float> sphere_mask(int N, float radius, float sigma);           // I
 std::vector<float> square_fft_mask(int N, int square);                      // FIM.h:642-664
 // resliced particle image (WSub.h:603-828): x-fastest float volume in, A(i,j,k) double out
 void reslice_image(const float* vol, int N, const ettk_particle& d, const M4* t, std::vector<double>& out);
+// the same sub-volume cut from an nx x ny x nz volume around position pos (WSub.h:603-802)
+void extract_image(const float* vol, int nx, int ny, int nz, const double* pos, int N, const ettk_particle& d,
+                   const M4* t, std::vector<double>& out);
 // transpose an x-fastest float volume into A(i,j,k) double
 void to_internal(const float* vol, int N, std::vector<double>& out);
 

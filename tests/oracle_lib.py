@@ -44,6 +44,12 @@ def oracle() -> C.CDLL:
         o.orc_spherical_wedge.argtypes = [C.c_int, C.POINTER(Particle), C.c_int, C.c_void_p]
         o.orc_masks.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         o.orc_distance.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Particle), C.c_int, C.POINTER(Result)]
+        o.orc_align_adjoint_many.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(Particle),
+                                             C.c_int, C.POINTER(Result), C.c_void_p, C.c_int]
+        o.orc_ccc_trace.restype = C.c_float
+        o.orc_ccc_trace.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Particle), C.c_void_p, C.c_void_p, C.c_void_p]
+        o.orc_extract.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.POINTER(Particle),
+                                  C.c_int, C.c_void_p]
         o.orc_ccc_count.restype = C.c_int
         o.orc_ccc_count.argtypes = [C.c_void_p]
         o.orc_average.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(Particle), C.c_void_p, C.c_void_p,
@@ -152,6 +158,29 @@ class OraclePlan:
         self.o.orc_align_many(self.h, n, vp(vols), particle_array(descs), res, nthreads)
         return list(res)
 
+    def align_adjoint_many(self, vols: np.ndarray, descs, n_iter: int = 20, nthreads: int = 0):
+        """The batched + adjoint form in FP64: search and candidate CCCs without Powell, n_iter ascent steps
+        from the best candidate, one CCC at the refined pose.  Returns (results, continuous scores)."""
+        from ettk_amd.abi import particle_array
+        n = vols.shape[0]
+        res = (Result * n)()
+        sc = np.zeros(n)
+        vols = np.ascontiguousarray(vols, dtype=np.float32)
+        self.o.orc_align_adjoint_many(self.h, n, vp(vols), particle_array(descs), int(n_iter), res, vp(sc), nthreads)
+        return list(res), sc
+
+    def ccc_trace(self, vol: np.ndarray, desc: Particle, t: np.ndarray):
+        """ccc with the search window of the upsampled correlation: (distance, transform out, T [ext]^3 before
+        the spherical mask, k1_m)."""
+        ext_max = 4 * self.params.N + 1
+        T = np.zeros(ext_max ** 3)
+        info = np.zeros(2)
+        tt = np.ascontiguousarray(np.asarray(t, dtype=np.float64).reshape(16)).copy()
+        d = self.o.orc_ccc_trace(self.h, vp(np.ascontiguousarray(vol, dtype=np.float32)), C.byref(desc), vp(tt),
+                                 vp(T), vp(info))
+        ext = int(info[1])
+        return float(d), tt.reshape(4, 4), T[: ext ** 3].reshape(ext, ext, ext), float(info[0])
+
     def ccc_count(self) -> int:
         return self.o.orc_ccc_count(self.h)
 
@@ -170,6 +199,18 @@ def avg_wedge(N: int, B: int, descs, weights, matrices):
                     vp(out["wedge"]), vp(sph), C.byref(eff))
     out["sph"] = sph
     out["effective"] = bool(eff.value)
+    return out
+
+
+def extract(tomo: np.ndarray, positions, descs, N: int) -> np.ndarray:
+    """Sub-volumes cut from a (nz, ny, nx) volume at (n, 3) positions (x, y, z): (n, N, N, N), x fastest."""
+    from ettk_amd.abi import particle_array
+    t = np.ascontiguousarray(tomo, dtype=np.float32)
+    pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1, 3))
+    n = pos.shape[0]
+    out = np.zeros((n, N, N, N))
+    nz, ny, nx = t.shape
+    oracle().orc_extract(vp(t), nx, ny, nz, n, vp(pos), particle_array(descs), N, vp(out))
     return out
 
 

@@ -1,8 +1,9 @@
 """Parity of the HIP constrained CCC (executeFourierNewHalf) with the FP64 oracle.
 
-Bar: identical half-voxel peak (shift) except at documented near-ties, where the two best values of the FP64
-correlation differ by less than the FP32 tolerance; distance within 2e-3 relative of the FP64 oracle (FP32
-FFT chain of five N^3 transforms and one upsampled partial DFT).
+Bar: identical half-voxel peak (shift) except at near-ties proven on the FP64 surface (parity.peak_tie: the
+oracle's correlation at the GPU's peak is within 2e-3 of the distance scale of its maximum, and the tie is
+counted and printed); distance within 2e-3 relative of the FP64 oracle (FP32 FFT chain of five N^3
+transforms and one upsampled partial DFT).
 """
 import math
 
@@ -13,8 +14,7 @@ from conftest import to_dev
 from ettk_amd.abi import Particle, default_params
 from ettk_amd.synth import blob_reference, make_particles, random_rotation
 from oracle_lib import OraclePlan
-
-DIST_RTOL = 2e-3
+from parity import DIST_RTOL, peak_tie
 
 
 def _cases(ps, n_rand, seed):
@@ -34,7 +34,8 @@ def _cases(ps, n_rand, seed):
 @pytest.mark.gpu
 @pytest.mark.parametrize("N,mcf,wedge,overlap,tr", [(32, 1, 1, 1, 8.0), (32, 0, 1, 0, 8.0), (32, 1, 0, 0, 8.0),
                                                      (64, 1, 1, 1, 8.0), (64, 0, 1, 0, 8.0), (64, 1, 0, 0, 8.0),
-                                                     (64, 1, 1, 0, -1.0), (48, 1, 1, 1, 8.0), (96, 1, 1, 0, 8.0)])
+                                                     (64, 1, 1, 0, -1.0), (48, 1, 1, 1, 8.0), (96, 1, 1, 0, 8.0),
+                                                     (128, 1, 1, 1, 8.0)])
 def test_ccc_matches_oracle(torch_gpu, ctx, N, mcf, wedge, overlap, tr):
     """Box 64 runs its own kernels (the i-pencil pass, the combine with the forward i transforms and the
     FFT-evaluated first two axes of the upsampled DFT): the bench's 8-voxel window (33 rows) and the unrestricted
@@ -46,7 +47,7 @@ def test_ccc_matches_oracle(torch_gpu, ctx, N, mcf, wedge, overlap, tr):
     ref = blob_reference(N, seed=11)
     ps = make_particles(ref, 2, seed=12, half_wedge=(50, 65), snr=(0.2, 0.2))
     # 16 evaluations at N <= 64 (a multiple of 8: the reslice and FFT combine deal one evaluation per XCD)
-    cases = _cases(ps, 6 if N <= 64 else 1, seed=N)
+    cases = _cases(ps, 6 if N <= 64 else (1 if N < 128 else 0), seed=N)
     orc = OraclePlan(p)
     orc.set_reference(ref)
     plan = Plan(ctx, p)
@@ -58,13 +59,11 @@ def test_ccc_matches_oracle(torch_gpu, ctx, N, mcf, wedge, overlap, tr):
         n_tie = 0
         for e, (i, t) in enumerate(cases):
             d_ref, t_ref = orc.ccc(ps.vols[i], ps.descs[i], t)
-            sh_gpu = np.linalg.solve(t[:3, :3], tout[e][:3, 3] - t[:3, 3])
-            sh_ref = np.linalg.solve(t[:3, :3], t_ref[:3, 3] - t[:3, 3])
             assert abs(dist[e] - d_ref) <= DIST_RTOL * abs(d_ref) + 1e-7, (e, dist[e], d_ref)
-            if not np.allclose(sh_gpu, sh_ref, atol=1e-6):
+            if not np.allclose(tout[e][:3, 3], t_ref[:3, 3], atol=1e-9):
+                tie, margin, so, sg = peak_tie(orc, ps.vols[i], ps.descs[i], t, tout[e])
+                assert tie, (e, "peak differs beyond a FP64-proven tie", so, sg, margin)
                 n_tie += 1
-                # allowed only as a near-tie: distances at both shifts agree to the FP32 tolerance
-                assert np.max(np.abs(sh_gpu - sh_ref)) <= 0.5 + 1e-6, (e, sh_gpu, sh_ref)
-        assert n_tie <= max(1, len(cases) // 10), f"{n_tie} peak mismatches out of {len(cases)}"
+        print(f"  {len(cases)} evaluations, {n_tie} near-ties (FP64 margin within {DIST_RTOL})", flush=True)
     finally:
         plan.close()

@@ -64,13 +64,26 @@ def test_adjoint_gradient_vs_finite_differences(torch_gpu, ctx):
             fd = (sp - sm) / (2 * h)
             for e in range(n):
                 scale = np.linalg.norm(g[e])
-                assert abs(fd[e] - g[e, r]) <= 0.05 * scale + 1e-6, (r, e, fd[e], g[e, r])
+                # per component: 5% of the component plus a floor of 1% of the gradient's norm (the FP32 score's
+                # rounding over the step h); components above 5% of the norm must agree in sign
+                assert abs(fd[e] - g[e, r]) <= 0.05 * abs(g[e, r]) + 0.01 * scale + 1e-6, (r, e, fd[e], g[e, r])
+                if abs(g[e, r]) > 0.05 * scale:
+                    assert np.sign(fd[e]) == np.sign(g[e, r]), (r, e, fd[e], g[e, r])
+        # directional derivatives along random directions
+        u = np.random.default_rng(9).standard_normal((4, 6))
+        for k in range(4):
+            xp, xm = x0 + h * u[k], x0 - h * u[k]
+            sp, _ = plan.ccc_grad(d, ps.descs, list(range(n)), seeds, xp)
+            sm, _ = plan.ccc_grad(d, ps.descs, list(range(n)), seeds, xm)
+            for e in range(n):
+                dd, gu = (sp[e] - sm[e]) / (2 * h), float(g[e] @ u[k])
+                assert abs(dd - gu) <= 0.05 * abs(gu) + 0.01 * np.linalg.norm(g[e]) * np.linalg.norm(u[k]), (k, e, dd, gu)
     finally:
         plan.close()
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N", [32, 64])
+@pytest.mark.parametrize("N", [32, 64, 96])
 def test_score_and_gradient_vs_fp64(torch_gpu, ctx, N):
     # 32 and 64 run different pencil transforms inside the score and adjoint i passes (LDS radix passes; the
     # 8 x 8 register transform)
@@ -94,6 +107,9 @@ def test_score_and_gradient_vs_fp64(torch_gpu, ctx, N):
             s64 = o.orc_refine_eval(orc.h, vp(vol), C.byref(ps.descs[e]), vp(sd), vp(xe), 1e-4, vp(grad64))
             assert abs(S[e] - s64) <= 2e-4 * max(1.0, abs(s64)), (e, S[e], s64)
             assert np.linalg.norm(g[e] - grad64) <= 0.03 * np.linalg.norm(grad64) + 1e-6, (e, g[e], grad64)
+            # per component against the FP64 gradient: 3% of the component plus 0.5% of the norm
+            nrm = np.linalg.norm(grad64)
+            assert np.all(np.abs(g[e] - grad64) <= 0.03 * np.abs(grad64) + 0.005 * nrm + 1e-6), (e, g[e], grad64)
     finally:
         plan.close()
 
@@ -117,9 +133,15 @@ def test_autograd_wrapper(torch_gpu, ctx):
 
 
 @pytest.mark.gpu
-def test_refinement_matches_fp64_ascent(torch_gpu, ctx):
-    print("[refine] lock-step ascent vs FP64 ascent", flush=True)
-    p, ref, ps, plan, d = _setup(torch_gpu, ctx, n=3, snr=1.0)
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("N,n", [(32, 3), (96, 2)])
+def test_refinement_matches_fp64_ascent(torch_gpu, ctx, N, n):
+    """Box 96 is the size of the refinement workload (seeds up to 3 degrees and 1 voxel off, 20 steps); the
+    FP64 ascents run in parallel host threads (the oracle plan is only read)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    print(f"[refine] lock-step ascent vs FP64 ascent N={N}", flush=True)
+    p, ref, ps, plan, d = _setup(torch_gpu, ctx, N=N, n=n, snr=1.0)
     orc = OraclePlan(p)
     orc.set_reference(ref)
     o = oracle()
@@ -130,12 +152,17 @@ def test_refinement_matches_fp64_ascent(torch_gpu, ctx):
         n = len(seeds)
         poses, scores = plan.refine(d, ps.descs, list(range(n)), seeds, n_iter=20)
         s0, _ = plan.ccc_grad(d, ps.descs, list(range(n)), seeds, np.zeros((n, 6)))
-        for e in range(n):
+
+        def fp64(e):
             out = np.zeros(16)
             vol = np.ascontiguousarray(ps.vols[e])
             s64 = o.orc_refine(orc.h, vp(vol), C.byref(ps.descs[e]), vp(np.ascontiguousarray(seeds[e].ravel())), 20,
                                vp(out))
-            P64 = out.reshape(4, 4)
+            return s64, out.reshape(4, 4)
+        with ThreadPoolExecutor(max_workers=n) as ex:
+            ref64 = list(ex.map(fp64, range(n)))
+        for e in range(n):
+            s64, P64 = ref64[e]
             assert rotation_error_deg(poses[e], P64) <= 0.5, e
             assert np.max(np.abs(poses[e][:3, 3] - P64[:3, 3])) <= 0.1, e
             assert abs(scores[e] - s64) <= 2e-4, (e, scores[e], s64)
