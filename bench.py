@@ -6,11 +6,15 @@ reference (S^2 projection, SO(3) search, direct/mirror constrained CCC at Kp=5 p
 Inputs are generated on the GPU and resident in HBM before timing starts.
 
 A step is one alignment pass of the headline form over the whole batch.  Forms measured:
-  plain    - the reference's control flow particle after particle (one CCC per launch), on a subset
-  batched  - every stage over all particles; per-particle control flow served by batched launches
-The JSON line reports the batched form's particles/s as `value`, the plain form and the ratio beside it,
-a roofline record for the dominant kernel (HIP events inside the library, on its stream) and the FP64 CPU
-oracle timed on this host as a reported baseline.
+  plain            - the reference's control flow particle after particle (one CCC per launch), on a subset
+  batched          - every stage over all particles; per-particle control flow served by batched launches
+  batched+adjoint  - the batched search and candidate CCCs, then 20 steps of the continuous refinement with
+                     the hand-written adjoint (the headline form)
+The JSON line reports the headline form's particles/s as `value`, the other forms and the ratios beside it,
+a roofline record for the dominant kernel (HIP events inside the library, on its stream), a pass-model
+roofline per form, the headline form's pose errors against the same form in FP64 on the first --cpu-n
+particles, and the FP64 restatement of the reference method timed on this host's cores as a reported
+baseline (all threads of the process's share, and one core).
 """
 from __future__ import annotations
 
@@ -55,17 +59,24 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1024, help="particles per step")
     ap.add_argument("--plain-n", type=int, default=16, help="particles timed in the plain form")
-    ap.add_argument("--cpu-n", type=int, default=8, help="particles of the CPU oracle sample (0 = skip)")
+    ap.add_argument("--cpu-n", type=int, default=32,
+                    help="particles of the FP64 oracle legs: accuracy of the headline form and the CPU baseline "
+                         "(0 = skip)")
+    ap.add_argument("--cpu-one-core", type=int, default=1, help="particles of the single-core CPU baseline")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--forms", default="plain,batched,adjoint",
                     help="forms to run (the headline form 'adjoint' always runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 256 MiB device copy (a known-size kernel for rocprof counter calibration)")
+    ap.add_argument("--workload", default="scale-up", choices=["scale-up", "uneven"],
+                    help="scale-up: the headline measurement; uneven: particles of box sizes 48/64/96/128")
     args = ap.parse_args()
     forms = set(args.forms.split(","))
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only (DESIGN.md)"}))
         return 2
+    if args.workload == "uneven":
+        return uneven(args)
 
     import torch
 
@@ -133,7 +144,8 @@ def main() -> int:
                  "k_r2c_rows<64>", "k_c2r_rows<64>", "k_reslice", "k_spec_filter", "k_mask_ref",
                  "k_combine_dftA", "k_dftBC_argmax", "k_combine_fftA", "k_dftBC_fft", "k_fft_strided<128>", "k_r2c_rows<128>",
                  "k_reslice_ref", "k_refine_dot", "k_refine_adj", "k_refine_ipass_dot", "k_refine_ipass_adj", "k_refine_moments", "k_sph_sample",
-                 "k_so3_synth_fft", "k_rotate_coef", "k_extrema", "k_combine", "k_localmax"):
+                 "k_so3_synth_fft", "k_so3_synth_mfma", "k_so3_fft2", "k_rotate_coef", "k_extrema", "k_combine",
+                 "k_localmax"):
         ms, nl, by = ctx.profile_query(name)
         if nl:
             kernels[name] = {"ms": ms, "launches": nl, "bytes": by}
@@ -183,23 +195,77 @@ def main() -> int:
                                "unit": "TFLOP/s", "frac": round(flops / (avg_ms / 1e3) / 157.3e12, 4),
                                "flops_per_launch": flops, "basis": "64-tap tricubic, 128 FLOP per output voxel, FP32 VALU"}
 
-    # ---------------------------------------------------------------- CPU baseline: FP64 oracle on host cores
-    cpu = None
+    # ---------------------------------------------------------------- per-form roofline (pass model)
+    # bytes per particle of the algorithm each form executes, with the CCC evaluations counted by the library:
+    # a pruned constrained CCC (the upsampled inverse transform evaluated on the search window only) moves
+    # 5R + 11H; the S^2 projection R + 2H2; the SO(3) correlations 3 x 11.9 MB at B = 32 (scaled by (B/32)^3);
+    # one continuous-refinement iteration 2R (reslice) + R + H (r2c) + 3H (spectral) + H + R (c2r) + 3R (moments)
+    R, H = N ** 3 * 4.0, N * N * (N // 2 + 1) * 8.0
+    R2, H2 = (2 * N) ** 3 * 4.0, (2 * N) ** 2 * (N + 1) * 8.0
+    search_b = R + 2 * H2 + 3 * 11.9e6 * (B / 32.0) ** 3
+    ccc_b = 5 * R + 11 * H
+    refine_it_b = 7 * R + 5 * H
+
+    def form_roofline(ccc_per_particle, refine_iters, pps):
+        bpp = search_b + ccc_per_particle * ccc_b + refine_iters * refine_it_b
+        ach = bpp * pps / 1e9
+        return {"bytes_per_particle": round(bpp), "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4)}
+    forms_roofline = {"basis": "pass model: search R+2H2+3x11.9MB(B/32)^3, pruned CCC 5R+11H per counted "
+                               "evaluation, refinement iteration 7R+5H"}
+    if plain_pps:
+        forms_roofline["plain"] = form_roofline(np.mean([r.n_ccc for r in res_plain]), 0, plain_pps)
+    if batched_pps:
+        forms_roofline["batched"] = form_roofline(np.mean([r.n_ccc for r in res_b]), 0, batched_pps)
+    forms_roofline["batched+adjoint"] = form_roofline(float(n_ccc.mean()), 20, value)
+
+    # ---------------------------------------------------------------- FP64 oracle legs (host cores)
+    cpu, acc = None, None
     if args.cpu_n > 0:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_lib import OraclePlan
 
-        cores = os.cpu_count() or 1
-        threads = max(1, min(args.cpu_n, 16, cores))
+        try:
+            aff = len(os.sched_getaffinity(0))
+        except AttributeError:
+            aff = os.cpu_count() or 1
+        try:
+            import subprocess
+            lines = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True).stdout.splitlines()
+            physical = len({ln for ln in lines if ln and not ln.startswith("#")})
+        except OSError:
+            physical = aff
+        threads = max(1, min(physical, aff, 16))  # the process's share of the host (16 per GPU)
         vols = d_parts[: args.cpu_n].cpu().numpy()
         orc = OraclePlan(p)
         orc.set_reference(ref)
+        # accuracy of the headline form: the same particles through the batched + adjoint form in FP64
+        t0 = time.perf_counter()
+        r64, _ = orc.align_adjoint_many(vols, descs[: args.cpu_n], 20, threads)
+        acc_s = time.perf_counter() - t0
+        ea = np.array([rotation_error_deg(res[i].transform(), r64[i].transform()) for i in range(args.cpu_n)])
+        es = np.array([np.abs(res[i].transform()[:3, 3] - r64[i].transform()[:3, 3]).max() for i in range(args.cpu_n)])
+        acc = {"particles": args.cpu_n, "rot_err_deg_max": round(float(ea.max()), 4),
+               "rot_err_deg_median": round(float(np.median(ea)), 4),
+               "shift_err_vox_max": round(float(es.max()), 4),
+               "frac_rot_within_0.5deg": round(float(np.mean(ea <= 0.5)), 4),
+               "frac_shift_within_0.1vox": round(float(np.mean(es <= 0.1)), 4),
+               "oracle": "FP64 batched+adjoint form: search, candidate CCCs, 20 ascent steps with FP64 gradients, "
+                         "final CCC", "oracle_s": round(acc_s, 1)}
+        # CPU baseline: the reference method (search + Powell refinement) in FP64, one particle per thread
         t0 = time.perf_counter()
         orc.align_many(vols, descs[: args.cpu_n], nthreads=threads)
         cpu_s = time.perf_counter() - t0
+        n1 = max(1, args.cpu_one_core)
+        t0 = time.perf_counter()
+        orc.align_many(vols[:n1], descs[:n1], nthreads=1)
+        one_s = time.perf_counter() - t0
         cpu = {"value": round(args.cpu_n / cpu_s, 4), "unit": "particles/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_n} particles of this workload, FP64 oracle, one particle per OpenMP thread, "
-                         f"{cpu_s:.1f} s wall"}
+               "sample": f"first {args.cpu_n} particles of this workload through the reference method (S2/SO(3) "
+                         f"search, direct/mirror CCCs, Powell refinement) in the FP64 restatement, one particle per "
+                         f"OpenMP thread on {threads} threads, {cpu_s:.1f} s wall",
+               "one_core": {"value": round(n1 / one_s, 4), "particles": n1, "wall_s": round(one_s, 1)},
+               "host": {"physical_cores": physical, "affinity_cpus": aff}}
 
     line = {
         "metric": METRIC,
@@ -231,11 +297,62 @@ def main() -> int:
         "kernels": kernels,
         "host_stages": host,
         "roofline": roofline,
+        "forms_roofline": forms_roofline,
+        "accuracy_vs_fp64": acc,
         "cpu_baseline": cpu,
         "gen_s": round(gen_s, 1),
     }
     print(json.dumps(line))
     plan.close()
+    ctx.close()
+    return 0
+
+
+def uneven(args) -> int:
+    """Particles of uneven box sizes: --n particles with boxes 48/64/96/128 drawn with probabilities
+    .3/.4/.2/.1 (bandwidths 16/32/32/64), one 40-blob reference per size, wedges +-U[45,70], SNR U[0.05,0.2];
+    one step aligns all of them in one ettk_align_buckets call (batched + adjoint form, CCC chunks replayed from
+    captured graphs, execution choices tuned per size on the first call)."""
+    import torch
+
+    from ettk_amd.abi import ETTK_BUCKETS_GRAPHS, ETTK_BUCKETS_TUNE, Particle, default_params
+    from ettk_amd.lib import Context
+    from ettk_amd.synth import _rng, blob_reference, make_particles_device
+
+    sizes = [(48, 16, 0.3), (64, 32, 0.4), (96, 32, 0.2), (128, 64, 0.1)]
+    g = _rng(args.seed, 7)
+    draw = g.choice(len(sizes), size=args.n, p=[s[2] for s in sizes])
+    counts = [int(np.sum(draw == k)) for k in range(len(sizes))]
+    t0 = time.time()
+    buckets = []
+    for k, (N, B, _) in enumerate(sizes):
+        ref = blob_reference(N, seed=args.seed + 10 * k)
+        d, descs, _ = make_particles_device(ref, counts[k], seed=args.seed + 10 * k + 1, half_wedge=(45.0, 70.0),
+                                            snr=(0.05, 0.2), max_shift=4.0 * N / 64)
+        buckets.append((default_params(N, B), torch.from_numpy(ref).cuda(), Particle(), d, descs))
+    gen_s = time.time() - t0
+    ctx = Context(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.align_buckets(buckets, mode=2, flags=ETTK_BUCKETS_GRAPHS | ETTK_BUCKETS_TUNE)  # warm-up: plans, tuning
+    torch.cuda.synchronize()
+    first_s = time.perf_counter() - t0
+    step_ms = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.align_buckets(buckets, mode=2, flags=ETTK_BUCKETS_GRAPHS)
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - t0) * 1e3)
+    ms = float(np.mean(step_ms))
+    line = {"metric": "aligned particles/s, uneven box sizes 48/64/96/128 (B 16/32/32/64), batched+adjoint",
+            "value": round(args.n / (ms / 1e3), 2), "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": 1, "ms_per_step": round(ms, 2), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic (per-size 40-blob references)",
+            "config": {"workload": "uneven box sizes", "particles": args.n, "counts": dict(zip(["48", "64", "96", "128"],
+                                                                                              counts))},
+            "first_call_s": round(first_s, 2), "gen_s": round(gen_s, 1)}
+    print(json.dumps(line))
     ctx.close()
     return 0
 

@@ -66,3 +66,15 @@ def particle_array(descs) -> C.Array:
     for i, d in enumerate(descs):
         arr[i] = d
     return arr
+
+
+ETTK_BUCKETS_GRAPHS = 1
+ETTK_BUCKETS_TUNE = 2
+
+
+class Bucket(C.Structure):
+    """ettk_bucket: particles of one box size and bandwidth with their reference (ettk_align_buckets)."""
+    _fields_ = [
+        ("params", Params), ("d_reference", C.c_void_p), ("reference", Particle), ("d_particles", C.c_void_p),
+        ("n", C.c_int), ("descs", C.POINTER(Particle)), ("out", C.POINTER(Result)),
+    ]

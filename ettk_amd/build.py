@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["fft.hip", "ccc.hip", "so3.hip", "refine.hip", "host.cpp", "align.cpp", "refine.cpp", "avg.hip", "avg.cpp",
-           "noalloc.hip"]
+           "noalloc.hip", "extract.hip", "classify.cpp", "tune.cpp"]
 
 
 def build_lib(verbose: bool = False) -> Path:
@@ -61,6 +61,18 @@ def build_cpp_tests() -> Path:
     return exe
 
 
+def build_io_test() -> Path:
+    """The driver of the on-disk formats (tests/cpp/test_io): host code only."""
+    src = ROOT / "tests" / "cpp" / "test_io.cpp"
+    exe = src.with_suffix("")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", str(ROOT / "include"), str(src), "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build of tests/cpp/test_io failed")
+    return exe
+
+
 def build_oracle() -> None:
     """The FP64 CPU oracle (test infrastructure) and, where the reference tree exists, the soft-1.0 checker."""
     subprocess.run(["make", "-C", str(ROOT / "oracle"), "-s", "liboracle.so"], check=True)
@@ -71,6 +83,7 @@ def build_oracle() -> None:
 if __name__ == "__main__":
     build_lib(verbose="-v" in sys.argv)
     build_cpp_tests()
+    build_io_test()
     if "--no-oracle" not in sys.argv:
         build_oracle()
     print(LIB)

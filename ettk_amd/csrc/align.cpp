@@ -306,7 +306,9 @@ struct SearchLists {
     }
     std::chrono::steady_clock::time_point t_ready;  // when its kernels had completed
 };
-static constexpr int kLmCap = 16384;  // local maxima kept per correlation volume
+// local maxima kept per correlation volume ((2B)^3 voxels): 16384 up to B = 32, four times that per doubling
+static int lm_cap(int B) { return B <= 32 ? 16384 : (B <= 64 ? 65536 : 262144); }
+static constexpr int kSearchSlots = 4;  // search chunks in flight (device list buffers and host threads)
 
 // GPU part of the search for particles [p0, p0+cn): projections, SO(3) correlations, local maxima.  Nothing
 // waits here: L.done marks the end of the chunk's kernels, the next chunk is enqueued right behind it.
@@ -315,13 +317,13 @@ static void search_gpu(ettk_plan* P, const float* d_parts, const ettk_particle* 
     So3State& S = so3_state(P);
     const int B = P->p.B, n = 2 * B, nn = n * n;
     const long n3 = (long)n * n * n;
-    hipStream_t s = P->ctx->stream;
+    hipStream_t s = P->stream;
     KernelProf* kp = P->ctx->fft.prof;
     L.p0 = p0;
     L.cn = cn;
     L.uw2.assign(in.uw2.begin() + p0, in.uw2.begin() + p0 + cn);
     L.uw1 = S.use_window1;
-    L.lm = &lm_slot(S, slot, 2 * cn, kLmCap);
+    L.lm = &lm_slot(S, slot, 2 * cn, lm_cap(B));
     S.Pp2.ensure((size_t)cn * nn);
     S.Mx.ensure(cn);
     project_images(P, d_parts, descs, nullptr, cn, false, S.Pp2.p, in.geom + p0);
@@ -378,12 +380,12 @@ static void search_gpu(ettk_plan* P, const float* d_parts, const ettk_particle* 
         ProfScope ps(kp, "k_combine", (double)(nq + 2 * cn) * n3 * 8, s);
         launch_combine(S.C.p, S.ops.p, 2 * cn, B, b0, S.Cc.p, s);
     }
-    ETTK_HIP(hipMemsetAsync(L.lm->count.p, 0, (size_t)2 * cn * kLmCountStride * sizeof(int), s));
+    launch_fill_zero(L.lm->count.p, (size_t)2 * cn * kLmCountStride, s);
     {
         ProfScope ps(kp, "k_localmax", 2.0 * cn * n3 * 8, s);
-        launch_localmax(S.Cc.p, 2 * cn, B, kLmCap, L.lm->count.p, L.lm->idx.p, L.lm->val.p, s);
+        launch_localmax(S.Cc.p, 2 * cn, B, lm_cap(B), L.lm->count.p, L.lm->idx.p, L.lm->val.p, s);
     }
-    launch_lm_compact(L.lm->count.p, L.lm->idx.p, L.lm->val.p, 2 * cn, kLmCap, L.lm->head.p, L.lm->pidx.p,
+    launch_lm_compact(L.lm->count.p, L.lm->idx.p, L.lm->val.p, 2 * cn, lm_cap(B), L.lm->head.p, L.lm->pidx.p,
                       L.lm->pval.p, s);
     ETTK_HIP(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
     ETTK_HIP(hipEventRecord(L.done, s));
@@ -399,7 +401,7 @@ static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOu
     const int p0 = L.p0, cn = L.cn;
     const bool uw1 = L.uw1;
     const int nv = 2 * cn;
-    hipStream_t cs = P->ctx->copy_stream;
+    hipStream_t cs = P->copy_stream;
     LmBuffers& lb = *L.lm;
     const int* head = lb.hhead.p;
     ETTK_HIP(hipMemcpyAsync(lb.hhead.p, lb.head.p, (size_t)(2 * nv + 1) * sizeof(int), hipMemcpyDeviceToHost, cs));
@@ -422,7 +424,8 @@ static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOu
         bool two = uw1 || L.uw2[q];
         if (v % 2 == 1 && !two) continue;
         const int c = head[nv + 1 + v];
-        ETTK_REQUIRE(c <= kLmCap, ETTK_ERR_UNSUPPORTED, "more SO(3) local maxima than the candidate list holds");
+        ETTK_REQUIRE(c <= lm_cap(P->p.B), ETTK_ERR_UNSUPPORTED,
+                     "more SO(3) local maxima than the candidate list holds (" + std::to_string(c) + ")");
         idx[v].assign(pidx + head[v], pidx + head[v] + c);
         val[v].assign(pval + head[v], pval + head[v] + c);
         if (kp && kp->on) kp->add_host("lm_count", (double)c);
@@ -481,9 +484,19 @@ static void search_all(ettk_plan* P, const float* d_parts, const ettk_particle* 
                 const int q0 = p0 + chunk, qn = std::min(chunk, n - q0);
                 next = std::async(std::launch::async, [P, descs, n, q0, qn, &in] { search_inputs(P, descs, n, q0, qn, in); });
             }
+            // chunk k reuses the device list buffers of chunk k - kSearchSlots: its host part must be done
+            if (k >= kSearchSlots) {
+                std::future<void>& old = pending[k - kSearchSlots];
+                try {
+                    old.get();
+                } catch (...) {
+                    if (!err) err = std::current_exception();
+                }
+                if (err) break;
+            }
             lists.emplace_back(new SearchLists());
             SearchLists* L = lists.back().get();
-            search_gpu(P, d_parts, descs, p0, cn, in, k, *L);
+            search_gpu(P, d_parts, descs, p0, cn, in, k % kSearchSlots, *L);
             if (kp) {
                 auto te = std::chrono::steady_clock::now();
                 kp->add_host("search_inputs", std::chrono::duration<double, std::milli>(tg - ti).count());
@@ -503,6 +516,7 @@ static void search_all(ettk_plan* P, const float* d_parts, const ettk_particle* 
         }
     }
     for (auto& f : pending) {
+        if (!f.valid()) continue;  // already collected when its slot was reused
         try {
             f.get();
         } catch (...) {
@@ -974,7 +988,7 @@ void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
     hipEvent_t a, b;
     ETTK_HIP(hipEventCreate(&a));
     ETTK_HIP(hipEventCreate(&b));
-    ETTK_HIP(hipEventRecord(a, P->ctx->stream));
+    ETTK_HIP(hipEventRecord(a, P->stream));
     std::vector<SearchOut> so(n);
     if (mode == 0) {
         EvalFn ev = [&](int pi, Mat4& t) {
@@ -1020,7 +1034,7 @@ void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
         }
     }
     set_overlaps(P, descs, n, out);
-    ETTK_HIP(hipEventRecord(b, P->ctx->stream));
+    ETTK_HIP(hipEventRecord(b, P->stream));
     ETTK_HIP(hipEventSynchronize(b));
     float ms = 0;
     ETTK_HIP(hipEventElapsedTime(&ms, a, b));
@@ -1226,8 +1240,8 @@ int ettk_project_batch(ettk_plan* P, const float* d_particles, int n, const ettk
         const int B = P->p.B;
         S.Pp2.ensure((size_t)n * 4 * B * B);
         project_images(P, d_particles, descs, nullptr, n, false, S.Pp2.p);
-        ETTK_HIP(hipMemcpyAsync(out_pp, S.Pp2.p, (size_t)n * 4 * B * B * sizeof(float), hipMemcpyDeviceToHost, P->ctx->stream));
-        ETTK_HIP(hipStreamSynchronize(P->ctx->stream));
+        ETTK_HIP(hipMemcpyAsync(out_pp, S.Pp2.p, (size_t)n * 4 * B * B * sizeof(float), hipMemcpyDeviceToHost, P->stream));
+        ETTK_HIP(hipStreamSynchronize(P->stream));
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;
@@ -1245,8 +1259,8 @@ int ettk_so3_correlate_batch(ettk_plan* P, int n, const double* sig, const doubl
         const size_t n3 = (size_t)8 * B * B * B;
         S.C.ensure(n3 * n);
         soft_given(P, n, sig, pat, S.C.p);
-        ETTK_HIP(hipMemcpyAsync(out_c, S.C.p, n3 * n * sizeof(double), hipMemcpyDeviceToHost, P->ctx->stream));
-        ETTK_HIP(hipStreamSynchronize(P->ctx->stream));
+        ETTK_HIP(hipMemcpyAsync(out_c, S.C.p, n3 * n * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+        ETTK_HIP(hipStreamSynchronize(P->stream));
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;

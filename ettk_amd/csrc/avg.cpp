@@ -24,7 +24,7 @@ void avg_wedge_compute(ettk_plan* P, int n, const ettk_particle* descs, const fl
     ETTK_HIP(hipSetDevice(P->ctx->device));
     const int N = P->N, B = P->p.B, ns = 2 * B;
     const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * P->H;
-    hipStream_t s = P->ctx->stream;
+    hipStream_t s = P->stream;
     std::vector<WedgeRef> w;
     std::vector<float> sphf((size_t)ns * ns, 0.0f);
     for (int i = 0; i < n; i++) {
@@ -79,9 +79,9 @@ int ettk_avg_reset(ettk_plan* P) {
         for (int w = 0; w < 3; w++) {
             DevBuf<double>& b = acc_buf(P, w);
             b.ensure(nh);
-            ETTK_HIP(hipMemsetAsync(b.p, 0, nh * sizeof(double), P->ctx->stream));
+            launch_fill_zero(b.p, nh, P->stream);
         }
-        ETTK_HIP(hipStreamSynchronize(P->ctx->stream));
+        ETTK_HIP(hipStreamSynchronize(P->stream));
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;
@@ -101,7 +101,7 @@ int ettk_avg_accumulate(ettk_plan* P, const float* d_particles, int n, const ett
         const int N = P->N;
         const long nh = (long)N * N * P->H;
         ETTK_REQUIRE(acc_buf(P, 0).n >= (size_t)nh, ETTK_ERR_STATE, "ettk_avg_accumulate: call ettk_avg_reset first");
-        hipStream_t s = P->ctx->stream;
+        hipStream_t s = P->stream;
         ensure_ccc_workspace(P, std::min(std::max(n, 1), 128));
         std::vector<int> sel;
         for (int i = 0; i < n; i++)
@@ -147,7 +147,7 @@ int ettk_avg_variance(ettk_plan* P, const float* d_particles, int n, const ettk_
         ETTK_HIP(hipSetDevice(P->ctx->device));
         const int N = P->N;
         const size_t n3 = (size_t)N * N * N;
-        hipStream_t s = P->ctx->stream;
+        hipStream_t s = P->stream;
         ensure_ccc_workspace(P, std::min(std::max(n, 1), 128));
         double wsum = 0;
         std::vector<int> sel;
@@ -158,7 +158,7 @@ int ettk_avg_variance(ettk_plan* P, const float* d_particles, int n, const ettk_
         ETTK_REQUIRE(!sel.empty(), ETTK_ERR_ARG, "ettk_avg_variance: no entry has a nonzero weight");
         DevBuf<double> var;
         var.ensure(n3);
-        ETTK_HIP(hipMemsetAsync(var.p, 0, n3 * sizeof(double), s));
+        launch_fill_zero(var.p, n3, s);
         std::vector<EvalGeom> g(P->cap);
         std::vector<float> w(P->cap);
         DevBuf<float> dw;
@@ -198,7 +198,7 @@ int ettk_avg_wedge(ettk_plan* P, int n, const ettk_particle* descs, const float*
         avg_wedge_compute(P, n, descs, weights, matrices);
         const int N = P->N, ns = 2 * P->p.B;
         const size_t n3 = (size_t)N * N * N;
-        hipStream_t s = P->ctx->stream;
+        hipStream_t s = P->stream;
         if (d_accum) ETTK_HIP(hipMemcpyAsync(d_accum, P->wacc.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, s));
         if (d_wedge) {
             if (P->wacc_effective) {
@@ -230,7 +230,7 @@ int ettk_avg_finalize(ettk_plan* P, float* d_out) {
         ETTK_HIP(hipSetDevice(P->ctx->device));
         const int N = P->N;
         ETTK_REQUIRE(acc_buf(P, 0).n >= (size_t)N * N * P->H, ETTK_ERR_STATE, "ettk_avg_finalize: nothing accumulated");
-        hipStream_t s = P->ctx->stream;
+        hipStream_t s = P->stream;
         ensure_ccc_workspace(P, 1);
         launch_avg_final(acc_buf(P, 0).p, acc_buf(P, 1).p, acc_buf(P, 2).p, N, P->S1.p, s);
         fft3_c2r(P->ctx->fft, P->S1.p, P->S1.p, P->R1.p, N, 1, s);

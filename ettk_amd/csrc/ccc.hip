@@ -14,6 +14,7 @@
 #include "kernels.hpp"
 #include "wedge.hpp"
 
+#include <algorithm>
 #include <cfloat>
 
 namespace ettk {
@@ -44,6 +45,20 @@ __device__ __forceinline__ void keys_w(float f, float* w) {  // vtkTricubicInter
     w[1] = ((ft3 - 2.f) * fd2 - 1.f) * fm1;
     w[2] = -((ft3 - 4.f) * f - 1.f) * fd2;
     w[3] = f * fd2 * fm1;
+}
+
+// Validity of a sample is decided in FP32 unless it lies within 1e-3 of the border, where the FP64 map decides
+// (the reference places samples in FP64: a sample that FP32 rounding moves across the border would otherwise
+// enter or leave the valid set and change the mean the volume is centred on).
+__device__ __forceinline__ bool near_border(float p, float lo, float hi) {
+    return fabsf(p - lo) < 1e-3f || fabsf(p - hi) < 1e-3f;
+}
+__device__ __forceinline__ bool inside_fp64(const double* md, int i, int j, int k, double lo, double hx, double hy,
+                                            double hz) {
+    const double x = md[0] * i + md[3] * j + md[6] * k + md[9];
+    const double y = md[1] * i + md[4] * j + md[7] * k + md[10];
+    const double z = md[2] * i + md[5] * j + md[8] * k + md[11];
+    return x >= lo && x <= hx && y >= lo && y <= hy && z >= lo && z <= hz;
 }
 
 // One workgroup reslices an 8x8x8 tile of the output.  The input voxels its 64-tap stencils can touch lie in
@@ -108,9 +123,20 @@ __global__ void __launch_bounds__(kThreads) k_reslice(const float* __restrict__ 
         float py = g.m[1] * i + g.m[4] * j + g.m[7] * k + g.m[10];
         float pz = g.m[2] * i + g.m[5] * j + g.m[8] * k + g.m[11];
         float v = -FLT_MAX;
-        if (px >= lob && px <= hib && py >= lob && py <= hib && pz >= lob && pz <= hib) {
+        bool in = px >= lob && px <= hib && py >= lob && py <= hib && pz >= lob && pz <= hib;
+        if (near_border(px, lob, hib) || near_border(py, lob, hib) || near_border(pz, lob, hib))
+            in = inside_fp64(geom[e].md, i, j, k, -0.5, N - 0.5, N - 0.5, N - 0.5);
+        if (in) {
             float xx = px + tol, yy = py + tol, zz = pz + tol;
             int ix = (int)floorf(xx), iy = (int)floorf(yy), iz = (int)floorf(zz);
+            if (fits) {
+                // the box was sized from the tile's corners; a sample within rounding of a voxel boundary can
+                // floor one voxel outside it: take the neighbouring cubic piece at f just outside [0, 1) instead
+                // (the Keys kernel is C1, the value agrees to O(f^2) of a few ulps)
+                ix = min(max(ix, lo[0] + 1), lo[0] + kBox - 3);
+                iy = min(max(iy, lo[1] + 1), lo[1] + kBox - 3);
+                iz = min(max(iz, lo[2] + 1), lo[2] + kBox - 3);
+            }
             float wx[4], wy[4], wz[4];
             keys_w(xx - ix, wx);
             keys_w(yy - iy, wy);
@@ -419,11 +445,7 @@ void launch_combine_dftA(const float2* F1m, const float2* F2, const float2* Q, c
                          hipStream_t s) {
     dim3 grid(z.N, n_eval);
     size_t shm = (size_t)(z.N * z.H + z.E * z.N) * sizeof(float2);
-    static size_t set_a = 0;
-    if (shm > set_a) {
-        ETTK_HIP(hipFuncSetAttribute((const void*)k_combine_dftA, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        set_a = shm;
-    }
+    ensure_dynamic_lds((const void*)k_combine_dftA, shm);
     k_combine_dftA<<<grid, kThreads, shm, s>>>(F1m, F2, Q, Mfft, scalars, tabA, z, G1);
     ETTK_HIP(hipGetLastError());
 }
@@ -494,11 +516,7 @@ void launch_dftBC_argmax(const float2* G1, const float2* tabA, const float2* tab
                          float* best_val, int* best_idx, hipStream_t s) {
     dim3 grid(z.E, n_eval);
     size_t shm = (size_t)(z.N * z.H + z.E * z.N + 2 * z.E * z.H) * sizeof(float2);
-    static size_t set_b = 0;
-    if (shm > set_b) {
-        ETTK_HIP(hipFuncSetAttribute((const void*)k_dftBC_argmax, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        set_b = shm;
-    }
+    ensure_dynamic_lds((const void*)k_dftBC_argmax, shm);
     k_dftBC_argmax<<<grid, kThreads, shm, s>>>(G1, tabA, tabK, z, best_val, best_idx);
     ETTK_HIP(hipGetLastError());
 }
@@ -543,6 +561,20 @@ void launch_peak_final(const float* best_val, const int* best_idx, const float* 
     k_peak_final<<<(n_eval + 127) / 128, 128, 0, s>>>(best_val, best_idx, scalars, sum_filter, overlap, z, n_eval, out);
     ETTK_HIP(hipGetLastError());
 }
+
+template <class T>
+__global__ void k_fill_zero(T* __restrict__ p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = T(0);
+}
+template <class T>
+static void fill_zero(T* p, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + kThreads - 1) / kThreads, 4096);
+    k_fill_zero<T><<<blocks, kThreads, 0, s>>>(p, n);
+    ETTK_HIP(hipGetLastError());
+}
+void launch_fill_zero(int* p, size_t n, hipStream_t s) { fill_zero(p, n, s); }
+void launch_fill_zero(double* p, size_t n, hipStream_t s) { fill_zero(p, n, s); }
 
 // ------------------------------------------------------------------ helpers used by the plan set-up
 __global__ void k_xfast_to_internal(const float* __restrict__ in, float* __restrict__ out, int N) {

@@ -35,6 +35,10 @@ struct Error {
         if (!(cond)) throw ::ettk::Error{(code), (msg)};                                                \
     } while (0)
 
+// Raises the dynamic-LDS limit of kernel `fn` to at least `bytes` (hipFuncSetAttribute), at most once per size
+// increase; safe when plans are driven from several host threads.
+void ensure_dynamic_lds(const void* fn, size_t bytes);
+
 // ------------------------------------------------------------------ complex helpers (float2)
 __host__ __device__ inline float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __host__ __device__ inline float2 cmulc(float2 a, float2 b) { return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); }  // a*conj(b)

@@ -1137,11 +1137,7 @@ void launch_dftBC_fft(const FftTables& T, const float2* G1, const float2* tabK, 
     const float2* tw64 = T.tw.p + T.offset[fft_len_index(64)];
     const float2* tw128 = T.tw.p + T.offset[fft_len_index(128)];
     const size_t shm = (size_t)z.E * z.H * sizeof(float2);
-    static size_t set = 0;
-    if (shm > set) {
-        ETTK_HIP(hipFuncSetAttribute((const void*)k_dftBC_fft, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        set = shm;
-    }
+    ensure_dynamic_lds((const void*)k_dftBC_fft, shm);
     k_dftBC_fft<<<dim3(z.E, n_eval), kCfT, shm, s>>>(G1, tabK, z, best_val, best_idx, tw64, tw128);
     ETTK_HIP(hipGetLastError());
 }

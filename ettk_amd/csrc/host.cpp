@@ -1,4 +1,5 @@
 // host.cpp -- plan set-up, the constrained-CCC batch driver and the C ABI entry points of libettk_align.
+#include <algorithm>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -12,6 +13,21 @@ namespace ettk {
 
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
+
+void ensure_dynamic_lds(const void* fn, size_t bytes) {
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, size_t>> done;
+    std::lock_guard<std::mutex> g(mu);
+    for (auto& d : done)
+        if (d.first == fn) {
+            if (bytes <= d.second) return;
+            ETTK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+            d.second = bytes;
+            return;
+        }
+    ETTK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    done.emplace_back(fn, bytes);
+}
 
 // ============================================================== FP64 transforms (vtkTransform semantics)
 Mat4 operator*(const Mat4& A, const Mat4& B) {
@@ -165,7 +181,7 @@ void wedge_normals(float lw, float uw, const Mat4& t, WedgeRef& w) {
 // the translation restriction changes
 static void search_window(ettk_plan* P) {
     const int N = P->N, H = P->H;
-    hipStream_t s = P->ctx->stream;
+    hipStream_t s = P->stream;
     double center = N;  // (2N)/2
     double restr = center;
     if (P->trans_restrict < (double)FLT_MAX) restr = std::max(P->trans_restrict * 2.0, 1.0);
@@ -208,7 +224,7 @@ static void plan_constants(ettk_plan* P) {
     const ettk_params& p = P->p;
     const int N = P->N, H = P->H;
     const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * H;
-    hipStream_t s = P->ctx->stream;
+    hipStream_t s = P->stream;
     // spherical raised-cosine image mask (nbfImageFilter::buildMask, IFilt.h:384-472)
     std::vector<float> m1(n3, 1.0f);
     if (p.mask_radius != 0.0f) {
@@ -358,6 +374,83 @@ void ensure_ccc_workspace(ettk_plan* P, int cap) {
     P->geom.ensure(cap);
     P->h_out4.resize((size_t)4 * cap);
     P->cap = cap;
+    ccc_graphs_reset(P);
+}
+
+void ccc_graphs_reset(ettk_plan* P) {
+    for (auto& g : P->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    P->graphs.clear();
+    P->graph_warm.clear();
+    P->ws_gen++;
+}
+
+// The CCC batch with its chunks replayed from graphs captured once per (chunk size, particle array): every chunk
+// reads its geometry from, and writes its results to, the plan's fixed per-chunk buffers, which are filled and
+// drained by copies enqueued around each replay.  A chunk size runs eagerly once before it is captured (the
+// kernels' attributes are set by then).
+static void ccc_batch_graph(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
+                            Mat4* t, float* dist, const ettk_particle* eval_descs) {
+    hipStream_t s = P->stream;
+    const int chunk = std::max(1, std::min(P->ccc_chunk, n));
+    ensure_ccc_workspace(P, chunk);
+    P->geom.ensure(P->cap);
+    P->out4.ensure((size_t)4 * P->cap);
+    P->h_geom.ensure(n);
+    P->h_out_pinned.ensure((size_t)4 * n);
+    EvalGeom* g = P->h_geom.p;
+    for (int e = 0; e < n; e++) g[e] = make_geom_public(P, eval_descs ? eval_descs[e] : descs[pidx[e]], t[e], pidx[e]);
+    CccWork W{P->R1.p, P->R2.p, P->p_res.p, P->p_n1.p, P->p_n2.p, P->p_wf.p, P->p_k1.p, P->stats.p, P->scalars.p,
+              P->bval.p, P->bidx.p, P->S1.p, P->S2.p, P->S3.p, P->G1.p};
+    ETTK_HIP(hipEventRecord(P->ev0, s));
+    for (int e0 = 0; e0 < n; e0 += chunk) {
+        const int E = std::min(chunk, n - e0);
+        ETTK_HIP(hipMemcpyAsync(P->geom.p, g + e0, (size_t)E * sizeof(EvalGeom), hipMemcpyHostToDevice, s));
+        CccGraph* cg = nullptr;
+        for (auto& x : P->graphs)
+            if (x.E == E && x.parts == d_parts && x.gen == P->ws_gen) cg = &x;
+        const bool warm = std::find(P->graph_warm.begin(), P->graph_warm.end(), E) != P->graph_warm.end();
+        if (!cg && warm) {
+            hipGraph_t graph = nullptr;
+            ETTK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                ccc_enqueue_chunk(P, W, d_parts, P->geom.p, E, P->out4.p, s);
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            ETTK_HIP(hipStreamEndCapture(s, &graph));
+            CccGraph ng;
+            ng.E = E;
+            ng.parts = d_parts;
+            ng.gen = P->ws_gen;
+            const hipError_t ie = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            ETTK_HIP(ie);
+            P->graphs.push_back(ng);
+            cg = &P->graphs.back();
+        }
+        if (cg) {
+            ETTK_HIP(hipGraphLaunch(cg->exec, s));
+        } else {
+            ccc_enqueue_chunk(P, W, d_parts, P->geom.p, E, P->out4.p, s);
+            if (!warm) P->graph_warm.push_back(E);
+        }
+        ETTK_HIP(hipMemcpyAsync(P->h_out_pinned.p + (size_t)4 * e0, P->out4.p, (size_t)4 * E * sizeof(float),
+                                hipMemcpyDeviceToHost, s));
+    }
+    ETTK_HIP(hipEventRecord(P->ev1, s));
+    ETTK_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    ETTK_HIP(hipEventElapsedTime(&ms, P->ev0, P->ev1));
+    P->timing.ccc_ms += ms;
+    P->timing.n_ccc += n;
+    const float* o = P->h_out_pinned.p;
+    for (int e = 0; e < n; e++) {
+        dist[e] = o[(size_t)e * 4];
+        t[e] = t[e] * mat_translate(-(double)o[(size_t)e * 4 + 1], -(double)o[(size_t)e * 4 + 2], -(double)o[(size_t)e * 4 + 3]);
+    }
 }
 
 // ============================================================== constrained CCC batch
@@ -370,10 +463,11 @@ EvalGeom make_geom_public(const ettk_plan* P, const ettk_particle& d, const Mat4
     // input index = fin3 * (q - half - cut z) + fin_t + pos
     double w0[3] = {-half, -half, -half - d.cut_offset};
     for (int r = 0; r < 3; r++) {
-        g.m[0 + r] = (float)fin(r, 0);
-        g.m[3 + r] = (float)fin(r, 1);
-        g.m[6 + r] = (float)fin(r, 2);
-        g.m[9 + r] = (float)(fin(r, 0) * w0[0] + fin(r, 1) * w0[1] + fin(r, 2) * w0[2] + fin(r, 3) + pos);
+        g.md[0 + r] = fin(r, 0);
+        g.md[3 + r] = fin(r, 1);
+        g.md[6 + r] = fin(r, 2);
+        g.md[9 + r] = fin(r, 0) * w0[0] + fin(r, 1) * w0[1] + fin(r, 2) * w0[2] + fin(r, 3) + pos;
+        for (int c = 0; c < 4; c++) g.m[3 * c + r] = (float)g.md[3 * c + r];
     }
     WedgeRef w{};
     wedge_normals(d.lwedge, d.uwedge, mat_inverse(fin), w);
@@ -392,7 +486,7 @@ EvalGeom make_geom_public(const ettk_plan* P, const ettk_particle& d, const Mat4
 // Wedge.h:70: limits other than exactly -90 / 90 after the clamp of set())
 void wedge_overlaps(ettk_plan* P, const ettk_particle* descs, const Mat4* t, int n, float* out) {
     if (n <= 0) return;
-    hipStream_t s = P->ctx->stream;
+    hipStream_t s = P->stream;
     const int N = P->N;
     auto effective = [](float lw, float uw) {
         const float l = lw < -90 ? -90.0f : lw, u = uw > 90 ? 90.0f : uw;
@@ -501,10 +595,15 @@ void ccc_enqueue_chunk(ettk_plan* P, const CccWork& W, const float* d_parts, con
 }
 
 void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx, Mat4* t,
-               float* dist) {
+               float* dist, const ettk_particle* eval_descs) {
     ETTK_REQUIRE(P->have_ref, ETTK_ERR_STATE, "reference not set");
-    hipStream_t s = P->ctx->stream;
-    const int CH = 256;  // evaluations per chunk
+    KernelProf* prof = P->ctx->fft.prof;
+    if (P->use_graphs && !(prof && prof->on)) {  // per-kernel timing needs the eager launches
+        ccc_batch_graph(P, d_parts, descs, n, pidx, t, dist, eval_descs);
+        return;
+    }
+    hipStream_t s = P->stream;
+    const int CH = std::max(1, P->ccc_chunk);  // evaluations per chunk
     ensure_ccc_workspace(P, std::min(n, CH));
     const int chunk = std::min(P->cap, CH);
     auto tw0 = std::chrono::steady_clock::now();
@@ -524,7 +623,9 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
     EvalGeom* g = P->h_geom.p;
     auto geom_chunk = [&](int e0, int E) {
         const int nthr = std::max(1, std::min<int>(E / 64, 4));
-        auto geom_of = [&](int e) { g[e] = make_geom_public(P, descs[pidx[e]], t[e], pidx[e]); };
+        auto geom_of = [&](int e) {
+            g[e] = make_geom_public(P, eval_descs ? eval_descs[e] : descs[pidx[e]], t[e], pidx[e]);
+        };
         if (nthr <= 1) {
             for (int e = e0; e < e0 + E; e++) geom_of(e);
         } else {
@@ -579,6 +680,22 @@ using namespace ettk;
     set_error("");                                       \
     return ETTK_OK;
 
+// frees a plan and everything it holds (also a partly built one)
+static void plan_release(ettk_plan* P) {
+    if (P->stream) (void)hipStreamSynchronize(P->stream);
+    if (P->copy_stream) (void)hipStreamSynchronize(P->copy_stream);
+    for (auto& g : P->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    P->graphs.clear();
+    so3_state_destroy(P);
+    if (P->ev0) (void)hipEventDestroy(P->ev0);
+    if (P->ev1) (void)hipEventDestroy(P->ev1);
+    hipStream_t s = P->stream, cs = P->copy_stream;
+    delete P;  // device buffers are released before their streams
+    if (s) (void)hipStreamDestroy(s);
+    if (cs) (void)hipStreamDestroy(cs);
+}
+
 extern "C" {
 
 const char* ettk_last_error(void) { return g_err.c_str(); }
@@ -631,8 +748,16 @@ int ettk_ctx_create(int device, ettk_ctx** out) {
     c->device = device;
     ETTK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     ETTK_HIP(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
-    fft_tables_init(c->fft, c->stream);
-    noalloc_scratch_init(c);
+    try {
+        fft_tables_init(c->fft, c->stream);
+        ETTK_HIP(hipStreamSynchronize(c->stream));  // the tables are read from every plan's stream
+        noalloc_scratch_init(c);
+    } catch (...) {
+        (void)hipStreamDestroy(c->stream);
+        (void)hipStreamDestroy(c->copy_stream);
+        delete c;
+        throw;
+    }
     *out = c;
     ETTK_API_END
 }
@@ -640,6 +765,7 @@ int ettk_ctx_create(int device, ettk_ctx** out) {
 int ettk_ctx_destroy(ettk_ctx* ctx) {
     ETTK_API_BEGIN
     if (ctx) {
+        ctx_plan_cache_destroy(ctx);
         (void)hipStreamSynchronize(ctx->stream);
         ctx->fft.tw.release();
         if (ctx->fft.prof) {
@@ -673,11 +799,18 @@ int ettk_plan_create(ettk_ctx* ctx, const ettk_params* params, ettk_plan** out) 
         P->H = p.N / 2 + 1;
         P->trans_restrict = p.trans_restrict < 0 ? (double)FLT_MAX : (double)p.trans_restrict;
         P->rot_restrict = std::max((double)p.rot_restrict, 1.0);
+        ETTK_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+        ETTK_HIP(hipStreamCreateWithFlags(&P->copy_stream, hipStreamNonBlocking));
         ETTK_HIP(hipEventCreate(&P->ev0));
         ETTK_HIP(hipEventCreate(&P->ev1));
         plan_constants(P);
+        // evaluations per CCC chunk: 256, or as many as about 4 GB of workspace holds (multiples of 8), unless the
+        // tuning cache holds a measured choice for this shape
+        const size_t per_eval = ccc_work_bytes(P, 1);
+        P->ccc_chunk = (int)std::max<size_t>(8, std::min<size_t>(256, (size_t)4e9 / per_eval)) & ~7;
+        tune_lookup(P);
     } catch (...) {
-        delete P;
+        plan_release(P);
         throw;
     }
     *out = P;
@@ -686,13 +819,7 @@ int ettk_plan_create(ettk_ctx* ctx, const ettk_params* params, ettk_plan** out) 
 
 int ettk_plan_destroy(ettk_plan* P) {
     ETTK_API_BEGIN
-    if (P) {
-        (void)hipStreamSynchronize(P->ctx->stream);
-        so3_state_destroy(P);
-        if (P->ev0) (void)hipEventDestroy(P->ev0);
-        if (P->ev1) (void)hipEventDestroy(P->ev1);
-        delete P;
-    }
+    if (P) plan_release(P);
     ETTK_API_END
 }
 
@@ -700,7 +827,7 @@ int ettk_plan_set_restrictions(ettk_plan* P, float rot_restrict, float trans_res
     ETTK_API_BEGIN
     ETTK_REQUIRE(P, ETTK_ERR_ARG, "ettk_plan_set_restrictions: null plan");
     ETTK_HIP(hipSetDevice(P->ctx->device));
-    ETTK_HIP(hipStreamSynchronize(P->ctx->stream));
+    ETTK_HIP(hipStreamSynchronize(P->stream));
     const ettk_params saved = P->p;
     const double tr0 = P->trans_restrict, rot0 = P->rot_restrict;
     const CccSizes z0 = P->z;
@@ -708,6 +835,7 @@ int ettk_plan_set_restrictions(ettk_plan* P, float rot_restrict, float trans_res
     P->p.trans_restrict = trans_restrict;
     P->trans_restrict = trans_restrict < 0 ? (double)FLT_MAX : (double)trans_restrict;
     P->rot_restrict = std::max((double)rot_restrict, 1.0);
+    ccc_graphs_reset(P);
     try {
         search_window(P);
     } catch (...) {
@@ -731,8 +859,9 @@ static void reference_load(ettk_plan* P, const float* d_ref, const ettk_particle
     ETTK_HIP(hipSetDevice(P->ctx->device));
     const int N = P->N, H = P->H;
     const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * H;
-    hipStream_t s = P->ctx->stream;
+    hipStream_t s = P->stream;
     P->have_ref = false;
+    ccc_graphs_reset(P);
     P->refd = desc;
     P->ref_wedge_kind = wedge_kind;
     P->ref.ensure(n3);

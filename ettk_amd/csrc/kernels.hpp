@@ -42,6 +42,8 @@ struct EvalGeom {
     int full;         // particle wedge spans >= 180 degrees (W2 == 1)
     int particle;     // index of the particle volume
     float cut;        // cut offset of the sub-volume along z (nbfWedgedSubImage3D::setCutOffset)
+    int pad;
+    double md[12];    // the reslice map in FP64: decides validity for samples within FP32 rounding of the border
 };
 
 // The CCC steps between the particle's r2c plane passes and the fused plane kernels, on pencils along i
@@ -181,6 +183,9 @@ void launch_wedge_overlap(const float* W1, const float* filter, const EvalGeom* 
 void launch_xfast_to_internal(const float* in, float* out, int N, hipStream_t s);
 void launch_internal_to_xfast(const float* in, float* out, int N, hipStream_t s);
 void launch_mcf_inplace(float2* F, long n, hipStream_t s);
+// zero n elements with a kernel (a fill node, not a memset node, when the stream is captured into a graph)
+void launch_fill_zero(int* p, size_t n, hipStream_t s);
+void launch_fill_zero(double* p, size_t n, hipStream_t s);
 // final per evaluation: first maximum over x, distance and shift
 void launch_peak_final(const float* best_val, const int* best_idx, const float* scalars, float sum_filter,
                        int overlap, const CccSizes& z, int n_eval, float* out /* [e][4] = dist, sx, sy, sz */,

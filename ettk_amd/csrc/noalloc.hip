@@ -66,7 +66,7 @@ extern "C" int ettk_ccc_eval_batch_noalloc(ettk_plan* P, const float* d_particle
         const size_t wb = ccc_work_bytes(P, chunk);
         EvalGeom* d_geo = reinterpret_cast<EvalGeom*>(base + wb);
         float* d_out4 = reinterpret_cast<float*>(base + wb + (((size_t)chunk * sizeof(EvalGeom) + 255) & ~(size_t)255));
-        hipStream_t s = P->ctx->stream;
+        hipStream_t s = P->stream;
         EvalGeom h_geo[kNoallocChunk];
         float h_out4[4 * kNoallocChunk];
         for (int e0 = 0; e0 < n_evals; e0 += chunk) {

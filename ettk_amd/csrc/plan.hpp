@@ -39,6 +39,13 @@ void smooth3(std::vector<float>& A, int n0, int n1, int n2, double sd, double rf
 
 // ------------------------------------------------------------------ state
 struct So3State;
+// a captured CCC chunk pipeline (E evaluations, particle base pointer, workspace generation)
+struct CccGraph {
+    int E = 0;
+    const float* parts = nullptr;
+    int gen = -1;
+    hipGraphExec_t exec = nullptr;
+};
 struct Timing {
     double ccc_ms = 0, total_ms = 0;
     long long n_ccc = 0;
@@ -52,10 +59,14 @@ struct ettk_ctx {
     hipStream_t copy_stream = nullptr;  // host downloads issued from worker threads (non-blocking)
     ettk::FftTables fft;
     void* scratch = nullptr;  // device address of this device's fixed CCC scratch region (noalloc.hip)
+    struct ettk_plan_cache* plans = nullptr;  // plans kept for ettk_align_buckets (tune.cpp)
 };
 
 struct ettk_plan {
     ettk_ctx* ctx = nullptr;
+    // the plan's own streams (default priority): plans of one context run concurrently from different host threads
+    hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;  // host downloads issued from worker threads (non-blocking)
     ettk_params p{};
     int N = 0, H = 0;
     bool dummy_filter = true;
@@ -80,6 +91,13 @@ struct ettk_plan {
     ettk::PinnedBuf<ettk::EvalGeom> h_geom;  // host geometry of a CCC batch (asynchronous uploads)
     ettk::DevBuf<float> out_all;
     std::vector<float> h_out4;
+    // execution choices (ettk_plan_tune): evaluations per CCC chunk, chunks replayed from captured graphs
+    int ccc_chunk = 256;
+    bool use_graphs = false;
+    int ws_gen = 0;  // bumped whenever a buffer or constant a captured chunk reads is replaced
+    std::vector<ettk::CccGraph> graphs;
+    std::vector<int> graph_warm;  // chunk sizes already run eagerly on this workspace generation
+    ettk::PinnedBuf<float> h_out_pinned;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ettk::Timing timing;
     // S^2 / SO(3) state (so3.hip)
@@ -124,9 +142,17 @@ void ccc_enqueue_chunk(ettk_plan* P, const CccWork& W, const float* d_parts, con
                        hipStream_t s);
 // device address of the fixed CCC scratch region of the context's device into ctx->scratch (noalloc.hip)
 void noalloc_scratch_init(ettk_ctx* ctx);
-// constrained CCC for a batch of evaluations: on return t[e] <- t[e] * Translate(-shift), dist[e]
+// the context's plans for ettk_align_buckets (created on first use) and their release (tune.cpp)
+struct ettk_plan_cache* ctx_plan_cache(ettk_ctx* ctx);
+void ctx_plan_cache_destroy(ettk_ctx* ctx);
+// applies the recorded tuner choice for the plan's shape, when the tuning cache holds one (tune.cpp)
+bool tune_lookup(ettk_plan* P);
+// drops the captured chunk pipelines (after a workspace, reference or restriction change)
+void ccc_graphs_reset(ettk_plan* P);
+// constrained CCC for a batch of evaluations: on return t[e] <- t[e] * Translate(-shift), dist[e].  Evaluation e
+// reads volume pidx[e] with descriptor descs[pidx[e]], or eval_descs[e] when eval_descs is given.
 void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
-               Mat4* t, float* dist);
+               Mat4* t, float* dist, const ettk_particle* eval_descs = nullptr);
 void ensure_ccc_workspace(ettk_plan* P, int cap);
 // getWedgeOverlap of the reference against descs[i] at t[i] (CorrMetric.h:533-566)
 void wedge_overlaps(ettk_plan* P, const ettk_particle* descs, const Mat4* t, int n, float* out);

@@ -48,7 +48,7 @@ static void particle_spectra(ettk_plan* P, const float* d_parts, int p0, int p1)
     const int N = P->N, H = P->H;
     const size_t n3 = (size_t)N * N * N, nh = (size_t)N * N * H;
     const int cnt = p1 - p0;
-    hipStream_t s = P->ctx->stream;
+    hipStream_t s = P->stream;
     P->r_fp.ensure((size_t)cnt * nh);
     P->r_mean.ensure(cnt);
     launch_vol_mean(d_parts + (size_t)p0 * n3, cnt, N, P->r_mean.p, s);
@@ -161,7 +161,7 @@ static void refine_geometry(const ettk_plan* P, const ettk_particle* descs, cons
 static void refine_enqueue(ettk_plan* P, RefineChunk& c) {
     const int N = P->N, H = P->H;
     const long nh = (long)N * N * H;
-    hipStream_t s = P->ctx->stream;
+    hipStream_t s = P->stream;
     KernelProf* kp = P->ctx->fft.prof;
     const int E = c.E, e0 = c.e0;
     RefineGeom* geo = P->r_geom.p + e0;

@@ -21,6 +21,8 @@ static constexpr int kT = 256;
 
 // the fused synthesis + LDS FFT path holds one (2B)^2 FP64 complex slice in LDS: 2B in {16, 32, 64}
 static inline bool fused_synthesis(int B) { return B == 8 || B == 16 || B == 32; }
+// the synthesis as per-order-pair FP64 matrix products on the matrix cores (B = 16, 32), then the 2-D DFT per slice
+static inline bool mfma_synthesis(int B) { return B == 16 || B == 32; }
 
 // ============================================================== tables (host, FP64)
 static double lfact(int n) { return std::lgamma((double)n + 1.0); }
@@ -66,7 +68,7 @@ __host__ __device__ static inline double wig_next(int l, int m1, int m2, double 
 void so3_tables(ettk_plan* P) {
     So3State& S = *P->so3;
     const int B = S.B, n = 2 * B, W = 2 * B - 1;
-    hipStream_t s = P->ctx->stream;
+    hipStream_t s = P->stream;
     // Driscoll-Healy weights
     std::vector<double> w(n);
     for (int j = 0; j < n; j++) {
@@ -127,6 +129,17 @@ void so3_tables(ettk_plan* P) {
                     for (int l = l0; l < B; l++) D2[(((size_t)kb * n + c1) * B + l) * n + c2] = D[o + (l - l0)];
                 }
         S.dtab2.upload(D2.data(), D2.size(), s);
+        if (mfma_synthesis(B)) {
+            // [c1 n + c2][l][beta]: for one order pair, the columns l of the (beta x l) matrix the synthesis
+            // multiplies, beta contiguous
+            std::vector<double> DP((size_t)n * n * B * n, 0.0);
+            for (int kb = 0; kb < n; kb++)
+                for (int c1 = 0; c1 < n; c1++)
+                    for (int l = 0; l < B; l++)
+                        for (int c2 = 0; c2 < n; c2++)
+                            DP[(((size_t)c1 * n + c2) * B + l) * n + kb] = D2[(((size_t)kb * n + c1) * B + l) * n + c2];
+            S.dtabP.upload(DP.data(), DP.size(), s);
+        }
     }
     std::vector<double2> tw(n);
     for (int t = 0; t < n; t++) tw[t] = make_double2(std::cos(2 * M_PI * t / n), -std::sin(2 * M_PI * t / n));
@@ -652,6 +665,86 @@ __global__ void __launch_bounds__(kSynT) k_so3_synth_fft(const double2* __restri
     }
 }
 
+// The synthesis as matrix products on the FP64 matrix cores.  For one order pair (m1, m2) and a group of up to 64
+// instances q the synthesis is
+//   T1[q][beta][m1][m2] = (-1)^(m1+m2) sum_l D(beta, l) X(l, q),  X(l, q) = sig_q(l, -m1) conj(pat_q(l, -m2)),
+// a (2B x B) by (B x 64) product per pair (real D, complex X: two products).  One workgroup per (pair, instance
+// group); wave w owns the 16 beta rows [16 w, 16 w + 16) and the four 16-column instance tiles, and issues
+// v_mfma_f64_16x16x4f64 over l in steps of four from max(|m1|, |m2|) rounded down to a multiple of four (the table is
+// zero below, so the extra terms are exact zeros).  Operand layout of the instruction on gfx950: lane L holds
+// A[L % 16][L / 16] and B[L / 16][L % 16]; result element i of lane L is D[4 i + L / 16][L % 16].
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+template <int B>
+__global__ void __launch_bounds__(2 * B * 4) k_so3_synth_mfma(const double2* __restrict__ cs, const double2* __restrict__ cp,
+                                                         const double* __restrict__ Dp, int nq, double2* __restrict__ T1) {
+    constexpr int n = 2 * B;
+    const int pair = blockIdx.x, c1 = pair / n, c2 = pair % n;
+    const int w = threadIdx.x >> 6, L = threadIdx.x & 63, lr = L & 15, lk = L >> 4;
+    const int q0 = blockIdx.y * 64;
+    mfma_d4 re[4], im[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        re[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+        im[t] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+    }
+    double sg = 1.0;
+    if (c1 != B && c2 != B) {
+        const int m1 = c1 < B ? c1 : c1 - n, m2 = c2 < B ? c2 : c2 - n;
+        const int am1 = m1 < 0 ? -m1 : m1, am2 = m2 < 0 ? -m2 : m2;
+        const int l0 = am1 > am2 ? am1 : am2;
+        const double* D = Dp + (size_t)pair * B * n + 16 * w + lr;
+        for (int l = l0 & ~3; l < B; l += 4) {
+            const int ll = l + lk;
+            const double a = D[(size_t)ll * n];
+            const int is = max(ll * ll + ll - m1, 0), ip = max(ll * ll + ll - m2, 0);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int q = q0 + 16 * t + lr;
+                double xr = 0.0, xi = 0.0;
+                if (q < nq) {
+                    const double2 sv = cs[(size_t)q * B * B + is], pv = cp[(size_t)q * B * B + ip];
+                    xr = sv.x * pv.x + sv.y * pv.y;
+                    xi = sv.y * pv.x - sv.x * pv.y;
+                }
+                re[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xr, re[t], 0, 0, 0);
+                im[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xi, im[t], 0, 0, 0);
+            }
+        }
+        if ((m1 + m2) & 1) sg = -1.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int q = q0 + 16 * t + lr;
+        if (q >= nq) continue;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int kb = 16 * w + 4 * i + lk;
+            T1[(((size_t)q * n + kb) * n + c1) * n + c2] = make_double2(sg * re[t][i], sg * im[t][i]);
+        }
+    }
+}
+
+// the two DFTs of one (instance, beta) slice of T1 in LDS: C[q][alpha][beta][gamma] = Re sum T1 e^{-i m1 alpha}
+// e^{-i m2 gamma}
+template <int LOGN>
+__global__ void __launch_bounds__(kSynT) k_so3_fft2(const double2* __restrict__ T1, const double2* __restrict__ tw,
+                                                   double* __restrict__ C) {
+    constexpr int n = 1 << LOGN, LD = n + 1;
+    extern __shared__ double2 X[];  // n * LD
+    __shared__ double2 stw[n];
+    const int q = blockIdx.x, kb = blockIdx.y;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = tw[t];
+    const double2* src = T1 + ((size_t)q * n + kb) * n * n;
+    block_copy<kSynT, n * n>([&](int t) { return src[t]; }, [&](int t, double2 v) { X[(t / n) * LD + (t % n)] = v; });
+    __syncthreads();
+    fft_lds_f64<LOGN>(X, LD, 1, stw);  // over m1 for every m2 -> alpha
+    fft_lds_f64<LOGN>(X, 1, LD, stw);  // over m2 for every alpha -> gamma
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+        const int a = t / n, g = t % n;
+        C[(((size_t)q * n + a) * n + kb) * n + g] = X[a * LD + g].x;
+    }
+}
+
 // ============================================================== softScaled pieces
 // first maximum and first minimum in row-major order (ProjRot.h:1562-1576)
 // min / max of each correlation volume and the first position of each (blitz minIndex / maxIndex order):
@@ -1030,7 +1123,7 @@ void project_images(ettk_plan* P, const float* d_parts, const ettk_particle* des
     So3State& S = so3(P);
     const int N = P->N, M = 2 * N, B = S.B, Hm = N + 1;
     const size_t n3 = (size_t)N * N * N, nbig = (size_t)M * M * Hm;
-    hipStream_t s = P->ctx->stream;
+    hipStream_t s = P->stream;
     const int CH = 32;
     S.projR.ensure(n3 * CH);
     S.projF.ensure(nbig * CH);
@@ -1088,11 +1181,11 @@ void so3_reference_set(ettk_plan* P) {
         W1 = P->ref_wedge_kind == 1 ? P->sph_acc_bin : std::vector<float>((size_t)4 * B * B, 1.0f);
         if (P->p.missing_wedge) smooth3(W1, 2 * B, 2 * B, 1, 2.0, 1.0);
     }
-    S.W1.upload(W1.data(), W1.size(), P->ctx->stream);
+    S.W1.upload(W1.data(), W1.size(), P->stream);
     double sw = 0;
     for (float v : W1) sw += v;
     S.use_window1 = P->p.missing_wedge && (sw <= 0.95f * (float)W1.size());
-    ETTK_HIP(hipStreamSynchronize(P->ctx->stream));
+    ETTK_HIP(hipStreamSynchronize(P->stream));
 }
 
 // soft / softScaled for a list of instances whose samples are already in S.sig / S.pat; C is [q][a][b][g]
@@ -1100,7 +1193,7 @@ void soft_core(ettk_plan* P, const std::vector<SoftInst>& inst, double* d_C) {
     So3State& S = so3(P);
     const int B = S.B, n = 2 * B, nq = (int)inst.size();
     const long n3 = (long)n * n * n;
-    hipStream_t s = P->ctx->stream;
+    hipStream_t s = P->stream;
     S.fm.ensure((size_t)nq * n * n);
     S.cs.ensure((size_t)nq * B * B);
     S.cp.ensure((size_t)nq * B * B);
@@ -1114,7 +1207,22 @@ void soft_core(ettk_plan* P, const std::vector<SoftInst>& inst, double* d_C) {
     k_legendre_fwd<<<dim3(8, nq), kT, 0, s>>>(S.fm.p, S.legF.p, B, S.cs.p);
     k_ring_dft<<<dim3(16, nq), kT, 0, s>>>(S.pat.p, S.tw.p, B, S.fm.p);
     k_legendre_fwd<<<dim3(8, nq), kT, 0, s>>>(S.fm.p, S.legF.p, B, S.cp.p);
-    if (fused_synthesis(B)) {
+    if (mfma_synthesis(B)) {
+        S.T1.ensure((size_t)nq * n3);
+        {
+            // algorithmic bytes: the coefficients and the T1 slices written
+            ProfScope ps(P->ctx->fft.prof, "k_so3_synth_mfma", (double)nq * ((double)n3 * 16 + 32.0 * B * B), s);
+            const dim3 grid(n * n, (nq + 63) / 64);
+            if (B == 16) k_so3_synth_mfma<16><<<grid, 128, 0, s>>>(S.cs.p, S.cp.p, S.dtabP.p, nq, S.T1.p);
+            else k_so3_synth_mfma<32><<<grid, 256, 0, s>>>(S.cs.p, S.cp.p, S.dtabP.p, nq, S.T1.p);
+        }
+        {
+            ProfScope ps(P->ctx->fft.prof, "k_so3_fft2", (double)nq * n3 * (16 + 8), s);
+            const size_t shm = (size_t)n * (n + 1) * sizeof(double2);
+            if (B == 16) k_so3_fft2<5><<<dim3(nq, n), kSynT, shm, s>>>(S.T1.p, S.tw.p, d_C);
+            else k_so3_fft2<6><<<dim3(nq, n), kSynT, shm, s>>>(S.T1.p, S.tw.p, d_C);
+        }
+    } else if (fused_synthesis(B)) {
         ProfScope ps(P->ctx->fft.prof, "k_so3_synth_fft", (double)nq * ((double)n3 * 8 + 32.0 * B * B), s);
         const size_t shm = (size_t)n * (n + 1) * sizeof(double2);
         switch (n) {
@@ -1198,7 +1306,7 @@ void run_soft(ettk_plan* P, const std::vector<SoftInst>& inst, const float* d_Pp
               double* d_C, const SoftInst* h_inst) {
     So3State& S = so3(P);
     const int B = S.B, n = 2 * B, nq = (int)inst.size();
-    hipStream_t s = P->ctx->stream;
+    hipStream_t s = P->stream;
     S.inst.ensure(nq);
     ETTK_HIP(hipMemcpyAsync(S.inst.p, h_inst ? h_inst : inst.data(), nq * sizeof(SoftInst), hipMemcpyHostToDevice, s));
     S.sig.ensure((size_t)nq * n * n);
@@ -1251,7 +1359,7 @@ void sph_rotate(ettk_plan* P, int n, const float* h_img, const double* h_t, floa
     So3State& S = so3(P);
     const int B = S.B, ns = 2 * B;
     const size_t nn = (size_t)ns * ns;
-    hipStream_t s = P->ctx->stream;
+    hipStream_t s = P->stream;
     std::vector<double> img((size_t)n * nn), ang((size_t)n * 3);
     for (size_t q = 0; q < img.size(); q++) img[q] = (double)h_img[q];
     for (int i = 0; i < n; i++) harmonics_angles(h_t + (size_t)i * 16, &ang[(size_t)i * 3]);
@@ -1279,7 +1387,7 @@ void sph_rotate(ettk_plan* P, int n, const float* h_img, const double* h_t, floa
 void soft_given(ettk_plan* P, int nq, const double* h_sig, const double* h_pat, double* d_C) {
     So3State& S = so3(P);
     const int B = S.B, n = 2 * B;
-    hipStream_t s = P->ctx->stream;
+    hipStream_t s = P->stream;
     std::vector<SoftInst> inst(nq, SoftInst{0, 0, 0, 0});
     S.inst.ensure(nq);
     ETTK_HIP(hipMemcpyAsync(S.inst.p, inst.data(), nq * sizeof(SoftInst), hipMemcpyHostToDevice, s));
@@ -1300,6 +1408,11 @@ static void so3_kernel_attributes(int B) {
         const int n = 2 * B, fs = n * (n + 1) * (int)sizeof(double2);
         const void* f = n == 16 ? (const void*)k_so3_synth_fft<4> : n == 32 ? (const void*)k_so3_synth_fft<5>
                                                                             : (const void*)k_so3_synth_fft<6>;
+        ETTK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, fs));
+    }
+    if (mfma_synthesis(B)) {
+        const int n = 2 * B, fs = n * (n + 1) * (int)sizeof(double2);
+        const void* f = n == 32 ? (const void*)k_so3_fft2<5> : (const void*)k_so3_fft2<6>;
         ETTK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, fs));
     }
 }

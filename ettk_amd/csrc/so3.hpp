@@ -46,6 +46,7 @@ struct So3State {
     int d_slice = 0;
     bool use_window1 = false;
     DevBuf<double> legF, legI, dtab, dtab2;
+    DevBuf<double> dtabP;  // the Wigner-d table per order pair for the matrix-core synthesis: [c1 n + c2][l][beta]
     DevBuf<double> rtab;  // d^l_{m m'} at the softScaled angles beta_b = (float)((2b+1) pi / 4B), [b][l][m][m']
     DevBuf<int> doff;
     DevBuf<double2> tw;

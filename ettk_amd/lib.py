@@ -10,7 +10,7 @@ from pathlib import Path
 
 import numpy as np
 
-from .abi import Params, Particle, Result, particle_array
+from .abi import Bucket, Params, Particle, Result, particle_array
 
 LIB_PATH = Path(__file__).resolve().parent / "libettk_align.so"
 _lib = None
@@ -50,6 +50,14 @@ def load() -> C.CDLL:
             "ettk_last_timing": ([vp, dp, C.POINTER(C.c_longlong), dp], C.c_int),
             "ettk_refine_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, ip, dp, C.c_int, dp, dp], C.c_int),
             "ettk_fft3": ([vp, C.c_int, C.c_int, C.c_int, vp, vp, vp], C.c_int),
+            "ettk_mra_iteration": ([vp, C.c_int, vp, ip, dp, vp, C.c_int, C.POINTER(Particle), C.c_int, C.c_int,
+                                    C.c_float, ip, ip, fp, dp], C.c_int),
+            "ettk_plan_tune": ([vp, vp, C.c_int, C.POINTER(Particle), ip, ip, dp], C.c_int),
+            "ettk_plan_set_execution": ([vp, C.c_int, C.c_int], C.c_int),
+            "ettk_plan_get_execution": ([vp, ip, ip], C.c_int),
+            "ettk_align_buckets": ([vp, C.c_int, C.POINTER(Bucket), C.c_int, C.c_int], C.c_int),
+            "ettk_extract_batch": ([vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.POINTER(Particle), C.c_int, vp],
+                                   C.c_int),
             "ettk_profile": ([vp, C.c_int], C.c_int),
             "ettk_profile_query": ([vp, C.c_char_p, dp, C.POINTER(C.c_longlong), dp], C.c_int),
         }
@@ -101,6 +109,36 @@ class Context:
         aux = C.c_void_p(d_aux.data_ptr()) if d_aux is not None else None
         check(self.lib.ettk_fft3(self.h, n, batch, kind, C.c_void_p(d_in.data_ptr()), C.c_void_p(d_out.data_ptr()),
                                  aux))
+
+    def extract(self, d_tomo, positions, descs, N: int, d_out) -> None:
+        """ettk_extract_batch: d_tomo (nz, ny, nx) float32 tensor, positions (n, 3) voxel coordinates (x, y, z),
+        d_out (n, N, N, N) float32 tensor."""
+        pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1, 3))
+        nz, ny, nx = d_tomo.shape
+        check(self.lib.ettk_extract_batch(self.h, C.c_void_p(d_tomo.data_ptr()), nx, ny, nz, pos.shape[0], _dp(pos),
+                                          particle_array(descs), N, C.c_void_p(d_out.data_ptr())))
+
+    def align_buckets(self, buckets, mode: int = 1, flags: int = 0):
+        """ettk_align_buckets over a list of (params, d_reference, reference Particle, d_particles, descs);
+        returns one list of results per bucket."""
+        arr = (Bucket * len(buckets))()
+        keep = []
+        outs = []
+        for i, (params, d_ref, rdesc, d_parts, descs) in enumerate(buckets):
+            n = d_parts.shape[0] if d_parts is not None else 0
+            res = (Result * max(n, 1))()
+            da = particle_array(descs) if n else None
+            keep += [res, da]
+            arr[i].params = params
+            arr[i].d_reference = d_ref.data_ptr()
+            arr[i].reference = rdesc
+            arr[i].d_particles = d_parts.data_ptr() if n else None
+            arr[i].n = n
+            arr[i].descs = C.cast(da, C.POINTER(Particle)) if n else None
+            arr[i].out = C.cast(res, C.POINTER(Result))
+            outs.append((res, n))
+        check(self.lib.ettk_align_buckets(self.h, len(buckets), arr, mode, flags))
+        return [list(res)[:n] for res, n in outs]
 
     def profile(self, enable: bool) -> None:
         check(self.lib.ettk_profile(self.h, int(enable)))
@@ -263,6 +301,41 @@ class Plan:
         self._keep = [d_ref]
         check(self.lib.ettk_reference_set_average(self.h, C.c_void_p(d_ref.data_ptr()), n, particle_array(descs),
                                                   _fp(w), _dp(m)))
+
+    def mra_iteration(self, d_refs, d_parts, descs, type: int = 0, form: int = 2, top_percentage: float = 1.0,
+                      ref_members=None, member_matrices=None):
+        """ettk_mra_iteration: d_refs (K, N, N, N) float32 tensor, replaced by the new class averages.  Returns
+        (class, selected, distance, matrices (n, 4, 4))."""
+        K = d_refs.shape[0]
+        n = d_parts.shape[0]
+        cls = np.zeros(n, np.int32)
+        sel = np.zeros(n, np.int32)
+        dist = np.zeros(n, np.float32)
+        mats = np.zeros((n, 16), np.float64)
+        rm = mm = None
+        if ref_members is not None:
+            rm_a = np.ascontiguousarray(ref_members, dtype=np.int32)
+            mm_a = np.ascontiguousarray(np.asarray(member_matrices, dtype=np.float64).reshape(n, 16))
+            rm, mm = _ip(rm_a), _dp(mm_a)
+        check(self.lib.ettk_mra_iteration(self.h, K, C.c_void_p(d_refs.data_ptr()), rm, mm,
+                                          C.c_void_p(d_parts.data_ptr()), n, particle_array(descs), type, form,
+                                          float(top_percentage), _ip(cls), _ip(sel), _fp(dist), _dp(mats)))
+        return cls, sel, dist, mats.reshape(n, 4, 4)
+
+    def tune(self, d_parts, descs):
+        """ettk_plan_tune on these particles: (chunk, graphs, microseconds per evaluation)."""
+        ch, gr, us = C.c_int(), C.c_int(), C.c_double()
+        check(self.lib.ettk_plan_tune(self.h, C.c_void_p(d_parts.data_ptr()), d_parts.shape[0], particle_array(descs),
+                                      C.byref(ch), C.byref(gr), C.byref(us)))
+        return ch.value, bool(gr.value), us.value
+
+    def set_execution(self, chunk: int, graphs: bool) -> None:
+        check(self.lib.ettk_plan_set_execution(self.h, int(chunk), int(graphs)))
+
+    def execution(self):
+        ch, gr = C.c_int(), C.c_int()
+        check(self.lib.ettk_plan_get_execution(self.h, C.byref(ch), C.byref(gr)))
+        return ch.value, bool(gr.value)
 
     def last_timing(self):
         ms = C.c_double()

@@ -247,6 +247,75 @@ int ettk_last_timing(ettk_plan* plan, double* ccc_ms, long long* n_ccc, double* 
 #define ETTK_FFT_SQUARE  3
 int ettk_fft3(ettk_ctx* ctx, int n, int batch, int kind, const void* in, void* out, const float* aux);
 
+/* One iteration of multi-reference alignment and classification: nbfLoopClustering::doMra
+   (inc/em/nbfLoopClustering.h:1218-1647) as MPI_Classification repeats it (src/MPI_Classification.cpp:880-1166).
+   d_refs holds K references (N^3 floats each, x fastest) and receives the K new class averages.  When ref_members
+   (n ints) and member_matrices (n x 16) are given, reference k is the average of the particles with
+   ref_members[i] == k at member_matrices[i] (setInput1 of an nbfWedgedAverageImage3D: its wedge is their
+   accumulated wedge); otherwise the references carry no wedge.
+     1. every particle is aligned to every reference with getDistances of metric `type` (0 search + refinement,
+        1 refinement from the stored alignment, 2 shift only, 3 search only) in execution `form`
+        (as ettk_distances_batch);
+     2. best-reference assignment (nbfLoopClustering.h:1523-1592): each particle under each of its K alignments
+        against every reference at zero shift; the alignment with the lowest sum of distances (first minimum)
+        gives out_class[i];
+     3. selection (MPI_Classification.cpp:924-935, 1137-1150): in each class the particles whose distance is at most
+        the value at index floor(top_percentage (m - 1)) of the class's m sorted distances (all when
+        top_percentage >= 1), out_selected[i] = 1;
+     4. each class's new reference: the amplitude-weighted average of its selected particles at their alignments
+        to it (getImage, CorrMetric.h:803-984), RMS-normalised; a class without selected particles keeps its
+        reference.
+   Outputs (out_selected, out_distance, out_matrices may be NULL): out_distance[i] and out_matrices[i*16] are the
+   distance and the alignment of particle i to its class; passing out_class masked by out_selected (-1 where not
+   selected) and out_matrices as the next call's ref_members and member_matrices continues the iterations. */
+int ettk_mra_iteration(ettk_plan* plan, int K, float* d_refs, const int* ref_members, const double* member_matrices,
+                       const float* d_particles, int n, const ettk_particle* descs, int type, int form,
+                       float top_percentage, int* out_class, int* out_selected, float* out_distance,
+                       double* out_matrices);
+
+/* Execution choices of a plan (no reference counterpart): evaluations per constrained-CCC chunk and whether the
+   chunks are replayed from HIP graphs captured once per chunk size and particle array.  Results do not depend on
+   them.  ettk_plan_tune times the plan's CCC batch for every candidate (chunks of 16 to 256 evaluations, eager or
+   graph launches) on evaluations of the given particles at fixed pseudo-random rotations, keeps the fastest and
+   appends it to the tuning cache (the file named by ETTK_TUNE_CACHE, else ~/.cache/ettk_amd_tune.txt) under the
+   key (device architecture, box size, bandwidth, translation window); a plan created later for the same key
+   starts from the recorded choice.  The plan's reference must be set. */
+int ettk_plan_tune(ettk_plan* plan, const float* d_particles, int n, const ettk_particle* descs, int* out_chunk,
+                   int* out_graphs, double* out_us_per_eval);
+int ettk_plan_set_execution(ettk_plan* plan, int ccc_chunk, int use_graphs);
+int ettk_plan_get_execution(ettk_plan* plan, int* ccc_chunk, int* use_graphs);
+
+/* Particles of uneven box sizes: several buckets, each of one box size and bandwidth with its own reference and
+   particles, aligned in one call (as MPI_Classification aligns volume lists cut at several sizes,
+   src/MPI_Classification.cpp:1087, each with nbfProjectionRotationMetric3D::execute, ProjRot.h:416-1052).  The
+   context keeps one plan per distinct parameter set across calls; each plan runs its buckets on its own streams
+   from its own host thread, so the buckets' kernels overlap on the GPU.  mode as ettk_align_batch.  Flags:
+   ETTK_BUCKETS_GRAPHS replays CCC chunks from captured graphs (unless the tuning cache says otherwise),
+   ETTK_BUCKETS_TUNE runs ettk_plan_tune on each bucket's particles first. */
+typedef struct ettk_bucket {
+    ettk_params params;
+    const float* d_reference;   /* N^3, x fastest */
+    ettk_particle reference;
+    const float* d_particles;   /* n x N^3 */
+    int n;
+    const ettk_particle* descs;
+    ettk_result* out;           /* n results */
+} ettk_bucket;
+#define ETTK_BUCKETS_GRAPHS 1
+#define ETTK_BUCKETS_TUNE   2
+int ettk_align_buckets(ettk_ctx* ctx, int n_buckets, const ettk_bucket* buckets, int mode, int flags);
+
+/* Sub-volumes cut from a tomogram: nbfWedgedSubImage3D::getImage with its tomogram loaded
+   (inc/em/nbfWedgedSubImage3D.h:603-802, normalize off), as src/CutVolumes3DFromPositions.cpp and the first
+   getImage of each particle do.  d_tomogram is nx x ny x nz floats, x fastest (the MRC order); for particle i,
+   output voxel q of the N^3 box reads the tomogram at positions[3i..3i+2] + fin (q - N/2 - cut_offset z) with
+   fin = Euler(descs[i].normal) descs[i].matrix (WSub.h:712-749), by VTK's tricubic interpolation (taps clamped to
+   the tomogram); samples outside [-0.5, n - 0.5] on any axis are invalid.  The mean of the valid samples is
+   subtracted and invalid voxels are set to 0 (WSub.h:794-802).  d_out receives n volumes of N^3 floats, x fastest,
+   ready to be passed as particles.  Runs on the context's stream and returns when the volumes are complete. */
+int ettk_extract_batch(ettk_ctx* ctx, const float* d_tomogram, int nx, int ny, int nz, int n, const double* positions,
+                       const ettk_particle* descs, int N, float* d_out);
+
 /* Measurement hooks (no reference counterpart).  ettk_profile(ctx, 1) resets and enables per-kernel HIP-event
    timing on the context's stream; ettk_profile_query returns, for one kernel name (e.g. "k_fft_strided<64>",
    "k_reslice", "k_combine_dftA"), the summed launch time in ms, the launch count and the algorithmic bytes
