@@ -19,16 +19,27 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = ["fft.hip", "ccc.hip", "so3.hip", "refine.hip", "host.cpp", "align.cpp", "refine.cpp", "avg.hip", "avg.cpp",
-           "noalloc.hip", "extract.hip", "classify.cpp", "tune.cpp"]
+           "noalloc.hip", "extract.hip", "classify.cpp", "tune.cpp", "cand.hip"]
 
 
 def build_lib(verbose: bool = False) -> Path:
-    objs = []
-    for src in SOURCES:
+    """Compiles every source (several hipcc processes at once) and links libettk_align.so in-tree."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def compile_one(src):
         obj = CSRC / ("_" + src + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
         cmd = [HIPCC, *HIP_FLAGS, *lang, "-c", str(CSRC / src), "-o", str(obj)]
-        r = subprocess.run(cmd, capture_output=True, text=True)
+        return src, obj, subprocess.run(cmd, capture_output=True, text=True)
+
+    try:
+        jobs = max(1, min(8, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        jobs = max(1, min(8, os.cpu_count() or 1))
+    objs = []
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(compile_one, SOURCES))
+    for src, obj, r in results:
         if r.returncode != 0:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError(f"hipcc failed on {src}")

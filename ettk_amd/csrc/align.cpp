@@ -87,6 +87,7 @@ std::vector<float> spherical_wedge_host(int B, const ettk_particle& d, bool wedg
 struct Cand {
     float a, b, g, v;
 };
+constexpr float kEquivTol = 0.25f;  // L1 distance of equivalent rotation matrices in the search (ProjRot.h:142, 1633)
 struct CandCmp {
     bool operator()(const Cand& x, const Cand& y) const { return x.v > y.v; }  // greaterPD: min-heap on [3]
 };
@@ -220,6 +221,18 @@ static const ZyzTables& zyz_tables(int B) {
     return cache.emplace(B, std::move(z)).first->second;
 }
 
+// the survivors of the device extraction (k_lm_candidates): valid, mutually non-equivalent maxima in index order,
+// pushed as candidates_from_maxima pushes its survivors
+static void candidates_push(const ettk_plan* P, const std::vector<int>& idx, const std::vector<double>& val,
+                            CandQueue& pq) {
+    const int B = P->p.B, n = 2 * B;
+    const ZyzTables& Z = zyz_tables(B);
+    for (size_t k = 0; k < idx.size(); k++) {
+        const int t = idx[k], a = t / (n * n), b = (t / n) % n, g = t % n;
+        pq.push(Cand{Z.alpha[a], Z.beta[b], Z.gamma[g], (float)(-val[k])});
+    }
+}
+
 static void candidates_from_maxima(const ettk_plan* P, std::vector<std::pair<int, double>>& lm, CandQueue& pq) {
     const int B = P->p.B, n = 2 * B;
     const ZyzTables& Z = zyz_tables(B);
@@ -235,7 +248,7 @@ static void candidates_from_maxima(const ettk_plan* P, std::vector<std::pair<int
             mats.push_back(m);
         }
     }
-    eliminate_equivalent(pos, mats, pq, 0.25f);
+    eliminate_equivalent(pos, mats, pq, kEquivTol);
 }
 
 struct SearchOut {
@@ -306,8 +319,20 @@ struct SearchLists {
     }
     std::chrono::steady_clock::time_point t_ready;  // when its kernels had completed
 };
-// local maxima kept per correlation volume ((2B)^3 voxels): 16384 up to B = 32, four times that per doubling
-static int lm_cap(int B) { return B <= 32 ? 16384 : (B <= 64 ? 65536 : 262144); }
+// the first beta bin the rotation restriction removes from the correlation volumes (they hold -FLT_MAX from it on,
+// ProjRot.h:601-602, 631-632)
+static int restrict_b0(const ettk_plan* P) {
+    const int B = P->p.B, n = 2 * B;
+    const double f = std::floor(P->rot_restrict * B / 90.0 - 1.0);
+    return (int)std::max(1.0, std::min(f, (double)(n - 1)));
+}
+// local maxima kept per correlation volume ((2B)^3 voxels): 16384 up to B = 32, four times that per doubling, plus
+// the restricted beta bins, every voxel of which is a maximum of the >= stencil (a plateau at -FLT_MAX)
+static int lm_cap(const ettk_plan* P) {
+    const int B = P->p.B, n = 2 * B;
+    const long base = B <= 32 ? 16384 : (B <= 64 ? 65536 : 262144);
+    return (int)std::min((long)n * n * n, base + (long)n * n * (n - restrict_b0(P)));
+}
 static constexpr int kSearchSlots = 4;  // search chunks in flight (device list buffers and host threads)
 
 // GPU part of the search for particles [p0, p0+cn): projections, SO(3) correlations, local maxima.  Nothing
@@ -323,7 +348,7 @@ static void search_gpu(ettk_plan* P, const float* d_parts, const ettk_particle* 
     L.cn = cn;
     L.uw2.assign(in.uw2.begin() + p0, in.uw2.begin() + p0 + cn);
     L.uw1 = S.use_window1;
-    L.lm = &lm_slot(S, slot, 2 * cn, lm_cap(B));
+    L.lm = &lm_slot(S, slot, 2 * cn, lm_cap(P));
     S.Pp2.ensure((size_t)cn * nn);
     S.Mx.ensure(cn);
     project_images(P, d_parts, descs, nullptr, cn, false, S.Pp2.p, in.geom + p0);
@@ -374,8 +399,7 @@ static void search_gpu(ettk_plan* P, const float* d_parts, const ettk_particle* 
     run_soft(P, inst, S.Pp2.p, S.W2.p, S.Mx.p, S.C.p, L.lm->hinst.p);
     S.ops.ensure(ops.size());
     ETTK_HIP(hipMemcpyAsync(S.ops.p, L.lm->hops.p, ops.size() * sizeof(ComboOp), hipMemcpyHostToDevice, s));
-    double f = std::floor(P->rot_restrict * B / 90.0 - 1.0);
-    int b0 = (int)std::max(1.0, std::min(f, (double)(n - 1)));
+    const int b0 = restrict_b0(P);
     {
         ProfScope ps(kp, "k_combine", (double)(nq + 2 * cn) * n3 * 8, s);
         launch_combine(S.C.p, S.ops.p, 2 * cn, B, b0, S.Cc.p, s);
@@ -383,10 +407,37 @@ static void search_gpu(ettk_plan* P, const float* d_parts, const ettk_particle* 
     launch_fill_zero(L.lm->count.p, (size_t)2 * cn * kLmCountStride, s);
     {
         ProfScope ps(kp, "k_localmax", 2.0 * cn * n3 * 8, s);
-        launch_localmax(S.Cc.p, 2 * cn, B, lm_cap(B), L.lm->count.p, L.lm->idx.p, L.lm->val.p, s);
+        launch_localmax(S.Cc.p, 2 * cn, B, lm_cap(P), L.lm->count.p, L.lm->idx.p, L.lm->val.p, s);
     }
-    launch_lm_compact(L.lm->count.p, L.lm->idx.p, L.lm->val.p, 2 * cn, lm_cap(B), L.lm->head.p, L.lm->pidx.p,
-                      L.lm->pval.p, s);
+    {
+        // candidate extraction of the active volumes on the device (the second volume of a particle is used only
+        // with a spherical window)
+        // ETTK_HOST_CANDIDATES=1 leaves every volume to the host's extraction (the same candidates; for checks)
+        LmBuffers& lb = *L.lm;
+        const char* hc = std::getenv("ETTK_HOST_CANDIDATES");
+        const int act = hc && hc[0] == '1' ? 2 : 1;
+        for (int v = 0; v < 2 * cn; v++) lb.hact.p[v] = ((v % 2 == 0) || uw1 || L.uw2[v / 2]) ? act : 0;
+        ETTK_HIP(hipMemcpyAsync(lb.act.p, lb.hact.p, (size_t)2 * cn * sizeof(int), hipMemcpyHostToDevice, s));
+        if (S.zyz_tabs.n == 0) {
+            const ZyzTables& Z = zyz_tables(B);
+            std::vector<double> t((size_t)3 * n * 16);
+            for (int i = 0; i < n; i++) {
+                std::memcpy(&t[(size_t)i * 16], Z.ra[i].a, sizeof(Z.ra[i].a));
+                std::memcpy(&t[((size_t)n + i) * 16], Z.rb[i].a, sizeof(Z.rb[i].a));
+                std::memcpy(&t[((size_t)2 * n + i) * 16], Z.rg[i].a, sizeof(Z.rg[i].a));
+            }
+            S.zyz_tabs.ensure(t.size());
+            ETTK_HIP(hipMemcpy(S.zyz_tabs.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        S.cand_mats.ensure((size_t)2 * cn * kCandMax * 16);
+        S.cand_adj.ensure((size_t)2 * cn * kCandMax * 64);
+        ProfScope ps(kp, "k_lm_candidates", 0.0, s);
+        launch_lm_candidates(lb.count.p, lb.idx.p, lb.val.p, lm_cap(P), lb.act.p, 2 * cn, S.zyz_tabs.p, n,
+                             (double)P->trans_restrict, (double)P->rot_restrict, kEquivTol, S.cand_mats.p,
+                             S.cand_adj.p, lb.kept.p, lb.mode.p, s);
+    }
+    launch_lm_compact(L.lm->count.p, L.lm->idx.p, L.lm->val.p, L.lm->kept.p, L.lm->mode.p, 2 * cn, lm_cap(P),
+                      L.lm->head.p, L.lm->pidx.p, L.lm->pval.p, s);
     ETTK_HIP(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
     ETTK_HIP(hipEventRecord(L.done, s));
 }
@@ -404,7 +455,7 @@ static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOu
     hipStream_t cs = P->copy_stream;
     LmBuffers& lb = *L.lm;
     const int* head = lb.hhead.p;
-    ETTK_HIP(hipMemcpyAsync(lb.hhead.p, lb.head.p, (size_t)(2 * nv + 1) * sizeof(int), hipMemcpyDeviceToHost, cs));
+    ETTK_HIP(hipMemcpyAsync(lb.hhead.p, lb.head.p, (size_t)(3 * nv + 1) * sizeof(int), hipMemcpyDeviceToHost, cs));
     ETTK_HIP(hipStreamSynchronize(cs));
     const int total = head[nv];
     if (total > 0) {
@@ -419,16 +470,22 @@ static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOu
     if (kp) kp->add_host("search_download", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count());
     std::vector<std::vector<int>> idx(nv);
     std::vector<std::vector<double>> val(nv);
+    std::vector<int> mode(nv, 0);
     for (int v = 0; v < nv; v++) {
         int q = v / 2;
         bool two = uw1 || L.uw2[q];
         if (v % 2 == 1 && !two) continue;
-        const int c = head[nv + 1 + v];
-        ETTK_REQUIRE(c <= lm_cap(P->p.B), ETTK_ERR_UNSUPPORTED,
-                     "more SO(3) local maxima than the candidate list holds (" + std::to_string(c) + ")");
+        const int raw = head[nv + 1 + v];
+        ETTK_REQUIRE(raw <= lm_cap(P), ETTK_ERR_UNSUPPORTED,
+                     "more SO(3) local maxima than the candidate list holds (" + std::to_string(raw) + ")");
+        mode[v] = head[2 * nv + 1 + v];
+        const int c = head[v + 1] - head[v];  // survivors (mode 0) or the raw maxima (mode 1)
         idx[v].assign(pidx + head[v], pidx + head[v] + c);
         val[v].assign(pval + head[v], pval + head[v] + c);
-        if (kp && kp->on) kp->add_host("lm_count", (double)c);
+        if (kp && kp->on) {
+            kp->add_host("lm_count", (double)raw);
+            if (mode[v]) kp->add_host("lm_host_volumes", 1.0);
+        }
     }
     for (int q = 0; q < cn; q++) {
         SearchOut& o = out[p0 + q];
@@ -438,10 +495,15 @@ static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOu
     auto work = [&](int v) {
         int q = v / 2;
         if (v % 2 == 1 && !out[p0 + q].two) return;
+        SearchOut& o = out[p0 + q];
+        CandQueue& pq = v % 2 == 0 ? o.pq1 : o.pq2;
+        if (mode[v] == 0) {
+            candidates_push(P, idx[v], val[v], pq);  // extracted on the device
+            return;
+        }
         std::vector<std::pair<int, double>> lm(idx[v].size());
         for (size_t k = 0; k < lm.size(); k++) lm[k] = {idx[v][k], val[v][k]};
-        SearchOut& o = out[p0 + q];
-        candidates_from_maxima(P, lm, v % 2 == 0 ? o.pq1 : o.pq2);
+        candidates_from_maxima(P, lm, pq);
     };
     // the next chunk's extraction and input preparation run at the same time: half of the threads each, all of
     // them for the last chunk of a call
@@ -916,14 +978,33 @@ class BatchEval {
     void done() {
         std::unique_lock<std::mutex> lk(mu_);
         active_--;
-        if (active_ > 0 && (int)reqs_.size() == active_) run_locked();
+        if (!failed_ && active_ > 0 && (int)reqs_.size() == active_) run_locked();
     }
+    // a worker's error: every waiting worker wakes and throws it, and the queued requests (which point into the
+    // waiting workers' frames) are dropped before those frames unwind
     void fail(const Error& e) {
         std::unique_lock<std::mutex> lk(mu_);
         if (!failed_) { failed_ = true; err_ = e; }
         active_--;
+        reqs_.clear();
         gen_++;
         cv_.notify_all();
+    }
+    // the body of one worker thread: runs work(), reports its completion or its error
+    template <class F>
+    void worker(F&& work) {
+        try {
+            work();
+            done();
+        } catch (const Error& e) {
+            fail(e);
+        } catch (const std::exception& e) {
+            fail(Error{ETTK_ERR_ALLOC, e.what()});
+        }
+    }
+    // after every worker has joined: the first error, if any, for the caller
+    void rethrow() const {
+        if (failed_) throw err_;
     }
     long batches = 0;
 
@@ -950,6 +1031,9 @@ class BatchEval {
         } catch (const Error& e) {
             failed_ = true;
             err_ = e;
+        } catch (const std::exception& e) {
+            failed_ = true;
+            err_ = Error{ETTK_ERR_ALLOC, e.what()};
         }
         batches++;
         reqs_.clear();
@@ -985,9 +1069,7 @@ void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
     ETTK_REQUIRE(mode >= 0 && mode <= 2, ETTK_ERR_ARG, "ettk_align_batch: mode must be 0, 1 or 2");
     std::memset(out, 0, sizeof(ettk_result) * n);
     P->timing = Timing{};
-    hipEvent_t a, b;
-    ETTK_HIP(hipEventCreate(&a));
-    ETTK_HIP(hipEventCreate(&b));
+    ScopedEvent a, b;
     ETTK_HIP(hipEventRecord(a, P->stream));
     std::vector<SearchOut> so(n);
     if (mode == 0) {
@@ -1019,14 +1101,12 @@ void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
             std::vector<std::thread> th;
             for (int w = 0; w < T; w++)
                 th.emplace_back([&, w] {
-                    try {
+                    be.worker([&] {
                         for (int i = w; i < n; i += T) final_candidates(P, i, sel[i], so[i], ev, out[i], nccc[i]);
-                        be.done();
-                    } catch (const Error& e) {
-                        be.fail(e);
-                    }
+                    });
                 });
             for (auto& t : th) t.join();
+            be.rethrow();
         } else {
             std::vector<SearchOut*> sp(n);
             for (int i = 0; i < n; i++) sp[i] = &so[i];
@@ -1039,8 +1119,6 @@ void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
     float ms = 0;
     ETTK_HIP(hipEventElapsedTime(&ms, a, b));
     P->timing.total_ms = ms;
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
 }
 
 void refine_with_seeds(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
@@ -1203,14 +1281,12 @@ void distances_run(ettk_plan* P, const float* d_parts, const ettk_particle* desc
             std::vector<std::thread> th;
             for (int w = 0; w < T; w++)
                 th.emplace_back([&, w] {
-                    try {
+                    be.worker([&] {
                         for (int i = w; i < n; i += T) run_one(i, ev);
-                        be.done();
-                    } catch (const Error& e) {
-                        be.fail(e);
-                    }
+                    });
                 });
             for (auto& x : th) x.join();
+            be.rethrow();
         }
     }
     for (int i = 0; i < n; i++) {
@@ -1245,6 +1321,9 @@ int ettk_project_batch(ettk_plan* P, const float* d_particles, int n, const ettk
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return ETTK_ERR_ALLOC;
     }
     set_error("");
     return ETTK_OK;
@@ -1264,6 +1343,9 @@ int ettk_so3_correlate_batch(ettk_plan* P, int n, const double* sig, const doubl
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return ETTK_ERR_ALLOC;
     }
     set_error("");
     return ETTK_OK;
@@ -1278,6 +1360,9 @@ int ettk_sph_rotate_batch(ettk_plan* P, int n, const float* images, const double
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return ETTK_ERR_ALLOC;
     }
     set_error("");
     return ETTK_OK;
@@ -1292,6 +1377,9 @@ int ettk_distances_batch(ettk_plan* P, const float* d_particles, int n, const et
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return ETTK_ERR_ALLOC;
     }
     set_error("");
     return ETTK_OK;
@@ -1306,6 +1394,9 @@ int ettk_align_batch(ettk_plan* P, const float* d_particles, int n, const ettk_p
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return ETTK_ERR_ALLOC;
     }
     set_error("");
     return ETTK_OK;

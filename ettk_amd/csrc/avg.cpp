@@ -85,6 +85,9 @@ int ettk_avg_reset(ettk_plan* P) {
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return ETTK_ERR_ALLOC;
     }
     set_error("");
     return ETTK_OK;
@@ -131,6 +134,9 @@ int ettk_avg_accumulate(ettk_plan* P, const float* d_particles, int n, const ett
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return ETTK_ERR_ALLOC;
     }
     set_error("");
     return ETTK_OK;
@@ -184,6 +190,9 @@ int ettk_avg_variance(ettk_plan* P, const float* d_particles, int n, const ettk_
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return ETTK_ERR_ALLOC;
     }
     set_error("");
     return ETTK_OK;
@@ -218,6 +227,9 @@ int ettk_avg_wedge(ettk_plan* P, int n, const ettk_particle* descs, const float*
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return ETTK_ERR_ALLOC;
     }
     set_error("");
     return ETTK_OK;
@@ -239,6 +251,9 @@ int ettk_avg_finalize(ettk_plan* P, float* d_out) {
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return ETTK_ERR_ALLOC;
     }
     set_error("");
     return ETTK_OK;

@@ -111,6 +111,18 @@ struct DevBuf {
     }
 };
 
+// a HIP event owned by a scope (destroyed on every exit, including an exception)
+struct ScopedEvent {
+    hipEvent_t e = nullptr;
+    explicit ScopedEvent(unsigned flags = hipEventDefault) { ETTK_HIP(hipEventCreateWithFlags(&e, flags)); }
+    ScopedEvent(const ScopedEvent&) = delete;
+    ScopedEvent& operator=(const ScopedEvent&) = delete;
+    ~ScopedEvent() {
+        if (e) (void)hipEventDestroy(e);
+    }
+    operator hipEvent_t() const { return e; }
+};
+
 // pinned host buffer (hipHostMalloc): host-to-device copies from it are asynchronous with respect to the host
 template <class T>
 struct PinnedBuf {

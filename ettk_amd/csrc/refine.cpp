@@ -352,6 +352,9 @@ extern "C" int ettk_refine_batch(ettk_plan* P, const float* d_particles, int n_p
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return ETTK_ERR_ALLOC;
     }
     set_error("");
     return ETTK_OK;
@@ -380,6 +383,9 @@ extern "C" int ettk_ccc_grad_batch(ettk_plan* P, const float* d_particles, int n
     } catch (const Error& e) {
         set_error(e.msg);
         return e.code;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return ETTK_ERR_ALLOC;
     }
     set_error("");
     return ETTK_OK;

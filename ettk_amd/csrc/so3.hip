@@ -667,20 +667,23 @@ __global__ void __launch_bounds__(kSynT) k_so3_synth_fft(const double2* __restri
 
 // The synthesis as matrix products on the FP64 matrix cores.  For one order pair (m1, m2) and a group of up to 64
 // instances q the synthesis is
-//   T1[q][beta][m1][m2] = (-1)^(m1+m2) sum_l D(beta, l) X(l, q),  X(l, q) = sig_q(l, -m1) conj(pat_q(l, -m2)),
+//   T1[m1, m2][beta][q] = (-1)^(m1+m2) sum_l D(beta, l) X(l, q),  X(l, q) = sig_q(l, -m1) conj(pat_q(l, -m2)),
 // a (2B x B) by (B x 64) product per pair (real D, complex X: two products).  One workgroup per (pair, instance
-// group); wave w owns the 16 beta rows [16 w, 16 w + 16) and the four 16-column instance tiles, and issues
+// group); wave w owns the 16 beta rows [16 w, 16 w + 16) and up to four 16-column instance tiles, and issues
 // v_mfma_f64_16x16x4f64 over l in steps of four from max(|m1|, |m2|) rounded down to a multiple of four (the table is
 // zero below, so the extra terms are exact zeros).  Operand layout of the instruction on gfx950: lane L holds
-// A[L % 16][L / 16] and B[L / 16][L % 16]; result element i of lane L is D[4 i + L / 16][L % 16].
+// A[L % 16][L / 16] and B[L / 16][L % 16]; result element i of lane L is D[4 i + L / 16][L % 16].  The coefficients
+// arrive instance-fastest (csT, cpT: [coefficient][qp], qp = nq rounded up to 16, zero beyond nq) and T1 is written
+// instance-fastest, so the sixteen lanes of a row load and store 256 contiguous bytes.
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 template <int B>
-__global__ void __launch_bounds__(2 * B * 4) k_so3_synth_mfma(const double2* __restrict__ cs, const double2* __restrict__ cp,
-                                                         const double* __restrict__ Dp, int nq, double2* __restrict__ T1) {
+__global__ void __launch_bounds__(2 * B * 4) k_so3_synth_mfma(const double2* __restrict__ csT, const double2* __restrict__ cpT,
+                                                         const double* __restrict__ Dp, int nqp, double2* __restrict__ T1) {
     constexpr int n = 2 * B;
     const int pair = blockIdx.x, c1 = pair / n, c2 = pair % n;
     const int w = threadIdx.x >> 6, L = threadIdx.x & 63, lr = L & 15, lk = L >> 4;
     const int q0 = blockIdx.y * 64;
+    const int ntile = min(4, (nqp - q0) >> 4);  // instance tiles of this group (uniform)
     mfma_d4 re[4], im[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
@@ -696,46 +699,61 @@ __global__ void __launch_bounds__(2 * B * 4) k_so3_synth_mfma(const double2* __r
         for (int l = l0 & ~3; l < B; l += 4) {
             const int ll = l + lk;
             const double a = D[(size_t)ll * n];
-            const int is = max(ll * ll + ll - m1, 0), ip = max(ll * ll + ll - m2, 0);
+            const double2* sp = csT + (size_t)max(ll * ll + ll - m1, 0) * nqp + q0 + lr;
+            const double2* pp = cpT + (size_t)max(ll * ll + ll - m2, 0) * nqp + q0 + lr;
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const int q = q0 + 16 * t + lr;
-                double xr = 0.0, xi = 0.0;
-                if (q < nq) {
-                    const double2 sv = cs[(size_t)q * B * B + is], pv = cp[(size_t)q * B * B + ip];
-                    xr = sv.x * pv.x + sv.y * pv.y;
-                    xi = sv.y * pv.x - sv.x * pv.y;
+                if (t < ntile) {
+                    const double2 sv = sp[16 * t], pv = pp[16 * t];
+                    const double xr = sv.x * pv.x + sv.y * pv.y, xi = sv.y * pv.x - sv.x * pv.y;
+                    re[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xr, re[t], 0, 0, 0);
+                    im[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xi, im[t], 0, 0, 0);
                 }
-                re[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xr, re[t], 0, 0, 0);
-                im[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, xi, im[t], 0, 0, 0);
             }
         }
         if ((m1 + m2) & 1) sg = -1.0;
     }
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        const int q = q0 + 16 * t + lr;
-        if (q >= nq) continue;
+        if (t >= ntile) continue;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int kb = 16 * w + 4 * i + lk;
-            T1[(((size_t)q * n + kb) * n + c1) * n + c2] = make_double2(sg * re[t][i], sg * im[t][i]);
+            T1[((size_t)pair * n + kb) * nqp + q0 + 16 * t + lr] = make_double2(sg * re[t][i], sg * im[t][i]);
         }
     }
 }
 
+// coefficients [q][c] -> [c][qp] for the matrix-core synthesis, zero for q >= nq (c < BB = B^2)
+__global__ void k_coef_transpose(const double2* __restrict__ cs, const double2* __restrict__ cp, int nq, int nqp, int BB,
+                                 double2* __restrict__ csT, double2* __restrict__ cpT) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)BB * nqp) return;
+    const int c = (int)(t / nqp), q = (int)(t - (long)c * nqp);
+    const double2 z = make_double2(0.0, 0.0);
+    csT[t] = q < nq ? cs[(size_t)q * BB + c] : z;
+    cpT[t] = q < nq ? cp[(size_t)q * BB + c] : z;
+}
+
 // the two DFTs of one (instance, beta) slice of T1 in LDS: C[q][alpha][beta][gamma] = Re sum T1 e^{-i m1 alpha}
-// e^{-i m2 gamma}
+// e^{-i m2 gamma}.  The slice is gathered from T1[pair][beta][qp] (one 16-byte element per pair); the eight
+// instances whose elements share a cache line run back to back on one XCD (workgroup b runs on XCD b mod 8), so
+// each line of T1 is fetched into that L2 once: workgroup b takes XCD x = b mod 8, step s = b / 8, line group
+// c = 8 (s / 8) + x = beta * Q8 + g (Q8 = ceil(nq / 8) groups of eight instances) and instance q = 8 g + s mod 8.
 template <int LOGN>
 __global__ void __launch_bounds__(kSynT) k_so3_fft2(const double2* __restrict__ T1, const double2* __restrict__ tw,
-                                                   double* __restrict__ C) {
+                                                   int nq, int nqp, int Q8, double* __restrict__ C) {
     constexpr int n = 1 << LOGN, LD = n + 1;
     extern __shared__ double2 X[];  // n * LD
     __shared__ double2 stw[n];
-    const int q = blockIdx.x, kb = blockIdx.y;
+    const int b = blockIdx.x, s = b >> 3;
+    const int c = ((s >> 3) << 3) | (b & 7);
+    const int kb = c / Q8, q = (c - kb * Q8) * 8 + (s & 7);
+    if (q >= nq) return;  // the whole workgroup: before any barrier
     for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = tw[t];
-    const double2* src = T1 + ((size_t)q * n + kb) * n * n;
-    block_copy<kSynT, n * n>([&](int t) { return src[t]; }, [&](int t, double2 v) { X[(t / n) * LD + (t % n)] = v; });
+    const double2* src = T1 + (size_t)kb * nqp + q;
+    block_copy<kSynT, n * n>([&](int t) { return src[(size_t)t * n * nqp]; },
+                             [&](int t, double2 v) { X[(t / n) * LD + (t % n)] = v; });
     __syncthreads();
     fft_lds_f64<LOGN>(X, LD, 1, stw);  // over m1 for every m2 -> alpha
     fft_lds_f64<LOGN>(X, 1, LD, stw);  // over m2 for every alpha -> gamma
@@ -1208,19 +1226,25 @@ void soft_core(ettk_plan* P, const std::vector<SoftInst>& inst, double* d_C) {
     k_ring_dft<<<dim3(16, nq), kT, 0, s>>>(S.pat.p, S.tw.p, B, S.fm.p);
     k_legendre_fwd<<<dim3(8, nq), kT, 0, s>>>(S.fm.p, S.legF.p, B, S.cp.p);
     if (mfma_synthesis(B)) {
-        S.T1.ensure((size_t)nq * n3);
+        const int nqp = (nq + 15) & ~15, Q8 = (nq + 7) / 8, BB = B * B;
+        S.csT.ensure((size_t)BB * nqp);
+        S.cpT.ensure((size_t)BB * nqp);
+        S.T1.ensure((size_t)nqp * n3);
+        k_coef_transpose<<<(unsigned)(((long)BB * nqp + kT - 1) / kT), kT, 0, s>>>(S.cs.p, S.cp.p, nq, nqp, BB, S.csT.p,
+                                                                                  S.cpT.p);
         {
             // algorithmic bytes: the coefficients and the T1 slices written
             ProfScope ps(P->ctx->fft.prof, "k_so3_synth_mfma", (double)nq * ((double)n3 * 16 + 32.0 * B * B), s);
-            const dim3 grid(n * n, (nq + 63) / 64);
-            if (B == 16) k_so3_synth_mfma<16><<<grid, 128, 0, s>>>(S.cs.p, S.cp.p, S.dtabP.p, nq, S.T1.p);
-            else k_so3_synth_mfma<32><<<grid, 256, 0, s>>>(S.cs.p, S.cp.p, S.dtabP.p, nq, S.T1.p);
+            const dim3 grid(n * n, (nqp + 63) / 64);
+            if (B == 16) k_so3_synth_mfma<16><<<grid, 128, 0, s>>>(S.csT.p, S.cpT.p, S.dtabP.p, nqp, S.T1.p);
+            else k_so3_synth_mfma<32><<<grid, 256, 0, s>>>(S.csT.p, S.cpT.p, S.dtabP.p, nqp, S.T1.p);
         }
         {
             ProfScope ps(P->ctx->fft.prof, "k_so3_fft2", (double)nq * n3 * (16 + 8), s);
             const size_t shm = (size_t)n * (n + 1) * sizeof(double2);
-            if (B == 16) k_so3_fft2<5><<<dim3(nq, n), kSynT, shm, s>>>(S.T1.p, S.tw.p, d_C);
-            else k_so3_fft2<6><<<dim3(nq, n), kSynT, shm, s>>>(S.T1.p, S.tw.p, d_C);
+            const unsigned nb = (unsigned)(Q8 * 8 * n);  // Q8 n is a multiple of 8 (n = 32, 64)
+            if (B == 16) k_so3_fft2<5><<<nb, kSynT, shm, s>>>(S.T1.p, S.tw.p, nq, nqp, Q8, d_C);
+            else k_so3_fft2<6><<<nb, kSynT, shm, s>>>(S.T1.p, S.tw.p, nq, nqp, Q8, d_C);
         }
     } else if (fused_synthesis(B)) {
         ProfScope ps(P->ctx->fft.prof, "k_so3_synth_fft", (double)nq * ((double)n3 * 8 + 32.0 * B * B), s);
@@ -1433,15 +1457,23 @@ void launch_localmax(const double* C, int np, int B, int cap, int* count, int* i
     ETTK_HIP(hipGetLastError());
 }
 // block v: offset of list v = sum of the clamped counts before it; copies its entries in slot order
+// packs the per-volume lists back to back for one download: head = [offsets (np + 1), raw counts (np), modes (np)];
+// a volume of mode 0 contributes its kept[v] candidate survivors, one of mode 1 its raw list of maxima
 __global__ void k_lm_compact(const int* __restrict__ count, const int* __restrict__ idx, const double* __restrict__ val,
-                             int np, int cap, int* __restrict__ head, int* __restrict__ pidx, double* __restrict__ pval) {
+                             const int* __restrict__ kept, const int* __restrict__ mode, int np, int cap,
+                             int* __restrict__ head, int* __restrict__ pidx, double* __restrict__ pval) {
     const int v = blockIdx.x;
+    auto packed = [&](int u) {
+        const int raw = min(count[(size_t)u * kLmCountStride], cap);
+        return mode[u] ? raw : kept[u];
+    };
     int off = 0;
-    for (int u = 0; u < v; u++) off += min(count[(size_t)u * kLmCountStride], cap);
-    const int c = min(count[(size_t)v * kLmCountStride], cap);
+    for (int u = 0; u < v; u++) off += packed(u);
+    const int c = packed(v);
     if (threadIdx.x == 0) {
         head[v] = off;
         head[np + 1 + v] = count[(size_t)v * kLmCountStride];
+        head[2 * np + 1 + v] = mode[v];
         if (v == np - 1) head[np] = off + c;
     }
     for (int k = threadIdx.x; k < c; k += blockDim.x) {
@@ -1449,9 +1481,9 @@ __global__ void k_lm_compact(const int* __restrict__ count, const int* __restric
         pval[off + k] = val[(size_t)v * cap + k];
     }
 }
-void launch_lm_compact(const int* count, const int* idx, const double* val, int np, int cap, int* head, int* pidx,
-                       double* pval, hipStream_t s) {
-    k_lm_compact<<<np, kT, 0, s>>>(count, idx, val, np, cap, head, pidx, pval);
+void launch_lm_compact(const int* count, const int* idx, const double* val, const int* kept, const int* mode, int np,
+                       int cap, int* head, int* pidx, double* pval, hipStream_t s) {
+    k_lm_compact<<<np, kT, 0, s>>>(count, idx, val, kept, mode, np, cap, head, pidx, pval);
     ETTK_HIP(hipGetLastError());
 }
 void launch_ppmax(const float* Pp1, const float* Pp2, int np, int B, float* Mx, hipStream_t s) {

@@ -29,6 +29,9 @@ constexpr int kLmCountStride = 32;
 struct LmBuffers {
     DevBuf<int> count, idx;
     DevBuf<double> val;
+    // device candidate extraction: volumes to process (uploaded from hact), survivors per volume, mode per volume
+    DevBuf<int> act, kept, mode;
+    PinnedBuf<int> hact;
     // compacted lists: head = [offsets (nvol + 1), counts (nvol)], then the packed indices and values
     DevBuf<int> head, pidx;
     DevBuf<double> pval;
@@ -62,6 +65,10 @@ struct So3State {
     DevBuf<double> sig, pat, vals, Rr, wv, pvals;
     DevBuf<int> ppos;
     DevBuf<double2> fm, cs, cp, T1, T2, rot, g;
+    DevBuf<double> zyz_tabs;  // [Ra; Rb; Rg] of the correlation grid, 2B 4x4 matrices each (device candidates)
+    DevBuf<double> cand_mats;  // scratch of the device candidates: matrices and close partners per volume
+    DevBuf<int> cand_adj;
+    DevBuf<double2> csT, cpT;  // coefficients instance-fastest for the matrix-core synthesis: [l l + l - m][q]
     DevBuf<int> pos;
     // search workspace (align driver)
     DevBuf<float> Pp2, W2, Mx;
@@ -79,11 +86,15 @@ inline LmBuffers& lm_slot(So3State& S, int slot, int nvol, int cap) {
     b.count.ensure((size_t)nvol * kLmCountStride);
     b.idx.ensure((size_t)nvol * cap);
     b.val.ensure((size_t)nvol * cap);
-    b.head.ensure((size_t)2 * nvol + 1);
+    b.head.ensure((size_t)3 * nvol + 1);
+    b.act.ensure(nvol);
+    b.kept.ensure(nvol);
+    b.mode.ensure(nvol);
+    b.hact.ensure(nvol);
     b.pidx.ensure((size_t)nvol * cap);
     b.pval.ensure((size_t)nvol * cap);
     // the host lists start at 2048 entries per volume (grown by the download when a chunk holds more)
-    b.hhead.ensure((size_t)2 * nvol + 1);
+    b.hhead.ensure((size_t)3 * nvol + 1);
     b.hpidx.ensure((size_t)nvol * std::min(cap, 2048));
     b.hpval.ensure((size_t)nvol * std::min(cap, 2048));
     return b;
@@ -109,8 +120,15 @@ So3State& so3_state(ettk_plan* P);
 void launch_combine(const double* C, const ComboOp* d_ops, int np, int B, int b0, double* out, hipStream_t s);
 void launch_localmax(const double* C, int np, int B, int cap, int* count, int* idx, double* val, hipStream_t s);
 // packs the np lists (at most `cap` entries each) back to back: head = [offsets (np + 1), counts (np)]
-void launch_lm_compact(const int* count, const int* idx, const double* val, int np, int cap, int* head, int* pidx,
-                       double* pval, hipStream_t s);
+void launch_lm_compact(const int* count, const int* idx, const double* val, const int* kept, const int* mode, int np,
+                       int cap, int* head, int* pidx, double* pval, hipStream_t s);
+// candidates of each active volume on the device (cand.hip): sorted, valid and mutually non-equivalent maxima
+// written over the volume's list (kept[v], mode[v] = 0), or mode[v] = 1 where the host extracts from the raw list
+// (active[v]: 0 unused, 1 device, 2 host)
+constexpr int kCandMax = 4096;  // maxima per volume the device extracts from (more: the host does)
+void launch_lm_candidates(const int* count, int* idx, double* val, int cap, const int* active, int np,
+                          const double* tabs, int n, double trans_restrict, double rot_restrict, float tol,
+                          double* mats, int* adj, int* kept, int* mode, hipStream_t s);
 void launch_ppmax(const float* Pp1, const float* Pp2, int np, int B, float* Mx, hipStream_t s);
 
 }  // namespace ettk

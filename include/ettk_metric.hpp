@@ -191,25 +191,67 @@ class CorrelationImageMetric {
             std::vector<ettk_result> res(todo.size());
             check(ettk_distances_batch(plan_, base, (int)todo.size(), descs.data(), type, form_, res.data()),
                   "ettk_distances_batch");
-            for (size_t q = 0; q < todo.size(); q++) {
-                const int j = todo[q];
-                const ettk_result& r = res[q];
-                if (K == 1) {
-                    D(i, j, 0, 0) = r.distance;
-                    if (off > 1) D(i, j, 1, 0) = 1.0f;        // getCorrelationScale: 1 for these metrics
-                    if (off > 2) D(i, j, 2, 0) = r.overlap;   // getWedgeOverlap(getTransform())
-                    for (int c = 0; c < 16; c++) D(i, j, off + c, 0) = (float)r.matrix[c];
-                } else {
-                    for (int k = 0; k < K; k++) D(i, j, 0, k) = std::numeric_limits<float>::max();
-                    for (int k = 0; k < std::min(K, r.n_cand); k++) {
-                        D(i, j, 0, k) = r.cand_distance[k];
-                        if (off > 1) D(i, j, 1, k) = 1.0f;    // candidate scales and overlaps are stored
-                        if (off > 2) D(i, j, 2, k) = 1.0f;    //   as 1 (ProjRot.h:879-880, FIM.h:1872-1876)
-                        for (int c = 0; c < 16; c++) D(i, j, off + c, k) = (float)r.cand_matrix[k][c];
-                    }
-                }
-            }
+            for (size_t q = 0; q < todo.size(); q++) store_row_(D, i, todo[q], res[q], K, off);
         }
+    }
+
+    /* getDistances(list, D, type) (CorrMetric.h:570-612): the symmetric matrix of a list of sub-volumes.  The
+       entries (i, j), i < j, whose D(i, j, 0, 0) == -1 are computed with list[i] as the reference (input1) and
+       list[j] as the moving volume (input2); makeDistanceMatrix then completes the matrix.  The reference side is
+       list[i]'s volume as stored with its wedge at its alignment; the reference reslices input1 through its
+       alignment, so the entries used as references must hold the identity (as sub-volumes do before any
+       alignment), otherwise MetricError(ETTK_ERR_UNSUPPORTED).  D is n x n x depth x K with depth 19 or 20
+       (distance, correlation scale, wedge overlap, M16: the layout makeDistanceMatrix reads). */
+    void getDistances(std::vector<WedgedSubImage3D>& list, DistanceArray& D, int type) {
+        const int n = (int)list.size();
+        if (D.rows() != n || D.cols() != n) throw MetricError(ETTK_ERR_ARG, "getDistances: D must be n x n");
+        if (D.depth() < 19 || D.depth() > 20)
+            throw MetricError(ETTK_ERR_ARG, "getDistances: the symmetric matrix holds rows of 19 or 20 columns");
+        const int K = D.extent3();
+        if (K < 1 || K > ETTK_MAX_CANDIDATES) throw MetricError(ETTK_ERR_ARG, "getDistances: bad candidate count");
+        if (type != 1 && type != 2 && K != p_.n_candidates)
+            throw MetricError(ETTK_ERR_ARG, "getDistances: D's last extent must equal the number of candidates");
+        ensure_plan_();
+        for (int i = 0; i < n; i++) {
+            std::vector<int> todo;
+            for (int j = i + 1; j < n; j++)
+                if (D(i, j, 0, 0) == -1.0f) todo.push_back(j);
+            if (todo.empty()) continue;
+            static const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+            if (std::memcmp(list[i].desc.matrix, I, sizeof(I)) != 0)
+                throw MetricError(ETTK_ERR_UNSUPPORTED,
+                                  "getDistances: a sub-volume used as the reference must hold the identity alignment");
+            check(ettk_reference_set(plan_, list[i].d_volume, &list[i].desc), "ettk_reference_set");
+            std::vector<ettk_particle> descs;
+            const float* base = gather_(list, todo, descs);
+            std::vector<ettk_result> res(todo.size());
+            check(ettk_distances_batch(plan_, base, (int)todo.size(), descs.data(), type, form_, res.data()),
+                  "ettk_distances_batch");
+            for (size_t q = 0; q < todo.size(); q++) store_row_(D, i, todo[q], res[q], K, 3);
+        }
+        makeDistanceMatrix(D);
+    }
+
+    /* makeDistanceMatrix (CorrMetric.h:2291-2328): (j, i) receives the distance, correlation scale and overlap
+       of (i, j) and the inverse of each of its candidate transforms; the diagonal is 0 with the identity. */
+    static void makeDistanceMatrix(DistanceArray& D) {
+        if (D.rows() != D.cols() || D.depth() < 19)
+            throw MetricError(ETTK_ERR_ARG, "makeDistanceMatrix: D must be n x n with at least 19 columns");
+        const int n = D.rows(), K = D.extent3();
+        for (int i = 0; i < n; i++)
+            for (int j = i + 1; j < n; j++)
+                for (int k = 0; k < K; k++) {
+                    for (int c = 0; c < 3; c++) D(j, i, c, k) = D(i, j, c, k);
+                    double m[16], inv[16];
+                    for (int c = 0; c < 16; c++) m[c] = D(i, j, 3 + c, k);
+                    invert4(m, inv);
+                    for (int c = 0; c < 16; c++) D(j, i, 3 + c, k) = (float)inv[c];
+                }
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < K; k++) {
+                for (int c = 0; c < D.depth(); c++) D(i, i, c, k) = 0.0f;
+                for (int c = 0; c < 16; c++) D(i, i, 3 + c, k) = (c % 5 == 0) ? 1.0f : 0.0f;
+            }
     }
 
     // ---------------------------------------------------------------- averaging
@@ -328,6 +370,47 @@ class CorrelationImageMetric {
     }
 
    private:
+    // one result into D(i, j, ., .): the layout of CorrMetric.h:1966-2000 (distance, [correlation scale],
+    // [wedge overlap], M16 from column `off`)
+    static void store_row_(DistanceArray& D, int i, int j, const ettk_result& r, int K, int off) {
+        if (K == 1) {
+            D(i, j, 0, 0) = r.distance;
+            if (off > 1) D(i, j, 1, 0) = 1.0f;        // getCorrelationScale: 1 for these metrics
+            if (off > 2) D(i, j, 2, 0) = r.overlap;   // getWedgeOverlap(getTransform())
+            for (int c = 0; c < 16; c++) D(i, j, off + c, 0) = (float)r.matrix[c];
+        } else {
+            for (int k = 0; k < K; k++) D(i, j, 0, k) = std::numeric_limits<float>::max();
+            for (int k = 0; k < std::min(K, r.n_cand); k++) {
+                D(i, j, 0, k) = r.cand_distance[k];
+                if (off > 1) D(i, j, 1, k) = 1.0f;    // candidate scales and overlaps are stored
+                if (off > 2) D(i, j, 2, k) = 1.0f;    //   as 1 (ProjRot.h:879-880, FIM.h:1872-1876)
+                for (int c = 0; c < 16; c++) D(i, j, off + c, k) = (float)r.cand_matrix[k][c];
+            }
+        }
+    }
+    // inverse of a row-major 4x4 matrix by cofactors (a singular matrix gives zeros, as vtkMatrix4x4::Invert
+    // leaves its output unchanged at determinant 0)
+    static void invert4(const double* a, double* out) {
+        double c[16];
+        c[0] = a[5] * a[10] * a[15] - a[5] * a[11] * a[14] - a[9] * a[6] * a[15] + a[9] * a[7] * a[14] + a[13] * a[6] * a[11] - a[13] * a[7] * a[10];
+        c[4] = -a[4] * a[10] * a[15] + a[4] * a[11] * a[14] + a[8] * a[6] * a[15] - a[8] * a[7] * a[14] - a[12] * a[6] * a[11] + a[12] * a[7] * a[10];
+        c[8] = a[4] * a[9] * a[15] - a[4] * a[11] * a[13] - a[8] * a[5] * a[15] + a[8] * a[7] * a[13] + a[12] * a[5] * a[11] - a[12] * a[7] * a[9];
+        c[12] = -a[4] * a[9] * a[14] + a[4] * a[10] * a[13] + a[8] * a[5] * a[14] - a[8] * a[6] * a[13] - a[12] * a[5] * a[10] + a[12] * a[6] * a[9];
+        c[1] = -a[1] * a[10] * a[15] + a[1] * a[11] * a[14] + a[9] * a[2] * a[15] - a[9] * a[3] * a[14] - a[13] * a[2] * a[11] + a[13] * a[3] * a[10];
+        c[5] = a[0] * a[10] * a[15] - a[0] * a[11] * a[14] - a[8] * a[2] * a[15] + a[8] * a[3] * a[14] + a[12] * a[2] * a[11] - a[12] * a[3] * a[10];
+        c[9] = -a[0] * a[9] * a[15] + a[0] * a[11] * a[13] + a[8] * a[1] * a[15] - a[8] * a[3] * a[13] - a[12] * a[1] * a[11] + a[12] * a[3] * a[9];
+        c[13] = a[0] * a[9] * a[14] - a[0] * a[10] * a[13] - a[8] * a[1] * a[14] + a[8] * a[2] * a[13] + a[12] * a[1] * a[10] - a[12] * a[2] * a[9];
+        c[2] = a[1] * a[6] * a[15] - a[1] * a[7] * a[14] - a[5] * a[2] * a[15] + a[5] * a[3] * a[14] + a[13] * a[2] * a[7] - a[13] * a[3] * a[6];
+        c[6] = -a[0] * a[6] * a[15] + a[0] * a[7] * a[14] + a[4] * a[2] * a[15] - a[4] * a[3] * a[14] - a[12] * a[2] * a[7] + a[12] * a[3] * a[6];
+        c[10] = a[0] * a[5] * a[15] - a[0] * a[7] * a[13] - a[4] * a[1] * a[15] + a[4] * a[3] * a[13] + a[12] * a[1] * a[7] - a[12] * a[3] * a[5];
+        c[14] = -a[0] * a[5] * a[14] + a[0] * a[6] * a[13] + a[4] * a[1] * a[14] - a[4] * a[2] * a[13] - a[12] * a[1] * a[6] + a[12] * a[2] * a[5];
+        c[3] = -a[1] * a[6] * a[11] + a[1] * a[7] * a[10] + a[5] * a[2] * a[11] - a[5] * a[3] * a[10] - a[9] * a[2] * a[7] + a[9] * a[3] * a[6];
+        c[7] = a[0] * a[6] * a[11] - a[0] * a[7] * a[10] - a[4] * a[2] * a[11] + a[4] * a[3] * a[10] + a[8] * a[2] * a[7] - a[8] * a[3] * a[6];
+        c[11] = -a[0] * a[5] * a[11] + a[0] * a[7] * a[9] + a[4] * a[1] * a[11] - a[4] * a[3] * a[9] - a[8] * a[1] * a[7] + a[8] * a[3] * a[5];
+        c[15] = a[0] * a[5] * a[10] - a[0] * a[6] * a[9] - a[4] * a[1] * a[10] + a[4] * a[2] * a[9] + a[8] * a[1] * a[6] - a[8] * a[2] * a[5];
+        const double det = a[0] * c[0] + a[1] * c[4] + a[2] * c[8] + a[3] * c[12];
+        for (int q = 0; q < 16; q++) out[q] = det == 0.0 ? 0.0 : c[q] / det;
+    }
     template <class T>
     void set_(T& field, T v, bool restriction = false) {
         if (field == v) return;

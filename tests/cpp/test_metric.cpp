@@ -1,5 +1,6 @@
 // Drives the C++ drop-in (include/ettk_metric.hpp) the way MPI_Classification.cpp and nbfLoopClustering.h
-// drive nbfCorrelationImageMetric: getDistances with resume entries, getImage, multi-reference assignment.
+// drive nbfCorrelationImageMetric: getDistances with resume entries, getImage, multi-reference assignment, the
+// symmetric sub-volume matrix.
 // Usage: test_metric <dir>; reads dir/in.txt and the raw float32 volumes, writes dir/out_*.txt / .bin.
 #include <cstdio>
 #include <fstream>
@@ -105,6 +106,18 @@ int main(int argc, char** argv) {
             for (int r = 0; r < R; r++)
                 for (int i = 0; i < n; i++)
                     for (int c = 0; c < 17; c++) o << A(r, i, c, 0) << (c < 16 ? " " : "\n");
+        }
+        // the symmetric sub-volume matrix of the search without refinement (type 3), 19-column rows, one entry
+        // resumed; makeDistanceMatrix completes the lower triangle and the diagonal
+        {
+            ettk::DistanceArray S(n, n, 19, 1);
+            S(0, 2, 0, 0) = 7.0f;
+            metric.getDistances(vols, S, 3);
+            std::ofstream o(dir + "/out_sym.txt");
+            o.precision(9);
+            for (int i = 0; i < n; i++)
+                for (int j = 0; j < n; j++)
+                    for (int c = 0; c < 19; c++) o << S(i, j, c, 0) << (c < 18 ? " " : "\n");
         }
         (void)hipFree(d_refs);
         (void)hipFree(d_parts);

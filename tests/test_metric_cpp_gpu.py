@@ -1,6 +1,6 @@
 """The C++ drop-in (include/ettk_metric.hpp) driven like nbfCorrelationImageMetric by its callers:
-getDistances with a resumed entry, getImage and the multi-reference assignment, checked against the same
-operations through the Python binding of the C ABI."""
+getDistances with a resumed entry, getImage, the multi-reference assignment and the symmetric sub-volume matrix
+(makeDistanceMatrix), checked against the same operations through the Python binding of the C ABI."""
 import subprocess
 from pathlib import Path
 
@@ -97,5 +97,24 @@ def test_cpp_metric_drop_in(torch_gpu, ctx, tmp_path):
         assert np.array_equal(best, np.argmin(sums, axis=1)), (best, sums)
         for i in range(n):
             assert A[best[i], i, 0] < 0
+
+        # symmetric matrix (type 3): entry (i, j) with volume i as the reference, (j, i) its inverse transform
+        S = np.loadtxt(tmp_path / "out_sym.txt").astype(np.float32).reshape(n, n, 19)
+        plan.set_restrictions(180.0, 8.0)
+        for i in range(n):
+            assert np.all(S[i, i, :3] == 0.0) and np.array_equal(S[i, i, 3:].reshape(4, 4), np.eye(4, dtype=np.float32))
+            plan.set_reference(to_dev(torch_gpu, vols[i]), descs[i])
+            rr = plan.distances(d, descs, type=3, form=1)
+            for j in range(i + 1, n):
+                if (i, j) == (0, 2):  # resumed entry: kept
+                    assert S[i, j, 0] == 7.0 and np.all(S[i, j, 1:] == -1.0)
+                    assert np.array_equal(S[j, i, :3], S[i, j, :3])
+                    continue
+                assert S[i, j, 0] == np.float32(rr[j].distance) and S[i, j, 1] == 1.0, (i, j)
+                assert S[i, j, 2] == np.float32(rr[j].overlap), (i, j)
+                assert np.array_equal(S[i, j, 3:], rr[j].transform().ravel().astype(np.float32)), (i, j)
+                assert np.array_equal(S[j, i, :3], S[i, j, :3])
+                inv = np.linalg.inv(S[i, j, 3:].astype(np.float64).reshape(4, 4))
+                assert np.allclose(S[j, i, 3:].reshape(4, 4), inv, rtol=0, atol=1e-5), (i, j)
     finally:
         plan.close()
