@@ -18,6 +18,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 in any kernel
 HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# align.cpp runs the Powell refinements as C++20 coroutines
+STD_OVERRIDE = {"align.cpp": "-std=c++20"}
 SOURCES = ["fft.hip", "ccc.hip", "so3.hip", "refine.hip", "host.cpp", "align.cpp", "refine.cpp", "avg.hip", "avg.cpp",
            "noalloc.hip", "extract.hip", "classify.cpp", "tune.cpp", "cand.hip"]
 
@@ -29,7 +31,8 @@ def build_lib(verbose: bool = False) -> Path:
     def compile_one(src):
         obj = CSRC / ("_" + src + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmd = [HIPCC, *HIP_FLAGS, *lang, "-c", str(CSRC / src), "-o", str(obj)]
+        flags = [STD_OVERRIDE.get(src, x) if x == "-std=c++17" else x for x in HIP_FLAGS]
+        cmd = [HIPCC, *flags, *lang, "-c", str(CSRC / src), "-o", str(obj)]
         return src, obj, subprocess.run(cmd, capture_output=True, text=True)
 
     try:

@@ -3,15 +3,16 @@
 // CCC test of the candidate peaks and the Powell refinement of the best candidates.
 //
 // Modes: 0 = plain (particle after particle, one CCC evaluation per launch, the reference's control flow);
-//        1 = batched (every stage over all particles; the per-particle control flow runs in one host thread per
-//            particle and every CCC request of all threads is served by one batched launch);
+//        1 = batched (every stage over all particles; the Powell refinements of all particles run as coroutines
+//            on the calling thread, and every round of their pending CCC requests is served by one batched launch);
 //        2 = batched + adjoint refinement (refine.cpp).
 #include <algorithm>
 #include <cstring>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
-#include <condition_variable>
+#include <coroutine>
+#include <exception>
 #include <functional>
 #include <map>
 #include <mutex>
@@ -221,15 +222,34 @@ static const ZyzTables& zyz_tables(int B) {
     return cache.emplace(B, std::move(z)).first->second;
 }
 
-// the survivors of the device extraction (k_lm_candidates): valid, mutually non-equivalent maxima in index order,
-// pushed as candidates_from_maxima pushes its survivors
-static void candidates_push(const ettk_plan* P, const std::vector<int>& idx, const std::vector<double>& val,
-                            CandQueue& pq) {
+// The elimination of equivalent angles on the device's output (k_lm_candidates: the valid maxima in index order,
+// deg[i] partners j > i of each, their indices row after row in adj): the rows in order, each row's partners in
+// increasing j, the worse of two alive partners dropped (j on a tie) -- the decisions of eliminate_equivalent, whose
+// visits (i, j) with j < i find one end dead -- then the survivors pushed in index order as there.
+static void candidates_select(const ettk_plan* P, const std::vector<int>& idx, const std::vector<double>& val,
+                              const std::vector<int>& deg, const std::vector<int>& adj, CandQueue& pq) {
     const int B = P->p.B, n = 2 * B;
     const ZyzTables& Z = zyz_tables(B);
-    for (size_t k = 0; k < idx.size(); k++) {
+    const size_t nv = idx.size();
+    std::vector<float> v(nv);
+    std::vector<char> alive(nv, 1);
+    for (size_t k = 0; k < nv; k++) v[k] = (float)(-val[k]);
+    size_t o = 0;
+    for (size_t i = 0; i < nv; i++) {
+        const size_t d = (size_t)deg[i];
+        for (size_t k = 0; k < d && alive[i]; k++) {
+            const int j = adj[o + k];
+            if (alive[j]) {
+                if (v[i] > v[j]) alive[i] = 0;
+                else alive[j] = 0;
+            }
+        }
+        o += d;
+    }
+    for (size_t k = 0; k < nv; k++) {
+        if (!alive[k]) continue;
         const int t = idx[k], a = t / (n * n), b = (t / n) % n, g = t % n;
-        pq.push(Cand{Z.alpha[a], Z.beta[b], Z.gamma[g], (float)(-val[k])});
+        pq.push(Cand{Z.alpha[a], Z.beta[b], Z.gamma[g], v[k]});
     }
 }
 
@@ -313,8 +333,9 @@ struct SearchLists {
     std::vector<SoftInst> inst;
     std::vector<ComboOp> ops;
     LmBuffers* lm = nullptr;
-    hipEvent_t done = nullptr;
+    hipEvent_t maxima = nullptr, done = nullptr;  // local maxima found (main stream); lists packed (aux stream)
     ~SearchLists() {
+        if (maxima) (void)hipEventDestroy(maxima);
         if (done) (void)hipEventDestroy(done);
     }
     std::chrono::steady_clock::time_point t_ready;  // when its kernels had completed
@@ -409,6 +430,11 @@ static void search_gpu(ettk_plan* P, const float* d_parts, const ettk_particle* 
         ProfScope ps(kp, "k_localmax", 2.0 * cn * n3 * 8, s);
         launch_localmax(S.Cc.p, 2 * cn, B, lm_cap(P), L.lm->count.p, L.lm->idx.p, L.lm->val.p, s);
     }
+    // the candidate extraction and the packing run on the plan's auxiliary stream, beside the next chunk's kernels
+    ETTK_HIP(hipEventCreateWithFlags(&L.maxima, hipEventDisableTiming));
+    ETTK_HIP(hipEventRecord(L.maxima, s));
+    hipStream_t as = P->aux_stream;
+    ETTK_HIP(hipStreamWaitEvent(as, L.maxima, 0));
     {
         // candidate extraction of the active volumes on the device (the second volume of a particle is used only
         // with a spherical window)
@@ -417,7 +443,7 @@ static void search_gpu(ettk_plan* P, const float* d_parts, const ettk_particle* 
         const char* hc = std::getenv("ETTK_HOST_CANDIDATES");
         const int act = hc && hc[0] == '1' ? 2 : 1;
         for (int v = 0; v < 2 * cn; v++) lb.hact.p[v] = ((v % 2 == 0) || uw1 || L.uw2[v / 2]) ? act : 0;
-        ETTK_HIP(hipMemcpyAsync(lb.act.p, lb.hact.p, (size_t)2 * cn * sizeof(int), hipMemcpyHostToDevice, s));
+        ETTK_HIP(hipMemcpyAsync(lb.act.p, lb.hact.p, (size_t)2 * cn * sizeof(int), hipMemcpyHostToDevice, as));
         if (S.zyz_tabs.n == 0) {
             const ZyzTables& Z = zyz_tables(B);
             std::vector<double> t((size_t)3 * n * 16);
@@ -430,16 +456,16 @@ static void search_gpu(ettk_plan* P, const float* d_parts, const ettk_particle* 
             ETTK_HIP(hipMemcpy(S.zyz_tabs.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
         }
         S.cand_mats.ensure((size_t)2 * cn * kCandMax * 16);
-        S.cand_adj.ensure((size_t)2 * cn * kCandMax * 64);
-        ProfScope ps(kp, "k_lm_candidates", 0.0, s);
+        ProfScope ps(kp, "k_lm_candidates", 0.0, as);
         launch_lm_candidates(lb.count.p, lb.idx.p, lb.val.p, lm_cap(P), lb.act.p, 2 * cn, S.zyz_tabs.p, n,
-                             (double)P->trans_restrict, (double)P->rot_restrict, kEquivTol, S.cand_mats.p,
-                             S.cand_adj.p, lb.kept.p, lb.mode.p, s);
+                             (double)P->trans_restrict, (double)P->rot_restrict, kEquivTol, S.cand_mats.p, lb.deg.p,
+                             lb.adj.p, lb.kept.p, lb.nadj.p, lb.mode.p, as);
     }
-    launch_lm_compact(L.lm->count.p, L.lm->idx.p, L.lm->val.p, L.lm->kept.p, L.lm->mode.p, 2 * cn, lm_cap(P),
-                      L.lm->head.p, L.lm->pidx.p, L.lm->pval.p, s);
+    launch_lm_compact(L.lm->count.p, L.lm->idx.p, L.lm->val.p, L.lm->kept.p, L.lm->mode.p, L.lm->deg.p, L.lm->nadj.p,
+                      L.lm->adj.p, 2 * cn, lm_cap(P), L.lm->head.p, L.lm->pidx.p, L.lm->pval.p, L.lm->pdeg.p,
+                      L.lm->padj.p, as);
     ETTK_HIP(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
-    ETTK_HIP(hipEventRecord(L.done, s));
+    ETTK_HIP(hipEventRecord(L.done, as));
 }
 
 // Host part: download the chunk's compacted maxima (after its kernels, on the context's copy stream: the plan's
@@ -455,20 +481,28 @@ static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOu
     hipStream_t cs = P->copy_stream;
     LmBuffers& lb = *L.lm;
     const int* head = lb.hhead.p;
-    ETTK_HIP(hipMemcpyAsync(lb.hhead.p, lb.head.p, (size_t)(3 * nv + 1) * sizeof(int), hipMemcpyDeviceToHost, cs));
+    ETTK_HIP(hipMemcpyAsync(lb.hhead.p, lb.head.p, (size_t)(4 * nv + 2) * sizeof(int), hipMemcpyDeviceToHost, cs));
     ETTK_HIP(hipStreamSynchronize(cs));
-    const int total = head[nv];
+    const int total = head[nv], atotal = head[4 * nv + 1];
     if (total > 0) {
         lb.hpidx.ensure(total);
         lb.hpval.ensure(total);
+        lb.hpdeg.ensure(total);
         ETTK_HIP(hipMemcpyAsync(lb.hpidx.p, lb.pidx.p, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, cs));
         ETTK_HIP(hipMemcpyAsync(lb.hpval.p, lb.pval.p, (size_t)total * sizeof(double), hipMemcpyDeviceToHost, cs));
-        ETTK_HIP(hipStreamSynchronize(cs));
+        ETTK_HIP(hipMemcpyAsync(lb.hpdeg.p, lb.pdeg.p, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, cs));
     }
+    if (atotal > 0) {
+        lb.hpadj.ensure(atotal);
+        ETTK_HIP(hipMemcpyAsync(lb.hpadj.p, lb.padj.p, (size_t)atotal * sizeof(int), hipMemcpyDeviceToHost, cs));
+    }
+    ETTK_HIP(hipStreamSynchronize(cs));
     const int* pidx = lb.hpidx.p;
     const double* pval = lb.hpval.p;
+    const int* pdeg = lb.hpdeg.p;
+    const int* padj = lb.hpadj.p;
     if (kp) kp->add_host("search_download", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th0).count());
-    std::vector<std::vector<int>> idx(nv);
+    std::vector<std::vector<int>> idx(nv), deg(nv), adj(nv);
     std::vector<std::vector<double>> val(nv);
     std::vector<int> mode(nv, 0);
     for (int v = 0; v < nv; v++) {
@@ -479,9 +513,14 @@ static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOu
         ETTK_REQUIRE(raw <= lm_cap(P), ETTK_ERR_UNSUPPORTED,
                      "more SO(3) local maxima than the candidate list holds (" + std::to_string(raw) + ")");
         mode[v] = head[2 * nv + 1 + v];
-        const int c = head[v + 1] - head[v];  // survivors (mode 0) or the raw maxima (mode 1)
+        const int c = head[v + 1] - head[v];  // sorted valid maxima (mode 0) or the raw maxima (mode 1)
         idx[v].assign(pidx + head[v], pidx + head[v] + c);
         val[v].assign(pval + head[v], pval + head[v] + c);
+        if (mode[v] == 0) {
+            deg[v].assign(pdeg + head[v], pdeg + head[v] + c);
+            const int a0 = head[3 * nv + 1 + v], a1 = head[3 * nv + 2 + v];
+            adj[v].assign(padj + a0, padj + a1);
+        }
         if (kp && kp->on) {
             kp->add_host("lm_count", (double)raw);
             if (mode[v]) kp->add_host("lm_host_volumes", 1.0);
@@ -498,7 +537,7 @@ static void search_host(const ettk_plan* P, SearchLists& L, std::vector<SearchOu
         SearchOut& o = out[p0 + q];
         CandQueue& pq = v % 2 == 0 ? o.pq1 : o.pq2;
         if (mode[v] == 0) {
-            candidates_push(P, idx[v], val[v], pq);  // extracted on the device
+            candidates_select(P, idx[v], val[v], deg[v], adj[v], pq);  // sorted, checked and paired on the device
             return;
         }
         std::vector<std::pair<int, double>> lm(idx[v].size());
@@ -594,13 +633,136 @@ static void search_all(ettk_plan* P, const float* d_parts, const ettk_particle* 
     }
 }
 
+// ------------------------------------------------------------------ evaluations awaited by coroutines
+// The refinement of a particle is a coroutine that suspends at every CCC evaluation it needs.  An evaluation is
+// served either at once (a sink with `direct` set: the plain form, or results computed beforehand) or queued:
+// the driver then evaluates every queued request of all particles in one batched launch and resumes them, so
+// the particles advance in lock-step rounds without a host thread each.
+namespace {
+struct EvalReq {
+    int pi;
+    Mat4* t;  // the transform; receives the peak's shift as ccc_batch returns it
+    float out;
+    std::coroutine_handle<> h;
+};
+struct EvalSink {
+    std::function<float(int, Mat4&)> direct;  // set: evaluate at once
+    std::vector<EvalReq*> pending;
+};
+struct EvalAwait {
+    EvalSink* sink;
+    EvalReq r;
+    bool await_ready() {
+        if (!sink->direct) return false;
+        r.out = sink->direct(r.pi, *r.t);
+        return true;
+    }
+    void await_suspend(std::coroutine_handle<> h) {
+        r.h = h;
+        sink->pending.push_back(&r);
+    }
+    float await_resume() const noexcept { return r.out; }
+};
+struct Evaluator {
+    EvalSink* sink;
+    int* count;  // evaluations made, or null
+    EvalAwait operator()(int pi, Mat4& t) const {
+        if (count) (*count)++;
+        return EvalAwait{sink, EvalReq{pi, &t, 0.0f, {}}};
+    }
+};
+
+// A lazily started coroutine returning T; awaiting it runs it and resumes the awaiting coroutine when it is done
+// (symmetric transfer), and rethrows its exception there.
+template <class T>
+struct TaskPromiseBase {
+    std::exception_ptr err;
+    std::coroutine_handle<> cont;
+    std::suspend_always initial_suspend() noexcept { return {}; }
+    template <class P>
+    struct Final {
+        bool await_ready() noexcept { return false; }
+        std::coroutine_handle<> await_suspend(std::coroutine_handle<P> h) noexcept {
+            const std::coroutine_handle<> c = h.promise().cont;
+            return c ? c : std::noop_coroutine();
+        }
+        void await_resume() noexcept {}
+    };
+    void unhandled_exception() { err = std::current_exception(); }
+};
+template <class T>
+class Task {
+   public:
+    struct promise_type : TaskPromiseBase<T> {
+        T value{};
+        Task get_return_object() { return Task(std::coroutine_handle<promise_type>::from_promise(*this)); }
+        typename TaskPromiseBase<T>::template Final<promise_type> final_suspend() noexcept { return {}; }
+        void return_value(T v) { value = std::move(v); }
+    };
+    explicit Task(std::coroutine_handle<promise_type> h) : h_(h) {}
+    Task(Task&& o) noexcept : h_(std::exchange(o.h_, {})) {}
+    Task(const Task&) = delete;
+    ~Task() {
+        if (h_) h_.destroy();
+    }
+    bool await_ready() const noexcept { return false; }
+    std::coroutine_handle<> await_suspend(std::coroutine_handle<> c) noexcept {
+        h_.promise().cont = c;
+        return h_;
+    }
+    T await_resume() {
+        if (h_.promise().err) std::rethrow_exception(h_.promise().err);
+        return std::move(h_.promise().value);
+    }
+    void start() { h_.resume(); }
+    bool done() const { return h_.done(); }
+    void rethrow() const {
+        if (h_.promise().err) std::rethrow_exception(h_.promise().err);
+    }
+
+   private:
+    std::coroutine_handle<promise_type> h_;
+};
+template <>
+class Task<void> {
+   public:
+    struct promise_type : TaskPromiseBase<void> {
+        Task get_return_object() { return Task(std::coroutine_handle<promise_type>::from_promise(*this)); }
+        TaskPromiseBase<void>::Final<promise_type> final_suspend() noexcept { return {}; }
+        void return_void() {}
+    };
+    explicit Task(std::coroutine_handle<promise_type> h) : h_(h) {}
+    Task(Task&& o) noexcept : h_(std::exchange(o.h_, {})) {}
+    Task(const Task&) = delete;
+    ~Task() {
+        if (h_) h_.destroy();
+    }
+    bool await_ready() const noexcept { return false; }
+    std::coroutine_handle<> await_suspend(std::coroutine_handle<> c) noexcept {
+        h_.promise().cont = c;
+        return h_;
+    }
+    void await_resume() {
+        if (h_.promise().err) std::rethrow_exception(h_.promise().err);
+    }
+    void start() { h_.resume(); }
+    bool done() const { return h_.done(); }
+    void rethrow() const {
+        if (h_.promise().err) std::rethrow_exception(h_.promise().err);
+    }
+
+   private:
+    std::coroutine_handle<promise_type> h_;
+};
+}  // namespace
+
 // ------------------------------------------------------------------ Powell (vnl_powell, maxfev = 1)
 // Restatement of vnl_powell::minimize (ITK 4.2.1 vnl/algo/vnl_powell.cxx:60-175) with
 // vnl_bracket_minimum and vnl_brent_minimizer::minimize_given_bounds, as nbfFourierImageMetric::refine
 // configures it (FIM.h:1742-1753): initial step 5, line-search x tolerance 1e-2, f tolerance 0.1.
 namespace {
 struct PowellState {
-    std::function<double(const double*)> f;
+    std::function<Task<double>(const double*)> f;
     double end_error = 0;
     long nevals = 0;
     void report(double e) {
@@ -613,20 +775,20 @@ const double GOLD = 1.618033988749894848, CGOLD = 0.381966011250105152;
 struct Line {
     PowellState* S;
     double x0[3], dx[3];
-    double operator()(double lam) {
+    Task<double> operator()(double lam) {
         double x[3] = {x0[0] + lam * dx[0], x0[1] + lam * dx[1], x0[2] + lam * dx[2]};
-        double e = S->f(x);
+        double e = co_await S->f(x);
         S->report(e);
-        return e;
+        co_return e;
     }
 };
-void bracket(Line& f, double& a, double& b, double& c) {
+Task<void> bracket(Line& f, double& a, double& b, double& c) {
     const double EPS = 1e-7, EPSqr = 1e-14;
     if (b == a) b = a + 1.0;
-    double fa = f(a), fb = f(b);
+    double fa = co_await f(a), fb = co_await f(b);
     if (fb > fa) { std::swap(a, b); std::swap(fa, fb); }
     c = b + GOLD * (b - a);
-    double fc = f(c);
+    double fc = co_await f(c);
     while (fc < fb) {
         double x1 = a - b, f1 = fa - fb, x2 = c - b, f2 = fc - fb;
         double p = x2 * x2 * f1 - x1 * x1 * f2, q = 2 * (x2 * f1 - x1 * f2);
@@ -640,40 +802,40 @@ void bracket(Line& f, double& a, double& b, double& c) {
         else if ((c - u) < tol && (c - u) >= 0) u -= tol;
         double ulim = b + 100 * (c - b), fu = 0.0;
         if ((u - b) * (c - u) > 0.0) {
-            fu = f(u);
+            fu = co_await f(u);
             if (fu < fc) {
                 a = b; fa = fb; b = u; fb = fu;
                 if (a > c) { std::swap(a, c); std::swap(fa, fc); }
-                return;
+                co_return;
             } else if (fu > fb) {
                 c = u; fc = fu;
                 if (a > c) { std::swap(a, c); std::swap(fa, fc); }
-                return;
+                co_return;
             }
             u = c + GOLD * (c - b);
-            fu = f(u);
+            fu = co_await f(u);
         } else if ((u - c) * (ulim - u) > 0.0) {
-            fu = f(u);
+            fu = co_await f(u);
             if (fu > fc) {
                 a = b; fa = fb; b = c; fb = fc; c = u; fc = fu;
                 if (a > c) { std::swap(a, c); std::swap(fa, fc); }
-                return;
+                co_return;
             }
         } else if ((ulim - c) * (u - ulim) >= 0) {
             u = ulim;
-            fu = f(u);
+            fu = co_await f(u);
         } else {
             u = c + GOLD * (c - b);
-            fu = f(u);
+            fu = co_await f(u);
         }
         a = b; b = c; c = u;
         fa = fb; fb = fc; fc = fu;
     }
     if (a > c) { std::swap(a, c); std::swap(fa, fc); }
 }
-double brent(Line& f, double ax, double bx, double cx, double xtol, double& fmin) {
+Task<double> brent(Line& f, double ax, double bx, double cx, double xtol, double& fmin) {
     const double EPS = 1e-8;
-    double fb = f(bx);
+    double fb = co_await f(bx);
     double x = bx, w = x, v = w, u, fx = fb, fw = fx, fv = fw, fu;
     double m = 0.5 * (ax + cx), tol = EPS * std::fabs(x) + xtol, tol2 = 2 * tol, d = 0.0, e = 0.0;
     while (std::fabs(x - m) > (tol2 - 0.5 * (cx - ax))) {
@@ -696,7 +858,7 @@ double brent(Line& f, double ax, double bx, double cx, double xtol, double& fmin
         }
         if (std::fabs(d) >= tol) u = x + d;
         else u = (d > 0) ? x + tol : x - tol;
-        fu = f(u);
+        fu = co_await f(u);
         if (fu <= fx) {
             if (u < x) cx = x; else ax = x;
             v = w; fv = fw; w = x; fw = fx; x = u; fx = fu;
@@ -710,12 +872,12 @@ double brent(Line& f, double ax, double bx, double cx, double xtol, double& fmin
         tol2 = 2 * tol;
     }
     fmin = fx;
-    return x;
+    co_return x;
 }
-void powell(PowellState& S, double p[3], double step, double lxtol, double ftol) {
+Task<void> powell(PowellState& S, double p[3], double step, double lxtol, double ftol) {
     const int n = 3;
     double xi[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, ptt[3], xit[3], pt[3];
-    double fret = S.f(p);
+    double fret = co_await S.f(p);
     S.report(fret);
     for (int j = 0; j < n; j++) pt[j] = p[j];
     for (int iter = 0; iter < 1; iter++) {
@@ -726,27 +888,27 @@ void powell(PowellState& S, double p[3], double step, double lxtol, double ftol)
             double fptt = fret;
             Line L{&S, {p[0], p[1], p[2]}, {xit[0], xit[1], xit[2]}};
             double ax = 0.0, xx = step, bx, fm;
-            bracket(L, ax, xx, bx);
-            xx = brent(L, ax, xx, bx, lxtol, fm);
+            co_await bracket(L, ax, xx, bx);
+            xx = co_await brent(L, ax, xx, bx, lxtol, fm);
             fret = fm;
             for (int j = 0; j < n; j++) p[j] = L.x0[j] + xx * L.dx[j];
             if (std::fabs(fptt - fret) > del) { del = std::fabs(fptt - fret); ibig = i; }
         }
-        if (2.0 * std::fabs(fp - fret) <= ftol * (std::fabs(fp) + std::fabs(fret))) return;
+        if (2.0 * std::fabs(fp - fret) <= ftol * (std::fabs(fp) + std::fabs(fret))) co_return;
         for (int j = 0; j < n; j++) {
             ptt[j] = 2.0 * p[j] - pt[j];
             xit[j] = p[j] - pt[j];
             pt[j] = p[j];
         }
-        double fptt = S.f(ptt);
+        double fptt = co_await S.f(ptt);
         S.report(fret);
         if (fptt < fp) {
             double t = 2.0 * (fp - 2.0 * fret + fptt) * (fp - fret - del) * (fp - fret - del) - del * (fp - fptt) * (fp - fptt);
             if (t < 0.0) {
                 Line L{&S, {p[0], p[1], p[2]}, {xit[0], xit[1], xit[2]}};
                 double ax = 0.0, xx = 1.0, bx, fm;
-                bracket(L, ax, xx, bx);
-                xx = brent(L, ax, xx, bx, lxtol, fm);
+                co_await bracket(L, ax, xx, bx);
+                xx = co_await brent(L, ax, xx, bx, lxtol, fm);
                 fret = fm;
                 for (int j = 0; j < n; j++) p[j] = L.x0[j] + xx * L.dx[j];
                 for (int j = 0; j < n; j++) {
@@ -763,31 +925,31 @@ void powell(PowellState& S, double p[3], double step, double lxtol, double ftol)
 using EvalFn = std::function<float(int particle, Mat4& t)>;
 
 // nbfFourierImageMetric::refine (FIM.h:1702-1809) with f (:129-190)
-static float refine_powell(const ettk_plan* P, int pi, Mat4& t, bool run, const EvalFn& eval) {
+static Task<float> refine_powell(const ettk_plan* P, int pi, Mat4& t, bool run, Evaluator eval) {
     Mat4 seed = strip_translation(t);
     float last = 0.0f;
     auto trial = [&](const double* x) {
         return seed * mat_rot(x[0], 1, 0, 0) * mat_rot(x[1], 0, 1, 0) * mat_rot(x[2], 0, 0, 1);
     };
     PowellState S;
-    S.f = [&](const double* x) -> double {
+    S.f = [&](const double* x) -> Task<double> {
         Mat4 tt = trial(x);
-        if (!transform_ok(tt, P->trans_restrict, P->rot_restrict)) return (double)FLT_MAX;
-        float peak = eval(pi, tt);
+        if (!transform_ok(tt, P->trans_restrict, P->rot_restrict)) co_return (double)FLT_MAX;
+        float peak = co_await eval(pi, tt);
         last = peak;
-        if (!transform_ok(tt, P->trans_restrict, P->rot_restrict)) return (double)FLT_MAX;
-        return (double)peak;
+        if (!transform_ok(tt, P->trans_restrict, P->rot_restrict)) co_return (double)FLT_MAX;
+        co_return (double)peak;
     };
     double x[3] = {0, 0, 0};
-    if (run) powell(S, x, 5.0, 1e-2, 1e-1);
+    if (run) co_await powell(S, x, 5.0, 1e-2, 1e-1);
     if (S.end_error != (double)FLT_MAX) {
         Mat4 vt = trial(x);
-        last = eval(pi, vt);
+        last = co_await eval(pi, vt);
         t = vt;
     } else {
-        last = eval(pi, t);
+        last = co_await eval(pi, t);
     }
-    return last;
+    co_return last;
 }
 
 // candidate loop of nbfProjectionRotationMetric3D::execute (ProjRot.h:678-901), split in two parts:
@@ -831,18 +993,15 @@ static void peak_loop(const ettk_plan* P, int pi, SearchOut& so, const EvalFn& e
     }
 }
 
-static void final_candidates(const ettk_plan* P, int pi, const std::vector<Cand>& selected, SearchOut& so,
-                             const EvalFn& eval, ettk_result& out, int nccc, std::vector<Mat4>* seeds = nullptr) {
-    auto counted = [&](int p, Mat4& t) {
-        nccc++;
-        return eval(p, t);
-    };
+static Task<void> final_candidates(const ettk_plan* P, int pi, const std::vector<Cand>& selected, SearchOut& so,
+                                   EvalSink* sink, ettk_result& out, int nccc, std::vector<Mat4>* seeds = nullptr) {
+    const Evaluator counted{sink, &nccc};
     float best = FLT_MAX;
     Mat4 bestT = Mat4::ident();
     int index = 0;
     for (const Cand& p : selected) {
         Mat4 t = zyz(p.a, p.b, p.g);
-        float dist = refine_powell(P, pi, t, seeds == nullptr && P->p.refine != 0, counted);
+        float dist = co_await refine_powell(P, pi, t, seeds == nullptr && P->p.refine != 0, counted);
         if (seeds) seeds->push_back(t);
         if (dist < best) {
             best = dist;
@@ -862,12 +1021,55 @@ static void final_candidates(const ettk_plan* P, int pi, const std::vector<Cand>
     out.use_window = so.use_window;
 }
 
-static void candidate_stage(const ettk_plan* P, int pi, SearchOut& so, const EvalFn& eval, ettk_result& out,
-                            std::vector<Mat4>* seeds = nullptr) {
+// runs a task whose evaluations are all served at once (a sink with `direct` set): it completes on its first resume
+static void run_direct(Task<void>& task) {
+    task.start();
+    ETTK_REQUIRE(task.done(), ETTK_ERR_STATE, "an evaluation with a direct sink suspended");
+    task.rethrow();
+}
+
+// Runs the tasks in lock-step rounds: each round evaluates every request the tasks have queued in `sink` in one
+// ccc_batch (at most `chunk` evaluations per launch, as ccc_batch splits them) and resumes the requesting tasks.
+static void run_batched(ettk_plan* P, const float* d_parts, const ettk_particle* descs, std::vector<Task<void>>& tasks,
+                        EvalSink& sink) {
+    for (auto& t : tasks) t.start();
+    std::vector<EvalReq*> batch;
+    std::vector<int> pidx;
+    std::vector<Mat4> tr;
+    std::vector<float> d;
+    while (!sink.pending.empty()) {
+        batch.swap(sink.pending);
+        sink.pending.clear();
+        const size_t m = batch.size();
+        pidx.resize(m);
+        tr.resize(m);
+        d.resize(m);
+        for (size_t k = 0; k < m; k++) {
+            pidx[k] = batch[k]->pi;
+            tr[k] = *batch[k]->t;
+        }
+        ccc_batch(P, d_parts, descs, (int)m, pidx.data(), tr.data(), d.data());
+        for (size_t k = 0; k < m; k++) {
+            *batch[k]->t = tr[k];
+            batch[k]->out = d[k];
+        }
+        for (size_t k = 0; k < m; k++) batch[k]->h.resume();  // each runs to its next evaluation or its end
+        batch.clear();
+    }
+    for (auto& t : tasks) {
+        ETTK_REQUIRE(t.done(), ETTK_ERR_STATE, "a refinement task neither finished nor asked for an evaluation");
+        t.rethrow();
+    }
+}
+
+static void candidate_stage(const ettk_plan* P, int pi, SearchOut& so, const EvalFn& eval, ettk_result& out) {
     std::vector<Cand> selected;
     int nccc = 0;
     peak_loop(P, pi, so, eval, selected, nccc);
-    final_candidates(P, pi, selected, so, eval, out, nccc, seeds);
+    EvalSink sink;
+    sink.direct = eval;
+    Task<void> task = final_candidates(P, pi, selected, so, &sink, out, nccc);
+    run_direct(task);
 }
 
 // The peak loop of every particle in lock-step rounds (batched modes).  A queue that still needs `need`
@@ -960,98 +1162,6 @@ static void peak_rounds(ettk_plan* P, const float* d_parts, const ettk_particle*
     }
 }
 
-class BatchEval {
-   public:
-    BatchEval(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n_threads)
-        : P_(P), d_parts_(d_parts), descs_(descs), active_(n_threads) {}
-    float eval(int pi, Mat4& t) {
-        std::unique_lock<std::mutex> lk(mu_);
-        if (failed_) throw err_;
-        float result = 0.f;
-        reqs_.push_back(Req{pi, &t, &result});
-        long my = gen_;
-        if ((int)reqs_.size() == active_) run_locked();
-        else cv_.wait(lk, [&] { return gen_ != my; });
-        if (failed_) throw err_;
-        return result;
-    }
-    void done() {
-        std::unique_lock<std::mutex> lk(mu_);
-        active_--;
-        if (!failed_ && active_ > 0 && (int)reqs_.size() == active_) run_locked();
-    }
-    // a worker's error: every waiting worker wakes and throws it, and the queued requests (which point into the
-    // waiting workers' frames) are dropped before those frames unwind
-    void fail(const Error& e) {
-        std::unique_lock<std::mutex> lk(mu_);
-        if (!failed_) { failed_ = true; err_ = e; }
-        active_--;
-        reqs_.clear();
-        gen_++;
-        cv_.notify_all();
-    }
-    // the body of one worker thread: runs work(), reports its completion or its error
-    template <class F>
-    void worker(F&& work) {
-        try {
-            work();
-            done();
-        } catch (const Error& e) {
-            fail(e);
-        } catch (const std::exception& e) {
-            fail(Error{ETTK_ERR_ALLOC, e.what()});
-        }
-    }
-    // after every worker has joined: the first error, if any, for the caller
-    void rethrow() const {
-        if (failed_) throw err_;
-    }
-    long batches = 0;
-
-   private:
-    struct Req {
-        int pi;
-        Mat4* t;
-        float* out;
-    };
-    void run_locked() {
-        std::vector<int> pidx(reqs_.size());
-        std::vector<Mat4> t(reqs_.size());
-        std::vector<float> d(reqs_.size());
-        for (size_t i = 0; i < reqs_.size(); i++) {
-            pidx[i] = reqs_[i].pi;
-            t[i] = *reqs_[i].t;
-        }
-        try {
-            ccc_batch(P_, d_parts_, descs_, (int)reqs_.size(), pidx.data(), t.data(), d.data());
-            for (size_t i = 0; i < reqs_.size(); i++) {
-                *reqs_[i].t = t[i];
-                *reqs_[i].out = d[i];
-            }
-        } catch (const Error& e) {
-            failed_ = true;
-            err_ = e;
-        } catch (const std::exception& e) {
-            failed_ = true;
-            err_ = Error{ETTK_ERR_ALLOC, e.what()};
-        }
-        batches++;
-        reqs_.clear();
-        gen_++;
-        cv_.notify_all();
-    }
-    ettk_plan* P_;
-    const float* d_parts_;
-    const ettk_particle* descs_;
-    std::mutex mu_;
-    std::condition_variable cv_;
-    int active_;
-    long gen_ = 0;
-    std::vector<Req> reqs_;
-    bool failed_ = false;
-    Error err_{0, ""};
-};
-
 void refine_adjoint_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n,
                           std::vector<SearchOut*>& so, ettk_result* out);
 
@@ -1095,18 +1205,11 @@ void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
             std::vector<std::vector<Cand>> sel;
             std::vector<int> nccc;
             peak_rounds(P, d_parts, descs, n, sp, sel, nccc);
-            int T = std::min(n, 1024);
-            BatchEval be(P, d_parts, descs, T);
-            EvalFn ev = [&](int pi, Mat4& t) { return be.eval(pi, t); };
-            std::vector<std::thread> th;
-            for (int w = 0; w < T; w++)
-                th.emplace_back([&, w] {
-                    be.worker([&] {
-                        for (int i = w; i < n; i += T) final_candidates(P, i, sel[i], so[i], ev, out[i], nccc[i]);
-                    });
-                });
-            for (auto& t : th) t.join();
-            be.rethrow();
+            EvalSink sink;
+            std::vector<Task<void>> tasks;
+            tasks.reserve(n);
+            for (int i = 0; i < n; i++) tasks.push_back(final_candidates(P, i, sel[i], so[i], &sink, out[i], nccc[i]));
+            run_batched(P, d_parts, descs, tasks, sink);
         } else {
             std::vector<SearchOut*> sp(n);
             for (int i = 0; i < n; i++) sp[i] = &so[i];
@@ -1149,13 +1252,15 @@ void refine_adjoint_batch(ettk_plan* P, const float* d_parts, const ettk_particl
         for (int i = 0; i < n; i++) {
             std::vector<size_t> mine;
             for (size_t c = 0; c < sel[i].size(); c++) mine.push_back(r + c);
-            EvalFn ev = [&](int, Mat4& tt) {
+            EvalSink sink;
+            sink.direct = [&](int, Mat4& tt) {
                 const size_t k = mine.front();
                 mine.erase(mine.begin());
                 tt = t[k];
                 return d[k];
             };
-            final_candidates(P, i, sel[i], *so[i], ev, out[i], nccc[i], &seeds[i]);
+            Task<void> task = final_candidates(P, i, sel[i], *so[i], &sink, out[i], nccc[i], &seeds[i]);
+            run_direct(task);
             r += sel[i].size();
         }
     }
@@ -1260,33 +1365,25 @@ void distances_run(ettk_plan* P, const float* d_parts, const ettk_particle* desc
         t = pose;
         for (int i = 0; i < n; i++) nccc[i] = 1;
     } else {
-        auto run_one = [&](int i, const EvalFn& ev) {
-            auto counted = [&](int p, Mat4& tt) {
-                nccc[p]++;
-                return ev(p, tt);
-            };
-            dist[i] = refine_powell(P, i, t[i], true, counted);
+        EvalSink sink;
+        auto run_one = [&](int i) -> Task<void> {
+            dist[i] = co_await refine_powell(P, i, t[i], true, Evaluator{&sink, &nccc[i]});
         };
         if (form == 0) {
-            EvalFn ev = [&](int pi, Mat4& tt) {
+            sink.direct = [&](int pi, Mat4& tt) {
                 float d;
                 ccc_batch(P, d_parts, dz.data(), 1, &pi, &tt, &d);
                 return d;
             };
-            for (int i = 0; i < n; i++) run_one(i, ev);
+            for (int i = 0; i < n; i++) {
+                Task<void> task = run_one(i);
+                run_direct(task);
+            }
         } else {
-            int T = std::min(n, 1024);
-            BatchEval be(P, d_parts, dz.data(), T);
-            EvalFn ev = [&](int pi, Mat4& tt) { return be.eval(pi, tt); };
-            std::vector<std::thread> th;
-            for (int w = 0; w < T; w++)
-                th.emplace_back([&, w] {
-                    be.worker([&] {
-                        for (int i = w; i < n; i += T) run_one(i, ev);
-                    });
-                });
-            for (auto& x : th) x.join();
-            be.rethrow();
+            std::vector<Task<void>> tasks;
+            tasks.reserve(n);
+            for (int i = 0; i < n; i++) tasks.push_back(run_one(i));
+            run_batched(P, d_parts, dz.data(), tasks, sink);
         }
     }
     for (int i = 0; i < n; i++) {

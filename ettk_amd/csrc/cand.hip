@@ -1,19 +1,18 @@
-// cand.hip -- candidate rotations of the SO(3) search from the local maxima of each correlation volume, on the
-// device: getTrueLocalMaximaPositions and eliminateEquivalentAngles (ProjRot.h:1593-1702), up to the priority
-// queue, which the host fills from the survivors in index order.
+// cand.hip -- the data-parallel part of the candidate extraction of the SO(3) search on the device
+// (getTrueLocalMaximaPositions and eliminateEquivalentAngles, ProjRot.h:1593-1702).
 //
-// Per volume (one workgroup): the maxima are sorted by index (blitz iteration order, alpha slowest); each is
+// Per volume (one workgroup): the local maxima are sorted by index (blitz iteration order, alpha slowest); each is
 // decoded to zyz(alpha, beta, gamma) = (Rg Rb) Ra from the host's rotation tables with the host's operation order
-// and rounding (no fused multiply-adds), and kept when isTransformValid accepts it; then the equivalent-angle
-// elimination: the reference visits every ordered pair (i, j) of kept maxima in index order and, when both are
-// still alive and their 4x4 matrices are within `tol` in L1 norm, drops the worse of the two (j on a tie).  A pair
-// visited as (i, j) with j < i can no longer have both ends alive (the pair was decided, or skipped because one
-// end had died, when row j was visited), so only the partners j > i of each row decide anything: the partners are
-// found in parallel, and the rows run in order on one wavefront, each row's partners on its lanes.  The survivors
-// are written over the volume's list, in index order.
+// and rounding (no fused multiply-adds) and kept when isTransformValid accepts it; then, for every kept maximum i,
+// the kept maxima j > i whose 4x4 matrices lie within `tol` of its own in L1 norm (its partners, in increasing j).
+// The sorted kept maxima are written over the volume's list and the partners as a compressed row list; the host
+// then runs the elimination itself, which is sequential and short: the reference visits every ordered pair (i, j)
+// in index order and, when both are still alive and close, drops the worse (j on a tie), and a pair visited as
+// (i, j) with j < i can no longer have both ends alive, so the partners j > i of the rows, taken in order, decide
+// everything.
 // A volume is left to the host (its raw list untouched, mode 1) when it holds more maxima than fit in LDS, when a
-// maximum has more close partners than a wavefront holds, or when a rotation angle lies within 1e-9 degree of the
-// rotation restriction (the host's acos decides there).
+// maximum has more partners than its row holds, or when a rotation angle lies within 1e-9 degree of the rotation
+// restriction (the host's acos decides there).
 #include <cfloat>
 
 #include "so3.hpp"
@@ -22,7 +21,7 @@ namespace ettk {
 
 namespace {
 
-constexpr int kCT = 256;  // threads per workgroup
+constexpr int kCT = 1024;  // threads per workgroup (four wavefronts per SIMD hide the LDS latency of the pair tests)
 
 __device__ __forceinline__ double dm(double a, double b) { return __dmul_rn(a, b); }
 __device__ __forceinline__ double da(double a, double b) { return __dadd_rn(a, b); }
@@ -70,37 +69,38 @@ __device__ __forceinline__ int block_scan_excl(int x, int* sh, int& total) {
 
 }  // namespace
 
-// active[v]: 0 = volume unused, 1 = extract here, 2 = leave to the host; mode[v]: 0 = survivors written (count in
-// kept[v]), 1 = the host extracts from the raw list (count[v])
+// active[v]: 0 = volume unused, 1 = extract here, 2 = leave to the host.  Results: mode[v] = 0 with kept[v] sorted
+// valid maxima over the volume's list, their partner counts in deg[v kCandMax + q] and the partners of row q in
+// adj[(v kCandMax + q) kCandSlots + .] (nadj[v] in all); mode[v] = 1: the host extracts from the raw list.
 __global__ void __launch_bounds__(kCT) k_lm_candidates(const int* __restrict__ count, int* __restrict__ idx,
                                                        double* __restrict__ val, int cap, const int* __restrict__ active,
                                                        const double* __restrict__ tabs, int n, double trans_restrict,
                                                        double rot_restrict, float tol, double* __restrict__ mats,
-                                                       int* __restrict__ adj, int* __restrict__ kept,
+                                                       int* __restrict__ deg, int* __restrict__ adj,
+                                                       int* __restrict__ kept, int* __restrict__ nadj,
                                                        int* __restrict__ mode) {
     __shared__ int key[kCandMax];
     __shared__ double vv[kCandMax];
-    __shared__ float pf[3][kCandMax];  // the first matrix row in float: a conservative pre-test of closeness
-    __shared__ int deg[kCandMax];
-    __shared__ unsigned char alive[kCandMax];
+    __shared__ float pf[6][kCandMax];  // the first two matrix rows in float: a conservative pre-test of closeness
     __shared__ int sh[kCT / 64 + 1];
     __shared__ int s_flag;
     const int v = blockIdx.x, tid = threadIdx.x;
     int* vidx = idx + (size_t)v * cap;
     double* vval = val + (size_t)v * cap;
-    if (!active[v]) {
+    auto leave = [&](int m) {
         if (tid == 0) {
             kept[v] = 0;
-            mode[v] = 0;
+            nadj[v] = 0;
+            mode[v] = m;
         }
+    };
+    if (!active[v]) {
+        leave(0);
         return;
     }
     const int c = count[(size_t)v * kLmCountStride];
     if (active[v] == 2 || c > cap || c > kCandMax) {
-        if (tid == 0) {
-            kept[v] = 0;
-            mode[v] = 1;
-        }
+        leave(1);
         return;
     }
     if (tid == 0) s_flag = 0;
@@ -133,21 +133,6 @@ __global__ void __launch_bounds__(kCT) k_lm_candidates(const int* __restrict__ c
     // isTransformValid of each maximum (in registers: at most kCandMax / kCT per thread)
     constexpr int NPT = kCandMax / kCT;
     int okm = 0;
-#pragma unroll
-    for (int u = 0; u < NPT; u++) {
-        const int k = tid + u * kCT;
-        if (k >= c) continue;
-        double m[16];
-        grid_rotation(tabs, n, key[k], m);
-        bool ok = !(fabs(m[3]) > trans_restrict || fabs(m[7]) > trans_restrict || fabs(m[11]) > trans_restrict);
-        const double px = m[2], py = m[6], pz = m[10];
-        const double nr = sqrt(da(da(dm(px, px), dm(py, py)), dm(pz, pz)));
-        const double cc = nr != 0.0 ? pz / nr : 0.0;
-        const double ang = cc >= 1.0 ? 0.0 : fabs(dm(acos(cc), 180.0) / M_PI);
-        if (fabs(ang - rot_restrict) <= 1e-9 * fmax(1.0, rot_restrict)) s_flag = 1;  // the host decides
-        if (ok && ang <= rot_restrict) okm |= 1 << u;
-    }
-    // keep the valid maxima in order: compacted in place (read before the barrier inside the scan)
     int myk[NPT];
     double myv[NPT];
 #pragma unroll
@@ -155,28 +140,34 @@ __global__ void __launch_bounds__(kCT) k_lm_candidates(const int* __restrict__ c
         const int k = tid + u * kCT;
         myk[u] = k < c ? key[k] : 0;
         myv[u] = k < c ? vv[k] : 0.0;
+        if (k >= c) continue;
+        double m[16];
+        grid_rotation(tabs, n, myk[u], m);
+        const bool ok = !(fabs(m[3]) > trans_restrict || fabs(m[7]) > trans_restrict || fabs(m[11]) > trans_restrict);
+        const double px = m[2], py = m[6], pz = m[10];
+        const double nr = sqrt(da(da(dm(px, px), dm(py, py)), dm(pz, pz)));
+        const double cc = nr != 0.0 ? pz / nr : 0.0;
+        const double ang = cc >= 1.0 ? 0.0 : fabs(dm(acos(cc), 180.0) / M_PI);
+        if (fabs(ang - rot_restrict) <= 1e-9 * fmax(1.0, rot_restrict)) s_flag = 1;  // the host decides
+        if (ok && ang <= rot_restrict) okm |= 1 << u;
     }
-    int nvalid = 0;
+    // keep the valid maxima in index order, compacted in place (all reads above precede the scan's barriers)
+    int nv = 0;
 #pragma unroll
     for (int u = 0; u < NPT; u++) {
-        // thread-major order must follow index order: process the elements of stride kCT block by block
         int tot = 0;
         const int pos = block_scan_excl((okm >> u) & 1, sh, tot);
         if ((okm >> u) & 1) {
-            key[nvalid + pos] = myk[u];
-            vv[nvalid + pos] = myv[u];
+            key[nv + pos] = myk[u];
+            vv[nv + pos] = myv[u];
         }
-        nvalid += tot;
+        nv += tot;
         __syncthreads();
     }
     if (s_flag) {
-        if (tid == 0) {
-            kept[v] = 0;
-            mode[v] = 1;
-        }
+        leave(1);
         return;
     }
-    const int nv = nvalid;
     double* vm = mats + (size_t)v * kCandMax * 16;
     for (int q = tid; q < nv; q += kCT) {
         double m[16];
@@ -186,19 +177,25 @@ __global__ void __launch_bounds__(kCT) k_lm_candidates(const int* __restrict__ c
         pf[0][q] = (float)m[0];
         pf[1][q] = (float)m[1];
         pf[2][q] = (float)m[2];
-        alive[q] = 1;
+        pf[3][q] = (float)m[4];
+        pf[4][q] = (float)m[5];
+        pf[5][q] = (float)m[6];
     }
     __syncthreads();
-    // close partners j > i of every kept maximum, in increasing j; the float pre-test rejects a pair only when the
-    // first three terms of the L1 distance already exceed tol by far more than their rounding
-    int* va = adj + (size_t)v * kCandMax * 64;
-    const float ftol = tol + 1e-5f;
+    // partners j > i of every kept maximum, in increasing j, into the row's kCandSlots slots (more: the host
+    // extracts).  The float pre-test rejects a pair only when six of the sixteen terms of the L1 distance already
+    // exceed tol by far more than their rounding (each float element within 2^-24 of the double one).
+    const float ftol = tol + 2e-5f;
+    int* va = adj + (size_t)v * kCandMax * kCandSlots;
+    int dsum = 0;
     for (int i = tid; i < nv; i += kCT) {
-        const float a0 = pf[0][i], a1 = pf[1][i], a2 = pf[2][i];
         const double* mi = vm + (size_t)i * 16;
+        const float a0 = pf[0][i], a1 = pf[1][i], a2 = pf[2][i], a3 = pf[3][i], a4 = pf[4][i], a5 = pf[5][i];
         int d = 0;
+#pragma unroll 4
         for (int j = i + 1; j < nv; j++) {
-            if (!(fabsf(a0 - pf[0][j]) + fabsf(a1 - pf[1][j]) + fabsf(a2 - pf[2][j]) < ftol)) continue;
+            const float e3 = fabsf(a0 - pf[0][j]) + fabsf(a1 - pf[1][j]) + fabsf(a2 - pf[2][j]);
+            if (!(e3 + fabsf(a3 - pf[3][j]) + fabsf(a4 - pf[4][j]) + fabsf(a5 - pf[5][j]) < ftol)) continue;
             const double* mj = vm + (size_t)j * 16;
             double err = 0.0;
             int e = 0;
@@ -207,84 +204,37 @@ __global__ void __launch_bounds__(kCT) k_lm_candidates(const int* __restrict__ c
                 if (!(err < (double)tol)) break;
             }
             if (e == 16) {
-                if (d < 64) va[(size_t)i * 64 + d] = j;
+                if (d < kCandSlots) va[(size_t)i * kCandSlots + d] = j;
                 d++;
             }
         }
-        deg[i] = d;
-        if (d > 64) s_flag = 1;
+        deg[(size_t)v * kCandMax + i] = d;
+        dsum += d;
+        if (d > kCandSlots) s_flag = 1;
     }
-    __syncthreads();
-    if (s_flag) {
-        if (tid == 0) {
-            kept[v] = 0;
-            mode[v] = 1;
-        }
+    int total = 0;
+    block_scan_excl(dsum, sh, total);
+    if (s_flag || total > kCandAdj) {
+        leave(1);
         return;
     }
-    // the rows that have partners, in order (over the pre-test array, free now)
-    int* rows = reinterpret_cast<int*>(&pf[0][0]);
-    int nrows = 0;
-    for (int q0 = 0; q0 < nv; q0 += kCT) {
-        const int q = q0 + tid;
-        const int has = q < nv && deg[q] > 0;
-        int tot = 0;
-        const int pos = block_scan_excl(has, sh, tot);
-        if (has) rows[nrows + pos] = q;
-        nrows += tot;
-    }
-    __syncthreads();
-    // the rows in order on wavefront 0, each row's partners on its lanes (the next row's partners are loaded while
-    // a row is decided): the alive partners before the first one that is better than row i die, and if there is
-    // such a partner, row i dies
-    if (tid < 64) {
-        const int lane = tid;
-        int jn = nrows > 0 && lane < deg[rows[0]] ? va[(size_t)rows[0] * 64 + lane] : 0;
-        for (int r = 0; r < nrows; r++) {
-            const int i = rows[r], di = deg[i];
-            const int j = jn;
-            if (r + 1 < nrows) {
-                const int i2 = rows[r + 1];
-                jn = lane < deg[i2] ? va[(size_t)i2 * 64 + lane] : 0;
-            }
-            if (!alive[i]) continue;
-            const float fi = (float)(-vv[i]);
-            const bool al = lane < di && alive[j];
-            const bool better = al && fi > (float)(-vv[j]);
-            const unsigned long long bal = __ballot(better);
-            const int first = bal ? __ffsll((long long)bal) - 1 : 64;
-            if (al && lane < first) alive[j] = 0;
-            if (lane == 0 && bal) alive[i] = 0;
-            __builtin_amdgcn_wave_barrier();
-        }
-    }
-    __syncthreads();
-    // survivors in index order over the volume's list
-    int written = 0;
-    for (int q0 = 0; q0 < nv; q0 += kCT) {
-        const int q = q0 + tid;
-        const int a = q < nv ? alive[q] : 0;
-        const int k = q < nv ? key[q] : 0;
-        const double x = q < nv ? vv[q] : 0.0;
-        int tot = 0;
-        const int pos = block_scan_excl(a, sh, tot);
-        if (a) {
-            vidx[written + pos] = k;
-            vval[written + pos] = x;
-        }
-        written += tot;
+    // the sorted kept maxima over the volume's list
+    for (int q = tid; q < nv; q += kCT) {
+        vidx[q] = key[q];
+        vval[q] = vv[q];
     }
     if (tid == 0) {
-        kept[v] = written;
+        kept[v] = nv;
+        nadj[v] = total;
         mode[v] = 0;
     }
 }
 
 void launch_lm_candidates(const int* count, int* idx, double* val, int cap, const int* active, int np,
                           const double* tabs, int n, double trans_restrict, double rot_restrict, float tol,
-                          double* mats, int* adj, int* kept, int* mode, hipStream_t s) {
+                          double* mats, int* deg, int* adj, int* kept, int* nadj, int* mode, hipStream_t s) {
     k_lm_candidates<<<np, kCT, 0, s>>>(count, idx, val, cap, active, tabs, n, trans_restrict, rot_restrict, tol, mats,
-                                       adj, kept, mode);
+                                       deg, adj, kept, nadj, mode);
     ETTK_HIP(hipGetLastError());
 }
 

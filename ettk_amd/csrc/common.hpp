@@ -150,6 +150,7 @@ struct PinnedBuf {
 // ettk_profile_query.  Algorithmic bytes per launch are accumulated beside the times.
 struct KernelProf {
     bool on = false;
+    std::string only;  // non-empty: only launches of this kernel are timed (host stages are always recorded)
     struct Rec {
         std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
         double bytes = 0;
@@ -220,7 +221,7 @@ struct ProfScope {
     hipEvent_t b = nullptr;
     hipStream_t s;
     ProfScope(KernelProf* prof, const char* name, double bytes, hipStream_t st) : P(prof), s(st) {
-        if (P && P->on) {
+        if (P && P->on && (P->only.empty() || P->only == name)) {
             {
                 std::lock_guard<std::mutex> g(P->mu);
                 r = &P->rec_locked(name);

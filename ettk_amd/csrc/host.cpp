@@ -684,16 +684,18 @@ using namespace ettk;
 static void plan_release(ettk_plan* P) {
     if (P->stream) (void)hipStreamSynchronize(P->stream);
     if (P->copy_stream) (void)hipStreamSynchronize(P->copy_stream);
+    if (P->aux_stream) (void)hipStreamSynchronize(P->aux_stream);
     for (auto& g : P->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     P->graphs.clear();
     so3_state_destroy(P);
     if (P->ev0) (void)hipEventDestroy(P->ev0);
     if (P->ev1) (void)hipEventDestroy(P->ev1);
-    hipStream_t s = P->stream, cs = P->copy_stream;
+    hipStream_t s = P->stream, cs = P->copy_stream, as = P->aux_stream;
     delete P;  // device buffers are released before their streams
     if (s) (void)hipStreamDestroy(s);
     if (cs) (void)hipStreamDestroy(cs);
+    if (as) (void)hipStreamDestroy(as);
 }
 
 extern "C" {
@@ -801,6 +803,7 @@ int ettk_plan_create(ettk_ctx* ctx, const ettk_params* params, ettk_plan** out) 
         P->rot_restrict = std::max((double)p.rot_restrict, 1.0);
         ETTK_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
         ETTK_HIP(hipStreamCreateWithFlags(&P->copy_stream, hipStreamNonBlocking));
+        ETTK_HIP(hipStreamCreateWithFlags(&P->aux_stream, hipStreamNonBlocking));
         ETTK_HIP(hipEventCreate(&P->ev0));
         ETTK_HIP(hipEventCreate(&P->ev1));
         plan_constants(P);
@@ -952,6 +955,14 @@ int ettk_profile(ettk_ctx* ctx, int enable) {
     if (!ctx->fft.prof) ctx->fft.prof = new KernelProf();
     ctx->fft.prof->reset();
     ctx->fft.prof->on = enable != 0;
+    ETTK_API_END
+}
+
+int ettk_profile_only(ettk_ctx* ctx, const char* kernel) {
+    ETTK_API_BEGIN
+    ETTK_REQUIRE(ctx, ETTK_ERR_ARG, "ettk_profile_only: null context");
+    if (!ctx->fft.prof) ctx->fft.prof = new KernelProf();
+    ctx->fft.prof->only = kernel ? kernel : "";
     ETTK_API_END
 }
 

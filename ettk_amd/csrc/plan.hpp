@@ -67,6 +67,7 @@ struct ettk_plan {
     // the plan's own streams (default priority): plans of one context run concurrently from different host threads
     hipStream_t stream = nullptr;
     hipStream_t copy_stream = nullptr;  // host downloads issued from worker threads (non-blocking)
+    hipStream_t aux_stream = nullptr;   // the candidate extraction of a search chunk, beside the next chunk's kernels
     ettk_params p{};
     int N = 0, H = 0;
     bool dummy_filter = true;

@@ -1457,33 +1457,65 @@ void launch_localmax(const double* C, int np, int B, int cap, int* count, int* i
     ETTK_HIP(hipGetLastError());
 }
 // block v: offset of list v = sum of the clamped counts before it; copies its entries in slot order
-// packs the per-volume lists back to back for one download: head = [offsets (np + 1), raw counts (np), modes (np)];
-// a volume of mode 0 contributes its kept[v] candidate survivors, one of mode 1 its raw list of maxima
+// packs the per-volume lists back to back for one download: head = [offsets (np + 1), raw counts (np), modes (np),
+// partner offsets (np + 1)]; a volume of mode 0 contributes its kept[v] sorted valid maxima with their partner
+// counts and its nadj[v] partners, one of mode 1 its raw list of maxima
 __global__ void k_lm_compact(const int* __restrict__ count, const int* __restrict__ idx, const double* __restrict__ val,
-                             const int* __restrict__ kept, const int* __restrict__ mode, int np, int cap,
-                             int* __restrict__ head, int* __restrict__ pidx, double* __restrict__ pval) {
+                             const int* __restrict__ kept, const int* __restrict__ mode, const int* __restrict__ deg,
+                             const int* __restrict__ nadj, const int* __restrict__ adj, int np, int cap,
+                             int* __restrict__ head, int* __restrict__ pidx, double* __restrict__ pval,
+                             int* __restrict__ pdeg, int* __restrict__ padj) {
     const int v = blockIdx.x;
-    auto packed = [&](int u) {
-        const int raw = min(count[(size_t)u * kLmCountStride], cap);
-        return mode[u] ? raw : kept[u];
-    };
-    int off = 0;
-    for (int u = 0; u < v; u++) off += packed(u);
-    const int c = packed(v);
+    auto packed = [&](int u) { return mode[u] ? min(count[(size_t)u * kLmCountStride], cap) : kept[u]; };
+    auto partners = [&](int u) { return mode[u] ? 0 : nadj[u]; };
+    int off = 0, aoff = 0;
+    for (int u = 0; u < v; u++) {
+        off += packed(u);
+        aoff += partners(u);
+    }
+    const int c = packed(v), na = partners(v), m = mode[v];
     if (threadIdx.x == 0) {
         head[v] = off;
         head[np + 1 + v] = count[(size_t)v * kLmCountStride];
-        head[2 * np + 1 + v] = mode[v];
-        if (v == np - 1) head[np] = off + c;
+        head[2 * np + 1 + v] = m;
+        head[3 * np + 1 + v] = aoff;
+        if (v == np - 1) {
+            head[np] = off + c;
+            head[4 * np + 1] = aoff + na;
+        }
     }
     for (int k = threadIdx.x; k < c; k += blockDim.x) {
         pidx[off + k] = idx[(size_t)v * cap + k];
         pval[off + k] = val[(size_t)v * cap + k];
+        pdeg[off + k] = m ? 0 : deg[(size_t)v * kCandMax + k];
+    }
+    if (m || na == 0) return;
+    // the partner rows back to back: row offsets by a running scan over blocks of rows
+    __shared__ int sc[kT];
+    int base = aoff;
+    for (int r0 = 0; r0 < c; r0 += blockDim.x) {
+        const int r = r0 + threadIdx.x;
+        const int d = r < c ? deg[(size_t)v * kCandMax + r] : 0;
+        sc[threadIdx.x] = d;
+        __syncthreads();
+        for (int o = 1; o < (int)blockDim.x; o <<= 1) {  // inclusive scan in LDS
+            const int y = threadIdx.x >= o ? sc[threadIdx.x - o] : 0;
+            __syncthreads();
+            sc[threadIdx.x] += y;
+            __syncthreads();
+        }
+        const int start = base + sc[threadIdx.x] - d;
+        const int* src = adj + ((size_t)v * kCandMax + (r < c ? r : 0)) * kCandSlots;
+        for (int k = 0; k < d; k++) padj[start + k] = src[k];
+        const int blk = sc[blockDim.x - 1];
+        __syncthreads();
+        base += blk;
     }
 }
-void launch_lm_compact(const int* count, const int* idx, const double* val, const int* kept, const int* mode, int np,
-                       int cap, int* head, int* pidx, double* pval, hipStream_t s) {
-    k_lm_compact<<<np, kT, 0, s>>>(count, idx, val, kept, mode, np, cap, head, pidx, pval);
+void launch_lm_compact(const int* count, const int* idx, const double* val, const int* kept, const int* mode,
+                       const int* deg, const int* nadj, const int* adj, int np, int cap, int* head, int* pidx,
+                       double* pval, int* pdeg, int* padj, hipStream_t s) {
+    k_lm_compact<<<np, kT, 0, s>>>(count, idx, val, kept, mode, deg, nadj, adj, np, cap, head, pidx, pval, pdeg, padj);
     ETTK_HIP(hipGetLastError());
 }
 void launch_ppmax(const float* Pp1, const float* Pp2, int np, int B, float* Mx, hipStream_t s) {

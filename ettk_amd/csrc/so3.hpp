@@ -26,12 +26,16 @@ struct ComboOp {
 // per-volume counters sit kLmCountStride ints apart, one 128-byte line each, so that appends to different
 // volumes do not serialise on one line
 constexpr int kLmCountStride = 32;
+constexpr int kCandMax = 4096;   // maxima per volume the device candidate extraction takes (more: the host does)
+constexpr int kCandSlots = 64;   // equivalent partners per maximum it keeps (more: the host extracts)
+constexpr int kCandAdj = 16384;  // packed partner entries per volume
 struct LmBuffers {
     DevBuf<int> count, idx;
     DevBuf<double> val;
-    // device candidate extraction: volumes to process (uploaded from hact), survivors per volume, mode per volume
-    DevBuf<int> act, kept, mode;
-    PinnedBuf<int> hact;
+    // device candidate extraction: volumes to process (uploaded from hact); per volume the kept maxima, the partner
+    // entries and the mode; per maximum its partner count, per volume its partner lists; their packed copies
+    DevBuf<int> act, kept, nadj, mode, deg, adj, pdeg, padj;
+    PinnedBuf<int> hact, hpdeg, hpadj;
     // compacted lists: head = [offsets (nvol + 1), counts (nvol)], then the packed indices and values
     DevBuf<int> head, pidx;
     DevBuf<double> pval;
@@ -66,8 +70,7 @@ struct So3State {
     DevBuf<int> ppos;
     DevBuf<double2> fm, cs, cp, T1, T2, rot, g;
     DevBuf<double> zyz_tabs;  // [Ra; Rb; Rg] of the correlation grid, 2B 4x4 matrices each (device candidates)
-    DevBuf<double> cand_mats;  // scratch of the device candidates: matrices and close partners per volume
-    DevBuf<int> cand_adj;
+    DevBuf<double> cand_mats;  // scratch of the device candidates: the matrices of the kept maxima per volume
     DevBuf<double2> csT, cpT;  // coefficients instance-fastest for the matrix-core synthesis: [l l + l - m][q]
     DevBuf<int> pos;
     // search workspace (align driver)
@@ -86,15 +89,20 @@ inline LmBuffers& lm_slot(So3State& S, int slot, int nvol, int cap) {
     b.count.ensure((size_t)nvol * kLmCountStride);
     b.idx.ensure((size_t)nvol * cap);
     b.val.ensure((size_t)nvol * cap);
-    b.head.ensure((size_t)3 * nvol + 1);
+    b.head.ensure((size_t)4 * nvol + 2);
     b.act.ensure(nvol);
     b.kept.ensure(nvol);
+    b.nadj.ensure(nvol);
     b.mode.ensure(nvol);
+    b.deg.ensure((size_t)nvol * kCandMax);
+    b.adj.ensure((size_t)nvol * kCandMax * kCandSlots);
+    b.pdeg.ensure((size_t)nvol * cap);
+    b.padj.ensure((size_t)nvol * kCandAdj);
     b.hact.ensure(nvol);
     b.pidx.ensure((size_t)nvol * cap);
     b.pval.ensure((size_t)nvol * cap);
     // the host lists start at 2048 entries per volume (grown by the download when a chunk holds more)
-    b.hhead.ensure((size_t)3 * nvol + 1);
+    b.hhead.ensure((size_t)4 * nvol + 2);
     b.hpidx.ensure((size_t)nvol * std::min(cap, 2048));
     b.hpval.ensure((size_t)nvol * std::min(cap, 2048));
     return b;
@@ -120,15 +128,15 @@ So3State& so3_state(ettk_plan* P);
 void launch_combine(const double* C, const ComboOp* d_ops, int np, int B, int b0, double* out, hipStream_t s);
 void launch_localmax(const double* C, int np, int B, int cap, int* count, int* idx, double* val, hipStream_t s);
 // packs the np lists (at most `cap` entries each) back to back: head = [offsets (np + 1), counts (np)]
-void launch_lm_compact(const int* count, const int* idx, const double* val, const int* kept, const int* mode, int np,
-                       int cap, int* head, int* pidx, double* pval, hipStream_t s);
-// candidates of each active volume on the device (cand.hip): sorted, valid and mutually non-equivalent maxima
-// written over the volume's list (kept[v], mode[v] = 0), or mode[v] = 1 where the host extracts from the raw list
-// (active[v]: 0 unused, 1 device, 2 host)
-constexpr int kCandMax = 4096;  // maxima per volume the device extracts from (more: the host does)
+void launch_lm_compact(const int* count, const int* idx, const double* val, const int* kept, const int* mode,
+                       const int* deg, const int* nadj, const int* adj, int np, int cap, int* head, int* pidx,
+                       double* pval, int* pdeg, int* padj, hipStream_t s);
+// the data-parallel part of the candidate extraction of each active volume on the device (cand.hip): its sorted
+// valid maxima over the volume's list with their equivalent partners (kept[v], deg, adj, nadj[v], mode[v] = 0), or
+// mode[v] = 1 where the host extracts from the raw list (active[v]: 0 unused, 1 device, 2 host)
 void launch_lm_candidates(const int* count, int* idx, double* val, int cap, const int* active, int np,
                           const double* tabs, int n, double trans_restrict, double rot_restrict, float tol,
-                          double* mats, int* adj, int* kept, int* mode, hipStream_t s);
+                          double* mats, int* deg, int* adj, int* kept, int* nadj, int* mode, hipStream_t s);
 void launch_ppmax(const float* Pp1, const float* Pp2, int np, int B, float* Mx, hipStream_t s);
 
 }  // namespace ettk

@@ -60,6 +60,7 @@ def load() -> C.CDLL:
                                    C.c_int),
             "ettk_profile": ([vp, C.c_int], C.c_int),
             "ettk_profile_query": ([vp, C.c_char_p, dp, C.POINTER(C.c_longlong), dp], C.c_int),
+            "ettk_profile_only": ([vp, C.c_char_p], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(lib, name)
@@ -142,6 +143,10 @@ class Context:
 
     def profile(self, enable: bool) -> None:
         check(self.lib.ettk_profile(self.h, int(enable)))
+
+    def profile_only(self, kernel: str | None) -> None:
+        """Time only the launches of `kernel` (None: every instrumented kernel)."""
+        check(self.lib.ettk_profile_only(self.h, kernel.encode() if kernel else None))
 
     def profile_query(self, kernel: str):
         """(total ms, launches, algorithmic bytes) of one instrumented kernel since ettk_profile(ctx, 1)."""

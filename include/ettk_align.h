@@ -322,6 +322,9 @@ int ettk_extract_batch(ettk_ctx* ctx, const float* d_tomogram, int nx, int ny, i
    those launches move (one read of every input and one write of every output). */
 int ettk_profile(ettk_ctx* ctx, int enable);
 int ettk_profile_query(ettk_ctx* ctx, const char* kernel, double* total_ms, long long* launches, double* bytes);
+/* Restricts the HIP-event timing to the launches of one kernel (NULL or "": every instrumented kernel), so that a
+   timed run carries two events per launch of that kernel only; host stages are recorded either way. */
+int ettk_profile_only(ettk_ctx* ctx, const char* kernel);
 
 #ifdef __cplusplus
 }
