@@ -47,6 +47,20 @@ __host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(
 __host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 __host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
+// Streaming accesses (the nontemporal bit): data that one evaluation reads or writes once per pass, marked so
+// that the L2 evicts it before the data every evaluation of a launch shares (reference spectrum, masks, filter).
+typedef float nt_float2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld_stream(const float2* p) {
+    const nt_float2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_float2*>(p));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st_stream(float2* p, float2 v) {
+    nt_float2 w;
+    w.x = v.x;
+    w.y = v.y;
+    __builtin_nontemporal_store(w, reinterpret_cast<nt_float2*>(p));
+}
+
 // Workgroup-wide copies in which every global load of a round is issued before the first loaded value is used,
 // so a round costs one memory latency instead of one per element.  Thread t moves elements t, t + NT, ...;
 // out-of-range slots re-read the last element and are dropped.

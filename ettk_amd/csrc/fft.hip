@@ -332,7 +332,9 @@ __global__ void __launch_bounds__(StridedShape<L>::THREADS) k_fft_strided(Stride
     using S = StridedShape<L>;
     __shared__ float2 lds[S::P * S::LD];
     __shared__ float2 tw[L];
-    const long q0 = (long)blockIdx.x * S::P;
+    // consecutive pencil groups on one XCD: at L = 128 a group's rows are half cache lines, whose other halves
+    // belong to the neighbouring groups (fetched into one L2 instead of two)
+    const long q0 = (long)xcd_tile(blockIdx.x, gridDim.x) * S::P;
     const float2 twv = twg[threadIdx.x % L];  // stored after the staging loads are in flight
     // THREADS is a multiple of P: a thread moves the elements l = t / P, t / P + THREADS / P, ... of the one
     // pencil t % P, so the pencil's base address is computed once
@@ -407,7 +409,7 @@ __global__ void __launch_bounds__(StridedShape<L>::THREADS) k_ccc_ipass(CccIpass
     {
         float2 v[NL];
 #pragma unroll
-        for (int u = 0; u < NL; u++) v[u] = a.S2[vol + (size_t)(l0 + u * STEP) * LH + pc];
+        for (int u = 0; u < NL; u++) v[u] = ld_stream(&a.S2[vol + (size_t)(l0 + u * STEP) * LH + pc]);
 #pragma unroll
         for (int u = 0; u < NL; u++) pen[l0 + u * STEP] = valid ? v[u] : make_float2(0.f, 0.f);
     }
@@ -444,7 +446,7 @@ __global__ void __launch_bounds__(StridedShape<L>::THREADS) k_ccc_ipass(CccIpass
         if (a.apply) z = cscale(z, fw[u]);
         if (dc) z = make_float2(0.f, 0.f);
         if (valid) {
-            a.S2[vol + (size_t)i * LH + pc] = z;
+            st_stream(&a.S2[vol + (size_t)i * LH + pc], z);
             s2 += hw * (z.x * z.x + z.y * z.y);
             swf += fw[u];
         }
@@ -458,7 +460,7 @@ __global__ void __launch_bounds__(StridedShape<L>::THREADS) k_ccc_ipass(CccIpass
     for (int u = 0; u < NL; u++) f1[u] = a.F1mcf[(size_t)(l0 + u * STEP) * LH + pc];
     if (valid)
 #pragma unroll
-        for (int u = 0; u < NL; u++) a.S3[vol + (size_t)(l0 + u * STEP) * LH + pc] = pen[l0 + u * STEP];
+        for (int u = 0; u < NL; u++) st_stream(&a.S3[vol + (size_t)(l0 + u * STEP) * LH + pc], pen[l0 + u * STEP]);
     float s1 = 0.f;
 #pragma unroll
     for (int u = 0; u < NL; u++) {
@@ -473,7 +475,7 @@ __global__ void __launch_bounds__(StridedShape<L>::THREADS) k_ccc_ipass(CccIpass
     pencil_fft<L>(&lds[q * S::LD], lane, tw, twc, true);
     if (valid)
 #pragma unroll
-        for (int u = 0; u < NL; u++) a.S1[vol + (size_t)(l0 + u * STEP) * LH + pc] = pen[l0 + u * STEP];
+        for (int u = 0; u < NL; u++) st_stream(&a.S1[vol + (size_t)(l0 + u * STEP) * LH + pc], pen[l0 + u * STEP]);
     const float t1 = block_sum_nt<S::THREADS>(s1, sred);
     const float tf = block_sum_nt<S::THREADS>(swf, sred);
     const float t2 = block_sum_nt<S::THREADS>(s2, sred);
@@ -886,8 +888,8 @@ __global__ void __launch_bounds__(kCfT) k_combine_fftA(const float2* __restrict_
             for (int u = 0; u < NE; u++) {
                 const int t = threadIdx.x + u * kCfT;
                 const long h = ((long)(t / H) * L + j) * H + t % H;
-                vf1[u] = f1p[h];
-                vq[u] = qp[h];
+                vf1[u] = ld_stream(&f1p[h]);
+                vq[u] = ld_stream(&qp[h]);
             }
 #pragma unroll
             for (int u = 0; u < NE; u++) {
@@ -903,7 +905,7 @@ __global__ void __launch_bounds__(kCfT) k_combine_fftA(const float2* __restrict_
         for (int u = 0; u < NE; u++) {
             const int t = threadIdx.x + u * kCfT;
             const long h = ((long)(t / H) * L + j) * H + t % H;
-            vf2[u] = f2p[h];
+            vf2[u] = ld_stream(&f2p[h]);
             vm[u] = Mfft[h];
         }
 #pragma unroll
@@ -922,7 +924,7 @@ __global__ void __launch_bounds__(kCfT) k_combine_fftA(const float2* __restrict_
         for (int u = 0; u < NE; u++) {
             const int t = threadIdx.x + u * kCfT;
             const long h = ((long)(t / H) * L + j) * H + t % H;
-            const float2 f1 = f1p[h], f2 = f2p[h], qv = qp[h], m = Mfft[h];
+            const float2 f1 = ld_stream(&f1p[h]), f2 = ld_stream(&f2p[h]), qv = ld_stream(&qp[h]), m = Mfft[h];
             const int i = t / H, k = t % H;
             float2 c = csub(cscale(cmulc(f1, f2), a), cscale(cmulc(m, qv), b));
             if ((i + j + k) & 1) c = make_float2(-c.x, -c.y);
