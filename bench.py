@@ -495,7 +495,7 @@ def classify128(args) -> int:
     s = time.perf_counter() - t0
     avgs = d_refs.cpu().numpy()
     agree = [round(agreement(it.classes, labels, K), 4) for it in out.iterations]
-    res = [fsc_resolution(fsc(avgs[k], refs[k])) for k in range(K)]
+    res = [1 + fsc_resolution(fsc(avgs[k], refs[k])[1:]) for k in range(K)]  # shells 1 .. N/2 - 1 (no DC term)
     line = {"metric": "particle-reference alignments/s, 128^3 B=64, classification into 8 classes",
             "value": round(n * K * args.iters / s, 2), "unit": "alignments/s", "n_gpus": 1, "steps": args.iters,
             "warmup": 0, "ms_per_step": round(s / args.iters * 1e3, 2), "higher_is_better": True, "scaling": "weak",

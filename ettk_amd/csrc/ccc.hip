@@ -12,6 +12,8 @@
 //               first-maximum search of locateMaxima (nbfCorrelationImageMetric.h:304-323)
 // Per-evaluation sums are written as per-block partials and reduced in a fixed order (deterministic).
 #include "kernels.hpp"
+
+#include <hip/hip_ext.h>
 #include "wedge.hpp"
 
 #include <algorithm>
@@ -73,6 +75,8 @@ __device__ __forceinline__ bool inside_fp64(const double* md, int i, int j, int 
 // Measured on 256 evaluations at 64^3 (one launch): 654 us, against 775 us for an 18^3 box at strides 19/343,
 // 8x8 planes per wavefront and two reductions; outputs bit-identical.
 static constexpr int kTile = 8, kBox = 17, kSY = 17, kSZ = 300;
+// kUse names the caller in kernel traces (0: CCC evaluations, 1: S^2 projection); the code is the same
+template <int kUse>
 __global__ void __launch_bounds__(kThreads) k_reslice(const float* __restrict__ parts, const EvalGeom* __restrict__ geom,
                                                        int N, float* __restrict__ out, float* __restrict__ partials) {
     __shared__ float2 sh[kThreads / 64];
@@ -226,10 +230,20 @@ __global__ void __launch_bounds__(64) k_reslice_mean(const float* __restrict__ p
 int reslice_blocks(int N) { return (N / kTile) * (N / kTile) * (N / kTile); }
 
 void launch_reslice(const float* parts, const EvalGeom* geom, int n_eval, int N, float* out, float* partials,
-                    int nb, hipStream_t s) {
+                    int nb, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop) {
     ETTK_REQUIRE(N % kTile == 0 && nb == reslice_blocks(N), ETTK_ERR_ARG, "launch_reslice: block count mismatch");
     dim3 grid(nb, n_eval);
-    k_reslice<<<grid, kThreads, 0, s>>>(parts, geom, N, out, partials);
+    if (ev_start || ev_stop)  // the events take the kernel's own start and end timestamps
+        hipExtLaunchKernelGGL(k_reslice<0>, grid, dim3(kThreads), 0, s, ev_start, ev_stop, 0, parts, geom, N, out,
+                              partials);
+    else
+        k_reslice<0><<<grid, kThreads, 0, s>>>(parts, geom, N, out, partials);
+    ETTK_HIP(hipGetLastError());
+}
+void launch_reslice_projection(const float* parts, const EvalGeom* geom, int n, int N, float* out, float* partials,
+                               int nb, hipStream_t s) {
+    ETTK_REQUIRE(N % kTile == 0 && nb == reslice_blocks(N), ETTK_ERR_ARG, "launch_reslice: block count mismatch");
+    k_reslice<1><<<dim3(nb, n), kThreads, 0, s>>>(parts, geom, N, out, partials);
     ETTK_HIP(hipGetLastError());
 }
 void launch_reslice_mean(const float* partials, int nb, int n_eval, float* stats, hipStream_t s) {

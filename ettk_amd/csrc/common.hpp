@@ -232,9 +232,18 @@ struct KernelProf {
 struct ProfScope {
     KernelProf* P;
     KernelProf::Rec* r = nullptr;
-    hipEvent_t b = nullptr;
+    hipEvent_t b = nullptr, e = nullptr;
     hipStream_t s;
-    ProfScope(KernelProf* prof, const char* name, double bytes, hipStream_t st) : P(prof), s(st) {
+    bool by_launch, launched = false;
+    // by_launch: the launch records the two events itself (hipExtLaunchKernelGGL with start() and stop(), whose
+    // timestamps are the kernel's own start and end, as a kernel trace reports them), so the scope records neither;
+    // otherwise the events bracket the launch on its stream
+    ProfScope(KernelProf* prof, const char* name, double bytes, hipStream_t st, bool by_launch_ = false)
+        : P(prof), s(st), by_launch(by_launch_) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        // a launch captured into a graph carries no events
+        if (by_launch && P && P->on && hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+            return;
         if (P && P->on && (P->only.empty() || P->only == name)) {
             {
                 std::lock_guard<std::mutex> g(P->mu);
@@ -242,21 +251,33 @@ struct ProfScope {
                 r->bytes += bytes;
                 r->launches++;
                 b = P->take_locked();
+                if (by_launch) e = P->take_locked();
             }
-            ETTK_HIP(hipEventRecord(b, s));
+            if (!by_launch) ETTK_HIP(hipEventRecord(b, s));
         }
     }
+    hipEvent_t start() const { return b; }
+    hipEvent_t stop() const { return e; }
+    void done() { launched = true; }  // by_launch: the launch that carries the events succeeded
     ~ProfScope() {
-        if (r) {
-            hipEvent_t e;
-            {
-                std::lock_guard<std::mutex> g(P->mu);
-                e = P->take_locked();
-            }
-            if (hipEventRecord(e, s) == hipSuccess) {
-                std::lock_guard<std::mutex> g(P->mu);
+        if (!r) return;
+        if (by_launch) {
+            std::lock_guard<std::mutex> g(P->mu);
+            if (launched) {
                 r->ev.emplace_back(b, e);
+            } else {
+                P->pool.push_back(b);
+                P->pool.push_back(e);
             }
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> g(P->mu);
+            e = P->take_locked();
+        }
+        if (hipEventRecord(e, s) == hipSuccess) {
+            std::lock_guard<std::mutex> g(P->mu);
+            r->ev.emplace_back(b, e);
         }
     }
 };

@@ -533,8 +533,9 @@ void ccc_enqueue_chunk(ettk_plan* P, const CccWork& W, const float* d_parts, con
     const double Gb = (double)E * P->z.E * N * H * sizeof(float2);
     // particle side: F2 = r2c((reslice - mean) maskfft shift)
     {
-        ProfScope ps(kp, "k_reslice", 2 * Rb, s);
-        launch_reslice(d_parts, geo, E, N, W.R2, W.p_res, P->nb_res, s);
+        ProfScope ps(kp, "k_reslice", 2 * Rb, s, true);
+        launch_reslice(d_parts, geo, E, N, W.R2, W.p_res, P->nb_res, s, ps.start(), ps.stop());
+        ps.done();
     }
     launch_reslice_mean(W.p_res, P->nb_res, E, W.stats, s);
     R2CArgs a2{W.R2, W.S2, N, E, PRO_RESLICED, P->maskfft.p, W.stats};

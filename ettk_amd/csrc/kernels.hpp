@@ -97,7 +97,11 @@ void launch_dftBC_fft(const FftTables& T, const float2* G1, const float2* tabK, 
 // reslice_blocks(N) (one workgroup per 8^3 output tile), partials hold 2 * nb floats per evaluation
 int reslice_blocks(int N);
 void launch_reslice(const float* parts, const EvalGeom* geom, int n_eval, int N, float* out, float* partials,
-                    int nb, hipStream_t s);
+                    int nb, hipStream_t s,
+                    hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
+// the same reslice for the S^2 projection of particles in their own frame (its own name in kernel traces)
+void launch_reslice_projection(const float* parts, const EvalGeom* geom, int n, int N, float* out, float* partials,
+                               int nb, hipStream_t s);
 // mean of the valid voxels per evaluation, from the reslice partials (nb per eval): stats[e] = mean
 void launch_reslice_mean(const float* partials, int nb, int n_eval, float* stats, hipStream_t s);
 // spectral weighting of both CCC sides: X1 = F1 filter W1 W2(e) (F1 shared), X2 = mcf(X2) filter W1 W2(e) in

@@ -1166,7 +1166,7 @@ void project_images(ettk_plan* P, const float* d_parts, const ettk_particle* des
             }
             ETTK_HIP(hipMemcpyAsync(S.projGeom.p, src, cn * sizeof(EvalGeom), hipMemcpyHostToDevice, s));
             if (!h_geom) ETTK_HIP(hipStreamSynchronize(s));  // g is reused by the next sub-chunk
-            launch_reslice(d_parts, S.projGeom.p, cn, N, S.projR.p, S.projPartials.p, nb_res, s);
+            launch_reslice_projection(d_parts, S.projGeom.p, cn, N, S.projR.p, S.projPartials.p, nb_res, s);
         }
         k_proj_partial<<<dim3(kProjParts, cn), kT, 0, s>>>(S.projR.p, S.projPartials.p, nb_res, N, is_ref ? 1 : 0,
                                                            S.projSum.p, S.projStats.p);
