@@ -1426,7 +1426,7 @@ int ettk_project_batch(ettk_plan* P, const float* d_particles, int n, const ettk
     return ETTK_OK;
 }
 
-int ettk_so3_correlate_batch(ettk_plan* P, int n, const double* sig, const double* pat, double* out_c) {
+static int so3_correlate_entry(ettk_plan* P, int n, const double* sig, const double* pat, double* out_c, int scaled) {
     try {
         ETTK_REQUIRE(P && sig && pat && out_c && n > 0, ETTK_ERR_ARG, "ettk_so3_correlate_batch: bad argument");
         ETTK_HIP(hipSetDevice(P->ctx->device));
@@ -1434,7 +1434,7 @@ int ettk_so3_correlate_batch(ettk_plan* P, int n, const double* sig, const doubl
         const int B = P->p.B;
         const size_t n3 = (size_t)8 * B * B * B;
         S.C.ensure(n3 * n);
-        soft_given(P, n, sig, pat, S.C.p);
+        soft_given(P, n, sig, pat, S.C.p, scaled);
         ETTK_HIP(hipMemcpyAsync(out_c, S.C.p, n3 * n * sizeof(double), hipMemcpyDeviceToHost, P->stream));
         ETTK_HIP(hipStreamSynchronize(P->stream));
     } catch (const Error& e) {
@@ -1446,6 +1446,14 @@ int ettk_so3_correlate_batch(ettk_plan* P, int n, const double* sig, const doubl
     }
     set_error("");
     return ETTK_OK;
+}
+
+int ettk_so3_correlate_batch(ettk_plan* P, int n, const double* sig, const double* pat, double* out_c) {
+    return so3_correlate_entry(P, n, sig, pat, out_c, 0);
+}
+
+int ettk_so3_correlate_scaled_batch(ettk_plan* P, int n, const double* sig, const double* pat, double* out_c) {
+    return so3_correlate_entry(P, n, sig, pat, out_c, 1);
 }
 
 int ettk_sph_rotate_batch(ettk_plan* P, int n, const float* images, const double* transforms, float* out) {

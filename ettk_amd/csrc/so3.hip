@@ -476,7 +476,9 @@ __global__ void k_ring_idft(const double2* __restrict__ g, const double2* __rest
 __global__ void __launch_bounds__(kT) k_so3_synth(const double2* __restrict__ cs, const double2* __restrict__ cp,
                                                    const double* __restrict__ dtab, const int* __restrict__ doff,
                                                    int d_slice, int B, double2* __restrict__ T1) {
-    const int kb = blockIdx.x, q = blockIdx.y;
+    // instances fastest: the workgroups that run together read the same beta slice of the table (2.8 MB at B = 64)
+    // through the L2 instead of one slice each from HBM
+    const int q = blockIdx.x, kb = blockIdx.y;
     const int n = 2 * B, W = 2 * B - 1;
     const double2* S = cs + (size_t)q * B * B;
     const double2* Pt = cp + (size_t)q * B * B;
@@ -537,6 +539,84 @@ __global__ void k_dft_m2(const double2* __restrict__ T2, const double2* __restri
             re += z.x * w.x - z.y * w.y;
         }
         C[(((size_t)q * n + a) * n + kb) * n + g] = re;
+    }
+}
+
+// In-place radix-2 FFT (e^{-2 pi i jk/n}) of V vectors of length n = 2^LOGN in LDS, element e of vector v at
+// X[v (n + 1) + e]; tw[j] = e^{-2 pi i j / n}
+template <int LOGN, int V>
+__device__ __forceinline__ void fft_lds_f64_vecs(double2* X, const double2* tw) {
+    constexpr int n = 1 << LOGN, LD = n + 1;
+    for (int t = threadIdx.x; t < n * V; t += blockDim.x) {
+        const int v = t / n, i = t % n;
+        const int r = (int)(__brev((unsigned)i) >> (32 - LOGN));
+        if (r > i) {
+            const double2 a = X[v * LD + i], b = X[v * LD + r];
+            X[v * LD + i] = b;
+            X[v * LD + r] = a;
+        }
+    }
+    __syncthreads();
+    for (int st = 1; st <= LOGN; st++) {
+        const int half = 1 << (st - 1);
+        for (int t = threadIdx.x; t < V * n / 2; t += blockDim.x) {
+            const int v = t / (n / 2), b = t % (n / 2);
+            const int j = b % half, k = (b / half) * 2 * half;
+            const double2 w = tw[j * (n >> st)];
+            const int i0 = v * LD + k + j, i1 = i0 + half;
+            const double2 u = X[i0], x = X[i1];
+            const double2 y = make_double2(w.x * x.x - w.y * x.y, w.x * x.y + w.y * x.x);
+            X[i0] = make_double2(u.x + y.x, u.y + y.y);
+            X[i1] = make_double2(u.x - y.x, u.y - y.y);
+        }
+        __syncthreads();
+    }
+}
+
+// The two DFTs of the SO(3) synthesis as FFTs through global memory, for slices too large for one workgroup's LDS
+// (B = 64: a 128 x 128 FP64 slice is 256 KB).  k_fft_m1: T2[q][beta][alpha][m2] = sum_m1 T1[q][beta][m1][m2]
+// e^{-i m1 alpha}, one workgroup per 16 columns m2 of a slice; k_fft_m2: C[q][alpha][beta][gamma] =
+// Re sum_m2 T2[q][beta][alpha][m2] e^{-i m2 gamma}, one workgroup per 16 rows alpha.  The same sums as k_dft_m1 and
+// k_dft_m2 in n log n operations (the negative orders m = c - n carry the same phases as c on the n-point grid).
+static constexpr int kFftV = 16;
+template <int LOGN>
+__global__ void __launch_bounds__(kT) k_fft_m1(const double2* __restrict__ T1, const double2* __restrict__ tw,
+                                               double2* __restrict__ T2) {
+    constexpr int n = 1 << LOGN, LD = n + 1, V = kFftV;
+    __shared__ double2 X[V * LD];
+    __shared__ double2 stw[n];
+    const int g = blockIdx.x, kb = blockIdx.y, q = blockIdx.z;
+    const size_t base = ((size_t)q * n + kb) * n * n;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = tw[t];
+    for (int t = threadIdx.x; t < n * V; t += blockDim.x) {
+        const int m1 = t / V, v = t % V;
+        X[v * LD + m1] = T1[base + (size_t)m1 * n + g * V + v];
+    }
+    __syncthreads();
+    fft_lds_f64_vecs<LOGN, V>(X, stw);
+    for (int t = threadIdx.x; t < n * V; t += blockDim.x) {
+        const int a = t / V, v = t % V;
+        T2[base + (size_t)a * n + g * V + v] = X[v * LD + a];
+    }
+}
+template <int LOGN>
+__global__ void __launch_bounds__(kT) k_fft_m2(const double2* __restrict__ T2, const double2* __restrict__ tw,
+                                               double* __restrict__ C) {
+    constexpr int n = 1 << LOGN, LD = n + 1, V = kFftV;
+    __shared__ double2 X[V * LD];
+    __shared__ double2 stw[n];
+    const int g = blockIdx.x, kb = blockIdx.y, q = blockIdx.z;
+    const size_t base = ((size_t)q * n + kb) * n * n;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = tw[t];
+    for (int t = threadIdx.x; t < n * V; t += blockDim.x) {
+        const int v = t / n, m2 = t % n;
+        X[v * LD + m2] = T2[base + (size_t)(g * V + v) * n + m2];
+    }
+    __syncthreads();
+    fft_lds_f64_vecs<LOGN, V>(X, stw);
+    for (int t = threadIdx.x; t < n * V; t += blockDim.x) {
+        const int v = t / n, gm = t % n;
+        C[(((size_t)q * n + g * V + v) * n + kb) * n + gm] = X[v * LD + gm].x;
     }
 }
 
@@ -1257,9 +1337,15 @@ void soft_core(ettk_plan* P, const std::vector<SoftInst>& inst, double* d_C) {
     } else {
         S.T1.ensure((size_t)nq * n3);
         S.T2.ensure((size_t)nq * n3);
-        k_so3_synth<<<dim3(n, nq), kT, 0, s>>>(S.cs.p, S.cp.p, S.dtab.p, S.doff.p, S.d_slice, B, S.T1.p);
-        k_dft_m1<<<dim3(n, nq), kT, 0, s>>>(S.T1.p, S.tw.p, B, S.T2.p);
-        k_dft_m2<<<dim3(n, nq), kT, 0, s>>>(S.T2.p, S.tw.p, B, d_C);
+        k_so3_synth<<<dim3(nq, n), kT, 0, s>>>(S.cs.p, S.cp.p, S.dtab.p, S.doff.p, S.d_slice, B, S.T1.p);
+        if (n == 128) {
+            const dim3 grid(n / kFftV, n, nq);
+            k_fft_m1<7><<<grid, kT, 0, s>>>(S.T1.p, S.tw.p, S.T2.p);
+            k_fft_m2<7><<<grid, kT, 0, s>>>(S.T2.p, S.tw.p, d_C);
+        } else {
+            k_dft_m1<<<dim3(n, nq), kT, 0, s>>>(S.T1.p, S.tw.p, B, S.T2.p);
+            k_dft_m2<<<dim3(n, nq), kT, 0, s>>>(S.T2.p, S.tw.p, B, d_C);
+        }
     }
     ETTK_HIP(hipGetLastError());
     bool any_scaled = false;
@@ -1408,11 +1494,11 @@ void sph_rotate(ettk_plan* P, int n, const float* h_img, const double* h_t, floa
     for (size_t q = 0; q < h.size(); q++) h_out[q] = (float)h[q];
 }
 
-void soft_given(ettk_plan* P, int nq, const double* h_sig, const double* h_pat, double* d_C) {
+void soft_given(ettk_plan* P, int nq, const double* h_sig, const double* h_pat, double* d_C, int scaled) {
     So3State& S = so3(P);
     const int B = S.B, n = 2 * B;
     hipStream_t s = P->stream;
-    std::vector<SoftInst> inst(nq, SoftInst{0, 0, 0, 0});
+    std::vector<SoftInst> inst(nq, SoftInst{0, 0, 0, scaled ? 1 : 0});
     S.inst.ensure(nq);
     ETTK_HIP(hipMemcpyAsync(S.inst.p, inst.data(), nq * sizeof(SoftInst), hipMemcpyHostToDevice, s));
     S.sig.upload(h_sig, (size_t)nq * n * n, s);

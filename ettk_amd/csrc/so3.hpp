@@ -121,7 +121,8 @@ void run_soft(ettk_plan* P, const std::vector<SoftInst>& inst, const float* d_Pp
 // sizes every device buffer of the search for chunks of up to `cn` particles (at most five SOFT instances each),
 // so that none is reallocated (a free synchronises the device) while chunks are in flight
 void so3_search_reserve(ettk_plan* P, int cn);
-void soft_given(ettk_plan* P, int nq, const double* h_sig, const double* h_pat, double* d_C);
+// soft (scaled = 0) or softScaled (scaled = 1) of nq host signal/pattern pairs into d_C ([q][a][b][g])
+void soft_given(ettk_plan* P, int nq, const double* h_sig, const double* h_pat, double* d_C, int scaled = 0);
 // harmonics(A, t, R) (ProjRot.h:1348-1454) of n (2B x 2B) host images by the rotation parts of h_t[i*16]
 void sph_rotate(ettk_plan* P, int n, const float* h_img, const double* h_t, float* h_out);
 So3State& so3_state(ettk_plan* P);

@@ -36,6 +36,7 @@ def load() -> C.CDLL:
                                             C.c_int),
             "ettk_project_batch": ([vp, vp, C.c_int, C.POINTER(Particle), fp], C.c_int),
             "ettk_so3_correlate_batch": ([vp, C.c_int, dp, dp, dp], C.c_int),
+            "ettk_so3_correlate_scaled_batch": ([vp, C.c_int, dp, dp, dp], C.c_int),
             "ettk_sph_rotate_batch": ([vp, C.c_int, fp, dp, fp], C.c_int),
             "ettk_align_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, C.POINTER(Result)], C.c_int),
             "ettk_distances_batch": ([vp, vp, C.c_int, C.POINTER(Particle), C.c_int, C.c_int, C.POINTER(Result)],
@@ -208,13 +209,15 @@ class Plan:
         check(self.lib.ettk_project_batch(self.h, C.c_void_p(d_parts.data_ptr()), n, particle_array(descs), _fp(out)))
         return out
 
-    def so3_correlate(self, sig: np.ndarray, pat: np.ndarray) -> np.ndarray:
+    def so3_correlate(self, sig: np.ndarray, pat: np.ndarray, scaled: bool = False) -> np.ndarray:
+        """soft (ettk_so3_correlate_batch) or softScaled (ettk_so3_correlate_scaled_batch) of signal/pattern pairs."""
         B = self.params.B
         sig = np.ascontiguousarray(sig, dtype=np.float64).reshape(-1, 2 * B, 2 * B)
         pat = np.ascontiguousarray(pat, dtype=np.float64).reshape(-1, 2 * B, 2 * B)
         n = sig.shape[0]
         out = np.zeros((n, 2 * B, 2 * B, 2 * B), np.float64)
-        check(self.lib.ettk_so3_correlate_batch(self.h, n, _dp(sig), _dp(pat), _dp(out)))
+        fn = self.lib.ettk_so3_correlate_scaled_batch if scaled else self.lib.ettk_so3_correlate_batch
+        check(fn(self.h, n, _dp(sig), _dp(pat), _dp(out)))
         return out
 
     def sph_rotate(self, images: np.ndarray, transforms) -> np.ndarray:

@@ -143,6 +143,12 @@ int ettk_project_batch(ettk_plan* plan, const float* d_particles, int n, const e
    pat[i] (2B x 2B doubles, host) give out_c[i] ((2B)^3 doubles, layout [alpha][beta][gamma]). */
 int ettk_so3_correlate_batch(ettk_plan* plan, int n, const double* sig, const double* pat, double* out_c);
 
+/* softScaled (nbfProjectionRotationMetric3D::softScaled, ProjRot.h:1457-1589): the SO(3) correlation above, then
+   rescaled per pair so that its minimum and maximum equal the S^2 inner products <sig, R pat> (sin(theta)-weighted)
+   of the pattern rotated to the rotations of the first minimum and first maximum in [alpha][beta][gamma] order
+   (rotateFct at alpha = a pi / B, beta = (2b + 1) pi / 4B, gamma = g pi / B, rotate_so3.c:695-751). */
+int ettk_so3_correlate_scaled_batch(ettk_plan* plan, int n, const double* sig, const double* pat, double* out_c);
+
 /* nbfProjectionRotationMetric3D::harmonics(A, t, R) (ProjRot.h:1348-1454): each (2B x 2B) spherical image
    images[i] (host floats, row-major [polar][azimuth]) rotated on the sphere by the rotation part of
    transforms[i*16]: Euler angles (psi, theta, phi) taken from the matrix as the reference does (including its

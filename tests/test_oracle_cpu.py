@@ -145,6 +145,31 @@ def test_so3_correlation_peaks_at_the_rotation():
     assert np.unravel_index(np.argmax(Cc), Cc.shape) == (ia, ib, ig)
 
 
+def test_so3_scaled_correlation_takes_the_weighted_inner_products():
+    """softScaled (ProjRot.h:1457-1589) for g = R(g0) f: the correlation peaks at g0's bin, the pattern rotated to
+    that bin is g itself (up to the float rounding of the bin's angles), so the rescaled maximum is the
+    sin(theta)-weighted inner product of g with itself; the rescaled volume is an increasing affine map of the
+    correlation."""
+    o = oracle()
+    B = 8
+    n = 2 * B
+    f = _bandlimited(B, 23)
+    ia, ib, ig = 5, 2, 9
+    al, be, ga = 2 * np.pi * ia / n, np.pi * (2 * ib + 1) / (4 * B), 2 * np.pi * ig / n
+    g = np.zeros_like(f)
+    o.orc_s2_rotate(B, vp(f), vp(g), al, be, ga)
+    Cc = so3_correlate(B, g, f)
+    Cs = so3_correlate(B, g, f, scaled=True)
+    assert np.unravel_index(np.argmax(Cs), Cs.shape) == (ia, ib, ig)
+    th = np.sin(np.pi * (2 * np.arange(n) + 1) / (4 * B))
+    gg = g.reshape(n, n)
+    wself = float(np.sum(gg * gg * th[:, None]))
+    assert abs(Cs.max() - wself) <= 1e-6 * abs(wself), (Cs.max(), wself)
+    slope, icpt = np.polyfit(Cc.ravel(), Cs.ravel(), 1)
+    assert slope > 0
+    assert np.max(np.abs(Cs - (slope * Cc + icpt))) <= 1e-9 * (Cs.max() - Cs.min())
+
+
 def test_local_maxima_brute_force():
     o = oracle()
     n = 10

@@ -33,6 +33,36 @@ def test_so3_correlation_matches_oracle(torch_gpu, ctx, B):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("B", [8, 16, 32, 64])
+def test_so3_scaled_correlation_matches_oracle(torch_gpu, ctx, B):
+    """softScaled (ProjRot.h:1457-1589): the correlation rescaled so that its minimum and maximum equal the S^2 inner
+    products of the signal with the pattern rotated to the first minimum's and first maximum's rotation.  The GPU
+    (FP64 extrema, Wigner-D rotation of the pattern's coefficients, inverse S^2 transform, weighted sums) against
+    the oracle's restatement; the extrema positions are identical and the volume agrees to 1e-9 of its range."""
+    from ettk_amd.lib import Plan
+
+    print(f"[so3 correlate scaled] B={B}", flush=True)
+    plan = Plan(ctx, default_params(32 if B <= 32 else 64, B))
+    try:
+        g = np.random.default_rng(100 + B)
+        n = 2 * B
+        m = 3 if B <= 32 else 2
+        sig = g.standard_normal((m, n, n))
+        pat = g.standard_normal((m, n, n))
+        out = plan.so3_correlate(sig, pat, scaled=True)
+        plain = plan.so3_correlate(sig, pat)
+        for i in range(m):
+            ref = so3_correlate(B, sig[i], pat[i], scaled=True)
+            assert np.argmax(out[i]) == np.argmax(ref) and np.argmin(out[i]) == np.argmin(ref)
+            span = np.max(ref) - np.min(ref)
+            assert np.max(np.abs(out[i] - ref)) <= 1e-9 * span, (i, np.max(np.abs(out[i] - ref)), span)
+            # an affine map of the unscaled volume: the same positions order the same way
+            assert np.argmax(plain[i]) == np.argmax(out[i])
+    finally:
+        plan.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("B", [16, 32])
 def test_spherical_rotation_matches_oracle(torch_gpu, ctx, B):
     """harmonics(A, t, R): Euler angles read from t on the host, FP64 S^2 transform / rotateFct / inverse on the
