@@ -87,6 +87,7 @@ std::vector<float> spherical_wedge_host(int B, const ettk_particle& d, bool wedg
 // ------------------------------------------------------------------ candidates (ProjRot.h:1593-1702)
 struct Cand {
     float a, b, g, v;
+    int k = -1;  // peak_rounds: index of this candidate's direct evaluation among the particle's evaluations
 };
 constexpr float kEquivTol = 0.25f;  // L1 distance of equivalent rotation matrices in the search (ProjRot.h:142, 1633)
 struct CandCmp {
@@ -1078,12 +1079,16 @@ static void candidate_stage(const ettk_plan* P, int pi, SearchOut& so, const Eva
 // are then consumed in the order of the sequential loop.  A mirror whose direct transform turns out invalid
 // is discarded (and not counted), so every particle ends with the same candidates, distances and CCC count as
 // peak_loop.
+// evaluated: when set, receives per particle the (transform with its peak shift, distance) of every direct
+// evaluation a combined candidate came from (Cand::k indexes it)
 static void peak_rounds(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n,
-                        std::vector<SearchOut*>& so, std::vector<std::vector<Cand>>& selected, std::vector<int>& nccc) {
+                        std::vector<SearchOut*>& so, std::vector<std::vector<Cand>>& selected, std::vector<int>& nccc,
+                        std::vector<std::vector<std::pair<Mat4, float>>>* evaluated = nullptr) {
     struct State {
         int q = 0, index = 0, peaks = 0;
         bool done = false;
         std::vector<Cand> combined, popped;
+        std::vector<std::pair<Mat4, float>> evald;
     };
     std::vector<State> st(n);
     for (int i = 0; i < n; i++) st[i].peaks = P->p.n_peaks;
@@ -1143,7 +1148,8 @@ static void peak_rounds(ettk_plan* P, const float* d_parts, const ettk_particle*
                 if (transform_ok(td, P->trans_restrict, P->rot_restrict)) {
                     nccc[i]++;
                     if (Dm >= D) s.index++;
-                    s.combined.push_back(Cand{p.a, p.b, p.g, D});
+                    if (evaluated) s.evald.emplace_back(td, D);
+                    s.combined.push_back(Cand{p.a, p.b, p.g, D, evaluated ? (int)s.evald.size() - 1 : -1});
                 }
             }
             CandQueue& pq = queue(i);
@@ -1159,6 +1165,10 @@ static void peak_rounds(ettk_plan* P, const float* d_parts, const ettk_particle*
             selected[i].push_back(pqc.top());
             pqc.pop();
         }
+    }
+    if (evaluated) {
+        evaluated->resize(n);
+        for (int i = 0; i < n; i++) (*evaluated)[i] = std::move(st[i].evald);
     }
 }
 
@@ -1237,17 +1247,36 @@ void refine_adjoint_batch(ettk_plan* P, const float* d_parts, const ettk_particl
     {
         std::vector<std::vector<Cand>> sel;
         std::vector<int> nccc;
-        peak_rounds(P, d_parts, descs, n, so, sel, nccc);
-        // refine(t, 0) of every selected candidate in one batch
-        std::vector<int> pidx;
-        std::vector<Mat4> t;
+        std::vector<std::vector<std::pair<Mat4, float>>> evald;
+        peak_rounds(P, d_parts, descs, n, so, sel, nccc, &evald);
+        // refine(t, 0) of every selected candidate: the constrained CCC at the candidate's rotation, which
+        // peak_rounds has already evaluated for a candidate that came from an SO(3) peak (the same transform, the
+        // same kernels: the same distance and shift); the others (the identity placeholder) in one batch.  The
+        // evaluation counts stay those of the reference method (final_candidates counts this one).
+        std::vector<int> pidx, slot;
+        std::vector<Mat4> t, tb;
+        std::vector<float> d;
         for (int i = 0; i < n; i++)
             for (const Cand& p : sel[i]) {
-                pidx.push_back(i);
-                t.push_back(strip_translation(zyz(p.a, p.b, p.g)));
+                if (p.k >= 0) {
+                    t.push_back(evald[i][p.k].first);
+                    d.push_back(evald[i][p.k].second);
+                } else {
+                    slot.push_back((int)t.size());
+                    pidx.push_back(i);
+                    tb.push_back(strip_translation(zyz(p.a, p.b, p.g)));
+                    t.push_back(Mat4::ident());
+                    d.push_back(0.0f);
+                }
             }
-        std::vector<float> d(pidx.size());
-        if (!pidx.empty()) ccc_batch(P, d_parts, descs, (int)pidx.size(), pidx.data(), t.data(), d.data());
+        if (!pidx.empty()) {
+            std::vector<float> db(pidx.size());
+            ccc_batch(P, d_parts, descs, (int)pidx.size(), pidx.data(), tb.data(), db.data());
+            for (size_t q = 0; q < pidx.size(); q++) {
+                t[slot[q]] = tb[q];
+                d[slot[q]] = db[q];
+            }
+        }
         size_t r = 0;
         for (int i = 0; i < n; i++) {
             std::vector<size_t> mine;

@@ -377,6 +377,12 @@ void ensure_ccc_workspace(ettk_plan* P, int cap) {
     ccc_graphs_reset(P);
 }
 
+void upload_before(ettk_plan* P, void* dst, const void* src, size_t bytes) {
+    ETTK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, P->up_stream));
+    ETTK_HIP(hipEventRecord(P->ev_up, P->up_stream));
+    ETTK_HIP(hipStreamWaitEvent(P->stream, P->ev_up, 0));
+}
+
 void ccc_graphs_reset(ettk_plan* P) {
     for (auto& g : P->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -637,7 +643,7 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
                 });
             for (auto& x : th) x.join();
         }
-        ETTK_HIP(hipMemcpyAsync(P->geom_all.p + e0, g + e0, (size_t)E * sizeof(EvalGeom), hipMemcpyHostToDevice, s));
+        upload_before(P, P->geom_all.p + e0, g + e0, (size_t)E * sizeof(EvalGeom));
     };
     ETTK_HIP(hipEventRecord(P->ev0, s));
     CccWork W{P->R1.p, P->R2.p, P->p_res.p, P->p_n1.p, P->p_n2.p, P->p_wf.p, P->p_k1.p, P->stats.p, P->scalars.p,
@@ -686,17 +692,20 @@ static void plan_release(ettk_plan* P) {
     if (P->stream) (void)hipStreamSynchronize(P->stream);
     if (P->copy_stream) (void)hipStreamSynchronize(P->copy_stream);
     if (P->aux_stream) (void)hipStreamSynchronize(P->aux_stream);
+    if (P->up_stream) (void)hipStreamSynchronize(P->up_stream);
     for (auto& g : P->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     P->graphs.clear();
     so3_state_destroy(P);
     if (P->ev0) (void)hipEventDestroy(P->ev0);
     if (P->ev1) (void)hipEventDestroy(P->ev1);
-    hipStream_t s = P->stream, cs = P->copy_stream, as = P->aux_stream;
+    if (P->ev_up) (void)hipEventDestroy(P->ev_up);
+    hipStream_t s = P->stream, cs = P->copy_stream, as = P->aux_stream, us = P->up_stream;
     delete P;  // device buffers are released before their streams
     if (s) (void)hipStreamDestroy(s);
     if (cs) (void)hipStreamDestroy(cs);
     if (as) (void)hipStreamDestroy(as);
+    if (us) (void)hipStreamDestroy(us);
 }
 
 extern "C" {
@@ -805,6 +814,8 @@ int ettk_plan_create(ettk_ctx* ctx, const ettk_params* params, ettk_plan** out) 
         ETTK_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
         ETTK_HIP(hipStreamCreateWithFlags(&P->copy_stream, hipStreamNonBlocking));
         ETTK_HIP(hipStreamCreateWithFlags(&P->aux_stream, hipStreamNonBlocking));
+        ETTK_HIP(hipStreamCreateWithFlags(&P->up_stream, hipStreamNonBlocking));
+        ETTK_HIP(hipEventCreateWithFlags(&P->ev_up, hipEventDisableTiming));
         ETTK_HIP(hipEventCreate(&P->ev0));
         ETTK_HIP(hipEventCreate(&P->ev1));
         plan_constants(P);

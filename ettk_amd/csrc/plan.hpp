@@ -68,6 +68,9 @@ struct ettk_plan {
     hipStream_t stream = nullptr;
     hipStream_t copy_stream = nullptr;  // host downloads issued from worker threads (non-blocking)
     hipStream_t aux_stream = nullptr;   // the candidate extraction of a search chunk, beside the next chunk's kernels
+    // uploads of per-chunk geometry, so that they run beside the previous chunk's kernels instead of between them
+    hipStream_t up_stream = nullptr;
+    hipEvent_t ev_up = nullptr;
     ettk_params p{};
     int N = 0, H = 0;
     bool dummy_filter = true;
@@ -155,6 +158,9 @@ void ccc_graphs_reset(ettk_plan* P);
 void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
                Mat4* t, float* dist, const ettk_particle* eval_descs = nullptr);
 void ensure_ccc_workspace(ettk_plan* P, int cap);
+// host-to-device copy on the plan's upload stream, ordered before the work enqueued next on the plan's stream
+// (src must be pinned and stay untouched until that work has started)
+void upload_before(ettk_plan* P, void* dst, const void* src, size_t bytes);
 // getWedgeOverlap of the reference against descs[i] at t[i] (CorrMetric.h:533-566)
 void wedge_overlaps(ettk_plan* P, const ettk_particle* descs, const Mat4* t, int n, float* out);
 void so3_state_destroy(ettk_plan* P);

@@ -70,12 +70,14 @@ struct RefineChunk {
     PinnedBuf<RefineGeom> hg;
     PinnedBuf<double> hsc, hmom;  // 4 and 12 per evaluation
     std::vector<Mat4> Ginv, EMs;
-    hipEvent_t done = nullptr;
+    hipEvent_t done = nullptr;   // scores and moments downloaded
+    hipEvent_t kdone = nullptr;  // the chunk's kernels finished (its downloads run on the copy stream after it)
     RefineChunk() = default;
     RefineChunk(const RefineChunk&) = delete;
     RefineChunk& operator=(const RefineChunk&) = delete;
     ~RefineChunk() {
         if (done) (void)hipEventDestroy(done);
+        if (kdone) (void)hipEventDestroy(kdone);
     }
 };
 
@@ -102,6 +104,7 @@ static void refine_prepare(ettk_plan* P, int n, std::vector<std::unique_ptr<Refi
         c.Ginv.resize(c.E);
         c.EMs.resize(c.E);
         ETTK_HIP(hipEventCreateWithFlags(&c.done, hipEventDisableTiming));
+        ETTK_HIP(hipEventCreateWithFlags(&c.kdone, hipEventDisableTiming));
     }
 }
 
@@ -165,7 +168,7 @@ static void refine_enqueue(ettk_plan* P, RefineChunk& c) {
     KernelProf* kp = P->ctx->fft.prof;
     const int E = c.E, e0 = c.e0;
     RefineGeom* geo = P->r_geom.p + e0;
-    ETTK_HIP(hipMemcpyAsync(geo, c.hg.p, (size_t)E * sizeof(RefineGeom), hipMemcpyHostToDevice, s));
+    upload_before(P, geo, c.hg.p, (size_t)E * sizeof(RefineGeom));
     const double Rb = (double)E * N * N * N * sizeof(float), Hb = (double)E * nh * sizeof(float2);
     {
         ProfScope ps(kp, "k_reslice_ref", Rb, s);
@@ -198,9 +201,13 @@ static void refine_enqueue(ettk_plan* P, RefineChunk& c) {
         ProfScope ps(kp, "k_refine_moments", Rb, s);
         launch_refine_moments(P->ref_x.p, geo, P->R1.p, P->maskfft.p, E, N, P->r_mpart.p, mom, s);
     }
-    ETTK_HIP(hipMemcpyAsync(c.hsc.p, sc, (size_t)4 * E * sizeof(double), hipMemcpyDeviceToHost, s));
-    ETTK_HIP(hipMemcpyAsync(c.hmom.p, mom, (size_t)12 * E * sizeof(double), hipMemcpyDeviceToHost, s));
-    ETTK_HIP(hipEventRecord(c.done, s));
+    // the downloads on the copy stream: the next chunk's kernels do not wait for them
+    hipStream_t cs = P->copy_stream;
+    ETTK_HIP(hipEventRecord(c.kdone, s));
+    ETTK_HIP(hipStreamWaitEvent(cs, c.kdone, 0));
+    ETTK_HIP(hipMemcpyAsync(c.hsc.p, sc, (size_t)4 * E * sizeof(double), hipMemcpyDeviceToHost, cs));
+    ETTK_HIP(hipMemcpyAsync(c.hmom.p, mom, (size_t)12 * E * sizeof(double), hipMemcpyDeviceToHost, cs));
+    ETTK_HIP(hipEventRecord(c.done, cs));
 }
 
 // score S[q] and gradient grad6[q * 6] of the chunk's evaluations once its kernels are done
