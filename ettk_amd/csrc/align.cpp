@@ -1105,13 +1105,19 @@ static void peak_rounds(ettk_plan* P, const float* d_parts, const ettk_particle*
             s.done = true;
         }
     };
-    std::vector<int> pidx;
-    std::vector<Mat4> t;
-    std::vector<float> d;
-    for (;;) {
-        pidx.clear();
-        t.clear();
-        for (int i = 0; i < n; i++) {
+    // the particles in one or two groups, each with its batch of the current round: with two, the host consumes
+    // one group's results and builds its next round while the GPU runs the other group's batch
+    struct Group {
+        int i0, i1;
+        std::vector<int> pidx;
+        std::vector<Mat4> t;
+        std::vector<float> d;
+        bool active = false;
+    };
+    auto build = [&](Group& G) {  // this round's evaluations of the group; false when it has none left
+        G.pidx.clear();
+        G.t.clear();
+        for (int i = G.i0; i < G.i1; i++) {
             State& s = st[i];
             s.popped.clear();
             while (!s.done) {
@@ -1126,23 +1132,24 @@ static void peak_rounds(ettk_plan* P, const float* d_parts, const ettk_particle*
             }
             for (const Cand& p : s.popped) {
                 Mat4 t1 = zyz(p.a, p.b, p.g);
-                pidx.push_back(i);
-                t.push_back(t1);
-                pidx.push_back(i);
-                t.push_back(mat_scale(-1, -1, -1) * t1);
+                G.pidx.push_back(i);
+                G.t.push_back(t1);
+                G.pidx.push_back(i);
+                G.t.push_back(mat_scale(-1, -1, -1) * t1);
             }
         }
-        if (pidx.empty()) break;
-        d.resize(pidx.size());
-        ccc_batch(P, d_parts, descs, (int)pidx.size(), pidx.data(), t.data(), d.data());
+        G.d.resize(G.pidx.size());
+        return !G.pidx.empty();
+    };
+    auto consume = [&](Group& G) {
         size_t r = 0;
-        for (int i = 0; i < n; i++) {
+        for (int i = G.i0; i < G.i1; i++) {
             State& s = st[i];
             if (s.popped.empty()) continue;
             for (const Cand& p : s.popped) {
-                const float D = d[r];
-                const Mat4& td = t[r];
-                const float Dm = d[r + 1];
+                const float D = G.d[r];
+                const Mat4& td = G.t[r];
+                const float Dm = G.d[r + 1];
                 r += 2;
                 nccc[i]++;
                 if (transform_ok(td, P->trans_restrict, P->rot_restrict)) {
@@ -1154,6 +1161,43 @@ static void peak_rounds(ettk_plan* P, const float* d_parts, const ettk_particle*
             }
             CandQueue& pq = queue(i);
             if (s.index >= s.peaks || pq.empty() || pq.top().v == FLT_MAX) advance(i);
+        }
+    };
+    if (n >= 2 && ccc_lanes_available(P)) {
+        ensure_ccc_workspace(P, P->ccc_chunk);  // no workspace growth while a batch is in flight
+        Group G[2];
+        G[0].i0 = 0;
+        G[0].i1 = n / 2;
+        G[1].i0 = n / 2;
+        G[1].i1 = n;
+        try {
+            for (int h = 0; h < 2; h++)
+                if ((G[h].active = build(G[h])))
+                    ccc_batch_begin(P, h, d_parts, descs, (int)G[h].pidx.size(), G[h].pidx.data(), G[h].t.data(),
+                                    G[h].d.data());
+            while (G[0].active || G[1].active) {
+                for (int h = 0; h < 2; h++) {
+                    if (!G[h].active) continue;
+                    ccc_batch_end(P, h);
+                    consume(G[h]);
+                    if ((G[h].active = build(G[h])))
+                        ccc_batch_begin(P, h, d_parts, descs, (int)G[h].pidx.size(), G[h].pidx.data(), G[h].t.data(),
+                                        G[h].d.data());
+                }
+            }
+        } catch (...) {
+            // the other lane may still be in flight: let it finish before its arrays go away
+            (void)hipStreamSynchronize(P->stream);
+            P->lane[0].pending = P->lane[1].pending = false;
+            throw;
+        }
+    } else {
+        Group all;
+        all.i0 = 0;
+        all.i1 = n;
+        while (build(all)) {
+            ccc_batch(P, d_parts, descs, (int)all.pidx.size(), all.pidx.data(), all.t.data(), all.d.data());
+            consume(all);
         }
     }
     for (int i = 0; i < n; i++) {

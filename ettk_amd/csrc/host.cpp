@@ -372,7 +372,6 @@ void ensure_ccc_workspace(ettk_plan* P, int cap) {
     P->bidx.ensure((size_t)P->z.E * cap);
     P->out4.ensure((size_t)4 * cap);
     P->geom.ensure(cap);
-    P->h_out4.resize((size_t)4 * cap);
     P->cap = cap;
     ccc_graphs_reset(P);
 }
@@ -601,33 +600,33 @@ void ccc_enqueue_chunk(ettk_plan* P, const CccWork& W, const float* d_parts, con
                       P->z, E, out4, s);
 }
 
-void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx, Mat4* t,
-               float* dist, const ettk_particle* eval_descs) {
+bool ccc_lanes_available(const ettk_plan* P) {
+    const KernelProf* prof = P->ctx->fft.prof;
+    return !(P->use_graphs && !(prof && prof->on));
+}
+
+void ccc_batch_begin(ettk_plan* P, int li, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
+                     Mat4* t, float* dist, const ettk_particle* eval_descs) {
     ETTK_REQUIRE(P->have_ref, ETTK_ERR_STATE, "reference not set");
-    KernelProf* prof = P->ctx->fft.prof;
-    if (P->use_graphs && !(prof && prof->on)) {  // per-kernel timing needs the eager launches
-        ccc_batch_graph(P, d_parts, descs, n, pidx, t, dist, eval_descs);
-        return;
-    }
+    ETTK_REQUIRE(li == 0 || li == 1, ETTK_ERR_ARG, "ccc_batch_begin: lane must be 0 or 1");
+    CccLane& L = P->lane[li];
+    ETTK_REQUIRE(!L.pending, ETTK_ERR_STATE, "ccc_batch_begin: the lane's previous batch was not ended");
     hipStream_t s = P->stream;
     const int CH = std::max(1, P->ccc_chunk);  // evaluations per chunk
     ensure_ccc_workspace(P, std::min(n, CH));
     const int chunk = std::min(P->cap, CH);
-    auto tw0 = std::chrono::steady_clock::now();
-    struct WallRec {
-        KernelProf* kp;
-        std::chrono::steady_clock::time_point t0;
-        ~WallRec() {
-            if (kp) kp->add_host("ccc_batch_wall", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
-        }
-    } wall_rec{P->ctx->fft.prof, tw0};
+    if (!L.ev0) ETTK_HIP(hipEventCreate(&L.ev0));
+    if (!L.ev1) ETTK_HIP(hipEventCreate(&L.ev1));
+    if (!L.done) ETTK_HIP(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+    L.t_begin = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
     // geometry of each chunk computed on the host just before its kernels are enqueued, in pinned memory (so the
     // upload does not wait for the earlier chunks' kernels, which run meanwhile); one download of the results at
     // the end
-    P->geom_all.ensure(n);
-    P->out_all.ensure((size_t)4 * n);
-    P->h_geom.ensure(n);
-    EvalGeom* g = P->h_geom.p;
+    L.geom_all.ensure(n);
+    L.out_all.ensure((size_t)4 * n);
+    L.h_geom.ensure(n);
+    L.h_out.ensure((size_t)4 * n);
+    EvalGeom* g = L.h_geom.p;
     auto geom_chunk = [&](int e0, int E) {
         const int nthr = std::max(1, std::min<int>(E / 64, 4));
         auto geom_of = [&](int e) {
@@ -643,29 +642,57 @@ void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
                 });
             for (auto& x : th) x.join();
         }
-        upload_before(P, P->geom_all.p + e0, g + e0, (size_t)E * sizeof(EvalGeom));
+        upload_before(P, L.geom_all.p + e0, g + e0, (size_t)E * sizeof(EvalGeom));
     };
-    ETTK_HIP(hipEventRecord(P->ev0, s));
+    ETTK_HIP(hipEventRecord(L.ev0, s));
     CccWork W{P->R1.p, P->R2.p, P->p_res.p, P->p_n1.p, P->p_n2.p, P->p_wf.p, P->p_k1.p, P->stats.p, P->scalars.p,
               P->bval.p, P->bidx.p, P->S1.p, P->S2.p, P->S3.p, P->G1.p};
     for (int e0 = 0; e0 < n; e0 += chunk) {
         const int E = std::min(chunk, n - e0);
         geom_chunk(e0, E);
-        ccc_enqueue_chunk(P, W, d_parts, P->geom_all.p + e0, E, P->out_all.p + (size_t)4 * e0, s);
+        ccc_enqueue_chunk(P, W, d_parts, L.geom_all.p + e0, E, L.out_all.p + (size_t)4 * e0, s);
     }
-    ETTK_HIP(hipEventRecord(P->ev1, s));
-    P->h_out4.resize((size_t)4 * n);
-    ETTK_HIP(hipMemcpyAsync(P->h_out4.data(), P->out_all.p, (size_t)4 * n * sizeof(float), hipMemcpyDeviceToHost, s));
-    ETTK_HIP(hipStreamSynchronize(s));
-    float ms = 0;
-    ETTK_HIP(hipEventElapsedTime(&ms, P->ev0, P->ev1));
-    P->timing.ccc_ms += ms;
+    ETTK_HIP(hipEventRecord(L.ev1, s));
+    ETTK_HIP(hipMemcpyAsync(L.h_out.p, L.out_all.p, (size_t)4 * n * sizeof(float), hipMemcpyDeviceToHost, s));
+    ETTK_HIP(hipEventRecord(L.done, s));
+    L.pending = true;
+    L.n = n;
+    L.t = t;
+    L.dist = dist;
     P->timing.n_ccc += n;
-    for (int e = 0; e < n; e++) {
-        dist[e] = P->h_out4[(size_t)e * 4];
-        double sx = P->h_out4[(size_t)e * 4 + 1], sy = P->h_out4[(size_t)e * 4 + 2], sz = P->h_out4[(size_t)e * 4 + 3];
-        t[e] = t[e] * mat_translate(-sx, -sy, -sz);
+}
+
+void ccc_batch_end(ettk_plan* P, int li) {
+    CccLane& L = P->lane[li];
+    if (!L.pending) return;
+    L.pending = false;
+    ETTK_HIP(hipEventSynchronize(L.done));
+    float ms = 0;
+    ETTK_HIP(hipEventElapsedTime(&ms, L.ev0, L.ev1));
+    P->timing.ccc_ms += ms;
+    if (KernelProf* kp = P->ctx->fft.prof) {
+        const double now =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+        kp->add_host("ccc_batch_wall", now - L.t_begin);
     }
+    const float* o = L.h_out.p;
+    for (int e = 0; e < L.n; e++) {
+        L.dist[e] = o[(size_t)e * 4];
+        const double sx = o[(size_t)e * 4 + 1], sy = o[(size_t)e * 4 + 2], sz = o[(size_t)e * 4 + 3];
+        L.t[e] = L.t[e] * mat_translate(-sx, -sy, -sz);
+    }
+}
+
+void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx, Mat4* t,
+               float* dist, const ettk_particle* eval_descs) {
+    ETTK_REQUIRE(P->have_ref, ETTK_ERR_STATE, "reference not set");
+    if (!ccc_lanes_available(P)) {  // chunks replayed from graphs (per-kernel timing needs the eager launches)
+        ccc_batch_graph(P, d_parts, descs, n, pidx, t, dist, eval_descs);
+        return;
+    }
+    ccc_batch_end(P, 0);  // a batch still in flight on lane 0 is finished first
+    ccc_batch_begin(P, 0, d_parts, descs, n, pidx, t, dist, eval_descs);
+    ccc_batch_end(P, 0);
 }
 
 }  // namespace ettk

@@ -39,6 +39,28 @@ void smooth3(std::vector<float>& A, int n0, int n1, int n2, double sd, double rf
 
 // ------------------------------------------------------------------ state
 struct So3State;
+// one eagerly launched CCC batch: its geometry (host, pinned, and device), its results (device, and pinned host),
+// and the caller's arrays that ccc_batch_end fills
+struct CccLane {
+    DevBuf<EvalGeom> geom_all;
+    PinnedBuf<EvalGeom> h_geom;
+    DevBuf<float> out_all;
+    PinnedBuf<float> h_out;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, done = nullptr;
+    bool pending = false;
+    int n = 0;
+    Mat4* t = nullptr;
+    float* dist = nullptr;
+    double t_begin = 0;  // host clock at ccc_batch_begin (ms)
+    CccLane() = default;
+    CccLane(const CccLane&) = delete;
+    CccLane& operator=(const CccLane&) = delete;
+    ~CccLane() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (done) (void)hipEventDestroy(done);
+    }
+};
 // a captured CCC chunk pipeline (E evaluations, particle base pointer, workspace generation)
 struct CccGraph {
     int E = 0;
@@ -91,10 +113,9 @@ struct ettk_plan {
     ettk::DevBuf<float> R1, R2, p_res, p_n1, p_n2, p_k1, p_wf, stats, scalars, bval, out4;
     ettk::DevBuf<int> bidx;
     ettk::DevBuf<float2> S1, S2, S3, G1;
-    ettk::DevBuf<ettk::EvalGeom> geom, geom_all;
-    ettk::PinnedBuf<ettk::EvalGeom> h_geom;  // host geometry of a CCC batch (asynchronous uploads)
-    ettk::DevBuf<float> out_all;
-    std::vector<float> h_out4;
+    ettk::DevBuf<ettk::EvalGeom> geom;
+    ettk::PinnedBuf<ettk::EvalGeom> h_geom;  // host geometry of a graph-replayed CCC batch
+    ettk::CccLane lane[2];                   // eagerly launched CCC batches in flight (ccc_batch_begin / _end)
     // execution choices (ettk_plan_tune): evaluations per CCC chunk, chunks replayed from captured graphs
     int ccc_chunk = 256;
     bool use_graphs = false;
@@ -157,6 +178,13 @@ void ccc_graphs_reset(ettk_plan* P);
 // reads volume pidx[e] with descriptor descs[pidx[e]], or eval_descs[e] when eval_descs is given.
 void ccc_batch(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
                Mat4* t, float* dist, const ettk_particle* eval_descs = nullptr);
+// the same batch split in two, on lane 0 or 1 of the plan (eager launches): _begin enqueues it and returns at
+// once; _end waits for it and fills t and dist.  Two lanes in flight let the host finish one batch's results while
+// the GPU runs the other's kernels; t, dist, pidx and the descriptors must stay valid until _end.
+bool ccc_lanes_available(const ettk_plan* P);
+void ccc_batch_begin(ettk_plan* P, int lane, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
+                     Mat4* t, float* dist, const ettk_particle* eval_descs = nullptr);
+void ccc_batch_end(ettk_plan* P, int lane);
 void ensure_ccc_workspace(ettk_plan* P, int cap);
 // host-to-device copy on the plan's upload stream, ordered before the work enqueued next on the plan's stream
 // (src must be pinned and stay untouched until that work has started)
