@@ -1081,9 +1081,12 @@ static void candidate_stage(const ettk_plan* P, int pi, SearchOut& so, const Eva
 // peak_loop.
 // evaluated: when set, receives per particle the (transform with its peak shift, distance) of every direct
 // evaluation a combined candidate came from (Cand::k indexes it)
+// after_rounds, when set, is called once the last round's results are in, before the host's elimination of
+// equivalent candidates (work that does not depend on the candidates can be enqueued there)
 static void peak_rounds(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n,
                         std::vector<SearchOut*>& so, std::vector<std::vector<Cand>>& selected, std::vector<int>& nccc,
-                        std::vector<std::vector<std::pair<Mat4, float>>>* evaluated = nullptr) {
+                        std::vector<std::vector<std::pair<Mat4, float>>>* evaluated = nullptr,
+                        const std::function<void()>* after_rounds = nullptr) {
     struct State {
         int q = 0, index = 0, peaks = 0;
         bool done = false;
@@ -1200,7 +1203,8 @@ static void peak_rounds(ettk_plan* P, const float* d_parts, const ettk_particle*
             consume(all);
         }
     }
-    for (int i = 0; i < n; i++) {
+    if (after_rounds) (*after_rounds)();
+    parallel_for(n, [&](int i) {  // particles are independent
         std::vector<Cand>& combined = st[i].combined;
         if (combined.empty()) combined.push_back(Cand{0, 0, 0, 1.0f});
         CandQueue pqc;
@@ -1209,7 +1213,7 @@ static void peak_rounds(ettk_plan* P, const float* d_parts, const ettk_particle*
             selected[i].push_back(pqc.top());
             pqc.pop();
         }
-    }
+    });
     if (evaluated) {
         evaluated->resize(n);
         for (int i = 0; i < n; i++) (*evaluated)[i] = std::move(st[i].evald);
@@ -1278,8 +1282,6 @@ void align_run(ettk_plan* P, const float* d_parts, const ettk_particle* descs, i
     P->timing.total_ms = ms;
 }
 
-void refine_with_seeds(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
-                       const Mat4* seeds, int n_iter, Mat4* out_pose, double* out_score);
 
 // batched + adjoint mode: candidate stage with batched CCCs, then lock-step continuous refinement of every
 // seed, then one constrained CCC per refined pose for the reported distance (the continuous pose is kept)
@@ -1292,7 +1294,10 @@ void refine_adjoint_batch(ettk_plan* P, const float* d_parts, const ettk_particl
         std::vector<std::vector<Cand>> sel;
         std::vector<int> nccc;
         std::vector<std::vector<std::pair<Mat4, float>>> evald;
-        peak_rounds(P, d_parts, descs, n, so, sel, nccc, &evald);
+        // the refinement's particle spectra depend on the particles only: enqueued while the host eliminates the
+        // equivalent candidates
+        const std::function<void()> spectra = [&] { refine_spectra_ahead(P, d_parts, 0, n); };
+        peak_rounds(P, d_parts, descs, n, so, sel, nccc, &evald, &spectra);
         // refine(t, 0) of every selected candidate: the constrained CCC at the candidate's rotation, which
         // peak_rounds has already evaluated for a candidate that came from an SO(3) peak (the same transform, the
         // same kernels: the same distance and shift); the others (the identity placeholder) in one batch.  The
@@ -1321,21 +1326,18 @@ void refine_adjoint_batch(ettk_plan* P, const float* d_parts, const ettk_particl
                 d[slot[q]] = db[q];
             }
         }
-        size_t r = 0;
-        for (int i = 0; i < n; i++) {
-            std::vector<size_t> mine;
-            for (size_t c = 0; c < sel[i].size(); c++) mine.push_back(r + c);
+        std::vector<size_t> first(n + 1, 0);  // the particle's first entry in t and d
+        for (int i = 0; i < n; i++) first[i + 1] = first[i] + sel[i].size();
+        parallel_for(n, [&](int i) {  // particles are independent
+            size_t k = first[i];
             EvalSink sink;
             sink.direct = [&](int, Mat4& tt) {
-                const size_t k = mine.front();
-                mine.erase(mine.begin());
                 tt = t[k];
-                return d[k];
+                return d[k++];
             };
             Task<void> task = final_candidates(P, i, sel[i], *so[i], &sink, out[i], nccc[i], &seeds[i]);
             run_direct(task);
-            r += sel[i].size();
-        }
+        });
     }
     std::vector<int> pidx, cand;
     std::vector<Mat4> sd;
@@ -1351,13 +1353,36 @@ void refine_adjoint_batch(ettk_plan* P, const float* d_parts, const ettk_particl
     std::vector<double> score(nj);
     auto tr0 = std::chrono::steady_clock::now();
     if (kp) kp->add_host("candidate_stage_wall", std::chrono::duration<double, std::milli>(tr0 - tc0).count());
-    refine_with_seeds(P, d_parts, descs, nj, pidx.data(), sd.data(), 20, pose.data(), score.data());
+    // the reported distance: one CCC per refined pose, begun for each refinement chunk as soon as its poses are
+    // final (alternating lanes), so that it runs while the other chunks finish their last steps
+    std::vector<float> dist(nj);
+    const bool lanes = ccc_lanes_available(P);
+    int next_lane = 0;
+    const std::function<void(int, int)> final_ccc = [&](int q0, int q1) {
+        for (int q = q0; q < q1; q++) fin[q] = pose[q];
+        const int h = next_lane;
+        next_lane ^= 1;
+        ccc_batch_end(P, h);
+        ccc_batch_begin(P, h, d_parts, descs, q1 - q0, pidx.data() + q0, fin.data() + q0, dist.data() + q0);
+    };
+    try {
+        refine_with_seeds(P, d_parts, descs, nj, pidx.data(), sd.data(), 20, pose.data(), score.data(),
+                          lanes ? &final_ccc : nullptr);
+    } catch (...) {
+        (void)hipStreamSynchronize(P->stream);
+        P->lane[0].pending = P->lane[1].pending = false;
+        throw;
+    }
     if (kp)
         kp->add_host("refine_wall",
                      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tr0).count());
-    std::vector<float> dist(nj);
-    fin = pose;
-    ccc_batch(P, d_parts, descs, nj, pidx.data(), fin.data(), dist.data());
+    if (lanes) {
+        ccc_batch_end(P, 0);
+        ccc_batch_end(P, 1);
+    } else {
+        fin = pose;
+        ccc_batch(P, d_parts, descs, nj, pidx.data(), fin.data(), dist.data());
+    }
     std::vector<float> best(n, FLT_MAX);
     for (int j = 0; j < nj; j++) {
         int i = pidx[j], c = cand[j];

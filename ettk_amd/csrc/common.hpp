@@ -8,6 +8,9 @@
 #include <mutex>
 #include <string>
 #include <vector>
+#include <thread>
+#include <exception>
+#include <algorithm>
 
 #include "../../include/ettk_align.h"
 
@@ -19,6 +22,9 @@ void set_error(const std::string& msg);
 // host worker threads for one parallel section: OMP_NUM_THREADS when set (the process's share of the CPU),
 // otherwise the hardware thread count, at most 16
 int host_threads();
+// fn(i) for i in [0, n) on up to host_threads() threads (in the calling thread when n is small)
+template <class F>
+void parallel_for(int n, F fn, int min_per_thread = 32);
 struct Error {
     int code;
     std::string msg;
@@ -34,6 +40,28 @@ struct Error {
     do {                                                                                                \
         if (!(cond)) throw ::ettk::Error{(code), (msg)};                                                \
     } while (0)
+
+template <class F>
+void parallel_for(int n, F fn, int min_per_thread) {
+    const int nthr = std::max(1, std::min(host_threads(), n / std::max(1, min_per_thread)));
+    if (nthr <= 1) {
+        for (int i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::vector<std::exception_ptr> err(nthr);
+    for (int w = 0; w < nthr; w++)
+        th.emplace_back([&, w] {
+            try {
+                for (int i = w; i < n; i += nthr) fn(i);
+            } catch (...) {
+                err[w] = std::current_exception();
+            }
+        });
+    for (auto& t : th) t.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+}
 
 // Raises the dynamic-LDS limit of kernel `fn` to at least `bytes` (hipFuncSetAttribute), at most once per size
 // increase; safe when plans are driven from several host threads.

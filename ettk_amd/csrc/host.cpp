@@ -500,7 +500,7 @@ void wedge_overlaps(ettk_plan* P, const ettk_particle* descs, const Mat4* t, int
     const bool eff1 = P->ref_wedge_kind == 0 ? effective(P->refd.lwedge, P->refd.uwedge) : P->ref_wedge_kind == 1;
     std::vector<EvalGeom> g(n);
     std::vector<int> eff2(n);
-    for (int i = 0; i < n; i++) {
+    parallel_for(n, [&](int i) {
         const ettk_particle& d = descs[i];
         Mat4 fin = euler_of(d) * Mat4::from(d.matrix) * t[i];
         if (mat_det(fin) == 0.0) fin = Mat4::ident();
@@ -514,15 +514,15 @@ void wedge_overlaps(ettk_plan* P, const ettk_particle* descs, const Mat4* t, int
         }
         eff2[i] = effective(d.lwedge, d.uwedge);
         e.full = (w.full || !eff2[i]) ? 1 : 0;
-    }
-    DevBuf<EvalGeom> dg;
-    DevBuf<double> part, sums;
-    dg.upload(g.data(), n, s);
-    part.ensure((size_t)n * 64);
-    sums.ensure(n);
-    launch_wedge_overlap(P->W1.p, P->filter.p, dg.p, n, N, part.p, sums.p, s);
+    });
+    // the plan's buffers (no allocation per call: a free would wait for the whole device)
+    P->ov_geom.ensure(n);
+    P->ov_part.ensure((size_t)n * 64);
+    P->ov_sums.ensure(n);
+    ETTK_HIP(hipMemcpyAsync(P->ov_geom.p, g.data(), (size_t)n * sizeof(EvalGeom), hipMemcpyHostToDevice, s));
+    launch_wedge_overlap(P->W1.p, P->filter.p, P->ov_geom.p, n, N, P->ov_part.p, P->ov_sums.p, s);
     std::vector<double> h(n);
-    ETTK_HIP(hipMemcpyAsync(h.data(), sums.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    ETTK_HIP(hipMemcpyAsync(h.data(), P->ov_sums.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
     ETTK_HIP(hipStreamSynchronize(s));
     for (int i = 0; i < n; i++) out[i] = (eff1 || eff2[i]) ? (float)(h[i] / P->sum_filter) : 1.0f;
 }

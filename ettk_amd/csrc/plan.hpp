@@ -3,6 +3,8 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <functional>
+#include <memory>
 
 #include "kernels.hpp"
 
@@ -61,6 +63,26 @@ struct CccLane {
         if (done) (void)hipEventDestroy(done);
     }
 };
+
+// One chunk of evaluations (at most 128, the CCC workspace) of the score and gradient: its geometry and results in
+// pinned host memory (asynchronous copies), the host matrices its gradient needs, and the event that marks its
+// kernels and downloads done.  Chunks share the device workspaces in stream order; their geometry, score and
+// moment regions of the device arrays are their own.
+struct RefineChunk {
+    int e0 = 0, E = 0;
+    PinnedBuf<RefineGeom> hg;
+    PinnedBuf<double> hsc, hmom;  // 4 and 12 per evaluation
+    std::vector<Mat4> Ginv, EMs;
+    hipEvent_t done = nullptr;   // scores and moments downloaded
+    hipEvent_t kdone = nullptr;  // the chunk's kernels finished (its downloads run on the copy stream after it)
+    RefineChunk() = default;
+    RefineChunk(const RefineChunk&) = delete;
+    RefineChunk& operator=(const RefineChunk&) = delete;
+    ~RefineChunk() {
+        if (done) (void)hipEventDestroy(done);
+        if (kdone) (void)hipEventDestroy(kdone);
+    }
+};
 // a captured CCC chunk pipeline (E evaluations, particle base pointer, workspace generation)
 struct CccGraph {
     int E = 0;
@@ -116,6 +138,10 @@ struct ettk_plan {
     ettk::DevBuf<ettk::EvalGeom> geom;
     ettk::PinnedBuf<ettk::EvalGeom> h_geom;  // host geometry of a graph-replayed CCC batch
     ettk::CccLane lane[2];                   // eagerly launched CCC batches in flight (ccc_batch_begin / _end)
+    std::vector<std::unique_ptr<ettk::RefineChunk>> refine_pool;  // refinement chunks (refine.cpp)
+    // wedge overlaps (wedge_overlaps): geometry and per-block sums, kept between calls
+    ettk::DevBuf<ettk::EvalGeom> ov_geom;
+    ettk::DevBuf<double> ov_part, ov_sums;
     // execution choices (ettk_plan_tune): evaluations per CCC chunk, chunks replayed from captured graphs
     int ccc_chunk = 256;
     bool use_graphs = false;
@@ -144,6 +170,8 @@ struct ettk_plan {
     std::vector<float> sph_acc_bin;
     ettk::DevBuf<float> ref_wbin;  // the reference's copy of wbin (kind 1), read by the refinement
     ettk::DevBuf<float2> r_fp;
+    const float* r_fp_parts = nullptr;  // r_fp holds the spectra of particles [r_fp_p0, r_fp_p1) of this array,
+    int r_fp_p0 = 0, r_fp_p1 = 0;       // enqueued ahead of the refinement that uses them (refine_spectra_ahead)
     ettk::DevBuf<float> r_mean, r_mpart;
     ettk::DevBuf<double> r_pdot, r_sc, r_mom;
     ettk::DevBuf<ettk::RefineGeom> r_geom;
@@ -185,6 +213,13 @@ bool ccc_lanes_available(const ettk_plan* P);
 void ccc_batch_begin(ettk_plan* P, int lane, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
                      Mat4* t, float* dist, const ettk_particle* eval_descs = nullptr);
 void ccc_batch_end(ettk_plan* P, int lane);
+// continuous refinement of n (particle, seed) jobs, n_iter ascent steps each (refine.cpp); on_done, when given, is
+// called with [q0, q1) as soon as those jobs' poses and scores are final (before the call returns)
+void refine_with_seeds(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
+                       const Mat4* seeds, int n_iter, Mat4* out_pose, double* out_score,
+                       const std::function<void(int, int)>* on_done = nullptr);
+// the particle spectra the refinement of particles [p0, p1) needs, enqueued ahead of it (it then skips them)
+void refine_spectra_ahead(ettk_plan* P, const float* d_parts, int p0, int p1);
 void ensure_ccc_workspace(ettk_plan* P, int cap);
 // host-to-device copy on the plan's upload stream, ordered before the work enqueued next on the plan's stream
 // (src must be pinned and stay untouched until that work has started)

@@ -2,6 +2,7 @@
 // lock-step gradient ascent that replaces Powell in the batched + adjoint mode.
 #include <algorithm>
 #include <cfloat>
+#include <functional>
 #include <cmath>
 #include <memory>
 
@@ -61,28 +62,9 @@ static void particle_spectra(ettk_plan* P, const float* d_parts, int p0, int p1)
     }
 }
 
-// One chunk of evaluations (at most 128, the CCC workspace) of the score and gradient: its geometry and results in
-// pinned host memory (asynchronous copies), the host matrices its gradient needs, and the event that marks its
-// kernels and downloads done.  Chunks share the device workspaces in stream order; their geometry, score and
-// moment regions of the device arrays are their own.
-struct RefineChunk {
-    int e0 = 0, E = 0;
-    PinnedBuf<RefineGeom> hg;
-    PinnedBuf<double> hsc, hmom;  // 4 and 12 per evaluation
-    std::vector<Mat4> Ginv, EMs;
-    hipEvent_t done = nullptr;   // scores and moments downloaded
-    hipEvent_t kdone = nullptr;  // the chunk's kernels finished (its downloads run on the copy stream after it)
-    RefineChunk() = default;
-    RefineChunk(const RefineChunk&) = delete;
-    RefineChunk& operator=(const RefineChunk&) = delete;
-    ~RefineChunk() {
-        if (done) (void)hipEventDestroy(done);
-        if (kdone) (void)hipEventDestroy(kdone);
-    }
-};
-
-// device arrays for n evaluations in chunks of `cap`, and the chunks themselves
-static void refine_prepare(ettk_plan* P, int n, std::vector<std::unique_ptr<RefineChunk>>& chunks) {
+// device arrays for n evaluations in chunks of `cap`, and the chunks themselves (kept by the plan between calls:
+// their pinned buffers and events are allocated once)
+static std::vector<RefineChunk*> refine_prepare(ettk_plan* P, int n) {
     const int N = P->N;
     ensure_ccc_workspace(P, std::min(n, 128));
     const int cap = std::min(P->cap, 128);
@@ -92,10 +74,11 @@ static void refine_prepare(ettk_plan* P, int n, std::vector<std::unique_ptr<Refi
     P->r_sc.ensure((size_t)4 * n);
     P->r_mom.ensure((size_t)12 * n);
     P->r_geom.ensure(n);
-    chunks.clear();
-    for (int e0 = 0; e0 < n; e0 += cap) {
-        chunks.emplace_back(new RefineChunk());
-        RefineChunk& c = *chunks.back();
+    std::vector<RefineChunk*> chunks;
+    size_t k = 0;
+    for (int e0 = 0; e0 < n; e0 += cap, k++) {
+        if (k == P->refine_pool.size()) P->refine_pool.emplace_back(new RefineChunk());
+        RefineChunk& c = *P->refine_pool[k];
         c.e0 = e0;
         c.E = std::min(cap, n - e0);
         c.hg.ensure(c.E);
@@ -103,9 +86,11 @@ static void refine_prepare(ettk_plan* P, int n, std::vector<std::unique_ptr<Refi
         c.hmom.ensure((size_t)12 * c.E);
         c.Ginv.resize(c.E);
         c.EMs.resize(c.E);
-        ETTK_HIP(hipEventCreateWithFlags(&c.done, hipEventDisableTiming));
-        ETTK_HIP(hipEventCreateWithFlags(&c.kdone, hipEventDisableTiming));
+        if (!c.done) ETTK_HIP(hipEventCreateWithFlags(&c.done, hipEventDisableTiming));
+        if (!c.kdone) ETTK_HIP(hipEventCreateWithFlags(&c.kdone, hipEventDisableTiming));
+        chunks.push_back(&c);
     }
+    return chunks;
 }
 
 // host geometry (FP64) of the chunk's evaluations q (particle pidx[q], seed seeds[q], parameters x6[q])
@@ -246,8 +231,7 @@ static void refine_finish(RefineChunk& c, const Mat4* seeds, const double* x6, d
 // spectra of the referenced particles must be in P->r_fp (slot = particle - p0).
 static void refine_eval_batch(ettk_plan* P, const ettk_particle* descs, int n, const int* pidx, int p0,
                               const Mat4* seeds, const double* x6, double* S, double* grad6) {
-    std::vector<std::unique_ptr<RefineChunk>> chunks;
-    refine_prepare(P, n, chunks);
+    const std::vector<RefineChunk*> chunks = refine_prepare(P, n);
     for (auto& c : chunks) {
         refine_geometry(P, descs, pidx, p0, seeds, x6, *c);
         refine_enqueue(P, *c);
@@ -261,12 +245,12 @@ static void refine_eval_batch(ettk_plan* P, const ettk_particle* descs, int n, c
 // each chunk of jobs advances on its own, round robin, so the host finishes one chunk's step and enqueues its next
 // evaluation while the GPU evaluates the other chunks.
 static void refine_jobs(ettk_plan* P, const ettk_particle* descs, int n, const int* pidx, int p0, const Mat4* seeds,
-                        int n_iter, Mat4* out_pose, double* out_score) {
+                        int n_iter, Mat4* out_pose, double* out_score,
+                        const std::function<void(int, int)>* on_done) {
     const double cm = P->N / 4.0 * M_PI / 180.0;
     std::vector<double> x((size_t)6 * n, 0.0), xt((size_t)6 * n, 0.0), g((size_t)6 * n), gt((size_t)6 * n);
     std::vector<double> Sc(n), St(n), step(n, 0.5);
-    std::vector<std::unique_ptr<RefineChunk>> chunks;
-    refine_prepare(P, n, chunks);
+    const std::vector<RefineChunk*> chunks = refine_prepare(P, n);
     for (auto& c : chunks) {
         refine_geometry(P, descs, pidx, p0, seeds, x.data(), *c);
         refine_enqueue(P, *c);
@@ -296,6 +280,12 @@ static void refine_jobs(ettk_plan* P, const ettk_particle* descs, int n, const i
                 }
             }
             if (++evals[ci] >= n_iter) {
+                // the chunk's jobs are final: their poses and scores, and the caller told
+                for (int q = q0; q < q1; q++) {
+                    out_pose[q] = pose(seeds[q], x.data() + (size_t)q * 6);
+                    out_score[q] = Sc[q];
+                }
+                if (on_done) (*on_done)(q0, q1);
                 active--;
                 continue;
             }
@@ -312,10 +302,6 @@ static void refine_jobs(ettk_plan* P, const ettk_particle* descs, int n, const i
             refine_enqueue(P, c);
         }
     }
-    for (int q = 0; q < n; q++) {
-        out_pose[q] = pose(seeds[q], x.data() + (size_t)q * 6);
-        out_score[q] = Sc[q];
-    }
 }
 
 // Range of particles referenced by a batch of jobs
@@ -327,14 +313,23 @@ static void particle_range(int n, const int* pidx, int& p0, int& p1) {
     p1 = *std::max_element(pidx, pidx + n) + 1;
 }
 
+void refine_spectra_ahead(ettk_plan* P, const float* d_parts, int p0, int p1) {
+    particle_spectra(P, d_parts, p0, p1);
+    P->r_fp_parts = d_parts;
+    P->r_fp_p0 = p0;
+    P->r_fp_p1 = p1;
+}
+
 void refine_with_seeds(ettk_plan* P, const float* d_parts, const ettk_particle* descs, int n, const int* pidx,
-                       const Mat4* seeds, int n_iter, Mat4* out_pose, double* out_score) {
+                       const Mat4* seeds, int n_iter, Mat4* out_pose, double* out_score,
+                       const std::function<void(int, int)>* on_done) {
     ETTK_REQUIRE(P->have_ref, ETTK_ERR_STATE, "reference not set");
     if (n == 0) return;
     int p0, p1;
     particle_range(n, pidx, p0, p1);
-    particle_spectra(P, d_parts, p0, p1);
-    refine_jobs(P, descs, n, pidx, p0, seeds, n_iter, out_pose, out_score);
+    if (!(P->r_fp_parts == d_parts && P->r_fp_p0 == p0 && P->r_fp_p1 == p1)) particle_spectra(P, d_parts, p0, p1);
+    P->r_fp_parts = nullptr;  // spectra enqueued ahead serve this call only
+    refine_jobs(P, descs, n, pidx, p0, seeds, n_iter, out_pose, out_score, on_done);
 }
 
 }  // namespace ettk
