@@ -76,7 +76,9 @@ typedef struct ettk_result {
     float  cand_distance[ETTK_MAX_CANDIDATES];       /* candidateCorrelationPeaks                      */
     double cand_matrix[ETTK_MAX_CANDIDATES][16];     /* candidateTransforms                            */
     float  so3_angles[ETTK_MAX_CANDIDATES][3];       /* (alpha, beta, gamma) of each refined candidate */
-    int    n_ccc;                                    /* constrained-CCC evaluations spent on the pair  */
+    int    n_ccc;                                    /* constrained-CCC evaluations the reference's
+                                                        method spends on the pair (an evaluation the
+                                                        device reuses still counts once here)          */
     int    use_window;                               /* bit0: reference window, bit1: particle window  */
     float  overlap;                                  /* getWedgeOverlap(getTransform()), CorrMetric.h:
                                                         533-566: sum(W1 W2 filter) / sum(filter) at the
@@ -236,7 +238,8 @@ int ettk_reference_set_average(ettk_plan* plan, const float* d_ref, int n, const
 
 /* Per-kernel timing of the last ettk_align_batch / ettk_ccc_eval_batch call, measured with HIP events on
    the plan's stream: total milliseconds spent in the constrained-CCC pipeline and number of CCC
-   evaluations it ran. */
+   evaluations the device ran (fewer than the sum of ettk_result.n_ccc in the batched + adjoint form, which
+   reuses the SO(3) peak's evaluation for the refinement's starting point). */
 int ettk_last_timing(ettk_plan* plan, double* ccc_ms, long long* n_ccc, double* total_ms);
 
 /* The path's 3-D transforms on device buffers, FFTW conventions: the r2c/c2r plans of
