@@ -1506,24 +1506,24 @@ void soft_given(ettk_plan* P, int nq, const double* h_sig, const double* h_pat, 
     soft_core(P, inst, d_C);
 }
 
+// The dynamic-LDS limit is a per-function, process-wide attribute: plans of different bandwidths (run from
+// several host threads by the size buckets) only ever raise it, so a smaller plan never lowers it under a
+// larger one's launches.
 static void so3_kernel_attributes(int B) {
-    int shm = B * (2 * B - 1) * (int)sizeof(double2);
-    ETTK_HIP(hipFuncSetAttribute((const void*)k_rotate_coef, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
-    ETTK_HIP(hipFuncSetAttribute((const void*)k_rotate_coef_angles, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
-    if (B <= 32) {
-        const int shm2 = (2 * B - 1) * (2 * B - 1) * (int)sizeof(double2);
-        ETTK_HIP(hipFuncSetAttribute((const void*)k_rotate_coef_tab, hipFuncAttributeMaxDynamicSharedMemorySize, shm2));
-    }
+    const size_t shm = (size_t)B * (2 * B - 1) * sizeof(double2);
+    ensure_dynamic_lds((const void*)k_rotate_coef, shm);
+    ensure_dynamic_lds((const void*)k_rotate_coef_angles, shm);
+    if (B <= 32) ensure_dynamic_lds((const void*)k_rotate_coef_tab, (size_t)(2 * B - 1) * (2 * B - 1) * sizeof(double2));
     if (fused_synthesis(B)) {
-        const int n = 2 * B, fs = n * (n + 1) * (int)sizeof(double2);
+        const int n = 2 * B;
         const void* f = n == 16 ? (const void*)k_so3_synth_fft<4> : n == 32 ? (const void*)k_so3_synth_fft<5>
                                                                             : (const void*)k_so3_synth_fft<6>;
-        ETTK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, fs));
+        ensure_dynamic_lds(f, (size_t)n * (n + 1) * sizeof(double2));
     }
     if (mfma_synthesis(B)) {
-        const int n = 2 * B, fs = n * (n + 1) * (int)sizeof(double2);
+        const int n = 2 * B;
         const void* f = n == 32 ? (const void*)k_so3_fft2<5> : (const void*)k_so3_fft2<6>;
-        ETTK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, fs));
+        ensure_dynamic_lds(f, (size_t)n * (n + 1) * sizeof(double2));
     }
 }
 
