@@ -794,6 +794,17 @@ int orc_powell_quadratic(const double* a, const double* c, double k, const doubl
     *fmin = S.end_error;
     return (int)S.nevals;
 }
+// The same sweep on any function of three variables supplied by the caller (a test's closed forms).
+int orc_powell_fn(double (*fn)(const double*), const double* x0, double step, double xtol, double ftol,
+                  double* xmin, double* fmin) {
+    PowellState S;
+    S.f = [fn](const double* x) -> double { return fn(x); };
+    double p[3] = {x0[0], x0[1], x0[2]};
+    powell_minimize(S, p, step, xtol, ftol);
+    for (int i = 0; i < 3; i++) xmin[i] = p[i];
+    *fmin = S.end_error;
+    return (int)S.nevals;
+}
 // 26-neighbour local maxima of an n^3 volume with a -FLT_MAX border (localMaxima3D3x3 + BordStrategyConst)
 void orc_local_maxima(const double* C, int n, double* out) {
     std::vector<double> c(C, C + (size_t)n * n * n), o;

@@ -216,6 +216,94 @@ def test_powell_restatement_on_quadratics():
     assert fm.value < 0.05 * f0
 
 
+_FN3 = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double))
+
+
+def _powell_fn(f, x0, step=5.0, xtol=1e-2, ftol=1e-1):
+    """The oracle's Powell sweep (maxfev = 1) on a Python function of three variables."""
+    o = oracle()
+    o.orc_powell_fn.restype = C.c_int
+    o.orc_powell_fn.argtypes = [_FN3, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p]
+    cb = _FN3(lambda p: float(f(np.array([p[0], p[1], p[2]]))))
+    x = np.ascontiguousarray(x0, np.float64)
+    xm = np.zeros(3)
+    fm = C.c_double()
+    nev = o.orc_powell_fn(cb, vp(x), step, xtol, ftol, vp(xm), C.byref(fm))
+    return xm, fm.value, nev
+
+
+def _exact_line_min(f, p, d):
+    """Minimiser of a quadratic f along p + s d, from three exact samples."""
+    f0, fp, fm = f(p), f(p + d), f(p - d)
+    return p - 0.5 * (fp - fm) / (fp + fm - 2 * f0) * d
+
+
+def _closed_form_sweep(f, x0, ftol=1e-1, order=(0, 1, 2), last_search=True):
+    """One Powell iteration with exact line minima: along the axes in turn, then, unless the decrease already meets
+    the tolerance, along the net displacement when the extrapolated point is lower and the criterion of Powell's
+    direction-set method (Numerical Recipes 10.5, which vnl_powell.cxx:60-175 implements) accepts it."""
+    p = np.array(x0, float)
+    pt, fret = p.copy(), f(p)
+    fp, big = fret, 0.0
+    for i in order:
+        f_prev = fret
+        p = _exact_line_min(f, p, np.eye(3)[i])
+        fret = f(p)
+        big = max(big, abs(f_prev - fret))
+    if 2 * abs(fp - fret) <= ftol * (abs(fp) + abs(fret)) or not last_search:
+        return p, False
+    fe = f(2 * p - pt)
+    if fe < fp and 2 * (fp - 2 * fret + fe) * (fp - fret - big) ** 2 - big * (fp - fe) ** 2 < 0:
+        return _exact_line_min(f, p, p - pt), True
+    return p, False
+
+
+def test_powell_sweep_equals_closed_form_line_minima():
+    """On coupled quadratics (random positive-definite Hessians) one sweep of the restated vnl Powell lands where
+    exact line minima put it: the three axis searches in order, then the search along the net displacement.  The
+    cases include sweeps that take that last search; reversing the axis order or leaving out the last search moves
+    the closed-form endpoint far beyond the tolerance, so the check fixes both."""
+    rng = np.random.default_rng(17)
+    with_last = reordered_far = 0
+    for _ in range(40):
+        Q = rng.standard_normal((3, 3))
+        H = Q @ Q.T + 0.3 * np.eye(3)
+        c = rng.uniform(-6, 6, 3)
+        x0 = rng.uniform(-3, 3, 3)
+        f = lambda x, H=H, c=c: float((x - c) @ H @ (x - c))
+        xm, fm, nev = _powell_fn(f, x0)
+        want, took_last = _closed_form_sweep(f, x0)
+        assert np.abs(xm - want).max() < 2e-2, (xm, want)
+        assert abs(fm - f(xm)) <= 1e-9 * max(1.0, abs(fm))
+        if took_last:
+            with_last += 1
+            assert np.abs(_closed_form_sweep(f, x0, last_search=False)[0] - want).max() > 0.1
+        reordered_far += np.abs(_closed_form_sweep(f, x0, order=(2, 1, 0))[0] - want).max() > 0.1
+    assert with_last >= 5 and reordered_far >= 30, (with_last, reordered_far)
+
+
+def test_powell_line_searches_on_a_non_quadratic():
+    """Separable even functions with their minimum far from the start (log cosh plus a quartic per axis): the bracket
+    has to expand several initial steps and Brent's golden-section steps have to close in, and each axis search
+    ends within the line tolerance of the minimiser; by symmetry the extrapolated point is never lower, so no
+    fourth search runs."""
+    g = lambda d: np.logaddexp(d, -d) - np.log(2.0) + 0.01 * d ** 4  # log cosh + quartic
+    c = np.array([23.0, -17.5, 2.25])
+    for x0 in (np.zeros(3), np.array([-4.0, 6.0, 30.0])):
+        xm, fm, nev = _powell_fn(lambda x: float(np.sum(g(x - c))), x0)
+        assert np.abs(xm - c).max() < 3e-2, (xm, c)
+        assert fm < 1e-3
+        assert nev > 20
+    # Brent's parabolic steps converge superlinearly on a smooth function: over minimisers up to 35 from the start
+    # a sweep takes 32-43 evaluations, where a copy of the restatement whose parabolic step has the wrong sign
+    # (every such step rejected, golden-section steps only) takes 53-62
+    rng = np.random.default_rng(4)
+    for _ in range(30):
+        c = rng.uniform(-20, 20, 3)
+        xm, fm, nev = _powell_fn(lambda x, c=c: float(np.sum(g(x - c))), np.zeros(3))
+        assert np.abs(xm - c).max() < 2e-2 and nev <= 48, (c, xm, nev)
+
+
 def _quadratic_volume(N: int) -> np.ndarray:
     z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
     return (0.01 * x * x - 0.02 * y * y + 0.015 * z * z + 0.005 * x * y - 0.3 * z + 0.2 * x + 1.0).astype(np.float32)
