@@ -15,12 +15,15 @@
  *     the header size as the file size minus the data size, so extended headers are skipped.
  *   - "matlab" arrays: nbfMatlabWriter::write (inc/io/nbfMatlabWriter.h:100-300): the extents as 32-bit ints,
  *     then the values as 4-byte floats with the first index fastest.
+ *   - the classification's outputs built from those: the per-iteration distance array and alignment lists of
+ *     nbfLoopClustering::doMra, the state files of nbfLoopClustering::saveState, and averages written negated.
  * Volumes in memory are float32 arrays with x fastest, v[(z * ny + y) * nx + x], the order the C ABI takes.
  * Errors throw std::runtime_error (the reference prints to cerr and returns false).
  */
 #pragma once
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -234,6 +237,91 @@ inline std::vector<float> readMatlab(const std::string& file, int ndims, std::ve
     in.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(n * sizeof(float)));
     if (!in) throw std::runtime_error("readMatlab: short data in " + file);
     return v;
+}
+
+// ------------------------------------------------------------------------------------------ classification outputs
+/* "_iteration_" and the iteration number padded to three digits, the tag of the classification's per-iteration
+   files (MPI_Classification.cpp:673-677, nbfLoopClustering.h:1597-1602 and 1636-1642). */
+inline std::string iterationTag(int iteration) {
+    std::ostringstream s;
+    s << "_iteration_" << (iteration < 10 ? "00" : iteration < 100 ? "0" : "") << iteration;
+    return s.str();
+}
+
+/* An average written as the classification writes its averages and global averages (MPI_Classification.cpp:
+   954-977, nbfLoopClustering::saveState at nbfLoopClustering.h:5610-5622): the values negated, float MRC.  The
+   variance map is written as it is (writeMrc). */
+inline void writeAverageMrc(const std::string& file, const Volume& average) {
+    Volume v = average;
+    for (float& x : v.data) x = -x;
+    writeMrc(file, v);
+}
+
+/* The per-iteration outputs of nbfLoopClustering::doMra (nbfLoopClustering.h:1595-1645), from the distance rows
+   `D` of K references x n volumes x `cols` columns, [k][j][c] with c fastest (the rows of getDistances:
+   distance, [correlation scale], [wedge overlap], M00..M33):
+     "<header><tag>_alignmentToReferences.matlab": D as a (K, n, cols, 1) array, the reference index fastest;
+     "<header><tag>_alignments_to_reference_<k>.txt" for every reference k: `volumes` with each cutOffset set to
+     |D(k, j, 0)| and each matrix to the 16 values from column 1 (rows of at most 17 columns) or column 2 (longer
+     rows), the columns the reference reads; for rows of 19 or 20 columns that is not where M00 sits, and the
+     files carry what the reference's would.
+   The volumes keep the alignment to the last reference afterwards, as the reference's input list does. */
+inline void saveMraIteration(const std::string& header, int iteration, const float* D, int K, int n, int cols,
+                             std::vector<VolumeRecord>& volumes) {
+    if (K < 1 || n < 1 || cols < 17 || (int)volumes.size() != n)
+        throw std::runtime_error("saveMraIteration: D must be K x n x (>= 17) with one row per volume");
+    const std::string tag = iterationTag(iteration);
+    std::vector<float> a((size_t)K * n * cols);
+    for (int k = 0; k < K; k++)
+        for (int j = 0; j < n; j++)
+            for (int c = 0; c < cols; c++) a[k + (size_t)K * (j + (size_t)n * c)] = D[((size_t)k * n + j) * cols + c];
+    writeMatlab(header + tag + "_alignmentToReferences.matlab", {K, n, cols, 1}, a.data());
+    const int offset = cols > 17 ? 2 : 1;
+    for (int k = 0; k < K; k++) {
+        for (int j = 0; j < n; j++) {
+            const float* row = D + ((size_t)k * n + j) * cols;
+            volumes[j].cutOffset = std::fabs(row[0]);
+            for (int e = 0; e < 16; e++) volumes[j].matrix[e] = row[offset + e];
+        }
+        writeVolumeList(header + tag + "_alignments_to_reference_" + std::to_string(k) + ".txt", volumes);
+    }
+}
+
+/* nbfLoopClustering::saveState (nbfLoopClustering.h:5542-5637) for classes without symmetry:
+     "<file>_volumes.txt": the volumes as given (their current alignments);
+     "<file>_classes.txt": the same list with each cutOffset replaced by the volume's class (0 for a volume in no
+     class, as the reference leaves it);
+     one MRC per average, negated (writeAverageMrc): "<file>_level_<level>_average_<i, three digits>.mrc", or
+     "<file>_average_<level>_<referenceIndexes[i]>.mrc" when reference indexes are given;
+     "<file>_averages.txt": the list of those files, each at position N/2 with the default wedge and identity.
+   `classOf[j]` is the class of volume j or -1.  The reference's binary dump of its average objects
+   ("_averages.bin", the serialisation of nbfWedgedAverageImage3D for its own resume) is not written. */
+inline void saveState(const std::string& file, const std::vector<VolumeRecord>& volumes, const std::vector<int>& classOf,
+                      const std::vector<Volume>& averages, int level, const std::vector<int>& referenceIndexes = {}) {
+    if (classOf.size() != volumes.size()) throw std::runtime_error("saveState: one class entry per volume");
+    if (!referenceIndexes.empty() && referenceIndexes.size() != averages.size())
+        throw std::runtime_error("saveState: one reference index per average");
+    writeVolumeList(file + "_volumes.txt", volumes);
+    std::vector<VolumeRecord> classes = volumes;
+    for (size_t j = 0; j < classes.size(); j++) classes[j].cutOffset = classOf[j] >= 0 ? classOf[j] : 0;
+    writeVolumeList(file + "_classes.txt", classes);
+    std::vector<VolumeRecord> list;
+    for (size_t i = 0; i < averages.size(); i++) {
+        std::ostringstream name;
+        if (!referenceIndexes.empty()) {
+            name << file << "_average_" << level << "_" << referenceIndexes[i] << ".mrc";
+        } else {
+            name << file << "_level_" << level << "_average_" << (i < 10 ? "00" : i < 100 ? "0" : "") << i << ".mrc";
+        }
+        writeAverageMrc(name.str(), averages[i]);
+        VolumeRecord r;
+        r.position[0] = averages[i].nx / 2.0;
+        r.position[1] = averages[i].ny / 2.0;
+        r.position[2] = averages[i].nz / 2.0;
+        r.fileName = name.str();
+        list.push_back(r);
+    }
+    writeVolumeList(file + "_averages.txt", list);
 }
 
 }  // namespace ettk

@@ -88,3 +88,70 @@ def test_formats_round_trip(exe, tmp_path):
     assert struct.unpack("<4i", m[:16]) == (2, 3, 17, 1)
     a = np.frombuffer(m[16:], np.float32)
     assert np.array_equal(a, 0.5 * np.arange(2 * 3 * 17, dtype=np.float32) - 3.0)
+
+
+def _volume_list_text(recs) -> str:
+    header = ("number\tlwedge\tuwedge\tposX\tposY\tposZ\tgeomX\tgeomY\tgeomZ\tnormalX\tnormalY\tnormalZ\t" +
+              "".join(f"matrix[{i}]\t" for i in range(16)) +
+              "magnification[0]\tmagnification[1]\tmagnification[2]\tcutOffset\tfilename")
+    rows = ["\t".join([str(i + 1)] + [_fmt(v) for v in vals] + [fn]) for i, (vals, fn) in enumerate(recs)]
+    return header + "\n" + "\n".join(rows) + "\n"
+
+
+def _record(lw, uw, pos, mat, cut, fn):
+    return [lw, uw, *pos, 64, 64, 64, 0.0, 0.0, 1.0, *mat, 1, 1, 1, cut], fn
+
+
+@pytest.mark.parametrize("cols", [17, 19])
+def test_classification_outputs(exe, tmp_path, cols):
+    """nbfLoopClustering::doMra's per-iteration files (nbfLoopClustering.h:1595-1645) and saveState
+    (:5542-5637), written from a known distance array, classes and averages: the .matlab array with the reference
+    index fastest; per reference, the volume list with cutOffset = |distance| and the matrix read from column 1
+    (17 columns) or 2 (longer rows, the reference's choice); the volume and class lists; the averages negated in
+    float MRC files named by level and index, and their list at position N/2."""
+    g = np.random.default_rng(cols)
+    K, n, N = 2, 3, 4
+    D = g.uniform(-1, 1, (K, n, cols)).astype(np.float32)
+    D[:, :, 0] = -g.uniform(0.2, 0.9, (K, n)).astype(np.float32)
+    avg = g.standard_normal((K, N, N, N)).astype(np.float32)
+    cls = [1, -1, 0]
+    recs = [_record(-60.0, 60.0, (10.5, 20.0, 30.25), np.eye(4).ravel(), 0.0, "p1.mrc"),
+            _record(-45.0, 70.0, (1.0, 2.0, 3.0), np.eye(4).ravel(), 2.5, "p2.mrc"),
+            _record(-90.0, 90.0, (7.0, 8.0, 9.0), np.eye(4).ravel(), 0.0, "p3.mrc")]
+    text = _volume_list_text(recs)
+    (tmp_path / "vols.txt").write_text(text)
+    (tmp_path / "state_in.txt").write_text(f"{K} {n} {cols} {N}\n" + " ".join(map(str, cls)) + "\n")
+    D.tofile(tmp_path / "D.raw")
+    avg.tofile(tmp_path / "avg.raw")
+    r = subprocess.run([str(exe), str(tmp_path), "state"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    m = (tmp_path / "run_iteration_007_alignmentToReferences.matlab").read_bytes()
+    assert struct.unpack("<4i", m[:16]) == (K, n, cols, 1)
+    assert np.array_equal(np.frombuffer(m[16:], np.float32), D.transpose(2, 1, 0).ravel())
+
+    off = 2 if cols > 17 else 1
+    for k in range(K):
+        lines = (tmp_path / f"run_iteration_007_alignments_to_reference_{k}.txt").read_text().splitlines()
+        assert lines[0] == text.splitlines()[0] and len(lines) == n + 1
+        for j in range(n):
+            f = lines[1 + j].split("\t")
+            orig = text.splitlines()[1 + j].split("\t")
+            assert f[:12] == orig[:12] and f[28:31] == orig[28:31] and f[32] == orig[32]
+            assert f[31] == _fmt(abs(float(D[k, j, 0])))
+            assert f[12:28] == [_fmt(float(x)) for x in D[k, j, off:off + 16]]
+
+    assert (tmp_path / "run_state_volumes.txt").read_text() == text
+    classes = (tmp_path / "run_state_classes.txt").read_text().splitlines()
+    assert [ln.split("\t")[31] for ln in classes[1:]] == ["1", "0", "0"]
+    assert [ln.split("\t")[:31] for ln in classes[1:]] == [ln.split("\t")[:31] for ln in text.splitlines()[1:]]
+
+    names = [str(tmp_path / f"run_state_level_3_average_00{k}.mrc") for k in range(K)]
+    for k in range(K):
+        assert Path(names[k]).read_bytes() == _mrc_bytes(-avg[k])
+        assert (tmp_path / f"run_idx_state_average_3_{4 + 5 * k}.mrc").read_bytes() == _mrc_bytes(-avg[k])
+    lst = (tmp_path / "run_state_averages.txt").read_text()
+    # default-constructed records: wedge -90 90, geometry and normal 0, identity, magnification 1, cutOffset 0
+    want = [([-90.0, 90.0, N / 2, N / 2, N / 2, 0, 0, 0, 0, 0, 0, *np.eye(4).ravel(), 1, 1, 1, 0], names[k])
+            for k in range(K)]
+    assert lst == _volume_list_text(want)
