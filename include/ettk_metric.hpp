@@ -94,6 +94,7 @@ class DistanceArray {
     int cols() const { return cols_; }
     int depth() const { return depth_; }
     int extent3() const { return K_; }
+    const float* data() const { return v_.data(); }  // [i][j][c][k], k fastest
 
    private:
     int rows_, cols_, depth_, K_;
