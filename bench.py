@@ -76,7 +76,7 @@ def main() -> int:
     args = ap.parse_args()
     forms = set(args.forms.split(","))
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only (DESIGN.md)"}))
+        print(json.dumps({"error": "bench.py measures one GPU: --gpus must be 1 (sharding across GPUs is designed, not built)"}))
         return 2
     if args.workload == "uneven":
         return uneven(args)
