@@ -295,20 +295,16 @@ __global__ void __launch_bounds__(kT) k_sph_sample(const float* __restrict__ mag
             const float* pb = direct ? mg + ((long)i0 * M + j0) * Hm + k0
                                      : mg + ((long)(M - i0) * M + (M - j0)) * Hm + (M - k0);
             // the 4 taps along k of row (cx, cy) are consecutive floats (ascending in the stored half, descending
-            // in the mirror): two aligned 16-byte loads cover them, the row's offset selects them
+            // in the mirror): one 16-byte load at their 4-byte-aligned start (global loads of four dwords need
+            // only dword alignment on this architecture)
             float tap[4][4][4];  // [cz][cy][cx]
 #pragma unroll
             for (int cy = 0; cy < 4; cy++)
 #pragma unroll
                 for (int cx = 0; cx < 4; cx++) {
                     const float* lo4 = pb + cy * sj + cx * si - (direct ? 0 : 3);
-                    const int sh = (int)(((uintptr_t)lo4 >> 2) & 3);
-                    const float4* q = reinterpret_cast<const float4*>(lo4 - sh);
-                    const float4 u = q[0], x = q[1];
-                    const float f[8] = {u.x, u.y, u.z, u.w, x.x, x.y, x.z, x.w};
                     float r[4];
-#pragma unroll
-                    for (int c = 0; c < 4; c++) r[c] = sh == 0 ? f[c] : sh == 1 ? f[c + 1] : sh == 2 ? f[c + 2] : f[c + 3];
+                    __builtin_memcpy(r, lo4, sizeof(r));
 #pragma unroll
                     for (int cz = 0; cz < 4; cz++) tap[cz][cy][cx] = direct ? r[cz] : r[3 - cz];
                 }
@@ -1225,7 +1221,7 @@ void project_images(ettk_plan* P, const float* d_parts, const ettk_particle* des
     const int CH = 32;
     S.projR.ensure(n3 * CH);
     S.projF.ensure(nbig * CH);
-    S.projMag.ensure(nbig * CH + 16);  // k_sph_sample's aligned 16-byte row loads read up to 3 floats past a row
+    S.projMag.ensure(nbig * CH);
     S.projStats.ensure(4 * CH);
     S.projSum.ensure((size_t)kProjParts * CH);
     int nb_res = reslice_blocks(N);
