@@ -355,6 +355,32 @@ def test_reslice_identity_and_quarter_turn():
     assert np.abs(out[inside] - (vals[inside] - mean)).max() < 2e-4
 
 
+def test_reslice_half_voxel_border():
+    """vtkImageReslice with its border on (the reference's setting): a sample is valid within half a voxel beyond the
+    outermost voxel centres, [-1/2, N - 1/2] per axis.  On a ramp in y that is constant in x, shifts of +-0.4 voxel
+    along x keep the edge plane valid with the ramp's exact values (the Keys weights over taps that reach past the
+    edge still sum to one), shifts of +-0.6 make it invalid (0), and the mean of the valid samples is what is
+    subtracted."""
+    N = 12
+    zi, yi, xi = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    v = (0.25 * yi + 3.0).astype(np.float32)  # [z][y][x]
+    ramp = 0.25 * np.arange(N)
+    for s in (0.4, -0.4, 0.6, -0.6):
+        t = np.eye(4)
+        t[0, 3] = s
+        out = _reslice(v, t)  # [x][y][z]
+        edge = N - 1 if s > 0 else 0
+        valid = np.ones(N, bool)
+        if abs(s) > 0.5:
+            valid[edge] = False
+        want = ramp - ramp.mean()  # the valid samples hold every y value equally often
+        for i in range(N):
+            if valid[i]:
+                assert np.abs(out[i] - want[None, :, None]).max() < 1e-5, (s, i)
+            else:
+                assert np.all(out[i] == 0.0), (s, i)
+
+
 def test_wedge_full_and_plane():
     o = oracle()
     N = 16
