@@ -50,14 +50,19 @@ void parallel_for(int n, F fn, int min_per_thread) {
     }
     std::vector<std::thread> th;
     std::vector<std::exception_ptr> err(nthr);
-    for (int w = 0; w < nthr; w++)
-        th.emplace_back([&, w] {
-            try {
-                for (int i = w; i < n; i += nthr) fn(i);
-            } catch (...) {
-                err[w] = std::current_exception();
-            }
-        });
+    try {
+        for (int w = 0; w < nthr; w++)
+            th.emplace_back([&, w] {
+                try {
+                    for (int i = w; i < n; i += nthr) fn(i);
+                } catch (...) {
+                    err[w] = std::current_exception();
+                }
+            });
+    } catch (...) {  // a thread that could not be started: join the running ones before reporting it
+        for (auto& t : th) t.join();
+        throw;
+    }
     for (auto& t : th) t.join();
     for (auto& e : err)
         if (e) std::rethrow_exception(e);
