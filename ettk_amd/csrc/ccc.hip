@@ -71,7 +71,7 @@ __device__ __forceinline__ bool inside_fp64(const double* md, int i, int j, int 
 // not fit reads the clamped taps from global memory instead.
 // Each wavefront interpolates a 4x4x4 block of the tile: the taps of its voxels then cover a compact region
 // of the box and, with the row and plane strides 17 and 300, fall on fewer common LDS banks than those of an
-// 8x8 plane.  A 17^3 box (20 KB) leaves room for 7 workgroups per CU.
+// 8x8 plane.  A 17^3 box at these strides (20 KB) lets 8 workgroups share a CU's 160 KB of LDS.
 // Measured on 256 evaluations at 64^3 (one launch): 654 us, against 775 us for an 18^3 box at strides 19/343,
 // 8x8 planes per wavefront and two reductions; outputs bit-identical.
 static constexpr int kTile = 8, kBox = 17, kSY = 17, kSZ = 300;
